@@ -1,0 +1,117 @@
+/* PRIA on MI355X: the joint-replenishment LP relaxation by restarted
+ * reflected Halpern PDHG, as hand-written gfx950 HIP kernels behind a C ABI.
+ *
+ * The first block is the reference's pria-cuda ABI, unchanged in layout and
+ * meaning:
+ *   pria_params / pria_result_info      jrp/pria-cuda/include/pria/pria.h:34-61
+ *   pria_params_auto                    jrp/pria-cuda/include/pria/pria.h:51-52
+ *   pria_solve_f64 / pria_solve_f32     jrp/pria-cuda/include/pria/pria.h:63-83
+ *   pria_solve_batch_f64 / _f32         jrp/pria-cuda/include/pria/pria.h:85-99
+ *   pria_objective_f64                  jrp/pria-cuda/include/pria/pria.h:101-106
+ * Inputs: H dense row-major [n][t][t] (upper triangle read), K [n], K0.
+ * Outputs x, y, z dense [n][t][t], upper triangular; y and z normalised by
+ * K_i and K0 (0 where the budget is 0).  Returns 0 on success, nonzero on a
+ * device error (and -1 for an invalid argument).
+ *
+ * The second block extends it (no counterpart in the reference): a solve
+ * taking FP64 inputs with the iterate precision as a flag, a tier choice, a
+ * time limit, and the full solver record.
+ */
+#ifndef REFORMULATION_PRIA_HIP_H
+#define REFORMULATION_PRIA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pria_params {
+    double omega;
+    double eta;
+    double beta_sufficient;
+    double beta_artificial;
+    double beta_necessary;
+    double theta_initial;
+    double theta_target;
+    double theta_decay;
+    double tolerance;
+    int64_t max_iter;        /* -1 for unlimited */
+    int use_halpern;
+    int warm_steps;
+    int op_steps;
+    int verbose;
+} pria_params;
+
+pria_params pria_params_auto(int64_t max_iter, double tolerance);
+
+typedef struct pria_result_info {
+    int64_t iterations;
+    double primal_objective;  /* upper bound at the final iterate (padded problem) */
+    double dual_objective;    /* lower bound at the final iterate (padded problem) */
+    double relative_gap;
+} pria_result_info;
+
+int pria_solve_f64(int64_t n, int64_t t, const double* order_costs, double joint_cost,
+                   const double* holding_costs, const pria_params* params,
+                   const double* warm_x, const double* warm_y, const double* warm_z,
+                   double* x, double* y, double* z, pria_result_info* info);
+
+int pria_solve_f32(int64_t n, int64_t t, const float* order_costs, float joint_cost,
+                   const float* holding_costs, const pria_params* params,
+                   const float* warm_x, const float* warm_y, const float* warm_z,
+                   float* x, float* y, float* z, pria_result_info* info);
+
+int pria_solve_batch_f64(int64_t count, int64_t n, int64_t t, const double* order_costs,
+                         const double* joint_costs, const double* holding_costs,
+                         const pria_params* params, double* x, double* y, double* z,
+                         pria_result_info* infos);
+
+int pria_solve_batch_f32(int64_t count, int64_t n, int64_t t, const float* order_costs,
+                         const float* joint_costs, const float* holding_costs,
+                         const pria_params* params, float* x, float* y, float* z,
+                         pria_result_info* infos);
+
+double pria_objective_f64(int64_t n, int64_t t, const double* order_costs, double joint_cost,
+                          const double* holding_costs, const double* x);
+
+/* ---- extensions -------------------------------------------------------- */
+
+enum {
+    PRIA_TIER_AUTO = 0,       /* by size                                      */
+    PRIA_TIER_WORKGROUP = 1,  /* the whole solve in one workgroup             */
+    PRIA_TIER_DEVICE = 2      /* every workgroup of the GPU per iteration     */
+};
+
+typedef struct {
+    int tier;                 /* PRIA_TIER_*                                  */
+    double time_limit;        /* seconds; <= 0: none (the workgroup tier still
+                               * stops after 600 s)                           */
+} pria_options;
+
+typedef struct {
+    int64_t iterations, restarts, checks;
+    double lower;             /* solver's own lower bound (padded problem)    */
+    double upper;             /* primal objective of x on the original problem */
+    double upper_iterate;     /* upper bound the termination test used        */
+    double relative_gap;      /* gap the termination test saw last            */
+    double omega, scale;
+    int status;               /* 0 converged, 1 iteration limit, 2 time limit */
+    int tier;
+} pria_info;
+
+pria_options pria_default_options(void);
+
+/* One solve with FP64 inputs/outputs; use_float32 selects FP32 iterates
+ * (reductions and scalar logic stay FP64).  Returns 0 (converged), 1 (limit),
+ * -1 (argument), -3 (device). */
+int pria_solve_ex(int64_t n, int64_t t, const double* order_costs, double joint_cost,
+                  const double* holding_costs, const pria_params* params,
+                  const pria_options* options, int use_float32, const double* warm_x,
+                  const double* warm_y, const double* warm_z, double* x, double* y, double* z,
+                  pria_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,123 @@
+// Shared device helpers for the gfx950 PDHG kernels: wave64 shuffles, fixed-
+// order (deterministic) FP64 block reductions, and the Halpern step scalars.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace reform {
+
+constexpr int kWave = 64;   // CDNA wavefront width
+
+// Sum over a wave64 by a fixed xor butterfly: every lane ends with the same
+// bits, and the order of additions depends only on the lane layout.
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off, kWave);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off, kWave);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// Sum of `width` consecutive lanes (a power of two <= 64), xor butterfly.
+template <class T>
+__device__ __forceinline__ T group_sum(T v, int width) {
+    for (int off = width >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// K FP64 sums over the whole workgroup.  `scratch` is LDS of at least
+// K * (blockDim.x / 64) doubles.  Every thread returns with the totals in
+// v[]; the order of additions depends only on the block shape.
+template <int K>
+__device__ __forceinline__ void block_sums(double (&v)[K], double* scratch) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int waves = blockDim.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        v[k] = wave_sum(v[k]);
+        if (lane == 0) scratch[k * 16 + wave] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double t = 0.0;
+        for (int w = 0; w < waves; ++w) t += scratch[k * 16 + w];
+        v[k] = t;
+    }
+    __syncthreads();
+}
+
+// One FP64 value: sum / max / min over the workgroup (scratch: 16 doubles).
+__device__ __forceinline__ double block_sum1(double v, double* scratch) {
+    double a[1] = {v};
+    block_sums<1>(a, scratch);
+    return a[0];
+}
+
+template <class T>
+__device__ __forceinline__ T block_max1(T v, T* scratch) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    v = wave_max(v);
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    T t = scratch[0];
+    for (int w = 1; w < (int)(blockDim.x / kWave); ++w) t = scratch[w] > t ? scratch[w] : t;
+    __syncthreads();
+    return t;
+}
+
+template <class T>
+__device__ __forceinline__ T block_min1(T v, T* scratch) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    v = wave_min(v);
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    T t = scratch[0];
+    for (int w = 1; w < (int)(blockDim.x / kWave); ++w) t = scratch[w] < t ? scratch[w] : t;
+    __syncthreads();
+    return t;
+}
+
+template <class Real>
+__device__ __forceinline__ Real clampr(Real v, Real lo, Real hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Halpern weights of a reflected step with restart counter r:
+// k = r + 1, left = k / (k + 1), right = 1 / (k + 1)
+// (qubo/praq-jax/praq/solver.py:276-292, jrp/pria-jax/pria/solver.py:428-447).
+__host__ __device__ inline void halpern_weights(int64_t r, double* lw, double* rw) {
+    double k = double(r + 1);
+    *lw = k / (k + 1.0);
+    *rw = 1.0 / (k + 1.0);
+}
+
+}  // namespace reform
+
+#define REFORM_HIP_CHECK(call)                                  \
+    do {                                                        \
+        hipError_t reform_err_ = (call);                        \
+        if (reform_err_ != hipSuccess) return kErrorDevice;     \
+    } while (0)

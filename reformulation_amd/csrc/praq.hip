@@ -1,0 +1,330 @@
+// PRAQ on gfx950: kernels, the two execution tiers and the C ABI of
+// include/praq_hip.h.
+//
+//   workgroup tier  praq_wg_solve: one workgroup runs a whole solve -- steps,
+//                   check sums and the restart/omega/termination logic -- in
+//                   one launch, with only workgroup barriers.  One workgroup
+//                   per instance is also how the batch runs many instances in
+//                   one launch.
+//   device tier     per step one node kernel and one edge kernel over the
+//                   whole GPU; per check a two-stage FP64 reduction whose nine
+//                   sums feed the same scalar logic on the host.
+//
+// Reference: qubo/praq-cuda/src/solver.cuh:227-800 (kernels and host loop),
+// qubo/praq-cuda/src/praq.cu:8-79 (entry points and validation).
+#include "praq_hip.h"
+#include "praq_device.hpp"
+#include "praq_host.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace reform {
+
+// ---------------------------------------------------------------------------
+// workgroup tier
+// ---------------------------------------------------------------------------
+
+template <class Real>
+__device__ void praq_wg_step(const PraqArrays<Real>& a, const PraqStep<Real>& st, int lanes) {
+    const int tid = threadIdx.x, nthreads = blockDim.x;
+    const int groups = nthreads / lanes, g = tid / lanes, lane = tid % lanes;
+    for (int base = 0; base < a.n; base += groups) praq_node(a, st, base + g, lane, lanes);
+    __syncthreads();
+    for (int e = tid; e < a.m; e += nthreads) praq_edge(a, st, e);
+    __syncthreads();
+}
+
+template <class Real>
+__device__ void praq_wg_sums(const PraqArrays<Real>& a, int lanes, const Real* bx, const Real* by,
+                             double* red, double (&out)[kPraqSums]) {
+    const int tid = threadIdx.x, nthreads = blockDim.x;
+    const int groups = nthreads / lanes, g = tid / lanes, lane = tid % lanes;
+    double acc[kPraqSums];
+#pragma unroll
+    for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
+    for (int base = 0; base < a.n; base += groups) praq_node_sums(a, base + g, lane, lanes, bx, by, acc);
+    for (int e = tid; e < a.m; e += nthreads) praq_edge_sums(a, e, bx, by, true, acc);
+    block_sums<kPraqSums>(acc, red);
+#pragma unroll
+    for (int k = 0; k < kPraqSums; ++k) out[k] = acc[k];
+}
+
+template <class Real>
+__device__ void praq_wg_copy(Real* dx, const Real* sx, int n, Real* dy, const Real* sy, int m2) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dx[i] = sx[i];
+    for (int e = threadIdx.x; e < m2; e += blockDim.x) dy[e] = sy[e];
+    __syncthreads();
+}
+
+template <class Real>
+__global__ void __launch_bounds__(1024) praq_wg_solve(const PraqJob<Real>* jobs) {
+    __shared__ double red[kPraqSums * 16];
+    __shared__ LogicState st;
+    __shared__ PraqJob<Real> job;
+    if (threadIdx.x == 0) job = jobs[blockIdx.x];
+    __syncthreads();
+    const PraqArrays<Real> a = job.a;
+    const LogicParams p = job.p;
+    const int lanes = job.lanes;
+    const long long t_start = wall_clock64();
+
+    // state from the projected start held in the anchors
+    praq_wg_copy(a.x, a.x0, a.n, a.y, a.y0, 2 * a.m);
+    praq_wg_copy(a.sx, a.x0, a.n, a.sy, a.y0, 2 * a.m);
+    if (threadIdx.x == 0) logic_init(st, p);
+    __syncthreads();
+
+    // the start: one uncounted vanilla step, check at (x, y)
+    praq_wg_step(a, praq_step_scalars<Real>(0, p.omega, p.eta, false, true), lanes);
+    double sums[kPraqSums];
+    praq_wg_sums(a, lanes, a.x, a.y, red, sums);
+    if (threadIdx.x == 0) logic_after_warm_check(st, p, praq_check_sums(sums, job.scale));
+    __syncthreads();
+
+    for (;;) {
+        const int action = st.action;
+        const int anchor_pending = st.anchor_pending;
+        const int64_t r0 = st.restart_iter;
+        const double omega = st.omega;
+        __syncthreads();
+        if (action == kActDoneX) break;
+        if (action == kActDoneSX) {
+            praq_wg_copy(a.x, a.sx, a.n, a.y, a.sy, 2 * a.m);
+            break;
+        }
+        if (action == kActWarmBlock) {
+            for (int64_t j = 0; j < p.op_steps; ++j)
+                praq_wg_step(a, praq_step_scalars<Real>(r0 + j, omega, p.eta, false, true), lanes);
+            praq_wg_sums(a, lanes, a.x, a.y, red, sums);
+            if (threadIdx.x == 0) {
+                st.restart_iter += p.op_steps;
+                st.iteration += p.op_steps;
+                logic_after_warm_check(st, p, praq_check_sums(sums, job.scale));
+            }
+        } else if (action == kActHalpernBlock) {
+            if (anchor_pending) {
+                // the last step was vanilla, whose yc already took the current
+                // point as the anchor; only the anchor arrays need the copy
+                praq_wg_copy(a.x0, a.x, a.n, a.y0, a.y, 2 * a.m);
+                if (threadIdx.x == 0) st.anchor_pending = 0;
+            }
+            for (int64_t j = 0; j < p.op_steps; ++j)
+                praq_wg_step(a, praq_step_scalars<Real>(r0 + j, omega, p.eta, true,
+                                                        j + 1 == p.op_steps), lanes);
+            praq_wg_sums(a, lanes, a.sx, a.sy, red, sums);
+            if (threadIdx.x == 0) {
+                st.restart_iter += p.op_steps;
+                st.iteration += p.op_steps;
+                logic_after_halpern_check(st, p, praq_check_sums(sums, job.scale));
+            }
+        } else {  // kActRestart
+            praq_wg_step(a, praq_step_scalars<Real>(0, omega, p.eta, false, true), lanes);
+            praq_wg_copy(a.x0, a.x, a.n, a.y0, a.y, 2 * a.m);
+            praq_wg_sums(a, lanes, a.sx, a.sy, red, sums);
+            if (threadIdx.x == 0) logic_after_restart(st, p, praq_check_sums(sums, job.scale));
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && job.time_limit_ticks > 0 &&
+            wall_clock64() - t_start > job.time_limit_ticks &&
+            (st.action == kActHalpernBlock || st.action == kActRestart || st.action == kActWarmBlock)) {
+            // stop at the point the last bounds were taken at
+            st.action = action == kActWarmBlock ? kActDoneX : kActDoneSX;
+            if (action == kActRestart) st.action = kActDoneX;
+            st.status = 2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *job.state = st;
+}
+
+// ---------------------------------------------------------------------------
+// device tier
+// ---------------------------------------------------------------------------
+
+template <class Real>
+__global__ void praq_node_kernel(PraqArrays<Real> a, PraqStep<Real> st, int lanes) {
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    praq_node(a, st, gid / lanes, gid % lanes, lanes);
+}
+
+template <class Real>
+__global__ void praq_edge_kernel(PraqArrays<Real> a, PraqStep<Real> st) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < a.m) praq_edge(a, st, e);
+}
+
+template <class Real>
+__global__ void praq_sums_kernel(PraqArrays<Real> a, int lanes, const Real* bx, const Real* by,
+                                 double* partial) {
+    __shared__ double red[kPraqSums * 16];
+    double acc[kPraqSums];
+#pragma unroll
+    for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
+    const int nthreads = gridDim.x * blockDim.x;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int groups = nthreads / lanes;
+    for (int base = 0; base < a.n; base += groups)
+        praq_node_sums(a, base + gid / lanes, gid % lanes, lanes, bx, by, acc);
+    for (int e = gid; e < a.m; e += nthreads) praq_edge_sums(a, e, bx, by, true, acc);
+    block_sums<kPraqSums>(acc, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kPraqSums; ++k) partial[blockIdx.x * kPraqSums + k] = acc[k];
+}
+
+// one block of kSumBlocks threads adds the partial rows in a fixed order
+__global__ void praq_sums_final(const double* partial, int rows, double* out) {
+    __shared__ double red[kPraqSums * 16];
+    double acc[kPraqSums];
+#pragma unroll
+    for (int k = 0; k < kPraqSums; ++k)
+        acc[k] = threadIdx.x < rows ? partial[threadIdx.x * kPraqSums + k] : 0.0;
+    block_sums<kPraqSums>(acc, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kPraqSums; ++k) out[k] = acc[k];
+}
+
+template <class Real>
+__global__ void copy2_kernel(Real* dx, const Real* sx, int n, Real* dy, const Real* sy, int m2) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dx[k] = sx[k];
+    if (k < m2) dy[k] = sy[k];
+}
+
+constexpr int kBlock = 256;
+constexpr int kSumBlocks = 256;
+
+static int blocks_for(int64_t threads) { return int((threads + kBlock - 1) / kBlock); }
+
+template <class Real>
+static int device_step(const PraqArrays<Real>& a, const PraqStep<Real>& st, int lanes,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(praq_node_kernel<Real>, dim3(blocks_for(int64_t(a.n) * lanes)),
+                       dim3(kBlock), 0, stream, a, st, lanes);
+    if (a.m > 0)
+        hipLaunchKernelGGL(praq_edge_kernel<Real>, dim3(blocks_for(a.m)), dim3(kBlock), 0,
+                           stream, a, st);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <class Real>
+static int device_copy(Real* dx, const Real* sx, int n, Real* dy, const Real* sy, int m2,
+                       hipStream_t stream) {
+    int count = std::max(n, m2);
+    hipLaunchKernelGGL(copy2_kernel<Real>, dim3(blocks_for(count)), dim3(kBlock), 0, stream, dx,
+                       sx, n, dy, sy, m2);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <class Real>
+static int device_sums(const PraqArrays<Real>& a, int lanes, const Real* bx, const Real* by,
+                       double* partial, double* dsums, double* host, hipStream_t stream) {
+    hipLaunchKernelGGL(praq_sums_kernel<Real>, dim3(kSumBlocks), dim3(kBlock), 0, stream, a,
+                       lanes, bx, by, partial);
+    hipLaunchKernelGGL(praq_sums_final, dim3(1), dim3(kSumBlocks), 0, stream, partial,
+                       kSumBlocks, dsums);
+    REFORM_HIP_CHECK(hipGetLastError());
+    REFORM_HIP_CHECK(hipMemcpyAsync(host, dsums, sizeof(double) * kPraqSums,
+                                    hipMemcpyDeviceToHost, stream));
+    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// The host loop of the device tier: the same state machine as praq_wg_solve.
+template <class Real>
+int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
+                      double time_limit_seconds, hipStream_t stream) {
+    const PraqArrays<Real>& a = job.a;
+    const LogicParams& p = job.p;
+    const int lanes = job.lanes;
+    double* partial = work;
+    double* dsums = work + kSumBlocks * kPraqSums;
+    double sums[kPraqSums];
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto t0 = now();
+
+    if (device_copy(a.x, a.x0, a.n, a.y, a.y0, 2 * a.m, stream)) return kErrorDevice;
+    if (device_copy(a.sx, a.x0, a.n, a.sy, a.y0, 2 * a.m, stream)) return kErrorDevice;
+    logic_init(st, p);
+    if (device_step(a, praq_step_scalars<Real>(0, p.omega, p.eta, false, true), lanes, stream))
+        return kErrorDevice;
+    if (device_sums(a, lanes, a.x, a.y, partial, dsums, sums, stream)) return kErrorDevice;
+    logic_after_warm_check(st, p, praq_check_sums(sums, job.scale));
+
+    for (;;) {
+        const int action = st.action;
+        const int64_t r0 = st.restart_iter;
+        const double omega = st.omega;
+        if (action == kActDoneX) break;
+        if (action == kActDoneSX) {
+            if (device_copy(a.x, a.sx, a.n, a.y, a.sy, 2 * a.m, stream)) return kErrorDevice;
+            break;
+        }
+        if (action == kActWarmBlock) {
+            for (int64_t j = 0; j < p.op_steps; ++j)
+                if (device_step(a, praq_step_scalars<Real>(r0 + j, omega, p.eta, false, true),
+                                lanes, stream))
+                    return kErrorDevice;
+            if (device_sums(a, lanes, a.x, a.y, partial, dsums, sums, stream)) return kErrorDevice;
+            st.restart_iter += p.op_steps;
+            st.iteration += p.op_steps;
+            logic_after_warm_check(st, p, praq_check_sums(sums, job.scale));
+        } else if (action == kActHalpernBlock) {
+            if (st.anchor_pending) {
+                if (device_copy(a.x0, a.x, a.n, a.y0, a.y, 2 * a.m, stream)) return kErrorDevice;
+                st.anchor_pending = 0;
+            }
+            for (int64_t j = 0; j < p.op_steps; ++j)
+                if (device_step(a, praq_step_scalars<Real>(r0 + j, omega, p.eta, true,
+                                                           j + 1 == p.op_steps),
+                                lanes, stream))
+                    return kErrorDevice;
+            if (device_sums(a, lanes, a.sx, a.sy, partial, dsums, sums, stream))
+                return kErrorDevice;
+            st.restart_iter += p.op_steps;
+            st.iteration += p.op_steps;
+            logic_after_halpern_check(st, p, praq_check_sums(sums, job.scale));
+        } else {
+            if (device_step(a, praq_step_scalars<Real>(0, omega, p.eta, false, true), lanes, stream))
+                return kErrorDevice;
+            if (device_copy(a.x0, a.x, a.n, a.y0, a.y, 2 * a.m, stream)) return kErrorDevice;
+            if (device_sums(a, lanes, a.sx, a.sy, partial, dsums, sums, stream))
+                return kErrorDevice;
+            logic_after_restart(st, p, praq_check_sums(sums, job.scale));
+        }
+        if (time_limit_seconds > 0.0 &&
+            std::chrono::duration<double>(now() - t0).count() > time_limit_seconds &&
+            st.action != kActDoneX && st.action != kActDoneSX) {
+            st.action = action == kActWarmBlock || action == kActRestart ? kActDoneX : kActDoneSX;
+            st.status = 2;
+        }
+    }
+    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template int praq_device_solve<float>(const PraqJob<float>&, LogicState&, double*, double,
+                                      hipStream_t);
+template int praq_device_solve<double>(const PraqJob<double>&, LogicState&, double*, double,
+                                       hipStream_t);
+
+size_t praq_device_work_bytes() { return sizeof(double) * (kSumBlocks + 1) * kPraqSums; }
+
+template <class Real>
+int praq_launch_wg(const PraqJob<Real>* jobs, int count, int block, hipStream_t stream) {
+    if (count <= 0) return 0;
+    hipLaunchKernelGGL(praq_wg_solve<Real>, dim3(count), dim3(block), 0, stream, jobs);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template int praq_launch_wg<float>(const PraqJob<float>*, int, int, hipStream_t);
+template int praq_launch_wg<double>(const PraqJob<double>*, int, int, hipStream_t);
+
+}  // namespace reform

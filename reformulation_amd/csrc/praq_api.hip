@@ -1,0 +1,232 @@
+// The C ABI of include/praq_hip.h.  Entry points and validation follow
+// qubo/praq-cuda/src/praq.cu:8-79; the extensions add options, the dual
+// output and the tier choice.
+#include "praq_hip.h"
+#include "praq_host.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+using namespace reform;
+
+namespace {
+
+constexpr double kDefaultWgLimitSeconds = 600.0;
+
+// Below this many nodes + edges the one-workgroup tier runs the plain solve.
+int64_t wg_tier_max() {
+    static int64_t v = [] {
+        const char* env = std::getenv("REFORM_PRAQ_WG_MAX");
+        return env ? std::atoll(env) : int64_t(1) << 17;
+    }();
+    return v;
+}
+
+double wall_clock_rate_hz() {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0.0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return 0.0;
+    return double(khz) * 1000.0;
+}
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    ~DeviceBuffer() {
+        if (ptr) (void)hipFree(ptr);
+    }
+    int alloc(size_t bytes) { return hipMalloc(&ptr, bytes ? bytes : 1) == hipSuccess ? 0 : -1; }
+};
+
+template <class Real>
+int solve_typed(const PraqHostInstance& s, const LogicParams& lp, const PraqOptions& o, double* x,
+                double* y, PraqInfo* info) {
+    PraqLayout L = praq_layout(s.n, s.m, sizeof(Real));
+    const size_t job_off = align256(L.total_bytes);
+    const size_t state_off = align256(job_off + sizeof(PraqJob<Real>));
+    const size_t work_off = align256(state_off + sizeof(LogicState));
+    const size_t bytes = work_off + praq_device_work_bytes();
+
+    DeviceBuffer buf;
+    if (buf.alloc(bytes)) return kErrorDevice;
+    char* base = static_cast<char*>(buf.ptr);
+    std::vector<char> image(L.input_bytes, 0);
+    praq_fill_image<Real>(s, L, image.data());
+
+    hipStream_t stream;
+    REFORM_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{stream};
+    REFORM_HIP_CHECK(hipMemcpyAsync(base, image.data(), L.input_bytes, hipMemcpyHostToDevice,
+                                    stream));
+
+    PraqJob<Real> job;
+    std::memset(&job, 0, sizeof(job));
+    job.a = praq_bind<Real>(L, base, s.n, s.m);
+    job.p = lp;
+    job.scale = s.scale;
+    job.lanes = s.lanes;
+    job.state = reinterpret_cast<LogicState*>(base + state_off);
+
+    int tier = o.tier;
+    if (tier == PRAQ_TIER_AUTO)
+        tier = int64_t(s.n) + int64_t(s.m) <= wg_tier_max() ? PRAQ_TIER_WORKGROUP : PRAQ_TIER_DEVICE;
+
+    LogicState st;
+    if (tier == PRAQ_TIER_WORKGROUP) {
+        double limit = o.time_limit > 0.0 ? std::min(o.time_limit, kDefaultWgLimitSeconds)
+                                          : kDefaultWgLimitSeconds;
+        job.time_limit_ticks = (long long)(limit * wall_clock_rate_hz());
+        REFORM_HIP_CHECK(hipMemcpyAsync(base + job_off, &job, sizeof(job), hipMemcpyHostToDevice,
+                                        stream));
+        if (praq_launch_wg<Real>(reinterpret_cast<PraqJob<Real>*>(base + job_off), 1, 1024,
+                                 stream))
+            return kErrorDevice;
+        REFORM_HIP_CHECK(hipMemcpyAsync(&st, job.state, sizeof(st), hipMemcpyDeviceToHost, stream));
+        REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+    } else {
+        if (praq_device_solve<Real>(job, st, reinterpret_cast<double*>(base + work_off),
+                                    o.time_limit, stream))
+            return kErrorDevice;
+    }
+
+    std::vector<Real> hx(s.n), hy(2 * size_t(s.m));
+    REFORM_HIP_CHECK(hipMemcpyAsync(hx.data(), job.a.x, sizeof(Real) * s.n, hipMemcpyDeviceToHost,
+                                    stream));
+    if (y && s.m > 0)
+        REFORM_HIP_CHECK(hipMemcpyAsync(hy.data(), job.a.y, sizeof(Real) * 2 * size_t(s.m),
+                                        hipMemcpyDeviceToHost, stream));
+    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+    for (int i = 0; i < s.n; ++i) x[i] = double(hx[i]);
+    if (y)
+        for (int e = 0; e < s.m; ++e) {
+            y[2 * size_t(e)] = double(hy[e]) * s.scale;
+            y[2 * size_t(e) + 1] = double(hy[s.m + e]) * s.scale;
+        }
+    if (info) {
+        info->lower = st.lower;
+        info->upper = st.upper;
+        info->relative_gap = st.gap;
+        info->iterations = st.iteration;
+        info->restarts = st.restarts;
+        info->checks = st.checks;
+        info->omega = st.omega;
+        info->scale = s.scale;
+        info->status = st.status;
+        info->tier = tier;
+    }
+    return 0;
+}
+
+bool params_valid(const PraqParameters& p) {
+    return p.tolerance >= 0.0 && p.omega > 0.0 && p.beta_sufficient > 0.0 &&
+           p.beta_necessary > 0.0 && p.beta_artificial >= 0.0 && std::isfinite(p.theta_initial) &&
+           std::isfinite(p.theta_target) && std::isfinite(p.theta_decay) &&
+           std::isfinite(p.interval_offset) && std::isfinite(p.interval_scale) &&
+           p.interval_power >= 0.0;
+}
+
+}  // namespace
+
+extern "C" PraqParameters praq_default_parameters(void) {
+    PraqParameters p;
+    p.tolerance = 1e-8;
+    p.max_iter = -1;
+    p.use_float32 = 0;
+    p.omega = 1.0;
+    p.beta_sufficient = 0.36787944117144233;  // exp(-1)
+    p.beta_necessary = 0.5706;
+    p.beta_artificial = 0.2468;
+    p.theta_initial = 0.81930;
+    p.theta_target = 0.42818;
+    p.theta_decay = 0.90474;
+    p.op_steps = 16;
+    p.interval_offset = 0.0;
+    p.interval_scale = 4.328085122666891;  // 3 / ln 2
+    p.interval_power = 0.0;
+    return p;
+}
+
+extern "C" PraqOptions praq_default_options(void) {
+    PraqOptions o;
+    o.use_halpern = 1;
+    o.warm_steps = 0;
+    o.dense_degree = 0;
+    o.tier = PRAQ_TIER_AUTO;
+    o.eta = 0.999;
+    o.time_limit = 0.0;
+    return o;
+}
+
+extern "C" int64_t praq_check_interval(int64_t n, const PraqParameters* parameters) {
+    if (!parameters || n <= 0) return 2;
+    double u = std::log(double(n) / 50000.0);
+    double power = parameters->interval_power;
+    double b = power == 0.0 ? u : std::expm1(power * u) / power;
+    double v = std::floor(parameters->interval_offset + parameters->interval_scale * b + 0.5);
+    if (!(v >= 2.0)) return 2;
+    if (v > 1e9) return 1000000000;
+    return int64_t(v);
+}
+
+extern "C" int praq_solve_ex(const PraqInstance* instance, const PraqParameters* parameters,
+                             const PraqOptions* options, const double* warm_x,
+                             const double* warm_y, double* x, double* y, PraqInfo* info) {
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
+        info->status = kErrorArgument;
+    }
+    if (!instance || !parameters || !x) return kErrorArgument;
+    PraqParameters p = *parameters;
+    if (!params_valid(p)) return kErrorArgument;
+    PraqOptions o = options ? *options : praq_default_options();
+    if (o.tier < PRAQ_TIER_AUTO || o.tier > PRAQ_TIER_DEVICE) return kErrorArgument;
+
+    PraqHostInstance s;
+    int status = praq_build_host(*instance, o.dense_degree, warm_x, warm_y, s);
+    if (status) {
+        if (info) info->status = status;
+        return status;
+    }
+    if (p.max_iter <= 0) p.max_iter = -1;
+    if (p.op_steps <= 0) p.op_steps = praq_check_interval(instance->n, &p);
+    if (p.op_steps < 2) p.op_steps = 2;
+    LogicParams lp = praq_logic_params(p, o);
+
+    PraqInfo local;
+    PraqInfo* out = info ? info : &local;
+    status = p.use_float32 ? solve_typed<float>(s, lp, o, x, y, out)
+                           : solve_typed<double>(s, lp, o, x, y, out);
+    if (status < 0) {
+        out->status = status;
+        return status;
+    }
+    return out->status == 0 ? 0 : 1;
+}
+
+extern "C" int praq_solve(const PraqInstance* instance, const PraqParameters* parameters,
+                          double* x, PraqResult* result) {
+    if (result) {
+        result->lower = std::numeric_limits<double>::quiet_NaN();
+        result->upper = std::numeric_limits<double>::quiet_NaN();
+        result->iterations = 0;
+        result->status = kErrorArgument;
+    }
+    if (!instance || !parameters || !x || !result) return kErrorArgument;
+    PraqInfo info;
+    int status = praq_solve_ex(instance, parameters, nullptr, nullptr, nullptr, x, nullptr, &info);
+    if (status < 0) {
+        result->status = status;
+        return status;
+    }
+    result->lower = info.lower;
+    result->upper = info.upper;
+    result->iterations = info.iterations;
+    result->status = status;
+    return status;
+}

@@ -1,0 +1,225 @@
+// PRAQ device code: the per-node and per-edge halves of one PDHG step with
+// the Halpern blend fused in, and the check-pass sums.  Used by both the
+// one-workgroup whole-solve kernel and the device-wide kernels.
+//
+// Data layout in HBM (all arrays of one instance contiguous, 4-byte indices):
+//   edges   e0[m], e1[m] int32 ends; w[m] signed scaled weight, 2 W_ij / s
+//           (> 0 covering, < 0 packing, 0 inert)
+//   nodes   diag[n], deg[n], x[n], sx[n], x0[n], refl[n]
+//   CSR     ptr[n+1], half[2m]: node i's incident half-edges, each encoded as
+//           slot * m + e, i.e. an index into a slot-major dual array
+//   duals   yc[2m], y[2m], sy[2m], y0[2m], slot-major (slot 0 of every edge,
+//           then slot 1), so the edge pass reads both slots coalesced
+//
+// yc holds the Halpern-combined dual the NEXT step starts from, written by the
+// edge pass as it finishes a step: the node pass's scatter, the one random
+// access of the iteration, then reads one value per half-edge.  y and sy are
+// written only on steps followed by a check or a restart.
+//
+// Reference semantics: _raw_step qubo/praq-jax/praq/solver.py:259-273,
+// _reflect_step :276-292, project_dual qubo/praq-jax/praq/projection.py:16-42,
+// bounds solver.py:320-337, canonical norm halpern_operator.py:7-35.
+#pragma once
+
+#include "common.hpp"
+#include "halpern.hpp"
+
+namespace reform {
+
+template <class Real>
+struct PraqArrays {
+    int n, m;
+    const int* e0;
+    const int* e1;
+    const Real* w;
+    const int* ptr;
+    const int* half;
+    const Real* diag;
+    const Real* deg;
+    Real* x;
+    Real* sx;
+    Real* x0;
+    Real* refl;
+    Real* yc;
+    Real* y;
+    Real* sy;
+    Real* y0;
+};
+
+// The scalars of one step, computed once per step from the restart counter.
+template <class Real>
+struct PraqStep {
+    Real lw, rw;      // Halpern weights of this step (reflected)
+    Real lwn, rwn;    // weights the next step will use (for yc)
+    Real tau, sigma;  // eta / omega, eta * omega
+    bool reflect;     // Halpern step (else vanilla: start from sx, sy)
+    bool store;       // write y, sy (a check or restart follows)
+};
+
+template <class Real>
+__host__ __device__ inline PraqStep<Real> praq_step_scalars(int64_t restart_iter, double omega,
+                                                            double eta, bool reflect, bool store) {
+    PraqStep<Real> s;
+    double lw, rw, lwn, rwn;
+    halpern_weights(restart_iter, &lw, &rw);
+    // after a reflected step the next one has counter + 1; after a vanilla
+    // step the counter restarts at 0 with y1 as its anchor
+    halpern_weights(reflect ? restart_iter + 1 : 0, &lwn, &rwn);
+    s.lw = Real(lw);
+    s.rw = Real(rw);
+    s.lwn = Real(lwn);
+    s.rwn = Real(rwn);
+    Real re = Real(eta), ro = Real(omega);
+    s.tau = re / ro;
+    s.sigma = re * ro;
+    s.reflect = reflect;
+    s.store = store || !reflect;
+    return s;
+}
+
+// Node i's update by a group of `lanes` consecutive lanes (a power of two,
+// <= 64); all lanes of the group must call it (the sum is a shuffle).  The
+// group's lanes stride the CSR list and meet in an xor butterfly, so the sum
+// depends only on the instance and `lanes`.
+template <class Real>
+__device__ __forceinline__ void praq_node(const PraqArrays<Real>& a, const PraqStep<Real>& st,
+                                          int i, int lane, int lanes) {
+    Real acc = Real(0);
+    int begin = 0, end = 0;
+    if (i < a.n) {
+        begin = a.ptr[i];
+        end = a.ptr[i + 1];
+    }
+    const Real* src = st.reflect ? a.yc : a.sy;
+    for (int k = begin + lane; k < end; k += lanes) acc += src[a.half[k]];
+    acc = group_sum(acc, lanes);
+    if (i >= a.n || lane != 0) return;
+    Real xc = a.x[i], sxi = a.sx[i];
+    Real x1 = st.reflect ? st.lw * (Real(2) * sxi - xc) + st.rw * a.x0[i] : sxi;
+    Real g = a.diag[i] - acc;
+    Real x2 = clampr(x1 - st.tau * (g / a.deg[i]), Real(0), Real(1));
+    a.x[i] = x1;
+    a.sx[i] = x2;
+    a.refl[i] = Real(2) * x2 - x1;
+}
+
+// Edge e's update: projection of y1 + sigma (q - S refl), and the next yc.
+template <class Real>
+__device__ __forceinline__ void praq_edge(const PraqArrays<Real>& a, const PraqStep<Real>& st,
+                                          int e) {
+    const int m = a.m;
+    Real y1a, y1b, ana, anb;
+    if (st.reflect) {
+        y1a = a.yc[e];
+        y1b = a.yc[m + e];
+        ana = a.y0[e];
+        anb = a.y0[m + e];
+    } else {
+        y1a = a.sy[e];
+        y1b = a.sy[m + e];
+        ana = y1a;
+        anb = y1b;
+    }
+    Real sw = a.w[e];
+    Real w = sw < Real(0) ? -sw : sw;
+    bool covering = sw > Real(0);
+    Real va = y1a + st.sigma * ((covering ? Real(1) : Real(0)) - a.refl[a.e0[e]]);
+    Real vb = y1b + st.sigma * (Real(0) - a.refl[a.e1[e]]);
+    Real y2a, y2b;
+    if (covering) {
+        Real mu = clampr(Real(0.5) * (va + vb), -w, Real(0));
+        y2a = mu;
+        y2b = mu;
+    } else {
+        Real line = va + Real(0.5) * (w - va - vb);
+        y2a = clampr(line, Real(0), w);
+        y2b = w - y2a;
+    }
+    if (st.store) {
+        a.y[e] = y1a;
+        a.y[m + e] = y1b;
+        a.sy[e] = y2a;
+        a.sy[m + e] = y2b;
+    }
+    a.yc[e] = st.lwn * (Real(2) * y2a - y1a) + st.rwn * ana;
+    a.yc[m + e] = st.lwn * (Real(2) * y2b - y1b) + st.rwn * anb;
+}
+
+// The nine check sums, accumulated per thread.
+enum PraqSum {
+    kPrimalNorm = 0,   // sum deg (x - sx)^2
+    kPrimalDrift = 1,  // sum deg (x - x0)^2
+    kUpperNode = 2,    // sum diag bx
+    kLowerNode = 3,    // sum min(diag - S^T by, 0)
+    kDualNorm = 4,     // sum |y - sy|^2
+    kCross = 5,        // sum <S (x - sx), y - sy>
+    kDualDrift = 6,    // sum |y - y0|^2
+    kUpperEdge = 7,    // sum w [covering: max(bi + bj - 1, 0); packing: -min(bi, bj)]
+    kLowerEdge = 8,    // sum over covering edges of by slot 0
+    kPraqSums = 9
+};
+
+// Node part, by a lane group as praq_node; bounds at (bx, by).
+template <class Real>
+__device__ __forceinline__ void praq_node_sums(const PraqArrays<Real>& a, int i, int lane,
+                                               int lanes, const Real* bx, const Real* by,
+                                               double (&acc)[kPraqSums]) {
+    double scat = 0.0;
+    int begin = 0, end = 0;
+    if (i < a.n) {
+        begin = a.ptr[i];
+        end = a.ptr[i + 1];
+    }
+    for (int k = begin + lane; k < end; k += lanes) scat += double(by[a.half[k]]);
+    scat = group_sum(scat, lanes);
+    if (i >= a.n || lane != 0) return;
+    double xi = a.x[i], sxi = a.sx[i], deg = a.deg[i], di = a.diag[i];
+    double dx = xi - sxi, dr = xi - double(a.x0[i]);
+    acc[kPrimalNorm] += deg * dx * dx;
+    acc[kPrimalDrift] += deg * dr * dr;
+    acc[kUpperNode] += di * double(bx[i]);
+    double rc = di - scat;
+    acc[kLowerNode] += rc < 0.0 ? rc : 0.0;
+}
+
+template <class Real>
+__device__ __forceinline__ void praq_edge_sums(const PraqArrays<Real>& a, int e, const Real* bx,
+                                               const Real* by, bool with_norms,
+                                               double (&acc)[kPraqSums]) {
+    const int m = a.m;
+    int i = a.e0[e], j = a.e1[e];
+    double sw = a.w[e];
+    double w = fabs(sw);
+    bool covering = sw > 0.0;
+    if (with_norms) {
+        double ya = a.y[e], yb = a.y[m + e];
+        double da = ya - double(a.sy[e]), db = yb - double(a.sy[m + e]);
+        acc[kDualNorm] += da * da + db * db;
+        acc[kCross] += (double(a.x[i]) - double(a.sx[i])) * da
+                     + (double(a.x[j]) - double(a.sx[j])) * db;
+        double ra = ya - double(a.y0[e]), rb = yb - double(a.y0[m + e]);
+        acc[kDualDrift] += ra * ra + rb * rb;
+    }
+    double bi = bx[i], bj = bx[j];
+    if (covering) {
+        double t = bi + bj - 1.0;
+        acc[kUpperEdge] += w * (t > 0.0 ? t : 0.0);
+        acc[kLowerEdge] += double(by[e]);
+    } else {
+        acc[kUpperEdge] += -w * (bi < bj ? bi : bj);
+    }
+}
+
+__host__ __device__ inline CheckSums praq_check_sums(const double* s, double scale) {
+    CheckSums c;
+    c.pnorm = s[kPrimalNorm];
+    c.dnorm = s[kDualNorm];
+    c.cross = s[kCross];
+    c.pdrift = s[kPrimalDrift];
+    c.ddrift = s[kDualDrift];
+    c.upper = (s[kUpperNode] + s[kUpperEdge]) * scale;
+    c.lower = (s[kLowerNode] + s[kLowerEdge]) * scale;
+    return c;
+}
+
+}  // namespace reform

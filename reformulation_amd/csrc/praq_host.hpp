@@ -1,0 +1,232 @@
+// PRAQ host side: instance validation and the COO -> half-edge CSR build,
+// the packed per-instance device image, and the job record the kernels read.
+//
+// Reference: Structure::build qubo/praq-cuda/src/solver.cuh:517-569 (CSR over
+// half-edges, degree floored at 1, objective scale), HalfEdge.centroid
+// qubo/praq-jax/praq/half_edge.py:87-94 (cold start), the warm-start
+// projection qubo/praq-jax/praq/solver.py:524-527.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "halpern.hpp"
+#include "praq_device.hpp"
+#include "praq_hip.h"
+
+namespace reform {
+
+constexpr int kErrorArgument = -1;
+constexpr int kErrorInstance = -2;
+constexpr int kErrorDevice = -3;
+
+template <class Real>
+struct PraqJob {
+    PraqArrays<Real> a;
+    LogicParams p;
+    double scale;
+    int lanes;
+    int pad;
+    long long time_limit_ticks;  // wall-clock ticks; <= 0 none
+    LogicState* state;           // device address the final state goes to
+};
+
+struct PraqHostInstance {
+    int n = 0, m = 0, lanes = 1;
+    double scale = 1.0;
+    std::vector<int> e0, e1, ptr, half;
+    std::vector<double> w, diag, deg, x_start, y_start;  // y_start slot-major, scaled
+};
+
+// Lanes per node in the scatter: the smallest power of two leaving each lane
+// at most 8 half-edges at the mean degree, capped at a wavefront.
+inline int praq_lanes_for(int n, int m) {
+    double average = 2.0 * double(m) / double(n > 0 ? n : 1);
+    int lanes = 1;
+    while (lanes < 64 && double(lanes) * 8.0 < average) lanes *= 2;
+    return lanes;
+}
+
+inline double clamp_host(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+inline void project_edge_host(double* v, double w, bool covering) {
+    if (covering) {
+        double mu = clamp_host(0.5 * (v[0] + v[1]), -w, 0.0);
+        v[0] = v[1] = mu;
+    } else {
+        double line = v[0] + 0.5 * (w - v[0] - v[1]);
+        v[0] = clamp_host(line, 0.0, w);
+        v[1] = w - v[0];
+    }
+}
+
+// Validates and builds; returns 0 or kErrorInstance.
+inline int praq_build_host(const PraqInstance& in, int dense_degree, const double* warm_x,
+                           const double* warm_y, PraqHostInstance& s) {
+    const int64_t limit = std::numeric_limits<int>::max();
+    if (in.n <= 0 || in.n > limit || in.m < 0 || in.m > limit / 2) return kErrorInstance;
+    if (!in.diag || (in.m > 0 && (!in.rows || !in.cols || !in.vals))) return kErrorInstance;
+    const int n = int(in.n), m = int(in.m);
+    s.n = n;
+    s.m = m;
+    s.e0.resize(m);
+    s.e1.resize(m);
+    s.w.resize(m);
+    for (int e = 0; e < m; ++e) {
+        int64_t i = in.rows[e], j = in.cols[e];
+        if (i < 0 || i >= in.n || j < 0 || j >= in.n || i == j) return kErrorInstance;
+        s.e0[e] = int(i);
+        s.e1[e] = int(j);
+    }
+    double squares = 0.0;
+    for (int i = 0; i < n; ++i) squares += in.diag[i] * in.diag[i];
+    for (int e = 0; e < m; ++e) squares += 4.0 * in.vals[e] * in.vals[e];
+    s.scale = squares > 0.0 ? std::sqrt(squares) : 1.0;
+    s.diag.resize(n);
+    for (int i = 0; i < n; ++i) s.diag[i] = in.diag[i] / s.scale;
+    for (int e = 0; e < m; ++e) s.w[e] = 2.0 * in.vals[e] / s.scale;  // signed
+
+    s.ptr.assign(size_t(n) + 1, 0);
+    for (int e = 0; e < m; ++e) {
+        ++s.ptr[s.e0[e] + 1];
+        ++s.ptr[s.e1[e] + 1];
+    }
+    for (int i = 0; i < n; ++i) s.ptr[i + 1] += s.ptr[i];
+    s.half.resize(2 * size_t(m));
+    std::vector<int> fill(s.ptr.begin(), s.ptr.end() - 1);
+    for (int e = 0; e < m; ++e) {
+        s.half[fill[s.e0[e]]++] = e;        // slot 0 of edge e
+        s.half[fill[s.e1[e]]++] = m + e;    // slot 1
+    }
+    s.deg.resize(n);
+    for (int i = 0; i < n; ++i) {
+        double incident = dense_degree ? double(n - 1) : double(s.ptr[i + 1] - s.ptr[i]);
+        s.deg[i] = incident > 1.0 ? incident : 1.0;
+    }
+    s.lanes = praq_lanes_for(n, m);
+
+    s.x_start.resize(n);
+    for (int i = 0; i < n; ++i) s.x_start[i] = clamp_host(warm_x ? warm_x[i] : 0.5, 0.0, 1.0);
+    s.y_start.resize(2 * size_t(m));
+    for (int e = 0; e < m; ++e) {
+        double w = std::fabs(s.w[e]);
+        bool covering = s.w[e] > 0.0;
+        double v[2];
+        if (warm_y) {
+            v[0] = warm_y[2 * size_t(e)] / s.scale;
+            v[1] = warm_y[2 * size_t(e) + 1] / s.scale;
+        } else {
+            v[0] = v[1] = -0.5 * s.w[e];  // centroid: -W_ij on both slots
+        }
+        project_edge_host(v, w, covering);
+        s.y_start[e] = v[0];
+        s.y_start[m + e] = v[1];
+    }
+    return 0;
+}
+
+// Byte offsets of one instance's arrays in its device image.  The input part
+// (uploaded) comes first, the iteration state after it.
+struct PraqLayout {
+    enum { kE0, kE1, kPtr, kHalf, kW, kDiag, kDeg, kX0, kY0, kX, kSX, kRefl, kYC, kY, kSY, kCount };
+    size_t off[kCount];
+    size_t input_bytes = 0, total_bytes = 0;
+};
+
+inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+inline PraqLayout praq_layout(int n, int m, size_t real_size) {
+    PraqLayout L;
+    size_t sizes[PraqLayout::kCount] = {
+        sizeof(int) * size_t(m),      sizeof(int) * size_t(m),     sizeof(int) * (size_t(n) + 1),
+        sizeof(int) * 2 * size_t(m),  real_size * size_t(m),       real_size * size_t(n),
+        real_size * size_t(n),        real_size * size_t(n),       real_size * 2 * size_t(m),
+        real_size * size_t(n),        real_size * size_t(n),       real_size * size_t(n),
+        real_size * 2 * size_t(m),    real_size * 2 * size_t(m),   real_size * 2 * size_t(m)};
+    size_t at = 0;
+    for (int k = 0; k < PraqLayout::kCount; ++k) {
+        L.off[k] = at;
+        at += align256(sizes[k] ? sizes[k] : 1);
+        if (k == PraqLayout::kY0) L.input_bytes = at;
+    }
+    L.total_bytes = at;
+    return L;
+}
+
+template <class Real>
+inline void praq_fill_image(const PraqHostInstance& s, const PraqLayout& L, char* image) {
+    auto put_int = [&](int k, const std::vector<int>& v) {
+        if (!v.empty()) std::memcpy(image + L.off[k], v.data(), sizeof(int) * v.size());
+    };
+    auto put_real = [&](int k, const std::vector<double>& v) {
+        Real* dst = reinterpret_cast<Real*>(image + L.off[k]);
+        for (size_t q = 0; q < v.size(); ++q) dst[q] = Real(v[q]);
+    };
+    put_int(PraqLayout::kE0, s.e0);
+    put_int(PraqLayout::kE1, s.e1);
+    put_int(PraqLayout::kPtr, s.ptr);
+    put_int(PraqLayout::kHalf, s.half);
+    put_real(PraqLayout::kW, s.w);
+    put_real(PraqLayout::kDiag, s.diag);
+    put_real(PraqLayout::kDeg, s.deg);
+    put_real(PraqLayout::kX0, s.x_start);
+    put_real(PraqLayout::kY0, s.y_start);
+}
+
+template <class Real>
+inline PraqArrays<Real> praq_bind(const PraqLayout& L, char* base, int n, int m) {
+    PraqArrays<Real> a;
+    a.n = n;
+    a.m = m;
+    a.e0 = reinterpret_cast<const int*>(base + L.off[PraqLayout::kE0]);
+    a.e1 = reinterpret_cast<const int*>(base + L.off[PraqLayout::kE1]);
+    a.ptr = reinterpret_cast<const int*>(base + L.off[PraqLayout::kPtr]);
+    a.half = reinterpret_cast<const int*>(base + L.off[PraqLayout::kHalf]);
+    a.w = reinterpret_cast<const Real*>(base + L.off[PraqLayout::kW]);
+    a.diag = reinterpret_cast<const Real*>(base + L.off[PraqLayout::kDiag]);
+    a.deg = reinterpret_cast<const Real*>(base + L.off[PraqLayout::kDeg]);
+    a.x0 = reinterpret_cast<Real*>(base + L.off[PraqLayout::kX0]);
+    a.y0 = reinterpret_cast<Real*>(base + L.off[PraqLayout::kY0]);
+    a.x = reinterpret_cast<Real*>(base + L.off[PraqLayout::kX]);
+    a.sx = reinterpret_cast<Real*>(base + L.off[PraqLayout::kSX]);
+    a.refl = reinterpret_cast<Real*>(base + L.off[PraqLayout::kRefl]);
+    a.yc = reinterpret_cast<Real*>(base + L.off[PraqLayout::kYC]);
+    a.y = reinterpret_cast<Real*>(base + L.off[PraqLayout::kY]);
+    a.sy = reinterpret_cast<Real*>(base + L.off[PraqLayout::kSY]);
+    return a;
+}
+
+inline LogicParams praq_logic_params(const PraqParameters& p, const PraqOptions& o) {
+    LogicParams q;
+    q.omega = p.omega;
+    q.eta = o.eta > 0.0 ? o.eta : 0.999;
+    q.beta_sufficient = p.beta_sufficient;
+    q.beta_necessary = p.beta_necessary;
+    q.beta_artificial = p.beta_artificial;
+    q.theta_initial = p.theta_initial;
+    q.theta_target = p.theta_target;
+    q.theta_decay = p.theta_decay;
+    q.tolerance = p.tolerance;
+    q.cross_sign = 2.0;
+    q.max_iter = p.max_iter <= 0 ? -1 : p.max_iter;
+    q.op_steps = p.op_steps;
+    q.use_halpern = o.use_halpern ? 1 : 0;
+    q.warm_max = q.use_halpern ? q.op_steps * (o.warm_steps > 0 ? o.warm_steps : 0) : q.max_iter;
+    q.fallback_nonpositive = 1;
+    return q;
+}
+
+// Implemented in praq.hip.
+template <class Real>
+int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
+                      double time_limit_seconds, hipStream_t stream);
+template <class Real>
+int praq_launch_wg(const PraqJob<Real>* jobs, int count, int block, hipStream_t stream);
+size_t praq_device_work_bytes();
+
+}  // namespace reform

@@ -1,0 +1,360 @@
+// PRIA on gfx950: kernels and the two execution tiers.
+//
+//   workgroup tier  pria_wg_solve: one 1024-thread workgroup runs a whole
+//                   solve (x tiles, y/z slices, checks, scalar logic) in one
+//                   launch; one workgroup per instance is also how a batch of
+//                   instances runs in one launch.
+//   device tier     per step an x kernel over (item, column tile) and a y/z
+//                   kernel over slices; per check a slice kernel, a column
+//                   kernel and a fixed-order final reduction.
+//
+// Reference: jrp/pria-cuda/src/solver.cuh:61-1253 (kernels, device logic,
+// whole-solve graph), jrp/pria-cuda/src/projection.cuh:298-538 (Michelot).
+#include "pria_host.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+namespace reform {
+
+// Tile and register shapes.  FP32: 64 columns per x tile; FP64: 32, so the
+// tile stays under 80 KB of LDS at T' = 300.  VR: row values per lane
+// (T' <= 64 VR); VZ: slice values per thread held in registers.
+template <class Real> struct PriaShape;
+template <> struct PriaShape<float> { static constexpr int TW = 64, VR = 10, VZ = 32; };
+template <> struct PriaShape<double> { static constexpr int TW = 32, VR = 10, VZ = 32; };
+
+template <class Real>
+__host__ __device__ inline size_t pria_x_lds_bytes(int T) {
+    return sizeof(Real) * (size_t(T) * (PriaShape<Real>::TW + 1) + PriaShape<Real>::TW);
+}
+
+// ---------------------------------------------------------------------------
+// workgroup tier
+// ---------------------------------------------------------------------------
+
+template <class Real>
+__device__ void pria_wg_step(const PriaArrays<Real>& a, const PriaStep<Real>& st, Real* lds,
+                             double* red) {
+    constexpr int TW = PriaShape<Real>::TW;
+    const int tiles = (a.T + TW - 1) / TW;
+    for (int i = 0; i < a.N; ++i)
+        for (int q = 0; q < tiles; ++q) pria_x_tile<Real, TW>(a, st, i, q * TW, lds);
+    for (int s = 0; s < a.T; ++s)
+        pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, s, red);
+}
+
+template <class Real>
+__device__ void pria_wg_sums(const PriaArrays<Real>& a, const Real* bx, const Real* by,
+                             const Real* bz, double* red, double (&out)[kPriaSums]) {
+    constexpr int TW = PriaShape<Real>::TW;
+    double acc[kPriaSums];
+    for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
+    for (int s = 0; s < a.T; ++s) pria_slice_sums(a, s, bx, red, acc);
+    const int tiles = (a.T + TW - 1) / TW;
+    for (int i = 0; i < a.N; ++i)
+        for (int q = 0; q < tiles; ++q) pria_column_sums<Real, TW>(a, i, q * TW, by, bz, red, acc);
+    for (int q = 0; q < kPriaSums; ++q) out[q] = acc[q];
+}
+
+template <class Real>
+__device__ void pria_wg_copy3(Real* d1, const Real* s1, Real* d2, const Real* s2, Real* d3,
+                              const Real* s3, int nd) {
+    for (int e = threadIdx.x; e < nd; e += blockDim.x) {
+        d1[e] = s1[e];
+        d2[e] = s2[e];
+        d3[e] = s3[e];
+    }
+    __syncthreads();
+}
+
+template <class Real>
+__global__ void __launch_bounds__(1024) pria_wg_solve(const PriaJob<Real>* jobs) {
+    extern __shared__ char dyn[];
+    Real* lds = reinterpret_cast<Real*>(dyn);
+    __shared__ double red[kPriaSums * 16];
+    __shared__ LogicState st;
+    __shared__ PriaJob<Real> job;
+    if (threadIdx.x == 0) job = jobs[blockIdx.x];
+    __syncthreads();
+    const PriaArrays<Real> a = job.a;
+    const LogicParams p = job.p;
+    const long long t_start = wall_clock64();
+
+    pria_wg_copy3(a.x, a.x0, a.y, a.y0, a.z, a.z0, a.ND);
+    pria_wg_copy3(a.sx, a.x0, a.sy, a.y0, a.sz, a.z0, a.ND);
+    if (threadIdx.x == 0) logic_init(st, p);
+    __syncthreads();
+
+    pria_wg_step(a, pria_step_scalars<Real>(0, p.omega, p.eta, false, true), lds, red);
+    double sums[kPriaSums];
+    pria_wg_sums(a, a.x, a.y, a.z, red, sums);
+    if (threadIdx.x == 0) logic_after_warm_check(st, p, pria_check_sums(sums, job.scale));
+    __syncthreads();
+
+    for (;;) {
+        const int action = st.action;
+        const int anchor_pending = st.anchor_pending;
+        const int64_t r0 = st.restart_iter;
+        const double omega = st.omega;
+        __syncthreads();
+        if (action == kActDoneX) break;
+        if (action == kActDoneSX) {
+            pria_wg_copy3(a.x, a.sx, a.y, a.sy, a.z, a.sz, a.ND);
+            break;
+        }
+        if (action == kActWarmBlock) {
+            for (int64_t j = 0; j < p.op_steps; ++j)
+                pria_wg_step(a, pria_step_scalars<Real>(r0 + j, omega, p.eta, false, true), lds,
+                             red);
+            pria_wg_sums(a, a.x, a.y, a.z, red, sums);
+            if (threadIdx.x == 0) {
+                st.restart_iter += p.op_steps;
+                st.iteration += p.op_steps;
+                logic_after_warm_check(st, p, pria_check_sums(sums, job.scale));
+            }
+        } else if (action == kActHalpernBlock) {
+            if (anchor_pending) {
+                pria_wg_copy3(a.x0, a.x, a.y0, a.y, a.z0, a.z, a.ND);
+                if (threadIdx.x == 0) st.anchor_pending = 0;
+            }
+            for (int64_t j = 0; j < p.op_steps; ++j)
+                pria_wg_step(a, pria_step_scalars<Real>(r0 + j, omega, p.eta, true,
+                                                        j + 1 == p.op_steps),
+                             lds, red);
+            pria_wg_sums(a, a.sx, a.sy, a.sz, red, sums);
+            if (threadIdx.x == 0) {
+                st.restart_iter += p.op_steps;
+                st.iteration += p.op_steps;
+                logic_after_halpern_check(st, p, pria_check_sums(sums, job.scale));
+            }
+        } else {  // kActRestart
+            pria_wg_step(a, pria_step_scalars<Real>(0, omega, p.eta, false, true), lds, red);
+            pria_wg_copy3(a.x0, a.x, a.y0, a.y, a.z0, a.z, a.ND);
+            pria_wg_sums(a, a.sx, a.sy, a.sz, red, sums);
+            if (threadIdx.x == 0) logic_after_restart(st, p, pria_check_sums(sums, job.scale));
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && job.time_limit_ticks > 0 &&
+            wall_clock64() - t_start > job.time_limit_ticks && st.action != kActDoneX &&
+            st.action != kActDoneSX) {
+            st.action = (action == kActWarmBlock || action == kActRestart) ? kActDoneX : kActDoneSX;
+            st.status = 2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *job.state = st;
+}
+
+// ---------------------------------------------------------------------------
+// device tier
+// ---------------------------------------------------------------------------
+
+constexpr int kPriaXBlock = 256;
+constexpr int kPriaYZBlock = 1024;
+constexpr int kPriaCheckBlock = 256;
+
+template <class Real>
+__global__ void __launch_bounds__(256) pria_x_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
+    extern __shared__ char dyn[];
+    constexpr int TW = PriaShape<Real>::TW;
+    const int tiles = (a.T + TW - 1) / TW;
+    // heaviest tiles (largest t) first
+    const int b = blockIdx.x;
+    const int q = tiles - 1 - (b % tiles), i = b / tiles;
+    pria_x_tile<Real, TW>(a, st, i, q * TW, reinterpret_cast<Real*>(dyn));
+}
+
+template <class Real>
+__global__ void __launch_bounds__(1024) pria_yz_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
+    __shared__ double red[2 * 16];
+    pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, blockIdx.x, red);
+}
+
+template <class Real>
+__global__ void __launch_bounds__(256) pria_check_slice_kernel(PriaArrays<Real> a, const Real* bx,
+                                                               double* partial) {
+    __shared__ double red[kPriaSums * 16];
+    double acc[kPriaSums];
+    for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
+    pria_slice_sums(a, blockIdx.x, bx, red, acc);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kPriaSums; ++q) partial[blockIdx.x * kPriaSums + q] = acc[q];
+}
+
+template <class Real>
+__global__ void __launch_bounds__(256) pria_check_column_kernel(PriaArrays<Real> a, const Real* by,
+                                                                const Real* bz, double* partial) {
+    __shared__ double red[16];
+    constexpr int TW = PriaShape<Real>::TW;
+    const int tiles = (a.T + TW - 1) / TW;
+    double acc[kPriaSums];
+    for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
+    pria_column_sums<Real, TW>(a, blockIdx.x / tiles, (blockIdx.x % tiles) * TW, by, bz, red, acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc[kPLower];
+}
+
+// adds rows in a fixed order: slices first, then column tiles
+__global__ void pria_sums_final(const double* slices, int nslices, const double* cols, int ncols,
+                                double* out) {
+    __shared__ double red[kPriaSums * 16];
+    double acc[kPriaSums];
+    for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
+    for (int s = threadIdx.x; s < nslices; s += blockDim.x)
+        for (int q = 0; q < kPLower; ++q) acc[q] += slices[s * kPriaSums + q];
+    for (int c = threadIdx.x; c < ncols; c += blockDim.x) acc[kPLower] += cols[c];
+    block_sums<kPriaSums>(acc, red);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kPriaSums; ++q) out[q] = acc[q];
+}
+
+template <class Real>
+__global__ void copy3_kernel(Real* d1, const Real* s1, Real* d2, const Real* s2, Real* d3,
+                             const Real* s3, int nd) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nd) {
+        d1[e] = s1[e];
+        d2[e] = s2[e];
+        d3[e] = s3[e];
+    }
+}
+
+size_t pria_device_work_bytes(int N, int T) {
+    constexpr int TW = 32;  // the smaller tile width bounds the column rows
+    size_t tiles = size_t(N) * ((T + TW - 1) / TW);
+    return sizeof(double) * (size_t(T) * kPriaSums + tiles + kPriaSums + 8);
+}
+
+template <class Real>
+struct PriaDevice {
+    const PriaArrays<Real>& a;
+    hipStream_t stream;
+    double* slices;
+    double* cols;
+    double* dsums;
+
+    int step(const PriaStep<Real>& st) {
+        constexpr int TW = PriaShape<Real>::TW;
+        const int tiles = (a.T + TW - 1) / TW;
+        hipLaunchKernelGGL(pria_x_kernel<Real>, dim3(a.N * tiles), dim3(kPriaXBlock),
+                           pria_x_lds_bytes<Real>(a.T), stream, a, st);
+        hipLaunchKernelGGL(pria_yz_kernel<Real>, dim3(a.T), dim3(kPriaYZBlock), 0, stream, a, st);
+        return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+    }
+    int copy3(Real* d1, const Real* s1, Real* d2, const Real* s2, Real* d3, const Real* s3) {
+        hipLaunchKernelGGL(copy3_kernel<Real>, dim3((a.ND + 255) / 256), dim3(256), 0, stream, d1,
+                           s1, d2, s2, d3, s3, a.ND);
+        return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+    }
+    int sums(const Real* bx, const Real* by, const Real* bz, double* host) {
+        constexpr int TW = PriaShape<Real>::TW;
+        const int tiles = (a.T + TW - 1) / TW;
+        hipLaunchKernelGGL(pria_check_slice_kernel<Real>, dim3(a.T), dim3(kPriaCheckBlock), 0,
+                           stream, a, bx, slices);
+        hipLaunchKernelGGL(pria_check_column_kernel<Real>, dim3(a.N * tiles), dim3(kPriaCheckBlock),
+                           0, stream, a, by, bz, cols);
+        hipLaunchKernelGGL(pria_sums_final, dim3(1), dim3(256), 0, stream, slices, a.T, cols,
+                           a.N * tiles, dsums);
+        if (hipGetLastError() != hipSuccess) return kPriaErrorDevice;
+        if (hipMemcpyAsync(host, dsums, sizeof(double) * kPriaSums, hipMemcpyDeviceToHost,
+                           stream) != hipSuccess)
+            return kPriaErrorDevice;
+        return hipStreamSynchronize(stream) == hipSuccess ? 0 : kPriaErrorDevice;
+    }
+};
+
+template <class Real>
+int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
+                      double time_limit_seconds, hipStream_t stream) {
+    const PriaArrays<Real>& a = job.a;
+    const LogicParams& p = job.p;
+    constexpr int TW = PriaShape<Real>::TW;
+    const int tiles = (a.T + TW - 1) / TW;
+    PriaDevice<Real> d{a, stream, work, work + size_t(a.T) * kPriaSums,
+                       work + size_t(a.T) * kPriaSums + size_t(a.N) * tiles};
+    if (pria_x_lds_bytes<Real>(a.T) > 65536) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(pria_x_kernel<Real>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                int(pria_x_lds_bytes<Real>(a.T))) != hipSuccess)
+            return kPriaErrorDevice;
+    }
+    double sums[kPriaSums];
+    auto t0 = std::chrono::steady_clock::now();
+
+    if (d.copy3(a.x, a.x0, a.y, a.y0, a.z, a.z0) || d.copy3(a.sx, a.x0, a.sy, a.y0, a.sz, a.z0))
+        return kPriaErrorDevice;
+    logic_init(st, p);
+    if (d.step(pria_step_scalars<Real>(0, p.omega, p.eta, false, true))) return kPriaErrorDevice;
+    if (d.sums(a.x, a.y, a.z, sums)) return kPriaErrorDevice;
+    logic_after_warm_check(st, p, pria_check_sums(sums, job.scale));
+
+    for (;;) {
+        const int action = st.action;
+        const int64_t r0 = st.restart_iter;
+        const double omega = st.omega;
+        if (action == kActDoneX) break;
+        if (action == kActDoneSX) {
+            if (d.copy3(a.x, a.sx, a.y, a.sy, a.z, a.sz)) return kPriaErrorDevice;
+            break;
+        }
+        if (action == kActWarmBlock) {
+            for (int64_t j = 0; j < p.op_steps; ++j)
+                if (d.step(pria_step_scalars<Real>(r0 + j, omega, p.eta, false, true)))
+                    return kPriaErrorDevice;
+            if (d.sums(a.x, a.y, a.z, sums)) return kPriaErrorDevice;
+            st.restart_iter += p.op_steps;
+            st.iteration += p.op_steps;
+            logic_after_warm_check(st, p, pria_check_sums(sums, job.scale));
+        } else if (action == kActHalpernBlock) {
+            if (st.anchor_pending) {
+                if (d.copy3(a.x0, a.x, a.y0, a.y, a.z0, a.z)) return kPriaErrorDevice;
+                st.anchor_pending = 0;
+            }
+            for (int64_t j = 0; j < p.op_steps; ++j)
+                if (d.step(pria_step_scalars<Real>(r0 + j, omega, p.eta, true,
+                                                   j + 1 == p.op_steps)))
+                    return kPriaErrorDevice;
+            if (d.sums(a.sx, a.sy, a.sz, sums)) return kPriaErrorDevice;
+            st.restart_iter += p.op_steps;
+            st.iteration += p.op_steps;
+            logic_after_halpern_check(st, p, pria_check_sums(sums, job.scale));
+        } else {
+            if (d.step(pria_step_scalars<Real>(0, omega, p.eta, false, true))) return kPriaErrorDevice;
+            if (d.copy3(a.x0, a.x, a.y0, a.y, a.z0, a.z)) return kPriaErrorDevice;
+            if (d.sums(a.sx, a.sy, a.sz, sums)) return kPriaErrorDevice;
+            logic_after_restart(st, p, pria_check_sums(sums, job.scale));
+        }
+        if (time_limit_seconds > 0.0 &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >
+                time_limit_seconds &&
+            st.action != kActDoneX && st.action != kActDoneSX) {
+            st.action = (action == kActWarmBlock || action == kActRestart) ? kActDoneX : kActDoneSX;
+            st.status = 2;
+        }
+    }
+    return hipStreamSynchronize(stream) == hipSuccess ? 0 : kPriaErrorDevice;
+}
+
+template int pria_device_solve<float>(const PriaJob<float>&, LogicState&, double*, double,
+                                      hipStream_t);
+template int pria_device_solve<double>(const PriaJob<double>&, LogicState&, double*, double,
+                                       hipStream_t);
+
+template <class Real>
+int pria_launch_wg(const PriaJob<Real>* jobs, int count, int max_T, hipStream_t stream) {
+    if (count <= 0) return 0;
+    size_t lds = pria_x_lds_bytes<Real>(max_T);
+    if (lds > 65536) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(pria_wg_solve<Real>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+            return kPriaErrorDevice;
+    }
+    hipLaunchKernelGGL(pria_wg_solve<Real>, dim3(count), dim3(1024), lds, stream, jobs);
+    return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+}
+
+template int pria_launch_wg<float>(const PriaJob<float>*, int, int, hipStream_t);
+template int pria_launch_wg<double>(const PriaJob<double>*, int, int, hipStream_t);
+
+}  // namespace reform

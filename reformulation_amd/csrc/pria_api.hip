@@ -1,0 +1,362 @@
+// The C ABI of include/pria_hip.h.  Entry points follow
+// jrp/pria-cuda/src/pria.cu:15-134; the batch runs every instance of a
+// same-shape batch as one workgroup of one launch instead of a thread fan-out.
+#include "pria_hip.h"
+#include "pria_host.hpp"
+
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+using namespace reform;
+
+namespace {
+
+constexpr double kDefaultWgLimitSeconds = 600.0;
+
+// Below this many entries (ND) the one-workgroup tier runs a plain solve.
+int64_t pria_wg_max() {
+    static int64_t v = [] {
+        const char* env = std::getenv("REFORM_PRIA_WG_MAX");
+        return env ? std::atoll(env) : int64_t(1) << 15;
+    }();
+    return v;
+}
+
+double wall_clock_rate_hz() {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0.0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return 0.0;
+    return double(khz) * 1000.0;
+}
+
+struct Buffer {
+    void* ptr = nullptr;
+    ~Buffer() {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+bool params_valid(const pria_params* p) {
+    return p && p->omega > 0.0 && p->eta > 0.0 && p->tolerance >= 0.0;
+}
+
+template <class Real>
+int solve_typed(const PriaHostInstance& s, const LogicParams& lp, const pria_options& o,
+                std::vector<Real>* xo, std::vector<Real>* yo, std::vector<Real>* zo,
+                pria_info* info) {
+    PriaLayout L = pria_layout(s.N, s.ND, sizeof(Real));
+    const size_t job_off = pria_align256(L.total_bytes);
+    const size_t state_off = pria_align256(job_off + sizeof(PriaJob<Real>));
+    const size_t work_off = pria_align256(state_off + sizeof(LogicState));
+    const size_t bytes = work_off + pria_device_work_bytes(s.N, s.T);
+    Buffer buf;
+    if (hipMalloc(&buf.ptr, bytes) != hipSuccess) return kPriaErrorDevice;
+    char* base = static_cast<char*>(buf.ptr);
+    std::vector<char> image(L.input_bytes, 0);
+    pria_fill_image<Real>(s, L, image.data());
+    Stream stream;
+    if (hipStreamCreateWithFlags(&stream.s, hipStreamNonBlocking) != hipSuccess)
+        return kPriaErrorDevice;
+    if (hipMemcpyAsync(base, image.data(), L.input_bytes, hipMemcpyHostToDevice, stream.s) !=
+        hipSuccess)
+        return kPriaErrorDevice;
+
+    PriaJob<Real> job;
+    std::memset(&job, 0, sizeof(job));
+    job.a = pria_bind<Real>(L, base, s);
+    job.p = lp;
+    job.scale = s.scale;
+    job.state = reinterpret_cast<LogicState*>(base + state_off);
+
+    int tier = o.tier;
+    if (tier == PRIA_TIER_AUTO) tier = s.ND <= pria_wg_max() ? PRIA_TIER_WORKGROUP : PRIA_TIER_DEVICE;
+    LogicState st;
+    if (tier == PRIA_TIER_WORKGROUP) {
+        double limit = o.time_limit > 0.0 ? std::min(o.time_limit, kDefaultWgLimitSeconds)
+                                          : kDefaultWgLimitSeconds;
+        job.time_limit_ticks = (long long)(limit * wall_clock_rate_hz());
+        if (hipMemcpyAsync(base + job_off, &job, sizeof(job), hipMemcpyHostToDevice, stream.s) !=
+            hipSuccess)
+            return kPriaErrorDevice;
+        if (pria_launch_wg<Real>(reinterpret_cast<PriaJob<Real>*>(base + job_off), 1, s.T,
+                                 stream.s))
+            return kPriaErrorDevice;
+        if (hipMemcpyAsync(&st, job.state, sizeof(st), hipMemcpyDeviceToHost, stream.s) !=
+            hipSuccess)
+            return kPriaErrorDevice;
+        if (hipStreamSynchronize(stream.s) != hipSuccess) return kPriaErrorDevice;
+    } else {
+        if (pria_device_solve<Real>(job, st, reinterpret_cast<double*>(base + work_off),
+                                    o.time_limit, stream.s))
+            return kPriaErrorDevice;
+    }
+    std::vector<Real>* outs[3] = {xo, yo, zo};
+    Real* srcs[3] = {job.a.x, job.a.y, job.a.z};
+    for (int q = 0; q < 3; ++q) {
+        if (!outs[q]) continue;
+        outs[q]->resize(s.ND);
+        if (hipMemcpyAsync(outs[q]->data(), srcs[q], sizeof(Real) * s.ND, hipMemcpyDeviceToHost,
+                           stream.s) != hipSuccess)
+            return kPriaErrorDevice;
+    }
+    if (hipStreamSynchronize(stream.s) != hipSuccess) return kPriaErrorDevice;
+    if (info) {
+        info->iterations = st.iteration;
+        info->restarts = st.restarts;
+        info->checks = st.checks;
+        info->lower = st.lower;
+        info->upper_iterate = st.upper;
+        info->relative_gap = st.gap;
+        info->omega = st.omega;
+        info->scale = s.scale;
+        info->status = st.status;
+        info->tier = tier;
+    }
+    return 0;
+}
+
+// Shared body of every single-instance entry: FP64 in and out.
+int solve_f64io(int64_t n, int64_t t, const double* k, double k0, const double* h,
+                const pria_params* params, const pria_options* options, int use_float32,
+                const double* wx, const double* wy, const double* wz, double* x, double* y,
+                double* z, pria_info* info) {
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
+        info->status = kPriaErrorArgument;
+    }
+    if (!params_valid(params)) return kPriaErrorArgument;
+    pria_options o = options ? *options : pria_default_options();
+    PriaHostInstance s;
+    if (pria_build_host(n, t, k, k0, h, wx, wy, wz, s)) return kPriaErrorArgument;
+    LogicParams lp = pria_logic_params(*params);
+    pria_info local;
+    pria_info* out = info ? info : &local;
+    std::vector<double> xd(size_t(n) * t * t);
+    int status;
+    if (use_float32) {
+        std::vector<float> vx, vy, vz;
+        status = solve_typed<float>(s, lp, o, &vx, y ? &vy : nullptr, z ? &vz : nullptr, out);
+        if (status == 0) {
+            pria_unpack(s, vx, 0, xd.data());
+            if (y) pria_unpack(s, vy, 1, y);
+            if (z) pria_unpack(s, vz, 2, z);
+        }
+    } else {
+        std::vector<double> vx, vy, vz;
+        status = solve_typed<double>(s, lp, o, &vx, y ? &vy : nullptr, z ? &vz : nullptr, out);
+        if (status == 0) {
+            pria_unpack(s, vx, 0, xd.data());
+            if (y) pria_unpack(s, vy, 1, y);
+            if (z) pria_unpack(s, vz, 2, z);
+        }
+    }
+    if (status) {
+        out->status = status;
+        return status;
+    }
+    out->upper = pria_objective_host(n, t, k, k0, h, xd.data());
+    if (x) std::memcpy(x, xd.data(), sizeof(double) * xd.size());
+    return out->status == 0 ? 0 : 1;
+}
+
+template <class Real>
+std::vector<double> widen(const Real* p, size_t count) {
+    std::vector<double> v(count);
+    for (size_t q = 0; q < count; ++q) v[q] = double(p[q]);
+    return v;
+}
+
+// pria_solve_f64/f32: the reference's return convention, 0 or a device code.
+template <class Real>
+int solve_reference_abi(int64_t n, int64_t t, const Real* k, Real k0, const Real* h,
+                        const pria_params* params, const Real* wx, const Real* wy,
+                        const Real* wz, Real* x, Real* y, Real* z, pria_result_info* info) {
+    if (n <= 0 || t <= 0 || !k || !h || !params_valid(params)) return kPriaErrorArgument;
+    const size_t sz = size_t(n) * t * t;
+    std::vector<double> kd = widen(k, n), hd = widen(h, sz), wxd, wyd, wzd;
+    if (wx) wxd = widen(wx, sz);
+    if (wy) wyd = widen(wy, sz);
+    if (wz) wzd = widen(wz, sz);
+    std::vector<double> xd(sz), yd(sz), zd(sz);
+    pria_info pi;
+    int st = solve_f64io(n, t, kd.data(), double(k0), hd.data(), params, nullptr,
+                         sizeof(Real) == 4, wx ? wxd.data() : nullptr, wy ? wyd.data() : nullptr,
+                         wz ? wzd.data() : nullptr, xd.data(), y ? yd.data() : nullptr,
+                         z ? zd.data() : nullptr, &pi);
+    if (st < 0) return st;
+    for (size_t q = 0; q < sz; ++q) {
+        if (x) x[q] = Real(xd[q]);
+        if (y) y[q] = Real(yd[q]);
+        if (z) z[q] = Real(zd[q]);
+    }
+    if (info) {
+        info->iterations = pi.iterations;
+        info->primal_objective = pi.upper_iterate;
+        info->dual_objective = pi.lower;
+        info->relative_gap = pi.relative_gap;
+    }
+    return 0;
+}
+
+template <class Real>
+int solve_batch(int64_t count, int64_t n, int64_t t, const Real* k, const Real* k0,
+                const Real* h, const pria_params* params, Real* x, Real* y, Real* z,
+                pria_result_info* infos) {
+    if (count <= 0 || n <= 0 || t <= 0 || !k || !k0 || !h || !params_valid(params))
+        return kPriaErrorArgument;
+    const size_t sz = size_t(n) * t * t;
+    LogicParams lp = pria_logic_params(*params);
+    std::vector<PriaHostInstance> hosts(count);
+    for (int64_t b = 0; b < count; ++b) {
+        std::vector<double> kd = widen(k + b * n, n), hd = widen(h + b * sz, sz);
+        if (pria_build_host(n, t, kd.data(), double(k0[b]), hd.data(), nullptr, nullptr, nullptr,
+                            hosts[b]))
+            return kPriaErrorArgument;
+    }
+    // every instance as one workgroup of one launch (same code as the solo
+    // workgroup tier, so each result is what a solo workgroup solve gives)
+    const PriaHostInstance& s0 = hosts[0];
+    PriaLayout L = pria_layout(s0.N, s0.ND, sizeof(Real));
+    const size_t per = pria_align256(L.total_bytes);
+    const size_t jobs_off = per * count;
+    const size_t states_off = pria_align256(jobs_off + sizeof(PriaJob<Real>) * count);
+    const size_t bytes = states_off + sizeof(LogicState) * count;
+    Buffer buf;
+    if (hipMalloc(&buf.ptr, bytes) != hipSuccess) return kPriaErrorDevice;
+    char* base = static_cast<char*>(buf.ptr);
+    Stream stream;
+    if (hipStreamCreateWithFlags(&stream.s, hipStreamNonBlocking) != hipSuccess)
+        return kPriaErrorDevice;
+    std::vector<char> image(L.input_bytes);
+    std::vector<PriaJob<Real>> jobs(count);
+    for (int64_t b = 0; b < count; ++b) {
+        std::fill(image.begin(), image.end(), 0);
+        pria_fill_image<Real>(hosts[b], L, image.data());
+        if (hipMemcpy(base + per * b, image.data(), L.input_bytes, hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return kPriaErrorDevice;
+        std::memset(&jobs[b], 0, sizeof(PriaJob<Real>));
+        jobs[b].a = pria_bind<Real>(L, base + per * b, hosts[b]);
+        jobs[b].p = lp;
+        jobs[b].scale = hosts[b].scale;
+        jobs[b].time_limit_ticks = (long long)(kDefaultWgLimitSeconds * wall_clock_rate_hz());
+        jobs[b].state = reinterpret_cast<LogicState*>(base + states_off) + b;
+    }
+    if (hipMemcpyAsync(base + jobs_off, jobs.data(), sizeof(PriaJob<Real>) * count,
+                       hipMemcpyHostToDevice, stream.s) != hipSuccess)
+        return kPriaErrorDevice;
+    if (pria_launch_wg<Real>(reinterpret_cast<PriaJob<Real>*>(base + jobs_off), int(count), s0.T,
+                             stream.s))
+        return kPriaErrorDevice;
+    std::vector<LogicState> states(count);
+    if (hipMemcpyAsync(states.data(), base + states_off, sizeof(LogicState) * count,
+                       hipMemcpyDeviceToHost, stream.s) != hipSuccess)
+        return kPriaErrorDevice;
+    if (hipStreamSynchronize(stream.s) != hipSuccess) return kPriaErrorDevice;
+    std::vector<Real> v(s0.ND);
+    std::vector<double> dense(sz);
+    for (int64_t b = 0; b < count; ++b) {
+        Real* outs[3] = {x, y, z};
+        Real* srcs[3] = {jobs[b].a.x, jobs[b].a.y, jobs[b].a.z};
+        for (int q = 0; q < 3; ++q) {
+            if (!outs[q]) continue;
+            if (hipMemcpy(v.data(), srcs[q], sizeof(Real) * s0.ND, hipMemcpyDeviceToHost) !=
+                hipSuccess)
+                return kPriaErrorDevice;
+            pria_unpack(hosts[b], v, q, dense.data());
+            for (size_t e = 0; e < sz; ++e) outs[q][b * sz + e] = Real(dense[e]);
+        }
+        if (infos) {
+            infos[b].iterations = states[b].iteration;
+            infos[b].primal_objective = states[b].upper;
+            infos[b].dual_objective = states[b].lower;
+            infos[b].relative_gap = states[b].gap;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" pria_params pria_params_auto(int64_t max_iter, double tolerance) {
+    pria_params p;
+    p.omega = 0.10835;
+    p.eta = 0.999 / std::sqrt(2.0);
+    p.beta_sufficient = std::exp(-1.0);
+    p.beta_artificial = 0.2468;
+    p.beta_necessary = 0.5706;
+    p.theta_initial = 0.81930;
+    p.theta_target = 0.42818;
+    p.theta_decay = 0.90474;
+    p.tolerance = tolerance;
+    p.max_iter = max_iter;
+    p.use_halpern = 1;
+    p.warm_steps = 0;
+    p.op_steps = 16;
+    p.verbose = 0;
+    return p;
+}
+
+extern "C" pria_options pria_default_options(void) {
+    pria_options o;
+    o.tier = PRIA_TIER_AUTO;
+    o.time_limit = 0.0;
+    return o;
+}
+
+extern "C" int pria_solve_ex(int64_t n, int64_t t, const double* order_costs, double joint_cost,
+                             const double* holding_costs, const pria_params* params,
+                             const pria_options* options, int use_float32, const double* warm_x,
+                             const double* warm_y, const double* warm_z, double* x, double* y,
+                             double* z, pria_info* info) {
+    return solve_f64io(n, t, order_costs, joint_cost, holding_costs, params, options, use_float32,
+                       warm_x, warm_y, warm_z, x, y, z, info);
+}
+
+extern "C" int pria_solve_f64(int64_t n, int64_t t, const double* order_costs, double joint_cost,
+                              const double* holding_costs, const pria_params* params,
+                              const double* warm_x, const double* warm_y, const double* warm_z,
+                              double* x, double* y, double* z, pria_result_info* info) {
+    return solve_reference_abi<double>(n, t, order_costs, joint_cost, holding_costs, params, warm_x,
+                                       warm_y, warm_z, x, y, z, info);
+}
+
+extern "C" int pria_solve_f32(int64_t n, int64_t t, const float* order_costs, float joint_cost,
+                              const float* holding_costs, const pria_params* params,
+                              const float* warm_x, const float* warm_y, const float* warm_z,
+                              float* x, float* y, float* z, pria_result_info* info) {
+    return solve_reference_abi<float>(n, t, order_costs, joint_cost, holding_costs, params, warm_x,
+                                      warm_y, warm_z, x, y, z, info);
+}
+
+extern "C" int pria_solve_batch_f64(int64_t count, int64_t n, int64_t t, const double* order_costs,
+                                    const double* joint_costs, const double* holding_costs,
+                                    const pria_params* params, double* x, double* y, double* z,
+                                    pria_result_info* infos) {
+    return solve_batch<double>(count, n, t, order_costs, joint_costs, holding_costs, params, x, y,
+                               z, infos);
+}
+
+extern "C" int pria_solve_batch_f32(int64_t count, int64_t n, int64_t t, const float* order_costs,
+                                    const float* joint_costs, const float* holding_costs,
+                                    const pria_params* params, float* x, float* y, float* z,
+                                    pria_result_info* infos) {
+    return solve_batch<float>(count, n, t, order_costs, joint_costs, holding_costs, params, x, y,
+                              z, infos);
+}
+
+extern "C" double pria_objective_f64(int64_t n, int64_t t, const double* order_costs,
+                                     double joint_cost, const double* holding_costs,
+                                     const double* x) {
+    return pria_objective_host(n, t, order_costs, joint_cost, holding_costs, x);
+}

@@ -1,0 +1,454 @@
+// PRIA device code: the fused passes of one PDHG step on the JRP LP and the
+// check-pass sums.  Shared by the one-workgroup whole-solve kernel and the
+// device-wide kernels.
+//
+// Layout in HBM.  T' is T padded to even, N items.  Every per-entry array
+// holds the upper triangle s <= t in "slice" order: slice s (one order period)
+// is the N (T' - s) entries {(i, t) : t >= s}, item-major, so
+//     idx(s, i, t) = off(s) + i (T' - s) + (t - s),
+//     off(s)       = N (s T' - s (s - 1) / 2),
+// and ND = N T' (T' + 1) / 2 entries in all.  In this order each projection
+// segment of y (row (i, s), target K_i) and z (slice s, target K0) is
+// contiguous; the x segments (column (i, t) over s <= t, target 1) are read
+// in tiles of consecutive t, one coalesced row of the tile per s.
+//
+// Arrays: h (scaled holding cost), and for v in {x, y, z}: vc (the
+// Halpern-combined point the next step starts from), v0 (anchor), v and sv
+// (the last point and its step, written only when a check or restart
+// follows), plus r = sigma (2 x2 - x1), which the x pass hands to the y/z pass.
+//
+// Per step:  x pass   x2 = P_X(x1 - tau (h + y1 + z1)), r, xc <- blend
+//            yz pass  y2 = P_Y(y1 + r), z2 = P_Z(z1 + r), yc, zc <- blend
+// Reference: _raw_step jrp/pria-jax/pria/solver.py:409-425, _reflect_step
+// :428-447, bounds :347-402, canonical norm halpern_operator.py:5-19.
+// Simplex projections use Michelot's method with FP64 accumulation: theta is
+// (sum of active - target) / |active|, repeated until the active set stops
+// shrinking; the same threshold the sort-based double_simplex_projection
+// (jrp/pria-jax/pria/projection.py:71-130) computes.  A target <= 0 gives 0.
+#pragma once
+
+#include "common.hpp"
+#include "halpern.hpp"
+
+namespace reform {
+
+template <class Real>
+struct PriaArrays {
+    int N, T;          // items, padded periods
+    int ND;
+    const Real* h;
+    const Real* k;     // [N] scaled K_i
+    Real k0;           // scaled K0
+    Real *xc, *x0, *x, *sx;
+    Real *yc, *y0, *y, *sy;
+    Real *zc, *z0, *z, *sz;
+    Real* r;
+};
+
+__host__ __device__ inline int pria_off(int N, int T, int s) {
+    return N * (s * T - (s * (s - 1)) / 2);
+}
+
+template <class Real>
+struct PriaStep {
+    Real lwn, rwn;     // weights of the next step (for the vc prediction)
+    Real tau, sigma;
+    bool reflect;      // start from vc (Halpern) or from sv (vanilla)
+    bool store;        // write v, sv
+};
+
+template <class Real>
+__host__ __device__ inline PriaStep<Real> pria_step_scalars(int64_t restart_iter, double omega,
+                                                            double eta, bool reflect, bool store) {
+    PriaStep<Real> s;
+    double lwn, rwn;
+    halpern_weights(reflect ? restart_iter + 1 : 0, &lwn, &rwn);
+    s.lwn = Real(lwn);
+    s.rwn = Real(rwn);
+    Real re = Real(eta), ro = Real(omega);
+    s.tau = re / ro;
+    s.sigma = re * ro;
+    s.reflect = reflect;
+    s.store = store || !reflect;
+    return s;
+}
+
+// --- x pass: one tile of TW consecutive columns t0.. of item i ------------
+//
+// lds: (T' x (TW + 1)) pre-projection values + TW thresholds.
+template <class Real, int TW>
+__device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st, int i, int t0,
+                            Real* lds) {
+    const int T = a.T, N = a.N;
+    const int BS = blockDim.x, tid = threadIdx.x;
+    const int G = BS / TW;            // threads per column
+    const int stride = TW + 1;
+    Real* theta = lds + T * stride;
+    const int tcount = min(TW, T - t0);
+    const int smax = t0 + tcount - 1;
+    const Real* srcx = st.reflect ? a.xc : a.sx;
+    const Real* srcy = st.reflect ? a.yc : a.sy;
+    const Real* srcz = st.reflect ? a.zc : a.sz;
+
+    // load: lanes over columns, G rows of s at a time (coalesced per s)
+    {
+        const int c = tid % TW, row = tid / TW, t = t0 + c;
+        for (int s = row; s <= smax; s += G) {
+            if (c < tcount && t >= s) {
+                int g = pria_off(N, T, s) + i * (T - s) + (t - s);
+                Real x1 = srcx[g];
+                lds[s * stride + c] = x1 - st.tau * ((a.h[g] + srcz[g]) + srcy[g]);
+            }
+        }
+    }
+    __syncthreads();
+    // project: G lanes per column, spread across the wave (column = wave-local
+    // index, q = the lane's share), reduced by xor over the q bits
+    {
+        const int cpw = kWave / G;                        // columns per wave
+        const int lane = tid & (kWave - 1), wave = tid / kWave;
+        const int c = wave * cpw + (lane % cpw), q = lane / cpw;
+        const int t = t0 + c;
+        const int len = c < tcount ? t + 1 : 0;
+        double S = 0.0, C = 0.0;
+        for (int s = q; s < len; s += G) {
+            S += double(lds[s * stride + c]);
+            C += 1.0;
+        }
+        for (int off = cpw; off < kWave; off <<= 1) {
+            S += __shfl_xor(S, off, kWave);
+            C += __shfl_xor(C, off, kWave);
+        }
+        double th = len > 0 ? (S - 1.0) / C : 0.0;
+        bool done = len == 0;
+        while (__any(!done)) {
+            double S2 = 0.0, C2 = 0.0;
+            if (!done) {
+                for (int s = q; s < len; s += G) {
+                    double v = double(lds[s * stride + c]);
+                    if (v > th) {
+                        S2 += v;
+                        C2 += 1.0;
+                    }
+                }
+            }
+            for (int off = cpw; off < kWave; off <<= 1) {
+                S2 += __shfl_xor(S2, off, kWave);
+                C2 += __shfl_xor(C2, off, kWave);
+            }
+            if (!done) {
+                if (C2 >= C || C2 == 0.0) {
+                    done = true;
+                } else {
+                    C = C2;
+                    th = (S2 - 1.0) / C2;
+                }
+            }
+        }
+        if (q == 0 && c < TW) theta[c] = Real(th);
+    }
+    __syncthreads();
+    // write: x2, r, the next xc (and x, sx when storing)
+    {
+        const int c = tid % TW, row = tid / TW, t = t0 + c;
+        const Real th = theta[c < TW ? c : 0];
+        for (int s = row; s <= smax; s += G) {
+            if (c < tcount && t >= s) {
+                int g = pria_off(N, T, s) + i * (T - s) + (t - s);
+                Real x1 = srcx[g];
+                Real d = lds[s * stride + c] - th;
+                Real x2 = d > Real(0) ? d : Real(0);
+                Real anchor = st.reflect ? a.x0[g] : x1;
+                Real refl = Real(2) * x2 - x1;
+                a.r[g] = st.sigma * refl;
+                a.xc[g] = st.lwn * refl + st.rwn * anchor;
+                if (st.store) {
+                    a.x[g] = x1;
+                    a.sx[g] = x2;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Michelot threshold over one wavefront's register-held values (VR per lane),
+// target `target`; len entries are valid (lane-major: e = lane + 64 k).
+template <int VR>
+__device__ __forceinline__ double wave_michelot(const double (&u)[VR], int len, double target) {
+    const int lane = threadIdx.x & (kWave - 1);
+    double S = 0.0, C = 0.0;
+#pragma unroll
+    for (int k = 0; k < VR; ++k) {
+        int e = lane + kWave * k;
+        if (e < len) {
+            S += u[k];
+            C += 1.0;
+        }
+    }
+    S = wave_sum(S);
+    C = wave_sum(C);
+    double th = (S - target) / C;
+    for (int round = 0; round <= len; ++round) {
+        double S2 = 0.0, C2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < VR; ++k) {
+            int e = lane + kWave * k;
+            if (e < len && u[k] > th) {
+                S2 += u[k];
+                C2 += 1.0;
+            }
+        }
+        S2 = wave_sum(S2);
+        C2 = wave_sum(C2);
+        if (C2 >= C || C2 == 0.0) break;
+        C = C2;
+        th = (S2 - target) / C2;
+    }
+    return th;
+}
+
+// --- y/z pass: one slice s -------------------------------------------------
+//
+// y rows: one wavefront per row (i, s), values in registers (T' <= 64 VR).
+// z slice: the whole workgroup, values in registers when the slice fits
+// (N (T' - s) <= blockDim * VZ), else re-read from memory each round.
+template <class Real, int VR, int VZ>
+__device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& st, int s,
+                              double* red) {
+    const int T = a.T, N = a.N;
+    const int BS = blockDim.x, tid = threadIdx.x;
+    const int lane = tid & (kWave - 1), wave = tid / kWave, waves = BS / kWave;
+    const int len = T - s;
+    const int base = pria_off(N, T, s);
+    const Real* srcy = st.reflect ? a.yc : a.sy;
+    const Real* srcz = st.reflect ? a.zc : a.sz;
+
+    // ---- y: rows of the slice ----
+    for (int i = wave; i < N; i += waves) {
+        const int rb = base + i * len;
+        double y1[VR], u[VR];
+#pragma unroll
+        for (int k = 0; k < VR; ++k) {
+            int e = lane + kWave * k;
+            if (e < len) {
+                y1[k] = double(srcy[rb + e]);
+                u[k] = double(Real(Real(y1[k]) + a.r[rb + e]));
+            } else {
+                y1[k] = 0.0;
+                u[k] = 0.0;
+            }
+        }
+        const double target = double(a.k[i]);
+        const double th = target > 0.0 ? wave_michelot<VR>(u, len, target) : 0.0;
+#pragma unroll
+        for (int k = 0; k < VR; ++k) {
+            int e = lane + kWave * k;
+            if (e < len) {
+                int g = rb + e;
+                Real v1 = Real(y1[k]);
+                Real d = Real(u[k]) - Real(th);
+                Real y2 = target > 0.0 ? (d > Real(0) ? d : Real(0)) : Real(0);
+                Real anchor = st.reflect ? a.y0[g] : v1;
+                a.yc[g] = st.lwn * (Real(2) * y2 - v1) + st.rwn * anchor;
+                if (st.store) {
+                    a.y[g] = v1;
+                    a.sy[g] = y2;
+                }
+            }
+        }
+    }
+
+    // ---- z: the whole slice ----
+    const int L = N * len;
+    const double target = double(a.k0);
+    double th = 0.0;
+    if (target > 0.0) {
+        if (L <= BS * VZ) {
+            double u[VZ];
+            double S = 0.0, C = 0.0;
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) {
+                int e = tid + BS * k;
+                u[k] = 0.0;
+                if (e < L) {
+                    u[k] = double(Real(srcz[base + e] + a.r[base + e]));
+                    S += u[k];
+                    C += 1.0;
+                }
+            }
+            double v2[2] = {S, C};
+            block_sums<2>(v2, red);
+            S = v2[0];
+            C = v2[1];
+            th = (S - target) / C;
+            for (int round = 0; round <= L; ++round) {
+                double S2 = 0.0, C2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < VZ; ++k) {
+                    int e = tid + BS * k;
+                    if (e < L && u[k] > th) {
+                        S2 += u[k];
+                        C2 += 1.0;
+                    }
+                }
+                double w2[2] = {S2, C2};
+                block_sums<2>(w2, red);
+                if (w2[1] >= C || w2[1] == 0.0) break;
+                C = w2[1];
+                th = (w2[0] - target) / w2[1];
+            }
+        } else {
+            double S = 0.0, C = 0.0;
+            for (int e = tid; e < L; e += BS) {
+                S += double(Real(srcz[base + e] + a.r[base + e]));
+                C += 1.0;
+            }
+            double v2[2] = {S, C};
+            block_sums<2>(v2, red);
+            S = v2[0];
+            C = v2[1];
+            th = (S - target) / C;
+            for (int round = 0; round <= L; ++round) {
+                double S2 = 0.0, C2 = 0.0;
+                for (int e = tid; e < L; e += BS) {
+                    double v = double(Real(srcz[base + e] + a.r[base + e]));
+                    if (v > th) {
+                        S2 += v;
+                        C2 += 1.0;
+                    }
+                }
+                double w2[2] = {S2, C2};
+                block_sums<2>(w2, red);
+                if (w2[1] >= C || w2[1] == 0.0) break;
+                C = w2[1];
+                th = (w2[0] - target) / w2[1];
+            }
+        }
+    }
+    const Real thr = Real(th);
+    for (int e = tid; e < L; e += BS) {
+        int g = base + e;
+        Real v1 = srcz[g];
+        Real u = v1 + a.r[g];
+        Real d = u - thr;
+        Real z2 = target > 0.0 ? (d > Real(0) ? d : Real(0)) : Real(0);
+        Real anchor = st.reflect ? a.z0[g] : v1;
+        a.zc[g] = st.lwn * (Real(2) * z2 - v1) + st.rwn * anchor;
+        if (st.store) {
+            a.z[g] = v1;
+            a.sz[g] = z2;
+        }
+    }
+    __syncthreads();
+}
+
+// --- check sums ------------------------------------------------------------
+
+enum PriaSum {
+    kPPNorm = 0,   // sum (x - sx)^2
+    kPDNorm = 1,   // sum (y - sy)^2 + (z - sz)^2
+    kPCross = 2,   // sum (x - sx)((y - sy) + (z - sz))
+    kPPDrift = 3,  // sum (x - x0)^2
+    kPDDrift = 4,  // sum (y - y0)^2 + (z - z0)^2
+    kPUpper = 5,   // sum_i K_i max_row bx + K0 max_slice bx + <h, bx>
+    kPLower = 6,   // sum over columns of min (h + by + bz)
+    kPriaSums = 7
+};
+
+// Slice part of the check (everything but the column minima), by one
+// workgroup; returns the six sums in acc[0..5] of thread 0 (all threads hold
+// the totals after block_sums).  Bounds at (bx): upper bound terms.
+template <class Real>
+__device__ void pria_slice_sums(const PriaArrays<Real>& a, int s, const Real* bx, double* red,
+                                double (&acc)[kPriaSums]) {
+    const int T = a.T, N = a.N;
+    const int BS = blockDim.x, tid = threadIdx.x;
+    const int lane = tid & (kWave - 1), wave = tid / kWave, waves = BS / kWave;
+    const int len = T - s, base = pria_off(N, T, s), L = N * len;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    double smax = -INFINITY;
+    for (int e = tid; e < L; e += BS) {
+        int g = base + e;
+        double dx = double(a.x[g]) - double(a.sx[g]);
+        double dy = double(a.y[g]) - double(a.sy[g]);
+        double dz = double(a.z[g]) - double(a.sz[g]);
+        v[0] += dx * dx;
+        v[1] += dy * dy + dz * dz;
+        v[2] += dx * dy + dx * dz;
+        double rx = double(a.x[g]) - double(a.x0[g]);
+        double ry = double(a.y[g]) - double(a.y0[g]);
+        double rz = double(a.z[g]) - double(a.z0[g]);
+        v[3] += rx * rx;
+        v[4] += ry * ry + rz * rz;
+        double b = bx[g];
+        v[5] += double(a.h[g]) * b;
+        smax = b > smax ? b : smax;
+    }
+    // row maxima: one wave per row, K_i max_t bx
+    double rows = 0.0;
+    for (int i = wave; i < N; i += waves) {
+        double m = -INFINITY;
+        for (int e = lane; e < len; e += kWave) {
+            double b = bx[base + i * len + e];
+            m = b > m ? b : m;
+        }
+        m = wave_max(m);
+        if (lane == 0) rows += double(a.k[i]) * m;
+    }
+    v[5] += rows;
+    double gmax = block_max1(smax, red);
+    block_sums<6>(v, red);
+    for (int q = 0; q < 5; ++q) acc[q] += v[q];
+    acc[kPUpper] += v[5] + double(a.k0) * gmax;
+}
+
+// Column minima of (h + by + bz) over s <= t for the tile of item i starting
+// at t0 (TW columns); adds the tile's sum of minima to acc[kPLower].
+template <class Real, int TW>
+__device__ void pria_column_sums(const PriaArrays<Real>& a, int i, int t0, const Real* by,
+                                 const Real* bz, double* red, double (&acc)[kPriaSums]) {
+    const int T = a.T, N = a.N;
+    const int BS = blockDim.x, tid = threadIdx.x;
+    const int G = BS / TW;
+    const int c = tid % TW, row = tid / TW, t = t0 + c;
+    const int tcount = min(TW, T - t0);
+    double m = INFINITY;
+    if (c < tcount) {
+        for (int s = row; s <= t; s += G) {
+            int g = pria_off(N, T, s) + i * (T - s) + (t - s);
+            double v = double(a.h[g]) + double(by[g]) + double(bz[g]);
+            m = v < m ? v : m;
+        }
+    }
+    // min over the G rows of each column: LDS reduce by column
+    __shared__ double colmin[1024];
+    colmin[tid] = m;
+    __syncthreads();
+    double total = 0.0;
+    if (tid < TW && tid < tcount) {
+        double mm = colmin[tid];
+        for (int q = 1; q < G; ++q) {
+            double o = colmin[q * TW + tid];
+            mm = o < mm ? o : mm;
+        }
+        total = mm;
+    }
+    __syncthreads();
+    total = block_sum1(total, red);
+    acc[kPLower] += total;
+}
+
+__host__ __device__ inline CheckSums pria_check_sums(const double* s, double scale) {
+    CheckSums c;
+    c.pnorm = s[kPPNorm];
+    c.dnorm = s[kPDNorm];
+    c.cross = s[kPCross];
+    c.pdrift = s[kPPDrift];
+    c.ddrift = s[kPDDrift];
+    c.upper = s[kPUpper] * scale;
+    c.lower = s[kPLower] * scale;
+    return c;
+}
+
+}  // namespace reform
