@@ -1,0 +1,190 @@
+"""Throughput of the batched PRAQ/PRIA solver on one MI355X.
+
+Workload: the batched mixture -- 1024 instances, 70% PRAQ (Erdos-Renyi, n
+log-uniform in [1e3, 1e5], mean degree U[4, 32], integer weights) and 30%
+PRIA (N in {10..100}, T in {50..300}), instance k drawn with seed k -- solved
+to 1e-6 relative duality gap with FP32 iterates and FP64 reductions.  A step
+is one solve of the whole batch from inputs resident in HBM; `value` is
+solves/s over the timed steps.  Beside it: the plain one-at-a-time GPU solver
+on the same resident instances (solves/s), the dominant launch's HBM roofline
+(algorithmic bytes: PRAQ 52 m + 32 n and PRIA 60 ND per iteration, FP32, over
+the iterations each instance ran, divided by the launch time measured with
+HIP events on its stream), and the oracle (FP64 C restatement, OpenMP) timed
+on a bounded sample of the same mixture on the host's cores.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--count 1024]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_batch(members, tolerance: float, use_float32: bool, max_iter: int):
+    from reformulation_amd import praq, pria
+    from reformulation_amd.batch import Batch
+    b = Batch(use_float32=use_float32)
+    index = []
+    for kind, inst in members:
+        if kind == "praq":
+            params = praq.Parameters.with_auto_step(max_iter, tolerance)
+            index.append(("praq", b.add_praq(inst.problem(), params), inst))
+        else:
+            params = pria.Parameters.with_auto_step(max_iter, tolerance)
+            index.append(("pria", b.add_pria(inst.problem(), params), inst))
+    return b, index
+
+
+def algorithmic_bytes(index, infos) -> tuple[float, float]:
+    """(PRAQ, PRIA) algorithmic HBM bytes of one batched solve, FP32."""
+    praq_b = pria_b = 0.0
+    for (kind, q, inst), info in zip(index, infos):
+        if kind == "praq":
+            praq_b += (52.0 * inst.edges + 32.0 * inst.n) * info.iterations
+        else:
+            t = inst.periods + (inst.periods & 1)
+            nd = inst.items * t * (t + 1) // 2
+            pria_b += 60.0 * nd * info.iterations
+    return praq_b, pria_b
+
+
+def cpu_baseline(members, tolerance: float, budget_s: float):
+    """The oracle (FP64, all host cores) on the first instances of the mixture
+    until `budget_s` of work."""
+    import oracle
+    cores = os.cpu_count() or 1
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    solved = 0
+    t0 = time.perf_counter()
+    for kind, inst in members:
+        if kind == "praq":
+            oracle.praq_solve(inst.n, inst.diag, inst.rows, inst.cols, inst.vals,
+                              oracle.praq_params(tolerance=tolerance, max_iter=200_000))
+        else:
+            oracle.pria_solve(inst.order_costs, inst.joint_order_cost, inst.holding_costs,
+                              oracle.pria_params(tolerance=tolerance, max_iter=200_000))
+        solved += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    elapsed = time.perf_counter() - t0
+    return dict(value=solved / elapsed, unit="solves/s", cores=int(os.environ["OMP_NUM_THREADS"]),
+                kind="port",
+                sample=f"first {solved} instances of the mixture, FP64 oracle, {elapsed:.1f} s")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--count", type=int, default=1024)
+    ap.add_argument("--tolerance", type=float, default=1e-6)
+    ap.add_argument("--fp64", action="store_true", help="FP64 iterates")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps(dict(error="this build measures one GPU; --gpus must be 1 "
+                                    "(instances are independent: N GPUs would each run "
+                                    "their own batch)")))
+        return 2
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    from reformulation_amd import workloads
+
+    t0 = time.perf_counter()
+    members = workloads.mix(args.count)
+    log(f"generated {len(members)} instances in {time.perf_counter() - t0:.1f} s")
+    use_f32 = not args.fp64
+    batch, index = build_batch(members, args.tolerance, use_f32, 200_000)
+    t0 = time.perf_counter()
+    batch.upload()
+    log(f"uploaded in {time.perf_counter() - t0:.1f} s, {batch.device_bytes / 1e9:.2f} GB on device")
+
+    for w in range(args.warmup):
+        dt = batch.solve("batched")
+        log(f"warmup {w}: {dt:.3f} s")
+    torch.cuda.synchronize()
+    times, kms = [], []
+    for s in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch.solve("batched")
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        kms.append(batch.kernel_ms())
+        log(f"step {s}: {times[-1]:.3f} s (praq launch {kms[-1][0]:.1f} ms, pria launch {kms[-1][1]:.1f} ms)")
+    total = sum(times)
+    value = args.count * args.steps / total
+
+    infos = []
+    for kind, q, inst in index:
+        infos.append(batch.praq_result(q, False)[2] if kind == "praq" else batch.pria_result(q, False)[3])
+    status = [i.status for i in infos]
+    gaps = np.array([i.relative_gap for i in infos])
+    iters = np.array([i.iterations for i in infos])
+    log(f"converged {status.count(0)}/{len(status)}, max gap {gaps.max():.2e}, "
+        f"iterations median {int(np.median(iters))} max {int(iters.max())}")
+
+    praq_b, pria_b = algorithmic_bytes(index, infos)
+    praq_ms = float(np.mean([k[0] for k in kms]))
+    pria_ms = float(np.mean([k[1] for k in kms]))
+    if pria_ms >= praq_ms:
+        dom, dom_bytes, dom_ms = "pria_wg_solve", pria_b, pria_ms
+    else:
+        dom, dom_bytes, dom_ms = "praq_wg_solve", praq_b, praq_ms
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+
+    plain = None
+    if not args.no_plain:
+        dt = batch.solve("sequential")
+        plain = args.count / dt
+        log(f"plain one-at-a-time: {dt:.2f} s -> {plain:.1f} solves/s")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(members, args.tolerance, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+
+    ms = 1000.0 * total / args.steps
+    line = dict(
+        metric="solves/sec to 1e-6 rel KKT (served vs plain, 1 GPU); p50/p99 solve latency",
+        value=value, unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=ms, higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32" if use_f32 else "f64", data="synthetic",
+        config=dict(workload=f"batched mixture of {args.count} instances (70% PRAQ ER n in "
+                             "[1e3,1e5] deg U[4,32]; 30% PRIA N in [10,100], T in [50,300]) "
+                             f"to {args.tolerance:g} relative gap, FP32 iterates + FP64 "
+                             "reductions, one launch per family",
+                    count=args.count, tolerance=args.tolerance, parallelism="none"),
+        plain_solves_per_s=plain, speedup_vs_plain=(value / plain) if plain else None,
+        latency_ms=dict(p50=ms, p99=ms, note="batched: every instance of a step completes "
+                                             "with its launch"),
+        converged=status.count(0), max_gap=float(gaps.max()),
+        roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=PEAK_HBM_GBS,
+                      unit="GB/s", frac=achieved / PEAK_HBM_GBS, traffic=None),
+        cpu_baseline=cpu,
+    )
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

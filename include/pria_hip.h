@@ -111,6 +111,14 @@ int pria_solve_ex(int64_t n, int64_t t, const double* order_costs, double joint_
                   const double* warm_y, const double* warm_z, double* x, double* y, double* z,
                   pria_info* info);
 
+/* Test hook: one PDHG step x2 = P_X(x1 - tau (h + y1 + z1)), r = sigma
+ * (2 x2 - x1), y2 = P_Y(y1 + r), z2 = P_Z(z1 + r) on an even-T instance whose
+ * h, K, K0 are already scaled; dense [n][t][t] in and out.  tier 1: one
+ * workgroup; 2: the device-wide kernels. */
+int pria_debug_step(int64_t n, int64_t t, const double* k, double k0, const double* h,
+                    const double* x1, const double* y1, const double* z1, double omega,
+                    double eta, int use_float32, int tier, double* x2, double* y2, double* z2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,79 @@
+/* Many PRAQ and PRIA instances of uneven size on one MI355X.
+ *
+ * No counterpart in the reference beyond pria_solve_batch_* (same-shape PRIA
+ * only, a thread fan-out of solo solves: jrp/pria-cuda/include/pria/pria.h:85-99,
+ * jrp/pria-cuda/src/pria.cu:15-44).  Here a batch holds instances of any size
+ * of both families, packed ragged in one device allocation, and solves them
+ * either
+ *   REFORM_BATCHED     all together: one persistent launch per family in
+ *                      which each instance is one workgroup running its
+ *                      whole solve (steps, checks, restarts, termination),
+ *                      instances ordered largest first; or
+ *   REFORM_SEQUENTIAL  one at a time with the plain solver's tier choice
+ *                      (praq_solve_ex / pria_solve_ex on resident inputs):
+ *                      the baseline the batched mode is measured against.
+ * Each instance's result is the same in either mode when the plain tier of
+ * that instance is the workgroup tier, bitwise, and it is the same alone and
+ * in any mix, since every workgroup runs one instance with the same code.
+ *
+ * Usage: create, add_* (host), upload (host build + one H2D copy), then
+ * solve any number of times from the resident inputs, then read results.
+ * Return codes: >= 0 success (add_* return the instance's index within its
+ * family), -1 argument, -2 instance, -3 device.
+ */
+#ifndef REFORMULATION_REFORM_BATCH_H
+#define REFORMULATION_REFORM_BATCH_H
+
+#include <stdint.h>
+
+#include "praq_hip.h"
+#include "pria_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct reform_batch reform_batch;
+
+enum { REFORM_BATCHED = 0, REFORM_SEQUENTIAL = 1 };
+
+/* use_float32: iterate precision of every instance of the batch
+ * (reductions and scalar logic stay FP64). */
+reform_batch* reform_batch_create(int use_float32);
+void reform_batch_destroy(reform_batch* batch);
+
+int64_t reform_batch_add_praq(reform_batch* batch, const PraqInstance* instance,
+                              const PraqParameters* parameters, const PraqOptions* options);
+int64_t reform_batch_add_pria(reform_batch* batch, int64_t n, int64_t t, const double* order_costs,
+                              double joint_cost, const double* holding_costs,
+                              const pria_params* params, const pria_options* options);
+
+/* Builds every instance's device image on the host (threads) and copies the
+ * inputs to the GPU in one transfer. */
+int reform_batch_upload(reform_batch* batch);
+
+/* Solves every instance from its resident inputs; synchronous. */
+int reform_batch_solve(reform_batch* batch, int mode);
+
+/* Workgroup sizes of the batched launches (0 keeps the default); a tuner
+ * sets these. */
+int reform_batch_set_shape(reform_batch* batch, int praq_block, int pria_block);
+
+int64_t reform_batch_count(const reform_batch* batch, int family /* 0 PRAQ, 1 PRIA */);
+
+/* Durations of the last batched solve's two launches, measured with HIP
+ * events on the streams they ran on (0 for an empty family). */
+int reform_batch_kernel_ms(const reform_batch* batch, double* praq_ms, double* pria_ms);
+int64_t reform_batch_device_bytes(const reform_batch* batch);
+
+/* Results of the last solve.  Outputs may be NULL.  PRAQ y is [m][2] in
+ * instance units; PRIA x, y, z are dense [n][t][t], y and z normalised. */
+int reform_batch_praq_result(reform_batch* batch, int64_t index, double* x, double* y,
+                             PraqInfo* info);
+int reform_batch_pria_result(reform_batch* batch, int64_t index, double* x, double* y, double* z,
+                             pria_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

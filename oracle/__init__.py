@@ -163,6 +163,50 @@ def praq_bounds(n, diag, rows, cols, vals, x, y):
     return lo.value, up.value
 
 
+def praq_structure(n, diag, rows, cols, vals):
+    """(scaled weights w_e, covering flags, scale) of the sparse build."""
+    m = len(vals)
+    w = np.zeros(max(m, 1))
+    pos = np.zeros(max(m, 1), dtype=np.uint8)
+    sc = ctypes.c_double()
+    st = lib().oracle_praq_structure(_i64(n), _i64(m), _d(np.ascontiguousarray(diag, dtype=np.float64)),
+                                     _i(np.ascontiguousarray(rows, dtype=np.int64)),
+                                     _i(np.ascontiguousarray(cols, dtype=np.int64)),
+                                     _d(np.ascontiguousarray(vals, dtype=np.float64)), _d(w),
+                                     pos.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                     ctypes.byref(sc))
+    if st:
+        raise ValueError(st)
+    return w[:m], pos[:m].astype(bool), sc.value
+
+
+def praq_raw_step(n, diag, rows, cols, vals, x1, y1, omega=1.0, eta=0.999, dense_degree=False):
+    m = len(vals)
+    x2 = np.zeros(n)
+    y2 = np.zeros((max(m, 1), 2))
+    lib().oracle_praq_raw_step(_i64(n), _i64(m), _d(np.ascontiguousarray(diag, dtype=np.float64)),
+                               _i(np.ascontiguousarray(rows, dtype=np.int64)),
+                               _i(np.ascontiguousarray(cols, dtype=np.int64)),
+                               _d(np.ascontiguousarray(vals, dtype=np.float64)),
+                               ctypes.c_int(int(dense_degree)), ctypes.c_double(omega),
+                               ctypes.c_double(eta), _d(np.ascontiguousarray(x1, dtype=np.float64)),
+                               _d(np.ascontiguousarray(y1, dtype=np.float64).reshape(-1)),
+                               _d(x2), _d(y2))
+    return x2, y2[:m]
+
+
+def praq_canonical(n, diag, rows, cols, vals, dx, dy, omega=1.0, eta=0.999, dense_degree=False):
+    f = lib().oracle_praq_canonical
+    f.restype = ctypes.c_double
+    return f(_i64(n), _i64(len(vals)), _d(np.ascontiguousarray(diag, dtype=np.float64)),
+             _i(np.ascontiguousarray(rows, dtype=np.int64)),
+             _i(np.ascontiguousarray(cols, dtype=np.int64)),
+             _d(np.ascontiguousarray(vals, dtype=np.float64)), ctypes.c_int(int(dense_degree)),
+             _d(np.ascontiguousarray(dx, dtype=np.float64)),
+             _d(np.ascontiguousarray(dy, dtype=np.float64).reshape(-1)), ctypes.c_double(eta),
+             ctypes.c_double(omega))
+
+
 def simplex_project(v, target) -> np.ndarray:
     v = np.array(v, dtype=np.float64)
     lib().oracle_simplex_project(_i64(v.size), _d(v), ctypes.c_double(target))

@@ -80,6 +80,18 @@ void oracle_praq_bounds(int64_t n, int64_t m, const double* diag, const int64_t*
                         const int64_t* cols, const double* vals, const double* x,
                         const double* y, double* lower, double* upper);
 
+/* One raw step from (x1, y1) (scaled units, y [m][2]) and the canonical norm
+ * of (dx, dy); the scaled edge weights / covering flags / scale. */
+int oracle_praq_raw_step(int64_t n, int64_t m, const double* diag, const int64_t* rows,
+                         const int64_t* cols, const double* vals, int dense_degree, double omega,
+                         double eta, const double* x1, const double* y1, double* x2, double* y2);
+double oracle_praq_canonical(int64_t n, int64_t m, const double* diag, const int64_t* rows,
+                             const int64_t* cols, const double* vals, int dense_degree,
+                             const double* dx, const double* dy, double eta, double omega);
+int oracle_praq_structure(int64_t n, int64_t m, const double* diag, const int64_t* rows,
+                          const int64_t* cols, const double* vals, double* w_out,
+                          uint8_t* pos_out, double* scale_out);
+
 /* ---- PRIA --------------------------------------------------------------- */
 
 /* Euclidean projection of v[len] onto {u >= 0, sum u = target}; target <= 0
@@ -93,6 +105,12 @@ int oracle_pria_solve(int64_t n, int64_t t, const double* k, double k0, const do
                       const OracleParams* p, const double* warm_x, const double* warm_y,
                       const double* warm_z, double* x_out, double* y_out, double* z_out,
                       OracleInfo* info);
+
+/* One raw step on an even-T instance whose h, k, k0 are already scaled;
+ * dense [n][t][t] in and out. */
+int oracle_pria_raw_step(int64_t n, int64_t t, const double* k, double k0, const double* h,
+                         const double* x1, const double* y1, const double* z1, double omega,
+                         double eta, double* x2, double* y2, double* z2);
 
 /* primal_objective (jrp_problem.py:143-161) at x [n][t][t]. */
 double oracle_pria_primal_objective(int64_t n, int64_t t, const double* k, double k0,

@@ -429,3 +429,47 @@ void oracle_praq_bounds(int64_t n, int64_t m, const double* diag, const int64_t*
     free(ys);
     praq_free(&s);
 }
+
+/* One raw step and the canonical norm, exposed for the metric tests
+ * (nonexpansiveness fixes the sign of the cross term). */
+int oracle_praq_raw_step(int64_t n, int64_t m, const double* diag, const int64_t* rows,
+                         const int64_t* cols, const double* vals, int dense_degree, double omega,
+                         double eta, const double* x1, const double* y1, double* x2, double* y2) {
+    Praq s;
+    int st = praq_build(&s, n, m, diag, rows, cols, vals, dense_degree);
+    if (st) { praq_free(&s); return st; }
+    double* refl = malloc(sizeof(double) * n);
+    raw_step(&s, x1, y1, omega, eta, x2, y2, refl);
+    free(refl);
+    praq_free(&s);
+    return 0;
+}
+
+double oracle_praq_canonical(int64_t n, int64_t m, const double* diag, const int64_t* rows,
+                             const int64_t* cols, const double* vals, int dense_degree,
+                             const double* dx, const double* dy, double eta, double omega) {
+    Praq s;
+    if (praq_build(&s, n, m, diag, rows, cols, vals, dense_degree)) { praq_free(&s); return NAN; }
+    double* zx = calloc(n, sizeof(double));
+    double* zy = calloc(2 * m + 1, sizeof(double));
+    double v = canonical(&s, dx, zx, dy, zy, eta, omega);
+    free(zx); free(zy);
+    praq_free(&s);
+    return v;
+}
+
+/* The scaled structure's weights and covering flags, for tests that build
+ * points of the dual set. */
+int oracle_praq_structure(int64_t n, int64_t m, const double* diag, const int64_t* rows,
+                          const int64_t* cols, const double* vals, double* w_out,
+                          uint8_t* pos_out, double* scale_out) {
+    Praq s;
+    int st = praq_build(&s, n, m, diag, rows, cols, vals, 0);
+    if (!st) {
+        memcpy(w_out, s.w, sizeof(double) * m);
+        memcpy(pos_out, s.pos, m);
+        *scale_out = s.scale;
+    }
+    praq_free(&s);
+    return st;
+}

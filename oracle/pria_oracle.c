@@ -25,6 +25,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -51,22 +52,28 @@ void oracle_simplex_project(int64_t len, double* v, double target) {
     double* u = malloc(sizeof(double) * len);
     memcpy(u, v, sizeof(double) * len);
     qsort(u, (size_t)len, sizeof(double), cmp_desc);
-    double running = 0.0, at_rho = 0.0;
+    /* rho is the COUNT of k with u_k k > cumsum_k - target, and theta uses
+     * the prefix sum at rho - 1 (projection.py:113-128, count-based): with
+     * entries tied at the threshold the test can hold off the prefix, and the
+     * count still lands theta on the tie. */
+    double* cum = malloc(sizeof(double) * len);
+    double running = 0.0;
     int64_t rho = 0;
     for (int64_t k = 0; k < len; ++k) {
         running += u[k];
-        /* u_k * k > cumsum_k - target; prefix-true, so rho is the count */
-        if (u[k] * (double)(k + 1) > running - target) { ++rho; at_rho = running; }
+        cum[k] = running;
+        if (u[k] * (double)(k + 1) > running - target) ++rho;
     }
     if (rho == 0) {
         for (int64_t k = 0; k < len; ++k) v[k] = 0.0;
     } else {
-        double theta = (at_rho - target) / (double)rho;
+        double theta = (cum[rho - 1] - target) / (double)rho;
         for (int64_t k = 0; k < len; ++k) {
             double d = v[k] - theta;
             v[k] = d > 0.0 ? d : 0.0;
         }
     }
+    free(cum);
     free(u);
 }
 
@@ -128,11 +135,6 @@ static void raw_step(const Pria* s, const double* x1, const double* y1, const do
     proj_z(s, z2);
 }
 
-/* upper-triangle mask: entries below the diagonal are not variables */
-static inline int upper(const Pria* s, int64_t g) {
-    int64_t b = g % s->t, a = (g / s->t) % s->t;
-    return a <= b;
-}
 
 /* canonical_norm, negative cross term */
 static double canonical(const Pria* s, const double* x, const double* sx, const double* y,
@@ -407,6 +409,10 @@ int oracle_pria_solve(int64_t n, int64_t torig, const double* kin, double k0in,
             gap = rel_gap(upper, lower);
             terminated = !(within_max_iter(t.iteration, p.max_iter) && !(gap <= p.tolerance));
 
+            if (getenv("ORACLE_TRACE"))
+                fprintf(stderr, "[check] it=%lld r=%lld gap=%.6e up=%.12e lo=%.12e new_err=%.12e fp=%.12e check=%.12e omega=%.12e\n",
+                        (long long)t.iteration, (long long)t.restart_iter, gap, upper, lower, new_err,
+                        fp_prev, check_prev, t.omega);
             int sufficient = new_err <= p.beta_sufficient * fp_prev;
             int necessary = (new_err <= p.beta_necessary * fp_prev) && (new_err > check_prev);
             int artificial = (double)t.restart_iter >= p.beta_artificial * (double)t.iteration;
@@ -481,4 +487,22 @@ int oracle_pria_solve(int64_t n, int64_t torig, const double* kin, double k0in,
     free(px); free(py); free(pz); free(s.h); free(s.k);
     (void)upper;
     return status;
+}
+
+/* One raw step on an even-T instance whose h, k, k0 are already scaled;
+ * dense [n][t][t] in and out. */
+int oracle_pria_raw_step(int64_t n, int64_t t, const double* k, double k0, const double* h,
+                         const double* x1, const double* y1, const double* z1, double omega,
+                         double eta, double* x2, double* y2, double* z2) {
+    if (n <= 0 || t <= 0 || (t & 1)) return -1;
+    Pria s;
+    s.n = n;
+    s.t = t;
+    s.sz = n * t * t;
+    s.scale = 1.0;
+    s.h = (double*)h;
+    s.k = (double*)k;
+    s.k0 = k0;
+    raw_step(&s, x1, y1, z1, omega, eta, x2, y2, z2);
+    return 0;
 }

@@ -78,6 +78,10 @@ EXPORTS = (
     "praq_solve_ex",
     "pria_params_auto", "pria_solve_f64", "pria_solve_f32", "pria_solve_batch_f64",
     "pria_solve_batch_f32", "pria_objective_f64", "pria_default_options", "pria_solve_ex",
+    "reform_batch_create", "reform_batch_destroy", "reform_batch_add_praq",
+    "reform_batch_add_pria", "reform_batch_upload", "reform_batch_solve",
+    "reform_batch_set_shape", "reform_batch_count", "reform_batch_kernel_ms",
+    "reform_batch_device_bytes", "reform_batch_praq_result", "reform_batch_pria_result",
 )
 
 _lib = None

@@ -21,6 +21,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 
 namespace reform {
 
@@ -52,6 +53,8 @@ struct LogicParams {
     int64_t warm_max;            // op_steps * warm_steps (Halpern) or max_iter
     int32_t use_halpern;
     int32_t fallback_nonpositive;  // PRAQ: omega fallback also on <= 0
+    int32_t verbose;               // device tier: one stderr line per check
+    int32_t pad;
 };
 
 struct LogicState {
@@ -63,6 +66,15 @@ struct LogicState {
     int32_t status;              // 0 converged, 1 limit, 2 time limit
     int32_t pad;
 };
+
+// One line per check of the device tier's host loop, when verbose.
+inline void logic_trace(const LogicState& st, const char* what) {
+    fprintf(stderr,
+            "[%s] it=%lld r=%lld gap=%.6e up=%.12e lo=%.12e new_err=%.12e fp=%.12e check=%.12e "
+            "omega=%.12e action=%d\n",
+            what, (long long)st.iteration, (long long)st.restart_iter, st.gap, st.upper, st.lower,
+            st.new_err, st.fp_error, st.check_error, st.omega, st.action);
+}
 
 __host__ __device__ inline double canonical_norm(const CheckSums& s, double eta, double omega,
                                                  double cross_sign) {

@@ -73,9 +73,10 @@ __global__ void __launch_bounds__(1024) praq_wg_solve(const PraqJob<Real>* jobs)
     const int lanes = job.lanes;
     const long long t_start = wall_clock64();
 
-    // state from the projected start held in the anchors
-    praq_wg_copy(a.x, a.x0, a.n, a.y, a.y0, 2 * a.m);
-    praq_wg_copy(a.sx, a.x0, a.n, a.sy, a.y0, 2 * a.m);
+    // state from the projected start
+    praq_wg_copy(a.x0, a.xs, a.n, a.y0, a.ys, 2 * a.m);
+    praq_wg_copy(a.x, a.xs, a.n, a.y, a.ys, 2 * a.m);
+    praq_wg_copy(a.sx, a.xs, a.n, a.sy, a.ys, 2 * a.m);
     if (threadIdx.x == 0) logic_init(st, p);
     __syncthreads();
 
@@ -249,8 +250,9 @@ int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto t0 = now();
 
-    if (device_copy(a.x, a.x0, a.n, a.y, a.y0, 2 * a.m, stream)) return kErrorDevice;
-    if (device_copy(a.sx, a.x0, a.n, a.sy, a.y0, 2 * a.m, stream)) return kErrorDevice;
+    if (device_copy(a.x0, a.xs, a.n, a.y0, a.ys, 2 * a.m, stream)) return kErrorDevice;
+    if (device_copy(a.x, a.xs, a.n, a.y, a.ys, 2 * a.m, stream)) return kErrorDevice;
+    if (device_copy(a.sx, a.xs, a.n, a.sy, a.ys, 2 * a.m, stream)) return kErrorDevice;
     logic_init(st, p);
     if (device_step(a, praq_step_scalars<Real>(0, p.omega, p.eta, false, true), lanes, stream))
         return kErrorDevice;

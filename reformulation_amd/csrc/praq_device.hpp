@@ -5,10 +5,10 @@
 // Data layout in HBM (all arrays of one instance contiguous, 4-byte indices):
 //   edges   e0[m], e1[m] int32 ends; w[m] signed scaled weight, 2 W_ij / s
 //           (> 0 covering, < 0 packing, 0 inert)
-//   nodes   diag[n], deg[n], x[n], sx[n], x0[n], refl[n]
+//   nodes   diag[n], deg[n], xs[n] (projected start), x[n], sx[n], x0[n], refl[n]
 //   CSR     ptr[n+1], half[2m]: node i's incident half-edges, each encoded as
 //           slot * m + e, i.e. an index into a slot-major dual array
-//   duals   yc[2m], y[2m], sy[2m], y0[2m], slot-major (slot 0 of every edge,
+//   duals   ys[2m] (projected start), yc[2m], y[2m], sy[2m], y0[2m], slot-major (slot 0 of every edge,
 //           then slot 1), so the edge pass reads both slots coalesced
 //
 // yc holds the Halpern-combined dual the NEXT step starts from, written by the
@@ -36,6 +36,8 @@ struct PraqArrays {
     const int* half;
     const Real* diag;
     const Real* deg;
+    const Real* xs;   // projected start [n]
+    const Real* ys;   // projected start [2m]
     Real* x;
     Real* sx;
     Real* x0;

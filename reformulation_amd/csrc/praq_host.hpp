@@ -133,7 +133,8 @@ inline int praq_build_host(const PraqInstance& in, int dense_degree, const doubl
 // Byte offsets of one instance's arrays in its device image.  The input part
 // (uploaded) comes first, the iteration state after it.
 struct PraqLayout {
-    enum { kE0, kE1, kPtr, kHalf, kW, kDiag, kDeg, kX0, kY0, kX, kSX, kRefl, kYC, kY, kSY, kCount };
+    enum { kE0, kE1, kPtr, kHalf, kW, kDiag, kDeg, kXS, kYS, kX0, kY0, kX, kSX, kRefl, kYC, kY, kSY,
+           kCount };
     size_t off[kCount];
     size_t input_bytes = 0, total_bytes = 0;
 };
@@ -142,17 +143,30 @@ inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 inline PraqLayout praq_layout(int n, int m, size_t real_size) {
     PraqLayout L;
-    size_t sizes[PraqLayout::kCount] = {
-        sizeof(int) * size_t(m),      sizeof(int) * size_t(m),     sizeof(int) * (size_t(n) + 1),
-        sizeof(int) * 2 * size_t(m),  real_size * size_t(m),       real_size * size_t(n),
-        real_size * size_t(n),        real_size * size_t(n),       real_size * 2 * size_t(m),
-        real_size * size_t(n),        real_size * size_t(n),       real_size * size_t(n),
-        real_size * 2 * size_t(m),    real_size * 2 * size_t(m),   real_size * 2 * size_t(m)};
+    const size_t ni = sizeof(int), nn = size_t(n), mm = size_t(m);
+    size_t sizes[PraqLayout::kCount];
+    sizes[PraqLayout::kE0] = ni * mm;
+    sizes[PraqLayout::kE1] = ni * mm;
+    sizes[PraqLayout::kPtr] = ni * (nn + 1);
+    sizes[PraqLayout::kHalf] = ni * 2 * mm;
+    sizes[PraqLayout::kW] = real_size * mm;
+    sizes[PraqLayout::kDiag] = real_size * nn;
+    sizes[PraqLayout::kDeg] = real_size * nn;
+    sizes[PraqLayout::kXS] = real_size * nn;
+    sizes[PraqLayout::kYS] = real_size * 2 * mm;
+    sizes[PraqLayout::kX0] = real_size * nn;
+    sizes[PraqLayout::kY0] = real_size * 2 * mm;
+    sizes[PraqLayout::kX] = real_size * nn;
+    sizes[PraqLayout::kSX] = real_size * nn;
+    sizes[PraqLayout::kRefl] = real_size * nn;
+    sizes[PraqLayout::kYC] = real_size * 2 * mm;
+    sizes[PraqLayout::kY] = real_size * 2 * mm;
+    sizes[PraqLayout::kSY] = real_size * 2 * mm;
     size_t at = 0;
     for (int k = 0; k < PraqLayout::kCount; ++k) {
         L.off[k] = at;
         at += align256(sizes[k] ? sizes[k] : 1);
-        if (k == PraqLayout::kY0) L.input_bytes = at;
+        if (k == PraqLayout::kYS) L.input_bytes = at;
     }
     L.total_bytes = at;
     return L;
@@ -174,8 +188,8 @@ inline void praq_fill_image(const PraqHostInstance& s, const PraqLayout& L, char
     put_real(PraqLayout::kW, s.w);
     put_real(PraqLayout::kDiag, s.diag);
     put_real(PraqLayout::kDeg, s.deg);
-    put_real(PraqLayout::kX0, s.x_start);
-    put_real(PraqLayout::kY0, s.y_start);
+    put_real(PraqLayout::kXS, s.x_start);
+    put_real(PraqLayout::kYS, s.y_start);
 }
 
 template <class Real>
@@ -190,6 +204,8 @@ inline PraqArrays<Real> praq_bind(const PraqLayout& L, char* base, int n, int m)
     a.w = reinterpret_cast<const Real*>(base + L.off[PraqLayout::kW]);
     a.diag = reinterpret_cast<const Real*>(base + L.off[PraqLayout::kDiag]);
     a.deg = reinterpret_cast<const Real*>(base + L.off[PraqLayout::kDeg]);
+    a.xs = reinterpret_cast<const Real*>(base + L.off[PraqLayout::kXS]);
+    a.ys = reinterpret_cast<const Real*>(base + L.off[PraqLayout::kYS]);
     a.x0 = reinterpret_cast<Real*>(base + L.off[PraqLayout::kX0]);
     a.y0 = reinterpret_cast<Real*>(base + L.off[PraqLayout::kY0]);
     a.x = reinterpret_cast<Real*>(base + L.off[PraqLayout::kX]);
@@ -218,6 +234,8 @@ inline LogicParams praq_logic_params(const PraqParameters& p, const PraqOptions&
     q.use_halpern = o.use_halpern ? 1 : 0;
     q.warm_max = q.use_halpern ? q.op_steps * (o.warm_steps > 0 ? o.warm_steps : 0) : q.max_iter;
     q.fallback_nonpositive = 1;
+    q.verbose = 0;
+    q.pad = 0;
     return q;
 }
 

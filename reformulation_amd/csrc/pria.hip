@@ -82,8 +82,9 @@ __global__ void __launch_bounds__(1024) pria_wg_solve(const PriaJob<Real>* jobs)
     const LogicParams p = job.p;
     const long long t_start = wall_clock64();
 
-    pria_wg_copy3(a.x, a.x0, a.y, a.y0, a.z, a.z0, a.ND);
-    pria_wg_copy3(a.sx, a.x0, a.sy, a.y0, a.sz, a.z0, a.ND);
+    pria_wg_copy3(a.x0, a.xs, a.y0, a.ys, a.z0, a.zs, a.ND);
+    pria_wg_copy3(a.x, a.xs, a.y, a.ys, a.z, a.zs, a.ND);
+    pria_wg_copy3(a.sx, a.xs, a.sy, a.ys, a.sz, a.zs, a.ND);
     if (threadIdx.x == 0) logic_init(st, p);
     __syncthreads();
 
@@ -220,6 +221,32 @@ __global__ void copy3_kernel(Real* d1, const Real* s1, Real* d2, const Real* s2,
     }
 }
 
+// The cold start, on the device: the projection of zero onto each simplex is
+// its centre, x = 1 / (t + 1) per column, y = K_i / (T' - s) per row,
+// z = K0 / (N (T' - s)) per slice (0 where the budget is 0).
+template <class Real>
+__global__ void pria_cold_start_kernel(PriaArrays<Real> a, Real* xs, Real* ys, Real* zs) {
+    const int s = blockIdx.x, len = a.T - s, L = a.N * len, base = pria_off(a.N, a.T, s);
+    const double k0 = double(a.k0);
+    for (int e = threadIdx.x; e < L; e += blockDim.x) {
+        int i = e / len, t = s + e % len;
+        double ki = double(a.k[i]);
+        xs[base + e] = Real(1.0 / double(t + 1));
+        ys[base + e] = Real(ki > 0.0 ? ki / double(len) : 0.0);
+        zs[base + e] = Real(k0 > 0.0 ? k0 / double(L) : 0.0);
+    }
+}
+
+template <class Real>
+int pria_cold_start(const PriaArrays<Real>& a, hipStream_t stream) {
+    hipLaunchKernelGGL(pria_cold_start_kernel<Real>, dim3(a.T), dim3(256), 0, stream, a,
+                       const_cast<Real*>(a.xs), const_cast<Real*>(a.ys), const_cast<Real*>(a.zs));
+    return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+}
+
+template int pria_cold_start<float>(const PriaArrays<float>&, hipStream_t);
+template int pria_cold_start<double>(const PriaArrays<double>&, hipStream_t);
+
 size_t pria_device_work_bytes(int N, int T) {
     constexpr int TW = 32;  // the smaller tile width bounds the column rows
     size_t tiles = size_t(N) * ((T + TW - 1) / TW);
@@ -282,7 +309,8 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
     double sums[kPriaSums];
     auto t0 = std::chrono::steady_clock::now();
 
-    if (d.copy3(a.x, a.x0, a.y, a.y0, a.z, a.z0) || d.copy3(a.sx, a.x0, a.sy, a.y0, a.sz, a.z0))
+    if (d.copy3(a.x0, a.xs, a.y0, a.ys, a.z0, a.zs) || d.copy3(a.x, a.xs, a.y, a.ys, a.z, a.zs) ||
+        d.copy3(a.sx, a.xs, a.sy, a.ys, a.sz, a.zs))
         return kPriaErrorDevice;
     logic_init(st, p);
     if (d.step(pria_step_scalars<Real>(0, p.omega, p.eta, false, true))) return kPriaErrorDevice;
@@ -319,11 +347,13 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
             st.restart_iter += p.op_steps;
             st.iteration += p.op_steps;
             logic_after_halpern_check(st, p, pria_check_sums(sums, job.scale));
+            if (p.verbose) logic_trace(st, "check");
         } else {
             if (d.step(pria_step_scalars<Real>(0, omega, p.eta, false, true))) return kPriaErrorDevice;
             if (d.copy3(a.x0, a.x, a.y0, a.y, a.z0, a.z)) return kPriaErrorDevice;
             if (d.sums(a.sx, a.sy, a.sz, sums)) return kPriaErrorDevice;
             logic_after_restart(st, p, pria_check_sums(sums, job.scale));
+            if (p.verbose) logic_trace(st, "restart");
         }
         if (time_limit_seconds > 0.0 &&
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >
@@ -338,6 +368,39 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
 
 template int pria_device_solve<float>(const PriaJob<float>&, LogicState&, double*, double,
                                       hipStream_t);
+
+// One vanilla step from (sx, sy, sz), in place, on the device tier (tier 2)
+// or by one workgroup (tier 1); a test hook for the step kernels.
+template <class Real>
+__global__ void __launch_bounds__(1024) pria_wg_step_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
+    extern __shared__ char dyn[];
+    __shared__ double red[kPriaSums * 16];
+    pria_wg_step(a, st, reinterpret_cast<Real*>(dyn), red);
+}
+
+template <class Real>
+int pria_one_step(const PriaArrays<Real>& a, double omega, double eta, int tier, hipStream_t stream) {
+    PriaStep<Real> st = pria_step_scalars<Real>(0, omega, eta, false, true);
+    if (tier == 1) {
+        size_t lds = pria_x_lds_bytes<Real>(a.T);
+        if (lds > 65536 &&
+            hipFuncSetAttribute(reinterpret_cast<const void*>(pria_wg_step_kernel<Real>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+            return kPriaErrorDevice;
+        hipLaunchKernelGGL(pria_wg_step_kernel<Real>, dim3(1), dim3(1024), lds, stream, a, st);
+        return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+    }
+    PriaDevice<Real> d{a, stream, nullptr, nullptr, nullptr};
+    if (pria_x_lds_bytes<Real>(a.T) > 65536 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(pria_x_kernel<Real>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            int(pria_x_lds_bytes<Real>(a.T))) != hipSuccess)
+        return kPriaErrorDevice;
+    return d.step(st);
+}
+
+template int pria_one_step<float>(const PriaArrays<float>&, double, double, int, hipStream_t);
+template int pria_one_step<double>(const PriaArrays<double>&, double, double, int, hipStream_t);
 template int pria_device_solve<double>(const PriaJob<double>&, LogicState&, double*, double,
                                        hipStream_t);
 

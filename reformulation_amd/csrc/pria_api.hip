@@ -61,18 +61,19 @@ int solve_typed(const PriaHostInstance& s, const LogicParams& lp, const pria_opt
     Buffer buf;
     if (hipMalloc(&buf.ptr, bytes) != hipSuccess) return kPriaErrorDevice;
     char* base = static_cast<char*>(buf.ptr);
-    std::vector<char> image(L.input_bytes, 0);
+    const size_t up = pria_upload_bytes(s, L);
+    std::vector<char> image(up, 0);
     pria_fill_image<Real>(s, L, image.data());
     Stream stream;
     if (hipStreamCreateWithFlags(&stream.s, hipStreamNonBlocking) != hipSuccess)
         return kPriaErrorDevice;
-    if (hipMemcpyAsync(base, image.data(), L.input_bytes, hipMemcpyHostToDevice, stream.s) !=
-        hipSuccess)
+    if (hipMemcpyAsync(base, image.data(), up, hipMemcpyHostToDevice, stream.s) != hipSuccess)
         return kPriaErrorDevice;
 
     PriaJob<Real> job;
     std::memset(&job, 0, sizeof(job));
     job.a = pria_bind<Real>(L, base, s);
+    if (s.cold && pria_cold_start<Real>(job.a, stream.s)) return kPriaErrorDevice;
     job.p = lp;
     job.scale = s.scale;
     job.state = reinterpret_cast<LogicState*>(base + state_off);
@@ -242,11 +243,12 @@ int solve_batch(int64_t count, int64_t n, int64_t t, const Real* k, const Real* 
     for (int64_t b = 0; b < count; ++b) {
         std::fill(image.begin(), image.end(), 0);
         pria_fill_image<Real>(hosts[b], L, image.data());
-        if (hipMemcpy(base + per * b, image.data(), L.input_bytes, hipMemcpyHostToDevice) !=
-            hipSuccess)
+        if (hipMemcpy(base + per * b, image.data(), pria_upload_bytes(hosts[b], L),
+                      hipMemcpyHostToDevice) != hipSuccess)
             return kPriaErrorDevice;
         std::memset(&jobs[b], 0, sizeof(PriaJob<Real>));
         jobs[b].a = pria_bind<Real>(L, base + per * b, hosts[b]);
+        if (hosts[b].cold && pria_cold_start<Real>(jobs[b].a, stream.s)) return kPriaErrorDevice;
         jobs[b].p = lp;
         jobs[b].scale = hosts[b].scale;
         jobs[b].time_limit_ticks = (long long)(kDefaultWgLimitSeconds * wall_clock_rate_hz());
@@ -359,4 +361,63 @@ extern "C" double pria_objective_f64(int64_t n, int64_t t, const double* order_c
                                      double joint_cost, const double* holding_costs,
                                      const double* x) {
     return pria_objective_host(n, t, order_costs, joint_cost, holding_costs, x);
+}
+
+namespace {
+
+template <class Real>
+int debug_step(int64_t n, int64_t t, const double* k, double k0, const double* h,
+               const double* x1, const double* y1, const double* z1, double omega, double eta,
+               int tier, double* x2, double* y2, double* z2) {
+    if (n <= 0 || t <= 0 || (t & 1)) return kPriaErrorArgument;
+    PriaHostInstance s;
+    if (pria_build_host(n, t, k, k0, h, nullptr, nullptr, nullptr, s)) return kPriaErrorArgument;
+    // the inputs are already scaled: undo the build's scaling
+    for (auto& v : s.h) v *= s.scale;
+    for (auto& v : s.k) v *= s.scale;
+    s.k0 *= s.scale;
+    s.scale = 1.0;
+    PriaLayout L = pria_layout(s.N, s.ND, sizeof(Real));
+    Buffer buf;
+    if (hipMalloc(&buf.ptr, L.total_bytes) != hipSuccess) return kPriaErrorDevice;
+    char* base = static_cast<char*>(buf.ptr);
+    std::vector<char> image(L.input_bytes, 0);
+    pria_fill_image<Real>(s, L, image.data());
+    if (hipMemcpy(base, image.data(), L.off[PriaLayout::kXS], hipMemcpyHostToDevice) != hipSuccess)
+        return kPriaErrorDevice;
+    PriaArrays<Real> a = pria_bind<Real>(L, base, s);
+    std::vector<Real> v(s.ND);
+    const double* ins[3] = {x1, y1, z1};
+    Real* dst[3] = {a.sx, a.sy, a.sz};
+    for (int q = 0; q < 3; ++q) {
+        for (int i = 0; i < s.N; ++i)
+            for (int sa = 0; sa < s.T; ++sa)
+                for (int tb = sa; tb < s.T; ++tb)
+                    v[pria_idx(s.N, s.T, i, sa, tb)] = Real(ins[q][(int64_t(i) * t + sa) * t + tb]);
+        if (hipMemcpy(dst[q], v.data(), sizeof(Real) * s.ND, hipMemcpyHostToDevice) != hipSuccess)
+            return kPriaErrorDevice;
+    }
+    if (pria_one_step<Real>(a, omega, eta, tier, nullptr)) return kPriaErrorDevice;
+    if (hipDeviceSynchronize() != hipSuccess) return kPriaErrorDevice;
+    double* outs[3] = {x2, y2, z2};
+    for (int q = 0; q < 3; ++q) {
+        if (hipMemcpy(v.data(), dst[q], sizeof(Real) * s.ND, hipMemcpyDeviceToHost) != hipSuccess)
+            return kPriaErrorDevice;
+        std::memset(outs[q], 0, sizeof(double) * size_t(n) * t * t);
+        for (int i = 0; i < s.N; ++i)
+            for (int sa = 0; sa < s.T; ++sa)
+                for (int tb = sa; tb < s.T; ++tb)
+                    outs[q][(int64_t(i) * t + sa) * t + tb] = double(v[pria_idx(s.N, s.T, i, sa, tb)]);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int pria_debug_step(int64_t n, int64_t t, const double* k, double k0, const double* h,
+                               const double* x1, const double* y1, const double* z1, double omega,
+                               double eta, int use_float32, int tier, double* x2, double* y2,
+                               double* z2) {
+    return use_float32 ? debug_step<float>(n, t, k, k0, h, x1, y1, z1, omega, eta, tier, x2, y2, z2)
+                       : debug_step<double>(n, t, k, k0, h, x1, y1, z1, omega, eta, tier, x2, y2, z2);
 }

@@ -12,7 +12,8 @@
 // contiguous; the x segments (column (i, t) over s <= t, target 1) are read
 // in tiles of consecutive t, one coalesced row of the tile per s.
 //
-// Arrays: h (scaled holding cost), and for v in {x, y, z}: vc (the
+// Arrays: h (scaled holding cost), and for v in {x, y, z}: vs (the projected
+// start), vc (the
 // Halpern-combined point the next step starts from), v0 (anchor), v and sv
 // (the last point and its step, written only when a check or restart
 // follows), plus r = sigma (2 x2 - x1), which the x pass hands to the y/z pass.
@@ -39,6 +40,7 @@ struct PriaArrays {
     const Real* h;
     const Real* k;     // [N] scaled K_i
     Real k0;           // scaled K0
+    const Real *xs, *ys, *zs;   // projected start
     Real *xc, *x0, *x, *sx;
     Real *yc, *y0, *y, *sy;
     Real *zc, *z0, *z, *sz;

@@ -39,6 +39,7 @@ struct PriaHostInstance {
     double scale = 1.0, k0 = 0.0;
     std::vector<double> k;                 // scaled
     std::vector<double> h, x0, y0, z0;     // slice order, scaled / projected
+    bool cold = true;                      // start computed on the device
 };
 
 inline int64_t pria_idx(int N, int T, int i, int s, int t) {
@@ -94,15 +95,20 @@ inline int pria_build_host(int64_t n, int64_t t, const double* k, double k0, con
     for (int64_t i = 0; i < n; ++i) s.k[i] = k[i] / s.scale;
     s.k0 = k0 / s.scale;
     s.h.assign(nd, 0.0);
+    const int N = s.N, TT = s.T;
+    for (int i = 0; i < N; ++i)
+        for (int a = 0; a < t; ++a)
+            for (int b = a; b < t; ++b)
+                s.h[pria_idx(N, TT, i, a, b)] = h[(int64_t(i) * t + a) * t + b] / s.scale;
+    s.cold = !warm_x && !warm_y && !warm_z;
+    if (s.cold) return 0;  // the centre of each simplex, filled on the device
     s.x0.assign(nd, 0.0);
     s.y0.assign(nd, 0.0);
     s.z0.assign(nd, 0.0);
-    const int N = s.N, TT = s.T;
     for (int i = 0; i < N; ++i)
         for (int a = 0; a < t; ++a)
             for (int b = a; b < t; ++b) {
                 int64_t g = pria_idx(N, TT, i, a, b), go = (int64_t(i) * t + a) * t + b;
-                s.h[g] = h[go] / s.scale;
                 if (warm_x) s.x0[g] = warm_x[go];
                 if (warm_y) s.y0[g] = warm_y[go] * s.k[i];
                 if (warm_z) s.z0[g] = warm_z[go] * s.k0;
@@ -133,7 +139,7 @@ inline int pria_build_host(int64_t n, int64_t t, const double* k, double k0, con
 }
 
 struct PriaLayout {
-    enum { kK, kH, kX0, kY0, kZ0, kXC, kX, kSX, kYC, kY, kSY, kZC, kZ, kSZ, kR, kCount };
+    enum { kK, kH, kXS, kYS, kZS, kX0, kY0, kZ0, kXC, kX, kSX, kYC, kY, kSY, kZC, kZ, kSZ, kR, kCount };
     size_t off[kCount];
     size_t input_bytes = 0, total_bytes = 0;
 };
@@ -147,7 +153,7 @@ inline PriaLayout pria_layout(int N, int ND, size_t real_size) {
         L.off[q] = at;
         size_t bytes = q == PriaLayout::kK ? real_size * size_t(N) : real_size * size_t(ND);
         at += pria_align256(bytes ? bytes : 1);
-        if (q == PriaLayout::kZ0) L.input_bytes = at;
+        if (q == PriaLayout::kZS) L.input_bytes = at;
     }
     L.total_bytes = at;
     return L;
@@ -161,9 +167,16 @@ inline void pria_fill_image(const PriaHostInstance& s, const PriaLayout& L, char
     };
     put(PriaLayout::kK, s.k);
     put(PriaLayout::kH, s.h);
-    put(PriaLayout::kX0, s.x0);
-    put(PriaLayout::kY0, s.y0);
-    put(PriaLayout::kZ0, s.z0);
+    if (!s.cold) {
+        put(PriaLayout::kXS, s.x0);
+        put(PriaLayout::kYS, s.y0);
+        put(PriaLayout::kZS, s.z0);
+    }
+}
+
+// Bytes of the image that must be uploaded: K and h only for a cold start.
+inline size_t pria_upload_bytes(const PriaHostInstance& s, const PriaLayout& L) {
+    return s.cold ? L.off[PriaLayout::kXS] : L.input_bytes;
 }
 
 template <class Real>
@@ -176,6 +189,9 @@ inline PriaArrays<Real> pria_bind(const PriaLayout& L, char* base, const PriaHos
     auto at = [&](int q) { return reinterpret_cast<Real*>(base + L.off[q]); };
     a.k = at(PriaLayout::kK);
     a.h = at(PriaLayout::kH);
+    a.xs = at(PriaLayout::kXS);
+    a.ys = at(PriaLayout::kYS);
+    a.zs = at(PriaLayout::kZS);
     a.x0 = at(PriaLayout::kX0);
     a.y0 = at(PriaLayout::kY0);
     a.z0 = at(PriaLayout::kZ0);
@@ -210,6 +226,8 @@ inline LogicParams pria_logic_params(const pria_params& p) {
     int64_t warm = p.warm_steps > 0 ? p.warm_steps : 0;
     q.warm_max = q.use_halpern ? q.op_steps * warm : q.max_iter;
     q.fallback_nonpositive = 0;
+    q.verbose = p.verbose;
+    q.pad = 0;
     return q;
 }
 
@@ -258,6 +276,10 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
                       double time_limit_seconds, hipStream_t stream);
 template <class Real>
 int pria_launch_wg(const PriaJob<Real>* jobs, int count, int max_T, hipStream_t stream);
+template <class Real>
+int pria_cold_start(const PriaArrays<Real>& a, hipStream_t stream);
+template <class Real>
+int pria_one_step(const PriaArrays<Real>& a, double omega, double eta, int tier, hipStream_t stream);
 size_t pria_device_work_bytes(int N, int T);
 
 }  // namespace reform

@@ -1,0 +1,118 @@
+"""Writes the golden fixtures under tests/golden/.
+
+  pria_ref.npz   outputs of the reference's own CPU PRIA solver
+                 (jrp/pria-cuda/tests/ref_impl.hpp, built by oracle/Makefile
+                 into oracle/_ref/pria_ref) on instances from its own
+                 generator (ref::random_instance, PCG32): the inputs, and the
+                 iterations, padded-problem primal/dual objectives, gap and
+                 x, y, z it returns at tolerance 1e-8.
+  praq_lp.json   independent LP values of plain-PRAQ instances (the seeded
+                 Erdos-Renyi workload, n = 2000, m = 8000): HiGHS on the
+                 explicit standard linearization, and the max-flow value of the
+                 Hammer-Hansen-Simeone implication network, which equals the
+                 roof dual.
+  pria_lp.json   HiGHS values of the explicit JRP LP on small instances.
+  pria_c1.json   the oracle's FP64 solve (iterations, bounds) of the plain-PRIA
+                 workload (N = 50, T = 200, seed 0) at 1e-6; the oracle itself
+                 is pinned to the reference CPU solver by pria_ref.npz.
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+import oracle  # noqa: E402
+from lp_models import praq_lp_value, praq_maxflow_value, pria_lp_value  # noqa: E402
+from reformulation_amd import workloads  # noqa: E402
+
+HERE = Path(__file__).resolve().parent
+
+PRIA_REF_CASES = [(1, 3, 6), (2, 4, 7), (3, 2, 10), (5, 6, 9), (7, 1, 8), (11, 5, 12),
+                  (13, 8, 5), (17, 3, 15)]
+
+
+def read_instance(path: Path):
+    raw = path.read_bytes()
+    head = np.frombuffer(raw[:24], np.int64)
+    n, t = int(head[1]), int(head[2])
+    k0 = float(np.frombuffer(raw[24:32], np.float64)[0])
+    k = np.frombuffer(raw[32:32 + 8 * n], np.float64).copy()
+    h = np.frombuffer(raw[32 + 8 * n:], np.float64).reshape(n, t, t).copy()
+    return k, k0, h
+
+
+def read_result(path: Path, n: int, t: int):
+    raw = path.read_bytes()
+    ih = np.frombuffer(raw[:32], np.int64)
+    dd = np.frombuffer(raw[32:56], np.float64)
+    sz = n * t * t
+    body = np.frombuffer(raw[56:], np.float64)
+    return (int(ih[2]), float(dd[0]), float(dd[1]), float(dd[2]),
+            body[:sz].reshape(n, t, t).copy(), body[sz:2 * sz].reshape(n, t, t).copy(),
+            body[2 * sz:3 * sz].reshape(n, t, t).copy())
+
+
+def make_pria_ref():
+    oracle.build()
+    binary = oracle.REF_BIN
+    if not binary.exists():
+        raise SystemExit("oracle/_ref/pria_ref is not built (needs the reference tree)")
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for q, (seed, n, t) in enumerate(PRIA_REF_CASES):
+            inst = Path(tmp) / "inst.bin"
+            res = Path(tmp) / "res.bin"
+            subprocess.run([str(binary), "gen", str(seed), str(n), str(t), str(inst)], check=True)
+            subprocess.run([str(binary), "solve", str(inst), "1e-8", "-1", str(res)], check=True)
+            k, k0, h = read_instance(inst)
+            it, primal, dual, gap, x, y, z = read_result(res, n, t)
+            for key, val in dict(k=k, k0=k0, h=h, iterations=it, primal=primal, dual=dual,
+                                 gap=gap, x=x, y=y, z=z).items():
+                out[f"c{q}_{key}"] = np.asarray(val)
+    out["cases"] = np.asarray(PRIA_REF_CASES)
+    np.savez_compressed(HERE / "pria_ref.npz", **out)
+
+
+def make_praq_lp():
+    rows = []
+    for seed in range(3):
+        inst = workloads.praq_er(seed, 2000, 8000)
+        rows.append(dict(seed=seed, n=2000, m=8000, highs=praq_lp_value(inst),
+                         maxflow=praq_maxflow_value(inst)))
+    (HERE / "praq_lp.json").write_text(json.dumps(rows, indent=1))
+
+
+def make_pria_lp():
+    rows = []
+    for seed, n, t in [(0, 3, 8), (1, 5, 12), (2, 4, 9)]:
+        inst = workloads.pria_c1(seed, n, t)
+        rows.append(dict(seed=seed, items=n, periods=t, highs=pria_lp_value(inst)))
+    (HERE / "pria_lp.json").write_text(json.dumps(rows, indent=1))
+
+
+def make_pria_c1():
+    inst = workloads.pria_c1(0, 50, 200)
+    s = oracle.pria_solve(inst.order_costs, inst.joint_order_cost, inst.holding_costs,
+                          oracle.pria_params(tolerance=1e-6))
+    (HERE / "pria_c1.json").write_text(json.dumps(dict(
+        seed=0, items=50, periods=200, tolerance=1e-6, iterations=s.iterations,
+        restarts=s.restarts, upper=s.upper, lower=s.lower, gap=s.relative_gap), indent=1))
+
+
+if __name__ == "__main__":
+    make_pria_c1()
+    make_pria_ref()
+    make_praq_lp()
+    make_pria_lp()
+    print("written:", sorted(p.name for p in HERE.iterdir() if p.suffix in (".npz", ".json")))

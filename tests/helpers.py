@@ -1,0 +1,52 @@
+"""Small instance generators and the half-integral enumeration oracle.
+
+The PRAQ enumeration follows qubo/praq-cuda/tests/test_solver.cu:76-98: the
+relaxation's vertices are half-integral (roof duality), so on a tiny instance
+its optimum is min F(x) over {0, 1/2, 1}^n, F recomputed from the dense
+definition (qubo/praq-jax/praq/qubo_problem.py:95-111).
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+
+def random_qubo(seed: int, n: int, density: float, magnitude: int = 3):
+    """Integer entries in +-[1, magnitude], off-diagonal kept with probability
+    `density`; (n, diag, rows, cols, vals)."""
+    rng = np.random.default_rng(seed)
+    diag = rng.integers(1, magnitude + 1, n) * rng.choice([-1, 1], n)
+    pairs = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < density]
+    rows = np.array([p[0] for p in pairs], dtype=np.int64)
+    cols = np.array([p[1] for p in pairs], dtype=np.int64)
+    vals = (rng.integers(1, magnitude + 1, len(pairs)) * rng.choice([-1, 1], len(pairs)))
+    return n, diag.astype(float), rows, cols, vals.astype(float)
+
+
+def relaxation_value(n, diag, rows, cols, vals, x) -> float:
+    x = np.asarray(x, dtype=float)
+    w = 2.0 * np.asarray(vals)
+    xi, xj = x[rows], x[cols]
+    edge = np.where(w > 0, w * np.maximum(xi + xj - 1.0, 0.0), w * np.minimum(xi, xj))
+    return float(np.asarray(diag) @ x + edge.sum())
+
+
+def half_integral_optimum(n, diag, rows, cols, vals) -> float:
+    best = np.inf
+    for x in itertools.product((0.0, 0.5, 1.0), repeat=n):
+        best = min(best, relaxation_value(n, diag, rows, cols, vals, x))
+    return best
+
+
+def small_jrp(seed: int, n: int, t: int):
+    """Integer-cost JRP instance with the structure of pria-jax's generator."""
+    rng = np.random.default_rng(seed)
+    mult = rng.uniform(0, 1, n)
+    s = np.arange(t)[:, None]
+    tt = np.arange(t)[None, :]
+    h = np.round(np.triu(mult[:, None, None] * (tt - s)[None]
+                         + rng.uniform(0, 1, (n, t, t)) * mult[:, None, None], 1))
+    k = np.maximum(1, np.round(rng.uniform(1, max(t, 1.01), n)))
+    k0 = float(np.round(rng.uniform(1, max(t, 1.01)))) if n > 1 else 0.0
+    return k.astype(float), k0, h

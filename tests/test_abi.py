@@ -1,0 +1,89 @@
+"""The C-ABI library builds for gfx950, loads without a GPU and exports every
+symbol include/*.h declares; the host-side records match.  CPU only."""
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    names = set()
+    for header in (ROOT / "include").glob("*.h"):
+        text = header.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "while", "for", "return", "sizeof"):
+                names.add(name)
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    from reformulation_amd import _native
+    lib = _native.lib()
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_native.EXPORTS) <= declared_symbols()
+
+
+def test_headers_declare_the_reference_abi():
+    names = declared_symbols()
+    for n in ("praq_default_parameters", "praq_check_interval", "praq_solve", "pria_params_auto",
+              "pria_solve_f64", "pria_solve_f32", "pria_solve_batch_f64", "pria_solve_batch_f32",
+              "pria_objective_f64"):
+        assert n in names
+
+
+def test_host_only_entry_points_without_a_gpu():
+    """Entry points that do no device work run on any host."""
+    import ctypes
+    import numpy as np
+    from reformulation_amd import _native
+    lib = _native.lib()
+    p = lib.praq_default_parameters()
+    assert p.tolerance == 1e-8 and p.op_steps == 16 and p.max_iter == -1
+    assert lib.praq_check_interval(400_000, ctypes.byref(p)) == 9
+    assert lib.praq_check_interval(2_000_000, ctypes.byref(p)) == 16
+    q = lib.pria_params_auto(-1, 1e-6)
+    assert abs(q.eta - 0.999 / 2 ** 0.5) < 1e-15 and q.op_steps == 16 and q.omega == 0.10835
+    # pria_objective_f64 on the host: all mass on the diagonal s = t
+    n, t = 2, 3
+    h = np.zeros((n, t, t))
+    x = np.zeros((n, t, t))
+    for i in range(n):
+        for s in range(t):
+            x[i, s, s] = 1.0
+    k = np.array([2.0, 3.0])
+    val = lib.pria_objective_f64(n, t, _native.dptr(k), 5.0, _native.dptr(h), _native.dptr(x))
+    assert val == pytest.approx(t * k.sum() + 5.0 * t)
+
+
+def test_praq_rejects_invalid_arguments_before_any_device_work():
+    import ctypes
+    import numpy as np
+    from reformulation_amd import _native
+    lib = _native.lib()
+    p = lib.praq_default_parameters()
+    res = _native.PraqResult()
+    x = np.zeros(3)
+    diag = np.zeros(3)
+    rows = np.array([0], dtype=np.int64)
+    cols = np.array([0], dtype=np.int64)
+    vals = np.array([1.0])
+    assert lib.praq_solve(None, ctypes.byref(p), _native.dptr(x), ctypes.byref(res)) == -1
+    inst = _native.PraqInstance(3, 1, _native.dptr(diag), _native.iptr(rows), _native.iptr(cols),
+                                _native.dptr(vals))
+    assert lib.praq_solve(ctypes.byref(inst), ctypes.byref(p), None, ctypes.byref(res)) == -1
+    # a self-loop, an index past n and n = 0 are instance errors (-2)
+    assert lib.praq_solve(ctypes.byref(inst), ctypes.byref(p), _native.dptr(x), ctypes.byref(res)) == -2
+    cols[0] = 3
+    assert lib.praq_solve(ctypes.byref(inst), ctypes.byref(p), _native.dptr(x), ctypes.byref(res)) == -2
+    inst.n = 0
+    cols[0] = 1
+    assert lib.praq_solve(ctypes.byref(inst), ctypes.byref(p), _native.dptr(x), ctypes.byref(res)) == -2
+    bad = lib.praq_default_parameters()
+    bad.omega = -1.0
+    inst.n = 3
+    assert lib.praq_solve(ctypes.byref(inst), ctypes.byref(bad), _native.dptr(x), ctypes.byref(res)) == -1

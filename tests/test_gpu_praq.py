@@ -1,0 +1,139 @@
+"""PRAQ on the GPU against the oracle, through the C ABI.  Marked gpu.
+
+Parity bar (FP64 iterates): the same iteration and restart counts as the
+oracle, bounds within 1e-9 relative, primal within 1e-8 absolute -- the
+arithmetic is the same up to summation order.  FP32 iterates: the bracket
+holds the LP value and closes to the requested gap."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import half_integral_optimum, random_qubo, relaxation_value
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _gpu(inst, tol, tier=0, f32=False, max_iter=None, dense=False, warm=None):
+    from reformulation_amd import praq
+    n, d, r, c, v = inst
+    problem = praq.QuboProblem.from_sparse(n, d, r, c, v)
+    params = praq.Parameters.with_auto_step(max_iter, tol, use_float32=f32, tier=tier,
+                                            dense_degree=dense, time_limit=60.0)
+    return praq.solve_with_info(problem, params, warm)
+
+
+@pytest.mark.parametrize("tier", [1, 2])
+def test_fp64_matches_the_oracle_step_for_step(tier):
+    seed = 0
+    for n in range(1, 9):
+        for density in (1.0, 0.5, 0.2, 0.0):
+            inst = random_qubo(seed, n, density)
+            seed += 1
+            for dense in (False, True):
+                o = oracle.praq_solve(*inst, params=oracle.praq_params(tolerance=1e-9,
+                                                                       dense_degree=dense))
+                x, y, info = _gpu(inst, 1e-9, tier=tier, dense=dense)
+                assert info.tier == tier
+                assert info.iterations == o.iterations and info.restarts == o.restarts, (n, density)
+                sc = 1.0 + abs(o.upper)
+                assert abs(info.upper - o.upper) <= 1e-9 * sc
+                assert abs(info.lower - o.lower) <= 1e-9 * sc
+                assert np.max(np.abs(x - o.x)) <= 1e-8
+                if len(inst[4]):
+                    assert np.max(np.abs(y - o.y)) <= 1e-8 * sc
+
+
+def test_matches_enumeration_and_certifies_itself():
+    seed = 500
+    for n in range(2, 8):
+        inst = random_qubo(seed, n, 0.6)
+        seed += 1
+        best = half_integral_optimum(*inst)
+        x, y, info = _gpu(inst, 1e-9)
+        sc = 1.0 + abs(best)
+        assert info.status == 0
+        assert info.lower <= best + 1e-9 * sc and info.upper >= best - 1e-9 * sc
+        assert abs(relaxation_value(*inst, x) - info.upper) <= 1e-9 * sc
+
+
+@pytest.mark.parametrize("tier", [1, 2])
+def test_config_size_fp64_matches_oracle_highs_and_maxflow(tier):
+    from reformulation_amd import workloads
+    rows = json.loads((GOLDEN / "praq_lp.json").read_text())
+    for row in rows[:2]:
+        inst = workloads.praq_er(row["seed"], row["n"], row["m"])
+        tup = (inst.n, inst.diag, inst.rows, inst.cols, inst.vals)
+        o = oracle.praq_solve(*tup, params=oracle.praq_params(tolerance=1e-6))
+        x, y, info = _gpu(tup, 1e-6, tier=tier)
+        assert info.status == 0
+        assert info.iterations == o.iterations
+        opt = row["highs"]
+        assert abs(row["maxflow"] - opt) <= 1e-9 * abs(opt)
+        assert info.lower <= opt + 1e-9 * abs(opt) <= info.upper + 2e-9 * abs(opt)
+        assert info.upper - opt <= 1e-6 * (1 + abs(info.upper) + abs(info.lower))
+        assert abs(info.upper - o.upper) <= 1e-9 * (1 + abs(o.upper))
+
+
+@pytest.mark.parametrize("tier", [1, 2])
+def test_fp32_iterates_reach_1e6(tier):
+    from reformulation_amd import workloads
+    rows = json.loads((GOLDEN / "praq_lp.json").read_text())
+    for row in rows[:2]:
+        inst = workloads.praq_er(row["seed"], row["n"], row["m"])
+        tup = (inst.n, inst.diag, inst.rows, inst.cols, inst.vals)
+        x, y, info = _gpu(tup, 1e-6, tier=tier, f32=True, max_iter=100_000)
+        assert info.status == 0
+        opt = row["highs"]
+        # the bracket is computed in FP64 from FP32 iterates: allow their rounding
+        assert info.lower <= opt + 1e-6 * abs(opt) and opt <= info.upper + 1e-6 * abs(opt)
+        assert abs(info.upper - opt) <= 2e-6 * (1 + abs(opt))
+
+
+def test_bounds_bracket_at_every_budget():
+    inst = random_qubo(101, 8, 0.6)
+    best = half_integral_optimum(*inst)
+    sc = 1.0 + abs(best)
+    for budget in (1, 4, 16, 64, 256):
+        x, y, info = _gpu(inst, 1e-12, max_iter=budget)
+        assert info.iterations <= budget + 15
+        assert info.lower <= best + 1e-9 * sc and info.upper >= best - 1e-9 * sc
+        assert info.status == 1 or info.relative_gap <= 1e-12
+
+
+def test_warm_start_and_special_cases():
+    inst = random_qubo(5, 7, 0.6)
+    x, y, info = _gpu(inst, 1e-9)
+    x2, y2, info2 = _gpu(inst, 1e-9, warm=(x, y))
+    assert info2.iterations <= 16
+    # n = 1
+    _, _, i1 = _gpu((1, np.array([-3.0]), np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0)), 1e-9)
+    assert abs(i1.upper + 3.0) < 1e-8
+    # max-cut form ends at iteration 0
+    n, _, rows, cols, vals = random_qubo(7, 9, 0.7)
+    vals = np.abs(vals)
+    diag = np.zeros(n)
+    np.add.at(diag, rows, -vals)
+    np.add.at(diag, cols, -vals)
+    _, _, im = _gpu((n, diag, rows, cols, vals), 1e-9)
+    assert im.iterations == 0
+
+
+def test_vanilla_and_warm_phase_options_follow_the_oracle():
+    inst = random_qubo(42, 8, 0.7)
+    from reformulation_amd import praq
+    problem = praq.QuboProblem.from_sparse(*inst)
+    for kw in (dict(use_halpern=False, max_iter=400), dict(warm_steps=2)):
+        params = praq.Parameters.with_auto_step(kw.get("max_iter"), 1e-9,
+                                                use_halpern=kw.get("use_halpern", True),
+                                                warm_steps=kw.get("warm_steps", 0))
+        x, y, info = praq.solve_with_info(problem, params)
+        o = oracle.praq_solve(*inst, params=oracle.praq_params(
+            tolerance=1e-9, max_iter=kw.get("max_iter", -1), use_halpern=kw.get("use_halpern", True),
+            warm_steps=kw.get("warm_steps", 0)))
+        assert info.iterations == o.iterations
+        assert abs(info.upper - o.upper) <= 1e-9 * (1 + abs(o.upper))

@@ -1,0 +1,106 @@
+"""PRIA on the GPU against the oracle and the reference CPU solver's outputs,
+through the C ABI.  Marked gpu.
+
+Parity bar (FP64 iterates): iterations equal to the oracle's (which equal the
+reference CPU solver's, tests/golden/pria_ref.npz); bounds within 1e-9
+relative; x, y, z within 1e-8 (simplex thresholds found by Michelot on the
+GPU and by sorting on the CPU agree to rounding).  FP32 iterates: the bracket
+closes to the requested gap around the LP value."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import small_jrp
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _gpu(k, k0, h, tol, tier=0, f32=False, max_iter=None, warm=None):
+    from reformulation_amd import pria
+    problem = pria.JrpProblem.new(k, k0, h)
+    params = pria.Parameters.with_auto_step(max_iter, tol, use_float32=f32, tier=tier,
+                                            time_limit=120.0)
+    return pria.solve_with_info(problem, params, warm)
+
+
+def _ref_cases():
+    data = np.load(GOLDEN / "pria_ref.npz")
+    for q in range(len(data["cases"])):
+        yield q, {k.split("_", 1)[1]: data[k] for k in data.files if k.startswith(f"c{q}_")}
+
+
+@pytest.mark.parametrize("tier", [1, 2])
+def test_fp64_matches_the_reference_cpu_solver(tier):
+    for q, c in _ref_cases():
+        x, y, z, info = _gpu(c["k"], float(c["k0"]), c["h"], 1e-8, tier=tier)
+        assert info.tier == tier
+        assert info.iterations == int(c["iterations"]), q
+        sc = 1.0 + abs(float(c["primal"]))
+        assert abs(info.upper_iterate - float(c["primal"])) <= 1e-9 * sc, q
+        assert abs(info.lower - float(c["dual"])) <= 1e-9 * sc, q
+        assert np.max(np.abs(x - c["x"])) <= 1e-8, q
+        assert np.max(np.abs(y - c["y"])) <= 1e-8, q
+        assert np.max(np.abs(z - c["z"])) <= 1e-8, q
+
+
+@pytest.mark.parametrize("tier", [1, 2])
+def test_fp64_matches_the_oracle(tier):
+    for seed, n, t in [(1, 3, 6), (2, 4, 7), (8, 12, 33), (9, 1, 5), (10, 20, 24)]:
+        k, k0, h = small_jrp(seed, n, t)
+        o = oracle.pria_solve(k, k0, h, oracle.pria_params(tolerance=1e-8))
+        x, y, z, info = _gpu(k, k0, h, 1e-8, tier=tier)
+        assert info.iterations == o.iterations and info.restarts == o.restarts, (seed, n, t)
+        sc = 1.0 + abs(o.upper)
+        assert abs(info.upper - o.upper) <= 1e-9 * sc
+        assert abs(info.lower - o.lower) <= 1e-9 * sc
+        assert np.max(np.abs(x - o.x)) <= 1e-8
+        assert np.max(np.abs(y - o.y)) <= 1e-8
+        assert np.max(np.abs(z - o.z)) <= 1e-8
+
+
+def test_highs_values_and_certificates():
+    from reformulation_amd import pria, workloads
+    rows = json.loads((GOLDEN / "pria_lp.json").read_text())
+    for row in rows:
+        inst = workloads.pria_c1(row["seed"], row["items"], row["periods"])
+        for f32 in (False, True):
+            x, y, z, info = _gpu(inst.order_costs, inst.joint_order_cost, inst.holding_costs,
+                                 1e-6 if f32 else 1e-9, f32=f32)
+            opt = row["highs"]
+            tol = 2e-6 if f32 else 1e-7
+            assert abs(info.upper - opt) <= tol * (1 + abs(opt))
+            lo, up = pria.relaxation_bounds(inst.problem(), x, y, z)
+            assert lo <= opt + tol * abs(opt) and up >= opt - tol * abs(opt)
+
+
+def test_special_cases():
+    # K0 = 0 gives z = 0; max_iter = 64 gives exactly 64; a returned solution
+    # warm-starts as a fixed point at even T
+    k, k0, h = small_jrp(3, 1, 8)
+    x, y, z, info = _gpu(k, 0.0, h, 1e-8)
+    assert np.all(z == 0.0)
+    k, k0, h = small_jrp(113, 6, 18)
+    x, y, z, info = _gpu(k, k0, h, 1e-14, max_iter=64)
+    assert info.iterations == 64 and info.status == 1
+    k, k0, h = small_jrp(4, 3, 8)
+    x, y, z, info = _gpu(k, k0, h, 1e-7)
+    x2, y2, z2, info2 = _gpu(k, k0, h, 1e-7, warm=(x, y, z))
+    assert info2.iterations == 0
+
+
+def test_plain_config_fp64_against_oracle():
+    """The plain-PRIA workload (N = 50, T = 200) on the device tier, FP64."""
+    from reformulation_amd import workloads
+    inst = workloads.pria_c1(0, 50, 200)
+    x, y, z, info = _gpu(inst.order_costs, inst.joint_order_cost, inst.holding_costs, 1e-6,
+                         tier=2, max_iter=50_000)
+    assert info.status == 0
+    gold = json.loads((GOLDEN / "pria_c1.json").read_text())
+    assert info.iterations == gold["iterations"]
+    assert abs(info.upper - gold["upper"]) <= 1e-9 * (1 + abs(gold["upper"]))
+    assert abs(info.lower - gold["lower"]) <= 1e-9 * (1 + abs(gold["lower"]))
