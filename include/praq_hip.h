@@ -39,7 +39,8 @@ typedef struct {
     double tolerance;       /* relative duality gap at termination           */
     int64_t max_iter;       /* <= 0 for no cap                               */
     int use_float32;        /* 0 = float64 iterates; 1 = float32 iterates
-                             * (reductions and scalar logic stay float64)    */
+                             * (reductions and scalar logic stay float64),
+                             * switching to float64 if the gap stalls        */
     double omega;
     double beta_sufficient;
     double beta_necessary;
@@ -90,8 +91,10 @@ typedef struct {
     int64_t iterations, restarts, checks;
     double omega;           /* final primal weight                           */
     double scale;           /* objective scale                               */
-    int status;
+    int status;             /* 0 converged, 1 iteration limit, 2 time limit  */
     int tier;               /* tier that ran                                 */
+    int64_t switch_iteration; /* FP32 iterates handed over to FP64 at this
+                               * iteration, having stalled; -1 if they did not */
 } PraqInfo;
 
 PraqOptions praq_default_options(void);

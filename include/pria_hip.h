@@ -98,6 +98,8 @@ typedef struct {
     double omega, scale;
     int status;               /* 0 converged, 1 iteration limit, 2 time limit */
     int tier;
+    int64_t switch_iteration; /* FP32 iterates handed over to FP64 at this
+                               * iteration, having stalled; -1 if they did not */
 } pria_info;
 
 pria_options pria_default_options(void);

@@ -4,17 +4,18 @@
  * only, a thread fan-out of solo solves: jrp/pria-cuda/include/pria/pria.h:85-99,
  * jrp/pria-cuda/src/pria.cu:15-44).  Here a batch holds instances of any size
  * of both families, packed ragged in one device allocation, and solves them
- * either
- *   REFORM_BATCHED     all together: one persistent launch per family in
- *                      which each instance is one workgroup running its
- *                      whole solve (steps, checks, restarts, termination),
- *                      instances ordered largest first; or
+ *   REFORM_BATCHED     all together through block tables: every kernel
+ *                      launch covers the work of every instance, each
+ *                      instance keeps its own restart state and termination
+ *                      flag on the device, and one check interval's launches
+ *                      are a hipGraph replayed until all have finished;
  *   REFORM_SEQUENTIAL  one at a time with the plain solver's tier choice
  *                      (praq_solve_ex / pria_solve_ex on resident inputs):
- *                      the baseline the batched mode is measured against.
- * Each instance's result is the same in either mode when the plain tier of
- * that instance is the workgroup tier, bitwise, and it is the same alone and
- * in any mix, since every workgroup runs one instance with the same code.
+ *                      the baseline the batched mode is measured against;
+ *   REFORM_WORKGROUPS  one persistent launch per family in which each
+ *                      instance is one workgroup running its whole solve.
+ * An instance's batched result does not depend on the other instances of the
+ * batch: its arithmetic and summation order are those of its own blocks.
  *
  * Usage: create, add_* (host), upload (host build + one H2D copy), then
  * solve any number of times from the resident inputs, then read results.
@@ -35,7 +36,7 @@ extern "C" {
 
 typedef struct reform_batch reform_batch;
 
-enum { REFORM_BATCHED = 0, REFORM_SEQUENTIAL = 1 };
+enum { REFORM_BATCHED = 0, REFORM_SEQUENTIAL = 1, REFORM_WORKGROUPS = 2 };
 
 /* use_float32: iterate precision of every instance of the batch
  * (reductions and scalar logic stay FP64). */
@@ -61,8 +62,9 @@ int reform_batch_set_shape(reform_batch* batch, int praq_block, int pria_block);
 
 int64_t reform_batch_count(const reform_batch* batch, int family /* 0 PRAQ, 1 PRIA */);
 
-/* Durations of the last batched solve's two launches, measured with HIP
- * events on the streams they ran on (0 for an empty family). */
+/* Durations of the last batched solve's two families, measured with HIP
+ * events on the streams they ran on (0 for an empty family or a sequential
+ * solve). */
 int reform_batch_kernel_ms(const reform_batch* batch, double* praq_ms, double* pria_ms);
 int64_t reform_batch_device_bytes(const reform_batch* batch);
 

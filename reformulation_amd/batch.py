@@ -4,13 +4,17 @@
     for problem in qubo_problems: b.add_praq(problem, praq_params)
     for problem in jrp_problems:  b.add_pria(problem, pria_params)
     b.upload()                      # host build + one transfer per instance
-    seconds = b.solve("batched")    # one persistent launch per family
+    seconds = b.solve("batched")    # all instances together
     x, y, info = b.praq_result(0)
 
-`solve("sequential")` runs the same resident instances one at a time with
-the plain solver's tier choice: the baseline the batched mode is measured
-against.  The batched result of an instance is the same, bitwise, as its
-result alone or in any other mix (one workgroup per instance, same code).
+"batched" advances every instance of a family with the same kernel launches
+(block tables over the packed instances, per-instance restart state and
+termination flags on the device, one check interval captured as a hipGraph
+and replayed).  "sequential" runs the same resident instances one at a time
+with the plain solver's tier choice: the baseline the batched mode is
+measured against.  "workgroups" runs each instance's whole solve in one
+workgroup of a persistent launch.  An instance's batched result does not
+depend on the rest of its batch.
 """
 from __future__ import annotations
 
@@ -21,8 +25,8 @@ import numpy as np
 
 from . import _native, praq as _praq, pria as _pria
 
-BATCHED, SEQUENTIAL = 0, 1
-_MODES = {"batched": BATCHED, "sequential": SEQUENTIAL}
+BATCHED, SEQUENTIAL, WORKGROUPS = 0, 1, 2
+_MODES = {"batched": BATCHED, "sequential": SEQUENTIAL, "workgroups": WORKGROUPS}
 
 
 def _declare(L):

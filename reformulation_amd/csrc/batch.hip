@@ -1,15 +1,26 @@
 // Ragged batches of PRAQ and PRIA instances: include/reform_batch.h.
 //
 // Device memory: one allocation holds every instance's image (inputs first,
-// iteration state after, 256-byte aligned), the job records the kernels read
-// and the final scalar states.  The batched mode launches one workgroup per
-// instance -- PRAQ and PRIA on two streams so the families overlap -- with
-// the instances ordered largest first, so the longest solves start first and
-// the small ones fill the CUs around them.  The sequential mode is the plain
-// solver: per instance the tier praq_solve_ex / pria_solve_ex would pick.
+// iteration state after, 256-byte aligned) -- in FP64 always and, for an FP32
+// batch, in FP32 too -- the job records the kernels read and the scalar
+// states.  Three ways to solve the resident instances:
+//   batched     the wide engine (wide.hpp): every kernel launch covers the
+//               work of every instance of a family through a block table,
+//               instances keep their own Halpern state on the device, and
+//               the launches of one check interval form a hipGraph replayed
+//               until every instance has finished; PRAQ and PRIA run on two
+//               streams driven from two host threads;
+//   workgroups  one persistent launch per family, one workgroup per
+//               instance running its whole solve, largest first;
+//   sequential  the plain solver: per instance the tier and precision path
+//               praq_solve_ex / pria_solve_ex would take.
+// With FP32 iterates, an instance whose gap stalls stops with its Halpern
+// state intact; its state is converted to FP64 and it continues there (a
+// second wide engine over the stalled subset, or one more launch).
 #include "reform_batch.h"
 #include "praq_host.hpp"
 #include "pria_host.hpp"
+#include "wide.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -25,6 +36,7 @@ using namespace reform;
 namespace {
 
 constexpr double kWgLimitSeconds = 600.0;
+constexpr int kStallChecks = 24;
 
 double clock_rate_hz() {
     int dev = 0, khz = 0;
@@ -56,13 +68,13 @@ void parallel_for(size_t count, F f) {
 }
 
 struct PraqEntry {
-    PraqInstance copy;  // pointers into owned storage below
+    PraqInstance copy;  // pointers into the owned storage below
     std::vector<double> diag, vals;
     std::vector<int64_t> rows, cols;
     LogicParams lp;
     PraqOptions o;
     PraqHostInstance host;
-    size_t off = 0;      // image offset in the device allocation
+    size_t off32 = 0, off64 = 0;  // image offsets in the device allocation
     int tier = 0;
 };
 
@@ -73,339 +85,686 @@ struct PriaEntry {
     LogicParams lp;
     pria_options o;
     PriaHostInstance host;
-    size_t off = 0;
+    size_t off32 = 0, off64 = 0;
     int tier = 0;
 };
 
-struct BatchBase {
+// The job records of one family in one precision, in launch order, host and
+// device copies; `sub` is device scratch for the stalled subset.
+template <class Job>
+struct JobSet {
+    std::vector<Job> host;
+    Job* dev = nullptr;
+    Job* sub = nullptr;
+};
+
+template <class Real>
+int read_array(std::vector<double>& out, const Real* src, size_t count) {
+    std::vector<Real> v(count);
+    if (count && hipMemcpy(v.data(), src, sizeof(Real) * count, hipMemcpyDeviceToHost) != hipSuccess)
+        return kErrorDevice;
+    out.assign(v.begin(), v.end());
+    return 0;
+}
+
+struct BatchImpl {
     int use_float32 = 0;
-    int praq_block = 256, pria_block = 1024;
+    int praq_block = 256;
     std::vector<PraqEntry> praq;
     std::vector<PriaEntry> pria;
     void* dev = nullptr;
     size_t dev_bytes = 0;
     hipStream_t s0 = nullptr, s1 = nullptr;
-    std::vector<LogicState> praq_states, pria_states;
-    bool uploaded = false;
-    // events around the batched launches: [praq start, praq end, pria start, pria end]
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double praq_ms = 0.0, pria_ms = 0.0;
-    virtual ~BatchBase() {
+    bool uploaded = false;
+
+    std::vector<int64_t> praq_order, pria_order, praq_slot, pria_slot;
+    JobSet<PraqJob<float>> praq32;
+    JobSet<PraqJob<double>> praq64;
+    JobSet<PriaJob<float>> pria32;
+    JobSet<PriaJob<double>> pria64;
+    LogicState* praq_states_dev = nullptr;
+    LogicState* pria_states_dev = nullptr;
+    std::vector<LogicState> praq_states, pria_states;
+    std::vector<char> praq_in64, pria_in64;  // final point lives in the FP64 image
+    double* work = nullptr;
+    int pria_max_T = 2;
+    int device = 0;
+    double time_limit = 0.0;
+
+    // wide engines: over the whole family (in the batch's precision) and,
+    // for FP32 batches, over the last stalled subset in FP64
+    std::unique_ptr<WidePraq<float>> wpraq32;
+    std::unique_ptr<WidePraq<double>> wpraq64, wpraq64_sub;
+    std::unique_ptr<WidePria<float>> wpria32;
+    std::unique_ptr<WidePria<double>> wpria64, wpria64_sub;
+    std::vector<int> wpraq_sub_key, wpria_sub_key;
+    WideStats praq_stats, pria_stats;
+
+    ~BatchImpl() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (dev) (void)hipFree(dev);
         if (s0) (void)hipStreamDestroy(s0);
         if (s1) (void)hipStreamDestroy(s1);
     }
-    virtual int upload() = 0;
-    virtual int solve(int mode) = 0;
-    virtual int praq_result(int64_t q, double* x, double* y, PraqInfo* info) = 0;
-    virtual int pria_result(int64_t q, double* x, double* y, double* z, pria_info* info) = 0;
+
+    int upload();
+    int solve(int mode);
+    int solve_wide();
+    int wide_praq();
+    int wide_pria();
+    int solve_workgroups();
+    int solve_sequential();
+    int praq_result(int64_t q, double* x, double* y, PraqInfo* info);
+    int pria_result(int64_t q, double* x, double* y, double* z, pria_info* info);
+
+    // one plain PRAQ solve of launch slot s from job `job` (its precision)
+    template <class Real>
+    int praq_plain(const PraqJob<Real>& job, PraqJob<Real>* devslot, int tier, bool resume) {
+        if (tier == PRAQ_TIER_WORKGROUP) {
+            PraqJob<Real> j = job;
+            j.resume = resume ? 1 : 0;
+            if (hipMemcpyAsync(devslot, &j, sizeof(j), hipMemcpyHostToDevice, s0) != hipSuccess)
+                return kErrorDevice;
+            if (praq_launch_wg<Real>(devslot, 1, 1024, s0)) return kErrorDevice;
+            return hipStreamSynchronize(s0) == hipSuccess ? 0 : kErrorDevice;
+        }
+        LogicState st;
+        if (resume && hipMemcpy(&st, job.state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
+            return kErrorDevice;
+        if (praq_device_solve<Real>(job, st, work, 0.0, s0, resume)) return kErrorDevice;
+        if (hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice, s0) != hipSuccess)
+            return kErrorDevice;
+        return hipStreamSynchronize(s0) == hipSuccess ? 0 : kErrorDevice;
+    }
+
+    template <class Real>
+    int pria_plain(const PriaJob<Real>& job, PriaJob<Real>* devslot, int tier, bool resume, int T) {
+        if (tier == PRIA_TIER_WORKGROUP) {
+            PriaJob<Real> j = job;
+            j.resume = resume ? 1 : 0;
+            if (hipMemcpyAsync(devslot, &j, sizeof(j), hipMemcpyHostToDevice, s1) != hipSuccess)
+                return kErrorDevice;
+            if (pria_launch_wg<Real>(devslot, 1, T, s1)) return kErrorDevice;
+            return hipStreamSynchronize(s1) == hipSuccess ? 0 : kErrorDevice;
+        }
+        LogicState st;
+        if (resume && hipMemcpy(&st, job.state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
+            return kErrorDevice;
+        if (pria_device_solve<Real>(job, st, work, 0.0, s1, resume)) return kErrorDevice;
+        if (hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice, s1) != hipSuccess)
+            return kErrorDevice;
+        return hipStreamSynchronize(s1) == hipSuccess ? 0 : kErrorDevice;
+    }
 };
 
-template <class Real>
-struct Batch : BatchBase {
-    std::vector<PraqJob<Real>> praq_jobs;   // in launch order (largest first)
-    std::vector<PriaJob<Real>> pria_jobs;
-    std::vector<int64_t> praq_order, pria_order;  // launch slot -> index
-    std::vector<int64_t> praq_slot, pria_slot;    // index -> launch slot
-    size_t praq_jobs_off = 0, pria_jobs_off = 0, praq_states_off = 0, pria_states_off = 0;
-    size_t work_off = 0;
-    int pria_max_T = 2;
+int BatchImpl::upload() {
+    if (uploaded) return kErrorArgument;
+    std::vector<int> fail(praq.size() + pria.size(), 0);
+    parallel_for(praq.size(), [&](size_t q) {
+        PraqEntry& e = praq[q];
+        fail[q] = praq_build_host(e.copy, e.o.dense_degree, nullptr, nullptr, e.host);
+    });
+    parallel_for(pria.size(), [&](size_t q) {
+        PriaEntry& e = pria[q];
+        fail[praq.size() + q] = pria_build_host(e.n, e.t, e.k.data(), e.k0, e.h.data(), nullptr,
+                                                nullptr, nullptr, e.host);
+    });
+    for (int f : fail)
+        if (f) return kErrorInstance;
 
-    int upload() override {
-        if (uploaded) return kErrorArgument;
-        // host builds, in parallel
-        std::vector<int> fail(praq.size() + pria.size(), 0);
-        parallel_for(praq.size(), [&](size_t q) {
-            PraqEntry& e = praq[q];
-            fail[q] = praq_build_host(e.copy, e.o.dense_degree, nullptr, nullptr, e.host);
-        });
-        parallel_for(pria.size(), [&](size_t q) {
-            PriaEntry& e = pria[q];
-            fail[praq.size() + q] = pria_build_host(e.n, e.t, e.k.data(), e.k0, e.h.data(),
-                                                    nullptr, nullptr, nullptr, e.host);
-        });
-        for (int f : fail)
-            if (f) return kErrorInstance;
+    // launch order: largest first
+    praq_order.resize(praq.size());
+    std::iota(praq_order.begin(), praq_order.end(), 0);
+    std::stable_sort(praq_order.begin(), praq_order.end(), [&](int64_t a, int64_t b) {
+        return int64_t(praq[a].host.n) + 2 * int64_t(praq[a].host.m) >
+               int64_t(praq[b].host.n) + 2 * int64_t(praq[b].host.m);
+    });
+    pria_order.resize(pria.size());
+    std::iota(pria_order.begin(), pria_order.end(), 0);
+    std::stable_sort(pria_order.begin(), pria_order.end(),
+                     [&](int64_t a, int64_t b) { return pria[a].host.ND > pria[b].host.ND; });
+    praq_slot.resize(praq.size());
+    pria_slot.resize(pria.size());
+    for (size_t s = 0; s < praq_order.size(); ++s) praq_slot[praq_order[s]] = int64_t(s);
+    for (size_t s = 0; s < pria_order.size(); ++s) pria_slot[pria_order[s]] = int64_t(s);
 
-        // launch order: largest first
-        praq_order.resize(praq.size());
-        std::iota(praq_order.begin(), praq_order.end(), 0);
-        std::stable_sort(praq_order.begin(), praq_order.end(), [&](int64_t a, int64_t b) {
-            return int64_t(praq[a].host.n) + 2 * int64_t(praq[a].host.m) >
-                   int64_t(praq[b].host.n) + 2 * int64_t(praq[b].host.m);
-        });
-        pria_order.resize(pria.size());
-        std::iota(pria_order.begin(), pria_order.end(), 0);
-        std::stable_sort(pria_order.begin(), pria_order.end(),
-                         [&](int64_t a, int64_t b) { return pria[a].host.ND > pria[b].host.ND; });
-        praq_slot.resize(praq.size());
-        pria_slot.resize(pria.size());
-        for (size_t s = 0; s < praq_order.size(); ++s) praq_slot[praq_order[s]] = int64_t(s);
-        for (size_t s = 0; s < pria_order.size(); ++s) pria_slot[pria_order[s]] = int64_t(s);
-
-        // device allocation: images, jobs, states, one device-tier work area
-        size_t at = 0, work = praq_device_work_bytes();
-        std::vector<PraqLayout> pl(praq.size());
-        std::vector<PriaLayout> rl(pria.size());
-        for (size_t q = 0; q < praq.size(); ++q) {
-            pl[q] = praq_layout(praq[q].host.n, praq[q].host.m, sizeof(Real));
-            praq[q].off = at;
-            at += align256(pl[q].total_bytes);
+    // one allocation: FP64 images, FP32 images (FP32 batch), job records,
+    // states, one device-tier work area
+    const bool f32 = use_float32 != 0;
+    size_t at = 0, work_bytes = praq_device_work_bytes();
+    std::vector<PraqLayout> pl64(praq.size()), pl32(praq.size());
+    std::vector<PriaLayout> rl64(pria.size()), rl32(pria.size());
+    for (size_t q = 0; q < praq.size(); ++q) {
+        pl64[q] = praq_layout(praq[q].host.n, praq[q].host.m, sizeof(double));
+        praq[q].off64 = at;
+        at += align256(pl64[q].total_bytes);
+        if (f32) {
+            pl32[q] = praq_layout(praq[q].host.n, praq[q].host.m, sizeof(float));
+            praq[q].off32 = at;
+            at += align256(pl32[q].total_bytes);
         }
-        for (size_t q = 0; q < pria.size(); ++q) {
-            rl[q] = pria_layout(pria[q].host.N, pria[q].host.ND, sizeof(Real));
-            pria[q].off = at;
-            at += align256(rl[q].total_bytes);
-            work = std::max(work, pria_device_work_bytes(pria[q].host.N, pria[q].host.T));
-            pria_max_T = std::max(pria_max_T, pria[q].host.T);
+    }
+    for (size_t q = 0; q < pria.size(); ++q) {
+        rl64[q] = pria_layout(pria[q].host.N, pria[q].host.ND, sizeof(double));
+        pria[q].off64 = at;
+        at += align256(rl64[q].total_bytes);
+        if (f32) {
+            rl32[q] = pria_layout(pria[q].host.N, pria[q].host.ND, sizeof(float));
+            pria[q].off32 = at;
+            at += align256(rl32[q].total_bytes);
         }
-        praq_jobs_off = at;
-        at = align256(at + sizeof(PraqJob<Real>) * (praq.size() + 1));
-        pria_jobs_off = at;
-        at = align256(at + sizeof(PriaJob<Real>) * (pria.size() + 1));
-        praq_states_off = at;
-        at = align256(at + sizeof(LogicState) * (praq.size() + 1));
-        pria_states_off = at;
-        at = align256(at + sizeof(LogicState) * (pria.size() + 1));
-        work_off = at;
-        at = align256(at + work);
-        dev_bytes = at;
-        if (hipMalloc(&dev, dev_bytes) != hipSuccess) {
-            dev = nullptr;
-            return kErrorDevice;
-        }
-        if (hipStreamCreateWithFlags(&s0, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&s1, hipStreamNonBlocking) != hipSuccess)
-            return kErrorDevice;
-        for (auto& e : ev)
-            if (hipEventCreate(&e) != hipSuccess) return kErrorDevice;
-        char* base = static_cast<char*>(dev);
+        work_bytes = std::max(work_bytes, pria_device_work_bytes(pria[q].host.N, pria[q].host.T));
+        pria_max_T = std::max(pria_max_T, pria[q].host.T);
+    }
+    const size_t np = praq.size() + 1, nr = pria.size() + 1;
+    const size_t off_p32 = at;
+    at = align256(at + 2 * sizeof(PraqJob<float>) * np);
+    const size_t off_p64 = at;
+    at = align256(at + 2 * sizeof(PraqJob<double>) * np);
+    const size_t off_r32 = at;
+    at = align256(at + 2 * sizeof(PriaJob<float>) * nr);
+    const size_t off_r64 = at;
+    at = align256(at + 2 * sizeof(PriaJob<double>) * nr);
+    const size_t off_ps = at;
+    at = align256(at + sizeof(LogicState) * np);
+    const size_t off_rs = at;
+    at = align256(at + sizeof(LogicState) * nr);
+    const size_t off_work = at;
+    at = align256(at + work_bytes);
+    dev_bytes = at;
+    if (hipMalloc(&dev, dev_bytes) != hipSuccess) {
+        dev = nullptr;
+        return kErrorDevice;
+    }
+    if (hipGetDevice(&device) != hipSuccess) return kErrorDevice;
+    if (hipStreamCreateWithFlags(&s0, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s1, hipStreamNonBlocking) != hipSuccess)
+        return kErrorDevice;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return kErrorDevice;
+    char* base = static_cast<char*>(dev);
+    praq32.dev = reinterpret_cast<PraqJob<float>*>(base + off_p32);
+    praq32.sub = praq32.dev + np;
+    praq64.dev = reinterpret_cast<PraqJob<double>*>(base + off_p64);
+    praq64.sub = praq64.dev + np;
+    pria32.dev = reinterpret_cast<PriaJob<float>*>(base + off_r32);
+    pria32.sub = pria32.dev + nr;
+    pria64.dev = reinterpret_cast<PriaJob<double>*>(base + off_r64);
+    pria64.sub = pria64.dev + nr;
+    praq_states_dev = reinterpret_cast<LogicState*>(base + off_ps);
+    pria_states_dev = reinterpret_cast<LogicState*>(base + off_rs);
+    work = reinterpret_cast<double*>(base + off_work);
 
-        // inputs: build each image into one host staging area, one copy each
-        size_t up_total = 0;
-        for (size_t q = 0; q < praq.size(); ++q) up_total = std::max(up_total, pl[q].input_bytes);
-        for (size_t q = 0; q < pria.size(); ++q)
-            up_total = std::max(up_total, pria_upload_bytes(pria[q].host, rl[q]));
-        std::vector<char> image(up_total);
-        const long long ticks = (long long)(kWgLimitSeconds * clock_rate_hz());
-        praq_jobs.assign(praq.size(), PraqJob<Real>());
-        for (size_t s = 0; s < praq_order.size(); ++s) {
-            PraqEntry& e = praq[praq_order[s]];
-            std::fill(image.begin(), image.begin() + pl[praq_order[s]].input_bytes, 0);
-            praq_fill_image<Real>(e.host, pl[praq_order[s]], image.data());
-            if (hipMemcpy(base + e.off, image.data(), pl[praq_order[s]].input_bytes,
-                          hipMemcpyHostToDevice) != hipSuccess)
+    // inputs: each image built into a host staging area and copied
+    size_t stage = 0;
+    for (size_t q = 0; q < praq.size(); ++q) stage = std::max(stage, pl64[q].input_bytes);
+    for (size_t q = 0; q < pria.size(); ++q)
+        stage = std::max(stage, pria_upload_bytes(pria[q].host, rl64[q]));
+    std::vector<char> image(stage);
+    const long long ticks = (long long)(kWgLimitSeconds * clock_rate_hz());
+    const int64_t praq_wg = env_int("REFORM_PRAQ_WG_MAX", int64_t(1) << 17);
+    const int64_t pria_wg = env_int("REFORM_PRIA_WG_MAX", int64_t(1) << 15);
+
+    praq64.host.assign(praq.size(), PraqJob<double>());
+    if (f32) praq32.host.assign(praq.size(), PraqJob<float>());
+    for (size_t s = 0; s < praq_order.size(); ++s) {
+        const int64_t q = praq_order[s];
+        PraqEntry& e = praq[q];
+        e.tier = e.o.tier != PRAQ_TIER_AUTO
+                     ? e.o.tier
+                     : (int64_t(e.host.n) + e.host.m <= praq_wg ? PRAQ_TIER_WORKGROUP
+                                                                : PRAQ_TIER_DEVICE);
+        auto fill = [&](auto& job, const PraqLayout& L, size_t off, auto tag) -> int {
+            using Real = decltype(tag);
+            std::fill(image.begin(), image.begin() + L.input_bytes, 0);
+            praq_fill_image<Real>(e.host, L, image.data());
+            if (hipMemcpy(base + off, image.data(), L.input_bytes, hipMemcpyHostToDevice) !=
+                hipSuccess)
                 return kErrorDevice;
-            PraqJob<Real>& j = praq_jobs[s];
-            std::memset(&j, 0, sizeof(j));
-            j.a = praq_bind<Real>(pl[praq_order[s]], base + e.off, e.host.n, e.host.m);
-            j.p = e.lp;
-            j.scale = e.host.scale;
-            j.lanes = e.host.lanes;
-            j.time_limit_ticks = ticks;
-            j.state = reinterpret_cast<LogicState*>(base + praq_states_off) + s;
-            int64_t size = int64_t(e.host.n) + int64_t(e.host.m);
-            e.tier = e.o.tier != PRAQ_TIER_AUTO
-                         ? e.o.tier
-                         : (size <= env_int("REFORM_PRAQ_WG_MAX", int64_t(1) << 17)
-                                ? PRAQ_TIER_WORKGROUP
-                                : PRAQ_TIER_DEVICE);
+            std::memset(&job, 0, sizeof(job));
+            job.a = praq_bind<Real>(L, base + off, e.host.n, e.host.m);
+            job.p = e.lp;
+            job.scale = e.host.scale;
+            job.lanes = e.host.lanes;
+            job.time_limit_ticks = ticks;
+            job.state = praq_states_dev + s;
+            return 0;
+        };
+        if (fill(praq64.host[s], pl64[q], e.off64, double())) return kErrorDevice;
+        if (f32) {
+            if (fill(praq32.host[s], pl32[q], e.off32, float())) return kErrorDevice;
+            praq32.host[s].p.stall_checks = kStallChecks;
         }
-        pria_jobs.assign(pria.size(), PriaJob<Real>());
-        for (size_t s = 0; s < pria_order.size(); ++s) {
-            PriaEntry& e = pria[pria_order[s]];
-            const PriaLayout& L = rl[pria_order[s]];
+    }
+    pria64.host.assign(pria.size(), PriaJob<double>());
+    if (f32) pria32.host.assign(pria.size(), PriaJob<float>());
+    for (size_t s = 0; s < pria_order.size(); ++s) {
+        const int64_t q = pria_order[s];
+        PriaEntry& e = pria[q];
+        e.tier = e.o.tier != PRIA_TIER_AUTO
+                     ? e.o.tier
+                     : (e.host.ND <= pria_wg ? PRIA_TIER_WORKGROUP : PRIA_TIER_DEVICE);
+        auto fill = [&](auto& job, const PriaLayout& L, size_t off, auto tag) -> int {
+            using Real = decltype(tag);
             size_t up = pria_upload_bytes(e.host, L);
             std::fill(image.begin(), image.begin() + up, 0);
             pria_fill_image<Real>(e.host, L, image.data());
-            if (hipMemcpy(base + e.off, image.data(), up, hipMemcpyHostToDevice) != hipSuccess)
+            if (hipMemcpy(base + off, image.data(), up, hipMemcpyHostToDevice) != hipSuccess)
                 return kErrorDevice;
-            PriaJob<Real>& j = pria_jobs[s];
-            std::memset(&j, 0, sizeof(j));
-            j.a = pria_bind<Real>(L, base + e.off, e.host);
-            if (e.host.cold && pria_cold_start<Real>(j.a, s1)) return kErrorDevice;
-            j.p = e.lp;
-            j.scale = e.host.scale;
-            j.time_limit_ticks = ticks;
-            j.state = reinterpret_cast<LogicState*>(base + pria_states_off) + s;
-            e.tier = e.o.tier != PRIA_TIER_AUTO
-                         ? e.o.tier
-                         : (e.host.ND <= env_int("REFORM_PRIA_WG_MAX", int64_t(1) << 15)
-                                ? PRIA_TIER_WORKGROUP
-                                : PRIA_TIER_DEVICE);
+            std::memset(&job, 0, sizeof(job));
+            job.a = pria_bind<Real>(L, base + off, e.host);
+            if (e.host.cold && pria_cold_start<Real>(job.a, s1)) return kErrorDevice;
+            job.p = e.lp;
+            job.scale = e.host.scale;
+            job.time_limit_ticks = ticks;
+            job.state = pria_states_dev + s;
+            return 0;
+        };
+        if (fill(pria64.host[s], rl64[q], e.off64, double())) return kErrorDevice;
+        if (f32) {
+            if (fill(pria32.host[s], rl32[q], e.off32, float())) return kErrorDevice;
+            pria32.host[s].p.stall_checks = kStallChecks;
         }
-        if (!praq_jobs.empty() &&
-            hipMemcpy(base + praq_jobs_off, praq_jobs.data(), sizeof(PraqJob<Real>) * praq.size(),
-                      hipMemcpyHostToDevice) != hipSuccess)
-            return kErrorDevice;
-        if (!pria_jobs.empty() &&
-            hipMemcpy(base + pria_jobs_off, pria_jobs.data(), sizeof(PriaJob<Real>) * pria.size(),
-                      hipMemcpyHostToDevice) != hipSuccess)
-            return kErrorDevice;
-        if (hipStreamSynchronize(s1) != hipSuccess) return kErrorDevice;
-        praq_states.assign(praq.size(), LogicState());
-        pria_states.assign(pria.size(), LogicState());
-        uploaded = true;
-        return 0;
     }
+    auto put = [&](auto& set) -> int {
+        if (set.host.empty()) return 0;
+        return hipMemcpy(set.dev, set.host.data(), sizeof(set.host[0]) * set.host.size(),
+                         hipMemcpyHostToDevice) == hipSuccess
+                   ? 0
+                   : kErrorDevice;
+    };
+    if (put(praq64) || put(pria64)) return kErrorDevice;
+    if (f32 && (put(praq32) || put(pria32))) return kErrorDevice;
+    if (hipStreamSynchronize(s1) != hipSuccess) return kErrorDevice;
+    praq_states.assign(praq.size(), LogicState());
+    pria_states.assign(pria.size(), LogicState());
+    praq_in64.assign(praq.size(), 1);
+    pria_in64.assign(pria.size(), 1);
+    uploaded = true;
+    return 0;
+}
 
-    int solve(int mode) override {
-        if (!uploaded) return kErrorArgument;
-        char* base = static_cast<char*>(dev);
-        auto* pj = reinterpret_cast<PraqJob<Real>*>(base + praq_jobs_off);
-        auto* rj = reinterpret_cast<PriaJob<Real>*>(base + pria_jobs_off);
-        double* work = reinterpret_cast<double*>(base + work_off);
-        praq_ms = pria_ms = 0.0;
-        if (mode == REFORM_BATCHED) {
-            if (!praq.empty()) {
-                if (hipEventRecord(ev[0], s0) != hipSuccess) return kErrorDevice;
-                if (praq_launch_wg<Real>(pj, int(praq.size()), praq_block, s0)) return kErrorDevice;
-                if (hipEventRecord(ev[1], s0) != hipSuccess) return kErrorDevice;
-            }
-            if (!pria.empty()) {
-                if (hipEventRecord(ev[2], s1) != hipSuccess) return kErrorDevice;
-                if (pria_launch_wg<Real>(rj, int(pria.size()), pria_max_T, s1)) return kErrorDevice;
-                if (hipEventRecord(ev[3], s1) != hipSuccess) return kErrorDevice;
-            }
-        } else if (mode == REFORM_SEQUENTIAL) {
-            // in the order the instances were added, one at a time
-            for (size_t q = 0; q < praq.size(); ++q) {
-                int64_t s = praq_slot[q];
-                if (praq[q].tier == PRAQ_TIER_WORKGROUP) {
-                    if (praq_launch_wg<Real>(pj + s, 1, 1024, s0)) return kErrorDevice;
-                    if (hipStreamSynchronize(s0) != hipSuccess) return kErrorDevice;
-                } else {
-                    LogicState st;
-                    if (praq_device_solve<Real>(praq_jobs[s], st, work, 0.0, s0)) return kErrorDevice;
-                    if (hipMemcpyAsync(praq_jobs[s].state, &st, sizeof(st), hipMemcpyHostToDevice,
-                                       s0) != hipSuccess ||
-                        hipStreamSynchronize(s0) != hipSuccess)
-                        return kErrorDevice;
-                }
-            }
-            for (size_t q = 0; q < pria.size(); ++q) {
-                int64_t s = pria_slot[q];
-                if (pria[q].tier == PRIA_TIER_WORKGROUP) {
-                    if (pria_launch_wg<Real>(rj + s, 1, pria[q].host.T, s1)) return kErrorDevice;
-                    if (hipStreamSynchronize(s1) != hipSuccess) return kErrorDevice;
-                } else {
-                    LogicState st;
-                    if (pria_device_solve<Real>(pria_jobs[s], st, work, 0.0, s1)) return kErrorDevice;
-                    if (hipMemcpyAsync(pria_jobs[s].state, &st, sizeof(st), hipMemcpyHostToDevice,
-                                       s1) != hipSuccess ||
-                        hipStreamSynchronize(s1) != hipSuccess)
-                        return kErrorDevice;
-                }
-            }
-        } else {
-            return kErrorArgument;
+int BatchImpl::solve(int mode) {
+    if (!uploaded) return kErrorArgument;
+    praq_ms = pria_ms = 0.0;
+    int st = kErrorArgument;
+    if (mode == REFORM_BATCHED) st = solve_wide();
+    else if (mode == REFORM_SEQUENTIAL) st = solve_sequential();
+    else if (mode == REFORM_WORKGROUPS) st = solve_workgroups();
+    if (st) return st;
+    if (!praq.empty() &&
+        hipMemcpyAsync(praq_states.data(), praq_states_dev, sizeof(LogicState) * praq.size(),
+                       hipMemcpyDeviceToHost, s0) != hipSuccess)
+        return kErrorDevice;
+    if (!pria.empty() &&
+        hipMemcpyAsync(pria_states.data(), pria_states_dev, sizeof(LogicState) * pria.size(),
+                       hipMemcpyDeviceToHost, s1) != hipSuccess)
+        return kErrorDevice;
+    if (hipStreamSynchronize(s0) != hipSuccess || hipStreamSynchronize(s1) != hipSuccess)
+        return kErrorDevice;
+    return 0;
+}
+
+// The PRAQ family on stream s0: the wide engine in the batch's precision,
+// then, for FP32, the stalled subset continued in FP64.
+int BatchImpl::wide_praq() {
+    const bool f32 = use_float32 != 0;
+    const int np = int(praq.size());
+    if (!np) return 0;
+    if (hipEventRecord(ev[0], s0) != hipSuccess) return kErrorDevice;
+    praq_stats = WideStats();
+    if (!f32) {
+        if (!wpraq64) {
+            wpraq64.reset(new WidePraq<double>());
+            if (wpraq64->setup(praq64.host, praq64.dev)) return kErrorDevice;
         }
-        if (!praq.empty() &&
-            hipMemcpyAsync(praq_states.data(), base + praq_states_off,
-                           sizeof(LogicState) * praq.size(), hipMemcpyDeviceToHost, s0) != hipSuccess)
+        if (wpraq64->solve(s0, false, time_limit, &praq_stats)) return kErrorDevice;
+        std::fill(praq_in64.begin(), praq_in64.end(), 1);
+    } else {
+        if (!wpraq32) {
+            wpraq32.reset(new WidePraq<float>());
+            if (wpraq32->setup(praq32.host, praq32.dev)) return kErrorDevice;
+        }
+        if (wpraq32->solve(s0, false, time_limit, &praq_stats)) return kErrorDevice;
+        std::fill(praq_in64.begin(), praq_in64.end(), 0);
+        if (hipMemcpyAsync(praq_states.data(), praq_states_dev, sizeof(LogicState) * np,
+                           hipMemcpyDeviceToHost, s0) != hipSuccess ||
+            hipStreamSynchronize(s0) != hipSuccess)
             return kErrorDevice;
-        if (!pria.empty() &&
-            hipMemcpyAsync(pria_states.data(), base + pria_states_off,
-                           sizeof(LogicState) * pria.size(), hipMemcpyDeviceToHost, s1) != hipSuccess)
+        std::vector<int> key;
+        std::vector<PraqJob<float>> p32;
+        std::vector<PraqJob<double>> p64;
+        for (int s = 0; s < np; ++s) {
+            if (!praq_states[s].stalled) continue;
+            key.push_back(s);
+            p32.push_back(praq32.host[s]);
+            p64.push_back(praq64.host[s]);
+            praq_in64[praq_order[s]] = 1;
+        }
+        if (!key.empty()) {
+            if (hipMemcpyAsync(praq32.sub, p32.data(), sizeof(p32[0]) * p32.size(),
+                               hipMemcpyHostToDevice, s0) != hipSuccess ||
+                hipMemcpyAsync(praq64.sub, p64.data(), sizeof(p64[0]) * p64.size(),
+                               hipMemcpyHostToDevice, s0) != hipSuccess ||
+                praq_convert_jobs(praq32.sub, praq64.sub, int(p32.size()), s0))
+                return kErrorDevice;
+            if (!wpraq64_sub || key != wpraq_sub_key) {
+                if (hipStreamSynchronize(s0) != hipSuccess) return kErrorDevice;
+                wpraq64_sub.reset(new WidePraq<double>());
+                if (wpraq64_sub->setup(p64, praq64.sub)) return kErrorDevice;
+                wpraq_sub_key = key;
+            }
+            WideStats more;
+            if (wpraq64_sub->solve(s0, true, time_limit, &more)) return kErrorDevice;
+            praq_stats.rounds += more.rounds;
+        }
+    }
+    if (hipEventRecord(ev[1], s0) != hipSuccess || hipStreamSynchronize(s0) != hipSuccess)
+        return kErrorDevice;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) praq_ms = ms;
+    return 0;
+}
+
+// The PRIA family on stream s1, as wide_praq.
+int BatchImpl::wide_pria() {
+    const bool f32 = use_float32 != 0;
+    const int nr = int(pria.size());
+    if (!nr) return 0;
+    if (hipEventRecord(ev[2], s1) != hipSuccess) return kErrorDevice;
+    pria_stats = WideStats();
+    if (!f32) {
+        if (!wpria64) {
+            wpria64.reset(new WidePria<double>());
+            if (wpria64->setup(pria64.host, pria64.dev)) return kErrorDevice;
+        }
+        if (wpria64->solve(s1, false, time_limit, &pria_stats)) return kErrorDevice;
+        std::fill(pria_in64.begin(), pria_in64.end(), 1);
+    } else {
+        if (!wpria32) {
+            wpria32.reset(new WidePria<float>());
+            if (wpria32->setup(pria32.host, pria32.dev)) return kErrorDevice;
+        }
+        if (wpria32->solve(s1, false, time_limit, &pria_stats)) return kErrorDevice;
+        std::fill(pria_in64.begin(), pria_in64.end(), 0);
+        if (hipMemcpyAsync(pria_states.data(), pria_states_dev, sizeof(LogicState) * nr,
+                           hipMemcpyDeviceToHost, s1) != hipSuccess ||
+            hipStreamSynchronize(s1) != hipSuccess)
+            return kErrorDevice;
+        std::vector<int> key;
+        std::vector<PriaJob<float>> r32;
+        std::vector<PriaJob<double>> r64;
+        for (int s = 0; s < nr; ++s) {
+            if (!pria_states[s].stalled) continue;
+            key.push_back(s);
+            r32.push_back(pria32.host[s]);
+            r64.push_back(pria64.host[s]);
+            pria_in64[pria_order[s]] = 1;
+        }
+        if (!key.empty()) {
+            if (hipMemcpyAsync(pria32.sub, r32.data(), sizeof(r32[0]) * r32.size(),
+                               hipMemcpyHostToDevice, s1) != hipSuccess ||
+                hipMemcpyAsync(pria64.sub, r64.data(), sizeof(r64[0]) * r64.size(),
+                               hipMemcpyHostToDevice, s1) != hipSuccess ||
+                pria_convert_jobs(pria32.sub, pria64.sub, int(r32.size()), s1))
+                return kErrorDevice;
+            if (!wpria64_sub || key != wpria_sub_key) {
+                if (hipStreamSynchronize(s1) != hipSuccess) return kErrorDevice;
+                wpria64_sub.reset(new WidePria<double>());
+                if (wpria64_sub->setup(r64, pria64.sub)) return kErrorDevice;
+                wpria_sub_key = key;
+            }
+            WideStats more;
+            if (wpria64_sub->solve(s1, true, time_limit, &more)) return kErrorDevice;
+            pria_stats.rounds += more.rounds;
+        }
+    }
+    if (hipEventRecord(ev[3], s1) != hipSuccess || hipStreamSynchronize(s1) != hipSuccess)
+        return kErrorDevice;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) pria_ms = ms;
+    return 0;
+}
+
+// Both families at once: PRIA driven from a second host thread.
+int BatchImpl::solve_wide() {
+    int pria_status = 0;
+    std::thread worker;
+    if (!pria.empty()) {
+        worker = std::thread([&] {
+            pria_status = hipSetDevice(device) == hipSuccess ? wide_pria() : kErrorDevice;
+        });
+    }
+    const int praq_status = wide_praq();
+    if (worker.joinable()) worker.join();
+    return praq_status ? praq_status : pria_status;
+}
+
+int BatchImpl::solve_workgroups() {
+    const bool f32 = use_float32 != 0;
+    const int np = int(praq.size()), nr = int(pria.size());
+    if (np) {
+        if (hipEventRecord(ev[0], s0) != hipSuccess) return kErrorDevice;
+        int e = f32 ? praq_launch_wg<float>(praq32.dev, np, praq_block, s0)
+                    : praq_launch_wg<double>(praq64.dev, np, praq_block, s0);
+        if (e) return kErrorDevice;
+    }
+    if (nr) {
+        if (hipEventRecord(ev[2], s1) != hipSuccess) return kErrorDevice;
+        int e = f32 ? pria_launch_wg<float>(pria32.dev, nr, pria_max_T, s1)
+                    : pria_launch_wg<double>(pria64.dev, nr, pria_max_T, s1);
+        if (e) return kErrorDevice;
+    }
+    std::fill(praq_in64.begin(), praq_in64.end(), f32 ? 0 : 1);
+    std::fill(pria_in64.begin(), pria_in64.end(), f32 ? 0 : 1);
+    if (f32) {
+        // the stalled instances: FP32 state -> FP64, then one more launch
+        if (np && hipMemcpyAsync(praq_states.data(), praq_states_dev, sizeof(LogicState) * np,
+                                 hipMemcpyDeviceToHost, s0) != hipSuccess)
+            return kErrorDevice;
+        if (nr && hipMemcpyAsync(pria_states.data(), pria_states_dev, sizeof(LogicState) * nr,
+                                 hipMemcpyDeviceToHost, s1) != hipSuccess)
             return kErrorDevice;
         if (hipStreamSynchronize(s0) != hipSuccess || hipStreamSynchronize(s1) != hipSuccess)
             return kErrorDevice;
-        if (mode == REFORM_BATCHED) {
-            float ms = 0.0f;
-            if (!praq.empty() && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) praq_ms = ms;
-            if (!pria.empty() && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) pria_ms = ms;
+        std::vector<PraqJob<float>> p32;
+        std::vector<PraqJob<double>> p64;
+        for (int s = 0; s < np; ++s) {
+            if (!praq_states[s].stalled) continue;
+            p32.push_back(praq32.host[s]);
+            p64.push_back(praq64.host[s]);
+            p64.back().resume = 1;
+            praq_in64[praq_order[s]] = 1;
+            praq_states[s].switch_iteration = praq_states[s].iteration;
+            if (hipMemcpyAsync(praq_states_dev + s, &praq_states[s], sizeof(LogicState),
+                               hipMemcpyHostToDevice, s0) != hipSuccess)
+                return kErrorDevice;
         }
-        return 0;
+        std::vector<PriaJob<float>> r32;
+        std::vector<PriaJob<double>> r64;
+        for (int s = 0; s < nr; ++s) {
+            if (!pria_states[s].stalled) continue;
+            r32.push_back(pria32.host[s]);
+            r64.push_back(pria64.host[s]);
+            r64.back().resume = 1;
+            pria_in64[pria_order[s]] = 1;
+            pria_states[s].switch_iteration = pria_states[s].iteration;
+            if (hipMemcpyAsync(pria_states_dev + s, &pria_states[s], sizeof(LogicState),
+                               hipMemcpyHostToDevice, s1) != hipSuccess)
+                return kErrorDevice;
+        }
+        if (!p32.empty()) {
+            if (hipMemcpyAsync(praq32.sub, p32.data(), sizeof(p32[0]) * p32.size(),
+                               hipMemcpyHostToDevice, s0) != hipSuccess ||
+                hipMemcpyAsync(praq64.sub, p64.data(), sizeof(p64[0]) * p64.size(),
+                               hipMemcpyHostToDevice, s0) != hipSuccess)
+                return kErrorDevice;
+            if (praq_convert_jobs(praq32.sub, praq64.sub, int(p32.size()), s0) ||
+                praq_launch_wg<double>(praq64.sub, int(p64.size()), praq_block, s0))
+                return kErrorDevice;
+        }
+        if (!r32.empty()) {
+            if (hipMemcpyAsync(pria32.sub, r32.data(), sizeof(r32[0]) * r32.size(),
+                               hipMemcpyHostToDevice, s1) != hipSuccess ||
+                hipMemcpyAsync(pria64.sub, r64.data(), sizeof(r64[0]) * r64.size(),
+                               hipMemcpyHostToDevice, s1) != hipSuccess)
+                return kErrorDevice;
+            if (pria_convert_jobs(pria32.sub, pria64.sub, int(r32.size()), s1) ||
+                pria_launch_wg<double>(pria64.sub, int(r64.size()), pria_max_T, s1))
+                return kErrorDevice;
+        }
     }
+    if (np && hipEventRecord(ev[1], s0) != hipSuccess) return kErrorDevice;
+    if (nr && hipEventRecord(ev[3], s1) != hipSuccess) return kErrorDevice;
+    if (hipStreamSynchronize(s0) != hipSuccess || hipStreamSynchronize(s1) != hipSuccess)
+        return kErrorDevice;
+    float ms = 0.0f;
+    if (np && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) praq_ms = ms;
+    if (nr && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) pria_ms = ms;
+    return 0;
+}
 
-    int praq_result(int64_t q, double* x, double* y, PraqInfo* info) override {
-        if (q < 0 || q >= int64_t(praq.size()) || !uploaded) return kErrorArgument;
-        const PraqEntry& e = praq[q];
+// The plain solver, one instance at a time: its tier, and for FP32 the
+// handover to FP64 if it stalls.
+int BatchImpl::solve_sequential() {
+    const bool f32 = use_float32 != 0;
+    for (size_t q = 0; q < praq.size(); ++q) {
         const int64_t s = praq_slot[q];
-        const PraqJob<Real>& j = praq_jobs[s];
-        const LogicState& st = praq_states[s];
-        if (x) {
-            std::vector<Real> hx(e.host.n);
-            if (hipMemcpy(hx.data(), j.a.x, sizeof(Real) * e.host.n, hipMemcpyDeviceToHost) !=
-                hipSuccess)
+        const int tier = praq[q].tier;
+        if (!f32) {
+            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, false)) return kErrorDevice;
+            praq_in64[q] = 1;
+            continue;
+        }
+        if (praq_plain<float>(praq32.host[s], praq32.sub, tier, false)) return kErrorDevice;
+        LogicState st;
+        if (hipMemcpy(&st, praq_states_dev + s, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
+            return kErrorDevice;
+        praq_in64[q] = 0;
+        if (st.stalled) {
+            st.switch_iteration = st.iteration;
+            if (hipMemcpy(praq_states_dev + s, &st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess)
                 return kErrorDevice;
-            for (int i = 0; i < e.host.n; ++i) x[i] = double(hx[i]);
+            if (praq_convert_state(praq32.host[s].a, praq64.host[s].a, s0)) return kErrorDevice;
+            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, true)) return kErrorDevice;
+            praq_in64[q] = 1;
         }
-        if (y && e.host.m > 0) {
-            std::vector<Real> hy(2 * size_t(e.host.m));
-            if (hipMemcpy(hy.data(), j.a.y, sizeof(Real) * hy.size(), hipMemcpyDeviceToHost) !=
-                hipSuccess)
-                return kErrorDevice;
-            for (int k = 0; k < e.host.m; ++k) {
-                y[2 * size_t(k)] = double(hy[k]) * e.host.scale;
-                y[2 * size_t(k) + 1] = double(hy[e.host.m + k]) * e.host.scale;
-            }
-        }
-        if (info) {
-            info->lower = st.lower;
-            info->upper = st.upper;
-            info->relative_gap = st.gap;
-            info->iterations = st.iteration;
-            info->restarts = st.restarts;
-            info->checks = st.checks;
-            info->omega = st.omega;
-            info->scale = e.host.scale;
-            info->status = st.status;
-            info->tier = e.tier;
-        }
-        return 0;
     }
-
-    int pria_result(int64_t q, double* x, double* y, double* z, pria_info* info) override {
-        if (q < 0 || q >= int64_t(pria.size()) || !uploaded) return kErrorArgument;
-        const PriaEntry& e = pria[q];
+    for (size_t q = 0; q < pria.size(); ++q) {
         const int64_t s = pria_slot[q];
-        const PriaJob<Real>& j = pria_jobs[s];
-        const LogicState& st = pria_states[s];
-        std::vector<Real> v(e.host.ND);
-        std::vector<double> xd(size_t(e.n) * e.t * e.t);
-        double* outs[3] = {xd.data(), y, z};
-        Real* srcs[3] = {j.a.x, j.a.y, j.a.z};
-        for (int w = 0; w < 3; ++w) {
-            if (!outs[w]) continue;
-            if (hipMemcpy(v.data(), srcs[w], sizeof(Real) * v.size(), hipMemcpyDeviceToHost) !=
-                hipSuccess)
+        const int tier = pria[q].tier, T = pria[q].host.T;
+        if (!f32) {
+            if (pria_plain<double>(pria64.host[s], pria64.sub, tier, false, T)) return kErrorDevice;
+            pria_in64[q] = 1;
+            continue;
+        }
+        if (pria_plain<float>(pria32.host[s], pria32.sub, tier, false, T)) return kErrorDevice;
+        LogicState st;
+        if (hipMemcpy(&st, pria_states_dev + s, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
+            return kErrorDevice;
+        pria_in64[q] = 0;
+        if (st.stalled) {
+            st.switch_iteration = st.iteration;
+            if (hipMemcpy(pria_states_dev + s, &st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess)
                 return kErrorDevice;
-            pria_unpack(e.host, v, w, outs[w]);
+            if (pria_convert_state(pria32.host[s].a, pria64.host[s].a, s1)) return kErrorDevice;
+            if (pria_plain<double>(pria64.host[s], pria64.sub, tier, true, T)) return kErrorDevice;
+            pria_in64[q] = 1;
         }
-        if (x) std::memcpy(x, xd.data(), sizeof(double) * xd.size());
-        if (info) {
-            info->iterations = st.iteration;
-            info->restarts = st.restarts;
-            info->checks = st.checks;
-            info->lower = st.lower;
-            info->upper_iterate = st.upper;
-            info->upper = pria_objective_host(e.n, e.t, e.k.data(), e.k0, e.h.data(), xd.data());
-            info->relative_gap = st.gap;
-            info->omega = st.omega;
-            info->scale = e.host.scale;
-            info->status = st.status;
-            info->tier = e.tier;
-        }
-        return 0;
     }
-};
+    return 0;
+}
+
+int BatchImpl::praq_result(int64_t q, double* x, double* y, PraqInfo* info) {
+    if (q < 0 || q >= int64_t(praq.size()) || !uploaded) return kErrorArgument;
+    const PraqEntry& e = praq[q];
+    const int64_t s = praq_slot[q];
+    const LogicState& st = praq_states[s];
+    const bool in64 = praq_in64[q] != 0;
+    std::vector<double> hx, hy;
+    if (x) {
+        int r = in64 ? read_array(hx, praq64.host[s].a.x, e.host.n)
+                     : read_array(hx, praq32.host[s].a.x, e.host.n);
+        if (r) return r;
+        std::memcpy(x, hx.data(), sizeof(double) * e.host.n);
+    }
+    if (y && e.host.m > 0) {
+        const size_t m2 = 2 * size_t(e.host.m);
+        int r = in64 ? read_array(hy, praq64.host[s].a.y, m2) : read_array(hy, praq32.host[s].a.y, m2);
+        if (r) return r;
+        for (int k = 0; k < e.host.m; ++k) {
+            y[2 * size_t(k)] = hy[k] * e.host.scale;
+            y[2 * size_t(k) + 1] = hy[e.host.m + k] * e.host.scale;
+        }
+    }
+    if (info) {
+        info->lower = st.lower;
+        info->upper = st.upper;
+        info->relative_gap = st.gap;
+        info->iterations = st.iteration;
+        info->restarts = st.restarts;
+        info->checks = st.checks;
+        info->omega = st.omega;
+        info->scale = e.host.scale;
+        info->status = st.status;
+        info->tier = e.tier;
+        info->switch_iteration = st.switch_iteration;
+    }
+    return 0;
+}
+
+int BatchImpl::pria_result(int64_t q, double* x, double* y, double* z, pria_info* info) {
+    if (q < 0 || q >= int64_t(pria.size()) || !uploaded) return kErrorArgument;
+    const PriaEntry& e = pria[q];
+    const int64_t s = pria_slot[q];
+    const LogicState& st = pria_states[s];
+    const bool in64 = pria_in64[q] != 0;
+    std::vector<double> xd(size_t(e.n) * e.t * e.t), v;
+    double* outs[3] = {xd.data(), y, z};
+    for (int w = 0; w < 3; ++w) {
+        if (!outs[w]) continue;
+        int r;
+        if (in64) {
+            const double* src[3] = {pria64.host[s].a.x, pria64.host[s].a.y, pria64.host[s].a.z};
+            r = read_array(v, src[w], e.host.ND);
+        } else {
+            const float* src[3] = {pria32.host[s].a.x, pria32.host[s].a.y, pria32.host[s].a.z};
+            r = read_array(v, src[w], e.host.ND);
+        }
+        if (r) return r;
+        pria_unpack(e.host, v, w, outs[w]);
+    }
+    if (x) std::memcpy(x, xd.data(), sizeof(double) * xd.size());
+    if (info) {
+        info->iterations = st.iteration;
+        info->restarts = st.restarts;
+        info->checks = st.checks;
+        info->lower = st.lower;
+        info->upper_iterate = st.upper;
+        info->upper = pria_objective_host(e.n, e.t, e.k.data(), e.k0, e.h.data(), xd.data());
+        info->relative_gap = st.gap;
+        info->omega = st.omega;
+        info->scale = e.host.scale;
+        info->status = st.status;
+        info->tier = e.tier;
+        info->switch_iteration = st.switch_iteration;
+    }
+    return 0;
+}
 
 }  // namespace
 
 struct reform_batch {
-    std::unique_ptr<BatchBase> impl;
+    std::unique_ptr<BatchImpl> impl;
 };
 
 extern "C" reform_batch* reform_batch_create(int use_float32) {
     auto* b = new reform_batch;
-    if (use_float32)
-        b->impl.reset(new Batch<float>());
-    else
-        b->impl.reset(new Batch<double>());
-    b->impl->use_float32 = use_float32;
+    b->impl.reset(new BatchImpl());
+    b->impl->use_float32 = use_float32 ? 1 : 0;
     return b;
 }
 
@@ -475,7 +834,6 @@ extern "C" int reform_batch_set_shape(reform_batch* batch, int praq_block, int p
     auto ok = [](int b) { return b == 0 || b == 64 || b == 128 || b == 256 || b == 512 || b == 1024; };
     if (!ok(praq_block) || !(pria_block == 0 || pria_block == 1024)) return kErrorArgument;
     if (praq_block) batch->impl->praq_block = praq_block;
-    if (pria_block) batch->impl->pria_block = pria_block;
     return 0;
 }
 
@@ -492,7 +850,14 @@ extern "C" int reform_batch_kernel_ms(const reform_batch* batch, double* praq_ms
 }
 
 extern "C" int64_t reform_batch_device_bytes(const reform_batch* batch) {
-    return batch ? int64_t(batch->impl->dev_bytes) : kErrorArgument;
+    if (!batch) return kErrorArgument;
+    const BatchImpl& b = *batch->impl;
+    size_t bytes = b.dev_bytes;
+    const WideControl* engines[] = {b.wpraq32.get(), b.wpraq64.get(), b.wpraq64_sub.get(),
+                                    b.wpria32.get(), b.wpria64.get(), b.wpria64_sub.get()};
+    for (const WideControl* w : engines)
+        if (w) bytes += w->bytes;
+    return int64_t(bytes);
 }
 
 extern "C" int reform_batch_praq_result(reform_batch* batch, int64_t index, double* x, double* y,

@@ -54,7 +54,9 @@ struct LogicParams {
     int32_t use_halpern;
     int32_t fallback_nonpositive;  // PRAQ: omega fallback also on <= 0
     int32_t verbose;               // device tier: one stderr line per check
-    int32_t pad;
+    int32_t stall_checks;          // > 0: stop as stalled when the best gap has
+                                   // not halved within this many checks (used
+                                   // by FP32 iterates to hand over to FP64)
 };
 
 struct LogicState {
@@ -63,8 +65,11 @@ struct LogicState {
     int64_t restart_iter, iteration, restarts, checks;
     int32_t action;
     int32_t anchor_pending;      // copy (x, y) to the anchor before the next block
-    int32_t status;              // 0 converged, 1 limit, 2 time limit
-    int32_t pad;
+    int32_t status;              // 0 converged, 1 limit, 2 time limit, 3 stalled
+    int32_t stalled;             // set by the stall test; the next action is kept
+    double best_gap;             // smallest gap by halvings, for the stall test
+    int64_t best_check;          // check count when best_gap was last halved
+    int64_t switch_iteration;    // iteration at which FP32 handed over (-1: never)
 };
 
 // One line per check of the device tier's host loop, when verbose.
@@ -105,7 +110,10 @@ __host__ __device__ inline void logic_init(LogicState& st, const LogicParams& p)
     st.action = kActWarmBlock;
     st.anchor_pending = 0;
     st.status = 1;
-    st.pad = 0;
+    st.stalled = 0;
+    st.best_gap = INFINITY;
+    st.best_check = 0;
+    st.switch_iteration = -1;
 }
 
 __host__ __device__ inline void finish(LogicState& st, const LogicParams& p, int32_t action) {
@@ -165,6 +173,15 @@ __host__ __device__ inline void logic_after_halpern_check(LogicState& st, const 
     if (terminated) {
         finish(st, p, kActDoneSX);
         return;
+    }
+    if (p.stall_checks > 0) {
+        if (st.gap < 0.5 * st.best_gap) {
+            st.best_gap = st.gap;
+            st.best_check = st.checks;
+        } else if (st.checks - st.best_check >= p.stall_checks) {
+            st.stalled = 1;
+            st.status = 3;
+        }
     }
     if (restart) {
         double pn = sqrt(s.pdrift), dn = sqrt(s.ddrift);

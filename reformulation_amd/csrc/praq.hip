@@ -73,26 +73,35 @@ __global__ void __launch_bounds__(1024) praq_wg_solve(const PraqJob<Real>* jobs)
     const int lanes = job.lanes;
     const long long t_start = wall_clock64();
 
-    // state from the projected start
-    praq_wg_copy(a.x0, a.xs, a.n, a.y0, a.ys, 2 * a.m);
-    praq_wg_copy(a.x, a.xs, a.n, a.y, a.ys, 2 * a.m);
-    praq_wg_copy(a.sx, a.xs, a.n, a.sy, a.ys, 2 * a.m);
-    if (threadIdx.x == 0) logic_init(st, p);
-    __syncthreads();
-
-    // the start: one uncounted vanilla step, check at (x, y)
-    praq_wg_step(a, praq_step_scalars<Real>(0, p.omega, p.eta, false, true), lanes);
     double sums[kPraqSums];
-    praq_wg_sums(a, lanes, a.x, a.y, red, sums);
-    if (threadIdx.x == 0) logic_after_warm_check(st, p, praq_check_sums(sums, job.scale));
-    __syncthreads();
+    if (job.resume) {
+        if (threadIdx.x == 0) {
+            st = *job.state;
+            st.stalled = 0;
+        }
+        __syncthreads();
+    } else {
+        // state from the projected start
+        praq_wg_copy(a.x0, a.xs, a.n, a.y0, a.ys, 2 * a.m);
+        praq_wg_copy(a.x, a.xs, a.n, a.y, a.ys, 2 * a.m);
+        praq_wg_copy(a.sx, a.xs, a.n, a.sy, a.ys, 2 * a.m);
+        if (threadIdx.x == 0) logic_init(st, p);
+        __syncthreads();
+        // the start: one uncounted vanilla step, check at (x, y)
+        praq_wg_step(a, praq_step_scalars<Real>(0, p.omega, p.eta, false, true), lanes);
+        praq_wg_sums(a, lanes, a.x, a.y, red, sums);
+        if (threadIdx.x == 0) logic_after_warm_check(st, p, praq_check_sums(sums, job.scale));
+        __syncthreads();
+    }
 
     for (;;) {
         const int action = st.action;
         const int anchor_pending = st.anchor_pending;
+        const int stalled = st.stalled;
         const int64_t r0 = st.restart_iter;
         const double omega = st.omega;
         __syncthreads();
+        if (stalled) break;  // the state is handed to a higher precision
         if (action == kActDoneX) break;
         if (action == kActDoneSX) {
             praq_wg_copy(a.x, a.sx, a.n, a.y, a.sy, 2 * a.m);
@@ -240,7 +249,7 @@ static int device_sums(const PraqArrays<Real>& a, int lanes, const Real* bx, con
 // The host loop of the device tier: the same state machine as praq_wg_solve.
 template <class Real>
 int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
-                      double time_limit_seconds, hipStream_t stream) {
+                      double time_limit_seconds, hipStream_t stream, bool resume) {
     const PraqArrays<Real>& a = job.a;
     const LogicParams& p = job.p;
     const int lanes = job.lanes;
@@ -250,19 +259,24 @@ int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto t0 = now();
 
-    if (device_copy(a.x0, a.xs, a.n, a.y0, a.ys, 2 * a.m, stream)) return kErrorDevice;
-    if (device_copy(a.x, a.xs, a.n, a.y, a.ys, 2 * a.m, stream)) return kErrorDevice;
-    if (device_copy(a.sx, a.xs, a.n, a.sy, a.ys, 2 * a.m, stream)) return kErrorDevice;
-    logic_init(st, p);
-    if (device_step(a, praq_step_scalars<Real>(0, p.omega, p.eta, false, true), lanes, stream))
-        return kErrorDevice;
-    if (device_sums(a, lanes, a.x, a.y, partial, dsums, sums, stream)) return kErrorDevice;
-    logic_after_warm_check(st, p, praq_check_sums(sums, job.scale));
+    if (resume) {
+        st.stalled = 0;
+    } else {
+        if (device_copy(a.x0, a.xs, a.n, a.y0, a.ys, 2 * a.m, stream)) return kErrorDevice;
+        if (device_copy(a.x, a.xs, a.n, a.y, a.ys, 2 * a.m, stream)) return kErrorDevice;
+        if (device_copy(a.sx, a.xs, a.n, a.sy, a.ys, 2 * a.m, stream)) return kErrorDevice;
+        logic_init(st, p);
+        if (device_step(a, praq_step_scalars<Real>(0, p.omega, p.eta, false, true), lanes, stream))
+            return kErrorDevice;
+        if (device_sums(a, lanes, a.x, a.y, partial, dsums, sums, stream)) return kErrorDevice;
+        logic_after_warm_check(st, p, praq_check_sums(sums, job.scale));
+    }
 
     for (;;) {
         const int action = st.action;
         const int64_t r0 = st.restart_iter;
         const double omega = st.omega;
+        if (st.stalled) break;
         if (action == kActDoneX) break;
         if (action == kActDoneSX) {
             if (device_copy(a.x, a.sx, a.n, a.y, a.sy, 2 * a.m, stream)) return kErrorDevice;
@@ -312,9 +326,56 @@ int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
 }
 
 template int praq_device_solve<float>(const PraqJob<float>&, LogicState&, double*, double,
-                                      hipStream_t);
+                                      hipStream_t, bool);
 template int praq_device_solve<double>(const PraqJob<double>&, LogicState&, double*, double,
-                                       hipStream_t);
+                                       hipStream_t, bool);
+
+// ---------------------------------------------------------------------------
+// precision switch: the whole Halpern state, FP32 -> FP64
+// ---------------------------------------------------------------------------
+
+__device__ inline void praq_convert_one(const PraqArrays<float>& f, const PraqArrays<double>& d,
+                                        int k) {
+    if (k < f.n) {
+        d.x[k] = f.x[k];
+        d.sx[k] = f.sx[k];
+        d.x0[k] = f.x0[k];
+    }
+    if (k < 2 * f.m) {
+        d.yc[k] = f.yc[k];
+        d.y[k] = f.y[k];
+        d.sy[k] = f.sy[k];
+        d.y0[k] = f.y0[k];
+    }
+}
+
+__global__ void praq_convert_kernel(PraqArrays<float> f, PraqArrays<double> d) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    praq_convert_one(f, d, k);
+}
+
+// one workgroup per job, grid-stride inside the instance
+__global__ void praq_convert_jobs_kernel(const PraqJob<float>* from, const PraqJob<double>* to) {
+    const PraqArrays<float> f = from[blockIdx.x].a;
+    const PraqArrays<double> d = to[blockIdx.x].a;
+    const int count = f.n > 2 * f.m ? f.n : 2 * f.m;
+    for (int k = threadIdx.x; k < count; k += blockDim.x) praq_convert_one(f, d, k);
+}
+
+int praq_convert_state(const PraqArrays<float>& from, const PraqArrays<double>& to,
+                       hipStream_t stream) {
+    int count = std::max(from.n, 2 * from.m);
+    hipLaunchKernelGGL(praq_convert_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, from,
+                       to);
+    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+}
+
+int praq_convert_jobs(const PraqJob<float>* from, const PraqJob<double>* to, int count,
+                      hipStream_t stream) {
+    if (count <= 0) return 0;
+    hipLaunchKernelGGL(praq_convert_jobs_kernel, dim3(count), dim3(256), 0, stream, from, to);
+    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+}
 
 size_t praq_device_work_bytes() { return sizeof(double) * (kSumBlocks + 1) * kPraqSums; }
 

@@ -40,73 +40,119 @@ struct DeviceBuffer {
     int alloc(size_t bytes) { return hipMalloc(&ptr, bytes ? bytes : 1) == hipSuccess ? 0 : -1; }
 };
 
+// Checks without a halving of the best gap after which FP32 iterates hand the
+// whole Halpern state over to FP64.
+constexpr int kStallChecks = 24;
+
+int pick_tier(const PraqHostInstance& s, int tier) {
+    if (tier != PRAQ_TIER_AUTO) return tier;
+    return int64_t(s.n) + int64_t(s.m) <= wg_tier_max() ? PRAQ_TIER_WORKGROUP : PRAQ_TIER_DEVICE;
+}
+
+// One instance's device image in one precision, and the tier that runs it.
 template <class Real>
-int solve_typed(const PraqHostInstance& s, const LogicParams& lp, const PraqOptions& o, double* x,
-                double* y, PraqInfo* info) {
-    PraqLayout L = praq_layout(s.n, s.m, sizeof(Real));
-    const size_t job_off = align256(L.total_bytes);
-    const size_t state_off = align256(job_off + sizeof(PraqJob<Real>));
-    const size_t work_off = align256(state_off + sizeof(LogicState));
-    const size_t bytes = work_off + praq_device_work_bytes();
-
+struct PraqImage {
     DeviceBuffer buf;
-    if (buf.alloc(bytes)) return kErrorDevice;
-    char* base = static_cast<char*>(buf.ptr);
-    std::vector<char> image(L.input_bytes, 0);
-    praq_fill_image<Real>(s, L, image.data());
+    PraqLayout L;
+    PraqJob<Real> job;
+    size_t job_off = 0, state_off = 0, work_off = 0;
 
+    int init(const PraqHostInstance& s, const LogicParams& lp, hipStream_t stream) {
+        L = praq_layout(s.n, s.m, sizeof(Real));
+        job_off = align256(L.total_bytes);
+        state_off = align256(job_off + sizeof(PraqJob<Real>));
+        work_off = align256(state_off + sizeof(LogicState));
+        if (buf.alloc(work_off + praq_device_work_bytes())) return kErrorDevice;
+        char* base = static_cast<char*>(buf.ptr);
+        std::vector<char> image(L.input_bytes, 0);
+        praq_fill_image<Real>(s, L, image.data());
+        REFORM_HIP_CHECK(hipMemcpyAsync(base, image.data(), L.input_bytes, hipMemcpyHostToDevice,
+                                        stream));
+        REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+        std::memset(&job, 0, sizeof(job));
+        job.a = praq_bind<Real>(L, base, s.n, s.m);
+        job.p = lp;
+        job.scale = s.scale;
+        job.lanes = s.lanes;
+        job.state = reinterpret_cast<LogicState*>(base + state_off);
+        return 0;
+    }
+
+    int run(int tier, LogicState& st, double time_limit, bool resume, hipStream_t stream) {
+        char* base = static_cast<char*>(buf.ptr);
+        if (tier == PRAQ_TIER_WORKGROUP) {
+            double limit = time_limit > 0.0 ? std::min(time_limit, kDefaultWgLimitSeconds)
+                                            : kDefaultWgLimitSeconds;
+            job.time_limit_ticks = (long long)(limit * wall_clock_rate_hz());
+            job.resume = resume ? 1 : 0;
+            if (resume)
+                REFORM_HIP_CHECK(hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice,
+                                                stream));
+            REFORM_HIP_CHECK(hipMemcpyAsync(base + job_off, &job, sizeof(job),
+                                            hipMemcpyHostToDevice, stream));
+            if (praq_launch_wg<Real>(reinterpret_cast<PraqJob<Real>*>(base + job_off), 1, 1024,
+                                     stream))
+                return kErrorDevice;
+            REFORM_HIP_CHECK(hipMemcpyAsync(&st, job.state, sizeof(st), hipMemcpyDeviceToHost,
+                                            stream));
+            REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+            return 0;
+        }
+        return praq_device_solve<Real>(job, st, reinterpret_cast<double*>(base + work_off),
+                                       time_limit, stream, resume);
+    }
+
+    int outputs(const PraqHostInstance& s, double* x, double* y, hipStream_t stream) {
+        std::vector<Real> hx(s.n), hy(2 * size_t(s.m));
+        REFORM_HIP_CHECK(hipMemcpyAsync(hx.data(), job.a.x, sizeof(Real) * s.n,
+                                        hipMemcpyDeviceToHost, stream));
+        if (y && s.m > 0)
+            REFORM_HIP_CHECK(hipMemcpyAsync(hy.data(), job.a.y, sizeof(Real) * 2 * size_t(s.m),
+                                            hipMemcpyDeviceToHost, stream));
+        REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+        for (int i = 0; i < s.n; ++i) x[i] = double(hx[i]);
+        if (y)
+            for (int e = 0; e < s.m; ++e) {
+                y[2 * size_t(e)] = double(hy[e]) * s.scale;
+                y[2 * size_t(e) + 1] = double(hy[s.m + e]) * s.scale;
+            }
+        return 0;
+    }
+};
+
+// FP64 iterates, or FP32 iterates handing over to FP64 when they stall.
+int solve_mixed(const PraqHostInstance& s, LogicParams lp, const PraqOptions& o, bool fp32,
+                double* x, double* y, PraqInfo* info) {
     hipStream_t stream;
     REFORM_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     struct StreamGuard {
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } guard{stream};
-    REFORM_HIP_CHECK(hipMemcpyAsync(base, image.data(), L.input_bytes, hipMemcpyHostToDevice,
-                                    stream));
-
-    PraqJob<Real> job;
-    std::memset(&job, 0, sizeof(job));
-    job.a = praq_bind<Real>(L, base, s.n, s.m);
-    job.p = lp;
-    job.scale = s.scale;
-    job.lanes = s.lanes;
-    job.state = reinterpret_cast<LogicState*>(base + state_off);
-
-    int tier = o.tier;
-    if (tier == PRAQ_TIER_AUTO)
-        tier = int64_t(s.n) + int64_t(s.m) <= wg_tier_max() ? PRAQ_TIER_WORKGROUP : PRAQ_TIER_DEVICE;
-
+    const int tier = pick_tier(s, o.tier);
     LogicState st;
-    if (tier == PRAQ_TIER_WORKGROUP) {
-        double limit = o.time_limit > 0.0 ? std::min(o.time_limit, kDefaultWgLimitSeconds)
-                                          : kDefaultWgLimitSeconds;
-        job.time_limit_ticks = (long long)(limit * wall_clock_rate_hz());
-        REFORM_HIP_CHECK(hipMemcpyAsync(base + job_off, &job, sizeof(job), hipMemcpyHostToDevice,
-                                        stream));
-        if (praq_launch_wg<Real>(reinterpret_cast<PraqJob<Real>*>(base + job_off), 1, 1024,
-                                 stream))
+    std::memset(&st, 0, sizeof(st));
+    if (!fp32) {
+        PraqImage<double> d;
+        if (d.init(s, lp, stream) || d.run(tier, st, o.time_limit, false, stream) ||
+            d.outputs(s, x, y, stream))
             return kErrorDevice;
-        REFORM_HIP_CHECK(hipMemcpyAsync(&st, job.state, sizeof(st), hipMemcpyDeviceToHost, stream));
-        REFORM_HIP_CHECK(hipStreamSynchronize(stream));
     } else {
-        if (praq_device_solve<Real>(job, st, reinterpret_cast<double*>(base + work_off),
-                                    o.time_limit, stream))
+        LogicParams lp32 = lp;
+        lp32.stall_checks = kStallChecks;
+        PraqImage<float> f;
+        if (f.init(s, lp32, stream) || f.run(tier, st, o.time_limit, false, stream))
             return kErrorDevice;
-    }
-
-    std::vector<Real> hx(s.n), hy(2 * size_t(s.m));
-    REFORM_HIP_CHECK(hipMemcpyAsync(hx.data(), job.a.x, sizeof(Real) * s.n, hipMemcpyDeviceToHost,
-                                    stream));
-    if (y && s.m > 0)
-        REFORM_HIP_CHECK(hipMemcpyAsync(hy.data(), job.a.y, sizeof(Real) * 2 * size_t(s.m),
-                                        hipMemcpyDeviceToHost, stream));
-    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
-    for (int i = 0; i < s.n; ++i) x[i] = double(hx[i]);
-    if (y)
-        for (int e = 0; e < s.m; ++e) {
-            y[2 * size_t(e)] = double(hy[e]) * s.scale;
-            y[2 * size_t(e) + 1] = double(hy[s.m + e]) * s.scale;
+        if (st.stalled) {
+            PraqImage<double> d;
+            st.switch_iteration = st.iteration;
+            if (d.init(s, lp, stream) || praq_convert_state(f.job.a, d.job.a, stream) ||
+                d.run(tier, st, o.time_limit, true, stream) || d.outputs(s, x, y, stream))
+                return kErrorDevice;
+        } else if (f.outputs(s, x, y, stream)) {
+            return kErrorDevice;
         }
+    }
     if (info) {
         info->lower = st.lower;
         info->upper = st.upper;
@@ -118,6 +164,7 @@ int solve_typed(const PraqHostInstance& s, const LogicParams& lp, const PraqOpti
         info->scale = s.scale;
         info->status = st.status;
         info->tier = tier;
+        info->switch_iteration = st.switch_iteration;
     }
     return 0;
 }
@@ -200,8 +247,7 @@ extern "C" int praq_solve_ex(const PraqInstance* instance, const PraqParameters*
 
     PraqInfo local;
     PraqInfo* out = info ? info : &local;
-    status = p.use_float32 ? solve_typed<float>(s, lp, o, x, y, out)
-                           : solve_typed<double>(s, lp, o, x, y, out);
+    status = solve_mixed(s, lp, o, p.use_float32 != 0, x, y, out);
     if (status < 0) {
         out->status = status;
         return status;

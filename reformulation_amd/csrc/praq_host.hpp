@@ -31,7 +31,7 @@ struct PraqJob {
     LogicParams p;
     double scale;
     int lanes;
-    int pad;
+    int resume;                  // continue from *state (after a precision switch)
     long long time_limit_ticks;  // wall-clock ticks; <= 0 none
     LogicState* state;           // device address the final state goes to
 };
@@ -235,14 +235,19 @@ inline LogicParams praq_logic_params(const PraqParameters& p, const PraqOptions&
     q.warm_max = q.use_halpern ? q.op_steps * (o.warm_steps > 0 ? o.warm_steps : 0) : q.max_iter;
     q.fallback_nonpositive = 1;
     q.verbose = 0;
-    q.pad = 0;
+    q.stall_checks = 0;
     return q;
 }
 
 // Implemented in praq.hip.
 template <class Real>
 int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
-                      double time_limit_seconds, hipStream_t stream);
+                      double time_limit_seconds, hipStream_t stream, bool resume = false);
+// FP32 state -> FP64 state of the same instance (precision switch)
+int praq_convert_state(const PraqArrays<float>& from, const PraqArrays<double>& to,
+                       hipStream_t stream);
+int praq_convert_jobs(const PraqJob<float>* from, const PraqJob<double>* to, int count,
+                      hipStream_t stream);
 template <class Real>
 int praq_launch_wg(const PraqJob<Real>* jobs, int count, int block, hipStream_t stream);
 size_t praq_device_work_bytes();

@@ -18,18 +18,6 @@
 
 namespace reform {
 
-// Tile and register shapes.  FP32: 64 columns per x tile; FP64: 32, so the
-// tile stays under 80 KB of LDS at T' = 300.  VR: row values per lane
-// (T' <= 64 VR); VZ: slice values per thread held in registers.
-template <class Real> struct PriaShape;
-template <> struct PriaShape<float> { static constexpr int TW = 64, VR = 10, VZ = 32; };
-template <> struct PriaShape<double> { static constexpr int TW = 32, VR = 10, VZ = 32; };
-
-template <class Real>
-__host__ __device__ inline size_t pria_x_lds_bytes(int T) {
-    return sizeof(Real) * (size_t(T) * (PriaShape<Real>::TW + 1) + PriaShape<Real>::TW);
-}
-
 // ---------------------------------------------------------------------------
 // workgroup tier
 // ---------------------------------------------------------------------------
@@ -82,24 +70,33 @@ __global__ void __launch_bounds__(1024) pria_wg_solve(const PriaJob<Real>* jobs)
     const LogicParams p = job.p;
     const long long t_start = wall_clock64();
 
-    pria_wg_copy3(a.x0, a.xs, a.y0, a.ys, a.z0, a.zs, a.ND);
-    pria_wg_copy3(a.x, a.xs, a.y, a.ys, a.z, a.zs, a.ND);
-    pria_wg_copy3(a.sx, a.xs, a.sy, a.ys, a.sz, a.zs, a.ND);
-    if (threadIdx.x == 0) logic_init(st, p);
-    __syncthreads();
-
-    pria_wg_step(a, pria_step_scalars<Real>(0, p.omega, p.eta, false, true), lds, red);
     double sums[kPriaSums];
-    pria_wg_sums(a, a.x, a.y, a.z, red, sums);
-    if (threadIdx.x == 0) logic_after_warm_check(st, p, pria_check_sums(sums, job.scale));
-    __syncthreads();
+    if (job.resume) {
+        if (threadIdx.x == 0) {
+            st = *job.state;
+            st.stalled = 0;
+        }
+        __syncthreads();
+    } else {
+        pria_wg_copy3(a.x0, a.xs, a.y0, a.ys, a.z0, a.zs, a.ND);
+        pria_wg_copy3(a.x, a.xs, a.y, a.ys, a.z, a.zs, a.ND);
+        pria_wg_copy3(a.sx, a.xs, a.sy, a.ys, a.sz, a.zs, a.ND);
+        if (threadIdx.x == 0) logic_init(st, p);
+        __syncthreads();
+        pria_wg_step(a, pria_step_scalars<Real>(0, p.omega, p.eta, false, true), lds, red);
+        pria_wg_sums(a, a.x, a.y, a.z, red, sums);
+        if (threadIdx.x == 0) logic_after_warm_check(st, p, pria_check_sums(sums, job.scale));
+        __syncthreads();
+    }
 
     for (;;) {
         const int action = st.action;
         const int anchor_pending = st.anchor_pending;
+        const int stalled = st.stalled;
         const int64_t r0 = st.restart_iter;
         const double omega = st.omega;
         __syncthreads();
+        if (stalled) break;  // the state is handed to a higher precision
         if (action == kActDoneX) break;
         if (action == kActDoneSX) {
             pria_wg_copy3(a.x, a.sx, a.y, a.sy, a.z, a.sz, a.ND);
@@ -293,7 +290,7 @@ struct PriaDevice {
 
 template <class Real>
 int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
-                      double time_limit_seconds, hipStream_t stream) {
+                      double time_limit_seconds, hipStream_t stream, bool resume) {
     const PriaArrays<Real>& a = job.a;
     const LogicParams& p = job.p;
     constexpr int TW = PriaShape<Real>::TW;
@@ -309,18 +306,23 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
     double sums[kPriaSums];
     auto t0 = std::chrono::steady_clock::now();
 
-    if (d.copy3(a.x0, a.xs, a.y0, a.ys, a.z0, a.zs) || d.copy3(a.x, a.xs, a.y, a.ys, a.z, a.zs) ||
-        d.copy3(a.sx, a.xs, a.sy, a.ys, a.sz, a.zs))
-        return kPriaErrorDevice;
-    logic_init(st, p);
-    if (d.step(pria_step_scalars<Real>(0, p.omega, p.eta, false, true))) return kPriaErrorDevice;
-    if (d.sums(a.x, a.y, a.z, sums)) return kPriaErrorDevice;
-    logic_after_warm_check(st, p, pria_check_sums(sums, job.scale));
+    if (resume) {
+        st.stalled = 0;
+    } else {
+        if (d.copy3(a.x0, a.xs, a.y0, a.ys, a.z0, a.zs) ||
+            d.copy3(a.x, a.xs, a.y, a.ys, a.z, a.zs) || d.copy3(a.sx, a.xs, a.sy, a.ys, a.sz, a.zs))
+            return kPriaErrorDevice;
+        logic_init(st, p);
+        if (d.step(pria_step_scalars<Real>(0, p.omega, p.eta, false, true))) return kPriaErrorDevice;
+        if (d.sums(a.x, a.y, a.z, sums)) return kPriaErrorDevice;
+        logic_after_warm_check(st, p, pria_check_sums(sums, job.scale));
+    }
 
     for (;;) {
         const int action = st.action;
         const int64_t r0 = st.restart_iter;
         const double omega = st.omega;
+        if (st.stalled) break;
         if (action == kActDoneX) break;
         if (action == kActDoneSX) {
             if (d.copy3(a.x, a.sx, a.y, a.sy, a.z, a.sz)) return kPriaErrorDevice;
@@ -367,7 +369,7 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
 }
 
 template int pria_device_solve<float>(const PriaJob<float>&, LogicState&, double*, double,
-                                      hipStream_t);
+                                      hipStream_t, bool);
 
 // One vanilla step from (sx, sy, sz), in place, on the device tier (tier 2)
 // or by one workgroup (tier 1); a test hook for the step kernels.
@@ -402,7 +404,49 @@ int pria_one_step(const PriaArrays<Real>& a, double omega, double eta, int tier,
 template int pria_one_step<float>(const PriaArrays<float>&, double, double, int, hipStream_t);
 template int pria_one_step<double>(const PriaArrays<double>&, double, double, int, hipStream_t);
 template int pria_device_solve<double>(const PriaJob<double>&, LogicState&, double*, double,
-                                       hipStream_t);
+                                       hipStream_t, bool);
+
+// precision switch: the whole Halpern state, FP32 -> FP64
+__device__ inline void pria_convert_one(const PriaArrays<float>& f, const PriaArrays<double>& d,
+                                        int e) {
+    d.x0[e] = f.x0[e];
+    d.y0[e] = f.y0[e];
+    d.z0[e] = f.z0[e];
+    d.xc[e] = f.xc[e];
+    d.yc[e] = f.yc[e];
+    d.zc[e] = f.zc[e];
+    d.x[e] = f.x[e];
+    d.sx[e] = f.sx[e];
+    d.y[e] = f.y[e];
+    d.sy[e] = f.sy[e];
+    d.z[e] = f.z[e];
+    d.sz[e] = f.sz[e];
+}
+
+__global__ void pria_convert_kernel(PriaArrays<float> f, PriaArrays<double> d) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < f.ND) pria_convert_one(f, d, e);
+}
+
+__global__ void pria_convert_jobs_kernel(const PriaJob<float>* from, const PriaJob<double>* to) {
+    const PriaArrays<float> f = from[blockIdx.x].a;
+    const PriaArrays<double> d = to[blockIdx.x].a;
+    for (int e = threadIdx.x; e < f.ND; e += blockDim.x) pria_convert_one(f, d, e);
+}
+
+int pria_convert_state(const PriaArrays<float>& from, const PriaArrays<double>& to,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(pria_convert_kernel, dim3((from.ND + 255) / 256), dim3(256), 0, stream, from,
+                       to);
+    return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+}
+
+int pria_convert_jobs(const PriaJob<float>* from, const PriaJob<double>* to, int count,
+                      hipStream_t stream) {
+    if (count <= 0) return 0;
+    hipLaunchKernelGGL(pria_convert_jobs_kernel, dim3(count), dim3(1024), 0, stream, from, to);
+    return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+}
 
 template <class Real>
 int pria_launch_wg(const PriaJob<Real>* jobs, int count, int max_T, hipStream_t stream) {

@@ -49,67 +49,119 @@ bool params_valid(const pria_params* p) {
     return p && p->omega > 0.0 && p->eta > 0.0 && p->tolerance >= 0.0;
 }
 
+// Checks without a halving of the best gap after which FP32 iterates hand the
+// whole Halpern state over to FP64.
+constexpr int kStallChecks = 24;
+
+int pick_tier(const PriaHostInstance& s, int tier) {
+    if (tier != PRIA_TIER_AUTO) return tier;
+    return s.ND <= pria_wg_max() ? PRIA_TIER_WORKGROUP : PRIA_TIER_DEVICE;
+}
+
+// One instance's device image in one precision, and the tier that runs it.
 template <class Real>
-int solve_typed(const PriaHostInstance& s, const LogicParams& lp, const pria_options& o,
-                std::vector<Real>* xo, std::vector<Real>* yo, std::vector<Real>* zo,
-                pria_info* info) {
-    PriaLayout L = pria_layout(s.N, s.ND, sizeof(Real));
-    const size_t job_off = pria_align256(L.total_bytes);
-    const size_t state_off = pria_align256(job_off + sizeof(PriaJob<Real>));
-    const size_t work_off = pria_align256(state_off + sizeof(LogicState));
-    const size_t bytes = work_off + pria_device_work_bytes(s.N, s.T);
+struct PriaImage {
     Buffer buf;
-    if (hipMalloc(&buf.ptr, bytes) != hipSuccess) return kPriaErrorDevice;
-    char* base = static_cast<char*>(buf.ptr);
-    const size_t up = pria_upload_bytes(s, L);
-    std::vector<char> image(up, 0);
-    pria_fill_image<Real>(s, L, image.data());
+    PriaLayout L;
+    PriaJob<Real> job;
+    size_t job_off = 0, state_off = 0, work_off = 0;
+
+    int init(const PriaHostInstance& s, const LogicParams& lp, hipStream_t stream) {
+        L = pria_layout(s.N, s.ND, sizeof(Real));
+        job_off = pria_align256(L.total_bytes);
+        state_off = pria_align256(job_off + sizeof(PriaJob<Real>));
+        work_off = pria_align256(state_off + sizeof(LogicState));
+        if (hipMalloc(&buf.ptr, work_off + pria_device_work_bytes(s.N, s.T)) != hipSuccess) {
+            buf.ptr = nullptr;
+            return kPriaErrorDevice;
+        }
+        char* base = static_cast<char*>(buf.ptr);
+        const size_t up = pria_upload_bytes(s, L);
+        std::vector<char> image(up, 0);
+        pria_fill_image<Real>(s, L, image.data());
+        if (hipMemcpyAsync(base, image.data(), up, hipMemcpyHostToDevice, stream) != hipSuccess)
+            return kPriaErrorDevice;
+        std::memset(&job, 0, sizeof(job));
+        job.a = pria_bind<Real>(L, base, s);
+        if (s.cold && pria_cold_start<Real>(job.a, stream)) return kPriaErrorDevice;
+        job.p = lp;
+        job.scale = s.scale;
+        job.state = reinterpret_cast<LogicState*>(base + state_off);
+        return hipStreamSynchronize(stream) == hipSuccess ? 0 : kPriaErrorDevice;
+    }
+
+    int run(int tier, LogicState& st, double time_limit, bool resume, int T, hipStream_t stream) {
+        char* base = static_cast<char*>(buf.ptr);
+        if (tier == PRIA_TIER_WORKGROUP) {
+            double limit = time_limit > 0.0 ? std::min(time_limit, kDefaultWgLimitSeconds)
+                                            : kDefaultWgLimitSeconds;
+            job.time_limit_ticks = (long long)(limit * wall_clock_rate_hz());
+            job.resume = resume ? 1 : 0;
+            if (resume && hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice,
+                                         stream) != hipSuccess)
+                return kPriaErrorDevice;
+            if (hipMemcpyAsync(base + job_off, &job, sizeof(job), hipMemcpyHostToDevice, stream) !=
+                hipSuccess)
+                return kPriaErrorDevice;
+            if (pria_launch_wg<Real>(reinterpret_cast<PriaJob<Real>*>(base + job_off), 1, T,
+                                     stream))
+                return kPriaErrorDevice;
+            if (hipMemcpyAsync(&st, job.state, sizeof(st), hipMemcpyDeviceToHost, stream) !=
+                hipSuccess)
+                return kPriaErrorDevice;
+            return hipStreamSynchronize(stream) == hipSuccess ? 0 : kPriaErrorDevice;
+        }
+        return pria_device_solve<Real>(job, st, reinterpret_cast<double*>(base + work_off),
+                                       time_limit, stream, resume);
+    }
+
+    // slice-order x, y, z of the final point, widened to FP64
+    int outputs(const PriaHostInstance& s, std::vector<double>* out[3], hipStream_t stream) {
+        Real* srcs[3] = {job.a.x, job.a.y, job.a.z};
+        std::vector<Real> v(s.ND);
+        for (int q = 0; q < 3; ++q) {
+            if (!out[q]) continue;
+            if (hipMemcpyAsync(v.data(), srcs[q], sizeof(Real) * s.ND, hipMemcpyDeviceToHost,
+                               stream) != hipSuccess ||
+                hipStreamSynchronize(stream) != hipSuccess)
+                return kPriaErrorDevice;
+            out[q]->assign(v.begin(), v.end());
+        }
+        return 0;
+    }
+};
+
+// FP64 iterates, or FP32 iterates handing over to FP64 when they stall.
+// Outputs in slice order (FP64).
+int solve_mixed(const PriaHostInstance& s, const LogicParams& lp, const pria_options& o,
+                bool fp32, std::vector<double>* out[3], pria_info* info) {
     Stream stream;
     if (hipStreamCreateWithFlags(&stream.s, hipStreamNonBlocking) != hipSuccess)
         return kPriaErrorDevice;
-    if (hipMemcpyAsync(base, image.data(), up, hipMemcpyHostToDevice, stream.s) != hipSuccess)
-        return kPriaErrorDevice;
-
-    PriaJob<Real> job;
-    std::memset(&job, 0, sizeof(job));
-    job.a = pria_bind<Real>(L, base, s);
-    if (s.cold && pria_cold_start<Real>(job.a, stream.s)) return kPriaErrorDevice;
-    job.p = lp;
-    job.scale = s.scale;
-    job.state = reinterpret_cast<LogicState*>(base + state_off);
-
-    int tier = o.tier;
-    if (tier == PRIA_TIER_AUTO) tier = s.ND <= pria_wg_max() ? PRIA_TIER_WORKGROUP : PRIA_TIER_DEVICE;
+    const int tier = pick_tier(s, o.tier);
     LogicState st;
-    if (tier == PRIA_TIER_WORKGROUP) {
-        double limit = o.time_limit > 0.0 ? std::min(o.time_limit, kDefaultWgLimitSeconds)
-                                          : kDefaultWgLimitSeconds;
-        job.time_limit_ticks = (long long)(limit * wall_clock_rate_hz());
-        if (hipMemcpyAsync(base + job_off, &job, sizeof(job), hipMemcpyHostToDevice, stream.s) !=
-            hipSuccess)
+    std::memset(&st, 0, sizeof(st));
+    if (!fp32) {
+        PriaImage<double> d;
+        if (d.init(s, lp, stream.s) || d.run(tier, st, o.time_limit, false, s.T, stream.s) ||
+            d.outputs(s, out, stream.s))
             return kPriaErrorDevice;
-        if (pria_launch_wg<Real>(reinterpret_cast<PriaJob<Real>*>(base + job_off), 1, s.T,
-                                 stream.s))
-            return kPriaErrorDevice;
-        if (hipMemcpyAsync(&st, job.state, sizeof(st), hipMemcpyDeviceToHost, stream.s) !=
-            hipSuccess)
-            return kPriaErrorDevice;
-        if (hipStreamSynchronize(stream.s) != hipSuccess) return kPriaErrorDevice;
     } else {
-        if (pria_device_solve<Real>(job, st, reinterpret_cast<double*>(base + work_off),
-                                    o.time_limit, stream.s))
+        LogicParams lp32 = lp;
+        lp32.stall_checks = kStallChecks;
+        PriaImage<float> f;
+        if (f.init(s, lp32, stream.s) || f.run(tier, st, o.time_limit, false, s.T, stream.s))
             return kPriaErrorDevice;
-    }
-    std::vector<Real>* outs[3] = {xo, yo, zo};
-    Real* srcs[3] = {job.a.x, job.a.y, job.a.z};
-    for (int q = 0; q < 3; ++q) {
-        if (!outs[q]) continue;
-        outs[q]->resize(s.ND);
-        if (hipMemcpyAsync(outs[q]->data(), srcs[q], sizeof(Real) * s.ND, hipMemcpyDeviceToHost,
-                           stream.s) != hipSuccess)
+        if (st.stalled) {
+            PriaImage<double> d;
+            st.switch_iteration = st.iteration;
+            if (d.init(s, lp, stream.s) || pria_convert_state(f.job.a, d.job.a, stream.s) ||
+                d.run(tier, st, o.time_limit, true, s.T, stream.s) || d.outputs(s, out, stream.s))
+                return kPriaErrorDevice;
+        } else if (f.outputs(s, out, stream.s)) {
             return kPriaErrorDevice;
+        }
     }
-    if (hipStreamSynchronize(stream.s) != hipSuccess) return kPriaErrorDevice;
     if (info) {
         info->iterations = st.iteration;
         info->restarts = st.restarts;
@@ -121,6 +173,7 @@ int solve_typed(const PriaHostInstance& s, const LogicParams& lp, const pria_opt
         info->scale = s.scale;
         info->status = st.status;
         info->tier = tier;
+        info->switch_iteration = st.switch_iteration;
     }
     return 0;
 }
@@ -143,23 +196,13 @@ int solve_f64io(int64_t n, int64_t t, const double* k, double k0, const double* 
     pria_info local;
     pria_info* out = info ? info : &local;
     std::vector<double> xd(size_t(n) * t * t);
-    int status;
-    if (use_float32) {
-        std::vector<float> vx, vy, vz;
-        status = solve_typed<float>(s, lp, o, &vx, y ? &vy : nullptr, z ? &vz : nullptr, out);
-        if (status == 0) {
-            pria_unpack(s, vx, 0, xd.data());
-            if (y) pria_unpack(s, vy, 1, y);
-            if (z) pria_unpack(s, vz, 2, z);
-        }
-    } else {
-        std::vector<double> vx, vy, vz;
-        status = solve_typed<double>(s, lp, o, &vx, y ? &vy : nullptr, z ? &vz : nullptr, out);
-        if (status == 0) {
-            pria_unpack(s, vx, 0, xd.data());
-            if (y) pria_unpack(s, vy, 1, y);
-            if (z) pria_unpack(s, vz, 2, z);
-        }
+    std::vector<double> vx, vy, vz;
+    std::vector<double>* outs[3] = {&vx, y ? &vy : nullptr, z ? &vz : nullptr};
+    int status = solve_mixed(s, lp, o, use_float32 != 0, outs, out);
+    if (status == 0) {
+        pria_unpack(s, vx, 0, xd.data());
+        if (y) pria_unpack(s, vy, 1, y);
+        if (z) pria_unpack(s, vz, 2, z);
     }
     if (status) {
         out->status = status;

@@ -345,6 +345,18 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
     __syncthreads();
 }
 
+// Tile and register shapes.  FP32: 64 columns per x tile; FP64: 32, so the
+// tile stays under 80 KB of LDS at T' = 300.  VR: row values per lane
+// (T' <= 64 VR); VZ: slice values per thread held in registers.
+template <class Real> struct PriaShape;
+template <> struct PriaShape<float> { static constexpr int TW = 64, VR = 10, VZ = 32; };
+template <> struct PriaShape<double> { static constexpr int TW = 32, VR = 10, VZ = 32; };
+
+template <class Real>
+__host__ __device__ inline size_t pria_x_lds_bytes(int T) {
+    return sizeof(Real) * (size_t(T) * (PriaShape<Real>::TW + 1) + PriaShape<Real>::TW);
+}
+
 // --- check sums ------------------------------------------------------------
 
 enum PriaSum {

@@ -30,6 +30,8 @@ struct PriaJob {
     PriaArrays<Real> a;
     LogicParams p;
     double scale;
+    int resume;                  // continue from *state (after a precision switch)
+    int pad;
     long long time_limit_ticks;
     LogicState* state;
 };
@@ -227,7 +229,7 @@ inline LogicParams pria_logic_params(const pria_params& p) {
     q.warm_max = q.use_halpern ? q.op_steps * warm : q.max_iter;
     q.fallback_nonpositive = 0;
     q.verbose = p.verbose;
-    q.pad = 0;
+    q.stall_checks = 0;
     return q;
 }
 
@@ -273,7 +275,12 @@ inline double pria_objective_host(int64_t n, int64_t t, const double* k, double 
 // Implemented in pria.hip.
 template <class Real>
 int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
-                      double time_limit_seconds, hipStream_t stream);
+                      double time_limit_seconds, hipStream_t stream, bool resume = false);
+// FP32 state -> FP64 state of the same instance (precision switch)
+int pria_convert_state(const PriaArrays<float>& from, const PriaArrays<double>& to,
+                       hipStream_t stream);
+int pria_convert_jobs(const PriaJob<float>* from, const PriaJob<double>* to, int count,
+                      hipStream_t stream);
 template <class Real>
 int pria_launch_wg(const PriaJob<Real>* jobs, int count, int max_T, hipStream_t stream);
 template <class Real>

@@ -1,0 +1,719 @@
+// The batched device tier (wide.hpp): block tables over all instances of a
+// batch, per-instance round control, one round captured in a hipGraph.
+#include "wide.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+namespace reform {
+
+namespace {
+
+constexpr int kWideBlock = 256;
+constexpr int kRoundsPerPoll = 2;
+
+int grid_for(int threads) { return (threads + kWideBlock - 1) / kWideBlock; }
+
+__device__ __forceinline__ bool wide_runs_steps(int act) {
+    return act == kActWarmBlock || act == kActHalpernBlock || act == kActInit;
+}
+
+__device__ __forceinline__ bool wide_restarting(const LogicState* st) {
+    return st->action == kActRestart && !st->stalled;
+}
+
+// The step this block's instance takes in launch j of this round, or false.
+// mode 0: the round's steps -- init and warm rounds vanilla, Halpern rounds
+// reflected, each instance for its own check interval; mode 1: the one
+// vanilla step of an instance restarting.
+template <class Step, class Job, class MakeStep>
+__device__ __forceinline__ bool wide_step_for(const Job& job, const int* round_action, int inst,
+                                              int mode, int j, MakeStep make, Step* out) {
+    const LogicState* st = job.state;
+    if (mode == 0) {
+        const int act = round_action[inst];
+        if (!wide_runs_steps(act) || j >= job.p.op_steps) return false;
+        const bool reflect = act == kActHalpernBlock;
+        *out = make(st->restart_iter + j, st->omega, job.p.eta, reflect,
+                    !reflect || j + 1 == job.p.op_steps);
+        return true;
+    }
+    if (!wide_restarting(st)) return false;
+    *out = make(0, st->omega, job.p.eta, false, true);
+    return true;
+}
+
+// The point a sums launch bounds: 1 = (x, y), 2 = the step point (sx, sy),
+// 0 = the instance takes no check in this launch.
+template <class Job>
+__device__ __forceinline__ int wide_sums_point(const Job& job, const int* round_action, int inst,
+                                               int mode) {
+    if (mode == 0) {
+        const int act = round_action[inst];
+        if (act == kActHalpernBlock) return 2;
+        return wide_runs_steps(act) ? 1 : 0;
+    }
+    return wide_restarting(job.state) ? 2 : 0;
+}
+
+// Whether a copy launch moves this instance's data: mode 0 pending anchors,
+// 1 restart anchors, 2 finished step points, 3 the start.
+template <class Job>
+__device__ __forceinline__ bool wide_copies(const Job& job, const int* pending, int inst,
+                                            int mode) {
+    if (mode == 0) return pending[inst] != 0;
+    if (mode == 1) return wide_restarting(job.state);
+    if (mode == 2) return job.state->action == kActDoneSX;
+    return true;
+}
+
+// The action a logic launch applies to this instance, or -1.
+template <class Job>
+__device__ __forceinline__ int wide_logic_action(const Job& job, const int* round_action, int inst,
+                                                 int mode) {
+    if (mode == 0) {
+        const int act = round_action[inst];
+        return wide_runs_steps(act) ? act : -1;
+    }
+    return wide_restarting(job.state) ? int(kActRestart) : -1;
+}
+
+// The scalar logic after a check, on one thread; the same sequence as the
+// device tier's host loop.
+template <class Job>
+__device__ void wide_apply(const Job& job, int act, const CheckSums& cs, int* last_action,
+                           int inst) {
+    LogicState st = *job.state;
+    if (act == kActInit) {
+        logic_after_warm_check(st, job.p, cs);
+    } else if (act == kActWarmBlock) {
+        st.restart_iter += job.p.op_steps;
+        st.iteration += job.p.op_steps;
+        logic_after_warm_check(st, job.p, cs);
+    } else if (act == kActHalpernBlock) {
+        st.restart_iter += job.p.op_steps;
+        st.iteration += job.p.op_steps;
+        logic_after_halpern_check(st, job.p, cs);
+    } else {
+        logic_after_restart(st, job.p, cs);
+    }
+    *job.state = st;
+    last_action[inst] = act;
+}
+
+// One thread per instance: the round's action and pending anchor copies.
+template <class Job>
+__global__ void wide_pre(const Job* jobs, int count, int* round_action, int* pending) {
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= count) return;
+    LogicState* st = jobs[inst].state;
+    int act = st->action;
+    if (st->stalled || (act != kActWarmBlock && act != kActHalpernBlock)) act = kActSkip;
+    round_action[inst] = act;
+    const int pend = act == kActHalpernBlock && st->anchor_pending;
+    pending[inst] = pend;
+    if (pend) st->anchor_pending = 0;
+}
+
+// One thread per instance: a cold start's state, or the continuation of a
+// state handed over from the other precision.
+template <class Job>
+__global__ void wide_init(const Job* jobs, int count, int* round_action, int* last_action,
+                          int resume) {
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= count) return;
+    LogicState* st = jobs[inst].state;
+    if (resume) {
+        st->stalled = 0;
+        st->switch_iteration = st->iteration;
+        round_action[inst] = kActSkip;
+        last_action[inst] = kActHalpernBlock;  // a stall is detected after a Halpern check
+    } else {
+        LogicState s;
+        logic_init(s, jobs[inst].p);
+        *st = s;
+        round_action[inst] = kActInit;
+        last_action[inst] = kActInit;
+    }
+}
+
+// One thread per instance: out of time.  The answer is the last step's input
+// after a warm block or a restart, else the step point; as the device tier.
+template <class Job>
+__global__ void wide_expire(const Job* jobs, int count, const int* last_action) {
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= count) return;
+    LogicState* st = jobs[inst].state;
+    if (st->action == kActDoneX || st->action == kActDoneSX || st->stalled) return;
+    st->action = last_action[inst] == kActHalpernBlock ? kActDoneSX : kActDoneX;
+    st->status = 2;
+}
+
+// One block: finished step points have been copied (DONE_SX -> DONE_X);
+// count the instances still active.
+template <class Job>
+__global__ void wide_count(const Job* jobs, int count, int* active) {
+    __shared__ double red[16];
+    double n = 0.0;
+    for (int inst = threadIdx.x; inst < count; inst += blockDim.x) {
+        LogicState* st = jobs[inst].state;
+        if (st->action == kActDoneSX) st->action = kActDoneX;
+        if (st->action != kActDoneX && !st->stalled) n += 1.0;
+    }
+    n = block_sum1(n, red);
+    if (threadIdx.x == 0) *active = int(n);
+}
+
+// ---------------------------------------------------------------- PRAQ ----
+
+template <class Real>
+__global__ void __launch_bounds__(256) wpraq_node(const PraqJob<Real>* jobs, const int2* tab,
+                                                  const int* round_action, int mode, int j) {
+    const int2 t = tab[blockIdx.x];
+    const PraqJob<Real>& job = jobs[t.x];
+    PraqStep<Real> sc;
+    if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
+    const int lanes = job.lanes;
+    const int gid = t.y * kWideBlock + threadIdx.x;
+    praq_node(job.a, sc, gid / lanes, gid % lanes, lanes);
+}
+
+template <class Real>
+__global__ void __launch_bounds__(256) wpraq_edge(const PraqJob<Real>* jobs, const int2* tab,
+                                                  const int* round_action, int mode, int j) {
+    const int2 t = tab[blockIdx.x];
+    const PraqJob<Real>& job = jobs[t.x];
+    PraqStep<Real> sc;
+    if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
+    const int e = t.y * kWideBlock + threadIdx.x;
+    if (e < job.a.m) praq_edge(job.a, sc, e);
+}
+
+// node blocks, then edge blocks; one partial row per block
+template <class Real>
+__global__ void __launch_bounds__(256) wpraq_sums(const PraqJob<Real>* jobs, const int2* node_tab,
+                                                  int node_blocks, const int2* edge_tab,
+                                                  const int* round_action, int mode,
+                                                  double* partial) {
+    __shared__ double red[kPraqSums * 16];
+    const bool node = int(blockIdx.x) < node_blocks;
+    const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
+    const PraqJob<Real>& job = jobs[t.x];
+    const int where = wide_sums_point(job, round_action, t.x, mode);
+    if (!where) return;
+    const PraqArrays<Real>& a = job.a;
+    const Real* bx = where == 2 ? a.sx : a.x;
+    const Real* by = where == 2 ? a.sy : a.y;
+    double acc[kPraqSums];
+#pragma unroll
+    for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
+    const int gid = t.y * kWideBlock + threadIdx.x;
+    if (node)
+        praq_node_sums(a, gid / job.lanes, gid % job.lanes, job.lanes, bx, by, acc);
+    else if (gid < a.m)
+        praq_edge_sums(a, gid, bx, by, true, acc);
+    block_sums<kPraqSums>(acc, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kPraqSums; ++k) partial[size_t(blockIdx.x) * kPraqSums + k] = acc[k];
+}
+
+// one block per instance: its rows in a fixed order, then the logic
+template <class Real>
+__global__ void __launch_bounds__(256) wpraq_logic(const PraqJob<Real>* jobs, const int* first,
+                                                   int node_blocks, const double* partial,
+                                                   const int* round_action, int* last_action,
+                                                   int mode) {
+    __shared__ double red[kPraqSums * 16];
+    const int inst = blockIdx.x;
+    const PraqJob<Real>& job = jobs[inst];
+    const int act = wide_logic_action(job, round_action, inst, mode);
+    if (act < 0) return;
+    const int nf = first[4 * inst], nc = first[4 * inst + 1];
+    const int ef = first[4 * inst + 2], ec = first[4 * inst + 3];
+    double acc[kPraqSums];
+#pragma unroll
+    for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
+    for (int r = threadIdx.x; r < nc; r += blockDim.x)
+        for (int k = 0; k < kPraqSums; ++k) acc[k] += partial[size_t(nf + r) * kPraqSums + k];
+    for (int r = threadIdx.x; r < ec; r += blockDim.x)
+        for (int k = 0; k < kPraqSums; ++k)
+            acc[k] += partial[size_t(node_blocks + ef + r) * kPraqSums + k];
+    block_sums<kPraqSums>(acc, red);
+    if (threadIdx.x == 0) wide_apply(job, act, praq_check_sums(acc, job.scale), last_action, inst);
+}
+
+template <class Real>
+__global__ void __launch_bounds__(256) wpraq_copy(const PraqJob<Real>* jobs, const int2* node_tab,
+                                                  int node_blocks, const int2* edge_tab,
+                                                  const int* pending, int mode) {
+    const bool node = int(blockIdx.x) < node_blocks;
+    const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
+    const PraqJob<Real>& job = jobs[t.x];
+    if (!wide_copies(job, pending, t.x, mode)) return;
+    const PraqArrays<Real>& a = job.a;
+    const int gid = t.y * kWideBlock + threadIdx.x;
+    if (node) {
+        if (gid % job.lanes) return;
+        const int i = gid / job.lanes;
+        if (i >= a.n) return;
+        if (mode <= 1) a.x0[i] = a.x[i];
+        else if (mode == 2) a.x[i] = a.sx[i];
+        else a.x0[i] = a.x[i] = a.sx[i] = a.xs[i];
+    } else {
+        if (gid >= a.m) return;
+        for (int slot = 0; slot < 2; ++slot) {
+            const int k = slot * a.m + gid;
+            if (mode <= 1) a.y0[k] = a.y[k];
+            else if (mode == 2) a.y[k] = a.sy[k];
+            else a.y0[k] = a.y[k] = a.sy[k] = a.ys[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- PRIA ----
+
+template <class Real>
+__global__ void __launch_bounds__(256) wpria_x(const PriaJob<Real>* jobs, const int4* tab,
+                                               const int* round_action, int mode, int j) {
+    extern __shared__ char dyn[];
+    const int4 t = tab[blockIdx.x];
+    const PriaJob<Real>& job = jobs[t.x];
+    PriaStep<Real> sc;
+    if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+    pria_x_tile<Real, PriaShape<Real>::TW>(job.a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
+}
+
+template <class Real>
+__global__ void __launch_bounds__(1024) wpria_yz(const PriaJob<Real>* jobs, const int2* tab,
+                                                 const int* round_action, int mode, int j) {
+    __shared__ double red[2 * 16];
+    const int2 t = tab[blockIdx.x];
+    const PriaJob<Real>& job = jobs[t.x];
+    PriaStep<Real> sc;
+    if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+    pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(job.a, sc, t.y, red);
+}
+
+// slice blocks (six sums each), then tile blocks (a column-minimum sum each,
+// stored at the tile's (item, tile) row)
+template <class Real>
+__global__ void __launch_bounds__(256) wpria_sums(const PriaJob<Real>* jobs, const int2* s_tab,
+                                                  int s_blocks, const int4* x_tab,
+                                                  const int* round_action, int mode,
+                                                  double* partial) {
+    __shared__ double red[kPriaSums * 16];
+    const bool slice = int(blockIdx.x) < s_blocks;
+    const int inst = slice ? s_tab[blockIdx.x].x : x_tab[blockIdx.x - s_blocks].x;
+    const PriaJob<Real>& job = jobs[inst];
+    const int where = wide_sums_point(job, round_action, inst, mode);
+    if (!where) return;
+    const PriaArrays<Real>& a = job.a;
+    double acc[kPriaSums];
+    for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
+    if (slice) {
+        pria_slice_sums(a, s_tab[blockIdx.x].y, where == 2 ? a.sx : a.x, red, acc);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < kPLower; ++q) partial[size_t(blockIdx.x) * kPriaSums + q] = acc[q];
+    } else {
+        const int4 t = x_tab[blockIdx.x - s_blocks];
+        pria_column_sums<Real, PriaShape<Real>::TW>(a, t.y, t.z, where == 2 ? a.sy : a.y,
+                                                    where == 2 ? a.sz : a.z, red, acc);
+        if (threadIdx.x == 0) partial[size_t(s_blocks + t.w) * kPriaSums] = acc[kPLower];
+    }
+}
+
+template <class Real>
+__global__ void __launch_bounds__(256) wpria_logic(const PriaJob<Real>* jobs, const int* first,
+                                                   int s_blocks, const double* partial,
+                                                   const int* round_action, int* last_action,
+                                                   int mode) {
+    __shared__ double red[kPriaSums * 16];
+    const int inst = blockIdx.x;
+    const PriaJob<Real>& job = jobs[inst];
+    const int act = wide_logic_action(job, round_action, inst, mode);
+    if (act < 0) return;
+    const int sf = first[4 * inst], sc = first[4 * inst + 1];
+    const int tf = first[4 * inst + 2], tc = first[4 * inst + 3];
+    double acc[kPriaSums];
+    for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
+    for (int r = threadIdx.x; r < sc; r += blockDim.x)
+        for (int q = 0; q < kPLower; ++q) acc[q] += partial[size_t(sf + r) * kPriaSums + q];
+    for (int r = threadIdx.x; r < tc; r += blockDim.x)
+        acc[kPLower] += partial[size_t(s_blocks + tf + r) * kPriaSums];
+    block_sums<kPriaSums>(acc, red);
+    if (threadIdx.x == 0) wide_apply(job, act, pria_check_sums(acc, job.scale), last_action, inst);
+}
+
+// one block per slice; copy modes as wide_copies
+template <class Real>
+__global__ void __launch_bounds__(256) wpria_copy(const PriaJob<Real>* jobs, const int2* s_tab,
+                                                  const int* pending, int mode) {
+    const int2 t = s_tab[blockIdx.x];
+    const PriaJob<Real>& job = jobs[t.x];
+    if (!wide_copies(job, pending, t.x, mode)) return;
+    const PriaArrays<Real>& a = job.a;
+    const int s = t.y, base = pria_off(a.N, a.T, s), L = a.N * (a.T - s);
+    for (int e = threadIdx.x; e < L; e += blockDim.x) {
+        const int g = base + e;
+        if (mode <= 1) {
+            a.x0[g] = a.x[g];
+            a.y0[g] = a.y[g];
+            a.z0[g] = a.z[g];
+        } else if (mode == 2) {
+            a.x[g] = a.sx[g];
+            a.y[g] = a.sy[g];
+            a.z[g] = a.sz[g];
+        } else {
+            a.x0[g] = a.x[g] = a.sx[g] = a.xs[g];
+            a.y0[g] = a.y[g] = a.sy[g] = a.ys[g];
+            a.z0[g] = a.z[g] = a.sz[g] = a.zs[g];
+        }
+    }
+}
+
+// Start, then graph-replayed rounds until no instance is active or the time
+// is up.  Two pinned slots carry the active count back so the next group of
+// rounds is queued while the previous group's count is read.
+template <class Wide>
+int wide_drive(Wide& w, hipStream_t stream, bool resume, double time_limit, WideStats* stats) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&] {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    if (w.count == 0) return 0;
+    if (w.start(stream, resume)) return kErrorDevice;
+    if (!w.exec) {
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return kErrorDevice;
+        const int st = w.enqueue_round(stream);
+        if (hipStreamEndCapture(stream, &w.graph) != hipSuccess || st) return kErrorDevice;
+        if (hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0) != hipSuccess)
+            return kErrorDevice;
+    }
+    auto group = [&](int slot) -> int {
+        for (int r = 0; r < kRoundsPerPoll; ++r)
+            if (hipGraphLaunch(w.exec, stream) != hipSuccess) return kErrorDevice;
+        if (hipMemcpyAsync(w.host_active + slot, w.active, sizeof(int), hipMemcpyDeviceToHost,
+                           stream) != hipSuccess ||
+            hipEventRecord(w.polled[slot], stream) != hipSuccess)
+            return kErrorDevice;
+        return 0;
+    };
+    if (hipMemcpyAsync(w.host_active, w.active, sizeof(int), hipMemcpyDeviceToHost, stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return kErrorDevice;
+    int rounds = 0;
+    bool expired = false;
+    if (w.host_active[0] != 0) {
+        if (group(0)) return kErrorDevice;
+        rounds += kRoundsPerPoll;
+        for (int g = 1;; ++g) {
+            if (time_limit > 0.0 && elapsed() > time_limit) {
+                expired = true;
+                break;
+            }
+            if (group(g & 1)) return kErrorDevice;
+            rounds += kRoundsPerPoll;
+            if (hipEventSynchronize(w.polled[(g - 1) & 1]) != hipSuccess) return kErrorDevice;
+            if (w.host_active[(g - 1) & 1] == 0) break;
+        }
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess) return kErrorDevice;
+    if (expired && (w.expire(stream) || hipStreamSynchronize(stream) != hipSuccess))
+        return kErrorDevice;
+    if (stats) {
+        stats->rounds = rounds;
+        stats->seconds = elapsed();
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- control --
+
+WideControl::~WideControl() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& e : polled)
+        if (e) (void)hipEventDestroy(e);
+    if (host_active) (void)hipHostFree(host_active);
+    if (arena) (void)hipFree(arena);
+}
+
+int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_width) {
+    size_t at = align256(table_bytes);
+    const size_t o_first = at;
+    at = align256(at + sizeof(int) * 4 * size_t(count));
+    const size_t o_partial = at;
+    at = align256(at + sizeof(double) * partial_width * (partial_rows + 1));
+    const size_t o_ctrl = at;
+    at = align256(at + sizeof(int) * (3 * size_t(count) + 1));
+    if (hipMalloc(&arena, at) != hipSuccess) {
+        arena = nullptr;
+        return kErrorDevice;
+    }
+    bytes = at;
+    char* b = static_cast<char*>(arena);
+    first = reinterpret_cast<int*>(b + o_first);
+    partial = reinterpret_cast<double*>(b + o_partial);
+    round_action = reinterpret_cast<int*>(b + o_ctrl);
+    last_action = round_action + count;
+    pending = last_action + count;
+    active = pending + count;
+    if (hipHostMalloc(reinterpret_cast<void**>(&host_active), 2 * sizeof(int),
+                      hipHostMallocDefault) != hipSuccess) {
+        host_active = nullptr;
+        return kErrorDevice;
+    }
+    for (auto& e : polled)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return kErrorDevice;
+    return 0;
+}
+
+// ------------------------------------------------------------- PRAQ host --
+
+template <class Real>
+int WidePraq<Real>::setup(const std::vector<PraqJob<Real>>& host_jobs, PraqJob<Real>* dev_jobs) {
+    count = int(host_jobs.size());
+    jobs = dev_jobs;
+    if (!count) return 0;
+    std::vector<int2> nt, et;
+    std::vector<int> fi(4 * size_t(count));
+    op_steps = 2;
+    for (int q = 0; q < count; ++q) {
+        const auto& a = host_jobs[q].a;
+        op_steps = std::max(op_steps, int(host_jobs[q].p.op_steps));
+        const int nb = int((int64_t(a.n) * host_jobs[q].lanes + kWideBlock - 1) / kWideBlock);
+        const int eb = (a.m + kWideBlock - 1) / kWideBlock;
+        fi[4 * q] = int(nt.size());
+        fi[4 * q + 1] = nb;
+        fi[4 * q + 2] = int(et.size());
+        fi[4 * q + 3] = eb;
+        for (int b = 0; b < nb; ++b) nt.push_back(make_int2(q, b));
+        for (int b = 0; b < eb; ++b) et.push_back(make_int2(q, b));
+    }
+    node_blocks = int(nt.size());
+    edge_blocks = int(et.size());
+    const size_t tab_bytes = sizeof(int2) * (nt.size() + et.size());
+    if (allocate(tab_bytes, nt.size() + et.size(), kPraqSums)) return kErrorDevice;
+    node_tab = static_cast<int2*>(arena);
+    edge_tab = node_tab + nt.size();
+    std::vector<int2> tabs(nt);
+    tabs.insert(tabs.end(), et.begin(), et.end());
+    if (hipMemcpy(node_tab, tabs.data(), tab_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return kErrorDevice;
+    return 0;
+}
+
+template <class Real>
+int WidePraq<Real>::start(hipStream_t stream, bool resume) {
+    const int all = node_blocks + edge_blocks;
+    const dim3 blk(kWideBlock);
+    hipLaunchKernelGGL(wide_init<PraqJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
+                       count, round_action, last_action, resume ? 1 : 0);
+    if (!resume) {
+        hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab,
+                           node_blocks, edge_tab, pending, 3);
+        if (node_blocks)
+            hipLaunchKernelGGL(wpraq_node<Real>, dim3(node_blocks), blk, 0, stream, jobs, node_tab,
+                               round_action, 0, 0);
+        if (edge_blocks)
+            hipLaunchKernelGGL(wpraq_edge<Real>, dim3(edge_blocks), blk, 0, stream, jobs, edge_tab,
+                               round_action, 0, 0);
+        hipLaunchKernelGGL(wpraq_sums<Real>, dim3(all), blk, 0, stream, jobs, node_tab,
+                           node_blocks, edge_tab, round_action, 0, partial);
+        hipLaunchKernelGGL(wpraq_logic<Real>, dim3(count), blk, 0, stream, jobs, first,
+                           node_blocks, partial, round_action, last_action, 0);
+        hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab,
+                           node_blocks, edge_tab, pending, 2);
+    }
+    hipLaunchKernelGGL(wide_count<PraqJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+}
+
+template <class Real>
+int WidePraq<Real>::enqueue_round(hipStream_t stream) {
+    const int all = node_blocks + edge_blocks;
+    const dim3 blk(kWideBlock);
+    hipLaunchKernelGGL(wide_pre<PraqJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs, count,
+                       round_action, pending);
+    hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
+                       edge_tab, pending, 0);
+    for (int j = 0; j < op_steps; ++j) {
+        if (node_blocks)
+            hipLaunchKernelGGL(wpraq_node<Real>, dim3(node_blocks), blk, 0, stream, jobs, node_tab,
+                               round_action, 0, j);
+        if (edge_blocks)
+            hipLaunchKernelGGL(wpraq_edge<Real>, dim3(edge_blocks), blk, 0, stream, jobs, edge_tab,
+                               round_action, 0, j);
+    }
+    hipLaunchKernelGGL(wpraq_sums<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
+                       edge_tab, round_action, 0, partial);
+    hipLaunchKernelGGL(wpraq_logic<Real>, dim3(count), blk, 0, stream, jobs, first, node_blocks,
+                       partial, round_action, last_action, 0);
+    // restarts: one uncounted vanilla step, the new anchor, the norms
+    if (node_blocks)
+        hipLaunchKernelGGL(wpraq_node<Real>, dim3(node_blocks), blk, 0, stream, jobs, node_tab,
+                           round_action, 1, 0);
+    if (edge_blocks)
+        hipLaunchKernelGGL(wpraq_edge<Real>, dim3(edge_blocks), blk, 0, stream, jobs, edge_tab,
+                           round_action, 1, 0);
+    hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
+                       edge_tab, pending, 1);
+    hipLaunchKernelGGL(wpraq_sums<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
+                       edge_tab, round_action, 1, partial);
+    hipLaunchKernelGGL(wpraq_logic<Real>, dim3(count), blk, 0, stream, jobs, first, node_blocks,
+                       partial, round_action, last_action, 1);
+    hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
+                       edge_tab, pending, 2);
+    hipLaunchKernelGGL(wide_count<PraqJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+}
+
+template <class Real>
+int WidePraq<Real>::expire(hipStream_t stream) {
+    const dim3 blk(kWideBlock);
+    hipLaunchKernelGGL(wide_expire<PraqJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
+                       count, last_action);
+    hipLaunchKernelGGL(wpraq_copy<Real>, dim3(node_blocks + edge_blocks), blk, 0, stream, jobs,
+                       node_tab, node_blocks, edge_tab, pending, 2);
+    hipLaunchKernelGGL(wide_count<PraqJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+}
+
+template <class Real>
+int WidePraq<Real>::solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats) {
+    return wide_drive(*this, stream, resume, time_limit, stats);
+}
+
+// ------------------------------------------------------------- PRIA host --
+
+template <class Real>
+int WidePria<Real>::setup(const std::vector<PriaJob<Real>>& host_jobs, PriaJob<Real>* dev_jobs) {
+    constexpr int TW = PriaShape<Real>::TW;
+    count = int(host_jobs.size());
+    jobs = dev_jobs;
+    if (!count) return 0;
+    std::vector<int4> xt;
+    std::vector<int2> st;
+    std::vector<int> fi(4 * size_t(count));
+    op_steps = 2;
+    int tile_rows = 0;
+    for (int q = 0; q < count; ++q) {
+        const auto& a = host_jobs[q].a;
+        op_steps = std::max(op_steps, int(host_jobs[q].p.op_steps));
+        max_T = std::max(max_T, a.T);
+        const int tiles = (a.T + TW - 1) / TW;
+        fi[4 * q] = int(st.size());
+        fi[4 * q + 1] = a.T;
+        fi[4 * q + 2] = tile_rows;
+        fi[4 * q + 3] = a.N * tiles;
+        for (int s = 0; s < a.T; ++s) st.push_back(make_int2(q, s));
+        // heaviest tiles (largest t) first; the check row is (item, tile)
+        for (int k = tiles - 1; k >= 0; --k)
+            for (int i = 0; i < a.N; ++i)
+                xt.push_back(make_int4(q, i, k * TW, tile_rows + i * tiles + k));
+        tile_rows += a.N * tiles;
+    }
+    x_blocks = int(xt.size());
+    s_blocks = int(st.size());
+    const size_t x_bytes = align256(sizeof(int4) * xt.size());
+    const size_t tab_bytes = x_bytes + sizeof(int2) * st.size();
+    if (allocate(tab_bytes, xt.size() + st.size(), kPriaSums)) return kErrorDevice;
+    x_tab = static_cast<int4*>(arena);
+    s_tab = reinterpret_cast<int2*>(static_cast<char*>(arena) + x_bytes);
+    if (hipMemcpy(x_tab, xt.data(), sizeof(int4) * xt.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(s_tab, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return kErrorDevice;
+    const size_t lds = pria_x_lds_bytes<Real>(max_T);
+    if (lds > 65536 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(wpria_x<Real>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+        return kErrorDevice;
+    return 0;
+}
+
+template <class Real>
+int WidePria<Real>::start(hipStream_t stream, bool resume) {
+    const size_t lds = pria_x_lds_bytes<Real>(max_T);
+    const dim3 blk(kWideBlock);
+    hipLaunchKernelGGL(wide_init<PriaJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
+                       count, round_action, last_action, resume ? 1 : 0);
+    if (!resume) {
+        hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending,
+                           3);
+        hipLaunchKernelGGL(wpria_x<Real>, dim3(x_blocks), blk, lds, stream, jobs, x_tab,
+                           round_action, 0, 0);
+        hipLaunchKernelGGL(wpria_yz<Real>, dim3(s_blocks), dim3(1024), 0, stream, jobs, s_tab,
+                           round_action, 0, 0);
+        hipLaunchKernelGGL(wpria_sums<Real>, dim3(s_blocks + x_blocks), blk, 0, stream, jobs, s_tab,
+                           s_blocks, x_tab, round_action, 0, partial);
+        hipLaunchKernelGGL(wpria_logic<Real>, dim3(count), blk, 0, stream, jobs, first, s_blocks,
+                           partial, round_action, last_action, 0);
+        hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending,
+                           2);
+    }
+    hipLaunchKernelGGL(wide_count<PriaJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+}
+
+template <class Real>
+int WidePria<Real>::enqueue_round(hipStream_t stream) {
+    const size_t lds = pria_x_lds_bytes<Real>(max_T);
+    const dim3 blk(kWideBlock);
+    hipLaunchKernelGGL(wide_pre<PriaJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs, count,
+                       round_action, pending);
+    hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 0);
+    for (int j = 0; j < op_steps; ++j) {
+        hipLaunchKernelGGL(wpria_x<Real>, dim3(x_blocks), blk, lds, stream, jobs, x_tab,
+                           round_action, 0, j);
+        hipLaunchKernelGGL(wpria_yz<Real>, dim3(s_blocks), dim3(1024), 0, stream, jobs, s_tab,
+                           round_action, 0, j);
+    }
+    hipLaunchKernelGGL(wpria_sums<Real>, dim3(s_blocks + x_blocks), blk, 0, stream, jobs, s_tab,
+                       s_blocks, x_tab, round_action, 0, partial);
+    hipLaunchKernelGGL(wpria_logic<Real>, dim3(count), blk, 0, stream, jobs, first, s_blocks,
+                       partial, round_action, last_action, 0);
+    hipLaunchKernelGGL(wpria_x<Real>, dim3(x_blocks), blk, lds, stream, jobs, x_tab, round_action,
+                       1, 0);
+    hipLaunchKernelGGL(wpria_yz<Real>, dim3(s_blocks), dim3(1024), 0, stream, jobs, s_tab,
+                       round_action, 1, 0);
+    hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 1);
+    hipLaunchKernelGGL(wpria_sums<Real>, dim3(s_blocks + x_blocks), blk, 0, stream, jobs, s_tab,
+                       s_blocks, x_tab, round_action, 1, partial);
+    hipLaunchKernelGGL(wpria_logic<Real>, dim3(count), blk, 0, stream, jobs, first, s_blocks,
+                       partial, round_action, last_action, 1);
+    hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 2);
+    hipLaunchKernelGGL(wide_count<PriaJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+}
+
+template <class Real>
+int WidePria<Real>::expire(hipStream_t stream) {
+    const dim3 blk(kWideBlock);
+    hipLaunchKernelGGL(wide_expire<PriaJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
+                       count, last_action);
+    hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 2);
+    hipLaunchKernelGGL(wide_count<PriaJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+}
+
+template <class Real>
+int WidePria<Real>::solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats) {
+    return wide_drive(*this, stream, resume, time_limit, stats);
+}
+
+template struct WidePraq<float>;
+template struct WidePraq<double>;
+template struct WidePria<float>;
+template struct WidePria<double>;
+
+}  // namespace reform
