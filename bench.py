@@ -4,13 +4,16 @@ Workload: the batched mixture -- 1024 instances, 70% PRAQ (Erdos-Renyi, n
 log-uniform in [1e3, 1e5], mean degree U[4, 32], integer weights) and 30%
 PRIA (N in {10..100}, T in {50..300}), instance k drawn with seed k -- solved
 to 1e-6 relative duality gap with FP32 iterates and FP64 reductions.  A step
-is one solve of the whole batch from inputs resident in HBM; `value` is
-solves/s over the timed steps.  Beside it: the plain one-at-a-time GPU solver
-on the same resident instances (solves/s), the dominant launch's HBM roofline
-(algorithmic bytes: PRAQ 52 m + 32 n and PRIA 60 ND per iteration, FP32, over
-the iterations each instance ran, divided by the launch time measured with
-HIP events on its stream), and the oracle (FP64 C restatement, OpenMP) timed
-on a bounded sample of the same mixture on the host's cores.
+is one solve of the whole batch from inputs resident in HBM by the batched
+mode (block tables over all instances, per-instance restart state on the
+device, one check interval per hipGraph replay); `value` is solves/s over
+the timed steps.  Beside it: the plain one-at-a-time GPU solver on the same
+resident instances (solves/s); the roofline of the dominant step kernel --
+its launches' summed algorithmic bytes (PRAQ node n(4+7b) + 2m(4+b), edge
+m(8+7b); PRIA x 7b ND, y/z 8b ND per step, b = 4 for FP32) over their summed
+durations, each launch bracketed by HIP events on its stream in one more
+solve of the same batch after the timed steps; and the oracle (FP64 C restatement, OpenMP) timed on a bounded sample
+of the same mixture on the host's cores.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--count 1024]
 """
@@ -48,19 +51,6 @@ def build_batch(members, tolerance: float, use_float32: bool, max_iter: int):
             params = pria.Parameters.with_auto_step(max_iter, tolerance)
             index.append(("pria", b.add_pria(inst.problem(), params), inst))
     return b, index
-
-
-def algorithmic_bytes(index, infos) -> tuple[float, float]:
-    """(PRAQ, PRIA) algorithmic HBM bytes of one batched solve, FP32."""
-    praq_b = pria_b = 0.0
-    for (kind, q, inst), info in zip(index, infos):
-        if kind == "praq":
-            praq_b += (52.0 * inst.edges + 32.0 * inst.n) * info.iterations
-        else:
-            t = inst.periods + (inst.periods & 1)
-            nd = inst.items * t * (t + 1) // 2
-            pria_b += 60.0 * nd * info.iterations
-    return praq_b, pria_b
 
 
 def cpu_baseline(members, tolerance: float, budget_s: float):
@@ -122,17 +112,26 @@ def main() -> int:
         dt = batch.solve("batched")
         log(f"warmup {w}: {dt:.3f} s")
     torch.cuda.synchronize()
-    times, kms = [], []
+    times, kernels = [], {}
     for s in range(args.steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         batch.solve("batched")
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
-        kms.append(batch.kernel_ms())
-        log(f"step {s}: {times[-1]:.3f} s (praq launch {kms[-1][0]:.1f} ms, pria launch {kms[-1][1]:.1f} ms)")
+        km = batch.kernel_ms()
+        log(f"step {s}: {times[-1]:.3f} s (praq family {km[0]:.1f} ms, pria family {km[1]:.1f} ms)")
     total = sum(times)
     value = args.count * args.steps / total
+
+    # one more solve of the same batch with every step launch bracketed by
+    # HIP events on its stream (rounds launched directly, not replayed)
+    batch.set_timing(True)
+    dt = batch.solve("batched")
+    batch.set_timing(False)
+    log(f"timed-launch solve: {dt:.3f} s")
+    for k in batch.kernel_stats():
+        kernels[k["name"]] = k
 
     infos = []
     for kind, q, inst in index:
@@ -143,14 +142,12 @@ def main() -> int:
     log(f"converged {status.count(0)}/{len(status)}, max gap {gaps.max():.2e}, "
         f"iterations median {int(np.median(iters))} max {int(iters.max())}")
 
-    praq_b, pria_b = algorithmic_bytes(index, infos)
-    praq_ms = float(np.mean([k[0] for k in kms]))
-    pria_ms = float(np.mean([k[1] for k in kms]))
-    if pria_ms >= praq_ms:
-        dom, dom_bytes, dom_ms = "pria_wg_solve", pria_b, pria_ms
-    else:
-        dom, dom_bytes, dom_ms = "praq_wg_solve", praq_b, praq_ms
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    dom = max(kernels, key=lambda name: kernels[name]["ms"])
+    K = kernels[dom]
+    achieved = K["bytes"] / (K["ms"] * 1e-3) / 1e9
+    for name, k in sorted(kernels.items(), key=lambda kv: -kv[1]["ms"]):
+        log(f"kernel {name}: {k['launches']} launches, {k['ms'] / k['launches'] * 1e3:.1f} us avg, "
+            f"{k['bytes'] / (k['ms'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
 
     plain = None
     if not args.no_plain:
@@ -171,15 +168,18 @@ def main() -> int:
         dtype="f32" if use_f32 else "f64", data="synthetic",
         config=dict(workload=f"batched mixture of {args.count} instances (70% PRAQ ER n in "
                              "[1e3,1e5] deg U[4,32]; 30% PRIA N in [10,100], T in [50,300]) "
-                             f"to {args.tolerance:g} relative gap, FP32 iterates + FP64 "
-                             "reductions, one launch per family",
+                             f"to {args.tolerance:g} relative gap, "
+                             + ("FP32 iterates + FP64 reductions" if use_f32 else "FP64")
+                             + ", batched mode (block tables, hipGraph rounds)",
                     count=args.count, tolerance=args.tolerance, parallelism="none"),
         plain_solves_per_s=plain, speedup_vs_plain=(value / plain) if plain else None,
         latency_ms=dict(p50=ms, p99=ms, note="batched: every instance of a step completes "
                                              "with its launch"),
         converged=status.count(0), max_gap=float(gaps.max()),
         roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=PEAK_HBM_GBS,
-                      unit="GB/s", frac=achieved / PEAK_HBM_GBS, traffic=None),
+                      unit="GB/s", frac=achieved / PEAK_HBM_GBS, traffic=None,
+                      launches=K["launches"], avg_launch_us=K["ms"] / K["launches"] * 1e3,
+                      bytes_per_launch=K["bytes"] / K["launches"]),
         cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)

@@ -62,11 +62,33 @@ int reform_batch_set_shape(reform_batch* batch, int praq_block, int pria_block);
 
 int64_t reform_batch_count(const reform_batch* batch, int family /* 0 PRAQ, 1 PRIA */);
 
+/* Text of the library's last failed HIP call (call, source place, HIP error
+ * string), "" if none; valid until the calling thread's next call. */
+const char* reform_last_error(void);
+
 /* Durations of the last batched solve's two families, measured with HIP
  * events on the streams they ran on (0 for an empty family or a sequential
  * solve). */
 int reform_batch_kernel_ms(const reform_batch* batch, double* praq_ms, double* pria_ms);
 int64_t reform_batch_device_bytes(const reform_batch* batch);
+
+/* Launch timing of the batched mode's two step kernels per family (PRAQ:
+ * node, edge; PRIA: x, y/z).  With timing on, the rounds are launched
+ * directly instead of replayed from their graph, and every step launch is
+ * bracketed by HIP events on its stream; the figures cover the last batched
+ * solve.  engine 0 is the batch's precision, engine 1 the FP64 continuation
+ * of stalled FP32 instances.  bytes: algorithmic HBM bytes those launches
+ * moved (per step, per instance: PRAQ node n (4 + 7b) + 2m (4 + b), edge
+ * m (8 + 7b); PRIA x 7b ND, y/z 8b ND; b = bytes per value). */
+typedef struct reform_kernel_stats {
+    char name[64];       /* kernel symbol, e.g. "wpria_yz<float>" */
+    int64_t launches;
+    double ms;           /* sum of the launches' durations */
+    double bytes;        /* sum of their algorithmic bytes */
+} reform_kernel_stats;
+int reform_batch_set_timing(reform_batch* batch, int on);
+int reform_batch_kernel_stats(const reform_batch* batch, int family, int engine,
+                              reform_kernel_stats out[2]);
 
 /* Results of the last solve.  Outputs may be NULL.  PRAQ y is [m][2] in
  * instance units; PRIA x, y, z are dense [n][t][t], y and z normalised. */

@@ -83,9 +83,15 @@ EXPORTS = (
     "reform_batch_add_pria", "reform_batch_upload", "reform_batch_solve",
     "reform_batch_set_shape", "reform_batch_count", "reform_batch_kernel_ms",
     "reform_batch_device_bytes", "reform_batch_praq_result", "reform_batch_pria_result",
+    "reform_last_error", "reform_batch_set_timing", "reform_batch_kernel_stats",
 )
 
 _lib = None
+
+
+def last_error() -> str:
+    """Text of the library's last failed HIP call ("" if none)."""
+    return lib().reform_last_error().decode()
 
 
 def available() -> bool:
@@ -100,6 +106,8 @@ def lib() -> ctypes.CDLL:
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
     L = ctypes.CDLL(str(LIB_PATH))
+    L.reform_last_error.restype = ctypes.c_char_p
+    L.reform_last_error.argtypes = []
     L.praq_default_parameters.restype = PraqParameters
     L.praq_default_options.restype = PraqOptions
     L.praq_check_interval.restype = _i64

@@ -29,9 +29,20 @@ BATCHED, SEQUENTIAL, WORKGROUPS = 0, 1, 2
 _MODES = {"batched": BATCHED, "sequential": SEQUENTIAL, "workgroups": WORKGROUPS}
 
 
+class KernelStats(ctypes.Structure):
+    """reform_kernel_stats of include/reform_batch.h."""
+    _fields_ = [("name", ctypes.c_char * 64), ("launches", ctypes.c_int64),
+                ("ms", ctypes.c_double), ("bytes", ctypes.c_double)]
+
+
 def _declare(L):
     if getattr(L, "_batch_declared", False):
         return
+    L.reform_batch_set_timing.restype = ctypes.c_int
+    L.reform_batch_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.reform_batch_kernel_stats.restype = ctypes.c_int
+    L.reform_batch_kernel_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(KernelStats)]
     L.reform_batch_create.restype = ctypes.c_void_p
     L.reform_batch_create.argtypes = [ctypes.c_int]
     L.reform_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -149,7 +160,7 @@ class Batch:
     def upload(self) -> None:
         st = self._L.reform_batch_upload(self._h)
         if st < 0:
-            raise RuntimeError(f"reform_batch_upload failed ({st})")
+            raise RuntimeError(f"reform_batch_upload failed ({st}): {_native.last_error()}")
 
     def solve(self, mode: str = "batched") -> float:
         """Solve every instance from its resident inputs; returns seconds."""
@@ -157,7 +168,7 @@ class Batch:
         st = self._L.reform_batch_solve(self._h, _MODES[mode])
         dt = time.perf_counter() - t0
         if st < 0:
-            raise RuntimeError(f"reform_batch_solve failed ({st})")
+            raise RuntimeError(f"reform_batch_solve failed ({st}): {_native.last_error()}")
         return dt
 
     def kernel_ms(self) -> tuple[float, float]:
@@ -166,6 +177,28 @@ class Batch:
         a, b = ctypes.c_double(), ctypes.c_double()
         self._L.reform_batch_kernel_ms(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def set_timing(self, on: bool = True) -> None:
+        """Bracket every step launch of the batched mode with HIP events."""
+        self._L.reform_batch_set_timing(self._h, int(on))
+
+    def kernel_stats(self) -> list[dict]:
+        """Per step kernel of the last batched solve (both families, the
+        batch's precision and the FP64 continuation): name, launches, summed
+        duration (ms) and algorithmic bytes."""
+        out = []
+        for family in (0, 1):
+            for engine in (0, 1):
+                arr = (KernelStats * 2)()
+                st = self._L.reform_batch_kernel_stats(self._h, family, engine, arr)
+                if st < 0:
+                    raise RuntimeError(f"reform_batch_kernel_stats failed ({st}): "
+                                       f"{_native.last_error()}")
+                for k in arr:
+                    if k.launches:
+                        out.append(dict(name=k.name.decode(), launches=int(k.launches),
+                                        ms=float(k.ms), bytes=float(k.bytes)))
+        return out
 
     @property
     def device_bytes(self) -> int:
@@ -183,7 +216,7 @@ class Batch:
         st = self._L.reform_batch_praq_result(self._h, q, _native.dptr(x), _native.dptr(y),
                                               ctypes.byref(info))
         if st < 0:
-            raise RuntimeError(f"reform_batch_praq_result failed ({st})")
+            raise RuntimeError(f"reform_batch_praq_result failed ({st}): {_native.last_error()}")
         return x, (y[:m] if y is not None else None), info
 
     def pria_result(self, q: int, with_solution: bool = True):
@@ -196,5 +229,5 @@ class Batch:
         st = self._L.reform_batch_pria_result(self._h, q, _native.dptr(x), _native.dptr(y),
                                               _native.dptr(z), ctypes.byref(info))
         if st < 0:
-            raise RuntimeError(f"reform_batch_pria_result failed ({st})")
+            raise RuntimeError(f"reform_batch_pria_result failed ({st}): {_native.last_error()}")
         return x, y, z, info

@@ -24,6 +24,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -141,6 +143,22 @@ struct BatchImpl {
     std::unique_ptr<WidePria<double>> wpria64, wpria64_sub;
     std::vector<int> wpraq_sub_key, wpria_sub_key;
     WideStats praq_stats, pria_stats;
+    bool timing = false;
+
+    WideControl* engine(int family, int which) const {
+        const bool f32 = use_float32 != 0;
+        if (family == 0)
+            return which == 0 ? (f32 ? static_cast<WideControl*>(wpraq32.get()) : wpraq64.get())
+                              : (f32 ? wpraq64_sub.get() : nullptr);
+        return which == 0 ? (f32 ? static_cast<WideControl*>(wpria32.get()) : wpria64.get())
+                          : (f32 ? wpria64_sub.get() : nullptr);
+    }
+    void set_timing(bool on) {
+        timing = on;
+        for (int f = 0; f < 2; ++f)
+            for (int w = 0; w < 2; ++w)
+                if (WideControl* e = engine(f, w)) e->set_timing(on);
+    }
 
     ~BatchImpl() {
         for (auto& e : ev)
@@ -153,8 +171,8 @@ struct BatchImpl {
     int upload();
     int solve(int mode);
     int solve_wide();
-    int wide_praq();
-    int wide_pria();
+    WideControl* praq_handover(int* status);
+    WideControl* pria_handover(int* status);
     int solve_workgroups();
     int solve_sequential();
     int praq_result(int64_t q, double* x, double* y, PraqInfo* info);
@@ -166,8 +184,7 @@ struct BatchImpl {
         if (tier == PRAQ_TIER_WORKGROUP) {
             PraqJob<Real> j = job;
             j.resume = resume ? 1 : 0;
-            if (hipMemcpyAsync(devslot, &j, sizeof(j), hipMemcpyHostToDevice, s0) != hipSuccess)
-                return kErrorDevice;
+            REFORM_HIP_CHECK(hipMemcpyAsync(devslot, &j, sizeof(j), hipMemcpyHostToDevice, s0));
             if (praq_launch_wg<Real>(devslot, 1, 1024, s0)) return kErrorDevice;
             return hipStreamSynchronize(s0) == hipSuccess ? 0 : kErrorDevice;
         }
@@ -175,8 +192,7 @@ struct BatchImpl {
         if (resume && hipMemcpy(&st, job.state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
             return kErrorDevice;
         if (praq_device_solve<Real>(job, st, work, 0.0, s0, resume)) return kErrorDevice;
-        if (hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice, s0) != hipSuccess)
-            return kErrorDevice;
+        REFORM_HIP_CHECK(hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice, s0));
         return hipStreamSynchronize(s0) == hipSuccess ? 0 : kErrorDevice;
     }
 
@@ -185,8 +201,7 @@ struct BatchImpl {
         if (tier == PRIA_TIER_WORKGROUP) {
             PriaJob<Real> j = job;
             j.resume = resume ? 1 : 0;
-            if (hipMemcpyAsync(devslot, &j, sizeof(j), hipMemcpyHostToDevice, s1) != hipSuccess)
-                return kErrorDevice;
+            REFORM_HIP_CHECK(hipMemcpyAsync(devslot, &j, sizeof(j), hipMemcpyHostToDevice, s1));
             if (pria_launch_wg<Real>(devslot, 1, T, s1)) return kErrorDevice;
             return hipStreamSynchronize(s1) == hipSuccess ? 0 : kErrorDevice;
         }
@@ -194,8 +209,7 @@ struct BatchImpl {
         if (resume && hipMemcpy(&st, job.state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
             return kErrorDevice;
         if (pria_device_solve<Real>(job, st, work, 0.0, s1, resume)) return kErrorDevice;
-        if (hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice, s1) != hipSuccess)
-            return kErrorDevice;
+        REFORM_HIP_CHECK(hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice, s1));
         return hipStreamSynchronize(s1) == hipSuccess ? 0 : kErrorDevice;
     }
 };
@@ -279,12 +293,12 @@ int BatchImpl::upload() {
         dev = nullptr;
         return kErrorDevice;
     }
-    if (hipGetDevice(&device) != hipSuccess) return kErrorDevice;
+    REFORM_HIP_CHECK(hipGetDevice(&device));
     if (hipStreamCreateWithFlags(&s0, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&s1, hipStreamNonBlocking) != hipSuccess)
         return kErrorDevice;
     for (auto& e : ev)
-        if (hipEventCreate(&e) != hipSuccess) return kErrorDevice;
+        REFORM_HIP_CHECK(hipEventCreate(&e));
     char* base = static_cast<char*>(dev);
     praq32.dev = reinterpret_cast<PraqJob<float>*>(base + off_p32);
     praq32.sub = praq32.dev + np;
@@ -352,8 +366,7 @@ int BatchImpl::upload() {
             size_t up = pria_upload_bytes(e.host, L);
             std::fill(image.begin(), image.begin() + up, 0);
             pria_fill_image<Real>(e.host, L, image.data());
-            if (hipMemcpy(base + off, image.data(), up, hipMemcpyHostToDevice) != hipSuccess)
-                return kErrorDevice;
+            REFORM_HIP_CHECK(hipMemcpy(base + off, image.data(), up, hipMemcpyHostToDevice));
             std::memset(&job, 0, sizeof(job));
             job.a = pria_bind<Real>(L, base + off, e.host);
             if (e.host.cold && pria_cold_start<Real>(job.a, s1)) return kErrorDevice;
@@ -378,7 +391,7 @@ int BatchImpl::upload() {
     };
     if (put(praq64) || put(pria64)) return kErrorDevice;
     if (f32 && (put(praq32) || put(pria32))) return kErrorDevice;
-    if (hipStreamSynchronize(s1) != hipSuccess) return kErrorDevice;
+    REFORM_HIP_CHECK(hipStreamSynchronize(s1));
     praq_states.assign(praq.size(), LogicState());
     pria_states.assign(pria.size(), LogicState());
     praq_in64.assign(praq.size(), 1);
@@ -408,152 +421,206 @@ int BatchImpl::solve(int mode) {
     return 0;
 }
 
-// The PRAQ family on stream s0: the wide engine in the batch's precision,
-// then, for FP32, the stalled subset continued in FP64.
-int BatchImpl::wide_praq() {
-    const bool f32 = use_float32 != 0;
+// FP32 batches: the PRAQ instances whose FP32 iterates stalled continue in
+// FP64 from their converted state, in a wide engine over that subset (kept
+// for the next solve while the subset is the same).  Returns the engine to
+// run, or nullptr when none stalled.
+WideControl* BatchImpl::praq_handover(int* status) {
     const int np = int(praq.size());
-    if (!np) return 0;
-    if (hipEventRecord(ev[0], s0) != hipSuccess) return kErrorDevice;
-    praq_stats = WideStats();
-    if (!f32) {
-        if (!wpraq64) {
-            wpraq64.reset(new WidePraq<double>());
-            if (wpraq64->setup(praq64.host, praq64.dev)) return kErrorDevice;
+    *status = 0;
+    if (hipMemcpyAsync(praq_states.data(), praq_states_dev, sizeof(LogicState) * np,
+                       hipMemcpyDeviceToHost, s0) != hipSuccess ||
+        hipStreamSynchronize(s0) != hipSuccess) {
+        *status = kErrorDevice;
+        return nullptr;
+    }
+    std::vector<int> key;
+    std::vector<PraqJob<float>> p32;
+    std::vector<PraqJob<double>> p64;
+    for (int s = 0; s < np; ++s) {
+        if (!praq_states[s].stalled) continue;
+        key.push_back(s);
+        p32.push_back(praq32.host[s]);
+        p64.push_back(praq64.host[s]);
+        praq_in64[praq_order[s]] = 1;
+    }
+    if (key.empty()) return nullptr;
+    if (hipMemcpyAsync(praq32.sub, p32.data(), sizeof(p32[0]) * p32.size(), hipMemcpyHostToDevice,
+                       s0) != hipSuccess ||
+        hipMemcpyAsync(praq64.sub, p64.data(), sizeof(p64[0]) * p64.size(), hipMemcpyHostToDevice,
+                       s0) != hipSuccess ||
+        praq_convert_jobs(praq32.sub, praq64.sub, int(p32.size()), s0) ||
+        hipStreamSynchronize(s0) != hipSuccess) {
+        *status = kErrorDevice;
+        return nullptr;
+    }
+    if (!wpraq64_sub || key != wpraq_sub_key) {
+        wpraq64_sub.reset(new WidePraq<double>());
+        wpraq64_sub->set_timing(timing);
+        if (wpraq64_sub->setup(p64, praq64.sub)) {
+            *status = kErrorDevice;
+            return nullptr;
         }
-        if (wpraq64->solve(s0, false, time_limit, &praq_stats)) return kErrorDevice;
-        std::fill(praq_in64.begin(), praq_in64.end(), 1);
-    } else {
-        if (!wpraq32) {
+        wpraq_sub_key = key;
+    }
+    return wpraq64_sub.get();
+}
+
+// As praq_handover, for PRIA on stream s1.
+WideControl* BatchImpl::pria_handover(int* status) {
+    const int nr = int(pria.size());
+    *status = 0;
+    if (hipMemcpyAsync(pria_states.data(), pria_states_dev, sizeof(LogicState) * nr,
+                       hipMemcpyDeviceToHost, s1) != hipSuccess ||
+        hipStreamSynchronize(s1) != hipSuccess) {
+        *status = kErrorDevice;
+        return nullptr;
+    }
+    std::vector<int> key;
+    std::vector<PriaJob<float>> r32;
+    std::vector<PriaJob<double>> r64;
+    for (int s = 0; s < nr; ++s) {
+        if (!pria_states[s].stalled) continue;
+        key.push_back(s);
+        r32.push_back(pria32.host[s]);
+        r64.push_back(pria64.host[s]);
+        pria_in64[pria_order[s]] = 1;
+    }
+    if (key.empty()) return nullptr;
+    if (hipMemcpyAsync(pria32.sub, r32.data(), sizeof(r32[0]) * r32.size(), hipMemcpyHostToDevice,
+                       s1) != hipSuccess ||
+        hipMemcpyAsync(pria64.sub, r64.data(), sizeof(r64[0]) * r64.size(), hipMemcpyHostToDevice,
+                       s1) != hipSuccess ||
+        pria_convert_jobs(pria32.sub, pria64.sub, int(r32.size()), s1) ||
+        hipStreamSynchronize(s1) != hipSuccess) {
+        *status = kErrorDevice;
+        return nullptr;
+    }
+    if (!wpria64_sub || key != wpria_sub_key) {
+        wpria64_sub.reset(new WidePria<double>());
+        wpria64_sub->set_timing(timing);
+        if (wpria64_sub->setup(r64, pria64.sub)) {
+            *status = kErrorDevice;
+            return nullptr;
+        }
+        wpria_sub_key = key;
+    }
+    return wpria64_sub.get();
+}
+
+// Both families at once from this thread: PRAQ on s0, PRIA on s1, each a
+// wide engine in the batch's precision, then (FP32) its stalled subset in
+// FP64.  The runs are advanced without blocking, so neither stream waits
+// for the other's host work.
+int BatchImpl::solve_wide() {
+    const bool f32 = use_float32 != 0;
+    const int np = int(praq.size()), nr = int(pria.size());
+    WideControl* praq_main = nullptr;
+    WideControl* pria_main = nullptr;
+    if (np) {
+        if (f32 && !wpraq32) {
             wpraq32.reset(new WidePraq<float>());
+            wpraq32->set_timing(timing);
             if (wpraq32->setup(praq32.host, praq32.dev)) return kErrorDevice;
         }
-        if (wpraq32->solve(s0, false, time_limit, &praq_stats)) return kErrorDevice;
-        std::fill(praq_in64.begin(), praq_in64.end(), 0);
-        if (hipMemcpyAsync(praq_states.data(), praq_states_dev, sizeof(LogicState) * np,
-                           hipMemcpyDeviceToHost, s0) != hipSuccess ||
-            hipStreamSynchronize(s0) != hipSuccess)
-            return kErrorDevice;
-        std::vector<int> key;
-        std::vector<PraqJob<float>> p32;
-        std::vector<PraqJob<double>> p64;
-        for (int s = 0; s < np; ++s) {
-            if (!praq_states[s].stalled) continue;
-            key.push_back(s);
-            p32.push_back(praq32.host[s]);
-            p64.push_back(praq64.host[s]);
-            praq_in64[praq_order[s]] = 1;
+        if (!f32 && !wpraq64) {
+            wpraq64.reset(new WidePraq<double>());
+            wpraq64->set_timing(timing);
+            if (wpraq64->setup(praq64.host, praq64.dev)) return kErrorDevice;
         }
-        if (!key.empty()) {
-            if (hipMemcpyAsync(praq32.sub, p32.data(), sizeof(p32[0]) * p32.size(),
-                               hipMemcpyHostToDevice, s0) != hipSuccess ||
-                hipMemcpyAsync(praq64.sub, p64.data(), sizeof(p64[0]) * p64.size(),
-                               hipMemcpyHostToDevice, s0) != hipSuccess ||
-                praq_convert_jobs(praq32.sub, praq64.sub, int(p32.size()), s0))
-                return kErrorDevice;
-            if (!wpraq64_sub || key != wpraq_sub_key) {
-                if (hipStreamSynchronize(s0) != hipSuccess) return kErrorDevice;
-                wpraq64_sub.reset(new WidePraq<double>());
-                if (wpraq64_sub->setup(p64, praq64.sub)) return kErrorDevice;
-                wpraq_sub_key = key;
-            }
-            WideStats more;
-            if (wpraq64_sub->solve(s0, true, time_limit, &more)) return kErrorDevice;
-            praq_stats.rounds += more.rounds;
-        }
+        praq_main = f32 ? static_cast<WideControl*>(wpraq32.get()) : wpraq64.get();
     }
-    if (hipEventRecord(ev[1], s0) != hipSuccess || hipStreamSynchronize(s0) != hipSuccess)
-        return kErrorDevice;
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) praq_ms = ms;
-    return 0;
-}
-
-// The PRIA family on stream s1, as wide_praq.
-int BatchImpl::wide_pria() {
-    const bool f32 = use_float32 != 0;
-    const int nr = int(pria.size());
-    if (!nr) return 0;
-    if (hipEventRecord(ev[2], s1) != hipSuccess) return kErrorDevice;
-    pria_stats = WideStats();
-    if (!f32) {
-        if (!wpria64) {
-            wpria64.reset(new WidePria<double>());
-            if (wpria64->setup(pria64.host, pria64.dev)) return kErrorDevice;
-        }
-        if (wpria64->solve(s1, false, time_limit, &pria_stats)) return kErrorDevice;
-        std::fill(pria_in64.begin(), pria_in64.end(), 1);
-    } else {
-        if (!wpria32) {
+    if (nr) {
+        if (f32 && !wpria32) {
             wpria32.reset(new WidePria<float>());
+            wpria32->set_timing(timing);
             if (wpria32->setup(pria32.host, pria32.dev)) return kErrorDevice;
         }
-        if (wpria32->solve(s1, false, time_limit, &pria_stats)) return kErrorDevice;
-        std::fill(pria_in64.begin(), pria_in64.end(), 0);
-        if (hipMemcpyAsync(pria_states.data(), pria_states_dev, sizeof(LogicState) * nr,
-                           hipMemcpyDeviceToHost, s1) != hipSuccess ||
-            hipStreamSynchronize(s1) != hipSuccess)
-            return kErrorDevice;
-        std::vector<int> key;
-        std::vector<PriaJob<float>> r32;
-        std::vector<PriaJob<double>> r64;
-        for (int s = 0; s < nr; ++s) {
-            if (!pria_states[s].stalled) continue;
-            key.push_back(s);
-            r32.push_back(pria32.host[s]);
-            r64.push_back(pria64.host[s]);
-            pria_in64[pria_order[s]] = 1;
+        if (!f32 && !wpria64) {
+            wpria64.reset(new WidePria<double>());
+            wpria64->set_timing(timing);
+            if (wpria64->setup(pria64.host, pria64.dev)) return kErrorDevice;
         }
-        if (!key.empty()) {
-            if (hipMemcpyAsync(pria32.sub, r32.data(), sizeof(r32[0]) * r32.size(),
-                               hipMemcpyHostToDevice, s1) != hipSuccess ||
-                hipMemcpyAsync(pria64.sub, r64.data(), sizeof(r64[0]) * r64.size(),
-                               hipMemcpyHostToDevice, s1) != hipSuccess ||
-                pria_convert_jobs(pria32.sub, pria64.sub, int(r32.size()), s1))
-                return kErrorDevice;
-            if (!wpria64_sub || key != wpria_sub_key) {
-                if (hipStreamSynchronize(s1) != hipSuccess) return kErrorDevice;
-                wpria64_sub.reset(new WidePria<double>());
-                if (wpria64_sub->setup(r64, pria64.sub)) return kErrorDevice;
-                wpria_sub_key = key;
-            }
-            WideStats more;
-            if (wpria64_sub->solve(s1, true, time_limit, &more)) return kErrorDevice;
-            pria_stats.rounds += more.rounds;
-        }
+        pria_main = f32 ? static_cast<WideControl*>(wpria32.get()) : wpria64.get();
     }
-    if (hipEventRecord(ev[3], s1) != hipSuccess || hipStreamSynchronize(s1) != hipSuccess)
-        return kErrorDevice;
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) pria_ms = ms;
-    return 0;
-}
+    std::fill(praq_in64.begin(), praq_in64.end(), f32 ? 0 : 1);
+    std::fill(pria_in64.begin(), pria_in64.end(), f32 ? 0 : 1);
+    praq_stats = pria_stats = WideStats();
+    for (int f = 0; f < 2; ++f)
+        for (int w = 0; w < 2; ++w)
+            if (WideControl* e = engine(f, w)) e->clear_timing();
 
-// Both families at once: PRIA driven from a second host thread.
-int BatchImpl::solve_wide() {
-    int pria_status = 0;
-    std::thread worker;
-    if (!pria.empty()) {
-        worker = std::thread([&] {
-            pria_status = hipSetDevice(device) == hipSuccess ? wide_pria() : kErrorDevice;
-        });
+    struct Family {
+        WideRun run;
+        int phase = 2;  // 0 main run, 1 FP64 continuation, 2 done
+        hipStream_t stream;
+        hipEvent_t begin_ev, end_ev;
+        WideStats* stats;
+    };
+    Family fam[2];
+    fam[0].stream = s0;
+    fam[0].begin_ev = ev[0];
+    fam[0].end_ev = ev[1];
+    fam[0].stats = &praq_stats;
+    fam[1].stream = s1;
+    fam[1].begin_ev = ev[2];
+    fam[1].end_ev = ev[3];
+    fam[1].stats = &pria_stats;
+    WideControl* mains[2] = {praq_main, pria_main};
+    for (int f = 0; f < 2; ++f) {
+        if (!mains[f]) continue;
+        REFORM_HIP_CHECK(hipEventRecord(fam[f].begin_ev, fam[f].stream));
+        if (fam[f].run.begin(mains[f], fam[f].stream, false, time_limit)) return kErrorDevice;
+        fam[f].phase = 0;
     }
-    const int praq_status = wide_praq();
-    if (worker.joinable()) worker.join();
-    return praq_status ? praq_status : pria_status;
+    while (fam[0].phase < 2 || fam[1].phase < 2) {
+        int changed = 0;
+        for (int f = 0; f < 2; ++f) {
+            Family& F = fam[f];
+            if (F.phase == 2) continue;
+            const int st = F.run.advance();
+            if (st < 0) return st;
+            changed |= st;
+            if (!F.run.finished) continue;
+            F.stats->rounds += F.run.stats.rounds;
+            WideControl* next = nullptr;
+            if (F.phase == 0 && f32) {
+                int hs = 0;
+                next = f == 0 ? praq_handover(&hs) : pria_handover(&hs);
+                if (hs) return hs;
+            }
+            if (next) {
+                const double left =
+                    time_limit > 0.0 ? std::max(1e-9, time_limit - F.run.stats.seconds) : 0.0;
+                if (F.run.begin(next, F.stream, true, left)) return kErrorDevice;
+                F.phase = 1;
+            } else {
+                REFORM_HIP_CHECK(hipEventRecord(F.end_ev, F.stream));
+                F.phase = 2;
+            }
+            changed = 1;
+        }
+        if (!changed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    REFORM_HIP_CHECK(hipStreamSynchronize(s0));
+    REFORM_HIP_CHECK(hipStreamSynchronize(s1));
+    float ms = 0.0f;
+    if (np && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) praq_ms = ms;
+    if (nr && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) pria_ms = ms;
+    return 0;
 }
 
 int BatchImpl::solve_workgroups() {
     const bool f32 = use_float32 != 0;
     const int np = int(praq.size()), nr = int(pria.size());
     if (np) {
-        if (hipEventRecord(ev[0], s0) != hipSuccess) return kErrorDevice;
+        REFORM_HIP_CHECK(hipEventRecord(ev[0], s0));
         int e = f32 ? praq_launch_wg<float>(praq32.dev, np, praq_block, s0)
                     : praq_launch_wg<double>(praq64.dev, np, praq_block, s0);
         if (e) return kErrorDevice;
     }
     if (nr) {
-        if (hipEventRecord(ev[2], s1) != hipSuccess) return kErrorDevice;
+        REFORM_HIP_CHECK(hipEventRecord(ev[2], s1));
         int e = f32 ? pria_launch_wg<float>(pria32.dev, nr, pria_max_T, s1)
                     : pria_launch_wg<double>(pria64.dev, nr, pria_max_T, s1);
         if (e) return kErrorDevice;
@@ -579,9 +646,8 @@ int BatchImpl::solve_workgroups() {
             p64.back().resume = 1;
             praq_in64[praq_order[s]] = 1;
             praq_states[s].switch_iteration = praq_states[s].iteration;
-            if (hipMemcpyAsync(praq_states_dev + s, &praq_states[s], sizeof(LogicState),
-                               hipMemcpyHostToDevice, s0) != hipSuccess)
-                return kErrorDevice;
+            REFORM_HIP_CHECK(hipMemcpyAsync(praq_states_dev + s, &praq_states[s], sizeof(LogicState),
+                               hipMemcpyHostToDevice, s0));
         }
         std::vector<PriaJob<float>> r32;
         std::vector<PriaJob<double>> r64;
@@ -592,9 +658,8 @@ int BatchImpl::solve_workgroups() {
             r64.back().resume = 1;
             pria_in64[pria_order[s]] = 1;
             pria_states[s].switch_iteration = pria_states[s].iteration;
-            if (hipMemcpyAsync(pria_states_dev + s, &pria_states[s], sizeof(LogicState),
-                               hipMemcpyHostToDevice, s1) != hipSuccess)
-                return kErrorDevice;
+            REFORM_HIP_CHECK(hipMemcpyAsync(pria_states_dev + s, &pria_states[s], sizeof(LogicState),
+                               hipMemcpyHostToDevice, s1));
         }
         if (!p32.empty()) {
             if (hipMemcpyAsync(praq32.sub, p32.data(), sizeof(p32[0]) * p32.size(),
@@ -641,13 +706,11 @@ int BatchImpl::solve_sequential() {
         }
         if (praq_plain<float>(praq32.host[s], praq32.sub, tier, false)) return kErrorDevice;
         LogicState st;
-        if (hipMemcpy(&st, praq_states_dev + s, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
-            return kErrorDevice;
+        REFORM_HIP_CHECK(hipMemcpy(&st, praq_states_dev + s, sizeof(st), hipMemcpyDeviceToHost));
         praq_in64[q] = 0;
         if (st.stalled) {
             st.switch_iteration = st.iteration;
-            if (hipMemcpy(praq_states_dev + s, &st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess)
-                return kErrorDevice;
+            REFORM_HIP_CHECK(hipMemcpy(praq_states_dev + s, &st, sizeof(st), hipMemcpyHostToDevice));
             if (praq_convert_state(praq32.host[s].a, praq64.host[s].a, s0)) return kErrorDevice;
             if (praq_plain<double>(praq64.host[s], praq64.sub, tier, true)) return kErrorDevice;
             praq_in64[q] = 1;
@@ -663,13 +726,11 @@ int BatchImpl::solve_sequential() {
         }
         if (pria_plain<float>(pria32.host[s], pria32.sub, tier, false, T)) return kErrorDevice;
         LogicState st;
-        if (hipMemcpy(&st, pria_states_dev + s, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
-            return kErrorDevice;
+        REFORM_HIP_CHECK(hipMemcpy(&st, pria_states_dev + s, sizeof(st), hipMemcpyDeviceToHost));
         pria_in64[q] = 0;
         if (st.stalled) {
             st.switch_iteration = st.iteration;
-            if (hipMemcpy(pria_states_dev + s, &st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess)
-                return kErrorDevice;
+            REFORM_HIP_CHECK(hipMemcpy(pria_states_dev + s, &st, sizeof(st), hipMemcpyHostToDevice));
             if (pria_convert_state(pria32.host[s].a, pria64.host[s].a, s1)) return kErrorDevice;
             if (pria_plain<double>(pria64.host[s], pria64.sub, tier, true, T)) return kErrorDevice;
             pria_in64[q] = 1;
@@ -821,6 +882,13 @@ extern "C" int64_t reform_batch_add_pria(reform_batch* batch, int64_t n, int64_t
     return int64_t(batch->impl->pria.size()) - 1;
 }
 
+extern "C" const char* reform_last_error(void) {
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lock(last_error_mutex());
+    copy = last_error_text();
+    return copy.c_str();
+}
+
 extern "C" int reform_batch_upload(reform_batch* batch) {
     return batch ? batch->impl->upload() : kErrorArgument;
 }
@@ -846,6 +914,32 @@ extern "C" int reform_batch_kernel_ms(const reform_batch* batch, double* praq_ms
     if (!batch) return kErrorArgument;
     if (praq_ms) *praq_ms = batch->impl->praq_ms;
     if (pria_ms) *pria_ms = batch->impl->pria_ms;
+    return 0;
+}
+
+extern "C" int reform_batch_set_timing(reform_batch* batch, int on) {
+    if (!batch) return kErrorArgument;
+    batch->impl->set_timing(on != 0);
+    return 0;
+}
+
+extern "C" int reform_batch_kernel_stats(const reform_batch* batch, int family, int engine,
+                                         reform_kernel_stats* out) {
+    if (!batch || !out || family < 0 || family > 1 || engine < 0 || engine > 1)
+        return kErrorArgument;
+    const BatchImpl& b = *batch->impl;
+    static const char* kNames[2][2] = {{"wpraq_node", "wpraq_edge"}, {"wpria_x", "wpria_yz"}};
+    const bool f32 = b.use_float32 != 0 && engine == 0;
+    WideControl* e = b.engine(family, engine);
+    double bytes[WideControl::kKinds] = {0.0, 0.0};
+    if (e && e->algorithmic_bytes(bytes)) return kErrorDevice;
+    for (int k = 0; k < 2; ++k) {
+        std::snprintf(out[k].name, sizeof(out[k].name), "%s<%s>", kNames[family][k],
+                      f32 ? "float" : "double");
+        out[k].launches = e ? e->launches[k] : 0;
+        out[k].ms = e ? e->kernel_ms[k] : 0.0;
+        out[k].bytes = bytes[k];
+    }
     return 0;
 }
 

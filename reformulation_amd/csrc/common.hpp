@@ -6,6 +6,8 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
+#include <string>
 
 namespace reform {
 
@@ -114,10 +116,29 @@ __host__ __device__ inline void halpern_weights(int64_t r, double* lw, double* r
     *rw = 1.0 / (k + 1.0);
 }
 
+// The last failed HIP call of the library (call text, place, HIP error),
+// readable through reform_last_error().
+inline std::mutex& last_error_mutex() {
+    static std::mutex m;
+    return m;
+}
+inline std::string& last_error_text() {
+    static std::string s;
+    return s;
+}
+inline void record_error(const char* call, hipError_t err, const char* file, int line) {
+    std::lock_guard<std::mutex> lock(last_error_mutex());
+    last_error_text() = std::string(call) + " at " + file + ":" + std::to_string(line) + ": " +
+                        hipGetErrorString(err);
+}
+
 }  // namespace reform
 
-#define REFORM_HIP_CHECK(call)                                  \
-    do {                                                        \
-        hipError_t reform_err_ = (call);                        \
-        if (reform_err_ != hipSuccess) return kErrorDevice;     \
+#define REFORM_HIP_CHECK(call)                                                \
+    do {                                                                      \
+        hipError_t reform_err_ = (call);                                      \
+        if (reform_err_ != hipSuccess) {                                      \
+            ::reform::record_error(#call, reform_err_, __FILE__, __LINE__);   \
+            return kErrorDevice;                                              \
+        }                                                                     \
     } while (0)

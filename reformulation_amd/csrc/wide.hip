@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace reform {
@@ -84,8 +85,9 @@ __device__ __forceinline__ int wide_logic_action(const Job& job, const int* roun
 // device tier's host loop.
 template <class Job>
 __device__ void wide_apply(const Job& job, int act, const CheckSums& cs, int* last_action,
-                           int inst) {
+                           int64_t* steps, int inst) {
     LogicState st = *job.state;
+    steps[inst] += act == kActWarmBlock || act == kActHalpernBlock ? job.p.op_steps : 1;
     if (act == kActInit) {
         logic_after_warm_check(st, job.p, cs);
     } else if (act == kActWarmBlock) {
@@ -121,10 +123,11 @@ __global__ void wide_pre(const Job* jobs, int count, int* round_action, int* pen
 // state handed over from the other precision.
 template <class Job>
 __global__ void wide_init(const Job* jobs, int count, int* round_action, int* last_action,
-                          int resume) {
+                          int64_t* steps, int resume) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= count) return;
     LogicState* st = jobs[inst].state;
+    steps[inst] = 0;
     if (resume) {
         st->stalled = 0;
         st->switch_iteration = st->iteration;
@@ -224,7 +227,7 @@ template <class Real>
 __global__ void __launch_bounds__(256) wpraq_logic(const PraqJob<Real>* jobs, const int* first,
                                                    int node_blocks, const double* partial,
                                                    const int* round_action, int* last_action,
-                                                   int mode) {
+                                                   int64_t* steps, int mode) {
     __shared__ double red[kPraqSums * 16];
     const int inst = blockIdx.x;
     const PraqJob<Real>& job = jobs[inst];
@@ -241,7 +244,8 @@ __global__ void __launch_bounds__(256) wpraq_logic(const PraqJob<Real>* jobs, co
         for (int k = 0; k < kPraqSums; ++k)
             acc[k] += partial[size_t(node_blocks + ef + r) * kPraqSums + k];
     block_sums<kPraqSums>(acc, red);
-    if (threadIdx.x == 0) wide_apply(job, act, praq_check_sums(acc, job.scale), last_action, inst);
+    if (threadIdx.x == 0)
+        wide_apply(job, act, praq_check_sums(acc, job.scale), last_action, steps, inst);
 }
 
 template <class Real>
@@ -328,7 +332,7 @@ template <class Real>
 __global__ void __launch_bounds__(256) wpria_logic(const PriaJob<Real>* jobs, const int* first,
                                                    int s_blocks, const double* partial,
                                                    const int* round_action, int* last_action,
-                                                   int mode) {
+                                                   int64_t* steps, int mode) {
     __shared__ double red[kPriaSums * 16];
     const int inst = blockIdx.x;
     const PriaJob<Real>& job = jobs[inst];
@@ -343,7 +347,8 @@ __global__ void __launch_bounds__(256) wpria_logic(const PriaJob<Real>* jobs, co
     for (int r = threadIdx.x; r < tc; r += blockDim.x)
         acc[kPLower] += partial[size_t(s_blocks + tf + r) * kPriaSums];
     block_sums<kPriaSums>(acc, red);
-    if (threadIdx.x == 0) wide_apply(job, act, pria_check_sums(acc, job.scale), last_action, inst);
+    if (threadIdx.x == 0)
+        wide_apply(job, act, pria_check_sums(acc, job.scale), last_action, steps, inst);
 }
 
 // one block per slice; copy modes as wide_copies
@@ -373,75 +378,158 @@ __global__ void __launch_bounds__(256) wpria_copy(const PriaJob<Real>* jobs, con
     }
 }
 
-// Start, then graph-replayed rounds until no instance is active or the time
-// is up.  Two pinned slots carry the active count back so the next group of
-// rounds is queued while the previous group's count is read.
-template <class Wide>
-int wide_drive(Wide& w, hipStream_t stream, bool resume, double time_limit, WideStats* stats) {
-    const auto t0 = std::chrono::steady_clock::now();
-    auto elapsed = [&] {
-        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    };
-    if (w.count == 0) return 0;
-    if (w.start(stream, resume)) return kErrorDevice;
-    if (!w.exec) {
-        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
-            return kErrorDevice;
-        const int st = w.enqueue_round(stream);
-        if (hipStreamEndCapture(stream, &w.graph) != hipSuccess || st) return kErrorDevice;
-        if (hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0) != hipSuccess)
-            return kErrorDevice;
+}  // namespace
+
+// ------------------------------------------------------------------ runs --
+
+int WideRun::begin(WideControl* engine, hipStream_t s, bool resume, double limit) {
+    w = engine;
+    stream = s;
+    time_limit = limit;
+    queued = retired = 0;
+    stats = WideStats();
+    t0 = std::chrono::steady_clock::now();
+    finished = true;
+    if (!w || w->count == 0) return 0;
+    w->mark_set = 2;
+    w->marks_used[2] = 0;
+    w->mark_kinds[2].clear();
+    if (w->start(stream, resume)) return kErrorDevice;
+    // the round graph (timed rounds are launched directly instead: events
+    // recorded by graph nodes carry no timestamps for hipEventElapsedTime)
+    if (!w->timing && !w->exec) {
+        REFORM_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        const int st = w->enqueue_round(stream);
+        REFORM_HIP_CHECK(hipStreamEndCapture(stream, &w->graph));
+        if (st) return kErrorDevice;
+        REFORM_HIP_CHECK(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0));
     }
-    auto group = [&](int slot) -> int {
-        for (int r = 0; r < kRoundsPerPoll; ++r)
-            if (hipGraphLaunch(w.exec, stream) != hipSuccess) return kErrorDevice;
-        if (hipMemcpyAsync(w.host_active + slot, w.active, sizeof(int), hipMemcpyDeviceToHost,
-                           stream) != hipSuccess ||
-            hipEventRecord(w.polled[slot], stream) != hipSuccess)
-            return kErrorDevice;
-        return 0;
-    };
-    if (hipMemcpyAsync(w.host_active, w.active, sizeof(int), hipMemcpyDeviceToHost, stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess)
-        return kErrorDevice;
-    int rounds = 0;
-    bool expired = false;
-    if (w.host_active[0] != 0) {
-        if (group(0)) return kErrorDevice;
-        rounds += kRoundsPerPoll;
-        for (int g = 1;; ++g) {
-            if (time_limit > 0.0 && elapsed() > time_limit) {
-                expired = true;
-                break;
-            }
-            if (group(g & 1)) return kErrorDevice;
-            rounds += kRoundsPerPoll;
-            if (hipEventSynchronize(w.polled[(g - 1) & 1]) != hipSuccess) return kErrorDevice;
-            if (w.host_active[(g - 1) & 1] == 0) break;
-        }
-    }
-    if (hipStreamSynchronize(stream) != hipSuccess) return kErrorDevice;
-    if (expired && (w.expire(stream) || hipStreamSynchronize(stream) != hipSuccess))
-        return kErrorDevice;
-    if (stats) {
-        stats->rounds = rounds;
-        stats->seconds = elapsed();
-    }
+    REFORM_HIP_CHECK(
+        hipMemcpyAsync(w->host_active, w->active, sizeof(int), hipMemcpyDeviceToHost, stream));
+    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+    if (w->timing && w->collect(2)) return kErrorDevice;
+    finished = w->host_active[0] == 0;
     return 0;
 }
 
-}  // namespace
+int WideRun::advance() {
+    if (finished) return 0;
+    int changed = 0;
+    bool drained = false;
+    while (retired < queued) {
+        const int slot = retired & 1;
+        const hipError_t q = hipEventQuery(w->polled[slot]);
+        if (q == hipErrorNotReady) break;
+        REFORM_HIP_CHECK(q);
+        if (w->timing && w->collect(slot)) return kErrorDevice;
+        ++retired;
+        changed = 1;
+        if (w->host_active[slot] == 0) {
+            drained = true;
+            break;
+        }
+    }
+    const double elapsed =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (drained || (time_limit > 0.0 && elapsed > time_limit)) {
+        REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+        if (!drained) {
+            if (w->expire(stream)) return kErrorDevice;
+            REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+        }
+        finished = true;
+        stats.rounds = queued * (w->timing ? 1 : kRoundsPerPoll);
+        stats.seconds = elapsed;
+        return 1;
+    }
+    while (queued - retired < 2) {
+        const int slot = queued & 1;
+        if (w->timing) {  // one round per group, its marks in the group's event set
+            w->mark_set = slot;
+            w->marks_used[slot] = 0;
+            w->mark_kinds[slot].clear();
+            if (w->enqueue_round(stream)) return kErrorDevice;
+        } else
+            for (int r = 0; r < kRoundsPerPoll; ++r)
+                REFORM_HIP_CHECK(hipGraphLaunch(w->exec, stream));
+        REFORM_HIP_CHECK(hipMemcpyAsync(w->host_active + slot, w->active, sizeof(int),
+                                        hipMemcpyDeviceToHost, stream));
+        REFORM_HIP_CHECK(hipEventRecord(w->polled[slot], stream));
+        ++queued;
+        changed = 1;
+    }
+    return changed;
+}
+
+int WideControl::solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats) {
+    WideRun run;
+    if (run.begin(this, stream, resume, time_limit)) return kErrorDevice;
+    while (!run.finished) {
+        const int st = run.advance();
+        if (st < 0) return st;
+        if (!st) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    if (stats) *stats = run.stats;
+    return 0;
+}
 
 // ---------------------------------------------------------------- control --
 
 WideControl::~WideControl() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (auto& set : marks)
+        for (hipEvent_t e : set) (void)hipEventDestroy(e);
     for (auto& e : polled)
         if (e) (void)hipEventDestroy(e);
     if (host_active) (void)hipHostFree(host_active);
     if (arena) (void)hipFree(arena);
+}
+
+int WideControl::mark(hipStream_t stream, int kind, bool begin) {
+    if (!timing) return 0;
+    std::vector<hipEvent_t>& set = marks[mark_set];
+    const int k = marks_used[mark_set]++;
+    if (k == int(set.size())) {
+        hipEvent_t e;
+        REFORM_HIP_CHECK(hipEventCreate(&e));
+        set.push_back(e);
+    }
+    if (begin) mark_kinds[mark_set].push_back(kind);
+    REFORM_HIP_CHECK(hipEventRecord(set[k], stream));
+    return 0;
+}
+
+int WideControl::collect(int set) {
+    const std::vector<int>& kinds = mark_kinds[set];
+    for (size_t p = 0; p < kinds.size(); ++p) {
+        float ms = 0.0f;
+        REFORM_HIP_CHECK(hipEventElapsedTime(&ms, marks[set][2 * p], marks[set][2 * p + 1]));
+        kernel_ms[kinds[p]] += ms;
+        launches[kinds[p]] += 1;
+    }
+    return 0;
+}
+
+void WideControl::clear_timing() {
+    for (int k = 0; k < kKinds; ++k) {
+        kernel_ms[k] = 0.0;
+        launches[k] = 0;
+    }
+}
+
+void WideControl::set_timing(bool on) { timing = on; }
+
+int WideControl::algorithmic_bytes(double (&out)[kKinds]) const {
+    std::vector<int64_t> n(count);
+    if (count) REFORM_HIP_CHECK(hipMemcpy(n.data(), steps, sizeof(int64_t) * count,
+                                          hipMemcpyDeviceToHost));
+    for (int k = 0; k < kKinds; ++k) {
+        out[k] = 0.0;
+        for (int q = 0; q < count && q < int(step_bytes[k].size()); ++q)
+            out[k] += double(n[q]) * step_bytes[k][q];
+    }
+    return 0;
 }
 
 int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_width) {
@@ -450,27 +538,38 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     at = align256(at + sizeof(int) * 4 * size_t(count));
     const size_t o_partial = at;
     at = align256(at + sizeof(double) * partial_width * (partial_rows + 1));
+    const size_t o_steps = at;
+    at = align256(at + sizeof(int64_t) * size_t(count));
     const size_t o_ctrl = at;
     at = align256(at + sizeof(int) * (3 * size_t(count) + 1));
-    if (hipMalloc(&arena, at) != hipSuccess) {
-        arena = nullptr;
-        return kErrorDevice;
+    {
+        hipError_t e = hipMalloc(&arena, at);
+        if (e != hipSuccess) {
+            arena = nullptr;
+            record_error("hipMalloc(wide arena)", e, __FILE__, __LINE__);
+            return kErrorDevice;
+        }
     }
     bytes = at;
     char* b = static_cast<char*>(arena);
     first = reinterpret_cast<int*>(b + o_first);
     partial = reinterpret_cast<double*>(b + o_partial);
+    steps = reinterpret_cast<int64_t*>(b + o_steps);
     round_action = reinterpret_cast<int*>(b + o_ctrl);
     last_action = round_action + count;
     pending = last_action + count;
     active = pending + count;
-    if (hipHostMalloc(reinterpret_cast<void**>(&host_active), 2 * sizeof(int),
-                      hipHostMallocDefault) != hipSuccess) {
-        host_active = nullptr;
-        return kErrorDevice;
+    {
+        hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host_active), 2 * sizeof(int),
+                                     hipHostMallocDefault);
+        if (e != hipSuccess) {
+            host_active = nullptr;
+            record_error("hipHostMalloc(active count)", e, __FILE__, __LINE__);
+            return kErrorDevice;
+        }
     }
     for (auto& e : polled)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return kErrorDevice;
+        REFORM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return 0;
 }
 
@@ -498,15 +597,46 @@ int WidePraq<Real>::setup(const std::vector<PraqJob<Real>>& host_jobs, PraqJob<R
     }
     node_blocks = int(nt.size());
     edge_blocks = int(et.size());
+    // algorithmic bytes of one step, b = sizeof(Real): node kernel: ptr 4,
+    // six per-node values, the reflected gather, and (index 4 + value b) per
+    // half-edge of the CSR scatter; edge kernel: ends 8, weight, the y pair,
+    // the anchor pair and the written pair
+    const double b = sizeof(Real);
+    for (int k = 0; k < kKinds; ++k) step_bytes[k].resize(count);
+    for (int q = 0; q < count; ++q) {
+        const double n = host_jobs[q].a.n, m = host_jobs[q].a.m;
+        step_bytes[0][q] = n * (4.0 + 7.0 * b) + 2.0 * m * (4.0 + b);
+        step_bytes[1][q] = m * (8.0 + 7.0 * b);
+    }
     const size_t tab_bytes = sizeof(int2) * (nt.size() + et.size());
     if (allocate(tab_bytes, nt.size() + et.size(), kPraqSums)) return kErrorDevice;
     node_tab = static_cast<int2*>(arena);
     edge_tab = node_tab + nt.size();
     std::vector<int2> tabs(nt);
     tabs.insert(tabs.end(), et.begin(), et.end());
-    if (hipMemcpy(node_tab, tabs.data(), tab_bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return kErrorDevice;
+    REFORM_HIP_CHECK(hipMemcpy(node_tab, tabs.data(), tab_bytes, hipMemcpyHostToDevice));
+    REFORM_HIP_CHECK(hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// One step of every instance the mode selects: the node kernel, then the
+// edge kernel, each bracketed by timing marks.
+template <class Real>
+int WidePraq<Real>::step(hipStream_t stream, int mode, int j) {
+    const dim3 blk(kWideBlock);
+    if (node_blocks) {
+        if (mark(stream, 0, true)) return kErrorDevice;
+        hipLaunchKernelGGL(wpraq_node<Real>, dim3(node_blocks), blk, 0, stream, jobs, node_tab,
+                           round_action, mode, j);
+        if (mark(stream, 0, false)) return kErrorDevice;
+    }
+    if (edge_blocks) {
+        if (mark(stream, 1, true)) return kErrorDevice;
+        hipLaunchKernelGGL(wpraq_edge<Real>, dim3(edge_blocks), blk, 0, stream, jobs, edge_tab,
+                           round_action, mode, j);
+        if (mark(stream, 1, false)) return kErrorDevice;
+    }
+    REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
@@ -515,25 +645,21 @@ int WidePraq<Real>::start(hipStream_t stream, bool resume) {
     const int all = node_blocks + edge_blocks;
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_init<PraqJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
-                       count, round_action, last_action, resume ? 1 : 0);
+                       count, round_action, last_action, steps, resume ? 1 : 0);
     if (!resume) {
         hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab,
                            node_blocks, edge_tab, pending, 3);
-        if (node_blocks)
-            hipLaunchKernelGGL(wpraq_node<Real>, dim3(node_blocks), blk, 0, stream, jobs, node_tab,
-                               round_action, 0, 0);
-        if (edge_blocks)
-            hipLaunchKernelGGL(wpraq_edge<Real>, dim3(edge_blocks), blk, 0, stream, jobs, edge_tab,
-                               round_action, 0, 0);
+        if (step(stream, 0, 0)) return kErrorDevice;
         hipLaunchKernelGGL(wpraq_sums<Real>, dim3(all), blk, 0, stream, jobs, node_tab,
                            node_blocks, edge_tab, round_action, 0, partial);
         hipLaunchKernelGGL(wpraq_logic<Real>, dim3(count), blk, 0, stream, jobs, first,
-                           node_blocks, partial, round_action, last_action, 0);
+                           node_blocks, partial, round_action, last_action, steps, 0);
         hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab,
                            node_blocks, edge_tab, pending, 2);
     }
     hipLaunchKernelGGL(wide_count<PraqJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
-    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 template <class Real>
@@ -544,35 +670,25 @@ int WidePraq<Real>::enqueue_round(hipStream_t stream) {
                        round_action, pending);
     hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
                        edge_tab, pending, 0);
-    for (int j = 0; j < op_steps; ++j) {
-        if (node_blocks)
-            hipLaunchKernelGGL(wpraq_node<Real>, dim3(node_blocks), blk, 0, stream, jobs, node_tab,
-                               round_action, 0, j);
-        if (edge_blocks)
-            hipLaunchKernelGGL(wpraq_edge<Real>, dim3(edge_blocks), blk, 0, stream, jobs, edge_tab,
-                               round_action, 0, j);
-    }
+    for (int j = 0; j < op_steps; ++j)
+        if (step(stream, 0, j)) return kErrorDevice;
     hipLaunchKernelGGL(wpraq_sums<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
                        edge_tab, round_action, 0, partial);
     hipLaunchKernelGGL(wpraq_logic<Real>, dim3(count), blk, 0, stream, jobs, first, node_blocks,
-                       partial, round_action, last_action, 0);
+                       partial, round_action, last_action, steps, 0);
     // restarts: one uncounted vanilla step, the new anchor, the norms
-    if (node_blocks)
-        hipLaunchKernelGGL(wpraq_node<Real>, dim3(node_blocks), blk, 0, stream, jobs, node_tab,
-                           round_action, 1, 0);
-    if (edge_blocks)
-        hipLaunchKernelGGL(wpraq_edge<Real>, dim3(edge_blocks), blk, 0, stream, jobs, edge_tab,
-                           round_action, 1, 0);
+    if (step(stream, 1, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
                        edge_tab, pending, 1);
     hipLaunchKernelGGL(wpraq_sums<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
                        edge_tab, round_action, 1, partial);
     hipLaunchKernelGGL(wpraq_logic<Real>, dim3(count), blk, 0, stream, jobs, first, node_blocks,
-                       partial, round_action, last_action, 1);
+                       partial, round_action, last_action, steps, 1);
     hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
                        edge_tab, pending, 2);
     hipLaunchKernelGGL(wide_count<PraqJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
-    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 template <class Real>
@@ -583,12 +699,8 @@ int WidePraq<Real>::expire(hipStream_t stream) {
     hipLaunchKernelGGL(wpraq_copy<Real>, dim3(node_blocks + edge_blocks), blk, 0, stream, jobs,
                        node_tab, node_blocks, edge_tab, pending, 2);
     hipLaunchKernelGGL(wide_count<PraqJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
-    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
-}
-
-template <class Real>
-int WidePraq<Real>::solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats) {
-    return wide_drive(*this, stream, resume, time_limit, stats);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 // ------------------------------------------------------------- PRIA host --
@@ -622,78 +734,91 @@ int WidePria<Real>::setup(const std::vector<PriaJob<Real>>& host_jobs, PriaJob<R
     }
     x_blocks = int(xt.size());
     s_blocks = int(st.size());
+    // algorithmic bytes of one step per entry, b = sizeof(Real): x kernel
+    // reads x, x0, h, y, z and writes x, r (7 b); y/z kernel reads y, y0, r,
+    // z, z0, r and writes y, z (8 b)
+    const double b = sizeof(Real);
+    for (int k = 0; k < kKinds; ++k) step_bytes[k].resize(count);
+    for (int q = 0; q < count; ++q) {
+        const double nd = host_jobs[q].a.ND;
+        step_bytes[0][q] = 7.0 * b * nd;
+        step_bytes[1][q] = 8.0 * b * nd;
+    }
     const size_t x_bytes = align256(sizeof(int4) * xt.size());
     const size_t tab_bytes = x_bytes + sizeof(int2) * st.size();
     if (allocate(tab_bytes, xt.size() + st.size(), kPriaSums)) return kErrorDevice;
     x_tab = static_cast<int4*>(arena);
     s_tab = reinterpret_cast<int2*>(static_cast<char*>(arena) + x_bytes);
-    if (hipMemcpy(x_tab, xt.data(), sizeof(int4) * xt.size(), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(s_tab, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return kErrorDevice;
+    REFORM_HIP_CHECK(hipMemcpy(x_tab, xt.data(), sizeof(int4) * xt.size(), hipMemcpyHostToDevice));
+    REFORM_HIP_CHECK(hipMemcpy(s_tab, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice));
+    REFORM_HIP_CHECK(hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice));
     const size_t lds = pria_x_lds_bytes<Real>(max_T);
-    if (lds > 65536 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(wpria_x<Real>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
-        return kErrorDevice;
+    if (lds > 65536)
+        REFORM_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(wpria_x<Real>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             int(lds)));
+    return 0;
+}
+
+// One step of every instance the mode selects: the x kernel, then the y/z
+// kernel, each bracketed by timing marks.
+template <class Real>
+int WidePria<Real>::step(hipStream_t stream, int mode, int j) {
+    const size_t lds = pria_x_lds_bytes<Real>(max_T);
+    if (mark(stream, 0, true)) return kErrorDevice;
+    hipLaunchKernelGGL(wpria_x<Real>, dim3(x_blocks), dim3(kWideBlock), lds, stream, jobs, x_tab,
+                       round_action, mode, j);
+    if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
+    hipLaunchKernelGGL(wpria_yz<Real>, dim3(s_blocks), dim3(1024), 0, stream, jobs, s_tab,
+                       round_action, mode, j);
+    if (mark(stream, 1, false)) return kErrorDevice;
+    REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
 template <class Real>
 int WidePria<Real>::start(hipStream_t stream, bool resume) {
-    const size_t lds = pria_x_lds_bytes<Real>(max_T);
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_init<PriaJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
-                       count, round_action, last_action, resume ? 1 : 0);
+                       count, round_action, last_action, steps, resume ? 1 : 0);
     if (!resume) {
         hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending,
                            3);
-        hipLaunchKernelGGL(wpria_x<Real>, dim3(x_blocks), blk, lds, stream, jobs, x_tab,
-                           round_action, 0, 0);
-        hipLaunchKernelGGL(wpria_yz<Real>, dim3(s_blocks), dim3(1024), 0, stream, jobs, s_tab,
-                           round_action, 0, 0);
+        if (step(stream, 0, 0)) return kErrorDevice;
         hipLaunchKernelGGL(wpria_sums<Real>, dim3(s_blocks + x_blocks), blk, 0, stream, jobs, s_tab,
                            s_blocks, x_tab, round_action, 0, partial);
         hipLaunchKernelGGL(wpria_logic<Real>, dim3(count), blk, 0, stream, jobs, first, s_blocks,
-                           partial, round_action, last_action, 0);
+                           partial, round_action, last_action, steps, 0);
         hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending,
                            2);
     }
     hipLaunchKernelGGL(wide_count<PriaJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
-    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 template <class Real>
 int WidePria<Real>::enqueue_round(hipStream_t stream) {
-    const size_t lds = pria_x_lds_bytes<Real>(max_T);
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_pre<PriaJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs, count,
                        round_action, pending);
     hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 0);
-    for (int j = 0; j < op_steps; ++j) {
-        hipLaunchKernelGGL(wpria_x<Real>, dim3(x_blocks), blk, lds, stream, jobs, x_tab,
-                           round_action, 0, j);
-        hipLaunchKernelGGL(wpria_yz<Real>, dim3(s_blocks), dim3(1024), 0, stream, jobs, s_tab,
-                           round_action, 0, j);
-    }
+    for (int j = 0; j < op_steps; ++j)
+        if (step(stream, 0, j)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_sums<Real>, dim3(s_blocks + x_blocks), blk, 0, stream, jobs, s_tab,
                        s_blocks, x_tab, round_action, 0, partial);
     hipLaunchKernelGGL(wpria_logic<Real>, dim3(count), blk, 0, stream, jobs, first, s_blocks,
-                       partial, round_action, last_action, 0);
-    hipLaunchKernelGGL(wpria_x<Real>, dim3(x_blocks), blk, lds, stream, jobs, x_tab, round_action,
-                       1, 0);
-    hipLaunchKernelGGL(wpria_yz<Real>, dim3(s_blocks), dim3(1024), 0, stream, jobs, s_tab,
-                       round_action, 1, 0);
+                       partial, round_action, last_action, steps, 0);
+    if (step(stream, 1, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 1);
     hipLaunchKernelGGL(wpria_sums<Real>, dim3(s_blocks + x_blocks), blk, 0, stream, jobs, s_tab,
                        s_blocks, x_tab, round_action, 1, partial);
     hipLaunchKernelGGL(wpria_logic<Real>, dim3(count), blk, 0, stream, jobs, first, s_blocks,
-                       partial, round_action, last_action, 1);
+                       partial, round_action, last_action, steps, 1);
     hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 2);
     hipLaunchKernelGGL(wide_count<PriaJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
-    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 template <class Real>
@@ -703,12 +828,8 @@ int WidePria<Real>::expire(hipStream_t stream) {
                        count, last_action);
     hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 2);
     hipLaunchKernelGGL(wide_count<PriaJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
-    return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
-}
-
-template <class Real>
-int WidePria<Real>::solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats) {
-    return wide_drive(*this, stream, resume, time_limit, stats);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 template struct WidePraq<float>;

@@ -29,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <vector>
 
 #include "praq_host.hpp"
@@ -58,16 +59,68 @@ struct WideControl {
     int* active = nullptr;         // [1] instances neither finished nor stalled
     int* host_active = nullptr;    // [2] pinned
     hipEvent_t polled[2] = {nullptr, nullptr};
+    int64_t* steps = nullptr;      // [count] steps each instance took in this engine
     void* arena = nullptr;
     size_t bytes = 0;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
 
-    ~WideControl();
+    // Launch timing of the two step kernels of the family (PRAQ node, edge;
+    // PRIA x, yz): with `timing` on, rounds are launched directly rather
+    // than replayed, every step launch bracketed by a pair of HIP events on
+    // its stream; two event sets alternate so one is read while the other's
+    // round runs.
+    static constexpr int kKinds = 2;
+    bool timing = false;
+    std::vector<hipEvent_t> marks[3];   // round sets 0, 1; set 2: the start
+    std::vector<int> mark_kinds[3];     // kind of each (begin, end) pair
+    int marks_used[3] = {0, 0, 0};
+    int mark_set = 0;
+    double kernel_ms[kKinds] = {0.0, 0.0};
+    int64_t launches[kKinds] = {0, 0};
+    std::vector<double> step_bytes[kKinds];  // algorithmic bytes of one step, per instance
+
+    virtual ~WideControl();
+    // bracket one step launch of `kind` (no-op unless timing)
+    int mark(hipStream_t stream, int kind, bool begin);
+    // add the event pairs of a completed set to kernel_ms / launches
+    int collect(int set);
+    // turn timing on or off (the totals of the last solve stay readable)
+    void set_timing(bool on);
+    void clear_timing();
+    // algorithmic bytes moved by each step kernel over the last solve
+    int algorithmic_bytes(double (&out)[kKinds]) const;
+    // the cold start (or the continuation of handed-over states) and the
+    // first count of active instances
+    virtual int start(hipStream_t stream, bool resume) = 0;
+    // one round's launches (captured into the graph)
+    virtual int enqueue_round(hipStream_t stream) = 0;
+    // out of time: every active instance ends at its current point
+    virtual int expire(hipStream_t stream) = 0;
+    // start, then rounds until no instance is active
+    int solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats);
     // arena: the caller's tables (table_bytes) followed by the control arrays
     int allocate(size_t table_bytes, size_t partial_rows, int partial_width);
 };
 
+// One engine's solve advanced in steps, so that one host thread can keep
+// several engines on different streams busy: at most two groups of rounds
+// are queued, each followed by a copy of the active count into its pinned
+// slot and an event; advance() retires the groups whose event has
+// completed, without blocking, and queues more while instances remain.
+struct WideRun {
+    WideControl* w = nullptr;
+    hipStream_t stream = nullptr;
+    double time_limit = 0.0;
+    bool finished = true;
+    int queued = 0, retired = 0;   // groups of rounds
+    std::chrono::steady_clock::time_point t0;
+    WideStats stats;
+
+    int begin(WideControl* engine, hipStream_t s, bool resume, double limit);
+    // returns < 0 on error, else whether anything changed
+    int advance();
+};
 // PRAQ: first = (node block range, edge block range) per instance; partial
 // rows (kPraqSums each) per node block, then per edge block.
 template <class Real>
@@ -78,10 +131,10 @@ struct WidePraq : WideControl {
     int node_blocks = 0, edge_blocks = 0;
 
     int setup(const std::vector<PraqJob<Real>>& host_jobs, PraqJob<Real>* dev_jobs);
-    int start(hipStream_t stream, bool resume);
-    int enqueue_round(hipStream_t stream);
-    int expire(hipStream_t stream);
-    int solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats);
+    int step(hipStream_t stream, int mode, int j);
+    int start(hipStream_t stream, bool resume) override;
+    int enqueue_round(hipStream_t stream) override;
+    int expire(hipStream_t stream) override;
 };
 
 // PRIA: first = (slice range, column-tile range) per instance; partial rows
@@ -95,10 +148,10 @@ struct WidePria : WideControl {
     int x_blocks = 0, s_blocks = 0;
 
     int setup(const std::vector<PriaJob<Real>>& host_jobs, PriaJob<Real>* dev_jobs);
-    int start(hipStream_t stream, bool resume);
-    int enqueue_round(hipStream_t stream);
-    int expire(hipStream_t stream);
-    int solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats);
+    int step(hipStream_t stream, int mode, int j);
+    int start(hipStream_t stream, bool resume) override;
+    int enqueue_round(hipStream_t stream) override;
+    int expire(hipStream_t stream) override;
 };
 
 }  // namespace reform
