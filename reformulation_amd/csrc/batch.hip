@@ -262,11 +262,11 @@ int BatchImpl::upload() {
         }
     }
     for (size_t q = 0; q < pria.size(); ++q) {
-        rl64[q] = pria_layout(pria[q].host.N, pria[q].host.ND, sizeof(double));
+        rl64[q] = pria_layout(pria[q].host.N, pria[q].host.T, pria[q].host.ND, sizeof(double));
         pria[q].off64 = at;
         at += align256(rl64[q].total_bytes);
         if (f32) {
-            rl32[q] = pria_layout(pria[q].host.N, pria[q].host.ND, sizeof(float));
+            rl32[q] = pria_layout(pria[q].host.N, pria[q].host.T, pria[q].host.ND, sizeof(float));
             pria[q].off32 = at;
             at += align256(rl32[q].total_bytes);
         }

@@ -81,6 +81,7 @@ __global__ void __launch_bounds__(1024) pria_wg_solve(const PriaJob<Real>* jobs)
         pria_wg_copy3(a.x0, a.xs, a.y0, a.ys, a.z0, a.zs, a.ND);
         pria_wg_copy3(a.x, a.xs, a.y, a.ys, a.z, a.zs, a.ND);
         pria_wg_copy3(a.sx, a.xs, a.sy, a.ys, a.sz, a.zs, a.ND);
+        pria_reset_thresholds(a, threadIdx.x, blockDim.x);
         if (threadIdx.x == 0) logic_init(st, p);
         __syncthreads();
         pria_wg_step(a, pria_step_scalars<Real>(0, p.omega, p.eta, false, true), lds, red);
@@ -166,7 +167,7 @@ __global__ void __launch_bounds__(256) pria_x_kernel(PriaArrays<Real> a, PriaSte
 
 template <class Real>
 __global__ void __launch_bounds__(1024) pria_yz_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
-    __shared__ double red[2 * 16];
+    __shared__ double red[4 * 16];
     pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, blockIdx.x, red);
 }
 
@@ -205,6 +206,11 @@ __global__ void pria_sums_final(const double* slices, int nslices, const double*
     block_sums<kPriaSums>(acc, red);
     if (threadIdx.x == 0)
         for (int q = 0; q < kPriaSums; ++q) out[q] = acc[q];
+}
+
+template <class Real>
+__global__ void pria_reset_thresholds_kernel(PriaArrays<Real> a) {
+    pria_reset_thresholds(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 template <class Real>
@@ -258,6 +264,12 @@ struct PriaDevice {
     double* cols;
     double* dsums;
 
+    int reset_thresholds() {
+        const int n = a.N * a.T;
+        hipLaunchKernelGGL(pria_reset_thresholds_kernel<Real>, dim3((n + 255) / 256), dim3(256), 0,
+                           stream, a);
+        return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+    }
     int step(const PriaStep<Real>& st) {
         constexpr int TW = PriaShape<Real>::TW;
         const int tiles = (a.T + TW - 1) / TW;
@@ -310,7 +322,8 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
         st.stalled = 0;
     } else {
         if (d.copy3(a.x0, a.xs, a.y0, a.ys, a.z0, a.zs) ||
-            d.copy3(a.x, a.xs, a.y, a.ys, a.z, a.zs) || d.copy3(a.sx, a.xs, a.sy, a.ys, a.sz, a.zs))
+            d.copy3(a.x, a.xs, a.y, a.ys, a.z, a.zs) ||
+            d.copy3(a.sx, a.xs, a.sy, a.ys, a.sz, a.zs) || d.reset_thresholds())
             return kPriaErrorDevice;
         logic_init(st, p);
         if (d.step(pria_step_scalars<Real>(0, p.omega, p.eta, false, true))) return kPriaErrorDevice;
@@ -383,6 +396,11 @@ __global__ void __launch_bounds__(1024) pria_wg_step_kernel(PriaArrays<Real> a, 
 template <class Real>
 int pria_one_step(const PriaArrays<Real>& a, double omega, double eta, int tier, hipStream_t stream) {
     PriaStep<Real> st = pria_step_scalars<Real>(0, omega, eta, false, true);
+    {
+        const int n = a.N * a.T;
+        hipLaunchKernelGGL(pria_reset_thresholds_kernel<Real>, dim3((n + 255) / 256), dim3(256), 0,
+                           stream, a);
+    }
     if (tier == 1) {
         size_t lds = pria_x_lds_bytes<Real>(a.T);
         if (lds > 65536 &&
@@ -409,6 +427,11 @@ template int pria_device_solve<double>(const PriaJob<double>&, LogicState&, doub
 // precision switch: the whole Halpern state, FP32 -> FP64
 __device__ inline void pria_convert_one(const PriaArrays<float>& f, const PriaArrays<double>& d,
                                         int e) {
+    if (e < f.N * f.T) {
+        d.thx[e] = f.thx[e];
+        d.thy[e] = f.thy[e];
+    }
+    if (e < f.T) d.thz[e] = f.thz[e];
     d.x0[e] = f.x0[e];
     d.y0[e] = f.y0[e];
     d.z0[e] = f.z0[e];

@@ -67,7 +67,7 @@ struct PriaImage {
     size_t job_off = 0, state_off = 0, work_off = 0;
 
     int init(const PriaHostInstance& s, const LogicParams& lp, hipStream_t stream) {
-        L = pria_layout(s.N, s.ND, sizeof(Real));
+        L = pria_layout(s.N, s.T, s.ND, sizeof(Real));
         job_off = pria_align256(L.total_bytes);
         state_off = pria_align256(job_off + sizeof(PriaJob<Real>));
         work_off = pria_align256(state_off + sizeof(LogicState));
@@ -270,7 +270,7 @@ int solve_batch(int64_t count, int64_t n, int64_t t, const Real* k, const Real* 
     // every instance as one workgroup of one launch (same code as the solo
     // workgroup tier, so each result is what a solo workgroup solve gives)
     const PriaHostInstance& s0 = hosts[0];
-    PriaLayout L = pria_layout(s0.N, s0.ND, sizeof(Real));
+    PriaLayout L = pria_layout(s0.N, s0.T, s0.ND, sizeof(Real));
     const size_t per = pria_align256(L.total_bytes);
     const size_t jobs_off = per * count;
     const size_t states_off = pria_align256(jobs_off + sizeof(PriaJob<Real>) * count);
@@ -420,7 +420,7 @@ int debug_step(int64_t n, int64_t t, const double* k, double k0, const double* h
     for (auto& v : s.k) v *= s.scale;
     s.k0 *= s.scale;
     s.scale = 1.0;
-    PriaLayout L = pria_layout(s.N, s.ND, sizeof(Real));
+    PriaLayout L = pria_layout(s.N, s.T, s.ND, sizeof(Real));
     Buffer buf;
     if (hipMalloc(&buf.ptr, L.total_bytes) != hipSuccess) return kPriaErrorDevice;
     char* base = static_cast<char*>(buf.ptr);

@@ -26,6 +26,15 @@
 // (sum of active - target) / |active|, repeated until the active set stops
 // shrinking; the same threshold the sort-based double_simplex_projection
 // (jrp/pria-jax/pria/projection.py:71-130) computes.  A target <= 0 gives 0.
+// Each segment keeps its last threshold (thx per column, thy per row, thz
+// per slice; NaN = none) and the next step starts from the active set
+// {v > last}: if that set lies inside the solution's support its theta is a
+// lower bound of the answer, so one more pass lands on a superset of the
+// support and the usual monotone shrinking follows; from a set that already
+// contains the support the usual argument applies.  The answer is theta of
+// the final active set either way, so the start changes the number of passes,
+// not the result; between PDHG steps the set rarely changes and two passes
+// usually suffice (against about seven from the full set on x columns).
 #pragma once
 
 #include "common.hpp"
@@ -45,6 +54,7 @@ struct PriaArrays {
     Real *yc, *y0, *y, *sy;
     Real *zc, *z0, *z, *sz;
     Real* r;
+    double *thx, *thy, *thz;    // last thresholds: [N T'] columns, [N T'] rows, [T'] slices
 };
 
 __host__ __device__ inline int pria_off(int N, int T, int s) {
@@ -73,6 +83,18 @@ __host__ __device__ inline PriaStep<Real> pria_step_scalars(int64_t restart_iter
     s.reflect = reflect;
     s.store = store || !reflect;
     return s;
+}
+
+// The thresholds of a fresh solve: none (NaN), so the first projection of
+// each segment starts from its full set; entries k, k + stride, ...
+template <class Real>
+__device__ inline void pria_reset_thresholds(const PriaArrays<Real>& a, int k, int stride) {
+    const double none = __builtin_nan("");
+    for (int e = k; e < a.N * a.T; e += stride) {
+        a.thx[e] = none;
+        a.thy[e] = none;
+    }
+    for (int e = k; e < a.T; e += stride) a.thz[e] = none;
 }
 
 // --- x pass: one tile of TW consecutive columns t0.. of item i ------------
@@ -112,14 +134,27 @@ __device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st,
         const int c = wave * cpw + (lane % cpw), q = lane / cpw;
         const int t = t0 + c;
         const int len = c < tcount ? t + 1 : 0;
-        double S = 0.0, C = 0.0;
+        const double start = len > 0 ? a.thx[i * T + t] : 0.0;
+        double S = 0.0, C = 0.0, Sw = 0.0, Cw = 0.0;
         for (int s = q; s < len; s += G) {
-            S += double(lds[s * stride + c]);
+            const double v = double(lds[s * stride + c]);
+            S += v;
             C += 1.0;
+            if (v > start) {
+                Sw += v;
+                Cw += 1.0;
+            }
         }
         for (int off = cpw; off < kWave; off <<= 1) {
             S += __shfl_xor(S, off, kWave);
             C += __shfl_xor(C, off, kWave);
+            Sw += __shfl_xor(Sw, off, kWave);
+            Cw += __shfl_xor(Cw, off, kWave);
+        }
+        bool first = Cw > 0.0;  // a start (NaN compares false: none)
+        if (first) {
+            S = Sw;
+            C = Cw;
         }
         double th = len > 0 ? (S - 1.0) / C : 0.0;
         bool done = len == 0;
@@ -139,15 +174,17 @@ __device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st,
                 C2 += __shfl_xor(C2, off, kWave);
             }
             if (!done) {
-                if (C2 >= C || C2 == 0.0) {
+                if (C2 == C || C2 == 0.0 || (C2 > C && !first)) {
                     done = true;
                 } else {
                     C = C2;
                     th = (S2 - 1.0) / C2;
                 }
+                first = false;
             }
         }
         if (q == 0 && c < TW) theta[c] = Real(th);
+        if (q == 0 && len > 0) a.thx[i * T + t] = th;
     }
     __syncthreads();
     // write: x2, r, the next xc (and x, sx when storing)
@@ -175,21 +212,34 @@ __device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st,
 }
 
 // Michelot threshold over one wavefront's register-held values (VR per lane),
-// target `target`; len entries are valid (lane-major: e = lane + 64 k).
+// target `target`; len entries are valid (lane-major: e = lane + 64 k); the
+// first active set is {u > start} when that is not empty, else all.
 template <int VR>
-__device__ __forceinline__ double wave_michelot(const double (&u)[VR], int len, double target) {
+__device__ __forceinline__ double wave_michelot(const double (&u)[VR], int len, double target,
+                                                double start) {
     const int lane = threadIdx.x & (kWave - 1);
-    double S = 0.0, C = 0.0;
+    double S = 0.0, C = 0.0, Sw = 0.0, Cw = 0.0;
 #pragma unroll
     for (int k = 0; k < VR; ++k) {
         int e = lane + kWave * k;
         if (e < len) {
             S += u[k];
             C += 1.0;
+            if (u[k] > start) {
+                Sw += u[k];
+                Cw += 1.0;
+            }
         }
     }
     S = wave_sum(S);
     C = wave_sum(C);
+    Sw = wave_sum(Sw);
+    Cw = wave_sum(Cw);
+    bool first = Cw > 0.0;
+    if (first) {
+        S = Sw;
+        C = Cw;
+    }
     double th = (S - target) / C;
     for (int round = 0; round <= len; ++round) {
         double S2 = 0.0, C2 = 0.0;
@@ -203,9 +253,10 @@ __device__ __forceinline__ double wave_michelot(const double (&u)[VR], int len, 
         }
         S2 = wave_sum(S2);
         C2 = wave_sum(C2);
-        if (C2 >= C || C2 == 0.0) break;
+        if (C2 == C || C2 == 0.0 || (C2 > C && !first)) break;
         C = C2;
         th = (S2 - target) / C2;
+        first = false;
     }
     return th;
 }
@@ -242,7 +293,11 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
             }
         }
         const double target = double(a.k[i]);
-        const double th = target > 0.0 ? wave_michelot<VR>(u, len, target) : 0.0;
+        double th = 0.0;
+        if (target > 0.0) {
+            th = wave_michelot<VR>(u, len, target, a.thy[i * T + s]);
+            if (lane == 0) a.thy[i * T + s] = th;
+        }
 #pragma unroll
         for (int k = 0; k < VR; ++k) {
             int e = lane + kWave * k;
@@ -266,23 +321,28 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
     const double target = double(a.k0);
     double th = 0.0;
     if (target > 0.0) {
+        const double start = a.thz[s];
+        bool first;
         if (L <= BS * VZ) {
             double u[VZ];
-            double S = 0.0, C = 0.0;
+            double v4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < VZ; ++k) {
                 int e = tid + BS * k;
                 u[k] = 0.0;
                 if (e < L) {
                     u[k] = double(Real(srcz[base + e] + a.r[base + e]));
-                    S += u[k];
-                    C += 1.0;
+                    v4[0] += u[k];
+                    v4[1] += 1.0;
+                    if (u[k] > start) {
+                        v4[2] += u[k];
+                        v4[3] += 1.0;
+                    }
                 }
             }
-            double v2[2] = {S, C};
-            block_sums<2>(v2, red);
-            S = v2[0];
-            C = v2[1];
+            block_sums<4>(v4, red);
+            first = v4[3] > 0.0;
+            double S = first ? v4[2] : v4[0], C = first ? v4[3] : v4[1];
             th = (S - target) / C;
             for (int round = 0; round <= L; ++round) {
                 double S2 = 0.0, C2 = 0.0;
@@ -296,20 +356,25 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
                 }
                 double w2[2] = {S2, C2};
                 block_sums<2>(w2, red);
-                if (w2[1] >= C || w2[1] == 0.0) break;
+                if (w2[1] == C || w2[1] == 0.0 || (w2[1] > C && !first)) break;
                 C = w2[1];
                 th = (w2[0] - target) / w2[1];
+                first = false;
             }
         } else {
-            double S = 0.0, C = 0.0;
+            double v4[4] = {0.0, 0.0, 0.0, 0.0};
             for (int e = tid; e < L; e += BS) {
-                S += double(Real(srcz[base + e] + a.r[base + e]));
-                C += 1.0;
+                const double v = double(Real(srcz[base + e] + a.r[base + e]));
+                v4[0] += v;
+                v4[1] += 1.0;
+                if (v > start) {
+                    v4[2] += v;
+                    v4[3] += 1.0;
+                }
             }
-            double v2[2] = {S, C};
-            block_sums<2>(v2, red);
-            S = v2[0];
-            C = v2[1];
+            block_sums<4>(v4, red);
+            first = v4[3] > 0.0;
+            double S = first ? v4[2] : v4[0], C = first ? v4[3] : v4[1];
             th = (S - target) / C;
             for (int round = 0; round <= L; ++round) {
                 double S2 = 0.0, C2 = 0.0;
@@ -322,11 +387,13 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
                 }
                 double w2[2] = {S2, C2};
                 block_sums<2>(w2, red);
-                if (w2[1] >= C || w2[1] == 0.0) break;
+                if (w2[1] == C || w2[1] == 0.0 || (w2[1] > C && !first)) break;
                 C = w2[1];
                 th = (w2[0] - target) / w2[1];
+                first = false;
             }
         }
+        if (tid == 0) a.thz[s] = th;
     }
     const Real thr = Real(th);
     for (int e = tid; e < L; e += BS) {

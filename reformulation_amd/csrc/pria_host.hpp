@@ -141,19 +141,26 @@ inline int pria_build_host(int64_t n, int64_t t, const double* k, double k0, con
 }
 
 struct PriaLayout {
-    enum { kK, kH, kXS, kYS, kZS, kX0, kY0, kZ0, kXC, kX, kSX, kYC, kY, kSY, kZC, kZ, kSZ, kR, kCount };
+    enum {
+        kK, kH, kXS, kYS, kZS, kX0, kY0, kZ0, kXC, kX, kSX, kYC, kY, kSY, kZC, kZ, kSZ, kR,
+        kThX, kThY, kThZ,  // FP64 thresholds: N T', N T', T'
+        kCount
+    };
     size_t off[kCount];
     size_t input_bytes = 0, total_bytes = 0;
 };
 
 inline size_t pria_align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-inline PriaLayout pria_layout(int N, int ND, size_t real_size) {
+inline PriaLayout pria_layout(int N, int T, int ND, size_t real_size) {
     PriaLayout L;
     size_t at = 0;
     for (int q = 0; q < PriaLayout::kCount; ++q) {
         L.off[q] = at;
-        size_t bytes = q == PriaLayout::kK ? real_size * size_t(N) : real_size * size_t(ND);
+        size_t bytes = real_size * size_t(ND);
+        if (q == PriaLayout::kK) bytes = real_size * size_t(N);
+        if (q == PriaLayout::kThX || q == PriaLayout::kThY) bytes = sizeof(double) * size_t(N) * T;
+        if (q == PriaLayout::kThZ) bytes = sizeof(double) * size_t(T);
         at += pria_align256(bytes ? bytes : 1);
         if (q == PriaLayout::kZS) L.input_bytes = at;
     }
@@ -207,6 +214,9 @@ inline PriaArrays<Real> pria_bind(const PriaLayout& L, char* base, const PriaHos
     a.z = at(PriaLayout::kZ);
     a.sz = at(PriaLayout::kSZ);
     a.r = at(PriaLayout::kR);
+    a.thx = reinterpret_cast<double*>(base + L.off[PriaLayout::kThX]);
+    a.thy = reinterpret_cast<double*>(base + L.off[PriaLayout::kThY]);
+    a.thz = reinterpret_cast<double*>(base + L.off[PriaLayout::kThZ]);
     return a;
 }
 

@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) wpria_x(const PriaJob<Real>* jobs, const 
 template <class Real>
 __global__ void __launch_bounds__(1024) wpria_yz(const PriaJob<Real>* jobs, const int2* tab,
                                                  const int* round_action, int mode, int j) {
-    __shared__ double red[2 * 16];
+    __shared__ double red[4 * 16];
     const int2 t = tab[blockIdx.x];
     const PriaJob<Real>& job = jobs[t.x];
     PriaStep<Real> sc;
@@ -360,6 +360,14 @@ __global__ void __launch_bounds__(256) wpria_copy(const PriaJob<Real>* jobs, con
     if (!wide_copies(job, pending, t.x, mode)) return;
     const PriaArrays<Real>& a = job.a;
     const int s = t.y, base = pria_off(a.N, a.T, s), L = a.N * (a.T - s);
+    if (mode == 3) {  // no thresholds yet: column (i, s), row (i, s), slice s
+        const double none = __builtin_nan("");
+        for (int i = threadIdx.x; i < a.N; i += blockDim.x) {
+            a.thx[i * a.T + s] = none;
+            a.thy[i * a.T + s] = none;
+        }
+        if (threadIdx.x == 0) a.thz[s] = none;
+    }
     for (int e = threadIdx.x; e < L; e += blockDim.x) {
         const int g = base + e;
         if (mode <= 1) {
