@@ -87,3 +87,29 @@ def test_praq_rejects_invalid_arguments_before_any_device_work():
     bad.omega = -1.0
     inst.n = 3
     assert lib.praq_solve(ctypes.byref(inst), ctypes.byref(bad), _native.dptr(x), ctypes.byref(res)) == -1
+
+
+def test_batch_validates_on_the_host_before_any_device_work():
+    import numpy as np
+    from reformulation_amd import praq, pria
+    from reformulation_amd.batch import Batch
+    b = Batch(use_float32=True)
+    assert b.counts() == (0, 0)
+    ok = praq.QuboProblem.from_sparse(3, np.zeros(3), np.array([0]), np.array([1]),
+                                      np.array([1.0]))
+    assert b.add_praq(ok, praq.Parameters.with_auto_step(100, 1e-6)) == 0
+    k, h = np.array([2.0, 3.0]), np.zeros((2, 4, 4))
+    assert b.add_pria(pria.JrpProblem.new(k, 5.0, h), pria.Parameters.with_auto_step(100, 1e-6)) == 0
+    assert b.counts() == (1, 1)
+    with pytest.raises(RuntimeError):          # nothing solved yet
+        b.praq_result(0)
+    with pytest.raises(RuntimeError):          # solve before upload
+        b.solve("batched")
+    # a self-loop is an instance error found by the host build, before any
+    # device allocation
+    loop = praq.QuboProblem.from_sparse(3, np.zeros(3), np.array([1]), np.array([1]),
+                                        np.array([1.0]))
+    b.add_praq(loop, praq.Parameters.with_auto_step(100, 1e-6))
+    with pytest.raises(RuntimeError, match=r"\(-2\)"):
+        b.upload()
+    b.close()

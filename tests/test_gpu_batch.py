@@ -1,0 +1,149 @@
+"""The batched mode (block tables over a ragged batch, per-instance Halpern
+state on the device, hipGraph rounds) against the plain solver and the
+oracle, through the batch C ABI.  Marked gpu.
+
+Parity bar: FP64 batched iterations equal the oracle's and the plain
+solver's; objectives within 1e-9 relative of the oracle's; an instance's
+batched result is bitwise the same alone and inside a mix, and the same on a
+repeated solve.  FP32 batched iterations equal the plain FP32 solver's and
+its bracket closes to the requested gap."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import random_qubo, small_jrp
+
+pytestmark = pytest.mark.gpu
+
+
+def _members():
+    """A ragged mix: PRAQ instances of the mixture's kind at small sizes and
+    tiny enumeration-size ones, PRIA instances with odd and even T."""
+    from reformulation_amd import workloads
+    out = []
+    for seed, (n, m) in enumerate([(1200, 4000), (3000, 9000), (800, 6400)]):
+        w = workloads.praq_er(100 + seed, n, m)
+        out.append(("praq", (w.n, w.diag, w.rows, w.cols, w.vals)))
+    out.append(("praq", random_qubo(7, 6, 0.7)))
+    for seed, (n, t) in enumerate([(12, 41), (20, 60), (8, 17), (30, 90)]):
+        j = workloads.pria_c1(200 + seed, n, t)
+        out.append(("pria", (j.order_costs, j.joint_order_cost, j.holding_costs)))
+    out.append(("pria", small_jrp(3, 5, 9)))
+    return out
+
+
+def _batch(members, f32, tol=1e-6):
+    from reformulation_amd import praq, pria
+    from reformulation_amd.batch import Batch
+    b = Batch(use_float32=f32)
+    index = []
+    for kind, inst in members:
+        if kind == "praq":
+            n, diag, rows, cols, vals = inst
+            p = praq.QuboProblem.from_sparse(n, diag, rows, cols, vals)
+            index.append(("praq", b.add_praq(p, praq.Parameters.with_auto_step(50_000, tol))))
+        else:
+            k, k0, h = inst
+            p = pria.JrpProblem.new(k, k0, h)
+            index.append(("pria", b.add_pria(p, pria.Parameters.with_auto_step(50_000, tol))))
+    b.upload()
+    return b, index
+
+
+def _results(b, index):
+    out = []
+    for kind, q in index:
+        if kind == "praq":
+            x, y, info = b.praq_result(q)
+            out.append((x, y, info))
+        else:
+            x, y, z, info = b.pria_result(q)
+            out.append((x, (y, z), info))
+    return out
+
+
+def _same_bits(a, b):
+    if isinstance(a, tuple):
+        return all(_same_bits(u, v) for u, v in zip(a, b))
+    return a is None and b is None or np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_batched_equals_plain_solver(f32):
+    members = _members()
+    b, index = _batch(members, f32)
+    b.solve("batched")
+    batched = _results(b, index)
+    b.solve("sequential")
+    plain = _results(b, index)
+    for (kind, _), (xb, _, ib), (xs, _, is_) in zip(index, batched, plain):
+        assert ib.status == 0 and is_.status == 0
+        assert ib.iterations == is_.iterations, kind
+        assert ib.restarts == is_.restarts, kind
+        assert ib.relative_gap <= 1e-6
+        sc = 1.0 + abs(is_.upper)
+        assert abs(ib.upper - is_.upper) <= 1e-9 * sc
+        assert abs(ib.lower - is_.lower) <= 1e-9 * sc
+        assert np.max(np.abs(xb - xs)) <= (1e-5 if f32 else 1e-8)
+    b.close()
+
+
+def test_batched_fp64_matches_the_oracle():
+    members = _members()
+    b, index = _batch(members, False)
+    b.solve("batched")
+    for (kind, inst), (x, _, info) in zip(members, _results(b, index)):
+        if kind == "praq":
+            o = oracle.praq_solve(*inst, params=oracle.praq_params(tolerance=1e-6))
+        else:
+            k, k0, h = inst
+            o = oracle.pria_solve(k, k0, h, oracle.pria_params(tolerance=1e-6))
+        assert info.iterations == o.iterations and info.restarts == o.restarts, kind
+        sc = 1.0 + abs(o.upper)
+        assert abs(info.upper - o.upper) <= 1e-9 * sc, kind
+        assert abs(info.lower - o.lower) <= 1e-9 * sc, kind
+        assert np.max(np.abs(x - o.x)) <= 1e-7, kind
+    b.close()
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_result_is_the_same_alone_in_the_mix_and_on_a_second_solve(f32):
+    members = _members()
+    b, index = _batch(members, f32)
+    b.solve("batched")
+    mixed = _results(b, index)
+    b.solve("batched")
+    again = _results(b, index)
+    for u, v in zip(mixed, again):
+        assert _same_bits(u[0], v[0]) and _same_bits(u[1], v[1])
+        assert u[2].iterations == v[2].iterations
+    for pick in ([0], [4, 1], [5, 8, 3]):
+        sub = [members[q] for q in pick]
+        b2, idx2 = _batch(sub, f32)
+        b2.solve("batched")
+        for q, (x, dual, info) in zip(pick, _results(b2, idx2)):
+            assert _same_bits(x, mixed[q][0]) and _same_bits(dual, mixed[q][1]), q
+            assert info.iterations == mixed[q][2].iterations
+            assert info.upper == mixed[q][2].upper and info.lower == mixed[q][2].lower
+        b2.close()
+    b.close()
+
+
+def test_timed_launches_give_the_same_results_and_count_every_step():
+    members = _members()
+    b, index = _batch(members, True)
+    b.solve("batched")
+    graph = _results(b, index)
+    b.set_timing(True)
+    b.solve("batched")
+    timed = _results(b, index)
+    stats = b.kernel_stats()
+    b.set_timing(False)
+    for u, v in zip(graph, timed):
+        assert _same_bits(u[0], v[0]) and u[2].iterations == v[2].iterations
+    names = {s["name"] for s in stats}
+    assert {"wpraq_node<float>", "wpraq_edge<float>", "wpria_x<float>",
+            "wpria_yz<float>"} <= names
+    for s in stats:
+        assert s["launches"] > 0 and s["ms"] > 0.0 and s["bytes"] > 0.0
+    b.close()
