@@ -38,7 +38,6 @@ using namespace reform;
 namespace {
 
 constexpr double kWgLimitSeconds = 600.0;
-constexpr int kStallChecks = 24;
 
 double clock_rate_hz() {
     int dev = 0, khz = 0;
@@ -350,7 +349,7 @@ int BatchImpl::upload() {
         if (fill(praq64.host[s], pl64[q], e.off64, double())) return kErrorDevice;
         if (f32) {
             if (fill(praq32.host[s], pl32[q], e.off32, float())) return kErrorDevice;
-            praq32.host[s].p.stall_checks = kStallChecks;
+            apply_fp32_policy(praq32.host[s].p);
         }
     }
     pria64.host.assign(pria.size(), PriaJob<double>());
@@ -379,7 +378,7 @@ int BatchImpl::upload() {
         if (fill(pria64.host[s], rl64[q], e.off64, double())) return kErrorDevice;
         if (f32) {
             if (fill(pria32.host[s], rl32[q], e.off32, float())) return kErrorDevice;
-            pria32.host[s].p.stall_checks = kStallChecks;
+            apply_fp32_policy(pria32.host[s].p);
         }
     }
     auto put = [&](auto& set) -> int {

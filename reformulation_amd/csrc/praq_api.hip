@@ -40,9 +40,6 @@ struct DeviceBuffer {
     int alloc(size_t bytes) { return hipMalloc(&ptr, bytes ? bytes : 1) == hipSuccess ? 0 : -1; }
 };
 
-// Checks without a halving of the best gap after which FP32 iterates hand the
-// whole Halpern state over to FP64.
-constexpr int kStallChecks = 24;
 
 int pick_tier(const PraqHostInstance& s, int tier) {
     if (tier != PRAQ_TIER_AUTO) return tier;
@@ -139,7 +136,7 @@ int solve_mixed(const PraqHostInstance& s, LogicParams lp, const PraqOptions& o,
             return kErrorDevice;
     } else {
         LogicParams lp32 = lp;
-        lp32.stall_checks = kStallChecks;
+        apply_fp32_policy(lp32);
         PraqImage<float> f;
         if (f.init(s, lp32, stream) || f.run(tier, st, o.time_limit, false, stream))
             return kErrorDevice;

@@ -236,6 +236,7 @@ inline LogicParams praq_logic_params(const PraqParameters& p, const PraqOptions&
     q.fallback_nonpositive = 1;
     q.verbose = 0;
     q.stall_checks = 0;
+    q.switch_gap = 0.0;
     return q;
 }
 

@@ -49,9 +49,6 @@ bool params_valid(const pria_params* p) {
     return p && p->omega > 0.0 && p->eta > 0.0 && p->tolerance >= 0.0;
 }
 
-// Checks without a halving of the best gap after which FP32 iterates hand the
-// whole Halpern state over to FP64.
-constexpr int kStallChecks = 24;
 
 int pick_tier(const PriaHostInstance& s, int tier) {
     if (tier != PRIA_TIER_AUTO) return tier;
@@ -148,7 +145,7 @@ int solve_mixed(const PriaHostInstance& s, const LogicParams& lp, const pria_opt
             return kPriaErrorDevice;
     } else {
         LogicParams lp32 = lp;
-        lp32.stall_checks = kStallChecks;
+        apply_fp32_policy(lp32);
         PriaImage<float> f;
         if (f.init(s, lp32, stream.s) || f.run(tier, st, o.time_limit, false, s.T, stream.s))
             return kPriaErrorDevice;

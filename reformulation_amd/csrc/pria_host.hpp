@@ -240,6 +240,7 @@ inline LogicParams pria_logic_params(const pria_params& p) {
     q.fallback_nonpositive = 0;
     q.verbose = p.verbose;
     q.stall_checks = 0;
+    q.switch_gap = 0.0;
     return q;
 }
 
