@@ -76,18 +76,17 @@ int64_t reform_batch_device_bytes(const reform_batch* batch);
  * node, edge; PRIA: x, y/z).  With timing on, the rounds are launched
  * directly instead of replayed from their graph, and every step launch is
  * bracketed by HIP events on its stream; the figures cover the last batched
- * solve.  engine 0 is the batch's precision, engine 1 the FP64 continuation
- * of stalled FP32 instances.  bytes: algorithmic HBM bytes those launches
- * moved (per step, per instance: PRAQ node n (4 + 7b) + 2m (4 + b), edge
- * m (8 + 7b); PRIA x 7b ND, y/z 8b ND; b = bytes per value). */
+ * solve.  bytes: algorithmic HBM bytes those launches moved (per step, per
+ * instance: PRAQ node n (4 + 7b) + 2m (4 + b), edge m (8 + 7b); PRIA x 7b ND,
+ * y/z 8b ND; b = bytes per value in the instance's precision at that step). */
 typedef struct reform_kernel_stats {
-    char name[64];       /* kernel symbol, e.g. "wpria_yz<float>" */
+    char name[64];       /* kernel symbol, e.g. "wpria_yz" */
     int64_t launches;
     double ms;           /* sum of the launches' durations */
     double bytes;        /* sum of their algorithmic bytes */
 } reform_kernel_stats;
 int reform_batch_set_timing(reform_batch* batch, int on);
-int reform_batch_kernel_stats(const reform_batch* batch, int family, int engine,
+int reform_batch_kernel_stats(const reform_batch* batch, int family,
                               reform_kernel_stats out[2]);
 
 /* Results of the last solve.  Outputs may be NULL.  PRAQ y is [m][2] in

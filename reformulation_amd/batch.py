@@ -41,7 +41,7 @@ def _declare(L):
     L.reform_batch_set_timing.restype = ctypes.c_int
     L.reform_batch_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.reform_batch_kernel_stats.restype = ctypes.c_int
-    L.reform_batch_kernel_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+    L.reform_batch_kernel_stats.argtypes = [ctypes.c_void_p, ctypes.c_int,
                                             ctypes.POINTER(KernelStats)]
     L.reform_batch_create.restype = ctypes.c_void_p
     L.reform_batch_create.argtypes = [ctypes.c_int]
@@ -183,21 +183,19 @@ class Batch:
         self._L.reform_batch_set_timing(self._h, int(on))
 
     def kernel_stats(self) -> list[dict]:
-        """Per step kernel of the last batched solve (both families, the
-        batch's precision and the FP64 continuation): name, launches, summed
-        duration (ms) and algorithmic bytes."""
+        """Per step kernel of the last batched solve (both families): name,
+        launches, summed duration (ms) and algorithmic bytes."""
         out = []
         for family in (0, 1):
-            for engine in (0, 1):
-                arr = (KernelStats * 2)()
-                st = self._L.reform_batch_kernel_stats(self._h, family, engine, arr)
-                if st < 0:
-                    raise RuntimeError(f"reform_batch_kernel_stats failed ({st}): "
-                                       f"{_native.last_error()}")
-                for k in arr:
-                    if k.launches:
-                        out.append(dict(name=k.name.decode(), launches=int(k.launches),
-                                        ms=float(k.ms), bytes=float(k.bytes)))
+            arr = (KernelStats * 2)()
+            st = self._L.reform_batch_kernel_stats(self._h, family, arr)
+            if st < 0:
+                raise RuntimeError(f"reform_batch_kernel_stats failed ({st}): "
+                                   f"{_native.last_error()}")
+            for k in arr:
+                if k.launches:
+                    out.append(dict(name=k.name.decode(), launches=int(k.launches),
+                                    ms=float(k.ms), bytes=float(k.bytes)))
         return out
 
     @property

@@ -134,29 +134,20 @@ struct BatchImpl {
     int device = 0;
     double time_limit = 0.0;
 
-    // wide engines: over the whole family (in the batch's precision) and,
-    // for FP32 batches, over the last stalled subset in FP64
-    std::unique_ptr<WidePraq<float>> wpraq32;
-    std::unique_ptr<WidePraq<double>> wpraq64, wpraq64_sub;
-    std::unique_ptr<WidePria<float>> wpria32;
-    std::unique_ptr<WidePria<double>> wpria64, wpria64_sub;
-    std::vector<int> wpraq_sub_key, wpria_sub_key;
+    // the wide engines of the batched mode, one per family (instances switch
+    // precision inside them)
+    std::unique_ptr<WidePraq> wpraq;
+    std::unique_ptr<WidePria> wpria;
     WideStats praq_stats, pria_stats;
     bool timing = false;
 
-    WideControl* engine(int family, int which) const {
-        const bool f32 = use_float32 != 0;
-        if (family == 0)
-            return which == 0 ? (f32 ? static_cast<WideControl*>(wpraq32.get()) : wpraq64.get())
-                              : (f32 ? wpraq64_sub.get() : nullptr);
-        return which == 0 ? (f32 ? static_cast<WideControl*>(wpria32.get()) : wpria64.get())
-                          : (f32 ? wpria64_sub.get() : nullptr);
+    WideControl* engine(int family) const {
+        return family == 0 ? static_cast<WideControl*>(wpraq.get()) : wpria.get();
     }
     void set_timing(bool on) {
         timing = on;
         for (int f = 0; f < 2; ++f)
-            for (int w = 0; w < 2; ++w)
-                if (WideControl* e = engine(f, w)) e->set_timing(on);
+            if (WideControl* e = engine(f)) e->set_timing(on);
     }
 
     ~BatchImpl() {
@@ -170,8 +161,6 @@ struct BatchImpl {
     int upload();
     int solve(int mode);
     int solve_wide();
-    WideControl* praq_handover(int* status);
-    WideControl* pria_handover(int* status);
     int solve_workgroups();
     int solve_sequential();
     int praq_result(int64_t q, double* x, double* y, PraqInfo* info);
@@ -420,189 +409,65 @@ int BatchImpl::solve(int mode) {
     return 0;
 }
 
-// FP32 batches: the PRAQ instances whose FP32 iterates stalled continue in
-// FP64 from their converted state, in a wide engine over that subset (kept
-// for the next solve while the subset is the same).  Returns the engine to
-// run, or nullptr when none stalled.
-WideControl* BatchImpl::praq_handover(int* status) {
-    const int np = int(praq.size());
-    *status = 0;
-    if (hipMemcpyAsync(praq_states.data(), praq_states_dev, sizeof(LogicState) * np,
-                       hipMemcpyDeviceToHost, s0) != hipSuccess ||
-        hipStreamSynchronize(s0) != hipSuccess) {
-        *status = kErrorDevice;
-        return nullptr;
-    }
-    std::vector<int> key;
-    std::vector<PraqJob<float>> p32;
-    std::vector<PraqJob<double>> p64;
-    for (int s = 0; s < np; ++s) {
-        if (!praq_states[s].stalled) continue;
-        key.push_back(s);
-        p32.push_back(praq32.host[s]);
-        p64.push_back(praq64.host[s]);
-        praq_in64[praq_order[s]] = 1;
-    }
-    if (key.empty()) return nullptr;
-    if (hipMemcpyAsync(praq32.sub, p32.data(), sizeof(p32[0]) * p32.size(), hipMemcpyHostToDevice,
-                       s0) != hipSuccess ||
-        hipMemcpyAsync(praq64.sub, p64.data(), sizeof(p64[0]) * p64.size(), hipMemcpyHostToDevice,
-                       s0) != hipSuccess ||
-        praq_convert_jobs(praq32.sub, praq64.sub, int(p32.size()), s0) ||
-        hipStreamSynchronize(s0) != hipSuccess) {
-        *status = kErrorDevice;
-        return nullptr;
-    }
-    if (!wpraq64_sub || key != wpraq_sub_key) {
-        wpraq64_sub.reset(new WidePraq<double>());
-        wpraq64_sub->set_timing(timing);
-        if (wpraq64_sub->setup(p64, praq64.sub)) {
-            *status = kErrorDevice;
-            return nullptr;
-        }
-        wpraq_sub_key = key;
-    }
-    return wpraq64_sub.get();
-}
-
-// As praq_handover, for PRIA on stream s1.
-WideControl* BatchImpl::pria_handover(int* status) {
-    const int nr = int(pria.size());
-    *status = 0;
-    if (hipMemcpyAsync(pria_states.data(), pria_states_dev, sizeof(LogicState) * nr,
-                       hipMemcpyDeviceToHost, s1) != hipSuccess ||
-        hipStreamSynchronize(s1) != hipSuccess) {
-        *status = kErrorDevice;
-        return nullptr;
-    }
-    std::vector<int> key;
-    std::vector<PriaJob<float>> r32;
-    std::vector<PriaJob<double>> r64;
-    for (int s = 0; s < nr; ++s) {
-        if (!pria_states[s].stalled) continue;
-        key.push_back(s);
-        r32.push_back(pria32.host[s]);
-        r64.push_back(pria64.host[s]);
-        pria_in64[pria_order[s]] = 1;
-    }
-    if (key.empty()) return nullptr;
-    if (hipMemcpyAsync(pria32.sub, r32.data(), sizeof(r32[0]) * r32.size(), hipMemcpyHostToDevice,
-                       s1) != hipSuccess ||
-        hipMemcpyAsync(pria64.sub, r64.data(), sizeof(r64[0]) * r64.size(), hipMemcpyHostToDevice,
-                       s1) != hipSuccess ||
-        pria_convert_jobs(pria32.sub, pria64.sub, int(r32.size()), s1) ||
-        hipStreamSynchronize(s1) != hipSuccess) {
-        *status = kErrorDevice;
-        return nullptr;
-    }
-    if (!wpria64_sub || key != wpria_sub_key) {
-        wpria64_sub.reset(new WidePria<double>());
-        wpria64_sub->set_timing(timing);
-        if (wpria64_sub->setup(r64, pria64.sub)) {
-            *status = kErrorDevice;
-            return nullptr;
-        }
-        wpria_sub_key = key;
-    }
-    return wpria64_sub.get();
-}
-
-// Both families at once from this thread: PRAQ on s0, PRIA on s1, each a
-// wide engine in the batch's precision, then (FP32) its stalled subset in
-// FP64.  The runs are advanced without blocking, so neither stream waits
-// for the other's host work.
+// Both families at once from this thread: PRAQ on s0, PRIA on s1, each one
+// wide engine whose instances switch to FP64 on their own.  The runs are
+// advanced without blocking, so neither stream waits for the other's host
+// work.
 int BatchImpl::solve_wide() {
     const bool f32 = use_float32 != 0;
     const int np = int(praq.size()), nr = int(pria.size());
-    WideControl* praq_main = nullptr;
-    WideControl* pria_main = nullptr;
-    if (np) {
-        if (f32 && !wpraq32) {
-            wpraq32.reset(new WidePraq<float>());
-            wpraq32->set_timing(timing);
-            if (wpraq32->setup(praq32.host, praq32.dev)) return kErrorDevice;
-        }
-        if (!f32 && !wpraq64) {
-            wpraq64.reset(new WidePraq<double>());
-            wpraq64->set_timing(timing);
-            if (wpraq64->setup(praq64.host, praq64.dev)) return kErrorDevice;
-        }
-        praq_main = f32 ? static_cast<WideControl*>(wpraq32.get()) : wpraq64.get();
+    if (np && !wpraq) {
+        wpraq.reset(new WidePraq());
+        wpraq->set_timing(timing);
+        if (wpraq->setup(praq64.host, f32 ? praq32.dev : nullptr, praq64.dev)) return kErrorDevice;
     }
-    if (nr) {
-        if (f32 && !wpria32) {
-            wpria32.reset(new WidePria<float>());
-            wpria32->set_timing(timing);
-            if (wpria32->setup(pria32.host, pria32.dev)) return kErrorDevice;
-        }
-        if (!f32 && !wpria64) {
-            wpria64.reset(new WidePria<double>());
-            wpria64->set_timing(timing);
-            if (wpria64->setup(pria64.host, pria64.dev)) return kErrorDevice;
-        }
-        pria_main = f32 ? static_cast<WideControl*>(wpria32.get()) : wpria64.get();
+    if (nr && !wpria) {
+        wpria.reset(new WidePria());
+        wpria->set_timing(timing);
+        if (wpria->setup(pria64.host, f32 ? pria32.dev : nullptr, pria64.dev)) return kErrorDevice;
     }
-    std::fill(praq_in64.begin(), praq_in64.end(), f32 ? 0 : 1);
-    std::fill(pria_in64.begin(), pria_in64.end(), f32 ? 0 : 1);
     praq_stats = pria_stats = WideStats();
-    for (int f = 0; f < 2; ++f)
-        for (int w = 0; w < 2; ++w)
-            if (WideControl* e = engine(f, w)) e->clear_timing();
-
-    struct Family {
-        WideRun run;
-        int phase = 2;  // 0 main run, 1 FP64 continuation, 2 done
-        hipStream_t stream;
-        hipEvent_t begin_ev, end_ev;
-        WideStats* stats;
-    };
-    Family fam[2];
-    fam[0].stream = s0;
-    fam[0].begin_ev = ev[0];
-    fam[0].end_ev = ev[1];
-    fam[0].stats = &praq_stats;
-    fam[1].stream = s1;
-    fam[1].begin_ev = ev[2];
-    fam[1].end_ev = ev[3];
-    fam[1].stats = &pria_stats;
-    WideControl* mains[2] = {praq_main, pria_main};
+    WideRun runs[2];
+    hipStream_t streams[2] = {s0, s1};
+    WideControl* engines[2] = {np ? wpraq.get() : nullptr, nr ? wpria.get() : nullptr};
     for (int f = 0; f < 2; ++f) {
-        if (!mains[f]) continue;
-        REFORM_HIP_CHECK(hipEventRecord(fam[f].begin_ev, fam[f].stream));
-        if (fam[f].run.begin(mains[f], fam[f].stream, false, time_limit)) return kErrorDevice;
-        fam[f].phase = 0;
+        if (!engines[f]) continue;
+        engines[f]->clear_timing();
+        REFORM_HIP_CHECK(hipEventRecord(ev[2 * f], streams[f]));
+        if (runs[f].begin(engines[f], streams[f], time_limit)) return kErrorDevice;
     }
-    while (fam[0].phase < 2 || fam[1].phase < 2) {
-        int changed = 0;
+    bool recorded[2] = {false, false};
+    for (;;) {
+        int changed = 0, running = 0;
         for (int f = 0; f < 2; ++f) {
-            Family& F = fam[f];
-            if (F.phase == 2) continue;
-            const int st = F.run.advance();
+            if (!engines[f] || recorded[f]) continue;
+            const int st = runs[f].advance();
             if (st < 0) return st;
             changed |= st;
-            if (!F.run.finished) continue;
-            F.stats->rounds += F.run.stats.rounds;
-            WideControl* next = nullptr;
-            if (F.phase == 0 && f32) {
-                int hs = 0;
-                next = f == 0 ? praq_handover(&hs) : pria_handover(&hs);
-                if (hs) return hs;
-            }
-            if (next) {
-                const double left =
-                    time_limit > 0.0 ? std::max(1e-9, time_limit - F.run.stats.seconds) : 0.0;
-                if (F.run.begin(next, F.stream, true, left)) return kErrorDevice;
-                F.phase = 1;
+            if (runs[f].finished) {
+                REFORM_HIP_CHECK(hipEventRecord(ev[2 * f + 1], streams[f]));
+                recorded[f] = true;
+                changed = 1;
             } else {
-                REFORM_HIP_CHECK(hipEventRecord(F.end_ev, F.stream));
-                F.phase = 2;
+                ++running;
             }
-            changed = 1;
         }
+        if (!running) break;
         if (!changed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
     REFORM_HIP_CHECK(hipStreamSynchronize(s0));
     REFORM_HIP_CHECK(hipStreamSynchronize(s1));
+    praq_stats = runs[0].stats;
+    pria_stats = runs[1].stats;
+    std::vector<int> prec;
+    if (np) {
+        if (wpraq->precisions(prec)) return kErrorDevice;
+        for (int s = 0; s < np; ++s) praq_in64[praq_order[s]] = char(prec[s]);
+    }
+    if (nr) {
+        if (wpria->precisions(prec)) return kErrorDevice;
+        for (int s = 0; s < nr; ++s) pria_in64[pria_order[s]] = char(prec[s]);
+    }
     float ms = 0.0f;
     if (np && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) praq_ms = ms;
     if (nr && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) pria_ms = ms;
@@ -922,19 +787,16 @@ extern "C" int reform_batch_set_timing(reform_batch* batch, int on) {
     return 0;
 }
 
-extern "C" int reform_batch_kernel_stats(const reform_batch* batch, int family, int engine,
+extern "C" int reform_batch_kernel_stats(const reform_batch* batch, int family,
                                          reform_kernel_stats* out) {
-    if (!batch || !out || family < 0 || family > 1 || engine < 0 || engine > 1)
-        return kErrorArgument;
+    if (!batch || !out || family < 0 || family > 1) return kErrorArgument;
     const BatchImpl& b = *batch->impl;
     static const char* kNames[2][2] = {{"wpraq_node", "wpraq_edge"}, {"wpria_x", "wpria_yz"}};
-    const bool f32 = b.use_float32 != 0 && engine == 0;
-    WideControl* e = b.engine(family, engine);
+    WideControl* e = b.engine(family);
     double bytes[WideControl::kKinds] = {0.0, 0.0};
     if (e && e->algorithmic_bytes(bytes)) return kErrorDevice;
     for (int k = 0; k < 2; ++k) {
-        std::snprintf(out[k].name, sizeof(out[k].name), "%s<%s>", kNames[family][k],
-                      f32 ? "float" : "double");
+        std::snprintf(out[k].name, sizeof(out[k].name), "%s", kNames[family][k]);
         out[k].launches = e ? e->launches[k] : 0;
         out[k].ms = e ? e->kernel_ms[k] : 0.0;
         out[k].bytes = bytes[k];
@@ -946,8 +808,7 @@ extern "C" int64_t reform_batch_device_bytes(const reform_batch* batch) {
     if (!batch) return kErrorArgument;
     const BatchImpl& b = *batch->impl;
     size_t bytes = b.dev_bytes;
-    const WideControl* engines[] = {b.wpraq32.get(), b.wpraq64.get(), b.wpraq64_sub.get(),
-                                    b.wpria32.get(), b.wpria64.get(), b.wpria64_sub.get()};
+    const WideControl* engines[] = {b.wpraq.get(), b.wpria.get()};
     for (const WideControl* w : engines)
         if (w) bytes += w->bytes;
     return int64_t(bytes);

@@ -27,6 +27,7 @@ constexpr int kErrorDevice = -3;
 
 template <class Real>
 struct PraqJob {
+    using real_type = Real;
     PraqArrays<Real> a;
     LogicParams p;
     double scale;

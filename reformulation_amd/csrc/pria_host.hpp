@@ -27,6 +27,7 @@ constexpr int kPriaErrorDevice = -3;
 
 template <class Real>
 struct PriaJob {
+    using real_type = Real;
     PriaArrays<Real> a;
     LogicParams p;
     double scale;

@@ -1,11 +1,13 @@
 // The batched device tier (wide.hpp): block tables over all instances of a
-// batch, per-instance round control, one round captured in a hipGraph.
+// batch, per-instance round control and precision, one round captured in a
+// hipGraph.
 #include "wide.hpp"
 
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace reform {
@@ -16,6 +18,17 @@ constexpr int kWideBlock = 256;
 constexpr int kRoundsPerPoll = 2;
 
 int grid_for(int threads) { return (threads + kWideBlock - 1) / kWideBlock; }
+
+template <class Job>
+using RealOf = typename std::decay<Job>::type::real_type;
+
+// Run `f` on the instance's job record in its current precision.
+template <class J32, class J64, class F>
+__device__ __forceinline__ void dispatch(const J32* j32, const J64* j64, const int* prec, int inst,
+                                         F&& f) {
+    if (prec[inst]) f(j64[inst]);
+    else f(j32[inst]);
+}
 
 __device__ __forceinline__ bool wide_runs_steps(int act) {
     return act == kActWarmBlock || act == kActHalpernBlock || act == kActInit;
@@ -82,7 +95,8 @@ __device__ __forceinline__ int wide_logic_action(const Job& job, const int* roun
 }
 
 // The scalar logic after a check, on one thread; the same sequence as the
-// device tier's host loop.
+// device tier's host loop.  `steps` counts the steps this instance took in
+// its current precision.
 template <class Job>
 __device__ void wide_apply(const Job& job, int act, const CheckSums& cs, int* last_action,
                            int64_t* steps, int inst) {
@@ -119,26 +133,33 @@ __global__ void wide_pre(const Job* jobs, int count, int* round_action, int* pen
     if (pend) st->anchor_pending = 0;
 }
 
-// One thread per instance: a cold start's state, or the continuation of a
-// state handed over from the other precision.
+// One thread per instance: the cold start's state and precision.
 template <class Job>
 __global__ void wide_init(const Job* jobs, int count, int* round_action, int* last_action,
-                          int64_t* steps, int resume) {
+                          int64_t* steps, int* prec, int first_prec) {
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= count) return;
+    LogicState s;
+    logic_init(s, jobs[inst].p);
+    *jobs[inst].state = s;
+    round_action[inst] = kActInit;
+    last_action[inst] = kActInit;
+    steps[inst] = 0;
+    steps[count + inst] = 0;
+    prec[inst] = first_prec;
+}
+
+// One thread per instance: FP32 instances whose logic asked for FP64 (their
+// state was converted by the launch before) continue in FP64 from here.
+template <class Job>
+__global__ void wide_promote(const Job* jobs, int count, int* prec) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= count) return;
     LogicState* st = jobs[inst].state;
-    steps[inst] = 0;
-    if (resume) {
+    if (prec[inst] == 0 && st->stalled) {
+        prec[inst] = 1;
         st->stalled = 0;
         st->switch_iteration = st->iteration;
-        round_action[inst] = kActSkip;
-        last_action[inst] = kActHalpernBlock;  // a stall is detected after a Halpern check
-    } else {
-        LogicState s;
-        logic_init(s, jobs[inst].p);
-        *st = s;
-        round_action[inst] = kActInit;
-        last_action[inst] = kActInit;
     }
 }
 
@@ -149,7 +170,7 @@ __global__ void wide_expire(const Job* jobs, int count, const int* last_action) 
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= count) return;
     LogicState* st = jobs[inst].state;
-    if (st->action == kActDoneX || st->action == kActDoneSX || st->stalled) return;
+    if (st->action == kActDoneX || st->action == kActDoneSX) return;
     st->action = last_action[inst] == kActHalpernBlock ? kActDoneSX : kActDoneX;
     st->status = 2;
 }
@@ -163,7 +184,7 @@ __global__ void wide_count(const Job* jobs, int count, int* active) {
     for (int inst = threadIdx.x; inst < count; inst += blockDim.x) {
         LogicState* st = jobs[inst].state;
         if (st->action == kActDoneSX) st->action = kActDoneX;
-        if (st->action != kActDoneX && !st->stalled) n += 1.0;
+        if (st->action != kActDoneX) n += 1.0;
     }
     n = block_sum1(n, red);
     if (threadIdx.x == 0) *active = int(n);
@@ -171,218 +192,310 @@ __global__ void wide_count(const Job* jobs, int count, int* active) {
 
 // ---------------------------------------------------------------- PRAQ ----
 
-template <class Real>
-__global__ void __launch_bounds__(256) wpraq_node(const PraqJob<Real>* jobs, const int2* tab,
-                                                  const int* round_action, int mode, int j) {
+__global__ void __launch_bounds__(256) wpraq_node(const PraqJob<float>* j32,
+                                                  const PraqJob<double>* j64, const int* prec,
+                                                  const int2* tab, const int* round_action,
+                                                  int mode, int j) {
     const int2 t = tab[blockIdx.x];
-    const PraqJob<Real>& job = jobs[t.x];
-    PraqStep<Real> sc;
-    if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
-    const int lanes = job.lanes;
-    const int gid = t.y * kWideBlock + threadIdx.x;
-    praq_node(job.a, sc, gid / lanes, gid % lanes, lanes);
+    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+        using Real = RealOf<decltype(job)>;
+        PraqStep<Real> sc;
+        if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
+        const int lanes = job.lanes;
+        const int gid = t.y * kWideBlock + threadIdx.x;
+        praq_node(job.a, sc, gid / lanes, gid % lanes, lanes);
+    });
 }
 
-template <class Real>
-__global__ void __launch_bounds__(256) wpraq_edge(const PraqJob<Real>* jobs, const int2* tab,
-                                                  const int* round_action, int mode, int j) {
+__global__ void __launch_bounds__(256) wpraq_edge(const PraqJob<float>* j32,
+                                                  const PraqJob<double>* j64, const int* prec,
+                                                  const int2* tab, const int* round_action,
+                                                  int mode, int j) {
     const int2 t = tab[blockIdx.x];
-    const PraqJob<Real>& job = jobs[t.x];
-    PraqStep<Real> sc;
-    if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
-    const int e = t.y * kWideBlock + threadIdx.x;
-    if (e < job.a.m) praq_edge(job.a, sc, e);
+    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+        using Real = RealOf<decltype(job)>;
+        PraqStep<Real> sc;
+        if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
+        const int e = t.y * kWideBlock + threadIdx.x;
+        if (e < job.a.m) praq_edge(job.a, sc, e);
+    });
 }
 
 // node blocks, then edge blocks; one partial row per block
-template <class Real>
-__global__ void __launch_bounds__(256) wpraq_sums(const PraqJob<Real>* jobs, const int2* node_tab,
-                                                  int node_blocks, const int2* edge_tab,
-                                                  const int* round_action, int mode,
-                                                  double* partial) {
+__global__ void __launch_bounds__(256) wpraq_sums(const PraqJob<float>* j32,
+                                                  const PraqJob<double>* j64, const int* prec,
+                                                  const int2* node_tab, int node_blocks,
+                                                  const int2* edge_tab, const int* round_action,
+                                                  int mode, double* partial) {
     __shared__ double red[kPraqSums * 16];
     const bool node = int(blockIdx.x) < node_blocks;
     const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
-    const PraqJob<Real>& job = jobs[t.x];
-    const int where = wide_sums_point(job, round_action, t.x, mode);
-    if (!where) return;
-    const PraqArrays<Real>& a = job.a;
-    const Real* bx = where == 2 ? a.sx : a.x;
-    const Real* by = where == 2 ? a.sy : a.y;
-    double acc[kPraqSums];
+    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+        using Real = RealOf<decltype(job)>;
+        const int where = wide_sums_point(job, round_action, t.x, mode);
+        if (!where) return;
+        const PraqArrays<Real>& a = job.a;
+        const Real* bx = where == 2 ? a.sx : a.x;
+        const Real* by = where == 2 ? a.sy : a.y;
+        double acc[kPraqSums];
 #pragma unroll
-    for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
-    const int gid = t.y * kWideBlock + threadIdx.x;
-    if (node)
-        praq_node_sums(a, gid / job.lanes, gid % job.lanes, job.lanes, bx, by, acc);
-    else if (gid < a.m)
-        praq_edge_sums(a, gid, bx, by, true, acc);
-    block_sums<kPraqSums>(acc, red);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < kPraqSums; ++k) partial[size_t(blockIdx.x) * kPraqSums + k] = acc[k];
+        for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
+        const int gid = t.y * kWideBlock + threadIdx.x;
+        if (node)
+            praq_node_sums(a, gid / job.lanes, gid % job.lanes, job.lanes, bx, by, acc);
+        else if (gid < a.m)
+            praq_edge_sums(a, gid, bx, by, true, acc);
+        block_sums<kPraqSums>(acc, red);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < kPraqSums; ++k)
+                partial[size_t(blockIdx.x) * kPraqSums + k] = acc[k];
+    });
 }
 
 // one block per instance: its rows in a fixed order, then the logic
-template <class Real>
-__global__ void __launch_bounds__(256) wpraq_logic(const PraqJob<Real>* jobs, const int* first,
-                                                   int node_blocks, const double* partial,
-                                                   const int* round_action, int* last_action,
-                                                   int64_t* steps, int mode) {
+__global__ void __launch_bounds__(256) wpraq_logic(const PraqJob<float>* j32,
+                                                   const PraqJob<double>* j64, const int* prec,
+                                                   int count, const int* first, int node_blocks,
+                                                   const double* partial, const int* round_action,
+                                                   int* last_action, int64_t* steps, int mode) {
     __shared__ double red[kPraqSums * 16];
     const int inst = blockIdx.x;
-    const PraqJob<Real>& job = jobs[inst];
-    const int act = wide_logic_action(job, round_action, inst, mode);
-    if (act < 0) return;
-    const int nf = first[4 * inst], nc = first[4 * inst + 1];
-    const int ef = first[4 * inst + 2], ec = first[4 * inst + 3];
-    double acc[kPraqSums];
+    dispatch(j32, j64, prec, inst, [&](const auto& job) {
+        const int act = wide_logic_action(job, round_action, inst, mode);
+        if (act < 0) return;
+        const int nf = first[4 * inst], nc = first[4 * inst + 1];
+        const int ef = first[4 * inst + 2], ec = first[4 * inst + 3];
+        double acc[kPraqSums];
 #pragma unroll
-    for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
-    for (int r = threadIdx.x; r < nc; r += blockDim.x)
-        for (int k = 0; k < kPraqSums; ++k) acc[k] += partial[size_t(nf + r) * kPraqSums + k];
-    for (int r = threadIdx.x; r < ec; r += blockDim.x)
-        for (int k = 0; k < kPraqSums; ++k)
-            acc[k] += partial[size_t(node_blocks + ef + r) * kPraqSums + k];
-    block_sums<kPraqSums>(acc, red);
-    if (threadIdx.x == 0)
-        wide_apply(job, act, praq_check_sums(acc, job.scale), last_action, steps, inst);
+        for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
+        for (int r = threadIdx.x; r < nc; r += blockDim.x)
+            for (int k = 0; k < kPraqSums; ++k) acc[k] += partial[size_t(nf + r) * kPraqSums + k];
+        for (int r = threadIdx.x; r < ec; r += blockDim.x)
+            for (int k = 0; k < kPraqSums; ++k)
+                acc[k] += partial[size_t(node_blocks + ef + r) * kPraqSums + k];
+        block_sums<kPraqSums>(acc, red);
+        if (threadIdx.x == 0)
+            wide_apply(job, act, praq_check_sums(acc, job.scale), last_action,
+                       steps + size_t(prec[inst]) * count, inst);
+    });
 }
 
-template <class Real>
-__global__ void __launch_bounds__(256) wpraq_copy(const PraqJob<Real>* jobs, const int2* node_tab,
-                                                  int node_blocks, const int2* edge_tab,
-                                                  const int* pending, int mode) {
+// copies over node blocks then edge blocks, in the instance's precision
+__global__ void __launch_bounds__(256) wpraq_copy(const PraqJob<float>* j32,
+                                                  const PraqJob<double>* j64, const int* prec,
+                                                  const int2* node_tab, int node_blocks,
+                                                  const int2* edge_tab, const int* pending,
+                                                  int mode) {
     const bool node = int(blockIdx.x) < node_blocks;
     const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
-    const PraqJob<Real>& job = jobs[t.x];
-    if (!wide_copies(job, pending, t.x, mode)) return;
-    const PraqArrays<Real>& a = job.a;
+    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+        if (!wide_copies(job, pending, t.x, mode)) return;
+        const auto& a = job.a;
+        const int gid = t.y * kWideBlock + threadIdx.x;
+        if (node) {
+            if (gid % job.lanes) return;
+            const int i = gid / job.lanes;
+            if (i >= a.n) return;
+            if (mode <= 1) a.x0[i] = a.x[i];
+            else if (mode == 2) a.x[i] = a.sx[i];
+            else a.x0[i] = a.x[i] = a.sx[i] = a.xs[i];
+        } else {
+            if (gid >= a.m) return;
+            for (int slot = 0; slot < 2; ++slot) {
+                const int k = slot * a.m + gid;
+                if (mode <= 1) a.y0[k] = a.y[k];
+                else if (mode == 2) a.y[k] = a.sy[k];
+                else a.y0[k] = a.y[k] = a.sy[k] = a.ys[k];
+            }
+        }
+    });
+}
+
+// FP32 instances whose logic asked for FP64: their Halpern state, block by
+// block, into the FP64 image (the precision flips in wide_promote)
+__global__ void __launch_bounds__(256) wpraq_convert(const PraqJob<float>* j32,
+                                                     const PraqJob<double>* j64, const int* prec,
+                                                     const int2* node_tab, int node_blocks,
+                                                     const int2* edge_tab) {
+    const bool node = int(blockIdx.x) < node_blocks;
+    const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
+    if (prec[t.x] != 0 || !j64[t.x].state->stalled) return;
+    const PraqArrays<float>& f = j32[t.x].a;
+    const PraqArrays<double>& d = j64[t.x].a;
     const int gid = t.y * kWideBlock + threadIdx.x;
     if (node) {
-        if (gid % job.lanes) return;
-        const int i = gid / job.lanes;
-        if (i >= a.n) return;
-        if (mode <= 1) a.x0[i] = a.x[i];
-        else if (mode == 2) a.x[i] = a.sx[i];
-        else a.x0[i] = a.x[i] = a.sx[i] = a.xs[i];
+        const int lanes = j32[t.x].lanes;
+        if (gid % lanes) return;
+        const int i = gid / lanes;
+        if (i >= f.n) return;
+        d.x[i] = f.x[i];
+        d.sx[i] = f.sx[i];
+        d.x0[i] = f.x0[i];
     } else {
-        if (gid >= a.m) return;
+        if (gid >= f.m) return;
         for (int slot = 0; slot < 2; ++slot) {
-            const int k = slot * a.m + gid;
-            if (mode <= 1) a.y0[k] = a.y[k];
-            else if (mode == 2) a.y[k] = a.sy[k];
-            else a.y0[k] = a.y[k] = a.sy[k] = a.ys[k];
+            const int k = slot * f.m + gid;
+            d.yc[k] = f.yc[k];
+            d.y[k] = f.y[k];
+            d.sy[k] = f.sy[k];
+            d.y0[k] = f.y0[k];
         }
     }
 }
 
 // ---------------------------------------------------------------- PRIA ----
 
-template <class Real>
-__global__ void __launch_bounds__(256) wpria_x(const PriaJob<Real>* jobs, const int4* tab,
-                                               const int* round_action, int mode, int j) {
+__global__ void __launch_bounds__(256) wpria_x(const PriaJob<float>* j32,
+                                               const PriaJob<double>* j64, const int* prec,
+                                               const int4* tab, const int* round_action,
+                                               int mode, int j) {
     extern __shared__ char dyn[];
     const int4 t = tab[blockIdx.x];
-    const PriaJob<Real>& job = jobs[t.x];
-    PriaStep<Real> sc;
-    if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
-    pria_x_tile<Real, PriaShape<Real>::TW>(job.a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
+    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+        using Real = RealOf<decltype(job)>;
+        PriaStep<Real> sc;
+        if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+        pria_x_tile<Real, kWideTW>(job.a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
+    });
 }
 
-template <class Real>
-__global__ void __launch_bounds__(1024) wpria_yz(const PriaJob<Real>* jobs, const int2* tab,
-                                                 const int* round_action, int mode, int j) {
+__global__ void __launch_bounds__(1024) wpria_yz(const PriaJob<float>* j32,
+                                                 const PriaJob<double>* j64, const int* prec,
+                                                 const int2* tab, const int* round_action,
+                                                 int mode, int j) {
     __shared__ double red[4 * 16];
     const int2 t = tab[blockIdx.x];
-    const PriaJob<Real>& job = jobs[t.x];
-    PriaStep<Real> sc;
-    if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
-    pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(job.a, sc, t.y, red);
+    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+        using Real = RealOf<decltype(job)>;
+        PriaStep<Real> sc;
+        if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+        pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(job.a, sc, t.y, red);
+    });
 }
 
 // slice blocks (six sums each), then tile blocks (a column-minimum sum each,
 // stored at the tile's (item, tile) row)
-template <class Real>
-__global__ void __launch_bounds__(256) wpria_sums(const PriaJob<Real>* jobs, const int2* s_tab,
-                                                  int s_blocks, const int4* x_tab,
-                                                  const int* round_action, int mode,
-                                                  double* partial) {
+__global__ void __launch_bounds__(256) wpria_sums(const PriaJob<float>* j32,
+                                                  const PriaJob<double>* j64, const int* prec,
+                                                  const int2* s_tab, int s_blocks,
+                                                  const int4* x_tab, const int* round_action,
+                                                  int mode, double* partial) {
     __shared__ double red[kPriaSums * 16];
     const bool slice = int(blockIdx.x) < s_blocks;
     const int inst = slice ? s_tab[blockIdx.x].x : x_tab[blockIdx.x - s_blocks].x;
-    const PriaJob<Real>& job = jobs[inst];
-    const int where = wide_sums_point(job, round_action, inst, mode);
-    if (!where) return;
-    const PriaArrays<Real>& a = job.a;
-    double acc[kPriaSums];
-    for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
-    if (slice) {
-        pria_slice_sums(a, s_tab[blockIdx.x].y, where == 2 ? a.sx : a.x, red, acc);
-        if (threadIdx.x == 0)
-            for (int q = 0; q < kPLower; ++q) partial[size_t(blockIdx.x) * kPriaSums + q] = acc[q];
-    } else {
-        const int4 t = x_tab[blockIdx.x - s_blocks];
-        pria_column_sums<Real, PriaShape<Real>::TW>(a, t.y, t.z, where == 2 ? a.sy : a.y,
-                                                    where == 2 ? a.sz : a.z, red, acc);
-        if (threadIdx.x == 0) partial[size_t(s_blocks + t.w) * kPriaSums] = acc[kPLower];
-    }
+    dispatch(j32, j64, prec, inst, [&](const auto& job) {
+        using Real = RealOf<decltype(job)>;
+        const int where = wide_sums_point(job, round_action, inst, mode);
+        if (!where) return;
+        const PriaArrays<Real>& a = job.a;
+        double acc[kPriaSums];
+        for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
+        if (slice) {
+            pria_slice_sums(a, s_tab[blockIdx.x].y, where == 2 ? a.sx : a.x, red, acc);
+            if (threadIdx.x == 0)
+                for (int q = 0; q < kPLower; ++q)
+                    partial[size_t(blockIdx.x) * kPriaSums + q] = acc[q];
+        } else {
+            const int4 t = x_tab[blockIdx.x - s_blocks];
+            pria_column_sums<Real, kWideTW>(a, t.y, t.z, where == 2 ? a.sy : a.y,
+                                            where == 2 ? a.sz : a.z, red, acc);
+            if (threadIdx.x == 0) partial[size_t(s_blocks + t.w) * kPriaSums] = acc[kPLower];
+        }
+    });
 }
 
-template <class Real>
-__global__ void __launch_bounds__(256) wpria_logic(const PriaJob<Real>* jobs, const int* first,
-                                                   int s_blocks, const double* partial,
-                                                   const int* round_action, int* last_action,
-                                                   int64_t* steps, int mode) {
+__global__ void __launch_bounds__(256) wpria_logic(const PriaJob<float>* j32,
+                                                   const PriaJob<double>* j64, const int* prec,
+                                                   int count, const int* first, int s_blocks,
+                                                   const double* partial, const int* round_action,
+                                                   int* last_action, int64_t* steps, int mode) {
     __shared__ double red[kPriaSums * 16];
     const int inst = blockIdx.x;
-    const PriaJob<Real>& job = jobs[inst];
-    const int act = wide_logic_action(job, round_action, inst, mode);
-    if (act < 0) return;
-    const int sf = first[4 * inst], sc = first[4 * inst + 1];
-    const int tf = first[4 * inst + 2], tc = first[4 * inst + 3];
-    double acc[kPriaSums];
-    for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
-    for (int r = threadIdx.x; r < sc; r += blockDim.x)
-        for (int q = 0; q < kPLower; ++q) acc[q] += partial[size_t(sf + r) * kPriaSums + q];
-    for (int r = threadIdx.x; r < tc; r += blockDim.x)
-        acc[kPLower] += partial[size_t(s_blocks + tf + r) * kPriaSums];
-    block_sums<kPriaSums>(acc, red);
-    if (threadIdx.x == 0)
-        wide_apply(job, act, pria_check_sums(acc, job.scale), last_action, steps, inst);
+    dispatch(j32, j64, prec, inst, [&](const auto& job) {
+        const int act = wide_logic_action(job, round_action, inst, mode);
+        if (act < 0) return;
+        const int sf = first[4 * inst], sc = first[4 * inst + 1];
+        const int tf = first[4 * inst + 2], tc = first[4 * inst + 3];
+        double acc[kPriaSums];
+        for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
+        for (int r = threadIdx.x; r < sc; r += blockDim.x)
+            for (int q = 0; q < kPLower; ++q) acc[q] += partial[size_t(sf + r) * kPriaSums + q];
+        for (int r = threadIdx.x; r < tc; r += blockDim.x)
+            acc[kPLower] += partial[size_t(s_blocks + tf + r) * kPriaSums];
+        block_sums<kPriaSums>(acc, red);
+        if (threadIdx.x == 0)
+            wide_apply(job, act, pria_check_sums(acc, job.scale), last_action,
+                       steps + size_t(prec[inst]) * count, inst);
+    });
 }
 
-// one block per slice; copy modes as wide_copies
-template <class Real>
-__global__ void __launch_bounds__(256) wpria_copy(const PriaJob<Real>* jobs, const int2* s_tab,
-                                                  const int* pending, int mode) {
+// one block per slice, in the instance's precision; copy modes as
+// wide_copies, the start also clearing the slice's thresholds
+__global__ void __launch_bounds__(256) wpria_copy(const PriaJob<float>* j32,
+                                                  const PriaJob<double>* j64, const int* prec,
+                                                  const int2* s_tab, const int* pending,
+                                                  int mode) {
     const int2 t = s_tab[blockIdx.x];
-    const PriaJob<Real>& job = jobs[t.x];
-    if (!wide_copies(job, pending, t.x, mode)) return;
-    const PriaArrays<Real>& a = job.a;
-    const int s = t.y, base = pria_off(a.N, a.T, s), L = a.N * (a.T - s);
-    if (mode == 3) {  // no thresholds yet: column (i, s), row (i, s), slice s
-        const double none = __builtin_nan("");
-        for (int i = threadIdx.x; i < a.N; i += blockDim.x) {
-            a.thx[i * a.T + s] = none;
-            a.thy[i * a.T + s] = none;
+    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+        if (!wide_copies(job, pending, t.x, mode)) return;
+        const auto& a = job.a;
+        const int s = t.y, base = pria_off(a.N, a.T, s), L = a.N * (a.T - s);
+        if (mode == 3) {  // no thresholds yet: column (i, s), row (i, s), slice s
+            const double none = __builtin_nan("");
+            for (int i = threadIdx.x; i < a.N; i += blockDim.x) {
+                a.thx[i * a.T + s] = none;
+                a.thy[i * a.T + s] = none;
+            }
+            if (threadIdx.x == 0) a.thz[s] = none;
         }
-        if (threadIdx.x == 0) a.thz[s] = none;
+        for (int e = threadIdx.x; e < L; e += blockDim.x) {
+            const int g = base + e;
+            if (mode <= 1) {
+                a.x0[g] = a.x[g];
+                a.y0[g] = a.y[g];
+                a.z0[g] = a.z[g];
+            } else if (mode == 2) {
+                a.x[g] = a.sx[g];
+                a.y[g] = a.sy[g];
+                a.z[g] = a.sz[g];
+            } else {
+                a.x0[g] = a.x[g] = a.sx[g] = a.xs[g];
+                a.y0[g] = a.y[g] = a.sy[g] = a.ys[g];
+                a.z0[g] = a.z[g] = a.sz[g] = a.zs[g];
+            }
+        }
+    });
+}
+
+// FP32 instances whose logic asked for FP64: one slice's state (and the
+// thresholds of the slice's column, row and slice) into the FP64 image
+__global__ void __launch_bounds__(256) wpria_convert(const PriaJob<float>* j32,
+                                                     const PriaJob<double>* j64, const int* prec,
+                                                     const int2* s_tab) {
+    const int2 t = s_tab[blockIdx.x];
+    if (prec[t.x] != 0 || !j64[t.x].state->stalled) return;
+    const PriaArrays<float>& f = j32[t.x].a;
+    const PriaArrays<double>& d = j64[t.x].a;
+    const int s = t.y, base = pria_off(f.N, f.T, s), L = f.N * (f.T - s);
+    for (int i = threadIdx.x; i < f.N; i += blockDim.x) {
+        d.thx[i * f.T + s] = f.thx[i * f.T + s];
+        d.thy[i * f.T + s] = f.thy[i * f.T + s];
     }
+    if (threadIdx.x == 0) d.thz[s] = f.thz[s];
     for (int e = threadIdx.x; e < L; e += blockDim.x) {
         const int g = base + e;
-        if (mode <= 1) {
-            a.x0[g] = a.x[g];
-            a.y0[g] = a.y[g];
-            a.z0[g] = a.z[g];
-        } else if (mode == 2) {
-            a.x[g] = a.sx[g];
-            a.y[g] = a.sy[g];
-            a.z[g] = a.sz[g];
-        } else {
-            a.x0[g] = a.x[g] = a.sx[g] = a.xs[g];
-            a.y0[g] = a.y[g] = a.sy[g] = a.ys[g];
-            a.z0[g] = a.z[g] = a.sz[g] = a.zs[g];
-        }
+        d.x0[g] = f.x0[g];
+        d.y0[g] = f.y0[g];
+        d.z0[g] = f.z0[g];
+        d.xc[g] = f.xc[g];
+        d.yc[g] = f.yc[g];
+        d.zc[g] = f.zc[g];
+        d.x[g] = f.x[g];
+        d.sx[g] = f.sx[g];
+        d.y[g] = f.y[g];
+        d.sy[g] = f.sy[g];
+        d.z[g] = f.z[g];
+        d.sz[g] = f.sz[g];
     }
 }
 
@@ -390,7 +503,7 @@ __global__ void __launch_bounds__(256) wpria_copy(const PriaJob<Real>* jobs, con
 
 // ------------------------------------------------------------------ runs --
 
-int WideRun::begin(WideControl* engine, hipStream_t s, bool resume, double limit) {
+int WideRun::begin(WideControl* engine, hipStream_t s, double limit) {
     w = engine;
     stream = s;
     time_limit = limit;
@@ -402,7 +515,7 @@ int WideRun::begin(WideControl* engine, hipStream_t s, bool resume, double limit
     w->mark_set = 2;
     w->marks_used[2] = 0;
     w->mark_kinds[2].clear();
-    if (w->start(stream, resume)) return kErrorDevice;
+    if (w->start(stream)) return kErrorDevice;
     // the round graph (timed rounds are launched directly instead: events
     // recorded by graph nodes carry no timestamps for hipEventElapsedTime)
     if (!w->timing && !w->exec) {
@@ -457,9 +570,10 @@ int WideRun::advance() {
             w->marks_used[slot] = 0;
             w->mark_kinds[slot].clear();
             if (w->enqueue_round(stream)) return kErrorDevice;
-        } else
+        } else {
             for (int r = 0; r < kRoundsPerPoll; ++r)
                 REFORM_HIP_CHECK(hipGraphLaunch(w->exec, stream));
+        }
         REFORM_HIP_CHECK(hipMemcpyAsync(w->host_active + slot, w->active, sizeof(int),
                                         hipMemcpyDeviceToHost, stream));
         REFORM_HIP_CHECK(hipEventRecord(w->polled[slot], stream));
@@ -469,9 +583,9 @@ int WideRun::advance() {
     return changed;
 }
 
-int WideControl::solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats) {
+int WideControl::solve(hipStream_t stream, double time_limit, WideStats* stats) {
     WideRun run;
-    if (run.begin(this, stream, resume, time_limit)) return kErrorDevice;
+    if (run.begin(this, stream, time_limit)) return kErrorDevice;
     while (!run.finished) {
         const int st = run.advance();
         if (st < 0) return st;
@@ -529,14 +643,23 @@ void WideControl::clear_timing() {
 void WideControl::set_timing(bool on) { timing = on; }
 
 int WideControl::algorithmic_bytes(double (&out)[kKinds]) const {
-    std::vector<int64_t> n(count);
-    if (count) REFORM_HIP_CHECK(hipMemcpy(n.data(), steps, sizeof(int64_t) * count,
-                                          hipMemcpyDeviceToHost));
+    std::vector<int64_t> n(2 * size_t(count));
+    if (count)
+        REFORM_HIP_CHECK(
+            hipMemcpy(n.data(), steps, sizeof(int64_t) * n.size(), hipMemcpyDeviceToHost));
     for (int k = 0; k < kKinds; ++k) {
         out[k] = 0.0;
-        for (int q = 0; q < count && q < int(step_bytes[k].size()); ++q)
-            out[k] += double(n[q]) * step_bytes[k][q];
+        for (int p = 0; p < 2; ++p)
+            for (int q = 0; q < count && q < int(step_bytes[k][p].size()); ++q)
+                out[k] += double(n[size_t(p) * count + q]) * step_bytes[k][p][q];
     }
+    return 0;
+}
+
+int WideControl::precisions(std::vector<int>& out) const {
+    out.assign(count, 1);
+    if (count)
+        REFORM_HIP_CHECK(hipMemcpy(out.data(), prec, sizeof(int) * count, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -547,9 +670,9 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     const size_t o_partial = at;
     at = align256(at + sizeof(double) * partial_width * (partial_rows + 1));
     const size_t o_steps = at;
-    at = align256(at + sizeof(int64_t) * size_t(count));
+    at = align256(at + sizeof(int64_t) * 2 * size_t(count));
     const size_t o_ctrl = at;
-    at = align256(at + sizeof(int) * (3 * size_t(count) + 1));
+    at = align256(at + sizeof(int) * (4 * size_t(count) + 1));
     {
         hipError_t e = hipMalloc(&arena, at);
         if (e != hipSuccess) {
@@ -566,7 +689,8 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     round_action = reinterpret_cast<int*>(b + o_ctrl);
     last_action = round_action + count;
     pending = last_action + count;
-    active = pending + count;
+    prec = pending + count;
+    active = prec + count;
     {
         hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host_active), 2 * sizeof(int),
                                      hipHostMallocDefault);
@@ -583,18 +707,27 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
 
 // ------------------------------------------------------------- PRAQ host --
 
-template <class Real>
-int WidePraq<Real>::setup(const std::vector<PraqJob<Real>>& host_jobs, PraqJob<Real>* dev_jobs) {
-    count = int(host_jobs.size());
-    jobs = dev_jobs;
+int WidePraq::setup(const std::vector<PraqJob<double>>& host64, const PraqJob<float>* dev32,
+                    const PraqJob<double>* dev64) {
+    count = int(host64.size());
+    jobs32 = dev32;
+    jobs64 = dev64;
+    has32 = dev32 != nullptr;
     if (!count) return 0;
     std::vector<int2> nt, et;
     std::vector<int> fi(4 * size_t(count));
     op_steps = 2;
+    // algorithmic bytes of one step, b = bytes per value: node kernel: ptr 4,
+    // six per-node values, the reflected gather, and (index 4 + value b) per
+    // half-edge of the CSR scatter; edge kernel: ends 8, weight, the y pair,
+    // the anchor pair and the written pair
+    for (int k = 0; k < kKinds; ++k)
+        for (int p = 0; p < 2; ++p) step_bytes[k][p].resize(count);
     for (int q = 0; q < count; ++q) {
-        const auto& a = host_jobs[q].a;
-        op_steps = std::max(op_steps, int(host_jobs[q].p.op_steps));
-        const int nb = int((int64_t(a.n) * host_jobs[q].lanes + kWideBlock - 1) / kWideBlock);
+        const auto& a = host64[q].a;
+        const int lanes = host64[q].lanes;
+        op_steps = std::max(op_steps, int(host64[q].p.op_steps));
+        const int nb = int((int64_t(a.n) * lanes + kWideBlock - 1) / kWideBlock);
         const int eb = (a.m + kWideBlock - 1) / kWideBlock;
         fi[4 * q] = int(nt.size());
         fi[4 * q + 1] = nb;
@@ -602,20 +735,14 @@ int WidePraq<Real>::setup(const std::vector<PraqJob<Real>>& host_jobs, PraqJob<R
         fi[4 * q + 3] = eb;
         for (int b = 0; b < nb; ++b) nt.push_back(make_int2(q, b));
         for (int b = 0; b < eb; ++b) et.push_back(make_int2(q, b));
+        for (int p = 0; p < 2; ++p) {
+            const double b = p ? 8.0 : 4.0, n = a.n, m = a.m;
+            step_bytes[0][p][q] = n * (4.0 + 7.0 * b) + 2.0 * m * (4.0 + b);
+            step_bytes[1][p][q] = m * (8.0 + 7.0 * b);
+        }
     }
     node_blocks = int(nt.size());
     edge_blocks = int(et.size());
-    // algorithmic bytes of one step, b = sizeof(Real): node kernel: ptr 4,
-    // six per-node values, the reflected gather, and (index 4 + value b) per
-    // half-edge of the CSR scatter; edge kernel: ends 8, weight, the y pair,
-    // the anchor pair and the written pair
-    const double b = sizeof(Real);
-    for (int k = 0; k < kKinds; ++k) step_bytes[k].resize(count);
-    for (int q = 0; q < count; ++q) {
-        const double n = host_jobs[q].a.n, m = host_jobs[q].a.m;
-        step_bytes[0][q] = n * (4.0 + 7.0 * b) + 2.0 * m * (4.0 + b);
-        step_bytes[1][q] = m * (8.0 + 7.0 * b);
-    }
     const size_t tab_bytes = sizeof(int2) * (nt.size() + et.size());
     if (allocate(tab_bytes, nt.size() + et.size(), kPraqSums)) return kErrorDevice;
     node_tab = static_cast<int2*>(arena);
@@ -629,104 +756,114 @@ int WidePraq<Real>::setup(const std::vector<PraqJob<Real>>& host_jobs, PraqJob<R
 
 // One step of every instance the mode selects: the node kernel, then the
 // edge kernel, each bracketed by timing marks.
-template <class Real>
-int WidePraq<Real>::step(hipStream_t stream, int mode, int j) {
+int WidePraq::step(hipStream_t stream, int mode, int j) {
     const dim3 blk(kWideBlock);
     if (node_blocks) {
         if (mark(stream, 0, true)) return kErrorDevice;
-        hipLaunchKernelGGL(wpraq_node<Real>, dim3(node_blocks), blk, 0, stream, jobs, node_tab,
-                           round_action, mode, j);
+        hipLaunchKernelGGL(wpraq_node, dim3(node_blocks), blk, 0, stream, jobs32, jobs64, prec,
+                           node_tab, round_action, mode, j);
         if (mark(stream, 0, false)) return kErrorDevice;
     }
     if (edge_blocks) {
         if (mark(stream, 1, true)) return kErrorDevice;
-        hipLaunchKernelGGL(wpraq_edge<Real>, dim3(edge_blocks), blk, 0, stream, jobs, edge_tab,
-                           round_action, mode, j);
+        hipLaunchKernelGGL(wpraq_edge, dim3(edge_blocks), blk, 0, stream, jobs32, jobs64, prec,
+                           edge_tab, round_action, mode, j);
         if (mark(stream, 1, false)) return kErrorDevice;
     }
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-template <class Real>
-int WidePraq<Real>::start(hipStream_t stream, bool resume) {
+int WidePraq::start(hipStream_t stream) {
     const int all = node_blocks + edge_blocks;
     const dim3 blk(kWideBlock);
-    hipLaunchKernelGGL(wide_init<PraqJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
-                       count, round_action, last_action, steps, resume ? 1 : 0);
-    if (!resume) {
-        hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab,
-                           node_blocks, edge_tab, pending, 3);
-        if (step(stream, 0, 0)) return kErrorDevice;
-        hipLaunchKernelGGL(wpraq_sums<Real>, dim3(all), blk, 0, stream, jobs, node_tab,
-                           node_blocks, edge_tab, round_action, 0, partial);
-        hipLaunchKernelGGL(wpraq_logic<Real>, dim3(count), blk, 0, stream, jobs, first,
-                           node_blocks, partial, round_action, last_action, steps, 0);
-        hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab,
-                           node_blocks, edge_tab, pending, 2);
-    }
-    hipLaunchKernelGGL(wide_count<PraqJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    hipLaunchKernelGGL(wide_init<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
+                       count, round_action, last_action, steps, prec, has32 ? 0 : 1);
+    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+                       node_blocks, edge_tab, pending, 3);
+    if (step(stream, 0, 0)) return kErrorDevice;
+    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+                       node_blocks, edge_tab, round_action, 0, partial);
+    hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
+                       first, node_blocks, partial, round_action, last_action, steps, 0);
+    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+                       node_blocks, edge_tab, pending, 2);
+    hipLaunchKernelGGL(wide_count<PraqJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
+                       active);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-template <class Real>
-int WidePraq<Real>::enqueue_round(hipStream_t stream) {
+int WidePraq::enqueue_round(hipStream_t stream) {
     const int all = node_blocks + edge_blocks;
     const dim3 blk(kWideBlock);
-    hipLaunchKernelGGL(wide_pre<PraqJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs, count,
-                       round_action, pending);
-    hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
-                       edge_tab, pending, 0);
+    hipLaunchKernelGGL(wide_pre<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
+                       count, round_action, pending);
+    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+                       node_blocks, edge_tab, pending, 0);
     for (int j = 0; j < op_steps; ++j)
         if (step(stream, 0, j)) return kErrorDevice;
-    hipLaunchKernelGGL(wpraq_sums<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
-                       edge_tab, round_action, 0, partial);
-    hipLaunchKernelGGL(wpraq_logic<Real>, dim3(count), blk, 0, stream, jobs, first, node_blocks,
-                       partial, round_action, last_action, steps, 0);
+    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+                       node_blocks, edge_tab, round_action, 0, partial);
+    hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
+                       first, node_blocks, partial, round_action, last_action, steps, 0);
+    if (has32) {
+        hipLaunchKernelGGL(wpraq_convert, dim3(all), blk, 0, stream, jobs32, jobs64, prec,
+                           node_tab, node_blocks, edge_tab);
+        hipLaunchKernelGGL(wide_promote<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream,
+                           jobs64, count, prec);
+    }
     // restarts: one uncounted vanilla step, the new anchor, the norms
     if (step(stream, 1, 0)) return kErrorDevice;
-    hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
-                       edge_tab, pending, 1);
-    hipLaunchKernelGGL(wpraq_sums<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
-                       edge_tab, round_action, 1, partial);
-    hipLaunchKernelGGL(wpraq_logic<Real>, dim3(count), blk, 0, stream, jobs, first, node_blocks,
-                       partial, round_action, last_action, steps, 1);
-    hipLaunchKernelGGL(wpraq_copy<Real>, dim3(all), blk, 0, stream, jobs, node_tab, node_blocks,
-                       edge_tab, pending, 2);
-    hipLaunchKernelGGL(wide_count<PraqJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+                       node_blocks, edge_tab, pending, 1);
+    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+                       node_blocks, edge_tab, round_action, 1, partial);
+    hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
+                       first, node_blocks, partial, round_action, last_action, steps, 1);
+    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+                       node_blocks, edge_tab, pending, 2);
+    hipLaunchKernelGGL(wide_count<PraqJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
+                       active);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-template <class Real>
-int WidePraq<Real>::expire(hipStream_t stream) {
+int WidePraq::expire(hipStream_t stream) {
     const dim3 blk(kWideBlock);
-    hipLaunchKernelGGL(wide_expire<PraqJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
-                       count, last_action);
-    hipLaunchKernelGGL(wpraq_copy<Real>, dim3(node_blocks + edge_blocks), blk, 0, stream, jobs,
-                       node_tab, node_blocks, edge_tab, pending, 2);
-    hipLaunchKernelGGL(wide_count<PraqJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    hipLaunchKernelGGL(wide_expire<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream,
+                       jobs64, count, last_action);
+    hipLaunchKernelGGL(wpraq_copy, dim3(node_blocks + edge_blocks), blk, 0, stream, jobs32,
+                       jobs64, prec, node_tab, node_blocks, edge_tab, pending, 2);
+    hipLaunchKernelGGL(wide_count<PraqJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
+                       active);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
 // ------------------------------------------------------------- PRIA host --
 
-template <class Real>
-int WidePria<Real>::setup(const std::vector<PriaJob<Real>>& host_jobs, PriaJob<Real>* dev_jobs) {
-    constexpr int TW = PriaShape<Real>::TW;
-    count = int(host_jobs.size());
-    jobs = dev_jobs;
+int WidePria::setup(const std::vector<PriaJob<double>>& host64, const PriaJob<float>* dev32,
+                    const PriaJob<double>* dev64) {
+    constexpr int TW = kWideTW;
+    count = int(host64.size());
+    jobs32 = dev32;
+    jobs64 = dev64;
+    has32 = dev32 != nullptr;
     if (!count) return 0;
     std::vector<int4> xt;
     std::vector<int2> st;
     std::vector<int> fi(4 * size_t(count));
     op_steps = 2;
     int tile_rows = 0;
+    // algorithmic bytes of one step per entry, b = bytes per value: x kernel
+    // reads x, x0, h, y, z and writes x, r (7 b); y/z kernel reads y, y0, r,
+    // z, z0, r and writes y, z (8 b)
+    for (int k = 0; k < kKinds; ++k)
+        for (int p = 0; p < 2; ++p) step_bytes[k][p].resize(count);
     for (int q = 0; q < count; ++q) {
-        const auto& a = host_jobs[q].a;
-        op_steps = std::max(op_steps, int(host_jobs[q].p.op_steps));
+        const auto& a = host64[q].a;
+        op_steps = std::max(op_steps, int(host64[q].p.op_steps));
         max_T = std::max(max_T, a.T);
         const int tiles = (a.T + TW - 1) / TW;
         fi[4 * q] = int(st.size());
@@ -739,19 +876,14 @@ int WidePria<Real>::setup(const std::vector<PriaJob<Real>>& host_jobs, PriaJob<R
             for (int i = 0; i < a.N; ++i)
                 xt.push_back(make_int4(q, i, k * TW, tile_rows + i * tiles + k));
         tile_rows += a.N * tiles;
+        for (int p = 0; p < 2; ++p) {
+            const double b = p ? 8.0 : 4.0, nd = a.ND;
+            step_bytes[0][p][q] = 7.0 * b * nd;
+            step_bytes[1][p][q] = 8.0 * b * nd;
+        }
     }
     x_blocks = int(xt.size());
     s_blocks = int(st.size());
-    // algorithmic bytes of one step per entry, b = sizeof(Real): x kernel
-    // reads x, x0, h, y, z and writes x, r (7 b); y/z kernel reads y, y0, r,
-    // z, z0, r and writes y, z (8 b)
-    const double b = sizeof(Real);
-    for (int k = 0; k < kKinds; ++k) step_bytes[k].resize(count);
-    for (int q = 0; q < count; ++q) {
-        const double nd = host_jobs[q].a.ND;
-        step_bytes[0][q] = 7.0 * b * nd;
-        step_bytes[1][q] = 8.0 * b * nd;
-    }
     const size_t x_bytes = align256(sizeof(int4) * xt.size());
     const size_t tab_bytes = x_bytes + sizeof(int2) * st.size();
     if (allocate(tab_bytes, xt.size() + st.size(), kPriaSums)) return kErrorDevice;
@@ -760,9 +892,10 @@ int WidePria<Real>::setup(const std::vector<PriaJob<Real>>& host_jobs, PriaJob<R
     REFORM_HIP_CHECK(hipMemcpy(x_tab, xt.data(), sizeof(int4) * xt.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(s_tab, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice));
-    const size_t lds = pria_x_lds_bytes<Real>(max_T);
+    // the tile of the widest precision an instance can reach: FP64
+    lds = sizeof(double) * (size_t(max_T) * (TW + 1) + TW);
     if (lds > 65536)
-        REFORM_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(wpria_x<Real>),
+        REFORM_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(wpria_x),
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              int(lds)));
     return 0;
@@ -770,79 +903,80 @@ int WidePria<Real>::setup(const std::vector<PriaJob<Real>>& host_jobs, PriaJob<R
 
 // One step of every instance the mode selects: the x kernel, then the y/z
 // kernel, each bracketed by timing marks.
-template <class Real>
-int WidePria<Real>::step(hipStream_t stream, int mode, int j) {
-    const size_t lds = pria_x_lds_bytes<Real>(max_T);
+int WidePria::step(hipStream_t stream, int mode, int j) {
     if (mark(stream, 0, true)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_x<Real>, dim3(x_blocks), dim3(kWideBlock), lds, stream, jobs, x_tab,
-                       round_action, mode, j);
+    hipLaunchKernelGGL(wpria_x, dim3(x_blocks), dim3(kWideBlock), lds, stream, jobs32, jobs64,
+                       prec, x_tab, round_action, mode, j);
     if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_yz<Real>, dim3(s_blocks), dim3(1024), 0, stream, jobs, s_tab,
-                       round_action, mode, j);
+    hipLaunchKernelGGL(wpria_yz, dim3(s_blocks), dim3(1024), 0, stream, jobs32, jobs64, prec,
+                       s_tab, round_action, mode, j);
     if (mark(stream, 1, false)) return kErrorDevice;
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-template <class Real>
-int WidePria<Real>::start(hipStream_t stream, bool resume) {
+int WidePria::start(hipStream_t stream) {
     const dim3 blk(kWideBlock);
-    hipLaunchKernelGGL(wide_init<PriaJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
-                       count, round_action, last_action, steps, resume ? 1 : 0);
-    if (!resume) {
-        hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending,
-                           3);
-        if (step(stream, 0, 0)) return kErrorDevice;
-        hipLaunchKernelGGL(wpria_sums<Real>, dim3(s_blocks + x_blocks), blk, 0, stream, jobs, s_tab,
-                           s_blocks, x_tab, round_action, 0, partial);
-        hipLaunchKernelGGL(wpria_logic<Real>, dim3(count), blk, 0, stream, jobs, first, s_blocks,
-                           partial, round_action, last_action, steps, 0);
-        hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending,
-                           2);
-    }
-    hipLaunchKernelGGL(wide_count<PriaJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    hipLaunchKernelGGL(wide_init<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
+                       count, round_action, last_action, steps, prec, has32 ? 0 : 1);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
+                       pending, 3);
+    if (step(stream, 0, 0)) return kErrorDevice;
+    hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
+                       prec, s_tab, s_blocks, x_tab, round_action, 0, partial);
+    hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
+                       first, s_blocks, partial, round_action, last_action, steps, 0);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
+                       pending, 2);
+    hipLaunchKernelGGL(wide_count<PriaJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
+                       active);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-template <class Real>
-int WidePria<Real>::enqueue_round(hipStream_t stream) {
+int WidePria::enqueue_round(hipStream_t stream) {
     const dim3 blk(kWideBlock);
-    hipLaunchKernelGGL(wide_pre<PriaJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs, count,
-                       round_action, pending);
-    hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 0);
+    hipLaunchKernelGGL(wide_pre<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
+                       count, round_action, pending);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
+                       pending, 0);
     for (int j = 0; j < op_steps; ++j)
         if (step(stream, 0, j)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_sums<Real>, dim3(s_blocks + x_blocks), blk, 0, stream, jobs, s_tab,
-                       s_blocks, x_tab, round_action, 0, partial);
-    hipLaunchKernelGGL(wpria_logic<Real>, dim3(count), blk, 0, stream, jobs, first, s_blocks,
-                       partial, round_action, last_action, steps, 0);
+    hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
+                       prec, s_tab, s_blocks, x_tab, round_action, 0, partial);
+    hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
+                       first, s_blocks, partial, round_action, last_action, steps, 0);
+    if (has32) {
+        hipLaunchKernelGGL(wpria_convert, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec,
+                           s_tab);
+        hipLaunchKernelGGL(wide_promote<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream,
+                           jobs64, count, prec);
+    }
     if (step(stream, 1, 0)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 1);
-    hipLaunchKernelGGL(wpria_sums<Real>, dim3(s_blocks + x_blocks), blk, 0, stream, jobs, s_tab,
-                       s_blocks, x_tab, round_action, 1, partial);
-    hipLaunchKernelGGL(wpria_logic<Real>, dim3(count), blk, 0, stream, jobs, first, s_blocks,
-                       partial, round_action, last_action, steps, 1);
-    hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 2);
-    hipLaunchKernelGGL(wide_count<PriaJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
+                       pending, 1);
+    hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
+                       prec, s_tab, s_blocks, x_tab, round_action, 1, partial);
+    hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
+                       first, s_blocks, partial, round_action, last_action, steps, 1);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
+                       pending, 2);
+    hipLaunchKernelGGL(wide_count<PriaJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
+                       active);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-template <class Real>
-int WidePria<Real>::expire(hipStream_t stream) {
+int WidePria::expire(hipStream_t stream) {
     const dim3 blk(kWideBlock);
-    hipLaunchKernelGGL(wide_expire<PriaJob<Real>>, dim3(grid_for(count)), blk, 0, stream, jobs,
-                       count, last_action);
-    hipLaunchKernelGGL(wpria_copy<Real>, dim3(s_blocks), blk, 0, stream, jobs, s_tab, pending, 2);
-    hipLaunchKernelGGL(wide_count<PriaJob<Real>>, dim3(1), blk, 0, stream, jobs, count, active);
+    hipLaunchKernelGGL(wide_expire<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream,
+                       jobs64, count, last_action);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
+                       pending, 2);
+    hipLaunchKernelGGL(wide_count<PriaJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
+                       active);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
-
-template struct WidePraq<float>;
-template struct WidePraq<double>;
-template struct WidePria<float>;
-template struct WidePria<double>;
 
 }  // namespace reform

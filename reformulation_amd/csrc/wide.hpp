@@ -1,5 +1,5 @@
 // The batched device tier: many instances of one family advanced together by
-// wide kernels.
+// wide kernels, each instance in its own precision.
 //
 // Every launch covers the work of every instance: a block table maps each
 // block to (instance, local work unit) -- PRAQ: 256 node lanes or 256 edges;
@@ -7,7 +7,9 @@
 // its instance's scalar state to decide what to do.  Instances therefore sit
 // at different points of the Halpern state machine in the same launch: some
 // take reflected steps, some vanilla warm-phase steps, some a restart step,
-// finished and stalled ones skip.  One round is
+// finished ones skip.  Each block also reads its instance's precision and
+// runs the FP32 or the FP64 body on that instance's FP32 or FP64 image.  One
+// round is
 //
 //   pre           per instance: the round's action, pending anchor copies
 //   anchor copy   x0 <- x, y0 <- y where pending
@@ -15,6 +17,8 @@
 //   check sums    per block partial rows of the check sums
 //   logic         one workgroup per instance: fixed-order reduction of its
 //                 rows, then the scalar logic of halpern.hpp on one thread
+//   handover      FP32 instances whose logic asked for FP64 (gap below the
+//                 switch gap, or stalled): state converted, precision flipped
 //   restart       the step kernels again, for instances restarting (one
 //                 uncounted vanilla step), anchor copy, norm sums, logic
 //   finalize      (sx, sy) -> (x, y) where the answer is the step point;
@@ -22,9 +26,11 @@
 //
 // and the rounds are captured once into a hipGraph and replayed until no
 // instance is active.  A round launches the largest check interval's worth
-// of step kernels; an instance with a shorter interval skips the rest.  Per instance the arithmetic is that of the device
-// tier (same device functions, same per-instance summation order of the
-// check sums), so an instance's result does not depend on its batch.
+// of step kernels; an instance with a shorter interval skips the rest.  Per
+// instance the arithmetic is that of the device tier (same device functions,
+// summation order fixed by the instance's own blocks), so an instance's
+// result does not depend on its batch, and its precision switch happens at
+// the same check as in the plain solver.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,9 +44,13 @@
 namespace reform {
 
 // Round actions beyond halpern.hpp's: the uncounted first vanilla step of a
-// cold start.
+// cold start, and nothing.
 constexpr int kActInit = 5;
 constexpr int kActSkip = -1;
+
+// x tile width of the batched PRIA kernels, both precisions (one tile table
+// serves an instance before and after its switch).
+constexpr int kWideTW = 32;
 
 struct WideStats {
     int rounds = 0;
@@ -56,12 +66,14 @@ struct WideControl {
     int* round_action = nullptr;   // [count] the action this round takes
     int* last_action = nullptr;    // [count] the last action executed
     int* pending = nullptr;        // [count] anchor copy due this round
-    int* active = nullptr;         // [1] instances neither finished nor stalled
+    int* prec = nullptr;           // [count] 0: FP32 iterates, 1: FP64
+    int* active = nullptr;         // [1] instances not finished
     int* host_active = nullptr;    // [2] pinned
+    int64_t* steps = nullptr;      // [2 count] steps taken in FP32, then in FP64
     hipEvent_t polled[2] = {nullptr, nullptr};
-    int64_t* steps = nullptr;      // [count] steps each instance took in this engine
     void* arena = nullptr;
     size_t bytes = 0;
+    bool has32 = false;            // FP32 images exist (an FP32 batch)
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
 
@@ -78,9 +90,20 @@ struct WideControl {
     int mark_set = 0;
     double kernel_ms[kKinds] = {0.0, 0.0};
     int64_t launches[kKinds] = {0, 0};
-    std::vector<double> step_bytes[kKinds];  // algorithmic bytes of one step, per instance
+    // algorithmic bytes of one step per instance: [kind][precision][instance]
+    std::vector<double> step_bytes[kKinds][2];
 
     virtual ~WideControl();
+    // the cold start and the first count of active instances
+    virtual int start(hipStream_t stream) = 0;
+    // one round's launches (captured into the graph)
+    virtual int enqueue_round(hipStream_t stream) = 0;
+    // out of time: every active instance ends at its current point
+    virtual int expire(hipStream_t stream) = 0;
+    // start, then rounds until no instance is active
+    int solve(hipStream_t stream, double time_limit, WideStats* stats);
+    // arena: the caller's tables (table_bytes) followed by the control arrays
+    int allocate(size_t table_bytes, size_t partial_rows, int partial_width);
     // bracket one step launch of `kind` (no-op unless timing)
     int mark(hipStream_t stream, int kind, bool begin);
     // add the event pairs of a completed set to kernel_ms / launches
@@ -90,17 +113,8 @@ struct WideControl {
     void clear_timing();
     // algorithmic bytes moved by each step kernel over the last solve
     int algorithmic_bytes(double (&out)[kKinds]) const;
-    // the cold start (or the continuation of handed-over states) and the
-    // first count of active instances
-    virtual int start(hipStream_t stream, bool resume) = 0;
-    // one round's launches (captured into the graph)
-    virtual int enqueue_round(hipStream_t stream) = 0;
-    // out of time: every active instance ends at its current point
-    virtual int expire(hipStream_t stream) = 0;
-    // start, then rounds until no instance is active
-    int solve(hipStream_t stream, bool resume, double time_limit, WideStats* stats);
-    // arena: the caller's tables (table_bytes) followed by the control arrays
-    int allocate(size_t table_bytes, size_t partial_rows, int partial_width);
+    // per instance: 1 where its answer is in the FP64 image (after a solve)
+    int precisions(std::vector<int>& out) const;
 };
 
 // One engine's solve advanced in steps, so that one host thread can keep
@@ -117,39 +131,43 @@ struct WideRun {
     std::chrono::steady_clock::time_point t0;
     WideStats stats;
 
-    int begin(WideControl* engine, hipStream_t s, bool resume, double limit);
+    int begin(WideControl* engine, hipStream_t s, double limit);
     // returns < 0 on error, else whether anything changed
     int advance();
 };
+
 // PRAQ: first = (node block range, edge block range) per instance; partial
 // rows (kPraqSums each) per node block, then per edge block.
-template <class Real>
 struct WidePraq : WideControl {
-    PraqJob<Real>* jobs = nullptr;      // device, launch order
-    int2* node_tab = nullptr;           // (instance, local block) per node block
+    const PraqJob<float>* jobs32 = nullptr;   // device, launch order (null: FP64 batch)
+    const PraqJob<double>* jobs64 = nullptr;
+    int2* node_tab = nullptr;                 // (instance, local block) per node block
     int2* edge_tab = nullptr;
     int node_blocks = 0, edge_blocks = 0;
 
-    int setup(const std::vector<PraqJob<Real>>& host_jobs, PraqJob<Real>* dev_jobs);
+    int setup(const std::vector<PraqJob<double>>& host64, const PraqJob<float>* dev32,
+              const PraqJob<double>* dev64);
     int step(hipStream_t stream, int mode, int j);
-    int start(hipStream_t stream, bool resume) override;
+    int start(hipStream_t stream) override;
     int enqueue_round(hipStream_t stream) override;
     int expire(hipStream_t stream) override;
 };
 
 // PRIA: first = (slice range, column-tile range) per instance; partial rows
 // (kPriaSums wide) per slice, then one per column tile in (item, tile) order.
-template <class Real>
 struct WidePria : WideControl {
     int max_T = 2;
-    PriaJob<Real>* jobs = nullptr;
+    const PriaJob<float>* jobs32 = nullptr;
+    const PriaJob<double>* jobs64 = nullptr;
     int4* x_tab = nullptr;              // (instance, item, t0, column-tile row) per x tile
     int2* s_tab = nullptr;              // (instance, slice) per slice
     int x_blocks = 0, s_blocks = 0;
+    size_t lds = 0;                     // dynamic LDS of the x kernel
 
-    int setup(const std::vector<PriaJob<Real>>& host_jobs, PriaJob<Real>* dev_jobs);
+    int setup(const std::vector<PriaJob<double>>& host64, const PriaJob<float>* dev32,
+              const PriaJob<double>* dev64);
     int step(hipStream_t stream, int mode, int j);
-    int start(hipStream_t stream, bool resume) override;
+    int start(hipStream_t stream) override;
     int enqueue_round(hipStream_t stream) override;
     int expire(hipStream_t stream) override;
 };
