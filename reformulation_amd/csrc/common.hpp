@@ -42,6 +42,29 @@ __device__ __forceinline__ T wave_min(T v) {
     return v;
 }
 
+// Cross-lane move by DPP within rows of 16 lanes (gfx9 controls), 64-bit.
+template <int Ctrl>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, Ctrl, 0xf, 0xf, false),
+                            __builtin_amdgcn_update_dpp(lo, lo, Ctrl, 0xf, 0xf, false));
+}
+
+// Sum over groups of `width` lanes (16, 32 or 64; uniform over the wave):
+// xor 1 and xor 2 by quad permutes, then the other quad of the half-row and
+// the other half of the row by mirrors -- each a pairwise exchange, so both
+// partners add the same two numbers and every lane of a group ends with the
+// same bits -- then xor 16 and xor 32 across rows by shuffles.
+__device__ __forceinline__ double group_sum_dpp(double v, int width) {
+    v += dpp_move<0xb1>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4e>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_move<0x141>(v);   // row_half_mirror: quad q <-> q ^ 1
+    v += dpp_move<0x140>(v);   // row_mirror: half h <-> h ^ 1
+    if (width > 16) v += __shfl_xor(v, 16, kWave);
+    if (width > 32) v += __shfl_xor(v, 32, kWave);
+    return v;
+}
+
 // Sum of `width` consecutive lanes (a power of two <= 64), xor butterfly.
 template <class T>
 __device__ __forceinline__ T group_sum(T v, int width) {

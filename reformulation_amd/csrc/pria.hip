@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(1024) pria_wg_solve(const PriaJob<Real>* jobs)
 // ---------------------------------------------------------------------------
 
 constexpr int kPriaXBlock = 256;
-constexpr int kPriaYZBlock = 1024;
+constexpr int kPriaYZBlock = 512;
 constexpr int kPriaCheckBlock = 256;
 
 template <class Real>
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(256) pria_x_kernel(PriaArrays<Real> a, PriaSte
 }
 
 template <class Real>
-__global__ void __launch_bounds__(1024) pria_yz_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
+__global__ void __launch_bounds__(512) pria_yz_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
     __shared__ double red[4 * 16];
     pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, blockIdx.x, red);
 }

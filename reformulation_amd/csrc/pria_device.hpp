@@ -114,15 +114,28 @@ __device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st,
     const Real* srcy = st.reflect ? a.yc : a.sy;
     const Real* srcz = st.reflect ? a.zc : a.sz;
 
-    // load: lanes over columns, G rows of s at a time (coalesced per s)
-    {
-        const int c = tid % TW, row = tid / TW, t = t0 + c;
-        for (int s = row; s <= smax; s += G) {
-            if (c < tcount && t >= s) {
-                int g = pria_off(N, T, s) + i * (T - s) + (t - s);
-                Real x1 = srcx[g];
-                lds[s * stride + c] = x1 - st.tau * ((a.h[g] + srcz[g]) + srcy[g]);
+    // load: lanes over columns, G rows of s at a time (coalesced per s); U
+    // rows' loads are issued before any is used, so their latencies overlap
+    constexpr int U = 8;
+    const int c = tid % TW, row = tid / TW, t = t0 + c;
+    const bool colok = c < tcount;
+    for (int s0 = row; s0 <= smax; s0 += G * U) {
+        Real vx[U], vh[U], vy[U], vz[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int s = s0 + u * G;
+            if (colok && s <= t) {
+                const int g = pria_off(N, T, s) + i * (T - s) + (t - s);
+                vx[u] = srcx[g];
+                vh[u] = a.h[g];
+                vz[u] = srcz[g];
+                vy[u] = srcy[g];
             }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int s = s0 + u * G;
+            if (colok && s <= t) lds[s * stride + c] = vx[u] - st.tau * ((vh[u] + vz[u]) + vy[u]);
         }
     }
     __syncthreads();
@@ -187,23 +200,35 @@ __device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st,
         if (q == 0 && len > 0) a.thx[i * T + t] = th;
     }
     __syncthreads();
-    // write: x2, r, the next xc (and x, sx when storing)
+    // write: x2, r, the next xc (and x, sx when storing); loads batched as above
     {
-        const int c = tid % TW, row = tid / TW, t = t0 + c;
         const Real th = theta[c < TW ? c : 0];
-        for (int s = row; s <= smax; s += G) {
-            if (c < tcount && t >= s) {
-                int g = pria_off(N, T, s) + i * (T - s) + (t - s);
-                Real x1 = srcx[g];
-                Real d = lds[s * stride + c] - th;
-                Real x2 = d > Real(0) ? d : Real(0);
-                Real anchor = st.reflect ? a.x0[g] : x1;
-                Real refl = Real(2) * x2 - x1;
-                a.r[g] = st.sigma * refl;
-                a.xc[g] = st.lwn * refl + st.rwn * anchor;
-                if (st.store) {
-                    a.x[g] = x1;
-                    a.sx[g] = x2;
+        for (int s0 = row; s0 <= smax; s0 += G * U) {
+            Real vx[U], v0[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int s = s0 + u * G;
+                if (colok && s <= t) {
+                    const int g = pria_off(N, T, s) + i * (T - s) + (t - s);
+                    vx[u] = srcx[g];
+                    v0[u] = st.reflect ? a.x0[g] : vx[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int s = s0 + u * G;
+                if (colok && s <= t) {
+                    const int g = pria_off(N, T, s) + i * (T - s) + (t - s);
+                    const Real x1 = vx[u];
+                    const Real d = lds[s * stride + c] - th;
+                    const Real x2 = d > Real(0) ? d : Real(0);
+                    const Real refl = Real(2) * x2 - x1;
+                    a.r[g] = st.sigma * refl;
+                    a.xc[g] = st.lwn * refl + st.rwn * v0[u];
+                    if (st.store) {
+                        a.x[g] = x1;
+                        a.sx[g] = x2;
+                    }
                 }
             }
         }
@@ -211,59 +236,70 @@ __device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st,
     __syncthreads();
 }
 
-// Michelot threshold over one wavefront's register-held values (VR per lane),
-// target `target`; len entries are valid (lane-major: e = lane + 64 k); the
-// first active set is {u > start} when that is not empty, else all.
-template <int VR>
-__device__ __forceinline__ double wave_michelot(const double (&u)[VR], int len, double target,
-                                                double start) {
-    const int lane = threadIdx.x & (kWave - 1);
+// Michelot thresholds of the row segments a wavefront holds, one row per
+// group of `width` lanes (16, 32 or 64), VR values per lane (e = lr + width k
+// for the group lane lr), len valid per row; rows whose `live` is false only
+// join the reductions.  The first active set is {u > start} when that is not
+// empty, else all; every group stops when its set stops shrinking.
+template <class Real, int VR>
+__device__ __forceinline__ double group_michelot(const Real (&u)[VR], int len, int width, int lr,
+                                                 bool live, double target, double start) {
     double S = 0.0, C = 0.0, Sw = 0.0, Cw = 0.0;
 #pragma unroll
     for (int k = 0; k < VR; ++k) {
-        int e = lane + kWave * k;
-        if (e < len) {
-            S += u[k];
+        const int e = lr + width * k;
+        if (live && e < len) {
+            const double v = double(u[k]);
+            S += v;
             C += 1.0;
-            if (u[k] > start) {
-                Sw += u[k];
+            if (v > start) {
+                Sw += v;
                 Cw += 1.0;
             }
         }
     }
-    S = wave_sum(S);
-    C = wave_sum(C);
-    Sw = wave_sum(Sw);
-    Cw = wave_sum(Cw);
+    S = group_sum_dpp(S, width);
+    C = group_sum_dpp(C, width);
+    Sw = group_sum_dpp(Sw, width);
+    Cw = group_sum_dpp(Cw, width);
     bool first = Cw > 0.0;
     if (first) {
         S = Sw;
         C = Cw;
     }
-    double th = (S - target) / C;
-    for (int round = 0; round <= len; ++round) {
+    double th = live ? (S - target) / C : 0.0;
+    bool done = !live;
+    for (int round = 0; round <= len && __any(!done); ++round) {
         double S2 = 0.0, C2 = 0.0;
 #pragma unroll
         for (int k = 0; k < VR; ++k) {
-            int e = lane + kWave * k;
-            if (e < len && u[k] > th) {
-                S2 += u[k];
+            const int e = lr + width * k;
+            const double v = double(u[k]);
+            if (!done && e < len && v > th) {
+                S2 += v;
                 C2 += 1.0;
             }
         }
-        S2 = wave_sum(S2);
-        C2 = wave_sum(C2);
-        if (C2 == C || C2 == 0.0 || (C2 > C && !first)) break;
-        C = C2;
-        th = (S2 - target) / C2;
-        first = false;
+        S2 = group_sum_dpp(S2, width);
+        C2 = group_sum_dpp(C2, width);
+        if (!done) {
+            if (C2 == C || C2 == 0.0 || (C2 > C && !first)) {
+                done = true;
+            } else {
+                C = C2;
+                th = (S2 - target) / C2;
+            }
+            first = false;
+        }
     }
     return th;
 }
 
 // --- y/z pass: one slice s -------------------------------------------------
 //
-// y rows: one wavefront per row (i, s), values in registers (T' <= 64 VR).
+// y rows: all N rows of the slice have len = T' - s entries; each row goes
+// to a group of 16, 32 or 64 lanes (the smallest holding len in VR values
+// per lane), so a wavefront projects 4, 2 or 1 rows at once (T' <= 64 VR).
 // z slice: the whole workgroup, values in registers when the slice fits
 // (N (T' - s) <= blockDim * VZ), else re-read from memory each round.
 template <class Real, int VR, int VZ>
@@ -277,37 +313,40 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
     const Real* srcy = st.reflect ? a.yc : a.sy;
     const Real* srcz = st.reflect ? a.zc : a.sz;
 
-    // ---- y: rows of the slice ----
-    for (int i = wave; i < N; i += waves) {
-        const int rb = base + i * len;
-        double y1[VR], u[VR];
+    // ---- y: rows of the slice, 64 / width rows per wavefront at a time ----
+    const int width = len <= 16 * VR ? 16 : (len <= 32 * VR ? 32 : 64);
+    const int rpw = kWave / width, lr = lane % width, sub = lane / width;
+    for (int i0 = wave * rpw; i0 < N; i0 += waves * rpw) {
+        const int i = i0 + sub;
+        const bool row = i < N;
+        const int rb = base + (row ? i : 0) * len;
+        Real y1[VR], u[VR], an[VR];
 #pragma unroll
         for (int k = 0; k < VR; ++k) {
-            int e = lane + kWave * k;
-            if (e < len) {
-                y1[k] = double(srcy[rb + e]);
-                u[k] = double(Real(Real(y1[k]) + a.r[rb + e]));
-            } else {
-                y1[k] = 0.0;
-                u[k] = 0.0;
+            const int e = lr + width * k;
+            y1[k] = Real(0);
+            u[k] = Real(0);
+            an[k] = Real(0);
+            if (row && e < len) {
+                y1[k] = srcy[rb + e];
+                u[k] = y1[k] + a.r[rb + e];
+                an[k] = st.reflect ? a.y0[rb + e] : y1[k];
             }
         }
-        const double target = double(a.k[i]);
-        double th = 0.0;
-        if (target > 0.0) {
-            th = wave_michelot<VR>(u, len, target, a.thy[i * T + s]);
-            if (lane == 0) a.thy[i * T + s] = th;
-        }
+        const double target = row ? double(a.k[i]) : 0.0;
+        const bool live = row && target > 0.0;
+        const double th = group_michelot<Real, VR>(u, len, width, lr, live, target,
+                                                   live ? a.thy[i * T + s] : 0.0);
+        if (live && lr == 0) a.thy[i * T + s] = th;
 #pragma unroll
         for (int k = 0; k < VR; ++k) {
-            int e = lane + kWave * k;
-            if (e < len) {
-                int g = rb + e;
-                Real v1 = Real(y1[k]);
-                Real d = Real(u[k]) - Real(th);
-                Real y2 = target > 0.0 ? (d > Real(0) ? d : Real(0)) : Real(0);
-                Real anchor = st.reflect ? a.y0[g] : v1;
-                a.yc[g] = st.lwn * (Real(2) * y2 - v1) + st.rwn * anchor;
+            const int e = lr + width * k;
+            if (row && e < len) {
+                const int g = rb + e;
+                const Real v1 = y1[k];
+                const Real d = u[k] - Real(th);
+                const Real y2 = live ? (d > Real(0) ? d : Real(0)) : Real(0);
+                a.yc[g] = st.lwn * (Real(2) * y2 - v1) + st.rwn * an[k];
                 if (st.store) {
                     a.y[g] = v1;
                     a.sy[g] = y2;
@@ -324,37 +363,38 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
         const double start = a.thz[s];
         bool first;
         if (L <= BS * VZ) {
-            double u[VZ];
+            Real u[VZ];
             double v4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < VZ; ++k) {
-                int e = tid + BS * k;
-                u[k] = 0.0;
+                const int e = tid + BS * k;
+                u[k] = Real(0);
                 if (e < L) {
-                    u[k] = double(Real(srcz[base + e] + a.r[base + e]));
-                    v4[0] += u[k];
+                    u[k] = srcz[base + e] + a.r[base + e];
+                    const double v = double(u[k]);
+                    v4[0] += v;
                     v4[1] += 1.0;
-                    if (u[k] > start) {
-                        v4[2] += u[k];
+                    if (v > start) {
+                        v4[2] += v;
                         v4[3] += 1.0;
                     }
                 }
             }
             block_sums<4>(v4, red);
             first = v4[3] > 0.0;
-            double S = first ? v4[2] : v4[0], C = first ? v4[3] : v4[1];
-            th = (S - target) / C;
+            double C = first ? v4[3] : v4[1];
+            th = ((first ? v4[2] : v4[0]) - target) / C;
             for (int round = 0; round <= L; ++round) {
-                double S2 = 0.0, C2 = 0.0;
+                double w2[2] = {0.0, 0.0};
 #pragma unroll
                 for (int k = 0; k < VZ; ++k) {
-                    int e = tid + BS * k;
-                    if (e < L && u[k] > th) {
-                        S2 += u[k];
-                        C2 += 1.0;
+                    const int e = tid + BS * k;
+                    const double v = double(u[k]);
+                    if (e < L && v > th) {
+                        w2[0] += v;
+                        w2[1] += 1.0;
                     }
                 }
-                double w2[2] = {S2, C2};
                 block_sums<2>(w2, red);
                 if (w2[1] == C || w2[1] == 0.0 || (w2[1] > C && !first)) break;
                 C = w2[1];
@@ -374,18 +414,17 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
             }
             block_sums<4>(v4, red);
             first = v4[3] > 0.0;
-            double S = first ? v4[2] : v4[0], C = first ? v4[3] : v4[1];
-            th = (S - target) / C;
+            double C = first ? v4[3] : v4[1];
+            th = ((first ? v4[2] : v4[0]) - target) / C;
             for (int round = 0; round <= L; ++round) {
-                double S2 = 0.0, C2 = 0.0;
+                double w2[2] = {0.0, 0.0};
                 for (int e = tid; e < L; e += BS) {
-                    double v = double(Real(srcz[base + e] + a.r[base + e]));
+                    const double v = double(Real(srcz[base + e] + a.r[base + e]));
                     if (v > th) {
-                        S2 += v;
-                        C2 += 1.0;
+                        w2[0] += v;
+                        w2[1] += 1.0;
                     }
                 }
-                double w2[2] = {S2, C2};
                 block_sums<2>(w2, red);
                 if (w2[1] == C || w2[1] == 0.0 || (w2[1] > C && !first)) break;
                 C = w2[1];
@@ -395,18 +434,33 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
         }
         if (tid == 0) a.thz[s] = th;
     }
+    // write: loads of U entries issued before any is used
+    constexpr int U = 4;
     const Real thr = Real(th);
-    for (int e = tid; e < L; e += BS) {
-        int g = base + e;
-        Real v1 = srcz[g];
-        Real u = v1 + a.r[g];
-        Real d = u - thr;
-        Real z2 = target > 0.0 ? (d > Real(0) ? d : Real(0)) : Real(0);
-        Real anchor = st.reflect ? a.z0[g] : v1;
-        a.zc[g] = st.lwn * (Real(2) * z2 - v1) + st.rwn * anchor;
-        if (st.store) {
-            a.z[g] = v1;
-            a.sz[g] = z2;
+    for (int e0 = tid; e0 < L; e0 += BS * U) {
+        Real v1[U], rr[U], an[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int e = e0 + BS * q;
+            if (e < L) {
+                v1[q] = srcz[base + e];
+                rr[q] = a.r[base + e];
+                an[q] = st.reflect ? a.z0[base + e] : v1[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int e = e0 + BS * q;
+            if (e < L) {
+                const int g = base + e;
+                const Real d = (v1[q] + rr[q]) - thr;
+                const Real z2 = target > 0.0 ? (d > Real(0) ? d : Real(0)) : Real(0);
+                a.zc[g] = st.lwn * (Real(2) * z2 - v1[q]) + st.rwn * an[q];
+                if (st.store) {
+                    a.z[g] = v1[q];
+                    a.sz[g] = z2;
+                }
+            }
         }
     }
     __syncthreads();

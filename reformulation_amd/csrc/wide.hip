@@ -355,11 +355,18 @@ __global__ void __launch_bounds__(256) wpria_x(const PriaJob<float>* j32,
         using Real = RealOf<decltype(job)>;
         PriaStep<Real> sc;
         if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
-        pria_x_tile<Real, kWideTW>(job.a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
+        if (sizeof(Real) == 4) {
+            pria_x_tile<Real, kWideTW>(job.a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
+        } else {  // FP64: the tile as two halves, so its LDS stays that of FP32
+            constexpr int H = kWideTW / 2;
+            pria_x_tile<Real, H>(job.a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
+            if (t.z + H < job.a.T)
+                pria_x_tile<Real, H>(job.a, sc, t.y, t.z + H, reinterpret_cast<Real*>(dyn));
+        }
     });
 }
 
-__global__ void __launch_bounds__(1024) wpria_yz(const PriaJob<float>* j32,
+__global__ void __launch_bounds__(512) wpria_yz(const PriaJob<float>* j32,
                                                  const PriaJob<double>* j64, const int* prec,
                                                  const int2* tab, const int* round_action,
                                                  int mode, int j) {
@@ -892,8 +899,9 @@ int WidePria::setup(const std::vector<PriaJob<double>>& host64, const PriaJob<fl
     REFORM_HIP_CHECK(hipMemcpy(x_tab, xt.data(), sizeof(int4) * xt.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(s_tab, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice));
-    // the tile of the widest precision an instance can reach: FP64
-    lds = sizeof(double) * (size_t(max_T) * (TW + 1) + TW);
+    // an FP32 tile, or one half of an FP64 tile
+    lds = std::max(sizeof(float) * (size_t(max_T) * (TW + 1) + TW),
+                   sizeof(double) * (size_t(max_T) * (TW / 2 + 1) + TW / 2));
     if (lds > 65536)
         REFORM_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(wpria_x),
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -908,7 +916,7 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
     hipLaunchKernelGGL(wpria_x, dim3(x_blocks), dim3(kWideBlock), lds, stream, jobs32, jobs64,
                        prec, x_tab, round_action, mode, j);
     if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_yz, dim3(s_blocks), dim3(1024), 0, stream, jobs32, jobs64, prec,
+    hipLaunchKernelGGL(wpria_yz, dim3(s_blocks), dim3(512), 0, stream, jobs32, jobs64, prec,
                        s_tab, round_action, mode, j);
     if (mark(stream, 1, false)) return kErrorDevice;
     REFORM_HIP_CHECK(hipGetLastError());
