@@ -13,14 +13,6 @@ namespace reform {
 
 constexpr int kWave = 64;   // CDNA wavefront width
 
-// Sum over a wave64 by a fixed xor butterfly: every lane ends with the same
-// bits, and the order of additions depends only on the lane layout.
-template <class T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
-}
 
 template <class T>
 __device__ __forceinline__ T wave_max(T v) {
@@ -50,20 +42,24 @@ __device__ __forceinline__ double dpp_move(double v) {
                             __builtin_amdgcn_update_dpp(lo, lo, Ctrl, 0xf, 0xf, false));
 }
 
-// Sum over groups of `width` lanes (16, 32 or 64; uniform over the wave):
-// xor 1 and xor 2 by quad permutes, then the other quad of the half-row and
-// the other half of the row by mirrors -- each a pairwise exchange, so both
-// partners add the same two numbers and every lane of a group ends with the
-// same bits -- then xor 16 and xor 32 across rows by shuffles.
+// Sum over groups of `width` consecutive lanes (4, 8, 16, 32 or 64; uniform
+// over the wave): xor 1 and xor 2 by quad permutes, then the other quad of
+// the half-row and the other half of the row by mirrors -- each a pairwise
+// exchange, so both partners add the same two numbers and every lane of a
+// group ends with the same bits -- then xor 16 and xor 32 by shuffles.
 __device__ __forceinline__ double group_sum_dpp(double v, int width) {
-    v += dpp_move<0xb1>(v);    // quad_perm [1, 0, 3, 2]
-    v += dpp_move<0x4e>(v);    // quad_perm [2, 3, 0, 1]
-    v += dpp_move<0x141>(v);   // row_half_mirror: quad q <-> q ^ 1
-    v += dpp_move<0x140>(v);   // row_mirror: half h <-> h ^ 1
+    v += dpp_move<0xb1>(v);                      // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4e>(v);                      // quad_perm [2, 3, 0, 1]
+    if (width > 4) v += dpp_move<0x141>(v);      // row_half_mirror: quad q <-> q ^ 1
+    if (width > 8) v += dpp_move<0x140>(v);      // row_mirror: half h <-> h ^ 1
     if (width > 16) v += __shfl_xor(v, 16, kWave);
     if (width > 32) v += __shfl_xor(v, 32, kWave);
     return v;
 }
+
+// Sum over a wave64: every lane ends with the same bits, and the order of
+// additions depends only on the lane layout.
+__device__ __forceinline__ double wave_sum(double v) { return group_sum_dpp(v, kWave); }
 
 // Sum of `width` consecutive lanes (a power of two <= 64), xor butterfly.
 template <class T>

@@ -113,3 +113,33 @@ def test_batch_validates_on_the_host_before_any_device_work():
     with pytest.raises(RuntimeError, match=r"\(-2\)"):
         b.upload()
     b.close()
+
+
+def test_file_clis_reject_usage_and_bad_files_before_any_device_work(tmp_path):
+    import subprocess
+    from reformulation_amd.harness import wire
+    for tool in ("praq_solve_file", "pria_solve_file"):
+        exe = wire.BIN / tool
+        assert exe.exists(), f"{exe} not built"
+        assert subprocess.run([str(exe)], capture_output=True).returncode == 2
+        bad = tmp_path / f"{tool}.bin"
+        bad.write_bytes(b"\x00" * 40)
+        r = subprocess.run([str(exe), str(bad), str(tmp_path / "out.bin")], capture_output=True)
+        assert r.returncode == 2 and b"is not a" in r.stderr
+        assert not (tmp_path / "out.bin").exists()
+    r = subprocess.run([str(wire.BIN / "pria_solve_file"), "x", "y", "1e-6", "-1", "bogus"],
+                       capture_output=True)
+    assert r.returncode == 2 and b"unknown flag" in r.stderr
+
+
+def test_wire_format_round_trip(tmp_path):
+    import numpy as np
+    from reformulation_amd.harness import wire
+    p = tmp_path / "q.bin"
+    wire.write_praq_instance(p, 3, [1.0, -2.0, 0.5], [0, 1], [1, 2], [0.25, -1.0])
+    raw = np.fromfile(p, dtype="<i8", count=3)
+    assert raw.tolist() == [wire.PRAQ_MAGIC, 3, 2]
+    assert p.stat().st_size == 24 + 8 * 3 + 8 * 2 * 3
+    q = tmp_path / "j.bin"
+    wire.write_pria_instance(q, [2.0, 3.0], 5.0, np.zeros((2, 4, 4)))
+    assert q.stat().st_size == 24 + 8 + 16 + 8 * 32
