@@ -459,14 +459,16 @@ int BatchImpl::solve_wide() {
     REFORM_HIP_CHECK(hipStreamSynchronize(s1));
     praq_stats = runs[0].stats;
     pria_stats = runs[1].stats;
-    std::vector<int> prec;
+    // where each answer lives: the precision the instance finished in
     if (np) {
-        if (wpraq->precisions(prec)) return kErrorDevice;
-        for (int s = 0; s < np; ++s) praq_in64[praq_order[s]] = char(prec[s]);
+        REFORM_HIP_CHECK(hipMemcpy(praq_states.data(), praq_states_dev, sizeof(LogicState) * np,
+                                   hipMemcpyDeviceToHost));
+        for (int s = 0; s < np; ++s) praq_in64[praq_order[s]] = char(praq_states[s].in64 != 0);
     }
     if (nr) {
-        if (wpria->precisions(prec)) return kErrorDevice;
-        for (int s = 0; s < nr; ++s) pria_in64[pria_order[s]] = char(prec[s]);
+        REFORM_HIP_CHECK(hipMemcpy(pria_states.data(), pria_states_dev, sizeof(LogicState) * nr,
+                                   hipMemcpyDeviceToHost));
+        for (int s = 0; s < nr; ++s) pria_in64[pria_order[s]] = char(pria_states[s].in64 != 0);
     }
     float ms = 0.0f;
     if (np && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) praq_ms = ms;
@@ -794,7 +796,7 @@ extern "C" int reform_batch_kernel_stats(const reform_batch* batch, int family,
     static const char* kNames[2][2] = {{"wpraq_node", "wpraq_edge"}, {"wpria_x", "wpria_yz"}};
     WideControl* e = b.engine(family);
     double bytes[WideControl::kKinds] = {0.0, 0.0};
-    if (e && e->algorithmic_bytes(bytes)) return kErrorDevice;
+    if (e) e->algorithmic_bytes(family == 0 ? b.praq_states : b.pria_states, bytes);
     for (int k = 0; k < 2; ++k) {
         std::snprintf(out[k].name, sizeof(out[k].name), "%s", kNames[family][k]);
         out[k].launches = e ? e->launches[k] : 0;

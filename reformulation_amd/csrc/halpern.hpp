@@ -95,6 +95,10 @@ struct LogicState {
     double best_gap;             // smallest gap by halvings, for the stall test
     int64_t best_check;          // check count when best_gap was last halved
     int64_t switch_iteration;    // iteration at which FP32 handed over (-1: never)
+    // kept by the batched tier (no other tier reads them)
+    int32_t in64;                // iterates currently in the FP64 image
+    int32_t last_action;         // the last action executed
+    int64_t steps32, steps64;    // steps taken in each precision
 };
 
 // One line per check of the device tier's host loop, when verbose.

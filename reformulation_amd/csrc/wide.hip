@@ -24,9 +24,8 @@ using RealOf = typename std::decay<Job>::type::real_type;
 
 // Run `f` on the instance's job record in its current precision.
 template <class J32, class J64, class F>
-__device__ __forceinline__ void dispatch(const J32* j32, const J64* j64, const int* prec, int inst,
-                                         F&& f) {
-    if (prec[inst]) f(j64[inst]);
+__device__ __forceinline__ void dispatch(const J32* j32, const J64* j64, int inst, F&& f) {
+    if (j64[inst].state->in64) f(j64[inst]);
     else f(j32[inst]);
 }
 
@@ -48,7 +47,7 @@ __device__ __forceinline__ bool wide_step_for(const Job& job, const int* round_a
     const LogicState* st = job.state;
     if (mode == 0) {
         const int act = round_action[inst];
-        if (!wide_runs_steps(act) || j >= job.p.op_steps) return false;
+        if (!wide_runs_steps(act) || j >= (act == kActInit ? 1 : job.p.op_steps)) return false;
         const bool reflect = act == kActHalpernBlock;
         *out = make(st->restart_iter + j, st->omega, job.p.eta, reflect,
                     !reflect || j + 1 == job.p.op_steps);
@@ -72,15 +71,17 @@ __device__ __forceinline__ int wide_sums_point(const Job& job, const int* round_
     return wide_restarting(job.state) ? 2 : 0;
 }
 
-// Whether a copy launch moves this instance's data: mode 0 pending anchors,
-// 1 restart anchors, 2 finished step points, 3 the start.
-template <class Job>
-__device__ __forceinline__ bool wide_copies(const Job& job, const int* pending, int inst,
-                                            int mode) {
-    if (mode == 0) return pending[inst] != 0;
-    if (mode == 1) return wide_restarting(job.state);
-    if (mode == 2) return job.state->action == kActDoneSX;
-    return true;
+// The copy a copy launch makes for this instance: mode 0 (a round's start)
+// 3 = the start copies of an instance starting, 0 = anchors where pending;
+// mode 1: 1 = restart anchors; mode 2: 2 = finished step points; -1: none.
+__device__ __forceinline__ int wide_copy_kind(const LogicState* st, const int* round_action,
+                                              const int* pending, int inst, int mode) {
+    if (mode == 0) {
+        if (round_action[inst] == kActInit) return 3;
+        return pending[inst] ? 0 : -1;
+    }
+    if (mode == 1) return wide_restarting(st) ? 1 : -1;
+    return st->action == kActDoneSX ? 2 : -1;
 }
 
 // The action a logic launch applies to this instance, or -1.
@@ -98,10 +99,11 @@ __device__ __forceinline__ int wide_logic_action(const Job& job, const int* roun
 // device tier's host loop.  `steps` counts the steps this instance took in
 // its current precision.
 template <class Job>
-__device__ void wide_apply(const Job& job, int act, const CheckSums& cs, int* last_action,
-                           int64_t* steps, int inst) {
+__device__ void wide_apply(const Job& job, int act, const CheckSums& cs) {
     LogicState st = *job.state;
-    steps[inst] += act == kActWarmBlock || act == kActHalpernBlock ? job.p.op_steps : 1;
+    const int64_t taken = act == kActWarmBlock || act == kActHalpernBlock ? job.p.op_steps : 1;
+    if (st.in64) st.steps64 += taken;
+    else st.steps32 += taken;
     if (act == kActInit) {
         logic_after_warm_check(st, job.p, cs);
     } else if (act == kActWarmBlock) {
@@ -115,8 +117,8 @@ __device__ void wide_apply(const Job& job, int act, const CheckSums& cs, int* la
     } else {
         logic_after_restart(st, job.p, cs);
     }
+    st.last_action = act;
     *job.state = st;
-    last_action[inst] = act;
 }
 
 // One thread per instance: the round's action and pending anchor copies.
@@ -126,38 +128,42 @@ __global__ void wide_pre(const Job* jobs, int count, int* round_action, int* pen
     if (inst >= count) return;
     LogicState* st = jobs[inst].state;
     int act = st->action;
-    if (st->stalled || (act != kActWarmBlock && act != kActHalpernBlock)) act = kActSkip;
+    if (act == kActNew) {
+        st->action = kActWarmBlock;  // what logic_init leaves; the Init round's check decides
+        act = kActInit;
+    } else if (st->stalled || (act != kActWarmBlock && act != kActHalpernBlock)) {
+        act = kActSkip;
+    }
     round_action[inst] = act;
     const int pend = act == kActHalpernBlock && st->anchor_pending;
     pending[inst] = pend;
     if (pend) st->anchor_pending = 0;
 }
 
-// One thread per instance: the cold start's state and precision.
+// One thread per instance: a cold start at its next round, in the first
+// precision.
 template <class Job>
-__global__ void wide_init(const Job* jobs, int count, int* round_action, int* last_action,
-                          int64_t* steps, int* prec, int first_prec) {
+__global__ void wide_init(const Job* jobs, int count, int first_in64) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= count) return;
     LogicState s;
     logic_init(s, jobs[inst].p);
+    s.action = kActNew;
+    s.in64 = first_in64;
+    s.last_action = kActInit;
+    s.steps32 = s.steps64 = 0;
     *jobs[inst].state = s;
-    round_action[inst] = kActInit;
-    last_action[inst] = kActInit;
-    steps[inst] = 0;
-    steps[count + inst] = 0;
-    prec[inst] = first_prec;
 }
 
 // One thread per instance: FP32 instances whose logic asked for FP64 (their
 // state was converted by the launch before) continue in FP64 from here.
 template <class Job>
-__global__ void wide_promote(const Job* jobs, int count, int* prec) {
+__global__ void wide_promote(const Job* jobs, int count) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= count) return;
     LogicState* st = jobs[inst].state;
-    if (prec[inst] == 0 && st->stalled) {
-        prec[inst] = 1;
+    if (!st->in64 && st->stalled) {
+        st->in64 = 1;
         st->stalled = 0;
         st->switch_iteration = st->iteration;
     }
@@ -166,12 +172,17 @@ __global__ void wide_promote(const Job* jobs, int count, int* prec) {
 // One thread per instance: out of time.  The answer is the last step's input
 // after a warm block or a restart, else the step point; as the device tier.
 template <class Job>
-__global__ void wide_expire(const Job* jobs, int count, const int* last_action) {
+__global__ void wide_expire(const Job* jobs, int count) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= count) return;
     LogicState* st = jobs[inst].state;
     if (st->action == kActDoneX || st->action == kActDoneSX) return;
-    st->action = last_action[inst] == kActHalpernBlock ? kActDoneSX : kActDoneX;
+    if (st->action == kActNew) {  // never started: its start point is the answer
+        st->action = kActDoneX;
+        st->status = 2;
+        return;
+    }
+    st->action = st->last_action == kActHalpernBlock ? kActDoneSX : kActDoneX;
     st->status = 2;
 }
 
@@ -192,12 +203,14 @@ __global__ void wide_count(const Job* jobs, int count, int* active) {
 
 // ---------------------------------------------------------------- PRAQ ----
 
-__global__ void __launch_bounds__(256) wpraq_node(const PraqJob<float>* j32,
-                                                  const PraqJob<double>* j64, const int* prec,
+using PJ32 = PraqJob<float>;
+using PJ64 = PraqJob<double>;
+
+__global__ void __launch_bounds__(256) wpraq_node(const PJ32* j32, const PJ64* j64,
                                                   const int2* tab, const int* round_action,
                                                   int mode, int j) {
     const int2 t = tab[blockIdx.x];
-    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+    dispatch(j32, j64, t.x, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         PraqStep<Real> sc;
         if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
@@ -207,12 +220,11 @@ __global__ void __launch_bounds__(256) wpraq_node(const PraqJob<float>* j32,
     });
 }
 
-__global__ void __launch_bounds__(256) wpraq_edge(const PraqJob<float>* j32,
-                                                  const PraqJob<double>* j64, const int* prec,
+__global__ void __launch_bounds__(256) wpraq_edge(const PJ32* j32, const PJ64* j64,
                                                   const int2* tab, const int* round_action,
                                                   int mode, int j) {
     const int2 t = tab[blockIdx.x];
-    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+    dispatch(j32, j64, t.x, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         PraqStep<Real> sc;
         if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
@@ -222,15 +234,14 @@ __global__ void __launch_bounds__(256) wpraq_edge(const PraqJob<float>* j32,
 }
 
 // node blocks, then edge blocks; one partial row per block
-__global__ void __launch_bounds__(256) wpraq_sums(const PraqJob<float>* j32,
-                                                  const PraqJob<double>* j64, const int* prec,
+__global__ void __launch_bounds__(256) wpraq_sums(const PJ32* j32, const PJ64* j64,
                                                   const int2* node_tab, int node_blocks,
                                                   const int2* edge_tab, const int* round_action,
                                                   int mode, double* partial) {
     __shared__ double red[kPraqSums * 16];
     const bool node = int(blockIdx.x) < node_blocks;
     const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
-    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+    dispatch(j32, j64, t.x, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         const int where = wide_sums_point(job, round_action, t.x, mode);
         if (!where) return;
@@ -253,14 +264,13 @@ __global__ void __launch_bounds__(256) wpraq_sums(const PraqJob<float>* j32,
 }
 
 // one block per instance: its rows in a fixed order, then the logic
-__global__ void __launch_bounds__(256) wpraq_logic(const PraqJob<float>* j32,
-                                                   const PraqJob<double>* j64, const int* prec,
-                                                   int count, const int* first, int node_blocks,
+__global__ void __launch_bounds__(256) wpraq_logic(const PJ32* j32, const PJ64* j64,
+                                                   const int* first, int node_blocks,
                                                    const double* partial, const int* round_action,
-                                                   int* last_action, int64_t* steps, int mode) {
+                                                   int mode) {
     __shared__ double red[kPraqSums * 16];
     const int inst = blockIdx.x;
-    dispatch(j32, j64, prec, inst, [&](const auto& job) {
+    dispatch(j32, j64, inst, [&](const auto& job) {
         const int act = wide_logic_action(job, round_action, inst, mode);
         if (act < 0) return;
         const int nf = first[4 * inst], nc = first[4 * inst + 1];
@@ -274,37 +284,35 @@ __global__ void __launch_bounds__(256) wpraq_logic(const PraqJob<float>* j32,
             for (int k = 0; k < kPraqSums; ++k)
                 acc[k] += partial[size_t(node_blocks + ef + r) * kPraqSums + k];
         block_sums<kPraqSums>(acc, red);
-        if (threadIdx.x == 0)
-            wide_apply(job, act, praq_check_sums(acc, job.scale), last_action,
-                       steps + size_t(prec[inst]) * count, inst);
+        if (threadIdx.x == 0) wide_apply(job, act, praq_check_sums(acc, job.scale));
     });
 }
 
 // copies over node blocks then edge blocks, in the instance's precision
-__global__ void __launch_bounds__(256) wpraq_copy(const PraqJob<float>* j32,
-                                                  const PraqJob<double>* j64, const int* prec,
+__global__ void __launch_bounds__(256) wpraq_copy(const PJ32* j32, const PJ64* j64,
                                                   const int2* node_tab, int node_blocks,
-                                                  const int2* edge_tab, const int* pending,
-                                                  int mode) {
+                                                  const int2* edge_tab, const int* round_action,
+                                                  const int* pending, int mode) {
     const bool node = int(blockIdx.x) < node_blocks;
     const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
-    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
-        if (!wide_copies(job, pending, t.x, mode)) return;
+    dispatch(j32, j64, t.x, [&](const auto& job) {
+        const int kind = wide_copy_kind(job.state, round_action, pending, t.x, mode);
+        if (kind < 0) return;
         const auto& a = job.a;
         const int gid = t.y * kWideBlock + threadIdx.x;
         if (node) {
             if (gid % job.lanes) return;
             const int i = gid / job.lanes;
             if (i >= a.n) return;
-            if (mode <= 1) a.x0[i] = a.x[i];
-            else if (mode == 2) a.x[i] = a.sx[i];
+            if (kind <= 1) a.x0[i] = a.x[i];
+            else if (kind == 2) a.x[i] = a.sx[i];
             else a.x0[i] = a.x[i] = a.sx[i] = a.xs[i];
         } else {
             if (gid >= a.m) return;
             for (int slot = 0; slot < 2; ++slot) {
                 const int k = slot * a.m + gid;
-                if (mode <= 1) a.y0[k] = a.y[k];
-                else if (mode == 2) a.y[k] = a.sy[k];
+                if (kind <= 1) a.y0[k] = a.y[k];
+                else if (kind == 2) a.y[k] = a.sy[k];
                 else a.y0[k] = a.y[k] = a.sy[k] = a.ys[k];
             }
         }
@@ -313,13 +321,13 @@ __global__ void __launch_bounds__(256) wpraq_copy(const PraqJob<float>* j32,
 
 // FP32 instances whose logic asked for FP64: their Halpern state, block by
 // block, into the FP64 image (the precision flips in wide_promote)
-__global__ void __launch_bounds__(256) wpraq_convert(const PraqJob<float>* j32,
-                                                     const PraqJob<double>* j64, const int* prec,
+__global__ void __launch_bounds__(256) wpraq_convert(const PJ32* j32, const PJ64* j64,
                                                      const int2* node_tab, int node_blocks,
                                                      const int2* edge_tab) {
     const bool node = int(blockIdx.x) < node_blocks;
     const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
-    if (prec[t.x] != 0 || !j64[t.x].state->stalled) return;
+    const LogicState* st = j64[t.x].state;
+    if (st->in64 || !st->stalled) return;
     const PraqArrays<float>& f = j32[t.x].a;
     const PraqArrays<double>& d = j64[t.x].a;
     const int gid = t.y * kWideBlock + threadIdx.x;
@@ -345,13 +353,14 @@ __global__ void __launch_bounds__(256) wpraq_convert(const PraqJob<float>* j32,
 
 // ---------------------------------------------------------------- PRIA ----
 
-__global__ void __launch_bounds__(256) wpria_x(const PriaJob<float>* j32,
-                                               const PriaJob<double>* j64, const int* prec,
-                                               const int4* tab, const int* round_action,
-                                               int mode, int j) {
+using RJ32 = PriaJob<float>;
+using RJ64 = PriaJob<double>;
+
+__global__ void __launch_bounds__(256) wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab,
+                                               const int* round_action, int mode, int j) {
     extern __shared__ char dyn[];
     const int4 t = tab[blockIdx.x];
-    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+    dispatch(j32, j64, t.x, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         PriaStep<Real> sc;
         if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
@@ -366,13 +375,11 @@ __global__ void __launch_bounds__(256) wpria_x(const PriaJob<float>* j32,
     });
 }
 
-__global__ void __launch_bounds__(512) wpria_yz(const PriaJob<float>* j32,
-                                                 const PriaJob<double>* j64, const int* prec,
-                                                 const int2* tab, const int* round_action,
-                                                 int mode, int j) {
+__global__ void __launch_bounds__(512) wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab,
+                                                const int* round_action, int mode, int j) {
     __shared__ double red[4 * 16];
     const int2 t = tab[blockIdx.x];
-    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
+    dispatch(j32, j64, t.x, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         PriaStep<Real> sc;
         if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
@@ -382,15 +389,14 @@ __global__ void __launch_bounds__(512) wpria_yz(const PriaJob<float>* j32,
 
 // slice blocks (six sums each), then tile blocks (a column-minimum sum each,
 // stored at the tile's (item, tile) row)
-__global__ void __launch_bounds__(256) wpria_sums(const PriaJob<float>* j32,
-                                                  const PriaJob<double>* j64, const int* prec,
+__global__ void __launch_bounds__(256) wpria_sums(const RJ32* j32, const RJ64* j64,
                                                   const int2* s_tab, int s_blocks,
                                                   const int4* x_tab, const int* round_action,
                                                   int mode, double* partial) {
     __shared__ double red[kPriaSums * 16];
     const bool slice = int(blockIdx.x) < s_blocks;
     const int inst = slice ? s_tab[blockIdx.x].x : x_tab[blockIdx.x - s_blocks].x;
-    dispatch(j32, j64, prec, inst, [&](const auto& job) {
+    dispatch(j32, j64, inst, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         const int where = wide_sums_point(job, round_action, inst, mode);
         if (!where) return;
@@ -411,14 +417,13 @@ __global__ void __launch_bounds__(256) wpria_sums(const PriaJob<float>* j32,
     });
 }
 
-__global__ void __launch_bounds__(256) wpria_logic(const PriaJob<float>* j32,
-                                                   const PriaJob<double>* j64, const int* prec,
-                                                   int count, const int* first, int s_blocks,
+__global__ void __launch_bounds__(256) wpria_logic(const RJ32* j32, const RJ64* j64,
+                                                   const int* first, int s_blocks,
                                                    const double* partial, const int* round_action,
-                                                   int* last_action, int64_t* steps, int mode) {
+                                                   int mode) {
     __shared__ double red[kPriaSums * 16];
     const int inst = blockIdx.x;
-    dispatch(j32, j64, prec, inst, [&](const auto& job) {
+    dispatch(j32, j64, inst, [&](const auto& job) {
         const int act = wide_logic_action(job, round_action, inst, mode);
         if (act < 0) return;
         const int sf = first[4 * inst], sc = first[4 * inst + 1];
@@ -430,24 +435,22 @@ __global__ void __launch_bounds__(256) wpria_logic(const PriaJob<float>* j32,
         for (int r = threadIdx.x; r < tc; r += blockDim.x)
             acc[kPLower] += partial[size_t(s_blocks + tf + r) * kPriaSums];
         block_sums<kPriaSums>(acc, red);
-        if (threadIdx.x == 0)
-            wide_apply(job, act, pria_check_sums(acc, job.scale), last_action,
-                       steps + size_t(prec[inst]) * count, inst);
+        if (threadIdx.x == 0) wide_apply(job, act, pria_check_sums(acc, job.scale));
     });
 }
 
-// one block per slice, in the instance's precision; copy modes as
-// wide_copies, the start also clearing the slice's thresholds
-__global__ void __launch_bounds__(256) wpria_copy(const PriaJob<float>* j32,
-                                                  const PriaJob<double>* j64, const int* prec,
-                                                  const int2* s_tab, const int* pending,
-                                                  int mode) {
+// one block per slice, in the instance's precision; the start copies also
+// clear the slice's thresholds
+__global__ void __launch_bounds__(256) wpria_copy(const RJ32* j32, const RJ64* j64,
+                                                  const int2* s_tab, const int* round_action,
+                                                  const int* pending, int mode) {
     const int2 t = s_tab[blockIdx.x];
-    dispatch(j32, j64, prec, t.x, [&](const auto& job) {
-        if (!wide_copies(job, pending, t.x, mode)) return;
+    dispatch(j32, j64, t.x, [&](const auto& job) {
+        const int kind = wide_copy_kind(job.state, round_action, pending, t.x, mode);
+        if (kind < 0) return;
         const auto& a = job.a;
         const int s = t.y, base = pria_off(a.N, a.T, s), L = a.N * (a.T - s);
-        if (mode == 3) {  // no thresholds yet: column (i, s), row (i, s), slice s
+        if (kind == 3) {  // no thresholds yet: column (i, s), row (i, s), slice s
             const double none = __builtin_nan("");
             for (int i = threadIdx.x; i < a.N; i += blockDim.x) {
                 a.thx[i * a.T + s] = none;
@@ -457,11 +460,11 @@ __global__ void __launch_bounds__(256) wpria_copy(const PriaJob<float>* j32,
         }
         for (int e = threadIdx.x; e < L; e += blockDim.x) {
             const int g = base + e;
-            if (mode <= 1) {
+            if (kind <= 1) {
                 a.x0[g] = a.x[g];
                 a.y0[g] = a.y[g];
                 a.z0[g] = a.z[g];
-            } else if (mode == 2) {
+            } else if (kind == 2) {
                 a.x[g] = a.sx[g];
                 a.y[g] = a.sy[g];
                 a.z[g] = a.sz[g];
@@ -476,11 +479,11 @@ __global__ void __launch_bounds__(256) wpria_copy(const PriaJob<float>* j32,
 
 // FP32 instances whose logic asked for FP64: one slice's state (and the
 // thresholds of the slice's column, row and slice) into the FP64 image
-__global__ void __launch_bounds__(256) wpria_convert(const PriaJob<float>* j32,
-                                                     const PriaJob<double>* j64, const int* prec,
+__global__ void __launch_bounds__(256) wpria_convert(const RJ32* j32, const RJ64* j64,
                                                      const int2* s_tab) {
     const int2 t = s_tab[blockIdx.x];
-    if (prec[t.x] != 0 || !j64[t.x].state->stalled) return;
+    const LogicState* st = j64[t.x].state;
+    if (st->in64 || !st->stalled) return;
     const PriaArrays<float>& f = j32[t.x].a;
     const PriaArrays<double>& d = j64[t.x].a;
     const int s = t.y, base = pria_off(f.N, f.T, s), L = f.N * (f.T - s);
@@ -519,13 +522,10 @@ int WideRun::begin(WideControl* engine, hipStream_t s, double limit) {
     t0 = std::chrono::steady_clock::now();
     finished = true;
     if (!w || w->count == 0) return 0;
-    w->mark_set = 2;
-    w->marks_used[2] = 0;
-    w->mark_kinds[2].clear();
     if (w->start(stream)) return kErrorDevice;
     // the round graph (timed rounds are launched directly instead: events
     // recorded by graph nodes carry no timestamps for hipEventElapsedTime)
-    if (!w->timing && !w->exec) {
+    if (w->use_graph && !w->timing && !w->exec) {
         REFORM_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         const int st = w->enqueue_round(stream);
         REFORM_HIP_CHECK(hipStreamEndCapture(stream, &w->graph));
@@ -535,7 +535,6 @@ int WideRun::begin(WideControl* engine, hipStream_t s, double limit) {
     REFORM_HIP_CHECK(
         hipMemcpyAsync(w->host_active, w->active, sizeof(int), hipMemcpyDeviceToHost, stream));
     REFORM_HIP_CHECK(hipStreamSynchronize(stream));
-    if (w->timing && w->collect(2)) return kErrorDevice;
     finished = w->host_active[0] == 0;
     return 0;
 }
@@ -559,32 +558,36 @@ int WideRun::advance() {
     }
     const double elapsed =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (drained || (time_limit > 0.0 && elapsed > time_limit)) {
+    // out of time: only once a round has run, so every instance has started
+    if (drained || (time_limit > 0.0 && elapsed > time_limit && retired > 0)) {
         REFORM_HIP_CHECK(hipStreamSynchronize(stream));
         if (!drained) {
             if (w->expire(stream)) return kErrorDevice;
             REFORM_HIP_CHECK(hipStreamSynchronize(stream));
         }
         finished = true;
-        stats.rounds = queued * (w->timing ? 1 : kRoundsPerPoll);
         stats.seconds = elapsed;
         return 1;
     }
     while (queued - retired < 2) {
         const int slot = queued & 1;
-        if (w->timing) {  // one round per group, its marks in the group's event set
-            w->mark_set = slot;
-            w->marks_used[slot] = 0;
-            w->mark_kinds[slot].clear();
-            if (w->enqueue_round(stream)) return kErrorDevice;
-        } else {
-            for (int r = 0; r < kRoundsPerPoll; ++r)
+        const bool graph = w->use_graph && !w->timing;
+        const int rounds = graph ? kRoundsPerPoll : 1;
+        for (int r = 0; r < rounds; ++r) {
+            if (graph) {
                 REFORM_HIP_CHECK(hipGraphLaunch(w->exec, stream));
+            } else {  // the round's marks (if timing) in the group's event set
+                w->mark_set = slot;
+                w->marks_used[slot] = 0;
+                w->mark_kinds[slot].clear();
+                if (w->enqueue_round(stream)) return kErrorDevice;
+            }
         }
         REFORM_HIP_CHECK(hipMemcpyAsync(w->host_active + slot, w->active, sizeof(int),
                                         hipMemcpyDeviceToHost, stream));
         REFORM_HIP_CHECK(hipEventRecord(w->polled[slot], stream));
         ++queued;
+        stats.rounds += rounds;
         changed = 1;
     }
     return changed;
@@ -605,14 +608,22 @@ int WideControl::solve(hipStream_t stream, double time_limit, WideStats* stats) 
 // ---------------------------------------------------------------- control --
 
 WideControl::~WideControl() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    release();
     for (auto& set : marks)
         for (hipEvent_t e : set) (void)hipEventDestroy(e);
     for (auto& e : polled)
         if (e) (void)hipEventDestroy(e);
     if (host_active) (void)hipHostFree(host_active);
+}
+
+void WideControl::release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (arena) (void)hipFree(arena);
+    exec = nullptr;
+    graph = nullptr;
+    arena = nullptr;
+    bytes = 0;
 }
 
 int WideControl::mark(hipStream_t stream, int kind, bool begin) {
@@ -649,37 +660,25 @@ void WideControl::clear_timing() {
 
 void WideControl::set_timing(bool on) { timing = on; }
 
-int WideControl::algorithmic_bytes(double (&out)[kKinds]) const {
-    std::vector<int64_t> n(2 * size_t(count));
-    if (count)
-        REFORM_HIP_CHECK(
-            hipMemcpy(n.data(), steps, sizeof(int64_t) * n.size(), hipMemcpyDeviceToHost));
+void WideControl::algorithmic_bytes(const std::vector<LogicState>& states,
+                                    double (&out)[kKinds]) const {
     for (int k = 0; k < kKinds; ++k) {
         out[k] = 0.0;
-        for (int p = 0; p < 2; ++p)
-            for (int q = 0; q < count && q < int(step_bytes[k][p].size()); ++q)
-                out[k] += double(n[size_t(p) * count + q]) * step_bytes[k][p][q];
+        for (int q = 0; q < count && q < int(states.size()); ++q)
+            out[k] += double(states[q].steps32) * step_bytes[k][0][q] +
+                      double(states[q].steps64) * step_bytes[k][1][q];
     }
-    return 0;
-}
-
-int WideControl::precisions(std::vector<int>& out) const {
-    out.assign(count, 1);
-    if (count)
-        REFORM_HIP_CHECK(hipMemcpy(out.data(), prec, sizeof(int) * count, hipMemcpyDeviceToHost));
-    return 0;
 }
 
 int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_width) {
+    release();
     size_t at = align256(table_bytes);
     const size_t o_first = at;
     at = align256(at + sizeof(int) * 4 * size_t(count));
     const size_t o_partial = at;
     at = align256(at + sizeof(double) * partial_width * (partial_rows + 1));
-    const size_t o_steps = at;
-    at = align256(at + sizeof(int64_t) * 2 * size_t(count));
     const size_t o_ctrl = at;
-    at = align256(at + sizeof(int) * (4 * size_t(count) + 1));
+    at = align256(at + sizeof(int) * (2 * size_t(count) + 1));
     {
         hipError_t e = hipMalloc(&arena, at);
         if (e != hipSuccess) {
@@ -692,13 +691,10 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     char* b = static_cast<char*>(arena);
     first = reinterpret_cast<int*>(b + o_first);
     partial = reinterpret_cast<double*>(b + o_partial);
-    steps = reinterpret_cast<int64_t*>(b + o_steps);
     round_action = reinterpret_cast<int*>(b + o_ctrl);
-    last_action = round_action + count;
-    pending = last_action + count;
-    prec = pending + count;
-    active = prec + count;
-    {
+    pending = round_action + count;
+    active = pending + count;
+    if (!host_active) {
         hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host_active), 2 * sizeof(int),
                                      hipHostMallocDefault);
         if (e != hipSuccess) {
@@ -708,7 +704,7 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
         }
     }
     for (auto& e : polled)
-        REFORM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        if (!e) REFORM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return 0;
 }
 
@@ -720,7 +716,10 @@ int WidePraq::setup(const std::vector<PraqJob<double>>& host64, const PraqJob<fl
     jobs32 = dev32;
     jobs64 = dev64;
     has32 = dev32 != nullptr;
-    if (!count) return 0;
+    if (!count) {
+        release();
+        return 0;
+    }
     std::vector<int2> nt, et;
     std::vector<int> fi(4 * size_t(count));
     op_steps = 2;
@@ -729,7 +728,7 @@ int WidePraq::setup(const std::vector<PraqJob<double>>& host64, const PraqJob<fl
     // half-edge of the CSR scatter; edge kernel: ends 8, weight, the y pair,
     // the anchor pair and the written pair
     for (int k = 0; k < kKinds; ++k)
-        for (int p = 0; p < 2; ++p) step_bytes[k][p].resize(count);
+        for (int p = 0; p < 2; ++p) step_bytes[k][p].assign(count, 0.0);
     for (int q = 0; q < count; ++q) {
         const auto& a = host64[q].a;
         const int lanes = host64[q].lanes;
@@ -767,14 +766,14 @@ int WidePraq::step(hipStream_t stream, int mode, int j) {
     const dim3 blk(kWideBlock);
     if (node_blocks) {
         if (mark(stream, 0, true)) return kErrorDevice;
-        hipLaunchKernelGGL(wpraq_node, dim3(node_blocks), blk, 0, stream, jobs32, jobs64, prec,
-                           node_tab, round_action, mode, j);
+        hipLaunchKernelGGL(wpraq_node, dim3(node_blocks), blk, 0, stream, jobs32, jobs64, node_tab,
+                           round_action, mode, j);
         if (mark(stream, 0, false)) return kErrorDevice;
     }
     if (edge_blocks) {
         if (mark(stream, 1, true)) return kErrorDevice;
-        hipLaunchKernelGGL(wpraq_edge, dim3(edge_blocks), blk, 0, stream, jobs32, jobs64, prec,
-                           edge_tab, round_action, mode, j);
+        hipLaunchKernelGGL(wpraq_edge, dim3(edge_blocks), blk, 0, stream, jobs32, jobs64, edge_tab,
+                           round_action, mode, j);
         if (mark(stream, 1, false)) return kErrorDevice;
     }
     REFORM_HIP_CHECK(hipGetLastError());
@@ -782,19 +781,9 @@ int WidePraq::step(hipStream_t stream, int mode, int j) {
 }
 
 int WidePraq::start(hipStream_t stream) {
-    const int all = node_blocks + edge_blocks;
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_init<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
-                       count, round_action, last_action, steps, prec, has32 ? 0 : 1);
-    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
-                       node_blocks, edge_tab, pending, 3);
-    if (step(stream, 0, 0)) return kErrorDevice;
-    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
-                       node_blocks, edge_tab, round_action, 0, partial);
-    hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
-                       first, node_blocks, partial, round_action, last_action, steps, 0);
-    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
-                       node_blocks, edge_tab, pending, 2);
+                       count, has32 ? 0 : 1);
     hipLaunchKernelGGL(wide_count<PraqJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());
@@ -806,30 +795,30 @@ int WidePraq::enqueue_round(hipStream_t stream) {
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_pre<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
                        count, round_action, pending);
-    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
-                       node_blocks, edge_tab, pending, 0);
+    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
+                       node_blocks, edge_tab, round_action, pending, 0);
     for (int j = 0; j < op_steps; ++j)
         if (step(stream, 0, j)) return kErrorDevice;
-    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
                        node_blocks, edge_tab, round_action, 0, partial);
-    hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
-                       first, node_blocks, partial, round_action, last_action, steps, 0);
+    hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first,
+                       node_blocks, partial, round_action, 0);
     if (has32) {
-        hipLaunchKernelGGL(wpraq_convert, dim3(all), blk, 0, stream, jobs32, jobs64, prec,
-                           node_tab, node_blocks, edge_tab);
+        hipLaunchKernelGGL(wpraq_convert, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
+                           node_blocks, edge_tab);
         hipLaunchKernelGGL(wide_promote<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream,
-                           jobs64, count, prec);
+                           jobs64, count);
     }
     // restarts: one uncounted vanilla step, the new anchor, the norms
     if (step(stream, 1, 0)) return kErrorDevice;
-    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
-                       node_blocks, edge_tab, pending, 1);
-    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
+    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
+                       node_blocks, edge_tab, round_action, pending, 1);
+    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
                        node_blocks, edge_tab, round_action, 1, partial);
-    hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
-                       first, node_blocks, partial, round_action, last_action, steps, 1);
-    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, prec, node_tab,
-                       node_blocks, edge_tab, pending, 2);
+    hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first,
+                       node_blocks, partial, round_action, 1);
+    hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
+                       node_blocks, edge_tab, round_action, pending, 2);
     hipLaunchKernelGGL(wide_count<PraqJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());
@@ -839,9 +828,9 @@ int WidePraq::enqueue_round(hipStream_t stream) {
 int WidePraq::expire(hipStream_t stream) {
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_expire<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream,
-                       jobs64, count, last_action);
+                       jobs64, count);
     hipLaunchKernelGGL(wpraq_copy, dim3(node_blocks + edge_blocks), blk, 0, stream, jobs32,
-                       jobs64, prec, node_tab, node_blocks, edge_tab, pending, 2);
+                       jobs64, node_tab, node_blocks, edge_tab, round_action, pending, 2);
     hipLaunchKernelGGL(wide_count<PraqJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());
@@ -857,17 +846,21 @@ int WidePria::setup(const std::vector<PriaJob<double>>& host64, const PriaJob<fl
     jobs32 = dev32;
     jobs64 = dev64;
     has32 = dev32 != nullptr;
-    if (!count) return 0;
+    if (!count) {
+        release();
+        return 0;
+    }
     std::vector<int4> xt;
     std::vector<int2> st;
     std::vector<int> fi(4 * size_t(count));
     op_steps = 2;
+    max_T = 2;
     int tile_rows = 0;
     // algorithmic bytes of one step per entry, b = bytes per value: x kernel
     // reads x, x0, h, y, z and writes x, r (7 b); y/z kernel reads y, y0, r,
     // z, z0, r and writes y, z (8 b)
     for (int k = 0; k < kKinds; ++k)
-        for (int p = 0; p < 2; ++p) step_bytes[k][p].resize(count);
+        for (int p = 0; p < 2; ++p) step_bytes[k][p].assign(count, 0.0);
     for (int q = 0; q < count; ++q) {
         const auto& a = host64[q].a;
         op_steps = std::max(op_steps, int(host64[q].p.op_steps));
@@ -914,10 +907,10 @@ int WidePria::setup(const std::vector<PriaJob<double>>& host64, const PriaJob<fl
 int WidePria::step(hipStream_t stream, int mode, int j) {
     if (mark(stream, 0, true)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_x, dim3(x_blocks), dim3(kWideBlock), lds, stream, jobs32, jobs64,
-                       prec, x_tab, round_action, mode, j);
+                       x_tab, round_action, mode, j);
     if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_yz, dim3(s_blocks), dim3(512), 0, stream, jobs32, jobs64, prec,
-                       s_tab, round_action, mode, j);
+    hipLaunchKernelGGL(wpria_yz, dim3(s_blocks), dim3(512), 0, stream, jobs32, jobs64, s_tab,
+                       round_action, mode, j);
     if (mark(stream, 1, false)) return kErrorDevice;
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
@@ -926,16 +919,7 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
 int WidePria::start(hipStream_t stream) {
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_init<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
-                       count, round_action, last_action, steps, prec, has32 ? 0 : 1);
-    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
-                       pending, 3);
-    if (step(stream, 0, 0)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
-                       prec, s_tab, s_blocks, x_tab, round_action, 0, partial);
-    hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
-                       first, s_blocks, partial, round_action, last_action, steps, 0);
-    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
-                       pending, 2);
+                       count, has32 ? 0 : 1);
     hipLaunchKernelGGL(wide_count<PriaJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());
@@ -946,29 +930,28 @@ int WidePria::enqueue_round(hipStream_t stream) {
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_pre<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
                        count, round_action, pending);
-    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
-                       pending, 0);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
+                       round_action, pending, 0);
     for (int j = 0; j < op_steps; ++j)
         if (step(stream, 0, j)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
-                       prec, s_tab, s_blocks, x_tab, round_action, 0, partial);
-    hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
-                       first, s_blocks, partial, round_action, last_action, steps, 0);
+                       s_tab, s_blocks, x_tab, round_action, 0, partial);
+    hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first, s_blocks,
+                       partial, round_action, 0);
     if (has32) {
-        hipLaunchKernelGGL(wpria_convert, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec,
-                           s_tab);
+        hipLaunchKernelGGL(wpria_convert, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab);
         hipLaunchKernelGGL(wide_promote<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream,
-                           jobs64, count, prec);
+                           jobs64, count);
     }
     if (step(stream, 1, 0)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
-                       pending, 1);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
+                       round_action, pending, 1);
     hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
-                       prec, s_tab, s_blocks, x_tab, round_action, 1, partial);
-    hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, prec, count,
-                       first, s_blocks, partial, round_action, last_action, steps, 1);
-    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
-                       pending, 2);
+                       s_tab, s_blocks, x_tab, round_action, 1, partial);
+    hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first, s_blocks,
+                       partial, round_action, 1);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
+                       round_action, pending, 2);
     hipLaunchKernelGGL(wide_count<PriaJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());
@@ -978,9 +961,9 @@ int WidePria::enqueue_round(hipStream_t stream) {
 int WidePria::expire(hipStream_t stream) {
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_expire<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream,
-                       jobs64, count, last_action);
-    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, prec, s_tab,
-                       pending, 2);
+                       jobs64, count);
+    hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
+                       round_action, pending, 2);
     hipLaunchKernelGGL(wide_count<PriaJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());

@@ -12,7 +12,7 @@
 // round is
 //
 //   pre           per instance: the round's action, pending anchor copies
-//   anchor copy   x0 <- x, y0 <- y where pending
+//   copies        start copies of instances starting, anchors where pending
 //   op_steps x    the step kernels (node + edge / x + y-z)
 //   check sums    per block partial rows of the check sums
 //   logic         one workgroup per instance: fixed-order reduction of its
@@ -43,9 +43,11 @@
 
 namespace reform {
 
-// Round actions beyond halpern.hpp's: the uncounted first vanilla step of a
-// cold start, and nothing.
+// Actions beyond halpern.hpp's: kActNew marks an instance admitted but not
+// started; its first round is kActInit (the start copies, the uncounted
+// first vanilla step, its check); kActSkip is a round that does nothing.
 constexpr int kActInit = 5;
+constexpr int kActNew = 6;
 constexpr int kActSkip = -1;
 
 // x tile width of the batched PRIA kernels, both precisions (one tile table
@@ -59,21 +61,22 @@ struct WideStats {
 
 // What every wide engine holds besides its block tables: per-instance round
 // control, the graph of one round and the host side of the active count.
+// The instances' lasting state -- Halpern scalars, precision, last action,
+// step counts -- lives in their LogicState records, so the tables can be
+// rebuilt for a new set of instances (setup() again) between rounds.
 struct WideControl {
     int count = 0, op_steps = 2;
     int* first = nullptr;          // [4 count]: two (first row, rows) ranges
     double* partial = nullptr;     // partial check-sum rows
     int* round_action = nullptr;   // [count] the action this round takes
-    int* last_action = nullptr;    // [count] the last action executed
     int* pending = nullptr;        // [count] anchor copy due this round
-    int* prec = nullptr;           // [count] 0: FP32 iterates, 1: FP64
     int* active = nullptr;         // [1] instances not finished
     int* host_active = nullptr;    // [2] pinned
-    int64_t* steps = nullptr;      // [2 count] steps taken in FP32, then in FP64
     hipEvent_t polled[2] = {nullptr, nullptr};
     void* arena = nullptr;
     size_t bytes = 0;
-    bool has32 = false;            // FP32 images exist (an FP32 batch)
+    bool has32 = false;            // FP32 images exist (FP32 iterates first)
+    bool use_graph = true;         // replay one captured round (else launch directly)
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
 
@@ -94,7 +97,8 @@ struct WideControl {
     std::vector<double> step_bytes[kKinds][2];
 
     virtual ~WideControl();
-    // the cold start and the first count of active instances
+    // every instance of the tables cold-starts at its next round (states set
+    // to kActNew); then the first active count
     virtual int start(hipStream_t stream) = 0;
     // one round's launches (captured into the graph)
     virtual int enqueue_round(hipStream_t stream) = 0;
@@ -102,8 +106,10 @@ struct WideControl {
     virtual int expire(hipStream_t stream) = 0;
     // start, then rounds until no instance is active
     int solve(hipStream_t stream, double time_limit, WideStats* stats);
-    // arena: the caller's tables (table_bytes) followed by the control arrays
+    // arena: the caller's tables (table_bytes) followed by the control arrays;
+    // drops the previous tables and graph
     int allocate(size_t table_bytes, size_t partial_rows, int partial_width);
+    void release();
     // bracket one step launch of `kind` (no-op unless timing)
     int mark(hipStream_t stream, int kind, bool begin);
     // add the event pairs of a completed set to kernel_ms / launches
@@ -111,10 +117,9 @@ struct WideControl {
     // turn timing on or off (the totals of the last solve stay readable)
     void set_timing(bool on);
     void clear_timing();
-    // algorithmic bytes moved by each step kernel over the last solve
-    int algorithmic_bytes(double (&out)[kKinds]) const;
-    // per instance: 1 where its answer is in the FP64 image (after a solve)
-    int precisions(std::vector<int>& out) const;
+    // algorithmic bytes moved by each step kernel, from the instances' step
+    // counts (states in table order)
+    void algorithmic_bytes(const std::vector<LogicState>& states, double (&out)[kKinds]) const;
 };
 
 // One engine's solve advanced in steps, so that one host thread can keep
@@ -145,6 +150,8 @@ struct WidePraq : WideControl {
     int2* edge_tab = nullptr;
     int node_blocks = 0, edge_blocks = 0;
 
+    // tables for the instances of host64 (device records dev32 / dev64 in the
+    // same order; dev32 null when FP32 images do not exist)
     int setup(const std::vector<PraqJob<double>>& host64, const PraqJob<float>* dev32,
               const PraqJob<double>* dev64);
     int step(hipStream_t stream, int mode, int j);
