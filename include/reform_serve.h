@@ -1,0 +1,96 @@
+/* Continuous serving of PRAQ and PRIA solves on one MI355X.
+ *
+ * No counterpart in the reference (its solvers take one instance per call,
+ * qubo/praq-cuda/src/praq.cu:35-79, jrp/pria-cuda/src/pria.cu:15-44).  A
+ * server owns a device memory pool (the HBM budget) and a serving thread:
+ * requests submitted from any thread are built on host threads, admitted as
+ * soon as their images fit in the pool, join the batched tier's rounds (each
+ * instance with its own Halpern state, precision and termination), and are
+ * retired as they converge -- their slots and memory backfilled by the next
+ * admissions.  Every request is solved with the same arithmetic as alone in a
+ * batch, so its result does not depend on what else was in flight.
+ *
+ * Requests are admitted in submission order; one whose images exceed the
+ * whole budget is rejected (status -2).
+ *
+ * Return codes: >= 0 success (submit returns the ticket), -1 argument,
+ * -2 instance, -3 device, -4 unknown or unfinished ticket, -5 timeout.
+ */
+#ifndef REFORMULATION_REFORM_SERVE_H
+#define REFORMULATION_REFORM_SERVE_H
+
+#include <stdint.h>
+
+#include "praq_hip.h"
+#include "pria_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct reform_server reform_server;
+
+typedef struct reform_server_config {
+    int use_float32;        /* FP32 iterates (FP64 reductions, FP64 finish), else FP64 */
+    int host_threads;       /* host builders of instance images (0: 4) */
+    int64_t hbm_budget;     /* bytes of device memory for instance images (0: 200e9) */
+    int64_t max_in_flight;  /* instances on the device at once (0: 16384) */
+    int borrow_inputs;      /* 1: the server reads the caller's input arrays, which stay
+                               valid and unchanged until the request has finished;
+                               0: they are copied at submission */
+    int build_ahead;        /* built requests waiting for device memory, at most (0: 32) */
+} reform_server_config;
+
+typedef struct reform_request_info {
+    int64_t ticket;
+    int family;             /* 0 PRAQ, 1 PRIA */
+    int status;             /* 0 converged, 1 iteration limit, 2 time limit,
+                               -1 not finished, -2 instance rejected, -3 device */
+    int64_t iterations;
+    int64_t restarts;
+    int64_t switch_iteration;   /* FP32 -> FP64 handover, -1 if none */
+    double lower, upper, relative_gap;
+    double submitted, admitted, completed;  /* seconds since the server started */
+    int64_t device_bytes;   /* pool bytes the request's images take */
+} reform_request_info;
+
+typedef struct reform_server_counters {
+    int64_t submitted, admitted, completed, rejected;
+    int64_t rounds, rebuilds;
+    int64_t in_flight;
+    int64_t bytes_in_use, bytes_peak, bytes_budget;
+} reform_server_counters;
+
+reform_server_config reform_server_default_config(void);
+reform_server* reform_server_create(const reform_server_config* config);
+/* Stops the serving thread (requests still in flight are dropped). */
+void reform_server_destroy(reform_server* server);
+
+/* Copies the instance and returns its ticket at once; the work happens on the
+ * server's threads. */
+int64_t reform_server_submit_praq(reform_server* server, const PraqInstance* instance,
+                                  const PraqParameters* parameters, const PraqOptions* options);
+int64_t reform_server_submit_pria(reform_server* server, int64_t n, int64_t t,
+                                  const double* order_costs, double joint_cost,
+                                  const double* holding_costs, const pria_params* params,
+                                  const pria_options* options);
+
+/* Tickets finished since the last poll (at most max); returns the count. */
+int64_t reform_server_poll(reform_server* server, int64_t* tickets, int64_t max);
+/* Blocks until every submitted request has finished, or the timeout. */
+int reform_server_wait_idle(reform_server* server, double timeout_seconds);
+
+int reform_server_info(reform_server* server, int64_t ticket, reform_request_info* info);
+/* Results of a finished request, shapes as reform_batch_*_result.  Outputs
+ * may be NULL. */
+int reform_server_praq_result(reform_server* server, int64_t ticket, double* x, double* y);
+int reform_server_pria_result(reform_server* server, int64_t ticket, double* x, double* y,
+                              double* z);
+/* Drops a finished request's record and results. */
+int reform_server_release(reform_server* server, int64_t ticket);
+int reform_server_stats(reform_server* server, reform_server_counters* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

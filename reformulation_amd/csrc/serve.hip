@@ -1,0 +1,774 @@
+// The serving loop of include/reform_serve.h.
+//
+// Threads: the caller's threads submit (copy inputs, enqueue); `host_threads`
+// builders turn inputs into device images in host memory (half-edge build or
+// slice layout, both precisions); one serving thread owns the GPU.  Its loop:
+//
+//   admit     move built requests into the device pool while they fit (first
+//             fit, 256-byte aligned, coalescing on free): upload the images,
+//             cold-start PRIA images, write the job records and a fresh
+//             LogicState (kActNew: the request starts at its next round)
+//   rebuild   when a family's membership changed, its wide engine's block
+//             tables and job arrays are rebuilt for the new members
+//   round     one round of each family's engine (PRAQ and PRIA on their own
+//             streams), a hipGraph captured once per membership and replayed
+//   retire    a per-member done flag comes back; finished members' answers
+//             and states are copied out, their memory and slots freed
+//
+// so requests join and leave at round boundaries while the others keep
+// their Halpern state on the device.
+#include "reform_serve.h"
+#include "wide.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using namespace reform;
+
+namespace {
+
+constexpr int kErrorTicket = -4;
+constexpr int kErrorTimeout = -5;
+constexpr int kPending = -1;
+
+__global__ void serve_done_flags(const PraqJob<double>* praq, int np, const PriaJob<double>* pria,
+                                 int nr, int* flags) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < np) flags[k] = praq[k].state->action == kActDoneX;
+    else if (k < np + nr) flags[k] = pria[k - np].state->action == kActDoneX;
+}
+
+// First-fit allocator over one device allocation.
+class Pool {
+public:
+    int init(size_t bytes) {
+        capacity_ = bytes & ~size_t(255);
+        hipError_t e = hipMalloc(&base_, capacity_);
+        if (e != hipSuccess) {
+            base_ = nullptr;
+            record_error("hipMalloc(serving pool)", e, __FILE__, __LINE__);
+            return kErrorDevice;
+        }
+        free_[0] = capacity_;
+        return 0;
+    }
+    ~Pool() {
+        if (base_) (void)hipFree(base_);
+    }
+    // offset of a block of `bytes`, or SIZE_MAX
+    size_t alloc(size_t bytes) {
+        bytes = align256(bytes ? bytes : 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, left = it->second - bytes;
+            free_.erase(it);
+            if (left) free_[off + bytes] = left;
+            used_ += bytes;
+            peak_ = std::max(peak_, used_);
+            return off;
+        }
+        return SIZE_MAX;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align256(bytes ? bytes : 1);
+        used_ -= bytes;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += bytes;
+                return;
+            }
+        }
+        free_[off] = bytes;
+    }
+    char* base() const { return static_cast<char*>(base_); }
+    size_t used() const { return used_; }
+    size_t peak() const { return peak_; }
+    size_t capacity() const { return capacity_; }
+
+private:
+    void* base_ = nullptr;
+    size_t capacity_ = 0, used_ = 0, peak_ = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct Request {
+    int64_t ticket = 0;
+    int family = 0;
+    // inputs: the caller's arrays (borrowed) or the copies below
+    const double *diag = nullptr, *vals = nullptr, *k = nullptr, *h = nullptr;
+    const int64_t *rows = nullptr, *cols = nullptr;
+    std::vector<double> own_diag, own_vals, own_k, own_h;
+    std::vector<int64_t> own_rows, own_cols;
+    int64_t n = 0, m = 0, t = 0;
+    double k0 = 0.0;
+    PraqOptions po{};
+    LogicParams lp{};
+    // host images
+    PraqHostInstance ph;
+    PriaHostInstance rh;
+    PraqLayout pl64{}, pl32{};
+    PriaLayout rl64{}, rl32{};
+    std::vector<char> img64, img32;
+    size_t up64 = 0, up32 = 0;
+    int build_status = 0;
+    // device
+    size_t off = SIZE_MAX, bytes = 0;
+    int slot = -1;
+    PraqJob<double> pj64{};
+    PraqJob<float> pj32{};
+    PriaJob<double> rj64{};
+    PriaJob<float> rj32{};
+    // outcome
+    LogicState final{};
+    bool in64 = true;
+    std::vector<double> d1, d2, d3;
+    std::vector<float> f1, f2, f3;
+    int status = kPending;
+    double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
+};
+
+}  // namespace
+
+struct reform_server {
+    reform_server_config cfg{};
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    std::mutex mu;
+    std::condition_variable cv_build, cv_ready, cv_done;
+    std::deque<Request*> to_build, ready;
+    std::deque<int64_t> finished;
+    std::unordered_map<int64_t, std::unique_ptr<Request>> requests;
+    std::atomic<bool> stop{false};
+    int64_t next_ticket = 0;
+    int64_t n_submitted = 0, n_admitted = 0, n_completed = 0, n_rejected = 0;
+    int n_building = 0;  // requests on the builders
+    std::atomic<int64_t> n_rounds{0}, n_rebuilds{0}, n_in_flight{0};
+    std::vector<std::thread> builders;
+    std::thread server;
+    int device = 0;
+    int start_status = 0;
+
+    // serving-thread state
+    Pool pool;
+    LogicState* states = nullptr;
+    std::vector<int> free_slots;
+    hipStream_t streams[2] = {nullptr, nullptr};
+    std::vector<Request*> members[2];
+    bool changed[2] = {false, false};
+    WidePraq praq_engine;
+    WidePria pria_engine;
+    void* jobs_dev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [family][precision]
+    size_t jobs_cap[2] = {0, 0};
+    int* flags = nullptr;
+    int flags_cap = 0;
+    std::atomic<size_t> bytes_in_use{0}, bytes_peak{0};
+
+    ~reform_server() {
+        stop = true;
+        cv_build.notify_all();
+        cv_ready.notify_all();
+        for (auto& b : builders)
+            if (b.joinable()) b.join();
+        if (server.joinable()) server.join();
+        for (auto& f : jobs_dev)
+            for (void* p : f)
+                if (p) (void)hipFree(p);
+        if (flags) (void)hipFree(flags);
+        if (states) (void)hipFree(states);
+        for (auto& s : streams)
+            if (s) (void)hipStreamDestroy(s);
+    }
+
+    void build_loop();
+    void serve_loop();
+    int build(Request& r);
+    int admit(Request& r);
+    int rebuild(int family);
+    int retire_finished();
+    void retire(Request& r, int status);
+};
+
+// ------------------------------------------------------------ host builds --
+
+int reform_server::build(Request& r) {
+    const bool f32 = cfg.use_float32 != 0;
+    if (r.family == 0) {
+        PraqInstance in{r.n, r.m, r.diag, r.rows, r.cols, r.vals};
+        int st = praq_build_host(in, r.po.dense_degree, nullptr, nullptr, r.ph);
+        if (st) return st;
+        r.pl64 = praq_layout(r.ph.n, r.ph.m, sizeof(double));
+        r.img64.assign(r.pl64.input_bytes, 0);
+        praq_fill_image<double>(r.ph, r.pl64, r.img64.data());
+        r.up64 = r.pl64.input_bytes;
+        r.bytes = align256(r.pl64.total_bytes);
+        if (f32) {
+            r.pl32 = praq_layout(r.ph.n, r.ph.m, sizeof(float));
+            r.img32.assign(r.pl32.input_bytes, 0);
+            praq_fill_image<float>(r.ph, r.pl32, r.img32.data());
+            r.up32 = r.pl32.input_bytes;
+            r.bytes += align256(r.pl32.total_bytes);
+        }
+    } else {
+        int st = pria_build_host(r.n, r.t, r.k, r.k0, r.h, nullptr, nullptr, nullptr, r.rh);
+        if (st) return kErrorInstance;
+        r.rl64 = pria_layout(r.rh.N, r.rh.T, r.rh.ND, sizeof(double));
+        r.up64 = pria_upload_bytes(r.rh, r.rl64);
+        r.img64.assign(r.up64, 0);
+        pria_fill_image<double>(r.rh, r.rl64, r.img64.data());
+        r.bytes = align256(r.rl64.total_bytes);
+        if (f32) {
+            r.rl32 = pria_layout(r.rh.N, r.rh.T, r.rh.ND, sizeof(float));
+            r.up32 = pria_upload_bytes(r.rh, r.rl32);
+            r.img32.assign(r.up32, 0);
+            pria_fill_image<float>(r.rh, r.rl32, r.img32.data());
+            r.bytes += align256(r.rl32.total_bytes);
+        }
+    }
+    return 0;
+}
+
+void reform_server::build_loop() {
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            // build ahead of admission only so far: built images wait in host memory
+            cv_build.wait(lock, [&] {
+                return stop || (!to_build.empty() &&
+                                int(ready.size()) + n_building < cfg.build_ahead);
+            });
+            if (stop) return;
+            r = to_build.front();
+            to_build.pop_front();
+            ++n_building;
+        }
+        int st = build(*r);
+        if (!st && r->bytes > pool.capacity()) st = kErrorInstance;  // never fits
+        std::lock_guard<std::mutex> lock(mu);
+        --n_building;
+        if (st) {
+            r->status = st < 0 ? st : kErrorInstance;
+            r->t_done = seconds_since(t0);
+            ++n_rejected;
+            ++n_completed;
+            finished.push_back(r->ticket);
+            cv_done.notify_all();
+            cv_build.notify_one();
+        } else {
+            ready.push_back(r);
+            cv_ready.notify_all();
+        }
+    }
+}
+
+// -------------------------------------------------------------- admission --
+
+int reform_server::admit(Request& r) {
+    const bool f32 = cfg.use_float32 != 0;
+    char* base = pool.base() + r.off;
+    hipStream_t s = streams[r.family];
+    LogicState* state = states + r.slot;
+    size_t off64 = 0, off32 = 0;
+    if (r.family == 0) {
+        off32 = align256(r.pl64.total_bytes);
+        REFORM_HIP_CHECK(hipMemcpyAsync(base + off64, r.img64.data(), r.up64, hipMemcpyHostToDevice, s));
+        if (f32)
+            REFORM_HIP_CHECK(
+                hipMemcpyAsync(base + off32, r.img32.data(), r.up32, hipMemcpyHostToDevice, s));
+        r.pj64 = PraqJob<double>{};
+        r.pj64.a = praq_bind<double>(r.pl64, base + off64, r.ph.n, r.ph.m);
+        r.pj64.p = r.lp;
+        r.pj64.scale = r.ph.scale;
+        r.pj64.lanes = r.ph.lanes;
+        r.pj64.state = state;
+        if (f32) {
+            r.pj32 = PraqJob<float>{};
+            r.pj32.a = praq_bind<float>(r.pl32, base + off32, r.ph.n, r.ph.m);
+            r.pj32.p = r.lp;
+            apply_fp32_policy(r.pj32.p);
+            r.pj32.scale = r.ph.scale;
+            r.pj32.lanes = r.ph.lanes;
+            r.pj32.state = state;
+        }
+    } else {
+        off32 = align256(r.rl64.total_bytes);
+        REFORM_HIP_CHECK(hipMemcpyAsync(base + off64, r.img64.data(), r.up64, hipMemcpyHostToDevice, s));
+        r.rj64 = PriaJob<double>{};
+        r.rj64.a = pria_bind<double>(r.rl64, base + off64, r.rh);
+        if (r.rh.cold && pria_cold_start<double>(r.rj64.a, s)) return kErrorDevice;
+        r.rj64.p = r.lp;
+        r.rj64.scale = r.rh.scale;
+        r.rj64.state = state;
+        if (f32) {
+            REFORM_HIP_CHECK(
+                hipMemcpyAsync(base + off32, r.img32.data(), r.up32, hipMemcpyHostToDevice, s));
+            r.rj32 = PriaJob<float>{};
+            r.rj32.a = pria_bind<float>(r.rl32, base + off32, r.rh);
+            if (r.rh.cold && pria_cold_start<float>(r.rj32.a, s)) return kErrorDevice;
+            r.rj32.p = r.lp;
+            apply_fp32_policy(r.rj32.p);
+            r.rj32.scale = r.rh.scale;
+            r.rj32.state = state;
+        }
+    }
+    LogicState init{};
+    logic_init(init, r.lp);
+    init.action = kActNew;
+    init.in64 = f32 ? 0 : 1;
+    init.last_action = kActInit;
+    init.steps32 = init.steps64 = 0;
+    init.switch_iteration = -1;
+    REFORM_HIP_CHECK(hipMemcpyAsync(state, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    // the host image is no longer needed once the copies are done
+    REFORM_HIP_CHECK(hipStreamSynchronize(s));
+    r.img64.clear();
+    r.img64.shrink_to_fit();
+    r.img32.clear();
+    r.img32.shrink_to_fit();
+    return 0;
+}
+
+// The family's engine over its current members: job arrays uploaded in
+// member order, block tables rebuilt.
+int reform_server::rebuild(int family) {
+    const bool f32 = cfg.use_float32 != 0;
+    auto& mem = members[family];
+    const size_t count = mem.size();
+    n_rebuilds++;
+    auto grow = [&](size_t rec32, size_t rec64) -> int {
+        if (count <= jobs_cap[family]) return 0;
+        size_t cap = std::max<size_t>(count, 2 * jobs_cap[family]);
+        for (int p = 0; p < 2; ++p)
+            if (jobs_dev[family][p]) (void)hipFree(jobs_dev[family][p]);
+        REFORM_HIP_CHECK(hipMalloc(&jobs_dev[family][0], rec32 * cap));
+        REFORM_HIP_CHECK(hipMalloc(&jobs_dev[family][1], rec64 * cap));
+        jobs_cap[family] = cap;
+        return 0;
+    };
+    if (family == 0) {
+        if (grow(sizeof(PraqJob<float>), sizeof(PraqJob<double>))) return kErrorDevice;
+        std::vector<PraqJob<double>> h64(count);
+        std::vector<PraqJob<float>> h32(count);
+        for (size_t q = 0; q < count; ++q) {
+            h64[q] = mem[q]->pj64;
+            h32[q] = mem[q]->pj32;
+        }
+        auto* d64 = static_cast<PraqJob<double>*>(jobs_dev[0][1]);
+        auto* d32 = static_cast<PraqJob<float>*>(jobs_dev[0][0]);
+        if (count) {
+            REFORM_HIP_CHECK(hipMemcpy(d64, h64.data(), sizeof(h64[0]) * count, hipMemcpyHostToDevice));
+            if (f32)
+                REFORM_HIP_CHECK(
+                    hipMemcpy(d32, h32.data(), sizeof(h32[0]) * count, hipMemcpyHostToDevice));
+        }
+        if (praq_engine.setup(h64, f32 ? d32 : nullptr, d64)) return kErrorDevice;
+    } else {
+        if (grow(sizeof(PriaJob<float>), sizeof(PriaJob<double>))) return kErrorDevice;
+        std::vector<PriaJob<double>> h64(count);
+        std::vector<PriaJob<float>> h32(count);
+        for (size_t q = 0; q < count; ++q) {
+            h64[q] = mem[q]->rj64;
+            h32[q] = mem[q]->rj32;
+        }
+        auto* d64 = static_cast<PriaJob<double>*>(jobs_dev[1][1]);
+        auto* d32 = static_cast<PriaJob<float>*>(jobs_dev[1][0]);
+        if (count) {
+            REFORM_HIP_CHECK(hipMemcpy(d64, h64.data(), sizeof(h64[0]) * count, hipMemcpyHostToDevice));
+            if (f32)
+                REFORM_HIP_CHECK(
+                    hipMemcpy(d32, h32.data(), sizeof(h32[0]) * count, hipMemcpyHostToDevice));
+        }
+        if (pria_engine.setup(h64, f32 ? d32 : nullptr, d64)) return kErrorDevice;
+    }
+    changed[family] = false;
+    return 0;
+}
+
+// ------------------------------------------------------------- retirement --
+
+template <class Real>
+static int copy_out(std::vector<Real>& out, const Real* src, size_t count, hipStream_t s) {
+    out.resize(count);
+    if (count)
+        REFORM_HIP_CHECK(hipMemcpyAsync(out.data(), src, sizeof(Real) * count,
+                                        hipMemcpyDeviceToHost, s));
+    return 0;
+}
+
+void reform_server::retire(Request& r, int status) {
+    pool.release(r.off, r.bytes);
+    bytes_in_use = pool.used();
+    free_slots.push_back(r.slot);
+    n_in_flight--;
+    std::lock_guard<std::mutex> lock(mu);
+    r.status = status;
+    r.t_done = seconds_since(t0);
+    ++n_completed;
+    finished.push_back(r.ticket);
+    cv_done.notify_all();
+}
+
+int reform_server::retire_finished() {
+    const int np = int(members[0].size()), nr = int(members[1].size());
+    const int total = np + nr;
+    if (!total) return 0;
+    if (total > flags_cap) {
+        if (flags) (void)hipFree(flags);
+        flags_cap = std::max(total, 2 * flags_cap);
+        REFORM_HIP_CHECK(hipMalloc(&flags, sizeof(int) * flags_cap));
+    }
+    hipLaunchKernelGGL(serve_done_flags, dim3((total + 255) / 256), dim3(256), 0, streams[0],
+                       static_cast<const PraqJob<double>*>(jobs_dev[0][1]), np,
+                       static_cast<const PriaJob<double>*>(jobs_dev[1][1]), nr, flags);
+    std::vector<int> done(total);
+    REFORM_HIP_CHECK(hipMemcpyAsync(done.data(), flags, sizeof(int) * total, hipMemcpyDeviceToHost,
+                                    streams[0]));
+    REFORM_HIP_CHECK(hipStreamSynchronize(streams[0]));
+    for (int f = 0; f < 2; ++f) {
+        auto& mem = members[f];
+        std::vector<Request*> keep;
+        const int shift = f == 0 ? 0 : np;
+        for (size_t q = 0; q < mem.size(); ++q) {
+            if (!done[shift + q]) {
+                keep.push_back(mem[q]);
+                continue;
+            }
+            Request& r = *mem[q];
+            hipStream_t s = streams[f];
+            REFORM_HIP_CHECK(hipMemcpyAsync(&r.final, states + r.slot, sizeof(LogicState),
+                                            hipMemcpyDeviceToHost, s));
+            REFORM_HIP_CHECK(hipStreamSynchronize(s));
+            r.in64 = r.final.in64 != 0;
+            int st = 0;
+            if (f == 0) {
+                const size_t n = r.ph.n, m2 = 2 * size_t(r.ph.m);
+                st = r.in64 ? (copy_out(r.d1, r.pj64.a.x, n, s) || copy_out(r.d2, r.pj64.a.y, m2, s))
+                            : (copy_out(r.f1, r.pj32.a.x, n, s) || copy_out(r.f2, r.pj32.a.y, m2, s));
+            } else {
+                const size_t nd = r.rh.ND;
+                st = r.in64 ? (copy_out(r.d1, r.rj64.a.x, nd, s) || copy_out(r.d2, r.rj64.a.y, nd, s) ||
+                               copy_out(r.d3, r.rj64.a.z, nd, s))
+                            : (copy_out(r.f1, r.rj32.a.x, nd, s) || copy_out(r.f2, r.rj32.a.y, nd, s) ||
+                               copy_out(r.f3, r.rj32.a.z, nd, s));
+            }
+            if (st || hipStreamSynchronize(s) != hipSuccess) return kErrorDevice;
+            retire(r, r.final.status);
+        }
+        if (keep.size() != mem.size()) {
+            mem.swap(keep);
+            changed[f] = true;
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------- serving thread --
+
+void reform_server::serve_loop() {
+    auto fail = [&](int code) {
+        std::lock_guard<std::mutex> lock(mu);
+        start_status = code;
+        stop = true;
+        cv_done.notify_all();
+    };
+    if (hipSetDevice(device) != hipSuccess) return fail(kErrorDevice);
+    for (int f = 0; f < 2; ++f) {
+        if (streams[f] == nullptr && hipStreamCreateWithFlags(&streams[f], hipStreamNonBlocking) !=
+                                         hipSuccess)
+            return fail(kErrorDevice);
+    }
+    while (!stop) {
+        // admit what fits
+        std::vector<Request*> admitted;
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            if (members[0].empty() && members[1].empty() && ready.empty())
+                cv_ready.wait_for(lock, std::chrono::milliseconds(5),
+                                  [&] { return stop || !ready.empty(); });
+            if (stop) break;
+            while (!ready.empty() && !free_slots.empty()) {
+                Request* r = ready.front();
+                const size_t off = pool.alloc(r->bytes);
+                if (off == SIZE_MAX) break;  // wait for memory: the queue keeps its order
+                r->off = off;
+                r->slot = free_slots.back();
+                free_slots.pop_back();
+                ready.pop_front();
+                admitted.push_back(r);
+            }
+        }
+        bytes_in_use = pool.used();
+        bytes_peak = pool.peak();
+        if (!admitted.empty()) cv_build.notify_all();
+        for (Request* r : admitted) {
+            if (admit(*r)) return fail(kErrorDevice);
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                r->t_admit = seconds_since(t0);
+                ++n_admitted;
+            }
+            n_in_flight++;
+            members[r->family].push_back(r);
+            changed[r->family] = true;
+        }
+        if (members[0].empty() && members[1].empty()) continue;
+        // one round of each family (tables first: a rebuild may reallocate)
+        for (int f = 0; f < 2; ++f)
+            if (changed[f] && rebuild(f)) return fail(kErrorDevice);
+        for (int f = 0; f < 2; ++f) {
+            if (members[f].empty()) continue;
+            WideControl& w = f == 0 ? static_cast<WideControl&>(praq_engine) : pria_engine;
+            if (w.launch_round(streams[f])) return fail(kErrorDevice);
+        }
+        n_rounds++;
+        if (hipStreamSynchronize(streams[0]) != hipSuccess ||
+            hipStreamSynchronize(streams[1]) != hipSuccess)
+            return fail(kErrorDevice);
+        if (retire_finished()) return fail(kErrorDevice);
+    }
+}
+
+// ------------------------------------------------------------------ C ABI --
+
+extern "C" reform_server_config reform_server_default_config(void) {
+    reform_server_config c;
+    c.use_float32 = 1;
+    c.host_threads = 4;
+    c.hbm_budget = int64_t(200) * 1000 * 1000 * 1000;
+    c.max_in_flight = 16384;
+    c.borrow_inputs = 0;
+    c.build_ahead = 32;
+    return c;
+}
+
+extern "C" reform_server* reform_server_create(const reform_server_config* config) {
+    reform_server_config c = config ? *config : reform_server_default_config();
+    if (c.host_threads <= 0) c.host_threads = 4;
+    if (c.hbm_budget <= 0) c.hbm_budget = int64_t(200) * 1000 * 1000 * 1000;
+    if (c.max_in_flight <= 0) c.max_in_flight = 16384;
+    if (c.build_ahead <= 0) c.build_ahead = 32;
+    auto srv = std::make_unique<reform_server>();
+    srv->cfg = c;
+    if (hipGetDevice(&srv->device) != hipSuccess) return nullptr;
+    if (srv->pool.init(size_t(c.hbm_budget))) return nullptr;
+    if (hipMalloc(&srv->states, sizeof(LogicState) * size_t(c.max_in_flight)) != hipSuccess) {
+        srv->states = nullptr;
+        return nullptr;
+    }
+    for (int64_t q = c.max_in_flight - 1; q >= 0; --q) srv->free_slots.push_back(int(q));
+    reform_server* raw = srv.get();
+    for (int q = 0; q < c.host_threads; ++q) raw->builders.emplace_back([raw] { raw->build_loop(); });
+    raw->server = std::thread([raw] { raw->serve_loop(); });
+    return srv.release();
+}
+
+extern "C" void reform_server_destroy(reform_server* server) { delete server; }
+
+static int64_t enqueue(reform_server* srv, std::unique_ptr<Request> r) {
+    std::lock_guard<std::mutex> lock(srv->mu);
+    if (srv->stop) return srv->start_status ? srv->start_status : kErrorDevice;
+    r->ticket = srv->next_ticket++;
+    r->t_submit = seconds_since(srv->t0);
+    const int64_t ticket = r->ticket;
+    srv->to_build.push_back(r.get());
+    srv->requests[ticket] = std::move(r);
+    ++srv->n_submitted;
+    srv->cv_build.notify_one();
+    return ticket;
+}
+
+extern "C" int64_t reform_server_submit_praq(reform_server* server, const PraqInstance* instance,
+                                             const PraqParameters* parameters,
+                                             const PraqOptions* options) {
+    if (!server || !instance || !parameters) return kErrorArgument;
+    if (instance->n <= 0 || instance->m < 0 || !instance->diag ||
+        (instance->m > 0 && (!instance->rows || !instance->cols || !instance->vals)))
+        return kErrorInstance;
+    auto r = std::make_unique<Request>();
+    r->family = 0;
+    r->n = instance->n;
+    r->m = instance->m;
+    r->diag = instance->diag;
+    r->rows = instance->rows;
+    r->cols = instance->cols;
+    r->vals = instance->vals;
+    if (!server->cfg.borrow_inputs) {
+        r->own_diag.assign(instance->diag, instance->diag + instance->n);
+        r->diag = r->own_diag.data();
+        if (instance->m > 0) {
+            r->own_rows.assign(instance->rows, instance->rows + instance->m);
+            r->own_cols.assign(instance->cols, instance->cols + instance->m);
+            r->own_vals.assign(instance->vals, instance->vals + instance->m);
+            r->rows = r->own_rows.data();
+            r->cols = r->own_cols.data();
+            r->vals = r->own_vals.data();
+        }
+    }
+    PraqParameters p = *parameters;
+    if (p.max_iter <= 0) p.max_iter = -1;
+    if (p.op_steps <= 0) p.op_steps = praq_check_interval(instance->n, &p);
+    if (p.op_steps < 2) p.op_steps = 2;
+    r->po = options ? *options : praq_default_options();
+    r->lp = praq_logic_params(p, r->po);
+    return enqueue(server, std::move(r));
+}
+
+extern "C" int64_t reform_server_submit_pria(reform_server* server, int64_t n, int64_t t,
+                                             const double* order_costs, double joint_cost,
+                                             const double* holding_costs, const pria_params* params,
+                                             const pria_options* options) {
+    if (!server || !params || n <= 0 || t <= 0 || !order_costs || !holding_costs)
+        return kErrorArgument;
+    (void)options;
+    auto r = std::make_unique<Request>();
+    r->family = 1;
+    r->n = n;
+    r->t = t;
+    r->k = order_costs;
+    r->h = holding_costs;
+    if (!server->cfg.borrow_inputs) {
+        r->own_k.assign(order_costs, order_costs + n);
+        r->own_h.assign(holding_costs, holding_costs + size_t(n) * t * t);
+        r->k = r->own_k.data();
+        r->h = r->own_h.data();
+    }
+    r->k0 = joint_cost;
+    r->lp = pria_logic_params(*params);
+    return enqueue(server, std::move(r));
+}
+
+extern "C" int64_t reform_server_poll(reform_server* server, int64_t* tickets, int64_t max) {
+    if (!server || (!tickets && max > 0)) return kErrorArgument;
+    std::lock_guard<std::mutex> lock(server->mu);
+    int64_t k = 0;
+    while (k < max && !server->finished.empty()) {
+        tickets[k++] = server->finished.front();
+        server->finished.pop_front();
+    }
+    return k;
+}
+
+extern "C" int reform_server_wait_idle(reform_server* server, double timeout_seconds) {
+    if (!server) return kErrorArgument;
+    std::unique_lock<std::mutex> lock(server->mu);
+    auto idle = [&] { return server->stop || server->n_completed == server->n_submitted; };
+    if (timeout_seconds > 0.0) {
+        if (!server->cv_done.wait_for(lock, std::chrono::duration<double>(timeout_seconds), idle))
+            return kErrorTimeout;
+    } else {
+        server->cv_done.wait(lock, idle);
+    }
+    return server->start_status;
+}
+
+static Request* find(reform_server* server, int64_t ticket) {
+    auto it = server->requests.find(ticket);
+    return it == server->requests.end() ? nullptr : it->second.get();
+}
+
+extern "C" int reform_server_info(reform_server* server, int64_t ticket, reform_request_info* info) {
+    if (!server || !info) return kErrorArgument;
+    std::lock_guard<std::mutex> lock(server->mu);
+    Request* r = find(server, ticket);
+    if (!r) return kErrorTicket;
+    std::memset(info, 0, sizeof(*info));
+    info->ticket = ticket;
+    info->family = r->family;
+    info->status = r->status;
+    info->submitted = r->t_submit;
+    info->admitted = r->t_admit;
+    info->completed = r->t_done;
+    info->device_bytes = int64_t(r->bytes);
+    info->switch_iteration = -1;
+    if (r->status >= 0) {
+        info->iterations = r->final.iteration;
+        info->restarts = r->final.restarts;
+        info->switch_iteration = r->final.switch_iteration;
+        info->lower = r->final.lower;
+        info->upper = r->final.upper;
+        info->relative_gap = r->final.gap;
+        if (r->family == 1) {  // the upper bound of the unpadded original at the answer
+            std::vector<double> x(size_t(r->n) * r->t * r->t);
+            if (r->in64) pria_unpack(r->rh, r->d1, 0, x.data());
+            else pria_unpack(r->rh, r->f1, 0, x.data());
+            info->upper = pria_objective_host(r->n, r->t, r->k, r->k0, r->h, x.data());
+        }
+    }
+    return 0;
+}
+
+extern "C" int reform_server_praq_result(reform_server* server, int64_t ticket, double* x,
+                                         double* y) {
+    if (!server) return kErrorArgument;
+    std::lock_guard<std::mutex> lock(server->mu);
+    Request* r = find(server, ticket);
+    if (!r || r->family != 0 || r->status < 0) return kErrorTicket;
+    const size_t n = r->ph.n, m = r->ph.m;
+    for (size_t i = 0; x && i < n; ++i) x[i] = r->in64 ? r->d1[i] : double(r->f1[i]);
+    for (size_t e = 0; y && e < m; ++e) {
+        const double a = r->in64 ? r->d2[e] : double(r->f2[e]);
+        const double b = r->in64 ? r->d2[m + e] : double(r->f2[m + e]);
+        y[2 * e] = a * r->ph.scale;
+        y[2 * e + 1] = b * r->ph.scale;
+    }
+    return 0;
+}
+
+extern "C" int reform_server_pria_result(reform_server* server, int64_t ticket, double* x,
+                                         double* y, double* z) {
+    if (!server) return kErrorArgument;
+    std::lock_guard<std::mutex> lock(server->mu);
+    Request* r = find(server, ticket);
+    if (!r || r->family != 1 || r->status < 0) return kErrorTicket;
+    double* outs[3] = {x, y, z};
+    for (int w = 0; w < 3; ++w) {
+        if (!outs[w]) continue;
+        if (r->in64) pria_unpack(r->rh, w == 0 ? r->d1 : (w == 1 ? r->d2 : r->d3), w, outs[w]);
+        else pria_unpack(r->rh, w == 0 ? r->f1 : (w == 1 ? r->f2 : r->f3), w, outs[w]);
+    }
+    return 0;
+}
+
+extern "C" int reform_server_release(reform_server* server, int64_t ticket) {
+    if (!server) return kErrorArgument;
+    std::lock_guard<std::mutex> lock(server->mu);
+    Request* r = find(server, ticket);
+    if (!r || (r->status == kPending)) return kErrorTicket;
+    server->requests.erase(ticket);
+    return 0;
+}
+
+extern "C" int reform_server_stats(reform_server* server, reform_server_counters* stats) {
+    if (!server || !stats) return kErrorArgument;
+    std::lock_guard<std::mutex> lock(server->mu);
+    stats->submitted = server->n_submitted;
+    stats->admitted = server->n_admitted;
+    stats->completed = server->n_completed;
+    stats->rejected = server->n_rejected;
+    stats->rounds = server->n_rounds;
+    stats->rebuilds = server->n_rebuilds;
+    stats->in_flight = server->n_in_flight;
+    stats->bytes_in_use = int64_t(server->bytes_in_use);
+    stats->bytes_peak = int64_t(server->bytes_peak);
+    stats->bytes_budget = int64_t(server->pool.capacity());
+    return server->start_status;
+}

@@ -593,6 +593,19 @@ int WideRun::advance() {
     return changed;
 }
 
+int WideControl::launch_round(hipStream_t stream) {
+    if (!use_graph || timing) return enqueue_round(stream);
+    if (!exec) {
+        REFORM_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        const int st = enqueue_round(stream);
+        REFORM_HIP_CHECK(hipStreamEndCapture(stream, &graph));
+        if (st) return kErrorDevice;
+        REFORM_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    REFORM_HIP_CHECK(hipGraphLaunch(exec, stream));
+    return 0;
+}
+
 int WideControl::solve(hipStream_t stream, double time_limit, WideStats* stats) {
     WideRun run;
     if (run.begin(this, stream, time_limit)) return kErrorDevice;
@@ -671,7 +684,10 @@ void WideControl::algorithmic_bytes(const std::vector<LogicState>& states,
 }
 
 int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_width) {
-    release();
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
     size_t at = align256(table_bytes);
     const size_t o_first = at;
     at = align256(at + sizeof(int) * 4 * size_t(count));
@@ -679,15 +695,19 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     at = align256(at + sizeof(double) * partial_width * (partial_rows + 1));
     const size_t o_ctrl = at;
     at = align256(at + sizeof(int) * (2 * size_t(count) + 1));
-    {
-        hipError_t e = hipMalloc(&arena, at);
+    // the previous arena serves while it is large enough (tables rebuilt
+    // between rounds); a larger one grows by half again
+    if (!arena || at > bytes) {
+        const size_t grow = arena ? std::max(at, bytes + bytes / 2) : at;
+        release();
+        hipError_t e = hipMalloc(&arena, grow);
         if (e != hipSuccess) {
             arena = nullptr;
             record_error("hipMalloc(wide arena)", e, __FILE__, __LINE__);
             return kErrorDevice;
         }
+        bytes = grow;
     }
-    bytes = at;
     char* b = static_cast<char*>(arena);
     first = reinterpret_cast<int*>(b + o_first);
     partial = reinterpret_cast<double*>(b + o_partial);

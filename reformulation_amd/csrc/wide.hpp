@@ -104,6 +104,9 @@ struct WideControl {
     virtual int enqueue_round(hipStream_t stream) = 0;
     // out of time: every active instance ends at its current point
     virtual int expire(hipStream_t stream) = 0;
+    // one round: the captured graph replayed (captured at the first call
+    // after a setup), or launched directly when use_graph is off
+    int launch_round(hipStream_t stream);
     // start, then rounds until no instance is active
     int solve(hipStream_t stream, double time_limit, WideStats* stats);
     // arena: the caller's tables (table_bytes) followed by the control arrays;

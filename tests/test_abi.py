@@ -143,3 +143,46 @@ def test_wire_format_round_trip(tmp_path):
     q = tmp_path / "j.bin"
     wire.write_pria_instance(q, [2.0, 3.0], 5.0, np.zeros((2, 4, 4)))
     assert q.stat().st_size == 24 + 8 + 16 + 8 * 32
+
+
+def test_server_validates_arguments_without_a_gpu():
+    """The server's entry points reject null handles and bad arguments before
+    touching a device; creating one needs a GPU and fails cleanly without."""
+    import ctypes
+    from reformulation_amd import _native, serve
+    lib = _native.lib()
+    serve._declare(lib)
+    c = lib.reform_server_default_config()
+    assert c.use_float32 == 1 and c.hbm_budget == 200 * 10**9 and c.borrow_inputs == 0
+    assert c.host_threads > 0 and c.max_in_flight > 0 and c.build_ahead > 0
+    p = lib.praq_default_parameters()
+    inst = _native.PraqInstance(3, 0, None, None, None, None)
+    assert lib.reform_server_submit_praq(None, ctypes.byref(inst), ctypes.byref(p), None) == -1
+    assert lib.reform_server_poll(None, None, 0) == -1
+    assert lib.reform_server_wait_idle(None, 1.0) == -1
+    assert lib.reform_server_info(None, 0, None) == -1
+    assert lib.reform_server_release(None, 0) == -1
+    assert lib.reform_server_stats(None, None) == -1
+    import torch
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError):
+            serve.Server()
+
+
+def test_fifo_queue_and_poisson_arrivals():
+    """The plain solver's queue model and the arrival process, against closed
+    forms: arrivals every 1 s with 2 s of service wait q s more each; a
+    Poisson process's count over a window has mean and variance rate*window."""
+    import numpy as np
+    from reformulation_amd import serve
+    lat = serve.fifo_latencies(np.arange(5.0), np.full(5, 2.0))
+    assert np.array_equal(lat, 2.0 + np.arange(5.0))
+    lat = serve.fifo_latencies(np.arange(5.0) * 3.0, np.full(5, 2.0))   # never queued
+    assert np.array_equal(lat, np.full(5, 2.0))
+    counts = [len(serve.poisson_arrivals(200.0, 1.0, seed)) for seed in range(200)]
+    assert abs(np.mean(counts) - 200.0) < 5.0 and abs(np.var(counts) / 200.0 - 1.0) < 0.3
+    a = serve.poisson_arrivals(50.0, 2.0, 7)
+    assert np.all(np.diff(a) > 0) and a[0] >= 0.0 and a[-1] < 2.0
+    s = serve.summarize(np.array([0.0, 1.0, 2.0]), np.array([0.5, 1.5, 4.0]))
+    assert s["requests"] == 3 and s["p50_ms"] == pytest.approx(500.0)
+    assert s["solves_per_s"] == pytest.approx(3 / 4.0)

@@ -1,0 +1,161 @@
+"""The serving loop (include/reform_serve.h) against the batched mode, through
+the server C ABI.  Marked gpu.
+
+Parity bar: a request served by the server -- admitted while others are in
+flight, its round shared with whatever else is resident, retired and
+backfilled -- returns bitwise the answer, iteration count and bounds the
+batched mode gives the same instance, in FP64 and with FP32 iterates: an
+instance's arithmetic does not depend on what else is on the device."""
+import time
+
+import numpy as np
+import pytest
+
+from test_gpu_batch import _members, _batch, _results, _same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems(members, tol=1e-6):
+    from reformulation_amd import praq, pria
+    out = []
+    for kind, inst in members:
+        if kind == "praq":
+            n, diag, rows, cols, vals = inst
+            p = praq.QuboProblem.from_sparse(n, diag, rows, cols, vals)
+            out.append(("praq", p, praq.Parameters.with_auto_step(50_000, tol)))
+        else:
+            k, k0, h = inst
+            out.append(("pria", pria.JrpProblem.new(k, k0, h),
+                        pria.Parameters.with_auto_step(50_000, tol)))
+    return out
+
+
+def _submit(srv, item):
+    kind, problem, params = item
+    return srv.submit_praq(problem, params) if kind == "praq" else srv.submit_pria(problem, params)
+
+
+def _served(srv, kind, t):
+    if kind == "praq":
+        x, y, info = srv.praq_result(t)
+        return x, y, info
+    x, y, z, info = srv.pria_result(t)
+    return x, (y, z), info
+
+
+def _check_same(served, batched, q):
+    x, dual, info = served
+    bx, bdual, binfo = batched
+    assert info.status == 0, q
+    assert _same_bits(x, bx), q
+    assert _same_bits(dual, bdual), q
+    assert info.iterations == binfo.iterations, q
+    assert info.lower == binfo.lower and info.upper == binfo.upper, q
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_served_equals_batched_with_staggered_arrivals(f32):
+    from reformulation_amd.serve import Server
+    members = _members()
+    b, index = _batch(members, f32)
+    b.solve("batched")
+    reference = _results(b, index)
+    b.close()
+    items = _problems(members)
+    print(f"serve f32={f32}: {len(items)} requests, staggered", flush=True)
+    with Server(use_float32=f32, host_threads=2) as srv:
+        tickets = []
+        for q, item in enumerate(items):
+            tickets.append(_submit(srv, item))
+            time.sleep(0.003 * (q % 3))   # later requests join rounds in progress
+        srv.wait_idle(120.0)
+        stats = srv.stats()
+        for q, ((kind, _, _), t) in enumerate(zip(items, tickets)):
+            _check_same(_served(srv, kind, t), reference[q], q)
+            srv.release(t)
+        assert stats["completed"] == len(items) and stats["rejected"] == 0
+        assert stats["bytes_in_use"] == 0 and stats["in_flight"] == 0
+
+
+def test_backfill_under_a_small_budget():
+    """A budget that holds about two of the instances: requests wait for
+    memory, are admitted as others retire, and still return the batched
+    answers; the pool never exceeds its budget."""
+    from reformulation_amd.serve import Server
+    members = _members()
+    b, index = _batch(members, True)
+    b.solve("batched")
+    reference = _results(b, index)
+    b.close()
+    items = _problems(members)
+    with Server(use_float32=True, host_threads=2) as srv:
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(120.0)
+        sizes = [srv.info(t).device_bytes for t in tickets]
+    budget = 2 * max(sizes) + (1 << 20)
+    print(f"backfill: budget {budget} bytes for requests of {sizes}", flush=True)
+    with Server(use_float32=True, host_threads=2, hbm_budget=budget, max_in_flight=3) as srv:
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(120.0)
+        infos = [srv.info(t) for t in tickets]
+        stats = srv.stats()
+        assert stats["bytes_peak"] <= stats["bytes_budget"] <= budget
+        assert stats["completed"] == len(items)
+        for q, ((kind, _, _), t) in enumerate(zip(items, tickets)):
+            _check_same(_served(srv, kind, t), reference[q], q)
+        # at most three resident at once: some request was admitted only after
+        # an earlier one had completed
+        admitted = np.array([i.admitted for i in infos])
+        completed = np.array([i.completed for i in infos])
+        assert any(admitted[q] >= completed[:q].min() for q in range(3, len(infos)))
+        for q in range(len(infos)):
+            live = np.sum((admitted <= admitted[q]) & (completed > admitted[q]))
+            assert live <= 3, q
+
+
+def test_a_request_larger_than_the_budget_is_rejected_and_others_proceed():
+    from reformulation_amd import praq, workloads
+    from reformulation_amd.serve import Server
+    big = workloads.praq_er(5, 20000, 80000)
+    small = workloads.praq_er(6, 500, 1500)
+    params = praq.Parameters.with_auto_step(50_000, 1e-6)
+    with Server(use_float32=False, host_threads=1, hbm_budget=4 << 20) as srv:
+        t_big = srv.submit_praq(big.problem(), params)
+        t_small = srv.submit_praq(small.problem(), params)
+        srv.wait_idle(60.0)
+        assert srv.info(t_big).status == -2
+        assert srv.info(t_small).status == 0
+        with pytest.raises(RuntimeError):
+            srv.praq_result(t_big)
+        assert srv.stats()["rejected"] == 1
+
+
+def test_open_loop_driver_and_borrowed_inputs():
+    """Poisson arrivals through open_loop with borrowed inputs: every request
+    completes, latencies are positive, results match the batched mode."""
+    from reformulation_amd import serve
+    members = _members()
+    b, index = _batch(members, True)
+    b.solve("batched")
+    reference = _results(b, index)
+    b.close()
+    items = _problems(members)
+    arrivals = serve.poisson_arrivals(400.0, 0.1, seed=3)
+    arrivals = arrivals[: 3 * len(items)]
+    seen = {}
+    with serve.Server(use_float32=True, host_threads=2, borrow_inputs=True) as srv:
+        def submit(q):
+            return _submit(srv, items[q % len(items)])
+
+        def on_done(q, t):
+            seen[q] = _served(srv, items[q % len(items)][0], t)
+
+        arrive, done, status = serve.open_loop(srv, arrivals, submit, 120.0, on_done)
+    assert np.all(status == 0)
+    assert np.all(done > arrive)
+    s = serve.summarize(arrive, done)
+    assert s["requests"] == len(arrivals) and s["p99_ms"] >= s["p50_ms"] > 0
+    for q, res in seen.items():
+        _check_same(res, reference[q % len(items)], q)
+    assert len(seen) == len(arrivals)
