@@ -8,7 +8,13 @@ is one solve of the whole batch from inputs resident in HBM by the batched
 mode (block tables over all instances, per-instance restart state on the
 device, one check interval per hipGraph replay); `value` is solves/s over
 the timed steps.  Beside it: the plain one-at-a-time GPU solver on the same
-resident instances (solves/s); the roofline of the dominant step kernel --
+resident instances (solves/s); serving -- open-loop Poisson arrivals at
+`--rates` requests/s for `--serve-seconds` each, request k an instance drawn
+from the same mixture with its tolerance drawn from {1e-4, 1e-6}, through
+the serving loop (HBM-budgeted admission, continuous batching, backfill),
+p50/p99 latency (completion - arrival) and solves/s, beside the same
+arrivals through a one-at-a-time queue whose service times are the plain
+solver's measured per-instance times with inputs resident; the roofline of the dominant step kernel --
 its launches' summed algorithmic bytes (PRAQ node n(4+7b) + 2m(4+b), edge
 m(8+7b); PRIA x 7b ND, y/z 8b ND per step, b = 4 for FP32) over their summed
 durations, each launch bracketed by HIP events on its stream in one more
@@ -16,6 +22,7 @@ solve of the same batch after the timed steps; and the oracle (FP64 C restatemen
 of the same mixture on the host's cores.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--count 1024]
+    python bench.py --serve-only --rates 50,200,1000 --serve-seconds 120
 """
 from __future__ import annotations
 
@@ -77,6 +84,60 @@ def cpu_baseline(members, tolerance: float, budget_s: float):
                 sample=f"first {solved} instances of the mixture, FP64 oracle, {elapsed:.1f} s")
 
 
+def plain_service_times(index, tolerance_batch, count, solved=()):
+    """Per mixture member: the plain solver's seconds at each tolerance, from
+    sequential solves of resident batches (those of `solved` have just run
+    one)."""
+    out = {}
+    for tol, b in tolerance_batch.items():
+        if tol not in solved:
+            b.solve("sequential")
+        sp, sr = b.instance_seconds()
+        out[tol] = np.array([sp[q] if kind == "praq" else sr[q] for kind, q, _ in index])
+        log(f"plain service times at {tol:g}: mean {out[tol].mean() * 1e3:.1f} ms, "
+            f"total {out[tol].sum():.1f} s over {count}")
+    return out
+
+
+def serving(members, rate: float, seconds: float, service, use_f32: bool, threads: int,
+            seed: int = 2024):
+    """One open-loop run of the serving loop and the plain queue on the same
+    arrivals."""
+    from reformulation_amd import praq, pria, serve
+    rng = np.random.default_rng(seed)
+    arrivals = serve.poisson_arrivals(rate, seconds, seed)
+    picks = rng.integers(0, len(members), len(arrivals))
+    tols = rng.choice([1e-4, 1e-6], len(arrivals))
+    problems = [(kind, inst.problem()) for kind, inst in members]
+    pp = {t: praq.Parameters.with_auto_step(200_000, t) for t in (1e-4, 1e-6)}
+    rp = {t: pria.Parameters.with_auto_step(200_000, t) for t in (1e-4, 1e-6)}
+    srv = serve.Server(use_float32=use_f32, host_threads=threads, borrow_inputs=True)
+
+    def submit(q):
+        kind, p = problems[picks[q]]
+        t = tols[q]
+        return srv.submit_praq(p, pp[t]) if kind == "praq" else srv.submit_pria(p, rp[t])
+
+    t0 = time.perf_counter()
+    arrive, done, status = serve.open_loop(srv, arrivals, submit)
+    wall = time.perf_counter() - t0
+    stats = srv.stats()
+    srv.close()
+    served = serve.summarize(arrive, done)
+    served["converged"] = int(np.sum(status == 0))
+    plain_lat = serve.fifo_latencies(arrivals, np.array(
+        [service[t][m] for m, t in zip(picks, tols)]))
+    plain = serve.summarize(arrivals, arrivals + plain_lat)
+    out = dict(rate=rate, seconds=seconds, requests=len(arrivals), wall_s=wall,
+               served=served, plain=plain,
+               speedup=(served["solves_per_s"] / plain["solves_per_s"])
+               if plain["solves_per_s"] else None,
+               rounds=stats["rounds"], bytes_peak=stats["bytes_peak"],
+               hbm_budget=stats["bytes_budget"])
+    log(f"serving at {rate:g}/s for {seconds:g} s: {json.dumps(out)}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +149,12 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--rates", default="200", help="serving arrival rates, comma-separated")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-threads", type=int, default=12)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving runs (one JSON line per rate)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps(dict(error="this build measures one GPU; --gpus must be 1 "
@@ -107,6 +174,24 @@ def main() -> int:
     t0 = time.perf_counter()
     batch.upload()
     log(f"uploaded in {time.perf_counter() - t0:.1f} s, {batch.device_bytes / 1e9:.2f} GB on device")
+    rates = [float(r) for r in args.rates.split(",") if r] if not args.no_serve else []
+
+    def service_times(solved=()):
+        other = 1e-4 if args.tolerance != 1e-4 else 1e-6
+        b2, _ = build_batch(members, other, use_f32, 200_000)
+        b2.upload()
+        out = plain_service_times(index, {args.tolerance: batch, other: b2}, args.count, solved)
+        b2.close()
+        return out
+
+    if args.serve_only:
+        service = service_times()
+        for rate in rates:
+            out = serving(members, rate, args.serve_seconds, service, use_f32,
+                          args.serve_threads)
+            print(json.dumps(dict(workload="serving", count=args.count,
+                                  dtype="f32" if use_f32 else "f64", **out)), flush=True)
+        return 0
 
     for w in range(args.warmup):
         dt = batch.solve("batched")
@@ -155,6 +240,13 @@ def main() -> int:
         plain = args.count / dt
         log(f"plain one-at-a-time: {dt:.2f} s -> {plain:.1f} solves/s")
 
+    serve_runs = []
+    if rates:
+        service = service_times(() if args.no_plain else (args.tolerance,))
+        for rate in rates:
+            serve_runs.append(serving(members, rate, args.serve_seconds, service, use_f32,
+                                      args.serve_threads))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(members, args.tolerance, args.cpu_seconds)
@@ -173,8 +265,13 @@ def main() -> int:
                              + ", batched mode (block tables, hipGraph rounds)",
                     count=args.count, tolerance=args.tolerance, parallelism="none"),
         plain_solves_per_s=plain, speedup_vs_plain=(value / plain) if plain else None,
-        latency_ms=dict(p50=ms, p99=ms, note="batched: every instance of a step completes "
-                                             "with its launch"),
+        latency_ms=(dict(p50=serve_runs[0]["served"]["p50_ms"],
+                         p99=serve_runs[0]["served"]["p99_ms"],
+                         plain_p50=serve_runs[0]["plain"]["p50_ms"],
+                         plain_p99=serve_runs[0]["plain"]["p99_ms"],
+                         note=f"serving at {serve_runs[0]['rate']:g} requests/s")
+                    if serve_runs else None),
+        serving=serve_runs or None,
         converged=status.count(0), max_gap=float(gaps.max()),
         roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=PEAK_HBM_GBS,
                       unit="GB/s", frac=achieved / PEAK_HBM_GBS, traffic=None,

@@ -71,6 +71,11 @@ const char* reform_last_error(void);
  * solve). */
 int reform_batch_kernel_ms(const reform_batch* batch, double* praq_ms, double* pria_ms);
 int64_t reform_batch_device_bytes(const reform_batch* batch);
+/* Host seconds the last sequential solve spent on instance `index` of the
+ * family (from its first launch to its synchronized end, the FP64 handover
+ * included), -1 if it has none: the plain solver's service time per
+ * instance, inputs already resident. */
+double reform_batch_instance_seconds(const reform_batch* batch, int family, int64_t index);
 
 /* Launch timing of the batched mode's two step kernels per family (PRAQ:
  * node, edge; PRIA: x, y/z).  With timing on, the rounds are launched

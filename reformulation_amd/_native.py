@@ -84,6 +84,7 @@ EXPORTS = (
     "reform_batch_set_shape", "reform_batch_count", "reform_batch_kernel_ms",
     "reform_batch_device_bytes", "reform_batch_praq_result", "reform_batch_pria_result",
     "reform_last_error", "reform_batch_set_timing", "reform_batch_kernel_stats",
+    "reform_batch_instance_seconds",
     "reform_server_default_config", "reform_server_create", "reform_server_destroy",
     "reform_server_submit_praq", "reform_server_submit_pria", "reform_server_poll",
     "reform_server_wait_idle", "reform_server_info", "reform_server_praq_result",

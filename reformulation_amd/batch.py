@@ -66,6 +66,8 @@ def _declare(L):
     L.reform_batch_kernel_ms.restype = ctypes.c_int
     L.reform_batch_kernel_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]
+    L.reform_batch_instance_seconds.restype = ctypes.c_double
+    L.reform_batch_instance_seconds.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
     L.reform_batch_device_bytes.restype = ctypes.c_int64
     L.reform_batch_device_bytes.argtypes = [ctypes.c_void_p]
     L.reform_batch_praq_result.restype = ctypes.c_int
@@ -197,6 +199,15 @@ class Batch:
                     out.append(dict(name=k.name.decode(), launches=int(k.launches),
                                     ms=float(k.ms), bytes=float(k.bytes)))
         return out
+
+    def instance_seconds(self) -> tuple[np.ndarray, np.ndarray]:
+        """(PRAQ, PRIA) per-instance host seconds of the last sequential
+        solve, in add order."""
+        np_, nr = self.counts()
+        return (np.array([self._L.reform_batch_instance_seconds(self._h, 0, q)
+                          for q in range(np_)]),
+                np.array([self._L.reform_batch_instance_seconds(self._h, 1, q)
+                          for q in range(nr)]))
 
     @property
     def device_bytes(self) -> int:

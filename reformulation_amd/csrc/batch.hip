@@ -129,6 +129,7 @@ struct BatchImpl {
     LogicState* pria_states_dev = nullptr;
     std::vector<LogicState> praq_states, pria_states;
     std::vector<char> praq_in64, pria_in64;  // final point lives in the FP64 image
+    std::vector<double> praq_seconds, pria_seconds;  // per instance, last sequential solve
     double* work = nullptr;
     int pria_max_T = 2;
     int device = 0;
@@ -562,12 +563,20 @@ int BatchImpl::solve_workgroups() {
 // handover to FP64 if it stalls.
 int BatchImpl::solve_sequential() {
     const bool f32 = use_float32 != 0;
+    using clock = std::chrono::steady_clock;
+    praq_seconds.assign(praq.size(), -1.0);
+    pria_seconds.assign(pria.size(), -1.0);
+    auto since = [](clock::time_point t) {
+        return std::chrono::duration<double>(clock::now() - t).count();
+    };
     for (size_t q = 0; q < praq.size(); ++q) {
         const int64_t s = praq_slot[q];
         const int tier = praq[q].tier;
+        const clock::time_point t0 = clock::now();
         if (!f32) {
             if (praq_plain<double>(praq64.host[s], praq64.sub, tier, false)) return kErrorDevice;
             praq_in64[q] = 1;
+            praq_seconds[q] = since(t0);
             continue;
         }
         if (praq_plain<float>(praq32.host[s], praq32.sub, tier, false)) return kErrorDevice;
@@ -581,13 +590,16 @@ int BatchImpl::solve_sequential() {
             if (praq_plain<double>(praq64.host[s], praq64.sub, tier, true)) return kErrorDevice;
             praq_in64[q] = 1;
         }
+        praq_seconds[q] = since(t0);
     }
     for (size_t q = 0; q < pria.size(); ++q) {
         const int64_t s = pria_slot[q];
         const int tier = pria[q].tier, T = pria[q].host.T;
+        const clock::time_point t0 = clock::now();
         if (!f32) {
             if (pria_plain<double>(pria64.host[s], pria64.sub, tier, false, T)) return kErrorDevice;
             pria_in64[q] = 1;
+            pria_seconds[q] = since(t0);
             continue;
         }
         if (pria_plain<float>(pria32.host[s], pria32.sub, tier, false, T)) return kErrorDevice;
@@ -601,6 +613,7 @@ int BatchImpl::solve_sequential() {
             if (pria_plain<double>(pria64.host[s], pria64.sub, tier, true, T)) return kErrorDevice;
             pria_in64[q] = 1;
         }
+        pria_seconds[q] = since(t0);
     }
     return 0;
 }
@@ -804,6 +817,14 @@ extern "C" int reform_batch_kernel_stats(const reform_batch* batch, int family,
         out[k].bytes = bytes[k];
     }
     return 0;
+}
+
+extern "C" double reform_batch_instance_seconds(const reform_batch* batch, int family,
+                                               int64_t index) {
+    if (!batch || family < 0 || family > 1) return -1.0;
+    const std::vector<double>& v =
+        family == 0 ? batch->impl->praq_seconds : batch->impl->pria_seconds;
+    return index >= 0 && index < int64_t(v.size()) ? v[index] : -1.0;
 }
 
 extern "C" int64_t reform_batch_device_bytes(const reform_batch* batch) {

@@ -41,11 +41,11 @@ constexpr int kErrorTicket = -4;
 constexpr int kErrorTimeout = -5;
 constexpr int kPending = -1;
 
-__global__ void serve_done_flags(const PraqJob<double>* praq, int np, const PriaJob<double>* pria,
-                                 int nr, int* flags) {
+// flags[k] = member k of a family's tables has finished
+template <class Job>
+__global__ void serve_done_flags(const Job* jobs, int count, int* flags) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < np) flags[k] = praq[k].state->action == kActDoneX;
-    else if (k < np + nr) flags[k] = pria[k - np].state->action == kActDoneX;
+    if (k < count) flags[k] = jobs[k].state->action == kActDoneX;
 }
 
 // First-fit allocator over one device allocation.
@@ -138,6 +138,7 @@ struct Request {
     PraqJob<float> pj32{};
     PriaJob<double> rj64{};
     PriaJob<float> rj32{};
+    LogicState init{};  // the first state, copied from here at admission
     // outcome
     LogicState final{};
     bool in64 = true;
@@ -171,15 +172,22 @@ struct reform_server {
     Pool pool;
     LogicState* states = nullptr;
     std::vector<int> free_slots;
-    hipStream_t streams[2] = {nullptr, nullptr};
-    std::vector<Request*> members[2];
+    hipStream_t streams[2] = {nullptr, nullptr};  // the families' rounds
+    hipStream_t upload = nullptr;                  // admissions' copies
+    std::vector<Request*> members[2];              // in table order
+    std::vector<Request*> joining[2];              // admitted, join at the next round
     bool changed[2] = {false, false};
+    // per family: a round in flight, its completion event and its done flags
+    bool in_flight[2] = {false, false};
+    hipEvent_t round_done[2] = {nullptr, nullptr};
+    int* flags_dev[2] = {nullptr, nullptr};
+    int* flags_host[2] = {nullptr, nullptr};       // pinned
+    int flags_cap[2] = {0, 0};
+    int stable[2] = {0, 0};                        // rounds since the tables changed
     WidePraq praq_engine;
     WidePria pria_engine;
     void* jobs_dev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [family][precision]
     size_t jobs_cap[2] = {0, 0};
-    int* flags = nullptr;
-    int flags_cap = 0;
     std::atomic<size_t> bytes_in_use{0}, bytes_peak{0};
 
     ~reform_server() {
@@ -192,10 +200,15 @@ struct reform_server {
         for (auto& f : jobs_dev)
             for (void* p : f)
                 if (p) (void)hipFree(p);
-        if (flags) (void)hipFree(flags);
+        for (int f = 0; f < 2; ++f) {
+            if (flags_dev[f]) (void)hipFree(flags_dev[f]);
+            if (flags_host[f]) (void)hipHostFree(flags_host[f]);
+            if (round_done[f]) (void)hipEventDestroy(round_done[f]);
+        }
         if (states) (void)hipFree(states);
         for (auto& s : streams)
             if (s) (void)hipStreamDestroy(s);
+        if (upload) (void)hipStreamDestroy(upload);
     }
 
     void build_loop();
@@ -203,7 +216,8 @@ struct reform_server {
     int build(Request& r);
     int admit(Request& r);
     int rebuild(int family);
-    int retire_finished();
+    int launch(int family);
+    int retire_finished(int family);
     void retire(Request& r, int status);
 };
 
@@ -285,7 +299,7 @@ void reform_server::build_loop() {
 int reform_server::admit(Request& r) {
     const bool f32 = cfg.use_float32 != 0;
     char* base = pool.base() + r.off;
-    hipStream_t s = streams[r.family];
+    hipStream_t s = upload;
     LogicState* state = states + r.slot;
     size_t off64 = 0, off32 = 0;
     if (r.family == 0) {
@@ -337,13 +351,8 @@ int reform_server::admit(Request& r) {
     init.last_action = kActInit;
     init.steps32 = init.steps64 = 0;
     init.switch_iteration = -1;
-    REFORM_HIP_CHECK(hipMemcpyAsync(state, &init, sizeof(init), hipMemcpyHostToDevice, s));
-    // the host image is no longer needed once the copies are done
-    REFORM_HIP_CHECK(hipStreamSynchronize(s));
-    r.img64.clear();
-    r.img64.shrink_to_fit();
-    r.img32.clear();
-    r.img32.shrink_to_fit();
+    r.init = init;
+    REFORM_HIP_CHECK(hipMemcpyAsync(state, &r.init, sizeof(init), hipMemcpyHostToDevice, s));
     return 0;
 }
 
@@ -427,57 +436,79 @@ void reform_server::retire(Request& r, int status) {
     cv_done.notify_all();
 }
 
-int reform_server::retire_finished() {
-    const int np = int(members[0].size()), nr = int(members[1].size());
-    const int total = np + nr;
-    if (!total) return 0;
-    if (total > flags_cap) {
-        if (flags) (void)hipFree(flags);
-        flags_cap = std::max(total, 2 * flags_cap);
-        REFORM_HIP_CHECK(hipMalloc(&flags, sizeof(int) * flags_cap));
+// After the family's round: members whose state reads finished (the done
+// flags copied back with the round) leave the tables; their answers and
+// states are copied out and their memory and slots freed.
+int reform_server::retire_finished(int f) {
+    auto& mem = members[f];
+    const int* done = flags_host[f];
+    std::vector<Request*> keep, gone;
+    for (size_t q = 0; q < mem.size(); ++q) (done[q] ? gone : keep).push_back(mem[q]);
+    if (gone.empty()) return 0;
+    hipStream_t s = streams[f];
+    for (Request* rp : gone) {
+        Request& r = *rp;
+        REFORM_HIP_CHECK(hipMemcpyAsync(&r.final, states + r.slot, sizeof(LogicState),
+                                        hipMemcpyDeviceToHost, s));
     }
-    hipLaunchKernelGGL(serve_done_flags, dim3((total + 255) / 256), dim3(256), 0, streams[0],
-                       static_cast<const PraqJob<double>*>(jobs_dev[0][1]), np,
-                       static_cast<const PriaJob<double>*>(jobs_dev[1][1]), nr, flags);
-    std::vector<int> done(total);
-    REFORM_HIP_CHECK(hipMemcpyAsync(done.data(), flags, sizeof(int) * total, hipMemcpyDeviceToHost,
-                                    streams[0]));
-    REFORM_HIP_CHECK(hipStreamSynchronize(streams[0]));
-    for (int f = 0; f < 2; ++f) {
-        auto& mem = members[f];
-        std::vector<Request*> keep;
-        const int shift = f == 0 ? 0 : np;
-        for (size_t q = 0; q < mem.size(); ++q) {
-            if (!done[shift + q]) {
-                keep.push_back(mem[q]);
-                continue;
-            }
-            Request& r = *mem[q];
-            hipStream_t s = streams[f];
-            REFORM_HIP_CHECK(hipMemcpyAsync(&r.final, states + r.slot, sizeof(LogicState),
-                                            hipMemcpyDeviceToHost, s));
-            REFORM_HIP_CHECK(hipStreamSynchronize(s));
-            r.in64 = r.final.in64 != 0;
-            int st = 0;
-            if (f == 0) {
-                const size_t n = r.ph.n, m2 = 2 * size_t(r.ph.m);
-                st = r.in64 ? (copy_out(r.d1, r.pj64.a.x, n, s) || copy_out(r.d2, r.pj64.a.y, m2, s))
-                            : (copy_out(r.f1, r.pj32.a.x, n, s) || copy_out(r.f2, r.pj32.a.y, m2, s));
-            } else {
-                const size_t nd = r.rh.ND;
-                st = r.in64 ? (copy_out(r.d1, r.rj64.a.x, nd, s) || copy_out(r.d2, r.rj64.a.y, nd, s) ||
-                               copy_out(r.d3, r.rj64.a.z, nd, s))
-                            : (copy_out(r.f1, r.rj32.a.x, nd, s) || copy_out(r.f2, r.rj32.a.y, nd, s) ||
-                               copy_out(r.f3, r.rj32.a.z, nd, s));
-            }
-            if (st || hipStreamSynchronize(s) != hipSuccess) return kErrorDevice;
-            retire(r, r.final.status);
+    REFORM_HIP_CHECK(hipStreamSynchronize(s));
+    for (Request* rp : gone) {
+        Request& r = *rp;
+        r.in64 = r.final.in64 != 0;
+        int st = 0;
+        if (f == 0) {
+            const size_t n = r.ph.n, m2 = 2 * size_t(r.ph.m);
+            st = r.in64 ? (copy_out(r.d1, r.pj64.a.x, n, s) || copy_out(r.d2, r.pj64.a.y, m2, s))
+                        : (copy_out(r.f1, r.pj32.a.x, n, s) || copy_out(r.f2, r.pj32.a.y, m2, s));
+        } else {
+            const size_t nd = r.rh.ND;
+            st = r.in64 ? (copy_out(r.d1, r.rj64.a.x, nd, s) || copy_out(r.d2, r.rj64.a.y, nd, s) ||
+                           copy_out(r.d3, r.rj64.a.z, nd, s))
+                        : (copy_out(r.f1, r.rj32.a.x, nd, s) || copy_out(r.f2, r.rj32.a.y, nd, s) ||
+                           copy_out(r.f3, r.rj32.a.z, nd, s));
         }
-        if (keep.size() != mem.size()) {
-            mem.swap(keep);
-            changed[f] = true;
-        }
+        if (st) return kErrorDevice;
     }
+    REFORM_HIP_CHECK(hipStreamSynchronize(s));
+    for (Request* rp : gone) retire(*rp, rp->final.status);
+    mem.swap(keep);
+    changed[f] = true;
+    return 0;
+}
+
+// One round of the family on its stream, then its done flags into pinned
+// memory and the completion event.  The round is replayed from a graph once
+// the tables have held for two rounds, else launched directly (a capture
+// costs more than the launches of one round).
+int reform_server::launch(int f) {
+    WideControl& w = f == 0 ? static_cast<WideControl&>(praq_engine) : pria_engine;
+    const int count = int(members[f].size());
+    hipStream_t s = streams[f];
+    if (count > flags_cap[f]) {
+        if (flags_dev[f]) REFORM_HIP_CHECK(hipFree(flags_dev[f]));
+        if (flags_host[f]) REFORM_HIP_CHECK(hipHostFree(flags_host[f]));
+        flags_dev[f] = flags_host[f] = nullptr;
+        flags_cap[f] = std::max(count, 2 * flags_cap[f]);
+        REFORM_HIP_CHECK(hipMalloc(&flags_dev[f], sizeof(int) * flags_cap[f]));
+        REFORM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&flags_host[f]),
+                                       sizeof(int) * flags_cap[f], hipHostMallocDefault));
+    }
+    w.use_graph = stable[f] >= 2;
+    if (w.launch_round(s)) return kErrorDevice;
+    ++stable[f];
+    const dim3 grid((count + 255) / 256), blk(256);
+    if (f == 0)
+        hipLaunchKernelGGL(serve_done_flags<PraqJob<double>>, grid, blk, 0, s,
+                           static_cast<const PraqJob<double>*>(jobs_dev[0][1]), count, flags_dev[0]);
+    else
+        hipLaunchKernelGGL(serve_done_flags<PriaJob<double>>, grid, blk, 0, s,
+                           static_cast<const PriaJob<double>*>(jobs_dev[1][1]), count, flags_dev[1]);
+    REFORM_HIP_CHECK(hipGetLastError());
+    REFORM_HIP_CHECK(hipMemcpyAsync(flags_host[f], flags_dev[f], sizeof(int) * count,
+                                    hipMemcpyDeviceToHost, s));
+    REFORM_HIP_CHECK(hipEventRecord(round_done[f], s));
+    in_flight[f] = true;
+    n_rounds++;
     return 0;
 }
 
@@ -492,16 +523,22 @@ void reform_server::serve_loop() {
     };
     if (hipSetDevice(device) != hipSuccess) return fail(kErrorDevice);
     for (int f = 0; f < 2; ++f) {
-        if (streams[f] == nullptr && hipStreamCreateWithFlags(&streams[f], hipStreamNonBlocking) !=
-                                         hipSuccess)
+        if (hipStreamCreateWithFlags(&streams[f], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&round_done[f], hipEventDisableTiming) != hipSuccess)
             return fail(kErrorDevice);
     }
+    if (hipStreamCreateWithFlags(&upload, hipStreamNonBlocking) != hipSuccess)
+        return fail(kErrorDevice);
     while (!stop) {
-        // admit what fits
+        bool progress = false;
+        // admit what fits, in submission order; the copies run on their own
+        // stream beside the rounds in flight
         std::vector<Request*> admitted;
         {
             std::unique_lock<std::mutex> lock(mu);
-            if (members[0].empty() && members[1].empty() && ready.empty())
+            const bool idle = !in_flight[0] && !in_flight[1] && members[0].empty() &&
+                              members[1].empty() && joining[0].empty() && joining[1].empty();
+            if (idle && ready.empty())
                 cv_ready.wait_for(lock, std::chrono::milliseconds(5),
                                   [&] { return stop || !ready.empty(); });
             if (stop) break;
@@ -518,33 +555,53 @@ void reform_server::serve_loop() {
         }
         bytes_in_use = pool.used();
         bytes_peak = pool.peak();
-        if (!admitted.empty()) cv_build.notify_all();
-        for (Request* r : admitted) {
-            if (admit(*r)) return fail(kErrorDevice);
+        if (!admitted.empty()) {
+            cv_build.notify_all();
+            for (Request* r : admitted)
+                if (admit(*r)) return fail(kErrorDevice);
+            if (hipStreamSynchronize(upload) != hipSuccess) return fail(kErrorDevice);
+            const double now = seconds_since(t0);
             {
                 std::lock_guard<std::mutex> lock(mu);
-                r->t_admit = seconds_since(t0);
-                ++n_admitted;
+                n_admitted += int64_t(admitted.size());
+                for (Request* r : admitted) r->t_admit = now;
             }
-            n_in_flight++;
-            members[r->family].push_back(r);
-            changed[r->family] = true;
+            for (Request* r : admitted) {
+                r->img64 = std::vector<char>();  // the device holds the images now
+                r->img32 = std::vector<char>();
+                n_in_flight++;
+                joining[r->family].push_back(r);
+            }
+            progress = true;
         }
-        if (members[0].empty() && members[1].empty()) continue;
-        // one round of each family (tables first: a rebuild may reallocate)
-        for (int f = 0; f < 2; ++f)
-            if (changed[f] && rebuild(f)) return fail(kErrorDevice);
+        // each family on its own: retire after its round, then the next round
         for (int f = 0; f < 2; ++f) {
+            if (in_flight[f]) {
+                const hipError_t q = hipEventQuery(round_done[f]);
+                if (q == hipErrorNotReady) continue;
+                if (q != hipSuccess) return fail(kErrorDevice);
+                in_flight[f] = false;
+                if (retire_finished(f)) return fail(kErrorDevice);
+                progress = true;
+            }
+            if (!joining[f].empty()) {
+                members[f].insert(members[f].end(), joining[f].begin(), joining[f].end());
+                joining[f].clear();
+                changed[f] = true;
+            }
             if (members[f].empty()) continue;
-            WideControl& w = f == 0 ? static_cast<WideControl&>(praq_engine) : pria_engine;
-            if (w.launch_round(streams[f])) return fail(kErrorDevice);
+            if (changed[f]) {
+                if (rebuild(f)) return fail(kErrorDevice);
+                stable[f] = 0;
+            }
+            if (launch(f)) return fail(kErrorDevice);
+            progress = true;
         }
-        n_rounds++;
-        if (hipStreamSynchronize(streams[0]) != hipSuccess ||
-            hipStreamSynchronize(streams[1]) != hipSuccess)
-            return fail(kErrorDevice);
-        if (retire_finished()) return fail(kErrorDevice);
+        if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
+    // leave no round running on the images the pool is about to free
+    for (int f = 0; f < 2; ++f)
+        if (streams[f]) (void)hipStreamSynchronize(streams[f]);
 }
 
 // ------------------------------------------------------------------ C ABI --
