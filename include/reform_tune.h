@@ -1,0 +1,47 @@
+/* Shape tuning (no counterpart in the reference, whose tuners tune restart
+ * parameters through praq.h, qubo/harness/praq_cuda_library.py:90-139).
+ *
+ * A process-wide table maps keys to integer shape choices; the autotuner
+ * (reformulation_amd/autotune.py) times candidates with
+ * reform_time_praq_steps on the GPU and writes a table, which the Python
+ * package loads at import.  Unset keys keep the built-in rules.
+ *
+ * Keys:
+ *   "praq.lanes.<b>", b = 0..7: lanes per node (1, 2, ..., 64) for PRAQ
+ *   instances whose mean degree 2m/n lies in [2^b, 2^(b+1)) (b = 0 also
+ *   below 1, b = 7 also above 256).  Lanes set the CSR gather's lane groups
+ *   in every tier, and hubs (degree above 64 lanes) take the chunked gather
+ *   in the device tier.
+ */
+#ifndef REFORMULATION_REFORM_TUNE_H
+#define REFORMULATION_REFORM_TUNE_H
+
+#include <stdint.h>
+
+#include "praq_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 0, or -1 for an unknown key or a value out of range. */
+int reform_tune_set(const char* key, int64_t value);
+/* The value, 0 if unset, -1 for an unknown key. */
+int64_t reform_tune_get(const char* key);
+void reform_tune_clear(void);
+
+/* Lanes per node the library picks for a PRAQ instance of this shape. */
+int reform_praq_lanes_for(int64_t n, int64_t m);
+
+/* Mean milliseconds of one device-tier PDHG step of `instance` (hub chunks,
+ * node, hub and edge kernels; a reflected Halpern step) with `lanes` lanes
+ * per node (0: the library's choice), FP32 or FP64 iterates, measured with
+ * HIP events over `steps` steps after `warmup` untimed ones.  Returns 0, -1
+ * argument, -2 instance, -3 device. */
+int reform_time_praq_steps(const PraqInstance* instance, int use_float32, int lanes, int warmup,
+                           int steps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

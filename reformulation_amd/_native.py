@@ -85,6 +85,8 @@ EXPORTS = (
     "reform_batch_device_bytes", "reform_batch_praq_result", "reform_batch_pria_result",
     "reform_last_error", "reform_batch_set_timing", "reform_batch_kernel_stats",
     "reform_batch_instance_seconds",
+    "reform_tune_set", "reform_tune_get", "reform_tune_clear", "reform_praq_lanes_for",
+    "reform_time_praq_steps",
     "reform_server_default_config", "reform_server_create", "reform_server_destroy",
     "reform_server_submit_praq", "reform_server_submit_pria", "reform_server_poll",
     "reform_server_wait_idle", "reform_server_info", "reform_server_praq_result",
@@ -146,6 +148,8 @@ def lib() -> ctypes.CDLL:
                                     ctypes.POINTER(PriaOptions), _int, _dp, _dp, _dp, _dp, _dp,
                                     _dp, ctypes.POINTER(PriaInfo)]
     _lib = L
+    from . import autotune        # the measured shape table, when one is present
+    autotune.load(L=L)
     return L
 
 

@@ -241,11 +241,11 @@ int BatchImpl::upload() {
     std::vector<PraqLayout> pl64(praq.size()), pl32(praq.size());
     std::vector<PriaLayout> rl64(pria.size()), rl32(pria.size());
     for (size_t q = 0; q < praq.size(); ++q) {
-        pl64[q] = praq_layout(praq[q].host.n, praq[q].host.m, sizeof(double));
+        pl64[q] = praq_layout(praq[q].host, sizeof(double));
         praq[q].off64 = at;
         at += align256(pl64[q].total_bytes);
         if (f32) {
-            pl32[q] = praq_layout(praq[q].host.n, praq[q].host.m, sizeof(float));
+            pl32[q] = praq_layout(praq[q].host, sizeof(float));
             praq[q].off32 = at;
             at += align256(pl32[q].total_bytes);
         }
@@ -328,7 +328,7 @@ int BatchImpl::upload() {
                 hipSuccess)
                 return kErrorDevice;
             std::memset(&job, 0, sizeof(job));
-            job.a = praq_bind<Real>(L, base + off, e.host.n, e.host.m);
+            job.a = praq_bind<Real>(L, base + off, e.host);
             job.p = e.lp;
             job.scale = e.host.scale;
             job.lanes = e.host.lanes;

@@ -162,6 +162,34 @@ __global__ void praq_node_kernel(PraqArrays<Real> a, PraqStep<Real> st, int lane
     praq_node(a, st, gid / lanes, gid % lanes, lanes);
 }
 
+// hub chunks, one wavefront each: FP64 sums of src into chunk_sum
+template <class Real>
+__global__ void praq_chunk_kernel(PraqArrays<Real> a, const Real* src) {
+    const int c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (c < a.nc) praq_chunk(a, src, c, threadIdx.x & (kWave - 1));
+}
+
+// hubs, one thread each: the chunk sums in order, then the node update
+template <class Real>
+__global__ void praq_hub_kernel(PraqArrays<Real> a, PraqStep<Real> st) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < a.nh) praq_node_update(a, st, a.heavy[h], Real(praq_hub_total(a, h)));
+}
+
+// the hubs' terms of the check sums into one more partial row
+template <class Real>
+__global__ void praq_hub_sums_kernel(PraqArrays<Real> a, const Real* bx, double* row) {
+    __shared__ double red[kPraqSums * 16];
+    double acc[kPraqSums];
+#pragma unroll
+    for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
+    for (int h = threadIdx.x; h < a.nh; h += blockDim.x)
+        praq_node_terms(a, a.heavy[h], praq_hub_total(a, h), bx, acc);
+    block_sums<kPraqSums>(acc, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kPraqSums; ++k) row[k] = acc[k];
+}
+
 template <class Real>
 __global__ void praq_edge_kernel(PraqArrays<Real> a, PraqStep<Real> st) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -186,13 +214,14 @@ __global__ void praq_sums_kernel(PraqArrays<Real> a, int lanes, const Real* bx, 
         for (int k = 0; k < kPraqSums; ++k) partial[blockIdx.x * kPraqSums + k] = acc[k];
 }
 
-// one block of kSumBlocks threads adds the partial rows in a fixed order
+// one block adds the partial rows in a fixed order
 __global__ void praq_sums_final(const double* partial, int rows, double* out) {
     __shared__ double red[kPraqSums * 16];
     double acc[kPraqSums];
 #pragma unroll
-    for (int k = 0; k < kPraqSums; ++k)
-        acc[k] = threadIdx.x < rows ? partial[threadIdx.x * kPraqSums + k] : 0.0;
+    for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x)
+        for (int k = 0; k < kPraqSums; ++k) acc[k] += partial[r * kPraqSums + k];
     block_sums<kPraqSums>(acc, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < kPraqSums; ++k) out[k] = acc[k];
@@ -206,15 +235,33 @@ __global__ void copy2_kernel(Real* dx, const Real* sx, int n, Real* dy, const Re
 }
 
 constexpr int kBlock = 256;
-constexpr int kSumBlocks = 256;
+constexpr int kSumBlocks = 1024;   // check-sum blocks (rows), plus one row for the hubs
 
 static int blocks_for(int64_t threads) { return int((threads + kBlock - 1) / kBlock); }
+
+// hub chunk sums of src (nothing without hubs)
+template <class Real>
+static void device_chunks(const PraqArrays<Real>& a, const Real* src, hipStream_t stream) {
+    if (a.nc > 0 && a.heavy_degree < std::numeric_limits<int>::max())
+        hipLaunchKernelGGL(praq_chunk_kernel<Real>,
+                           dim3((a.nc + kBlock / kWave - 1) / (kBlock / kWave)), dim3(kBlock), 0,
+                           stream, a, src);
+}
+
+template <class Real>
+static bool has_hubs(const PraqArrays<Real>& a) {
+    return a.nh > 0 && a.heavy_degree < std::numeric_limits<int>::max();
+}
 
 template <class Real>
 static int device_step(const PraqArrays<Real>& a, const PraqStep<Real>& st, int lanes,
                        hipStream_t stream) {
+    device_chunks(a, st.reflect ? static_cast<const Real*>(a.yc) : a.sy, stream);
     hipLaunchKernelGGL(praq_node_kernel<Real>, dim3(blocks_for(int64_t(a.n) * lanes)),
                        dim3(kBlock), 0, stream, a, st, lanes);
+    if (has_hubs(a))
+        hipLaunchKernelGGL(praq_hub_kernel<Real>, dim3(blocks_for(a.nh)), dim3(kBlock), 0, stream,
+                           a, st);
     if (a.m > 0)
         hipLaunchKernelGGL(praq_edge_kernel<Real>, dim3(blocks_for(a.m)), dim3(kBlock), 0,
                            stream, a, st);
@@ -235,10 +282,16 @@ static int device_copy(Real* dx, const Real* sx, int n, Real* dy, const Real* sy
 template <class Real>
 static int device_sums(const PraqArrays<Real>& a, int lanes, const Real* bx, const Real* by,
                        double* partial, double* dsums, double* host, hipStream_t stream) {
+    device_chunks(a, by, stream);
     hipLaunchKernelGGL(praq_sums_kernel<Real>, dim3(kSumBlocks), dim3(kBlock), 0, stream, a,
                        lanes, bx, by, partial);
-    hipLaunchKernelGGL(praq_sums_final, dim3(1), dim3(kSumBlocks), 0, stream, partial,
-                       kSumBlocks, dsums);
+    int rows = kSumBlocks;
+    if (has_hubs(a)) {
+        hipLaunchKernelGGL(praq_hub_sums_kernel<Real>, dim3(1), dim3(kBlock), 0, stream, a, bx,
+                           partial + size_t(kSumBlocks) * kPraqSums);
+        ++rows;
+    }
+    hipLaunchKernelGGL(praq_sums_final, dim3(1), dim3(1024), 0, stream, partial, rows, dsums);
     REFORM_HIP_CHECK(hipGetLastError());
     REFORM_HIP_CHECK(hipMemcpyAsync(host, dsums, sizeof(double) * kPraqSums,
                                     hipMemcpyDeviceToHost, stream));
@@ -250,11 +303,13 @@ static int device_sums(const PraqArrays<Real>& a, int lanes, const Real* bx, con
 template <class Real>
 int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
                       double time_limit_seconds, hipStream_t stream, bool resume) {
-    const PraqArrays<Real>& a = job.a;
     const LogicParams& p = job.p;
     const int lanes = job.lanes;
+    // hubs by chunks in this tier
+    PraqArrays<Real> a = job.a;
+    a.heavy_degree = praq_heavy_degree(lanes);
     double* partial = work;
-    double* dsums = work + kSumBlocks * kPraqSums;
+    double* dsums = work + (kSumBlocks + 1) * kPraqSums;
     double sums[kPraqSums];
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto t0 = now();
@@ -325,6 +380,40 @@ int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
     return 0;
 }
 
+template <class Real>
+int praq_time_steps(const PraqJob<Real>& job, int warmup, int steps, hipStream_t stream,
+                    double* ms) {
+    PraqArrays<Real> a = job.a;
+    a.heavy_degree = praq_heavy_degree(job.lanes);
+    if (device_copy(a.x0, a.xs, a.n, a.y0, a.ys, 2 * a.m, stream) ||
+        device_copy(a.x, a.xs, a.n, a.y, a.ys, 2 * a.m, stream) ||
+        device_copy(a.sx, a.xs, a.n, a.sy, a.ys, 2 * a.m, stream) ||
+        device_copy(a.x, a.xs, a.n, a.yc, a.ys, 2 * a.m, stream))
+        return kErrorDevice;
+    hipEvent_t ev[2];
+    REFORM_HIP_CHECK(hipEventCreate(&ev[0]));
+    REFORM_HIP_CHECK(hipEventCreate(&ev[1]));
+    int st = 0;
+    for (int j = 0; j < warmup + steps && !st; ++j) {
+        if (j == warmup) st = hipEventRecord(ev[0], stream) == hipSuccess ? 0 : kErrorDevice;
+        if (!st)
+            st = device_step(a, praq_step_scalars<Real>(j, job.p.omega, job.p.eta, true, false),
+                             job.lanes, stream);
+    }
+    float t = 0.0f;
+    if (!st && (hipEventRecord(ev[1], stream) != hipSuccess ||
+                hipEventSynchronize(ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess))
+        st = kErrorDevice;
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (!st) *ms = double(t) / double(steps > 0 ? steps : 1);
+    return st;
+}
+
+template int praq_time_steps<float>(const PraqJob<float>&, int, int, hipStream_t, double*);
+template int praq_time_steps<double>(const PraqJob<double>&, int, int, hipStream_t, double*);
+
 template int praq_device_solve<float>(const PraqJob<float>&, LogicState&, double*, double,
                                       hipStream_t, bool);
 template int praq_device_solve<double>(const PraqJob<double>&, LogicState&, double*, double,
@@ -377,7 +466,7 @@ int praq_convert_jobs(const PraqJob<float>* from, const PraqJob<double>* to, int
     return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
 }
 
-size_t praq_device_work_bytes() { return sizeof(double) * (kSumBlocks + 1) * kPraqSums; }
+size_t praq_device_work_bytes() { return sizeof(double) * (kSumBlocks + 2) * kPraqSums; }
 
 template <class Real>
 int praq_launch_wg(const PraqJob<Real>* jobs, int count, int block, hipStream_t stream) {

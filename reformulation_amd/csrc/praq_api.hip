@@ -3,6 +3,7 @@
 // output and the tier choice.
 #include "praq_hip.h"
 #include "praq_host.hpp"
+#include "reform_tune.h"
 
 #include <algorithm>
 #include <cstdlib>
@@ -55,7 +56,7 @@ struct PraqImage {
     size_t job_off = 0, state_off = 0, work_off = 0;
 
     int init(const PraqHostInstance& s, const LogicParams& lp, hipStream_t stream) {
-        L = praq_layout(s.n, s.m, sizeof(Real));
+        L = praq_layout(s, sizeof(Real));
         job_off = align256(L.total_bytes);
         state_off = align256(job_off + sizeof(PraqJob<Real>));
         work_off = align256(state_off + sizeof(LogicState));
@@ -67,7 +68,7 @@ struct PraqImage {
                                         stream));
         REFORM_HIP_CHECK(hipStreamSynchronize(stream));
         std::memset(&job, 0, sizeof(job));
-        job.a = praq_bind<Real>(L, base, s.n, s.m);
+        job.a = praq_bind<Real>(L, base, s);
         job.p = lp;
         job.scale = s.scale;
         job.lanes = s.lanes;
@@ -272,4 +273,31 @@ extern "C" int praq_solve(const PraqInstance* instance, const PraqParameters* pa
     result->iterations = info.iterations;
     result->status = status;
     return status;
+}
+
+extern "C" int reform_time_praq_steps(const PraqInstance* instance, int use_float32, int lanes,
+                                      int warmup, int steps, double* ms) {
+    if (!instance || !ms || steps <= 0 || warmup < 0 || lanes < 0 || lanes > 64 ||
+        (lanes & (lanes - 1)) != 0)
+        return kErrorArgument;
+    PraqHostInstance s;
+    if (int status = praq_build_host(*instance, 0, nullptr, nullptr, s)) return status;
+    if (lanes > 0) praq_set_lanes(s, lanes);
+    PraqParameters p = praq_default_parameters();
+    p.op_steps = 16;
+    const LogicParams lp = praq_logic_params(p, praq_default_options());
+    hipStream_t stream;
+    REFORM_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    int st;
+    if (use_float32) {
+        PraqImage<float> f;
+        st = f.init(s, lp, stream) ? kErrorDevice
+                                   : praq_time_steps<float>(f.job, warmup, steps, stream, ms);
+    } else {
+        PraqImage<double> d;
+        st = d.init(s, lp, stream) ? kErrorDevice
+                                   : praq_time_steps<double>(d.job, warmup, steps, stream, ms);
+    }
+    (void)hipStreamDestroy(stream);
+    return st;
 }

@@ -10,6 +10,9 @@
 //           slot * m + e, i.e. an index into a slot-major dual array
 //   duals   ys[2m] (projected start), yc[2m], y[2m], sy[2m], y0[2m], slot-major (slot 0 of every edge,
 //           then slot 1), so the edge pass reads both slots coalesced
+//   hubs    heavy[nh] nodes of degree above heavy_degree, hchunk[nh + 1] their
+//           first chunk, chunk_node[nc] the hub of each chunk of kPraqChunk
+//           half-edges, chunk_sum[nc] FP64 chunk sums (device tier only)
 //
 // yc holds the Halpern-combined dual the NEXT step starts from, written by the
 // edge pass as it finishes a step: the node pass's scatter, the one random
@@ -46,7 +49,18 @@ struct PraqArrays {
     Real* y;
     Real* sy;
     Real* y0;
+    // hubs: gathered by one wavefront per chunk in the device tier; nodes of
+    // degree above heavy_degree are skipped by the lane-group passes (every
+    // tier but the device tier binds heavy_degree = INT_MAX)
+    int nh, nc, heavy_degree;
+    const int* heavy;
+    const int* hchunk;
+    const int* chunk_node;
+    double* chunk_sum;
 };
+
+// Half-edges per hub chunk: one wavefront, 16 per lane.
+constexpr int kPraqChunk = 1024;
 
 // The scalars of one step, computed once per step from the restart counter.
 template <class Real>
@@ -84,6 +98,18 @@ __host__ __device__ inline PraqStep<Real> praq_step_scalars(int64_t restart_iter
 // group's lanes stride the CSR list and meet in an xor butterfly, so the sum
 // depends only on the instance and `lanes`.
 template <class Real>
+__device__ __forceinline__ void praq_node_update(const PraqArrays<Real>& a,
+                                                 const PraqStep<Real>& st, int i, Real acc) {
+    Real xc = a.x[i], sxi = a.sx[i];
+    Real x1 = st.reflect ? st.lw * (Real(2) * sxi - xc) + st.rw * a.x0[i] : sxi;
+    Real g = a.diag[i] - acc;
+    Real x2 = clampr(x1 - st.tau * (g / a.deg[i]), Real(0), Real(1));
+    a.x[i] = x1;
+    a.sx[i] = x2;
+    a.refl[i] = Real(2) * x2 - x1;
+}
+
+template <class Real>
 __device__ __forceinline__ void praq_node(const PraqArrays<Real>& a, const PraqStep<Real>& st,
                                           int i, int lane, int lanes) {
     Real acc = Real(0);
@@ -92,17 +118,35 @@ __device__ __forceinline__ void praq_node(const PraqArrays<Real>& a, const PraqS
         begin = a.ptr[i];
         end = a.ptr[i + 1];
     }
+    const bool hub = end - begin > a.heavy_degree;  // done by its chunks instead
+    if (hub) end = begin;
     const Real* src = st.reflect ? a.yc : a.sy;
     for (int k = begin + lane; k < end; k += lanes) acc += src[a.half[k]];
     acc = group_sum(acc, lanes);
-    if (i >= a.n || lane != 0) return;
-    Real xc = a.x[i], sxi = a.sx[i];
-    Real x1 = st.reflect ? st.lw * (Real(2) * sxi - xc) + st.rw * a.x0[i] : sxi;
-    Real g = a.diag[i] - acc;
-    Real x2 = clampr(x1 - st.tau * (g / a.deg[i]), Real(0), Real(1));
-    a.x[i] = x1;
-    a.sx[i] = x2;
-    a.refl[i] = Real(2) * x2 - x1;
+    if (i >= a.n || lane != 0 || hub) return;
+    praq_node_update(a, st, i, acc);
+}
+
+// One hub chunk by one wavefront (all 64 lanes call it): the FP64 sum of
+// src over the chunk's half-edges, lanes striding, DPP reduction.
+template <class Real>
+__device__ __forceinline__ void praq_chunk(const PraqArrays<Real>& a, const Real* src, int c,
+                                           int lane) {
+    const int h = a.chunk_node[c], i = a.heavy[h];
+    const int begin = a.ptr[i] + (c - a.hchunk[h]) * kPraqChunk;
+    const int end = min(a.ptr[i + 1], begin + kPraqChunk);
+    double acc = 0.0;
+    for (int k = begin + lane; k < end; k += kWave) acc += double(src[a.half[k]]);
+    acc = wave_sum(acc);
+    if (lane == 0) a.chunk_sum[c] = acc;
+}
+
+// Hub h's total, its chunks added in order.
+template <class Real>
+__device__ __forceinline__ double praq_hub_total(const PraqArrays<Real>& a, int h) {
+    double acc = 0.0;
+    for (int c = a.hchunk[h]; c < a.hchunk[h + 1]; ++c) acc += a.chunk_sum[c];
+    return acc;
 }
 
 // Edge e's update: projection of y1 + sigma (q - S refl), and the next yc.
@@ -161,6 +205,19 @@ enum PraqSum {
     kPraqSums = 9
 };
 
+// Node i's terms of the check sums given (S^T by)_i.
+template <class Real>
+__device__ __forceinline__ void praq_node_terms(const PraqArrays<Real>& a, int i, double scat,
+                                                const Real* bx, double (&acc)[kPraqSums]) {
+    double xi = a.x[i], sxi = a.sx[i], deg = a.deg[i], di = a.diag[i];
+    double dx = xi - sxi, dr = xi - double(a.x0[i]);
+    acc[kPrimalNorm] += deg * dx * dx;
+    acc[kPrimalDrift] += deg * dr * dr;
+    acc[kUpperNode] += di * double(bx[i]);
+    double rc = di - scat;
+    acc[kLowerNode] += rc < 0.0 ? rc : 0.0;
+}
+
 // Node part, by a lane group as praq_node; bounds at (bx, by).
 template <class Real>
 __device__ __forceinline__ void praq_node_sums(const PraqArrays<Real>& a, int i, int lane,
@@ -172,16 +229,12 @@ __device__ __forceinline__ void praq_node_sums(const PraqArrays<Real>& a, int i,
         begin = a.ptr[i];
         end = a.ptr[i + 1];
     }
+    const bool hub = end - begin > a.heavy_degree;
+    if (hub) end = begin;
     for (int k = begin + lane; k < end; k += lanes) scat += double(by[a.half[k]]);
     scat = group_sum(scat, lanes);
-    if (i >= a.n || lane != 0) return;
-    double xi = a.x[i], sxi = a.sx[i], deg = a.deg[i], di = a.diag[i];
-    double dx = xi - sxi, dr = xi - double(a.x0[i]);
-    acc[kPrimalNorm] += deg * dx * dx;
-    acc[kPrimalDrift] += deg * dr * dr;
-    acc[kUpperNode] += di * double(bx[i]);
-    double rc = di - scat;
-    acc[kLowerNode] += rc < 0.0 ? rc : 0.0;
+    if (i >= a.n || lane != 0 || hub) return;
+    praq_node_terms(a, i, scat, bx, acc);
 }
 
 template <class Real>

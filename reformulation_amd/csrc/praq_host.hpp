@@ -42,15 +42,45 @@ struct PraqHostInstance {
     double scale = 1.0;
     std::vector<int> e0, e1, ptr, half;
     std::vector<double> w, diag, deg, x_start, y_start;  // y_start slot-major, scaled
+    // hubs (degree > heavy_degree) and their chunks, for the device tier
+    int heavy_degree = 0;
+    std::vector<int> heavy, hchunk, chunk_node;
 };
 
-// Lanes per node in the scatter: the smallest power of two leaving each lane
-// at most 8 half-edges at the mean degree, capped at a wavefront.
+// Hubs: nodes whose CSR list is longer than 64 half-edges per lane of the
+// instance's lane groups; a lane group would walk them alone.
+inline int praq_heavy_degree(int lanes) { return 64 * lanes; }
+
+// The tuning table's lanes for a mean degree, 0 if unset (tune.hip).
+int praq_tuned_lanes(double mean_degree);
+
+// Lanes per node in the scatter: the tuning table's choice for the mean
+// degree, else the smallest power of two leaving each lane at most 8
+// half-edges at the mean degree, capped at a wavefront.
 inline int praq_lanes_for(int n, int m) {
     double average = 2.0 * double(m) / double(n > 0 ? n : 1);
+    const int tuned = praq_tuned_lanes(average);
+    if (tuned > 0) return tuned;
     int lanes = 1;
     while (lanes < 64 && double(lanes) * 8.0 < average) lanes *= 2;
     return lanes;
+}
+
+// Lanes per node and the hubs and chunks they imply.
+inline void praq_set_lanes(PraqHostInstance& s, int lanes) {
+    s.lanes = lanes;
+    s.heavy_degree = praq_heavy_degree(lanes);
+    s.heavy.clear();
+    s.chunk_node.clear();
+    s.hchunk.assign(1, 0);
+    for (int i = 0; i < s.n; ++i) {
+        const int len = s.ptr[i + 1] - s.ptr[i];
+        if (len <= s.heavy_degree) continue;
+        const int h = int(s.heavy.size());
+        s.heavy.push_back(i);
+        for (int c = 0; c < (len + kPraqChunk - 1) / kPraqChunk; ++c) s.chunk_node.push_back(h);
+        s.hchunk.push_back(int(s.chunk_node.size()));
+    }
 }
 
 inline double clamp_host(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -109,7 +139,7 @@ inline int praq_build_host(const PraqInstance& in, int dense_degree, const doubl
         double incident = dense_degree ? double(n - 1) : double(s.ptr[i + 1] - s.ptr[i]);
         s.deg[i] = incident > 1.0 ? incident : 1.0;
     }
-    s.lanes = praq_lanes_for(n, m);
+    praq_set_lanes(s, praq_lanes_for(n, m));
 
     s.x_start.resize(n);
     for (int i = 0; i < n; ++i) s.x_start[i] = clamp_host(warm_x ? warm_x[i] : 0.5, 0.0, 1.0);
@@ -134,22 +164,27 @@ inline int praq_build_host(const PraqInstance& in, int dense_degree, const doubl
 // Byte offsets of one instance's arrays in its device image.  The input part
 // (uploaded) comes first, the iteration state after it.
 struct PraqLayout {
-    enum { kE0, kE1, kPtr, kHalf, kW, kDiag, kDeg, kXS, kYS, kX0, kY0, kX, kSX, kRefl, kYC, kY, kSY,
-           kCount };
+    enum { kE0, kE1, kPtr, kHalf, kHeavy, kHChunk, kChunkNode, kW, kDiag, kDeg, kXS, kYS, kX0, kY0,
+           kX, kSX, kRefl, kYC, kY, kSY, kChunkSum, kCount };
     size_t off[kCount];
     size_t input_bytes = 0, total_bytes = 0;
 };
 
 inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-inline PraqLayout praq_layout(int n, int m, size_t real_size) {
+inline PraqLayout praq_layout(const PraqHostInstance& s, size_t real_size) {
     PraqLayout L;
-    const size_t ni = sizeof(int), nn = size_t(n), mm = size_t(m);
+    const size_t ni = sizeof(int), nn = size_t(s.n), mm = size_t(s.m);
+    const size_t nh = s.heavy.size(), nc = s.chunk_node.size();
     size_t sizes[PraqLayout::kCount];
     sizes[PraqLayout::kE0] = ni * mm;
     sizes[PraqLayout::kE1] = ni * mm;
     sizes[PraqLayout::kPtr] = ni * (nn + 1);
     sizes[PraqLayout::kHalf] = ni * 2 * mm;
+    sizes[PraqLayout::kHeavy] = ni * nh;
+    sizes[PraqLayout::kHChunk] = ni * (nh + 1);
+    sizes[PraqLayout::kChunkNode] = ni * nc;
+    sizes[PraqLayout::kChunkSum] = sizeof(double) * nc;
     sizes[PraqLayout::kW] = real_size * mm;
     sizes[PraqLayout::kDiag] = real_size * nn;
     sizes[PraqLayout::kDeg] = real_size * nn;
@@ -186,6 +221,9 @@ inline void praq_fill_image(const PraqHostInstance& s, const PraqLayout& L, char
     put_int(PraqLayout::kE1, s.e1);
     put_int(PraqLayout::kPtr, s.ptr);
     put_int(PraqLayout::kHalf, s.half);
+    put_int(PraqLayout::kHeavy, s.heavy);
+    put_int(PraqLayout::kHChunk, s.hchunk);
+    put_int(PraqLayout::kChunkNode, s.chunk_node);
     put_real(PraqLayout::kW, s.w);
     put_real(PraqLayout::kDiag, s.diag);
     put_real(PraqLayout::kDeg, s.deg);
@@ -193,11 +231,20 @@ inline void praq_fill_image(const PraqHostInstance& s, const PraqLayout& L, char
     put_real(PraqLayout::kYS, s.y_start);
 }
 
+// Binds with the hub path off (heavy_degree = INT_MAX): every tier but the
+// device tier walks every node with its lane group.
 template <class Real>
-inline PraqArrays<Real> praq_bind(const PraqLayout& L, char* base, int n, int m) {
+inline PraqArrays<Real> praq_bind(const PraqLayout& L, char* base, const PraqHostInstance& s) {
     PraqArrays<Real> a;
-    a.n = n;
-    a.m = m;
+    a.n = s.n;
+    a.m = s.m;
+    a.nh = int(s.heavy.size());
+    a.nc = int(s.chunk_node.size());
+    a.heavy_degree = std::numeric_limits<int>::max();
+    a.heavy = reinterpret_cast<const int*>(base + L.off[PraqLayout::kHeavy]);
+    a.hchunk = reinterpret_cast<const int*>(base + L.off[PraqLayout::kHChunk]);
+    a.chunk_node = reinterpret_cast<const int*>(base + L.off[PraqLayout::kChunkNode]);
+    a.chunk_sum = reinterpret_cast<double*>(base + L.off[PraqLayout::kChunkSum]);
     a.e0 = reinterpret_cast<const int*>(base + L.off[PraqLayout::kE0]);
     a.e1 = reinterpret_cast<const int*>(base + L.off[PraqLayout::kE1]);
     a.ptr = reinterpret_cast<const int*>(base + L.off[PraqLayout::kPtr]);
@@ -253,5 +300,10 @@ int praq_convert_jobs(const PraqJob<float>* from, const PraqJob<double>* to, int
 template <class Real>
 int praq_launch_wg(const PraqJob<Real>* jobs, int count, int block, hipStream_t stream);
 size_t praq_device_work_bytes();
+// Mean milliseconds of one reflected device-tier step after `warmup` steps
+// (HIP events over `steps` steps), from the job's start point.
+template <class Real>
+int praq_time_steps(const PraqJob<Real>& job, int warmup, int steps, hipStream_t stream,
+                    double* ms);
 
 }  // namespace reform

@@ -229,13 +229,13 @@ int reform_server::build(Request& r) {
         PraqInstance in{r.n, r.m, r.diag, r.rows, r.cols, r.vals};
         int st = praq_build_host(in, r.po.dense_degree, nullptr, nullptr, r.ph);
         if (st) return st;
-        r.pl64 = praq_layout(r.ph.n, r.ph.m, sizeof(double));
+        r.pl64 = praq_layout(r.ph, sizeof(double));
         r.img64.assign(r.pl64.input_bytes, 0);
         praq_fill_image<double>(r.ph, r.pl64, r.img64.data());
         r.up64 = r.pl64.input_bytes;
         r.bytes = align256(r.pl64.total_bytes);
         if (f32) {
-            r.pl32 = praq_layout(r.ph.n, r.ph.m, sizeof(float));
+            r.pl32 = praq_layout(r.ph, sizeof(float));
             r.img32.assign(r.pl32.input_bytes, 0);
             praq_fill_image<float>(r.ph, r.pl32, r.img32.data());
             r.up32 = r.pl32.input_bytes;
@@ -309,14 +309,14 @@ int reform_server::admit(Request& r) {
             REFORM_HIP_CHECK(
                 hipMemcpyAsync(base + off32, r.img32.data(), r.up32, hipMemcpyHostToDevice, s));
         r.pj64 = PraqJob<double>{};
-        r.pj64.a = praq_bind<double>(r.pl64, base + off64, r.ph.n, r.ph.m);
+        r.pj64.a = praq_bind<double>(r.pl64, base + off64, r.ph);
         r.pj64.p = r.lp;
         r.pj64.scale = r.ph.scale;
         r.pj64.lanes = r.ph.lanes;
         r.pj64.state = state;
         if (f32) {
             r.pj32 = PraqJob<float>{};
-            r.pj32.a = praq_bind<float>(r.pl32, base + off32, r.ph.n, r.ph.m);
+            r.pj32.a = praq_bind<float>(r.pl32, base + off32, r.ph);
             r.pj32.p = r.lp;
             apply_fp32_policy(r.pj32.p);
             r.pj32.scale = r.ph.scale;

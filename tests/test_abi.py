@@ -186,3 +186,33 @@ def test_fifo_queue_and_poisson_arrivals():
     s = serve.summarize(np.array([0.0, 1.0, 2.0]), np.array([0.5, 1.5, 4.0]))
     assert s["requests"] == 3 and s["p50_ms"] == pytest.approx(500.0)
     assert s["solves_per_s"] == pytest.approx(3 / 4.0)
+
+
+def test_tuning_table_round_trip_and_lane_rule_without_a_gpu(tmp_path):
+    """The tuning table accepts only known keys and power-of-two lanes, the
+    lane rule reads it per mean-degree bucket, and a table file loads."""
+    import json
+    from reformulation_amd import _native, autotune
+    L = _native.lib()
+    autotune._declare(L)
+    L.reform_tune_clear()
+    # built-in rule: smallest power of two with <= 8 half-edges per lane
+    assert autotune.lanes_for(1000, 4000) == 1      # mean degree 8
+    assert autotune.lanes_for(1000, 16000) == 4     # 32
+    assert autotune.lanes_for(1000, 400000) == 64   # 800
+    assert L.reform_tune_set(b"praq.lanes.5", 16) == 0
+    assert L.reform_tune_get(b"praq.lanes.5") == 16
+    assert autotune.lanes_for(1000, 16000) == 16    # mean degree 32: bucket 5
+    assert autotune.lanes_for(1000, 15000) == 4     # mean 30: bucket 4, unset
+    for bad in ((b"praq.lanes.8", 4), (b"praq.lanes.5", 3), (b"praq.lanes.5", 128),
+                (b"pria.tile", 32), (b"praq.lanes.", 4)):
+        assert L.reform_tune_set(*bad) == -1
+    assert L.reform_tune_get(b"nope") == -1
+    table = {"table": {"praq.lanes.0": 2, "praq.lanes.7": 64}}
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps(table))
+    L.reform_tune_clear()
+    assert autotune.load(path) == table["table"]
+    assert autotune.lanes_for(1000, 100) == 2 and autotune.lanes_for(10, 2000) == 64
+    L.reform_tune_clear()
+    autotune.load()                                  # the package's own table, if any

@@ -137,3 +137,37 @@ def test_vanilla_and_warm_phase_options_follow_the_oracle():
             warm_steps=kw.get("warm_steps", 0)))
         assert info.iterations == o.iterations
         assert abs(info.upper - o.upper) <= 1e-9 * (1 + abs(o.upper))
+
+
+def _hub_instance(seed=41, n=3000):
+    """Two hubs (degrees n - 1 and n / 2, several chunks each) over an
+    Erdos-Renyi graph of mean degree 4, integer weights."""
+    from reformulation_amd import workloads
+    base = workloads.praq_er(seed, n, 2 * n)
+    rng = np.random.default_rng(seed)
+    pairs = {(int(i), int(j)) for i, j in zip(base.rows, base.cols)}
+    pairs |= {(0, j) for j in range(1, n)}
+    pairs |= {(1, int(j)) for j in rng.choice(np.arange(2, n), n // 2, replace=False)}
+    rows, cols = (np.array(v, dtype=np.int64) for v in zip(*sorted(pairs)))
+    vals = 0.5 * (rng.integers(1, 101, rows.size) * rng.choice([-1, 1], rows.size))
+    return n, base.diag, rows, cols, vals.astype(float)
+
+
+def test_hub_nodes_by_chunks_match_the_oracle_and_the_workgroup_tier():
+    """Nodes of degree above 64 lanes take the chunked gather in the device
+    tier (one wavefront per 1024 half-edges, FP64 chunk sums in order): FP64
+    results keep the oracle's iteration counts and bounds to 1e-9, and the
+    workgroup tier (which walks hubs with its lane groups) agrees."""
+    inst = _hub_instance()
+    o = oracle.praq_solve(*inst, params=oracle.praq_params(tolerance=1e-7))
+    sc = 1.0 + abs(o.upper)
+    print(f"hub instance: n={inst[0]} m={len(inst[4])}, oracle {o.iterations} iterations",
+          flush=True)
+    for tier in (2, 1):
+        x, y, info = _gpu(inst, 1e-7, tier=tier)
+        assert info.tier == tier and info.status == 0
+        assert info.iterations == o.iterations and info.restarts == o.restarts, tier
+        assert abs(info.upper - o.upper) <= 1e-9 * sc and abs(info.lower - o.lower) <= 1e-9 * sc
+        assert np.max(np.abs(x - o.x)) <= 1e-7
+    x, y, info = _gpu(inst, 1e-6, tier=2, f32=True, max_iter=200_000)
+    assert info.status == 0 and abs(info.upper - o.upper) <= 2e-6 * sc
