@@ -110,6 +110,24 @@ inline void logic_trace(const LogicState& st, const char* what) {
             st.new_err, st.fp_error, st.check_error, st.omega, st.action);
 }
 
+// Progress of a long host-loop solve: one stderr line per REFORM_PROGRESS
+// seconds (unset or 0: none).
+struct LogicProgress {
+    double every = 0.0, next = 0.0;
+    LogicProgress() {
+        const char* v = std::getenv("REFORM_PROGRESS");
+        every = v ? std::atof(v) : 0.0;
+        next = every;
+    }
+    void maybe(const LogicState& st, const char* family, double elapsed) {
+        if (every <= 0.0 || elapsed < next) return;
+        next = elapsed + every;
+        fprintf(stderr, "[%s] %.1f s: iteration %lld, restarts %lld, gap %.3e, upper %.9e\n",
+                family, elapsed, (long long)st.iteration, (long long)st.restarts, st.gap,
+                st.upper);
+    }
+};
+
 __host__ __device__ inline double canonical_norm(const CheckSums& s, double eta, double omega,
                                                  double cross_sign) {
     double v = (omega / eta) * s.pnorm + (1.0 / (eta * omega)) * s.dnorm + cross_sign * s.cross;

@@ -313,6 +313,7 @@ int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
     double sums[kPraqSums];
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto t0 = now();
+    LogicProgress progress;
 
     if (resume) {
         st.stalled = 0;
@@ -369,6 +370,7 @@ int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
                 return kErrorDevice;
             logic_after_restart(st, p, praq_check_sums(sums, job.scale));
         }
+        progress.maybe(st, "praq", std::chrono::duration<double>(now() - t0).count());
         if (time_limit_seconds > 0.0 &&
             std::chrono::duration<double>(now() - t0).count() > time_limit_seconds &&
             st.action != kActDoneX && st.action != kActDoneSX) {

@@ -317,6 +317,7 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
     }
     double sums[kPriaSums];
     auto t0 = std::chrono::steady_clock::now();
+    LogicProgress progress;
 
     if (resume) {
         st.stalled = 0;
@@ -370,6 +371,8 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
             logic_after_restart(st, p, pria_check_sums(sums, job.scale));
             if (p.verbose) logic_trace(st, "restart");
         }
+        progress.maybe(st, "pria",
+                       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         if (time_limit_seconds > 0.0 &&
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >
                 time_limit_seconds &&
