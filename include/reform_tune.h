@@ -7,11 +7,13 @@
  * package loads at import.  Unset keys keep the built-in rules.
  *
  * Keys:
- *   "praq.lanes.<b>", b = 0..7: lanes per node (1, 2, ..., 64) for PRAQ
- *   instances whose mean degree 2m/n lies in [2^b, 2^(b+1)) (b = 0 also
- *   below 1, b = 7 also above 256).  Lanes set the CSR gather's lane groups
- *   in every tier, and hubs (degree above 64 lanes) take the chunked gather
- *   in the device tier.
+ *   "praq.lanes.<b>", b = 0..7: lanes per node (1, 2, ..., 64) of
+ *   single-instance device-tier PRAQ solves whose mean degree 2m/n lies in
+ *   [2^b, 2^(b+1)) (b = 0 also below 1, b = 7 also above 256).  Lanes set
+ *   the CSR gather's lane groups, and hubs (degree above 64 lanes) take the
+ *   chunked gather.  The batched and one-workgroup tiers keep the built-in
+ *   rule (at most 8 half-edges per lane at the mean degree), which measured
+ *   faster there.
  */
 #ifndef REFORMULATION_REFORM_TUNE_H
 #define REFORMULATION_REFORM_TUNE_H
@@ -30,7 +32,7 @@ int reform_tune_set(const char* key, int64_t value);
 int64_t reform_tune_get(const char* key);
 void reform_tune_clear(void);
 
-/* Lanes per node the library picks for a PRAQ instance of this shape. */
+/* Lanes per node of a device-tier PRAQ solve of an instance of this shape. */
 int reform_praq_lanes_for(int64_t n, int64_t m);
 
 /* Mean milliseconds of one device-tier PDHG step of `instance` (hub chunks,

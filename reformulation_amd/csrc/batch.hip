@@ -135,23 +135,41 @@ struct BatchImpl {
     int device = 0;
     double time_limit = 0.0;
 
-    // the wide engines of the batched mode, one per family (instances switch
-    // precision inside them)
-    std::unique_ptr<WidePraq> wpraq;
-    std::unique_ptr<WidePria> wpria;
-    WideStats praq_stats, pria_stats;
+    // The wide engines of the batched mode: per family, `groups[f]` engines
+    // over contiguous slot ranges of balanced work (upload deals the
+    // size-sorted instances round the groups), each on its own stream, so
+    // the launches of one group can overlap another's (REFORM_PRAQ_GROUPS,
+    // REFORM_PRIA_GROUPS; one each by default: more measured within 3% on
+    // the 1024-instance mixture).  Instances switch precision inside their
+    // engine.
+    int groups[2] = {1, 1};
+    std::vector<std::unique_ptr<WideControl>> wide[2];
+    std::vector<int> group_first[2];           // slot range of group g: [first[g], first[g+1])
+    std::vector<hipStream_t> group_streams;    // streams beyond s0 / s1
+    std::vector<hipEvent_t> group_events;      // (start, end) per engine
     bool timing = false;
 
-    WideControl* engine(int family) const {
-        return family == 0 ? static_cast<WideControl*>(wpraq.get()) : wpria.get();
+    hipStream_t stream_for(int family, int g) {
+        if (g == 0) return family == 0 ? s0 : s1;
+        const size_t k = size_t(family) * 8 + size_t(g);
+        if (group_streams.size() <= k) group_streams.resize(k + 1, nullptr);
+        if (!group_streams[k] &&
+            hipStreamCreateWithFlags(&group_streams[k], hipStreamNonBlocking) != hipSuccess)
+            group_streams[k] = nullptr;
+        return group_streams[k];
     }
     void set_timing(bool on) {
         timing = on;
-        for (int f = 0; f < 2; ++f)
-            if (WideControl* e = engine(f)) e->set_timing(on);
+        for (auto& family : wide)
+            for (auto& e : family) e->set_timing(on);
     }
 
     ~BatchImpl() {
+        for (auto& family : wide) family.clear();
+        for (hipStream_t gs : group_streams)
+            if (gs) (void)hipStreamDestroy(gs);
+        for (hipEvent_t e : group_events)
+            if (e) (void)hipEventDestroy(e);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (dev) (void)hipFree(dev);
@@ -229,6 +247,21 @@ int BatchImpl::upload() {
     std::iota(pria_order.begin(), pria_order.end(), 0);
     std::stable_sort(pria_order.begin(), pria_order.end(),
                      [&](int64_t a, int64_t b) { return pria[a].host.ND > pria[b].host.ND; });
+    // the groups of the batched mode: size-sorted instances dealt round the
+    // groups, each group a contiguous slot range
+    groups[0] = int(std::max<int64_t>(1, std::min<int64_t>(8, env_int("REFORM_PRAQ_GROUPS", 1))));
+    groups[1] = int(std::max<int64_t>(1, std::min<int64_t>(8, env_int("REFORM_PRIA_GROUPS", 1))));
+    auto deal = [](std::vector<int64_t>& order, int g, std::vector<int>& first) {
+        std::vector<int64_t> dealt;
+        first.assign(1, 0);
+        for (int k = 0; k < g; ++k) {
+            for (size_t s = size_t(k); s < order.size(); s += size_t(g)) dealt.push_back(order[s]);
+            first.push_back(int(dealt.size()));
+        }
+        order.swap(dealt);
+    };
+    deal(praq_order, groups[0], group_first[0]);
+    deal(pria_order, groups[1], group_first[1]);
     praq_slot.resize(praq.size());
     pria_slot.resize(pria.size());
     for (size_t s = 0; s < praq_order.size(); ++s) praq_slot[praq_order[s]] = int64_t(s);
@@ -417,37 +450,67 @@ int BatchImpl::solve(int mode) {
 int BatchImpl::solve_wide() {
     const bool f32 = use_float32 != 0;
     const int np = int(praq.size()), nr = int(pria.size());
-    if (np && !wpraq) {
-        wpraq.reset(new WidePraq());
-        wpraq->set_timing(timing);
-        if (wpraq->setup(praq64.host, f32 ? praq32.dev : nullptr, praq64.dev)) return kErrorDevice;
-    }
-    if (nr && !wpria) {
-        wpria.reset(new WidePria());
-        wpria->set_timing(timing);
-        if (wpria->setup(pria64.host, f32 ? pria32.dev : nullptr, pria64.dev)) return kErrorDevice;
-    }
-    praq_stats = pria_stats = WideStats();
-    WideRun runs[2];
-    hipStream_t streams[2] = {s0, s1};
-    WideControl* engines[2] = {np ? wpraq.get() : nullptr, nr ? wpria.get() : nullptr};
+    const int counts[2] = {np, nr};
     for (int f = 0; f < 2; ++f) {
-        if (!engines[f]) continue;
-        engines[f]->clear_timing();
-        REFORM_HIP_CHECK(hipEventRecord(ev[2 * f], streams[f]));
-        if (runs[f].begin(engines[f], streams[f], time_limit)) return kErrorDevice;
+        if (!counts[f] || !wide[f].empty()) continue;
+        for (int g = 0; g < groups[f]; ++g) {
+            const int lo = group_first[f][g], hi = group_first[f][g + 1];
+            if (lo == hi) continue;
+            if (f == 0) {
+                auto e = std::make_unique<WidePraq>();
+                e->set_timing(timing);
+                std::vector<PraqJob<double>> sub(praq64.host.begin() + lo, praq64.host.begin() + hi);
+                if (e->setup(sub, f32 ? praq32.dev + lo : nullptr, praq64.dev + lo))
+                    return kErrorDevice;
+                wide[0].push_back(std::move(e));
+            } else {
+                auto e = std::make_unique<WidePria>();
+                e->set_timing(timing);
+                std::vector<PriaJob<double>> sub(pria64.host.begin() + lo, pria64.host.begin() + hi);
+                if (e->setup(sub, f32 ? pria32.dev + lo : nullptr, pria64.dev + lo))
+                    return kErrorDevice;
+                wide[1].push_back(std::move(e));
+            }
+        }
     }
-    bool recorded[2] = {false, false};
+    struct Lane {
+        WideControl* engine;
+        hipStream_t stream;
+        int family;
+        hipEvent_t begin, end;
+        WideRun run;
+        bool done;
+    };
+    std::vector<Lane> lanes;
+    for (int f = 0; f < 2; ++f)
+        for (size_t g = 0; g < wide[f].size(); ++g) {
+            hipStream_t st = stream_for(f, int(g));
+            if (!st) return kErrorDevice;
+            lanes.push_back(Lane{wide[f][g].get(), st, f, nullptr, nullptr, WideRun(), false});
+        }
+    while (group_events.size() < 2 * lanes.size()) {
+        hipEvent_t e;
+        REFORM_HIP_CHECK(hipEventCreate(&e));
+        group_events.push_back(e);
+    }
+    for (size_t k = 0; k < lanes.size(); ++k) {
+        Lane& l = lanes[k];
+        l.begin = group_events[2 * k];
+        l.end = group_events[2 * k + 1];
+        l.engine->clear_timing();
+        REFORM_HIP_CHECK(hipEventRecord(l.begin, l.stream));
+        if (l.run.begin(l.engine, l.stream, time_limit)) return kErrorDevice;
+    }
     for (;;) {
         int changed = 0, running = 0;
-        for (int f = 0; f < 2; ++f) {
-            if (!engines[f] || recorded[f]) continue;
-            const int st = runs[f].advance();
+        for (Lane& l : lanes) {
+            if (l.done) continue;
+            const int st = l.run.advance();
             if (st < 0) return st;
             changed |= st;
-            if (runs[f].finished) {
-                REFORM_HIP_CHECK(hipEventRecord(ev[2 * f + 1], streams[f]));
-                recorded[f] = true;
+            if (l.run.finished) {
+                REFORM_HIP_CHECK(hipEventRecord(l.end, l.stream));
+                l.done = true;
                 changed = 1;
             } else {
                 ++running;
@@ -456,10 +519,15 @@ int BatchImpl::solve_wide() {
         if (!running) break;
         if (!changed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
-    REFORM_HIP_CHECK(hipStreamSynchronize(s0));
-    REFORM_HIP_CHECK(hipStreamSynchronize(s1));
-    praq_stats = runs[0].stats;
-    pria_stats = runs[1].stats;
+    for (Lane& l : lanes) REFORM_HIP_CHECK(hipStreamSynchronize(l.stream));
+    // per family: the longest group, start to end
+    praq_ms = pria_ms = 0.0;
+    for (Lane& l : lanes) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, l.begin, l.end) != hipSuccess) continue;
+        double& out = l.family == 0 ? praq_ms : pria_ms;
+        out = std::max(out, double(ms));
+    }
     // where each answer lives: the precision the instance finished in
     if (np) {
         REFORM_HIP_CHECK(hipMemcpy(praq_states.data(), praq_states_dev, sizeof(LogicState) * np,
@@ -471,9 +539,6 @@ int BatchImpl::solve_wide() {
                                    hipMemcpyDeviceToHost));
         for (int s = 0; s < nr; ++s) pria_in64[pria_order[s]] = char(pria_states[s].in64 != 0);
     }
-    float ms = 0.0f;
-    if (np && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) praq_ms = ms;
-    if (nr && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) pria_ms = ms;
     return 0;
 }
 
@@ -807,14 +872,26 @@ extern "C" int reform_batch_kernel_stats(const reform_batch* batch, int family,
     if (!batch || !out || family < 0 || family > 1) return kErrorArgument;
     const BatchImpl& b = *batch->impl;
     static const char* kNames[2][2] = {{"wpraq_node", "wpraq_edge"}, {"wpria_x", "wpria_yz"}};
-    WideControl* e = b.engine(family);
-    double bytes[WideControl::kKinds] = {0.0, 0.0};
-    if (e) e->algorithmic_bytes(family == 0 ? b.praq_states : b.pria_states, bytes);
+    const std::vector<LogicState>& states = family == 0 ? b.praq_states : b.pria_states;
     for (int k = 0; k < 2; ++k) {
         std::snprintf(out[k].name, sizeof(out[k].name), "%s", kNames[family][k]);
-        out[k].launches = e ? e->launches[k] : 0;
-        out[k].ms = e ? e->kernel_ms[k] : 0.0;
-        out[k].bytes = bytes[k];
+        out[k].launches = 0;
+        out[k].ms = 0.0;
+        out[k].bytes = 0.0;
+    }
+    const auto& engines = b.wide[family];
+    for (size_t g = 0; g < engines.size(); ++g) {
+        // the group's states, in its table order
+        const size_t lo = std::min<size_t>(size_t(b.group_first[family][g]), states.size());
+        const size_t hi = std::min<size_t>(lo + size_t(engines[g]->count), states.size());
+        const std::vector<LogicState> sub(states.begin() + lo, states.begin() + hi);
+        double bytes[WideControl::kKinds] = {0.0, 0.0};
+        engines[g]->algorithmic_bytes(sub, bytes);
+        for (int k = 0; k < 2; ++k) {
+            out[k].launches += engines[g]->launches[k];
+            out[k].ms += engines[g]->kernel_ms[k];
+            out[k].bytes += bytes[k];
+        }
     }
     return 0;
 }
@@ -831,9 +908,8 @@ extern "C" int64_t reform_batch_device_bytes(const reform_batch* batch) {
     if (!batch) return kErrorArgument;
     const BatchImpl& b = *batch->impl;
     size_t bytes = b.dev_bytes;
-    const WideControl* engines[] = {b.wpraq.get(), b.wpria.get()};
-    for (const WideControl* w : engines)
-        if (w) bytes += w->bytes;
+    for (const auto& family : b.wide)
+        for (const auto& w : family) bytes += w->bytes;
     return int64_t(bytes);
 }
 

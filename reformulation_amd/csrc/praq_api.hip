@@ -242,6 +242,7 @@ extern "C" int praq_solve_ex(const PraqInstance* instance, const PraqParameters*
     if (p.op_steps <= 0) p.op_steps = praq_check_interval(instance->n, &p);
     if (p.op_steps < 2) p.op_steps = 2;
     LogicParams lp = praq_logic_params(p, o);
+    if (pick_tier(s, o.tier) == PRAQ_TIER_DEVICE) praq_set_lanes(s, praq_device_lanes(s.n, s.m));
 
     PraqInfo local;
     PraqInfo* out = info ? info : &local;
@@ -282,7 +283,7 @@ extern "C" int reform_time_praq_steps(const PraqInstance* instance, int use_floa
         return kErrorArgument;
     PraqHostInstance s;
     if (int status = praq_build_host(*instance, 0, nullptr, nullptr, s)) return status;
-    if (lanes > 0) praq_set_lanes(s, lanes);
+    praq_set_lanes(s, lanes > 0 ? lanes : praq_device_lanes(s.n, s.m));
     PraqParameters p = praq_default_parameters();
     p.op_steps = 16;
     const LogicParams lp = praq_logic_params(p, praq_default_options());

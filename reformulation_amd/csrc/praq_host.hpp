@@ -54,16 +54,21 @@ inline int praq_heavy_degree(int lanes) { return 64 * lanes; }
 // The tuning table's lanes for a mean degree, 0 if unset (tune.hip).
 int praq_tuned_lanes(double mean_degree);
 
-// Lanes per node in the scatter: the tuning table's choice for the mean
-// degree, else the smallest power of two leaving each lane at most 8
-// half-edges at the mean degree, capped at a wavefront.
+// Lanes per node in the scatter: the smallest power of two leaving each
+// lane at most 8 half-edges at the mean degree, capped at a wavefront.  The
+// batched and one-workgroup tiers use this rule (measured best there).
 inline int praq_lanes_for(int n, int m) {
     double average = 2.0 * double(m) / double(n > 0 ? n : 1);
-    const int tuned = praq_tuned_lanes(average);
-    if (tuned > 0) return tuned;
     int lanes = 1;
     while (lanes < 64 && double(lanes) * 8.0 < average) lanes *= 2;
     return lanes;
+}
+
+// Lanes of a single-instance device-tier solve: the tuning table's choice
+// for the mean degree (it was measured on that tier), else the rule.
+inline int praq_device_lanes(int n, int m) {
+    const int tuned = praq_tuned_lanes(2.0 * double(m) / double(n > 0 ? n : 1));
+    return tuned > 0 ? tuned : praq_lanes_for(n, m);
 }
 
 // Lanes per node and the hubs and chunks they imply.

@@ -54,5 +54,5 @@ extern "C" void reform_tune_clear(void) {
 
 extern "C" int reform_praq_lanes_for(int64_t n, int64_t m) {
     if (n <= 0 || m < 0) return -1;
-    return praq_lanes_for(int(n), int(m));
+    return praq_device_lanes(int(n), int(m));
 }
