@@ -18,7 +18,9 @@ solver's measured per-instance times with inputs resident; the roofline of the d
 its launches' summed algorithmic bytes (PRAQ node n(4+7b) + 2m(4+b), edge
 m(8+7b); PRIA x 7b ND, y/z 8b ND per step, b = 4 for FP32) over their summed
 durations, each launch bracketed by HIP events on its stream in one more
-solve of the same batch after the timed steps; and the oracle (FP64 C restatement, OpenMP) timed on a bounded sample
+solve of the same batch after the timed steps, with `traffic` the measured
+HBM bytes per launch of that kernel (two rocprofv3 counter passes over this
+workload, calibrated on a device copy, kept in profiles/); and the oracle (FP64 C restatement, OpenMP) timed on a bounded sample
 of the same mixture on the host's cores.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--count 1024]
@@ -39,6 +41,17 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0
+TRAFFIC = ROOT / "profiles" / "pmc_traffic_mix1024_fp32.json"
+
+
+def measured_traffic(kernel: str, count: int, use_f32: bool, tolerance: float):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE passes over this workload (copy-calibrated), when the run is
+    that workload; else None."""
+    if not (TRAFFIC.exists() and count == 1024 and use_f32 and tolerance == 1e-6):
+        return None
+    k = json.loads(TRAFFIC.read_text())["kernels"].get(kernel)
+    return None if k is None else k["traffic_bytes"]
 
 
 def log(msg: str) -> None:
@@ -274,7 +287,10 @@ def main() -> int:
         serving=serve_runs or None,
         converged=status.count(0), max_gap=float(gaps.max()),
         roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=PEAK_HBM_GBS,
-                      unit="GB/s", frac=achieved / PEAK_HBM_GBS, traffic=None,
+                      unit="GB/s", frac=achieved / PEAK_HBM_GBS,
+                      traffic=measured_traffic(dom, args.count, use_f32, args.tolerance),
+                      traffic_source=("profiles/pmc_traffic_mix1024_fp32.json: FETCH_SIZE and "
+                                      "WRITE_SIZE per launch, copy-calibrated"),
                       launches=K["launches"], avg_launch_us=K["ms"] / K["launches"] * 1e3,
                       bytes_per_launch=K["bytes"] / K["launches"]),
         cpu_baseline=cpu,
