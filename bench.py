@@ -132,7 +132,10 @@ def serving(members, rate: float, seconds: float, service, use_f32: bool, thread
         return srv.submit_praq(p, pp[t]) if kind == "praq" else srv.submit_pria(p, rp[t])
 
     t0 = time.perf_counter()
-    arrive, done, status = serve.open_loop(srv, arrivals, submit)
+    arrive, done, status = serve.open_loop(
+        srv, arrivals, submit, progress=lambda got, n: log(
+            f"serving at {rate:g}/s: {got}/{n} done, {time.perf_counter() - t0:.0f} s, "
+            f"{srv.stats()['in_flight']} in flight"))
     wall = time.perf_counter() - t0
     stats = srv.stats()
     srv.close()
@@ -199,6 +202,7 @@ def main() -> int:
 
     if args.serve_only:
         service = service_times()
+        batch.close()   # the serving pool takes the HBM budget
         for rate in rates:
             out = serving(members, rate, args.serve_seconds, service, use_f32,
                           args.serve_threads)
@@ -256,6 +260,7 @@ def main() -> int:
     serve_runs = []
     if rates:
         service = service_times(() if args.no_plain else (args.tolerance,))
+        batch.close()   # the serving pool takes the HBM budget
         for rate in rates:
             serve_runs.append(serving(members, rate, args.serve_seconds, service, use_f32,
                                       args.serve_threads))

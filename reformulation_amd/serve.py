@@ -246,14 +246,15 @@ def poisson_arrivals(rate: float, seconds: float, seed: int) -> np.ndarray:
 
 
 def open_loop(server: Server, arrivals: np.ndarray, submit, drain_timeout: float = 1800.0,
-              on_done=None):
+              on_done=None, progress=None, every: float = 10.0):
     """Submit request q at arrivals[q] seconds (submit(q) -> ticket) from this
     thread while a second thread collects completions; returns per request
     (arrival, completion) on the server clock plus the run's wall span.
 
     Arrivals do not wait for completions (open loop): a request late against
     its arrival time (the submitting thread fell behind) is submitted at
-    once and its latency still counts from its scheduled arrival."""
+    once and its latency still counts from its scheduled arrival.
+    `progress(completed, total)` is called every `every` seconds."""
     n = len(arrivals)
     tickets = np.full(n, -1, dtype=np.int64)
     arrive = np.zeros(n)
@@ -266,8 +267,12 @@ def open_loop(server: Server, arrivals: np.ndarray, submit, drain_timeout: float
 
     def collect():
         got = 0
+        next_report = time.perf_counter() + every
         try:
             while got < n:
+                if progress is not None and time.perf_counter() >= next_report:
+                    next_report += every
+                    progress(got, n)
                 ready = server.poll()
                 if not ready:
                     if finished.wait(0.001) and got >= n:
