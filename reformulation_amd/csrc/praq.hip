@@ -235,7 +235,7 @@ __global__ void copy2_kernel(Real* dx, const Real* sx, int n, Real* dy, const Re
 }
 
 constexpr int kBlock = 256;
-constexpr int kSumBlocks = 4096;   // check-sum blocks (rows), plus one row for the hubs
+constexpr int kSumBlocks = 1024;   // check-sum blocks (rows), plus one row for the hubs
 
 static int blocks_for(int64_t threads) { return int((threads + kBlock - 1) / kBlock); }
 

@@ -150,12 +150,12 @@ __global__ void __launch_bounds__(1024) pria_wg_solve(const PriaJob<Real>* jobs)
 // device tier
 // ---------------------------------------------------------------------------
 
-constexpr int kPriaXBlock = 256;
+constexpr int kPriaXBlock = 512;
 constexpr int kPriaYZBlock = 512;
 constexpr int kPriaCheckBlock = 256;
 
 template <class Real>
-__global__ void __launch_bounds__(256) pria_x_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
+__global__ void __launch_bounds__(512) pria_x_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
     extern __shared__ char dyn[];
     constexpr int TW = PriaShape<Real>::TW;
     const int tiles = (a.T + TW - 1) / TW;
@@ -251,7 +251,9 @@ template int pria_cold_start<float>(const PriaArrays<float>&, hipStream_t);
 template int pria_cold_start<double>(const PriaArrays<double>&, hipStream_t);
 
 size_t pria_device_work_bytes(int N, int T) {
-    constexpr int TW = 32;  // the smaller tile width bounds the column rows
+    // the narrower precision's tiles bound the column rows
+    constexpr int TW = PriaShape<float>::TW < PriaShape<double>::TW ? PriaShape<float>::TW
+                                                                    : PriaShape<double>::TW;
     size_t tiles = size_t(N) * ((T + TW - 1) / TW);
     return sizeof(double) * (size_t(T) * kPriaSums + tiles + kPriaSums + 8);
 }

@@ -466,12 +466,13 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
     __syncthreads();
 }
 
-// Tile and register shapes.  FP32: 64 columns per x tile; FP64: 32, so the
-// tile stays under 80 KB of LDS at T' = 300.  VR: row values per lane
-// (T' <= 64 VR); VZ: slice values per thread held in registers.
+// Tile and register shapes.  x tiles of 128-byte rows: 32 FP32 or 16 FP64
+// columns, so a 512-thread tile of T' = 450 rows takes 60 KB of LDS and two
+// fit a CU.  VR: row values per lane (T' <= 64 VR); VZ: slice values per
+// thread held in registers.
 template <class Real> struct PriaShape;
-template <> struct PriaShape<float> { static constexpr int TW = 64, VR = 10, VZ = 32; };
-template <> struct PriaShape<double> { static constexpr int TW = 32, VR = 10, VZ = 32; };
+template <> struct PriaShape<float> { static constexpr int TW = 32, VR = 10, VZ = 32; };
+template <> struct PriaShape<double> { static constexpr int TW = 16, VR = 10, VZ = 32; };
 
 template <class Real>
 __host__ __device__ inline size_t pria_x_lds_bytes(int T) {
