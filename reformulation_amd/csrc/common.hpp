@@ -90,6 +90,33 @@ __device__ __forceinline__ void block_sums(double (&v)[K], double* scratch) {
     __syncthreads();
 }
 
+// block_sums (K <= 4) with one barrier instead of two: successive calls
+// alternate between the two halves of `scratch` (2 x 64 doubles, whatever
+// each call's K), so a call's writes cannot meet the previous call's reads
+// (every wave has passed the barrier in between).  `parity` is the caller's
+// count of calls, the same on every thread; the sums equal block_sums' bit
+// for bit.
+template <int K>
+__device__ __forceinline__ void block_sums_alt(double (&v)[K], double* scratch, int& parity) {
+    static_assert(K <= 4, "a half holds 4 x 16 doubles");
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int waves = blockDim.x / kWave;
+    double* half = scratch + (parity & 1) * 4 * 16;
+    ++parity;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        v[k] = wave_sum(v[k]);
+        if (lane == 0) half[k * 16 + wave] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double t = 0.0;
+        for (int w = 0; w < waves; ++w) t += half[k * 16 + w];
+        v[k] = t;
+    }
+}
+
 // One FP64 value: sum / max / min over the workgroup (scratch: 16 doubles).
 __device__ __forceinline__ double block_sum1(double v, double* scratch) {
     double a[1] = {v};

@@ -61,7 +61,7 @@ template <class Real>
 __global__ void __launch_bounds__(1024) pria_wg_solve(const PriaJob<Real>* jobs) {
     extern __shared__ char dyn[];
     Real* lds = reinterpret_cast<Real*>(dyn);
-    __shared__ double red[kPriaSums * 16];
+    __shared__ double red[kPriaSums * 16 > kPriaYZRed ? kPriaSums * 16 : kPriaYZRed];
     __shared__ LogicState st;
     __shared__ PriaJob<Real> job;
     if (threadIdx.x == 0) job = jobs[blockIdx.x];
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(512) pria_x_kernel(PriaArrays<Real> a, PriaSte
 
 template <class Real>
 __global__ void __launch_bounds__(512) pria_yz_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
-    __shared__ double red[4 * 16];
+    __shared__ double red[kPriaYZRed];
     pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, blockIdx.x, red);
 }
 
@@ -394,7 +394,7 @@ template int pria_device_solve<float>(const PriaJob<float>&, LogicState&, double
 template <class Real>
 __global__ void __launch_bounds__(1024) pria_wg_step_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
     extern __shared__ char dyn[];
-    __shared__ double red[kPriaSums * 16];
+    __shared__ double red[kPriaYZRed];
     pria_wg_step(a, st, reinterpret_cast<Real*>(dyn), red);
 }
 

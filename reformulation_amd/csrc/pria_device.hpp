@@ -301,7 +301,11 @@ __device__ __forceinline__ double group_michelot(const Real (&u)[VR], int len, i
 // to a group of 16, 32 or 64 lanes (the smallest holding len in VR values
 // per lane), so a wavefront projects 4, 2 or 1 rows at once (T' <= 64 VR).
 // z slice: the whole workgroup, values in registers when the slice fits
-// (N (T' - s) <= blockDim * VZ), else re-read from memory each round.
+// (N (T' - s) <= blockDim * VZ), else re-read from memory each round; its
+// reductions take one barrier each.  `red`: 8 16 doubles of LDS.
+// LDS doubles pria_yz_slice's `red` must hold.
+constexpr int kPriaYZRed = 8 * 16;
+
 template <class Real, int VR, int VZ>
 __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& st, int s,
                               double* red) {
@@ -359,6 +363,7 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
     const int L = N * len;
     const double target = double(a.k0);
     double th = 0.0;
+    int parity = 0;   // block_sums_alt's LDS half
     if (target > 0.0) {
         const double start = a.thz[s];
         bool first;
@@ -380,7 +385,7 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
                     }
                 }
             }
-            block_sums<4>(v4, red);
+            block_sums_alt<4>(v4, red, parity);
             first = v4[3] > 0.0;
             double C = first ? v4[3] : v4[1];
             th = ((first ? v4[2] : v4[0]) - target) / C;
@@ -395,7 +400,7 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
                         w2[1] += 1.0;
                     }
                 }
-                block_sums<2>(w2, red);
+                block_sums_alt<2>(w2, red, parity);
                 if (w2[1] == C || w2[1] == 0.0 || (w2[1] > C && !first)) break;
                 C = w2[1];
                 th = (w2[0] - target) / w2[1];
@@ -412,7 +417,7 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
                     v4[3] += 1.0;
                 }
             }
-            block_sums<4>(v4, red);
+            block_sums_alt<4>(v4, red, parity);
             first = v4[3] > 0.0;
             double C = first ? v4[3] : v4[1];
             th = ((first ? v4[2] : v4[0]) - target) / C;
@@ -425,7 +430,7 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
                         w2[1] += 1.0;
                     }
                 }
-                block_sums<2>(w2, red);
+                block_sums_alt<2>(w2, red, parity);
                 if (w2[1] == C || w2[1] == 0.0 || (w2[1] > C && !first)) break;
                 C = w2[1];
                 th = (w2[0] - target) / w2[1];

@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(256) wpria_x(const RJ32* j32, const RJ64* j64,
 
 __global__ void __launch_bounds__(512) wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab,
                                                 const int* round_action, int mode, int j) {
-    __shared__ double red[4 * 16];
+    __shared__ double red[kPriaYZRed];
     const int2 t = tab[blockIdx.x];
     dispatch(j32, j64, t.x, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
