@@ -165,10 +165,24 @@ __global__ void __launch_bounds__(512) pria_x_kernel(PriaArrays<Real> a, PriaSte
     pria_x_tile<Real, TW>(a, st, i, q * TW, reinterpret_cast<Real*>(dyn));
 }
 
+// The device tier's y rows: kPriaYRows rows of one slice per 256-thread
+// block (rows are independent, so a large instance's rows fill the chip),
+// then the z slices one 512-thread block each; per entry the arithmetic of
+// the fused pass.
+constexpr int kPriaYRows = 16;
+
+template <class Real>
+__global__ void __launch_bounds__(256) pria_y_kernel(PriaArrays<Real> a, PriaStep<Real> st,
+                                                     int chunks) {
+    const int s = blockIdx.x / chunks, i_lo = (blockIdx.x % chunks) * kPriaYRows;
+    pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, s, nullptr, 1, i_lo,
+                                                                  i_lo + kPriaYRows);
+}
+
 template <class Real>
 __global__ void __launch_bounds__(512) pria_yz_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
     __shared__ double red[kPriaYZRed];
-    pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, blockIdx.x, red);
+    pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, blockIdx.x, red, 2);
 }
 
 template <class Real>
@@ -277,6 +291,9 @@ struct PriaDevice {
         const int tiles = (a.T + TW - 1) / TW;
         hipLaunchKernelGGL(pria_x_kernel<Real>, dim3(a.N * tiles), dim3(kPriaXBlock),
                            pria_x_lds_bytes<Real>(a.T), stream, a, st);
+        const int chunks = (a.N + kPriaYRows - 1) / kPriaYRows;
+        hipLaunchKernelGGL(pria_y_kernel<Real>, dim3(a.T * chunks), dim3(256), 0, stream, a, st,
+                           chunks);
         hipLaunchKernelGGL(pria_yz_kernel<Real>, dim3(a.T), dim3(kPriaYZBlock), 0, stream, a, st);
         return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
     }

@@ -297,6 +297,8 @@ __device__ __forceinline__ double group_michelot(const Real (&u)[VR], int len, i
 
 // --- y/z pass: one slice s -------------------------------------------------
 //
+// part: 1 the y rows [i_lo, i_hi), 2 the z slice, 3 both; a block running
+// only rows needs no LDS and no barrier.
 // y rows: all N rows of the slice have len = T' - s entries; each row goes
 // to a group of 16, 32 or 64 lanes (the smallest holding len in VR values
 // per lane), so a wavefront projects 4, 2 or 1 rows at once (T' <= 64 VR).
@@ -308,7 +310,7 @@ constexpr int kPriaYZRed = 8 * 16;
 
 template <class Real, int VR, int VZ>
 __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& st, int s,
-                              double* red) {
+                              double* red, int part = 3, int i_lo = 0, int i_hi = 1 << 30) {
     const int T = a.T, N = a.N;
     const int BS = blockDim.x, tid = threadIdx.x;
     const int lane = tid & (kWave - 1), wave = tid / kWave, waves = BS / kWave;
@@ -317,12 +319,14 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
     const Real* srcy = st.reflect ? a.yc : a.sy;
     const Real* srcz = st.reflect ? a.zc : a.sz;
 
-    // ---- y: rows of the slice, 64 / width rows per wavefront at a time ----
+    // ---- y: rows [i_lo, i_hi) of the slice, 64 / width rows per wavefront
+    // at a time ----
     const int width = len <= 16 * VR ? 16 : (len <= 32 * VR ? 32 : 64);
     const int rpw = kWave / width, lr = lane % width, sub = lane / width;
-    for (int i0 = wave * rpw; i0 < N; i0 += waves * rpw) {
+    const int rows_end = (part & 1) ? min(N, i_hi) : 0;
+    for (int i0 = i_lo + wave * rpw; i0 < rows_end; i0 += waves * rpw) {
         const int i = i0 + sub;
-        const bool row = i < N;
+        const bool row = i < rows_end;
         const int rb = base + (row ? i : 0) * len;
         Real y1[VR], u[VR], an[VR];
 #pragma unroll
@@ -360,6 +364,7 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
     }
 
     // ---- z: the whole slice ----
+    if (!(part & 2)) return;
     const int L = N * len;
     const double target = double(a.k0);
     double th = 0.0;
