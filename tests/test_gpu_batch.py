@@ -152,3 +152,65 @@ def test_timed_launches_give_the_same_results_and_count_every_step():
     for s in stats:
         assert s["launches"] > 0 and s["ms"] > 0.0 and s["bytes"] > 0.0
     b.close()
+
+
+def _edge_members():
+    """The special cases of the reference's tests, mixed in one batch: one
+    node (d = -3: x = 1), isolated nodes, the zero instance, a max-cut form
+    (ends at iteration 0), no couplings at all; PRIA with K0 = 0 (z = 0),
+    one item, one period (padded to two), odd T."""
+    n, _, rows, cols, vals = random_qubo(7, 9, 0.7)
+    vals = np.abs(vals)
+    diag = np.zeros(n)
+    np.add.at(diag, rows, -vals)
+    np.add.at(diag, cols, -vals)
+    empty = np.zeros(0, np.int64)
+    out = [("praq", (1, np.array([-3.0]), empty, empty, np.zeros(0))),
+           ("praq", (5, np.array([2.0, -1.0, 0.0, 3.0, -4.0]), empty, empty, np.zeros(0))),
+           ("praq", (4, np.zeros(4), np.array([0, 1]), np.array([2, 3]), np.zeros(2))),
+           ("praq", (n, diag, rows, cols, vals)),
+           ("praq", random_qubo(9, 6, 0.5))]
+    k, _, h = small_jrp(4, 3, 8)
+    out.append(("pria", (k, 0.0, h)))
+    out.append(("pria", small_jrp(5, 1, 6)))
+    out.append(("pria", small_jrp(6, 3, 1)))
+    out.append(("pria", small_jrp(7, 2, 5)))
+    return out
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_special_cases_batched_and_served_match_the_oracle(f32):
+    from reformulation_amd.serve import Server
+    from test_gpu_serve import _problems, _served, _submit
+    members = _edge_members()
+    b, index = _batch(members, f32, tol=1e-8)
+    b.solve("batched")
+    batched = _results(b, index)
+    b.close()
+    with Server(use_float32=f32, host_threads=2) as srv:
+        tickets = [_submit(srv, item) for item in _problems(members, tol=1e-8)]
+        srv.wait_idle(60.0)
+        served = [_served(srv, kind, t) for (kind, _), t in zip(members, tickets)]
+    for q, ((kind, inst), (x, dual, info)) in enumerate(zip(members, batched)):
+        if kind == "praq":
+            o = oracle.praq_solve(*inst, params=oracle.praq_params(tolerance=1e-8))
+        else:
+            k, k0, h = inst
+            o = oracle.pria_solve(k, k0, h, oracle.pria_params(tolerance=1e-8))
+        assert info.status == 0, q
+        sc = 1.0 + abs(o.upper)
+        if not f32:
+            assert info.iterations == o.iterations, (q, info.iterations, o.iterations)
+            assert abs(info.upper - o.upper) <= 1e-9 * sc, q
+        else:
+            assert abs(info.upper - o.upper) <= 2e-6 * sc, q
+        if kind == "pria" and inst[1] == 0.0:
+            assert np.all(dual[1] == 0.0), q          # z = 0 when K0 = 0
+        sx, sdual, sinfo = served[q]
+        assert _same_bits(sx, x) and _same_bits(sdual, dual), q
+        assert sinfo.iterations == info.iterations, q
+    assert abs(batched[0][2].upper + 3.0) < 1e-8       # n = 1, d = -3
+    if not f32:
+        # the max-cut form certifies at its first check; FP32 iterates round
+        # its bounds apart by more than 1e-8 and finish after the handover
+        assert batched[3][2].iterations == 0
