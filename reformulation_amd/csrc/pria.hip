@@ -171,18 +171,27 @@ __global__ void __launch_bounds__(512) pria_x_kernel(PriaArrays<Real> a, PriaSte
 // the fused pass.
 constexpr int kPriaYRows = 16;
 
-template <class Real>
+// Shape by T' (the compact register shape up to T' = 320; the sums do not
+// depend on the shape).
+template <class Real, bool compact>
+struct PriaPick {
+    static constexpr int VR = compact ? PriaCompactShape<Real>::VR : PriaShape<Real>::VR;
+    static constexpr int VZ = compact ? PriaCompactShape<Real>::VZ : PriaShape<Real>::VZ;
+};
+
+template <class Real, bool compact>
 __global__ void __launch_bounds__(256) pria_y_kernel(PriaArrays<Real> a, PriaStep<Real> st,
                                                      int chunks) {
     const int s = blockIdx.x / chunks, i_lo = (blockIdx.x % chunks) * kPriaYRows;
-    pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, s, nullptr, 1, i_lo,
-                                                                  i_lo + kPriaYRows);
+    pria_yz_slice<Real, PriaPick<Real, compact>::VR, PriaPick<Real, compact>::VZ>(
+        a, st, s, nullptr, 1, i_lo, i_lo + kPriaYRows);
 }
 
-template <class Real>
+template <class Real, bool compact>
 __global__ void __launch_bounds__(512) pria_yz_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
     __shared__ double red[kPriaYZRed];
-    pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, st, blockIdx.x, red, 2);
+    pria_yz_slice<Real, PriaPick<Real, compact>::VR, PriaPick<Real, compact>::VZ>(
+        a, st, blockIdx.x, red, 2);
 }
 
 template <class Real>
@@ -292,9 +301,17 @@ struct PriaDevice {
         hipLaunchKernelGGL(pria_x_kernel<Real>, dim3(a.N * tiles), dim3(kPriaXBlock),
                            pria_x_lds_bytes<Real>(a.T), stream, a, st);
         const int chunks = (a.N + kPriaYRows - 1) / kPriaYRows;
-        hipLaunchKernelGGL(pria_y_kernel<Real>, dim3(a.T * chunks), dim3(256), 0, stream, a, st,
-                           chunks);
-        hipLaunchKernelGGL(pria_yz_kernel<Real>, dim3(a.T), dim3(kPriaYZBlock), 0, stream, a, st);
+        if (a.T <= kPriaCompactT) {
+            hipLaunchKernelGGL((pria_y_kernel<Real, true>), dim3(a.T * chunks), dim3(256), 0,
+                               stream, a, st, chunks);
+            hipLaunchKernelGGL((pria_yz_kernel<Real, true>), dim3(a.T), dim3(kPriaYZBlock), 0,
+                               stream, a, st);
+        } else {
+            hipLaunchKernelGGL((pria_y_kernel<Real, false>), dim3(a.T * chunks), dim3(256), 0,
+                               stream, a, st, chunks);
+            hipLaunchKernelGGL((pria_yz_kernel<Real, false>), dim3(a.T), dim3(kPriaYZBlock), 0,
+                               stream, a, st);
+        }
         return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
     }
     int copy3(Real* d1, const Real* s1, Real* d2, const Real* s2, Real* d3, const Real* s3) {

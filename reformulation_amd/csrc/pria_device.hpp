@@ -302,11 +302,20 @@ __device__ __forceinline__ double group_michelot(const Real (&u)[VR], int len, i
 // y rows: all N rows of the slice have len = T' - s entries; each row goes
 // to a group of 16, 32 or 64 lanes (the smallest holding len in VR values
 // per lane), so a wavefront projects 4, 2 or 1 rows at once (T' <= 64 VR).
+// Widths follow kPriaRow16 / kPriaRow32, not VR.
 // z slice: the whole workgroup, values in registers when the slice fits
 // (N (T' - s) <= blockDim * VZ), else re-read from memory each round; its
 // reductions take one barrier each.  `red`: 8 16 doubles of LDS.
 // LDS doubles pria_yz_slice's `red` must hold.
 constexpr int kPriaYZRed = 8 * 16;
+
+// Row group widths by row length, the same for every register shape: a row
+// of at most 80 entries takes 16 lanes, at most 160 takes 32, longer rows a
+// whole wavefront.  VR (values per lane) only has to hold the row, so every
+// shape with VR >= ceil(len / width) gives the same sums bit for bit; so does
+// VZ (the register path and the memory path add a thread's values in the
+// same order).
+constexpr int kPriaRow16 = 80, kPriaRow32 = 160;
 
 template <class Real, int VR, int VZ>
 __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& st, int s,
@@ -321,7 +330,7 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
 
     // ---- y: rows [i_lo, i_hi) of the slice, 64 / width rows per wavefront
     // at a time ----
-    const int width = len <= 16 * VR ? 16 : (len <= 32 * VR ? 32 : 64);
+    const int width = len <= kPriaRow16 ? 16 : (len <= kPriaRow32 ? 32 : 64);
     const int rpw = kWave / width, lr = lane % width, sub = lane / width;
     const int rows_end = (part & 1) ? min(N, i_hi) : 0;
     for (int i0 = i_lo + wave * rpw; i0 < rows_end; i0 += waves * rpw) {
@@ -479,10 +488,15 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
 // Tile and register shapes.  x tiles of 128-byte rows: 32 FP32 or 16 FP64
 // columns, so a 512-thread tile of T' = 450 rows takes 60 KB of LDS and two
 // fit a CU.  VR: row values per lane (T' <= 64 VR); VZ: slice values per
-// thread held in registers.
+// thread held in registers.  The compact shape (T' <= 320) keeps the
+// batched y/z kernel within 80 VGPRs, three 512-thread blocks per CU.
 template <class Real> struct PriaShape;
 template <> struct PriaShape<float> { static constexpr int TW = 32, VR = 10, VZ = 32; };
 template <> struct PriaShape<double> { static constexpr int TW = 16, VR = 10, VZ = 32; };
+template <class Real> struct PriaCompactShape;
+template <> struct PriaCompactShape<float> { static constexpr int VR = 5, VZ = 16; };
+template <> struct PriaCompactShape<double> { static constexpr int VR = 5, VZ = 8; };
+constexpr int kPriaCompactT = 64 * 5;   // largest T' the compact shape holds
 
 template <class Real>
 __host__ __device__ inline size_t pria_x_lds_bytes(int T) {

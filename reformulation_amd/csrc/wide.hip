@@ -375,15 +375,26 @@ __global__ void __launch_bounds__(256) wpria_x(const RJ32* j32, const RJ64* j64,
     });
 }
 
-__global__ void __launch_bounds__(512) wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab,
-                                                const int* round_action, int mode, int j) {
+// The y/z pass, one slice per 512-thread block.  With `compact` (every
+// instance's T' <= 320) the register shape fits 80 VGPRs and three blocks
+// share a CU; the shapes give the same sums (row widths follow the row
+// length only).
+template <bool compact>
+__global__ void __launch_bounds__(512, compact ? 6 : 4)
+wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab, const int* round_action, int mode,
+         int j) {
     __shared__ double red[kPriaYZRed];
     const int2 t = tab[blockIdx.x];
+    if (t.x < 0) return;
     dispatch(j32, j64, t.x, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         PriaStep<Real> sc;
         if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
-        pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(job.a, sc, t.y, red);
+        if (compact)
+            pria_yz_slice<Real, PriaCompactShape<Real>::VR, PriaCompactShape<Real>::VZ>(
+                job.a, sc, t.y, red);
+        else
+            pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(job.a, sc, t.y, red);
     });
 }
 
@@ -929,8 +940,12 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
     hipLaunchKernelGGL(wpria_x, dim3(x_blocks), dim3(kWideBlock), lds, stream, jobs32, jobs64,
                        x_tab, round_action, mode, j);
     if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_yz, dim3(s_blocks), dim3(512), 0, stream, jobs32, jobs64, s_tab,
-                       round_action, mode, j);
+    if (max_T <= kPriaCompactT)
+        hipLaunchKernelGGL(wpria_yz<true>, dim3(s_blocks), dim3(512), 0, stream, jobs32, jobs64,
+                           s_tab, round_action, mode, j);
+    else
+        hipLaunchKernelGGL(wpria_yz<false>, dim3(s_blocks), dim3(512), 0, stream, jobs32, jobs64,
+                           s_tab, round_action, mode, j);
     if (mark(stream, 1, false)) return kErrorDevice;
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
