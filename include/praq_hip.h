@@ -106,6 +106,19 @@ int praq_solve_ex(const PraqInstance* instance, const PraqParameters* parameters
                   const PraqOptions* options, const double* warm_x, const double* warm_y,
                   double* x, double* y, PraqInfo* info);
 
+/* praq_solve_ex over device memory (no counterpart in the reference, whose
+ * entry points take host arrays, qubo/praq-cuda/src/praq.cu:35-79): the
+ * instance's four arrays and x [n], y [m][2] (y may be NULL) are device
+ * pointers -- caller-owned allocations such as torch tensors' data_ptr() --
+ * and every launch and copy is ordered on `stream` (a hipStream_t; NULL: the
+ * default stream).  The COO arrays are copied to the host for the O(n + m)
+ * half-edge build (the reference also builds on the host,
+ * qubo/praq-cuda/src/solver.cuh:552-560); the answer is written on the
+ * device.  Returns when the answer is in place; codes as praq_solve_ex. */
+int praq_solve_device(const PraqInstance* instance, const PraqParameters* parameters,
+                      const PraqOptions* options, double* x, double* y, PraqInfo* info,
+                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,20 @@ int pria_solve_ex(int64_t n, int64_t t, const double* order_costs, double joint_
                   const double* warm_y, const double* warm_z, double* x, double* y, double* z,
                   pria_info* info);
 
+/* pria_solve_ex over device memory, cold start (no counterpart in the
+ * reference, whose entry points take host arrays, jrp/pria-cuda/src/pria.cu:72-108):
+ * order_costs [n], holding_costs [n][t][t] and the outputs x, y, z
+ * [n][t][t] (each may be NULL) are device pointers -- caller-owned
+ * allocations such as torch tensors' data_ptr() -- and every launch and
+ * copy is ordered on `stream` (a hipStream_t; NULL: the default stream).
+ * The scale, the slice-order image, the dense answers and the upper bound
+ * are computed on the device; only scalars cross PCIe.  Returns when the
+ * answer is in place; codes as pria_solve_ex. */
+int pria_solve_device(int64_t n, int64_t t, const double* order_costs, double joint_cost,
+                      const double* holding_costs, const pria_params* params,
+                      const pria_options* options, int use_float32, double* x, double* y,
+                      double* z, pria_info* info, void* stream);
+
 /* Test hook: one PDHG step x2 = P_X(x1 - tau (h + y1 + z1)), r = sigma
  * (2 x2 - x1), y2 = P_Y(y1 + r), z2 = P_Z(z1 + r) on an even-T instance whose
  * h, K, K0 are already scaled; dense [n][t][t] in and out.  tier 1: one

@@ -16,6 +16,9 @@
  *                      instance is one workgroup running its whole solve.
  * An instance's batched result does not depend on the other instances of the
  * batch: its arithmetic and summation order are those of its own blocks.
+ * An instance's options.time_limit (seconds, > 0) counts from the start of
+ * each solve in every mode: past it the instance stops at its current point
+ * (between two rounds in the batched mode) with status 2.
  *
  * Usage: create, add_* (host), upload (host build + one H2D copy), then
  * solve any number of times from the resident inputs, then read results.

@@ -11,7 +11,10 @@
  * batch, so its result does not depend on what else was in flight.
  *
  * Requests are admitted in submission order; one whose images exceed the
- * whole budget is rejected (status -2).
+ * whole budget is rejected (status -2).  A request's options.time_limit
+ * (seconds, > 0) counts from its admission: past it the request stops at its
+ * current point between two rounds and finishes with status 2, as the plain
+ * solvers do at their limits.
  *
  * Return codes: >= 0 success (submit returns the ticket), -1 argument,
  * -2 instance, -3 device, -4 unknown or unfinished ticket, -5 timeout.

@@ -76,9 +76,10 @@ class PriaInfo(ctypes.Structure):
 #: Every symbol include/*.h declares; a test checks the library exports them.
 EXPORTS = (
     "praq_default_parameters", "praq_check_interval", "praq_solve", "praq_default_options",
-    "praq_solve_ex",
+    "praq_solve_ex", "praq_solve_device",
     "pria_params_auto", "pria_solve_f64", "pria_solve_f32", "pria_solve_batch_f64",
     "pria_solve_batch_f32", "pria_objective_f64", "pria_default_options", "pria_solve_ex",
+    "pria_solve_device",
     "reform_batch_create", "reform_batch_destroy", "reform_batch_add_praq",
     "reform_batch_add_pria", "reform_batch_upload", "reform_batch_solve",
     "reform_batch_set_shape", "reform_batch_count", "reform_batch_kernel_ms",
@@ -105,6 +106,21 @@ def available() -> bool:
     return LIB_PATH.exists()
 
 
+def _torch_runtime_first() -> None:
+    """torch's wheel ships its own HIP runtime under the system runtime's
+    soname (libamdhip64.so.7), but its libraries name the file
+    libamdhip64.so.  Loaded after this library (which names the soname),
+    torch would map a second runtime into the process, and the two share no
+    devices, streams or allocations: torch then finds no device.  Loaded
+    first, torch's runtime is the one this library binds to, so tensors'
+    data pointers and torch streams can be handed to the device-pointer entry
+    points.  torch is imported here, when it is installed, for that order."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib() -> ctypes.CDLL:
     """Load the library, declaring argument and return types."""
     global _lib
@@ -112,6 +128,7 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    _torch_runtime_first()
     L = ctypes.CDLL(str(LIB_PATH))
     L.reform_last_error.restype = ctypes.c_char_p
     L.reform_last_error.argtypes = []
@@ -126,6 +143,11 @@ def lib() -> ctypes.CDLL:
     L.praq_solve_ex.argtypes = [ctypes.POINTER(PraqInstance), ctypes.POINTER(PraqParameters),
                                 ctypes.POINTER(PraqOptions), _dp, _dp, _dp, _dp,
                                 ctypes.POINTER(PraqInfo)]
+    # device pointers and the stream travel as plain addresses
+    L.praq_solve_device.restype = _int
+    L.praq_solve_device.argtypes = [ctypes.POINTER(PraqInstance), ctypes.POINTER(PraqParameters),
+                                    ctypes.POINTER(PraqOptions), ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.POINTER(PraqInfo), ctypes.c_void_p]
     if hasattr(L, "pria_params_auto"):
         L.pria_params_auto.restype = PriaParams
         L.pria_params_auto.argtypes = [_i64, _dbl]
@@ -147,6 +169,11 @@ def lib() -> ctypes.CDLL:
         L.pria_solve_ex.argtypes = [_i64, _i64, _dp, _dbl, _dp, ctypes.POINTER(PriaParams),
                                     ctypes.POINTER(PriaOptions), _int, _dp, _dp, _dp, _dp, _dp,
                                     _dp, ctypes.POINTER(PriaInfo)]
+        L.pria_solve_device.restype = _int
+        L.pria_solve_device.argtypes = [_i64, _i64, ctypes.c_void_p, _dbl, ctypes.c_void_p,
+                                        ctypes.POINTER(PriaParams), ctypes.POINTER(PriaOptions),
+                                        _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.POINTER(PriaInfo), ctypes.c_void_p]
     _lib = L
     from . import autotune        # the measured shape table, when one is present
     autotune.load(L=L)

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <numeric>
 #include <thread>
@@ -180,6 +181,7 @@ struct BatchImpl {
     int upload();
     int solve(int mode);
     int solve_wide();
+    std::vector<double> deadlines(int family, int lo, int hi) const;
     int solve_workgroups();
     int solve_sequential();
     int praq_result(int64_t q, double* x, double* y, PraqInfo* info);
@@ -187,7 +189,8 @@ struct BatchImpl {
 
     // one plain PRAQ solve of launch slot s from job `job` (its precision)
     template <class Real>
-    int praq_plain(const PraqJob<Real>& job, PraqJob<Real>* devslot, int tier, bool resume) {
+    int praq_plain(const PraqJob<Real>& job, PraqJob<Real>* devslot, int tier, bool resume,
+                   double time_limit) {
         if (tier == PRAQ_TIER_WORKGROUP) {
             PraqJob<Real> j = job;
             j.resume = resume ? 1 : 0;
@@ -198,13 +201,14 @@ struct BatchImpl {
         LogicState st;
         if (resume && hipMemcpy(&st, job.state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
             return kErrorDevice;
-        if (praq_device_solve<Real>(job, st, work, 0.0, s0, resume)) return kErrorDevice;
+        if (praq_device_solve<Real>(job, st, work, time_limit, s0, resume)) return kErrorDevice;
         REFORM_HIP_CHECK(hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice, s0));
         return hipStreamSynchronize(s0) == hipSuccess ? 0 : kErrorDevice;
     }
 
     template <class Real>
-    int pria_plain(const PriaJob<Real>& job, PriaJob<Real>* devslot, int tier, bool resume, int T) {
+    int pria_plain(const PriaJob<Real>& job, PriaJob<Real>* devslot, int tier, bool resume, int T,
+                   double time_limit) {
         if (tier == PRIA_TIER_WORKGROUP) {
             PriaJob<Real> j = job;
             j.resume = resume ? 1 : 0;
@@ -215,7 +219,7 @@ struct BatchImpl {
         LogicState st;
         if (resume && hipMemcpy(&st, job.state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
             return kErrorDevice;
-        if (pria_device_solve<Real>(job, st, work, 0.0, s1, resume)) return kErrorDevice;
+        if (pria_device_solve<Real>(job, st, work, time_limit, s1, resume)) return kErrorDevice;
         REFORM_HIP_CHECK(hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice, s1));
         return hipStreamSynchronize(s1) == hipSuccess ? 0 : kErrorDevice;
     }
@@ -340,7 +344,12 @@ int BatchImpl::upload() {
     for (size_t q = 0; q < pria.size(); ++q)
         stage = std::max(stage, pria_upload_bytes(pria[q].host, rl64[q]));
     std::vector<char> image(stage);
-    const long long ticks = (long long)(kWgLimitSeconds * clock_rate_hz());
+    // the workgroup tier's own clock limit: the instance's, at most kWgLimitSeconds
+    const double rate = clock_rate_hz();
+    auto ticks_for = [&](double limit) {
+        const double sec = limit > 0.0 ? std::min(limit, kWgLimitSeconds) : kWgLimitSeconds;
+        return (long long)(sec * rate);
+    };
     const int64_t praq_wg = env_int("REFORM_PRAQ_WG_MAX", int64_t(1) << 17);
     const int64_t pria_wg = env_int("REFORM_PRIA_WG_MAX", int64_t(1) << 15);
 
@@ -365,7 +374,7 @@ int BatchImpl::upload() {
             job.p = e.lp;
             job.scale = e.host.scale;
             job.lanes = e.host.lanes;
-            job.time_limit_ticks = ticks;
+            job.time_limit_ticks = ticks_for(e.o.time_limit);
             job.state = praq_states_dev + s;
             return 0;
         };
@@ -394,7 +403,7 @@ int BatchImpl::upload() {
             if (e.host.cold && pria_cold_start<Real>(job.a, s1)) return kErrorDevice;
             job.p = e.lp;
             job.scale = e.host.scale;
-            job.time_limit_ticks = ticks;
+            job.time_limit_ticks = ticks_for(e.o.time_limit);
             job.state = pria_states_dev + s;
             return 0;
         };
@@ -443,6 +452,23 @@ int BatchImpl::solve(int mode) {
     return 0;
 }
 
+// The instances' own time limits (options.time_limit, seconds from the start
+// of the solve) for the slots [lo, hi) of a family; empty when none has one.
+std::vector<double> BatchImpl::deadlines(int family, int lo, int hi) const {
+    std::vector<double> d(size_t(hi - lo), std::numeric_limits<double>::infinity());
+    bool any = false;
+    for (int s = lo; s < hi; ++s) {
+        const double limit = family == 0 ? praq[praq_order[s]].o.time_limit
+                                         : pria[pria_order[s]].o.time_limit;
+        if (limit > 0.0) {
+            d[s - lo] = limit;
+            any = true;
+        }
+    }
+    if (!any) d.clear();
+    return d;
+}
+
 // Both families at once from this thread: PRAQ on s0, PRIA on s1, each one
 // wide engine whose instances switch to FP64 on their own.  The runs are
 // advanced without blocking, so neither stream waits for the other's host
@@ -462,6 +488,7 @@ int BatchImpl::solve_wide() {
                 std::vector<PraqJob<double>> sub(praq64.host.begin() + lo, praq64.host.begin() + hi);
                 if (e->setup(sub, f32 ? praq32.dev + lo : nullptr, praq64.dev + lo))
                     return kErrorDevice;
+                e->set_deadlines(deadlines(0, lo, hi));
                 wide[0].push_back(std::move(e));
             } else {
                 auto e = std::make_unique<WidePria>();
@@ -469,6 +496,7 @@ int BatchImpl::solve_wide() {
                 std::vector<PriaJob<double>> sub(pria64.host.begin() + lo, pria64.host.begin() + hi);
                 if (e->setup(sub, f32 ? pria32.dev + lo : nullptr, pria64.dev + lo))
                     return kErrorDevice;
+                e->set_deadlines(deadlines(1, lo, hi));
                 wide[1].push_back(std::move(e));
             }
         }
@@ -637,14 +665,15 @@ int BatchImpl::solve_sequential() {
     for (size_t q = 0; q < praq.size(); ++q) {
         const int64_t s = praq_slot[q];
         const int tier = praq[q].tier;
+        const double limit = praq[q].o.time_limit;
         const clock::time_point t0 = clock::now();
         if (!f32) {
-            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, false)) return kErrorDevice;
+            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, false, limit)) return kErrorDevice;
             praq_in64[q] = 1;
             praq_seconds[q] = since(t0);
             continue;
         }
-        if (praq_plain<float>(praq32.host[s], praq32.sub, tier, false)) return kErrorDevice;
+        if (praq_plain<float>(praq32.host[s], praq32.sub, tier, false, limit)) return kErrorDevice;
         LogicState st;
         REFORM_HIP_CHECK(hipMemcpy(&st, praq_states_dev + s, sizeof(st), hipMemcpyDeviceToHost));
         praq_in64[q] = 0;
@@ -652,7 +681,7 @@ int BatchImpl::solve_sequential() {
             st.switch_iteration = st.iteration;
             REFORM_HIP_CHECK(hipMemcpy(praq_states_dev + s, &st, sizeof(st), hipMemcpyHostToDevice));
             if (praq_convert_state(praq32.host[s].a, praq64.host[s].a, s0)) return kErrorDevice;
-            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, true)) return kErrorDevice;
+            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, true, limit)) return kErrorDevice;
             praq_in64[q] = 1;
         }
         praq_seconds[q] = since(t0);
@@ -660,14 +689,15 @@ int BatchImpl::solve_sequential() {
     for (size_t q = 0; q < pria.size(); ++q) {
         const int64_t s = pria_slot[q];
         const int tier = pria[q].tier, T = pria[q].host.T;
+        const double limit = pria[q].o.time_limit;
         const clock::time_point t0 = clock::now();
         if (!f32) {
-            if (pria_plain<double>(pria64.host[s], pria64.sub, tier, false, T)) return kErrorDevice;
+            if (pria_plain<double>(pria64.host[s], pria64.sub, tier, false, T, limit)) return kErrorDevice;
             pria_in64[q] = 1;
             pria_seconds[q] = since(t0);
             continue;
         }
-        if (pria_plain<float>(pria32.host[s], pria32.sub, tier, false, T)) return kErrorDevice;
+        if (pria_plain<float>(pria32.host[s], pria32.sub, tier, false, T, limit)) return kErrorDevice;
         LogicState st;
         REFORM_HIP_CHECK(hipMemcpy(&st, pria_states_dev + s, sizeof(st), hipMemcpyDeviceToHost));
         pria_in64[q] = 0;
@@ -675,7 +705,7 @@ int BatchImpl::solve_sequential() {
             st.switch_iteration = st.iteration;
             REFORM_HIP_CHECK(hipMemcpy(pria_states_dev + s, &st, sizeof(st), hipMemcpyHostToDevice));
             if (pria_convert_state(pria32.host[s].a, pria64.host[s].a, s1)) return kErrorDevice;
-            if (pria_plain<double>(pria64.host[s], pria64.sub, tier, true, T)) return kErrorDevice;
+            if (pria_plain<double>(pria64.host[s], pria64.sub, tier, true, T, limit)) return kErrorDevice;
             pria_in64[q] = 1;
         }
         pria_seconds[q] = since(t0);

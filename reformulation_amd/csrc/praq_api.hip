@@ -42,6 +42,19 @@ struct DeviceBuffer {
 };
 
 
+// The answer into caller-owned device memory: x widened to FP64, the dual
+// from slot-major halves to [m][2] pairs in instance units (as outputs()).
+template <class Real>
+__global__ void __launch_bounds__(256) praq_put_outputs(const Real* x, const Real* y, int n, int m,
+                                                        double scale, double* dx, double* dy) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) dx[g] = double(x[g]);
+    if (dy && g < m) {
+        dy[2 * size_t(g)] = double(y[g]) * scale;
+        dy[2 * size_t(g) + 1] = double(y[m + g]) * scale;
+    }
+}
+
 int pick_tier(const PraqHostInstance& s, int tier) {
     if (tier != PRAQ_TIER_AUTO) return tier;
     return int64_t(s.n) + int64_t(s.m) <= wg_tier_max() ? PRAQ_TIER_WORKGROUP : PRAQ_TIER_DEVICE;
@@ -116,24 +129,32 @@ struct PraqImage {
             }
         return 0;
     }
+
+    // the same answer written to device memory, in stream order
+    int outputs_device(const PraqHostInstance& s, double* x, double* y, hipStream_t stream) {
+        const int count = std::max(s.n, s.m);
+        if (count > 0)
+            hipLaunchKernelGGL(praq_put_outputs<Real>, dim3((count + 255) / 256), dim3(256), 0,
+                               stream, job.a.x, job.a.y, s.n, s.m, s.scale, x,
+                               s.m > 0 ? y : nullptr);
+        REFORM_HIP_CHECK(hipGetLastError());
+        REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+        return 0;
+    }
 };
 
-// FP64 iterates, or FP32 iterates handing over to FP64 when they stall.
+// FP64 iterates, or FP32 iterates handing over to FP64 when they stall; all
+// work on `stream`.  `emit(image)` writes the answer of the image the solve
+// finished in.
+template <class Emit>
 int solve_mixed(const PraqHostInstance& s, LogicParams lp, const PraqOptions& o, bool fp32,
-                double* x, double* y, PraqInfo* info) {
-    hipStream_t stream;
-    REFORM_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } guard{stream};
+                hipStream_t stream, Emit&& emit, PraqInfo* info) {
     const int tier = pick_tier(s, o.tier);
     LogicState st;
     std::memset(&st, 0, sizeof(st));
     if (!fp32) {
         PraqImage<double> d;
-        if (d.init(s, lp, stream) || d.run(tier, st, o.time_limit, false, stream) ||
-            d.outputs(s, x, y, stream))
+        if (d.init(s, lp, stream) || d.run(tier, st, o.time_limit, false, stream) || emit(d))
             return kErrorDevice;
     } else {
         LogicParams lp32 = lp;
@@ -145,9 +166,9 @@ int solve_mixed(const PraqHostInstance& s, LogicParams lp, const PraqOptions& o,
             PraqImage<double> d;
             st.switch_iteration = st.iteration;
             if (d.init(s, lp, stream) || praq_convert_state(f.job.a, d.job.a, stream) ||
-                d.run(tier, st, o.time_limit, true, stream) || d.outputs(s, x, y, stream))
+                d.run(tier, st, o.time_limit, true, stream) || emit(d))
                 return kErrorDevice;
-        } else if (f.outputs(s, x, y, stream)) {
+        } else if (emit(f)) {
             return kErrorDevice;
         }
     }
@@ -218,15 +239,23 @@ extern "C" int64_t praq_check_interval(int64_t n, const PraqParameters* paramete
     return int64_t(v);
 }
 
-extern "C" int praq_solve_ex(const PraqInstance* instance, const PraqParameters* parameters,
-                             const PraqOptions* options, const double* warm_x,
-                             const double* warm_y, double* x, double* y, PraqInfo* info) {
-    if (info) {
-        std::memset(info, 0, sizeof(*info));
-        info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
-        info->status = kErrorArgument;
+namespace {
+
+struct StreamOwner {
+    hipStream_t s = nullptr;
+    ~StreamOwner() {
+        if (s) (void)hipStreamDestroy(s);
     }
-    if (!instance || !parameters || !x) return kErrorArgument;
+};
+
+// The body of praq_solve_ex and praq_solve_device: `instance` in host
+// memory, every launch and copy on `stream` (own_stream: on a stream of its
+// own, created once the instance has been validated), the answer through
+// `emit(image, host instance, stream)`.
+template <class Emit>
+int solve_entry(const PraqInstance* instance, const PraqParameters* parameters,
+                const PraqOptions* options, const double* warm_x, const double* warm_y,
+                hipStream_t stream, bool own_stream, Emit&& emit, PraqInfo* info) {
     PraqParameters p = *parameters;
     if (!params_valid(p)) return kErrorArgument;
     PraqOptions o = options ? *options : praq_default_options();
@@ -246,12 +275,81 @@ extern "C" int praq_solve_ex(const PraqInstance* instance, const PraqParameters*
 
     PraqInfo local;
     PraqInfo* out = info ? info : &local;
-    status = solve_mixed(s, lp, o, p.use_float32 != 0, x, y, out);
+    StreamOwner owned;
+    if (own_stream) {
+        const hipError_t e = hipStreamCreateWithFlags(&owned.s, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            owned.s = nullptr;
+            record_error("hipStreamCreateWithFlags", e, __FILE__, __LINE__);
+            out->status = kErrorDevice;
+            return kErrorDevice;
+        }
+        stream = owned.s;
+    }
+    status = solve_mixed(s, lp, o, p.use_float32 != 0, stream,
+                         [&](auto& image) { return emit(image, s, stream); }, out);
     if (status < 0) {
         out->status = status;
         return status;
     }
     return out->status == 0 ? 0 : 1;
+}
+
+void clear_info(PraqInfo* info) {
+    if (!info) return;
+    std::memset(info, 0, sizeof(*info));
+    info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
+    info->status = kErrorArgument;
+}
+
+}  // namespace
+
+extern "C" int praq_solve_ex(const PraqInstance* instance, const PraqParameters* parameters,
+                             const PraqOptions* options, const double* warm_x,
+                             const double* warm_y, double* x, double* y, PraqInfo* info) {
+    clear_info(info);
+    if (!instance || !parameters || !x) return kErrorArgument;
+    return solve_entry(instance, parameters, options, warm_x, warm_y, nullptr, true,
+                       [&](auto& image, const PraqHostInstance& s, hipStream_t stream) {
+                           return image.outputs(s, x, y, stream);
+                       },
+                       info);
+}
+
+extern "C" int praq_solve_device(const PraqInstance* instance, const PraqParameters* parameters,
+                                 const PraqOptions* options, double* x, double* y,
+                                 PraqInfo* info, void* stream_handle) {
+    clear_info(info);
+    if (!instance || !parameters || !x || !params_valid(*parameters)) return kErrorArgument;
+    if (options && (options->tier < PRAQ_TIER_AUTO || options->tier > PRAQ_TIER_DEVICE))
+        return kErrorArgument;
+    const int64_t n = instance->n, m = instance->m;
+    if (n <= 0 || m < 0 || !instance->diag || (m > 0 && (!instance->rows || !instance->cols ||
+                                                         !instance->vals)))
+        return kErrorInstance;
+    if (n > std::numeric_limits<int>::max() || m > std::numeric_limits<int>::max() / 2)
+        return kErrorInstance;
+    hipStream_t stream = static_cast<hipStream_t>(stream_handle);
+    // the COO instance to the host for the O(n + m) CSR build
+    std::vector<double> diag(n), vals(m);
+    std::vector<int64_t> rows(m), cols(m);
+    REFORM_HIP_CHECK(hipMemcpyAsync(diag.data(), instance->diag, sizeof(double) * n,
+                                    hipMemcpyDeviceToHost, stream));
+    if (m > 0) {
+        REFORM_HIP_CHECK(hipMemcpyAsync(rows.data(), instance->rows, sizeof(int64_t) * m,
+                                        hipMemcpyDeviceToHost, stream));
+        REFORM_HIP_CHECK(hipMemcpyAsync(cols.data(), instance->cols, sizeof(int64_t) * m,
+                                        hipMemcpyDeviceToHost, stream));
+        REFORM_HIP_CHECK(hipMemcpyAsync(vals.data(), instance->vals, sizeof(double) * m,
+                                        hipMemcpyDeviceToHost, stream));
+    }
+    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+    const PraqInstance host{n, m, diag.data(), rows.data(), cols.data(), vals.data()};
+    return solve_entry(&host, parameters, options, nullptr, nullptr, stream, false,
+                       [&](auto& image, const PraqHostInstance& s, hipStream_t on) {
+                           return image.outputs_device(s, x, y, on);
+                       },
+                       info);
 }
 
 extern "C" int praq_solve(const PraqInstance* instance, const PraqParameters* parameters,

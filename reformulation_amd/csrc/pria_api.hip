@@ -4,9 +4,11 @@
 #include "pria_hip.h"
 #include "pria_host.hpp"
 
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 using namespace reform;
@@ -50,6 +52,125 @@ bool params_valid(const pria_params* p) {
 }
 
 
+// An instance whose arrays are in device memory: order_costs [n] and the
+// dense holding costs [n][t][t] (upper triangle read), both FP64.
+struct PriaDeviceInput {
+    const double* k = nullptr;
+    const double* h = nullptr;
+    int t = 0;
+};
+
+constexpr int kDevBlock = 256;
+constexpr int kDevScaleBlocks = 512;
+
+// Sum of squares of the upper triangles of h's rows (i, a); rows are dealt
+// to blocks as r = blockIdx.x + q gridDim.x, so the order of additions is
+// fixed by the grid.
+__global__ void __launch_bounds__(kDevBlock) pria_sumsq_rows(const double* h, int64_t rows, int t,
+                                                             double* partial) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int a = int(r % t);
+        const double* row = h + r * t;
+        for (int b = a + threadIdx.x; b < t; b += blockDim.x) acc += row[b] * row[b];
+    }
+    acc = block_sum1(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// ||(H, K, K0)||^2: the row partials, then K, then K0 (one block).
+__global__ void __launch_bounds__(kDevBlock) pria_sumsq_total(const double* k, int n, double k0,
+                                                              const double* partial, int parts,
+                                                              double* out) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < parts; p += blockDim.x) acc += partial[p];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += k[i] * k[i];
+    acc = block_sum1(acc, red);
+    if (threadIdx.x == 0) out[0] = acc + k0 * k0;
+}
+
+// The scaled image from device inputs: one block per row (i, a) of h, its
+// upper triangle to slice order (pria_build_host + pria_fill_image); block
+// (i, 0) also writes K_i / scale.
+template <class Real>
+__global__ void __launch_bounds__(kDevBlock) pria_gather_rows(const double* k, const double* h,
+                                                              int N, int t, int T, double scale,
+                                                              Real* ak, Real* ah) {
+    const int64_t r = blockIdx.x;
+    const int i = int(r / t), a = int(r % t);
+    const double* row = h + r * t;
+    Real* dst = ah + pria_off(N, T, a) + int64_t(i) * (T - a) - a;
+    for (int b = a + threadIdx.x; b < t; b += blockDim.x) dst[b] = Real(row[b] / scale);
+    if (a == 0 && threadIdx.x == 0) ak[i] = Real(k[i] / scale);
+}
+
+// Slice order -> the caller's dense [n][t][t] (pria_unpack): one block per
+// row (i, a); which 1: y / K_i, 2: z / K0 (0 where that budget is 0).
+template <class Real>
+__global__ void __launch_bounds__(kDevBlock) pria_scatter_rows(const Real* v, const double* k,
+                                                               double k0s, int which, int N, int t,
+                                                               int T, double scale, double* out) {
+    const int64_t r = blockIdx.x;
+    const int i = int(r / t), a = int(r % t);
+    double norm = 1.0;
+    if (which == 1) norm = k[i] / scale;
+    if (which == 2) norm = k0s;
+    const Real* src = v + pria_off(N, T, a) + int64_t(i) * (T - a) - a;
+    double* dst = out + r * t;
+    for (int b = threadIdx.x; b < t; b += blockDim.x) {
+        double val = 0.0;
+        if (b >= a) {
+            val = double(src[b]);
+            if (which) val = norm > 0.0 ? val / norm : 0.0;
+        }
+        dst[b] = val;
+    }
+}
+
+// primal_objective of dense x (pria_objective_host), per row (i, a): the
+// row's max over all t and its <h, x> over the upper triangle.
+__global__ void __launch_bounds__(kDevBlock) pria_objective_rows(const double* h, const double* x,
+                                                                 int t, double* row_max,
+                                                                 double* row_dot) {
+    __shared__ double red[16];
+    const int64_t r = blockIdx.x;
+    const int a = int(r % t);
+    const double* hr = h + r * t;
+    const double* xr = x + r * t;
+    double mx = -INFINITY, dot = 0.0;
+    for (int b = threadIdx.x; b < t; b += blockDim.x) {
+        mx = fmax(mx, xr[b]);
+        if (b >= a) dot += hr[b] * xr[b];
+    }
+    mx = block_max1(mx, red);
+    dot = block_sum1(dot, red);
+    if (threadIdx.x == 0) {
+        row_max[r] = mx;
+        row_dot[r] = dot;
+    }
+}
+
+// sum_{i,a} K_i max_b x + K0 sum_a max_{i,b} x + <H, x> (one block).
+__global__ void __launch_bounds__(kDevBlock) pria_objective_total(const double* k, double k0,
+                                                                  int N, int t,
+                                                                  const double* row_max,
+                                                                  const double* row_dot,
+                                                                  double* out) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int64_t r = threadIdx.x; r < int64_t(N) * t; r += blockDim.x)
+        acc += k[r / t] * row_max[r] + row_dot[r];
+    for (int a = threadIdx.x; a < t; a += blockDim.x) {
+        double mx = -INFINITY;
+        for (int i = 0; i < N; ++i) mx = fmax(mx, row_max[int64_t(i) * t + a]);
+        acc += k0 * mx;
+    }
+    acc = block_sum1(acc, red);
+    if (threadIdx.x == 0) out[0] = acc;
+}
+
 int pick_tier(const PriaHostInstance& s, int tier) {
     if (tier != PRIA_TIER_AUTO) return tier;
     return s.ND <= pria_wg_max() ? PRIA_TIER_WORKGROUP : PRIA_TIER_DEVICE;
@@ -63,7 +184,10 @@ struct PriaImage {
     PriaJob<Real> job;
     size_t job_off = 0, state_off = 0, work_off = 0;
 
-    int init(const PriaHostInstance& s, const LogicParams& lp, hipStream_t stream) {
+    // `din` set: K and h gathered on the device from the caller's arrays
+    // (cold start only); else the host image of `s` uploaded
+    int init(const PriaHostInstance& s, const LogicParams& lp, hipStream_t stream,
+             const PriaDeviceInput* din = nullptr) {
         L = pria_layout(s.N, s.T, s.ND, sizeof(Real));
         job_off = pria_align256(L.total_bytes);
         state_off = pria_align256(job_off + sizeof(PriaJob<Real>));
@@ -73,13 +197,27 @@ struct PriaImage {
             return kPriaErrorDevice;
         }
         char* base = static_cast<char*>(buf.ptr);
-        const size_t up = pria_upload_bytes(s, L);
-        std::vector<char> image(up, 0);
-        pria_fill_image<Real>(s, L, image.data());
-        if (hipMemcpyAsync(base, image.data(), up, hipMemcpyHostToDevice, stream) != hipSuccess)
-            return kPriaErrorDevice;
         std::memset(&job, 0, sizeof(job));
         job.a = pria_bind<Real>(L, base, s);
+        std::vector<char> image;
+        if (din) {
+            Real* k = reinterpret_cast<Real*>(base + L.off[PriaLayout::kK]);
+            Real* h = reinterpret_cast<Real*>(base + L.off[PriaLayout::kH]);
+            if (s.T != din->t &&  // padded entries of h are zero
+                hipMemsetAsync(h, 0, sizeof(Real) * size_t(s.ND), stream) != hipSuccess)
+                return kPriaErrorDevice;
+            hipLaunchKernelGGL(pria_gather_rows<Real>, dim3(unsigned(int64_t(s.N) * din->t)),
+                               dim3(kDevBlock), 0, stream, din->k, din->h, s.N, din->t, s.T,
+                               s.scale, k, h);
+            if (hipGetLastError() != hipSuccess) return kPriaErrorDevice;
+        } else {
+            const size_t up = pria_upload_bytes(s, L);
+            image.assign(up, 0);
+            pria_fill_image<Real>(s, L, image.data());
+            if (hipMemcpyAsync(base, image.data(), up, hipMemcpyHostToDevice, stream) !=
+                hipSuccess)
+                return kPriaErrorDevice;
+        }
         if (s.cold && pria_cold_start<Real>(job.a, stream)) return kPriaErrorDevice;
         job.p = lp;
         job.scale = s.scale;
@@ -128,34 +266,34 @@ struct PriaImage {
     }
 };
 
-// FP64 iterates, or FP32 iterates handing over to FP64 when they stall.
-// Outputs in slice order (FP64).
+// FP64 iterates, or FP32 iterates handing over to FP64 when they stall; all
+// work on `stream`.  `emit(image)` takes the answer of the image the solve
+// finished in.
+template <class Emit>
 int solve_mixed(const PriaHostInstance& s, const LogicParams& lp, const pria_options& o,
-                bool fp32, std::vector<double>* out[3], pria_info* info) {
-    Stream stream;
-    if (hipStreamCreateWithFlags(&stream.s, hipStreamNonBlocking) != hipSuccess)
-        return kPriaErrorDevice;
+                bool fp32, const PriaDeviceInput* din, hipStream_t stream, Emit&& emit,
+                pria_info* info) {
     const int tier = pick_tier(s, o.tier);
     LogicState st;
     std::memset(&st, 0, sizeof(st));
     if (!fp32) {
         PriaImage<double> d;
-        if (d.init(s, lp, stream.s) || d.run(tier, st, o.time_limit, false, s.T, stream.s) ||
-            d.outputs(s, out, stream.s))
+        if (d.init(s, lp, stream, din) || d.run(tier, st, o.time_limit, false, s.T, stream) ||
+            emit(d))
             return kPriaErrorDevice;
     } else {
         LogicParams lp32 = lp;
         apply_fp32_policy(lp32);
         PriaImage<float> f;
-        if (f.init(s, lp32, stream.s) || f.run(tier, st, o.time_limit, false, s.T, stream.s))
+        if (f.init(s, lp32, stream, din) || f.run(tier, st, o.time_limit, false, s.T, stream))
             return kPriaErrorDevice;
         if (st.stalled) {
             PriaImage<double> d;
             st.switch_iteration = st.iteration;
-            if (d.init(s, lp, stream.s) || pria_convert_state(f.job.a, d.job.a, stream.s) ||
-                d.run(tier, st, o.time_limit, true, s.T, stream.s) || d.outputs(s, out, stream.s))
+            if (d.init(s, lp, stream, din) || pria_convert_state(f.job.a, d.job.a, stream) ||
+                d.run(tier, st, o.time_limit, true, s.T, stream) || emit(d))
                 return kPriaErrorDevice;
-        } else if (f.outputs(s, out, stream.s)) {
+        } else if (emit(f)) {
             return kPriaErrorDevice;
         }
     }
@@ -195,7 +333,11 @@ int solve_f64io(int64_t n, int64_t t, const double* k, double k0, const double* 
     std::vector<double> xd(size_t(n) * t * t);
     std::vector<double> vx, vy, vz;
     std::vector<double>* outs[3] = {&vx, y ? &vy : nullptr, z ? &vz : nullptr};
-    int status = solve_mixed(s, lp, o, use_float32 != 0, outs, out);
+    Stream stream;
+    if (hipStreamCreateWithFlags(&stream.s, hipStreamNonBlocking) != hipSuccess)
+        return kPriaErrorDevice;
+    int status = solve_mixed(s, lp, o, use_float32 != 0, nullptr, stream.s,
+                             [&](auto& image) { return image.outputs(s, outs, stream.s); }, out);
     if (status == 0) {
         pria_unpack(s, vx, 0, xd.data());
         if (y) pria_unpack(s, vy, 1, y);
@@ -363,6 +505,100 @@ extern "C" int pria_solve_ex(int64_t n, int64_t t, const double* order_costs, do
                              double* z, pria_info* info) {
     return solve_f64io(n, t, order_costs, joint_cost, holding_costs, params, options, use_float32,
                        warm_x, warm_y, warm_z, x, y, z, info);
+}
+
+namespace {
+
+struct DeviceScratch {
+    void* ptr = nullptr;
+    ~DeviceScratch() {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
+}  // namespace
+
+extern "C" int pria_solve_device(int64_t n, int64_t t, const double* order_costs,
+                                 double joint_cost, const double* holding_costs,
+                                 const pria_params* params, const pria_options* options,
+                                 int use_float32, double* x, double* y, double* z,
+                                 pria_info* info, void* stream_handle) {
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
+        info->status = kPriaErrorArgument;
+    }
+    if (!params_valid(params) || n <= 0 || t <= 0 || !order_costs || !holding_costs)
+        return kPriaErrorArgument;
+    const int64_t T = t + (t & 1);
+    const int64_t nd = n * T * (T + 1) / 2;
+    if (nd >= (int64_t(1) << 31) || T > 640) return kPriaErrorArgument;
+    hipStream_t stream = static_cast<hipStream_t>(stream_handle);
+    const pria_options o = options ? *options : pria_default_options();
+    const int64_t rows = n * t;
+    // scratch: scale partials, the objective's row maxima and dots, and x
+    // when the caller takes none
+    const size_t f_partial = 0, f_total = kDevScaleBlocks, f_max = f_total + 1,
+                 f_dot = f_max + size_t(rows), f_x = f_dot + size_t(rows);
+    const size_t doubles = f_x + (x ? 0 : size_t(rows) * t);
+    DeviceScratch scratch;
+    if (hipMalloc(&scratch.ptr, sizeof(double) * doubles) != hipSuccess) {
+        scratch.ptr = nullptr;
+        return kPriaErrorDevice;
+    }
+    double* w = static_cast<double*>(scratch.ptr);
+    hipLaunchKernelGGL(pria_sumsq_rows, dim3(kDevScaleBlocks), dim3(kDevBlock), 0, stream,
+                       holding_costs, rows, int(t), w + f_partial);
+    hipLaunchKernelGGL(pria_sumsq_total, dim3(1), dim3(kDevBlock), 0, stream, order_costs,
+                       int(n), joint_cost, w + f_partial, kDevScaleBlocks, w + f_total);
+    double sq = 0.0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&sq, w + f_total, sizeof(double), hipMemcpyDeviceToHost, stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return kPriaErrorDevice;
+    PriaHostInstance s;  // shape and scale only: the arrays stay on the device
+    s.N = int(n);
+    s.T = int(T);
+    s.Torig = int(t);
+    s.ND = int(nd);
+    s.scale = sq > 0.0 ? std::sqrt(sq) : 1.0;
+    s.k0 = joint_cost / s.scale;
+    s.cold = true;
+    const PriaDeviceInput din{order_costs, holding_costs, int(t)};
+    const LogicParams lp = pria_logic_params(*params);
+    pria_info local;
+    pria_info* out = info ? info : &local;
+    double* xd = x ? x : w + f_x;
+    const dim3 grid{static_cast<unsigned>(rows)}, blk{unsigned(kDevBlock)};
+    auto emit = [&](auto& image) -> int {
+        using Real = std::remove_reference_t<decltype(*image.job.a.x)>;
+        hipLaunchKernelGGL(pria_scatter_rows<Real>, grid, blk, 0, stream, image.job.a.x,
+                           order_costs, s.k0, 0, s.N, int(t), s.T, s.scale, xd);
+        if (y)
+            hipLaunchKernelGGL(pria_scatter_rows<Real>, grid, blk, 0, stream, image.job.a.y,
+                               order_costs, s.k0, 1, s.N, int(t), s.T, s.scale, y);
+        if (z)
+            hipLaunchKernelGGL(pria_scatter_rows<Real>, grid, blk, 0, stream, image.job.a.z,
+                               order_costs, s.k0, 2, s.N, int(t), s.T, s.scale, z);
+        return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+    };
+    int status = solve_mixed(s, lp, o, use_float32 != 0, &din, stream, emit, out);
+    if (status) {
+        out->status = status;
+        return status;
+    }
+    // the upper bound of the returned point on the caller's instance
+    hipLaunchKernelGGL(pria_objective_rows, grid, blk, 0, stream, holding_costs, xd, int(t),
+                       w + f_max, w + f_dot);
+    hipLaunchKernelGGL(pria_objective_total, dim3(1), blk, 0, stream, order_costs, joint_cost,
+                       int(n), int(t), w + f_max, w + f_dot, w + f_total);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&out->upper, w + f_total, sizeof(double), hipMemcpyDeviceToHost,
+                       stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return kPriaErrorDevice;
+    return out->status == 0 ? 0 : 1;
 }
 
 extern "C" int pria_solve_f64(int64_t n, int64_t t, const double* order_costs, double joint_cost,

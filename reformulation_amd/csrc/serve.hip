@@ -26,6 +26,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -146,6 +147,7 @@ struct Request {
     std::vector<float> f1, f2, f3;
     int status = kPending;
     double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
+    double time_limit = 0.0;  // seconds from admission; <= 0: none
 };
 
 }  // namespace
@@ -408,6 +410,17 @@ int reform_server::rebuild(int family) {
         }
         if (pria_engine.setup(h64, f32 ? d32 : nullptr, d64)) return kErrorDevice;
     }
+    // the members' own time limits, on the server's clock
+    std::vector<double> due(count, std::numeric_limits<double>::infinity());
+    bool any = false;
+    for (size_t q = 0; q < count; ++q)
+        if (mem[q]->time_limit > 0.0) {
+            due[q] = mem[q]->t_admit + mem[q]->time_limit;
+            any = true;
+        }
+    if (!any) due.clear();
+    WideControl& w = family == 0 ? static_cast<WideControl&>(praq_engine) : pria_engine;
+    w.set_deadlines(std::move(due));
     changed[family] = false;
     return 0;
 }
@@ -493,6 +506,9 @@ int reform_server::launch(int f) {
         REFORM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&flags_host[f]),
                                        sizeof(int) * flags_cap[f], hipHostMallocDefault));
     }
+    // members past their own time limits stop at their current points; the
+    // round then passes them by and the done flags retire them (status 2)
+    if (w.expire_due(s, seconds_since(t0)) < 0) return kErrorDevice;
     w.use_graph = stable[f] >= 2;
     if (w.launch_round(s)) return kErrorDevice;
     ++stable[f];
@@ -686,6 +702,7 @@ extern "C" int64_t reform_server_submit_praq(reform_server* server, const PraqIn
     if (p.op_steps < 2) p.op_steps = 2;
     r->po = options ? *options : praq_default_options();
     r->lp = praq_logic_params(p, r->po);
+    r->time_limit = r->po.time_limit;
     return enqueue(server, std::move(r));
 }
 
@@ -695,9 +712,9 @@ extern "C" int64_t reform_server_submit_pria(reform_server* server, int64_t n, i
                                              const pria_options* options) {
     if (!server || !params || n <= 0 || t <= 0 || !order_costs || !holding_costs)
         return kErrorArgument;
-    (void)options;
     auto r = std::make_unique<Request>();
     r->family = 1;
+    if (options) r->time_limit = options->time_limit;
     r->n = n;
     r->t = t;
     r->k = order_costs;

@@ -169,12 +169,14 @@ __global__ void wide_promote(const Job* jobs, int count) {
     }
 }
 
-// One thread per instance: out of time.  The answer is the last step's input
-// after a warm block or a restart, else the step point; as the device tier.
+// One thread per instance: out of time (every instance, or those with a
+// nonzero mask entry).  The answer is the last step's input after a warm
+// block or a restart, else the step point; as the device tier.
 template <class Job>
-__global__ void wide_expire(const Job* jobs, int count) {
+__global__ void wide_expire(const Job* jobs, int count, const int* mask) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= count) return;
+    if (mask && !mask[inst]) return;
     LogicState* st = jobs[inst].state;
     if (st->action == kActDoneX || st->action == kActDoneSX) return;
     if (st->action == kActNew) {  // never started: its start point is the answer
@@ -533,6 +535,7 @@ int WideRun::begin(WideControl* engine, hipStream_t s, double limit) {
     t0 = std::chrono::steady_clock::now();
     finished = true;
     if (!w || w->count == 0) return 0;
+    w->expired.assign(w->deadlines.size(), 0);
     if (w->start(stream)) return kErrorDevice;
     // the round graph (timed rounds are launched directly instead: events
     // recorded by graph nodes carry no timestamps for hipEventElapsedTime)
@@ -579,6 +582,11 @@ int WideRun::advance() {
         finished = true;
         stats.seconds = elapsed;
         return 1;
+    }
+    // instances past their own limits (again only once a round has run)
+    if (!w->deadlines.empty() && retired > 0) {
+        const int n = w->expire_due(stream, elapsed);
+        if (n < 0) return n;
     }
     while (queued - retired < 2) {
         const int slot = queued & 1;
@@ -630,6 +638,33 @@ int WideControl::solve(hipStream_t stream, double time_limit, WideStats* stats) 
 }
 
 // ---------------------------------------------------------------- control --
+
+void WideControl::set_deadlines(std::vector<double> d) {
+    deadlines = std::move(d);
+    expired.assign(deadlines.size(), 0);
+}
+
+// The mask is copied from the host latches on the stream (pageable memory:
+// the copy may read them after this call returns).  The latches only ever
+// go from 0 to 1, and only once their deadline has passed, so a copy that
+// sees a later write stops an instance that is already due, nothing else.
+int WideControl::expire_due(hipStream_t stream, double now) {
+    const int n = std::min(count, int(deadlines.size()));
+    expired.resize(deadlines.size(), 0);
+    int due = 0;
+    for (int q = 0; q < n; ++q)
+        if (!expired[q] && now > deadlines[q]) {
+            expired[q] = 1;
+            ++due;
+        }
+    if (!due) return 0;
+    if (n < count)
+        REFORM_HIP_CHECK(hipMemsetAsync(expire_mask + n, 0, sizeof(int) * size_t(count - n), stream));
+    REFORM_HIP_CHECK(hipMemcpyAsync(expire_mask, expired.data(), sizeof(int) * size_t(n),
+                                    hipMemcpyHostToDevice, stream));
+    if (expire(stream, expire_mask)) return kErrorDevice;
+    return due;
+}
 
 WideControl::~WideControl() {
     release();
@@ -705,7 +740,7 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     const size_t o_partial = at;
     at = align256(at + sizeof(double) * partial_width * (partial_rows + 1));
     const size_t o_ctrl = at;
-    at = align256(at + sizeof(int) * (2 * size_t(count) + 1));
+    at = align256(at + sizeof(int) * (3 * size_t(count) + 1));
     // the previous arena serves while it is large enough (tables rebuilt
     // between rounds); a larger one grows by half again
     if (!arena || at > bytes) {
@@ -725,6 +760,7 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     round_action = reinterpret_cast<int*>(b + o_ctrl);
     pending = round_action + count;
     active = pending + count;
+    expire_mask = active + 1;
     if (!host_active) {
         hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host_active), 2 * sizeof(int),
                                      hipHostMallocDefault);
@@ -856,10 +892,10 @@ int WidePraq::enqueue_round(hipStream_t stream) {
     return 0;
 }
 
-int WidePraq::expire(hipStream_t stream) {
+int WidePraq::expire(hipStream_t stream, const int* mask) {
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_expire<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream,
-                       jobs64, count);
+                       jobs64, count, mask);
     hipLaunchKernelGGL(wpraq_copy, dim3(node_blocks + edge_blocks), blk, 0, stream, jobs32,
                        jobs64, node_tab, node_blocks, edge_tab, round_action, pending, 2);
     hipLaunchKernelGGL(wide_count<PraqJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
@@ -993,10 +1029,10 @@ int WidePria::enqueue_round(hipStream_t stream) {
     return 0;
 }
 
-int WidePria::expire(hipStream_t stream) {
+int WidePria::expire(hipStream_t stream, const int* mask) {
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_expire<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream,
-                       jobs64, count);
+                       jobs64, count, mask);
     hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
                        round_action, pending, 2);
     hipLaunchKernelGGL(wide_count<PriaJob<double>>, dim3(1), blk, 0, stream, jobs64, count,

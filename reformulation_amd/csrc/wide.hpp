@@ -71,6 +71,7 @@ struct WideControl {
     int* round_action = nullptr;   // [count] the action this round takes
     int* pending = nullptr;        // [count] anchor copy due this round
     int* active = nullptr;         // [1] instances not finished
+    int* expire_mask = nullptr;    // [count] instances stopped by their own time limit
     int* host_active = nullptr;    // [2] pinned
     hipEvent_t polled[2] = {nullptr, nullptr};
     void* arena = nullptr;
@@ -95,6 +96,12 @@ struct WideControl {
     int64_t launches[kKinds] = {0, 0};
     // algorithmic bytes of one step per instance: [kind][precision][instance]
     std::vector<double> step_bytes[kKinds][2];
+    // Per-instance time limits, in table order: the clock reading (seconds,
+    // on the owner's clock) past which the instance is stopped at its
+    // current point; +inf for none, empty when no instance has a limit.
+    // `expired` latches the instances already stopped.
+    std::vector<double> deadlines;
+    std::vector<int> expired;
 
     virtual ~WideControl();
     // every instance of the tables cold-starts at its next round (states set
@@ -102,8 +109,16 @@ struct WideControl {
     virtual int start(hipStream_t stream) = 0;
     // one round's launches (captured into the graph)
     virtual int enqueue_round(hipStream_t stream) = 0;
-    // out of time: every active instance ends at its current point
-    virtual int expire(hipStream_t stream) = 0;
+    // out of time: every active instance (mask null) or those with a
+    // nonzero mask entry (device, [count]) end at their current point
+    virtual int expire(hipStream_t stream, const int* mask = nullptr) = 0;
+    // the instances whose deadline lies before `now` and that have not been
+    // stopped yet: stopped (expire with the latched mask) on the stream;
+    // returns < 0 on error, else how many
+    int expire_due(hipStream_t stream, double now);
+    // set the deadlines (table order) and clear the latches; with no work of
+    // this engine's in flight
+    void set_deadlines(std::vector<double> d);
     // one round: the captured graph replayed (captured at the first call
     // after a setup), or launched directly when use_graph is off
     int launch_round(hipStream_t stream);
@@ -160,7 +175,7 @@ struct WidePraq : WideControl {
     int step(hipStream_t stream, int mode, int j);
     int start(hipStream_t stream) override;
     int enqueue_round(hipStream_t stream) override;
-    int expire(hipStream_t stream) override;
+    int expire(hipStream_t stream, const int* mask = nullptr) override;
 };
 
 // PRIA: first = (slice range, column-tile range) per instance; partial rows
@@ -179,7 +194,7 @@ struct WidePria : WideControl {
     int step(hipStream_t stream, int mode, int j);
     int start(hipStream_t stream) override;
     int enqueue_round(hipStream_t stream) override;
-    int expire(hipStream_t stream) override;
+    int expire(hipStream_t stream, const int* mask = nullptr) override;
 };
 
 }  // namespace reform

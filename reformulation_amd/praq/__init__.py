@@ -190,11 +190,7 @@ class QuboProblem:
         return float(self.diag @ x + edge.sum())
 
 
-def _solve(instance: QuboProblem, params: Parameters, warm):
-    L = _native.lib()
-    n, m = instance.n(), instance.m()
-    inst = _native.PraqInstance(n, m, _native.dptr(instance.diag), _native.iptr(instance.rows),
-                                _native.iptr(instance.cols), _native.dptr(instance.vals))
+def _native_params(L, params: Parameters, dense: bool):
     p = L.praq_default_parameters()
     p.tolerance = params.tolerance
     p.max_iter = params.max_iter if params.max_iter > 0 else -1
@@ -213,11 +209,20 @@ def _solve(instance: QuboProblem, params: Parameters, warm):
     o = L.praq_default_options()
     o.use_halpern = int(params.use_halpern)
     o.warm_steps = params.warm_steps
-    dense = instance.dense if params.dense_degree is None else params.dense_degree
     o.dense_degree = int(bool(dense))
     o.tier = params.tier
     o.eta = params.eta
     o.time_limit = params.time_limit
+    return p, o
+
+
+def _solve(instance: QuboProblem, params: Parameters, warm):
+    L = _native.lib()
+    n, m = instance.n(), instance.m()
+    inst = _native.PraqInstance(n, m, _native.dptr(instance.diag), _native.iptr(instance.rows),
+                                _native.iptr(instance.cols), _native.dptr(instance.vals))
+    dense = instance.dense if params.dense_degree is None else params.dense_degree
+    p, o = _native_params(L, params, dense)
     x = np.zeros(n)
     y = np.zeros((max(m, 1), 2))
     wx = wy = None
@@ -231,6 +236,41 @@ def _solve(instance: QuboProblem, params: Parameters, warm):
     if status < 0:
         raise ValueError(f"praq_solve_ex failed with status {status}")
     return x, y[:m], info
+
+
+def solve_device(diag, rows, cols, vals, params: Parameters, x, y=None, stream=None):
+    """One solve over device tensors (`praq_solve_device`): diag [n] and vals
+    [m] float64, rows and cols [m] int64 (i < j), the answer written into the
+    float64 tensors x [n] and y [m, 2] (y optional; dual in instance units),
+    every launch and copy on `stream` (a torch stream; None: the current one).
+    The sparse build (true degrees) unless params.dense_degree is set.
+    Returns the PraqInfo record."""
+    import torch
+    L = _native.lib()
+    n, m = int(diag.numel()), int(rows.numel())
+    for name, t, dtype in (("diag", diag, torch.float64), ("rows", rows, torch.int64),
+                           ("cols", cols, torch.int64), ("vals", vals, torch.float64),
+                           ("x", x, torch.float64)):
+        if not (t.is_cuda and t.dtype == dtype and t.is_contiguous()):
+            raise ValueError(f"{name} must be a contiguous {dtype} device tensor")
+    if y is not None and not (y.is_cuda and y.dtype == torch.float64 and y.is_contiguous()
+                              and y.numel() == 2 * m):
+        raise ValueError("y must be a contiguous float64 device tensor of shape [m, 2]")
+    if x.numel() != n or cols.numel() != m or vals.numel() != m:
+        raise ValueError("inconsistent sizes")
+    p, o = _native_params(L, params, bool(params.dense_degree))
+    inst = _native.PraqInstance(n, m, ctypes.cast(diag.data_ptr(), _native._dp),
+                                ctypes.cast(rows.data_ptr(), _native._ip),
+                                ctypes.cast(cols.data_ptr(), _native._ip),
+                                ctypes.cast(vals.data_ptr(), _native._dp))
+    s = stream if stream is not None else torch.cuda.current_stream(x.device)
+    info = _native.PraqInfo()
+    status = L.praq_solve_device(ctypes.byref(inst), ctypes.byref(p), ctypes.byref(o),
+                                 x.data_ptr(), None if y is None else y.data_ptr(),
+                                 ctypes.byref(info), s.cuda_stream)
+    if status < 0:
+        raise ValueError(f"praq_solve_device failed with status {status}: {_native.last_error()}")
+    return info
 
 
 def _pairs_to_edges(instance: QuboProblem, dual) -> np.ndarray:
@@ -301,5 +341,5 @@ def persistent_variables(primal, tolerance: float = 1e-6):
 
 
 __all__ = ["Parameters", "QuboProblem", "check_interval", "solve_qubo_problem",
-           "solve_qubo_problem_warm_start", "solve_with_info", "relaxation_bounds",
+           "solve_qubo_problem_warm_start", "solve_with_info", "solve_device", "relaxation_bounds",
            "persistent_variables", "TIER_AUTO", "TIER_WORKGROUP", "TIER_DEVICE"]

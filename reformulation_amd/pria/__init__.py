@@ -173,6 +173,43 @@ def _solve(instance: JrpProblem, params: Parameters, warm):
     return x, y, z, info
 
 
+def solve_device(order_costs, joint_order_cost: float, holding_costs, params: Parameters,
+                 x=None, y=None, z=None, stream=None):
+    """One cold-start solve over device tensors (`pria_solve_device`):
+    order_costs [n] and holding_costs [n, t, t] float64 (upper triangle
+    read); the answer written into the float64 tensors x, y, z [n, t, t] (each
+    optional; y, z normalised as solve_jrp_problem returns them), every launch
+    and copy on `stream` (a torch stream; None: the current one).  Returns the
+    PriaInfo record."""
+    import torch
+    L = _native.lib()
+    if holding_costs.dim() != 3 or holding_costs.shape[1] != holding_costs.shape[2]:
+        raise ValueError("holding_costs must have shape [n, t, t]")
+    n, t = int(holding_costs.shape[0]), int(holding_costs.shape[1])
+    for name, a, count in (("order_costs", order_costs, n), ("holding_costs", holding_costs,
+                                                               n * t * t),
+                           ("x", x, n * t * t), ("y", y, n * t * t), ("z", z, n * t * t)):
+        if a is None:
+            continue
+        if not (a.is_cuda and a.dtype == torch.float64 and a.is_contiguous()
+                and a.numel() == count):
+            raise ValueError(f"{name} must be a contiguous float64 device tensor of {count}")
+    p = params.native()
+    o = L.pria_default_options()
+    o.tier = params.tier
+    o.time_limit = params.time_limit
+    s = stream if stream is not None else torch.cuda.current_stream(holding_costs.device)
+    ptr = [None if a is None else a.data_ptr() for a in (x, y, z)]
+    info = _native.PriaInfo()
+    status = L.pria_solve_device(n, t, order_costs.data_ptr(), float(joint_order_cost),
+                                 holding_costs.data_ptr(), ctypes.byref(p), ctypes.byref(o),
+                                 int(params.use_float32), ptr[0], ptr[1], ptr[2],
+                                 ctypes.byref(info), s.cuda_stream)
+    if status < 0:
+        raise ValueError(f"pria_solve_device failed with status {status}: {_native.last_error()}")
+    return info
+
+
 def solve_jrp_problem_warm_start(instance: JrpProblem, params: Parameters, warm_start):
     """(x, y, z, iterations, lower, upper) from a warm start whose duals are
     normalised as the solver returns them (solver.py:608-724)."""
@@ -191,5 +228,5 @@ def solve_with_info(instance: JrpProblem, params: Parameters, warm_start=None):
 
 
 __all__ = ["Parameters", "JrpProblem", "solve_jrp_problem", "solve_jrp_problem_warm_start",
-           "solve_with_info", "relaxation_bounds", "primal_objective", "TIER_AUTO",
+           "solve_with_info", "solve_device", "relaxation_bounds", "primal_objective", "TIER_AUTO",
            "TIER_WORKGROUP", "TIER_DEVICE"]

@@ -167,6 +167,8 @@ class Server:
     def submit_pria(self, problem: _pria.JrpProblem, params: _pria.Parameters) -> int:
         p = params.native()
         o = self._L.pria_default_options()
+        o.tier = params.tier
+        o.time_limit = params.time_limit
         k = np.ascontiguousarray(problem.order_costs, dtype=np.float64)
         h = np.ascontiguousarray(problem.holding_costs, dtype=np.float64)
         t = self._L.reform_server_submit_pria(self._h, problem.n(), problem.t(), _native.dptr(k),

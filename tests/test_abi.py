@@ -89,6 +89,44 @@ def test_praq_rejects_invalid_arguments_before_any_device_work():
     assert lib.praq_solve(ctypes.byref(inst), ctypes.byref(bad), _native.dptr(x), ctypes.byref(res)) == -1
 
 
+def test_device_pointer_entries_validate_before_any_device_work():
+    """praq_solve_device / pria_solve_device reject bad shapes and parameters
+    before they touch a pointer (the pointers here are not device memory)."""
+    import ctypes
+    from reformulation_amd import _native
+    lib = _native.lib()
+    p = lib.praq_default_parameters()
+    o = lib.praq_default_options()
+    info = _native.PraqInfo()
+    fake = ctypes.cast(ctypes.c_void_p(4096), _native._dp)
+    ifake = ctypes.cast(ctypes.c_void_p(4096), _native._ip)
+    inst = _native.PraqInstance(0, 1, fake, ifake, ifake, fake)
+    call = lambda inst, p, x: lib.praq_solve_device(ctypes.byref(inst), ctypes.byref(p),
+                                                    ctypes.byref(o), x, None,
+                                                    ctypes.byref(info), None)
+    assert call(inst, p, 4096) == -2                       # n = 0
+    inst.n = 3
+    assert call(inst, p, None) == -1                       # no x
+    inst.rows = None
+    assert call(inst, p, 4096) == -2                       # m > 0 without rows
+    bad = lib.praq_default_parameters()
+    bad.omega = -1.0
+    assert call(inst, bad, 4096) == -1
+    assert info.status == -1 and info.lower != info.lower   # NaN bounds on error
+    q = lib.pria_params_auto(100, 1e-6)
+    po = lib.pria_default_options()
+    pinfo = _native.PriaInfo()
+    pcall = lambda n, t, q: lib.pria_solve_device(n, t, 4096, 1.0, 4096, ctypes.byref(q),
+                                                  ctypes.byref(po), 0, 4096, None, None,
+                                                  ctypes.byref(pinfo), None)
+    assert pcall(0, 10, q) == -1
+    assert pcall(2, 641, q) == -1                          # T' above 640
+    q.omega = 0.0
+    assert pcall(2, 10, q) == -1
+    assert lib.pria_solve_device(2, 10, None, 1.0, 4096, ctypes.byref(q), None, 0, None, None,
+                                 None, None, None) == -1
+
+
 def test_batch_validates_on_the_host_before_any_device_work():
     import numpy as np
     from reformulation_amd import praq, pria

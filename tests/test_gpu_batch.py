@@ -33,6 +33,13 @@ def _members():
 
 
 def _batch(members, f32, tol=1e-6):
+    b, index = _batch_open(members, f32, tol)
+    b.upload()
+    return b, index
+
+
+def _batch_open(members, f32, tol=1e-6):
+    """The batch with the members added, not yet uploaded."""
     from reformulation_amd import praq, pria
     from reformulation_amd.batch import Batch
     b = Batch(use_float32=f32)
@@ -46,7 +53,6 @@ def _batch(members, f32, tol=1e-6):
             k, k0, h = inst
             p = pria.JrpProblem.new(k, k0, h)
             index.append(("pria", b.add_pria(p, pria.Parameters.with_auto_step(50_000, tol))))
-    b.upload()
     return b, index
 
 
@@ -214,3 +220,42 @@ def test_special_cases_batched_and_served_match_the_oracle(f32):
         # the max-cut form certifies at its first check; FP32 iterates round
         # its bounds apart by more than 1e-8 and finish after the handover
         assert batched[3][2].iterations == 0
+
+
+def _limited():
+    """A PRIA instance that cannot reach its tolerance (1e-15) within its
+    time limit (0.05 s): N = 50, T = 200."""
+    from reformulation_amd import pria, workloads
+    j = workloads.pria_c1(900, 50, 200)
+    params = pria.Parameters.with_auto_step(100_000_000, 1e-15, time_limit=0.05)
+    return pria.JrpProblem.new(j.order_costs, j.joint_order_cost, j.holding_costs), params
+
+
+@pytest.mark.parametrize("f32", [False, True])
+@pytest.mark.parametrize("mode", ["batched", "sequential"])
+def test_instance_time_limit_stops_only_that_instance(f32, mode):
+    """An instance's own time limit (options.time_limit) ends it at its current
+    point with status 2; the other instances of the batch run to their
+    tolerance with the results they have without it."""
+    members = _members()
+    b0, index0 = _batch(members, f32)
+    b0.solve(mode)
+    reference = _results(b0, index0)
+    b0.close()
+    b, index = _batch_open(members, f32)
+    problem, params = _limited()
+    q_limited = b.add_pria(problem, params)
+    b.upload()
+    print(f"time limit: mode={mode} f32={f32}", flush=True)
+    b.solve(mode)
+    for q, (u, v) in enumerate(zip(_results(b, index), reference)):
+        assert u[2].status == 0, q
+        assert _same_bits(u[0], v[0]) and _same_bits(u[1], v[1]), q
+        assert u[2].iterations == v[2].iterations, q
+    x, y, z, info = b.pria_result(q_limited)
+    assert info.status == 2
+    assert 0 < info.iterations < params.max_iter
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(y)) and np.all(np.isfinite(z))
+    assert np.isfinite(info.lower) and np.isfinite(info.upper)
+    assert info.relative_gap > 1e-15
+    b.close()

@@ -159,3 +159,31 @@ def test_open_loop_driver_and_borrowed_inputs():
     for q, res in seen.items():
         _check_same(res, reference[q % len(items)], q)
     assert len(seen) == len(arrivals)
+
+
+def test_request_time_limit_stops_only_that_request():
+    """A request's own time limit counts from its admission: past it the
+    request finishes with status 2 at its current point, between two rounds;
+    the requests beside it return the batched answers."""
+    from reformulation_amd.serve import Server
+    from test_gpu_batch import _limited
+    members = _members()
+    b, index = _batch(members, True)
+    b.solve("batched")
+    reference = _results(b, index)
+    b.close()
+    items = _problems(members)
+    problem, params = _limited()
+    with Server(use_float32=True, host_threads=2) as srv:
+        t_limited = srv.submit_pria(problem, params)
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(120.0)
+        for q, ((kind, _, _), t) in enumerate(zip(items, tickets)):
+            _check_same(_served(srv, kind, t), reference[q], q)
+        info = srv.info(t_limited)
+        assert info.status == 2
+        assert 0 < info.iterations < params.max_iter
+        assert info.completed - info.admitted >= params.time_limit
+        x, y, z, rinfo = srv.pria_result(t_limited)
+        assert np.all(np.isfinite(x)) and np.all(np.isfinite(z))
+        assert np.isfinite(rinfo.lower) and np.isfinite(rinfo.upper)
