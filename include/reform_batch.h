@@ -94,6 +94,10 @@ typedef struct reform_kernel_stats {
     double bytes;        /* sum of their algorithmic bytes */
 } reform_kernel_stats;
 int reform_batch_set_timing(reform_batch* batch, int on);
+/* Batched mode: rebuild the block tables over the instances still running
+ * whenever at most half of a table's instances are (default on).  Results do
+ * not change; later rounds launch only the running instances' blocks. */
+int reform_batch_set_compaction(reform_batch* batch, int on);
 int reform_batch_kernel_stats(const reform_batch* batch, int family,
                               reform_kernel_stats out[2]);
 

@@ -85,6 +85,7 @@ EXPORTS = (
     "reform_batch_set_shape", "reform_batch_count", "reform_batch_kernel_ms",
     "reform_batch_device_bytes", "reform_batch_praq_result", "reform_batch_pria_result",
     "reform_last_error", "reform_batch_set_timing", "reform_batch_kernel_stats",
+    "reform_batch_set_compaction",
     "reform_batch_instance_seconds",
     "reform_tune_set", "reform_tune_get", "reform_tune_clear", "reform_praq_lanes_for",
     "reform_time_praq_steps",

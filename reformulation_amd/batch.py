@@ -39,6 +39,8 @@ def _declare(L):
     if getattr(L, "_batch_declared", False):
         return
     L.reform_batch_set_timing.restype = ctypes.c_int
+    L.reform_batch_set_compaction.restype = ctypes.c_int
+    L.reform_batch_set_compaction.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.reform_batch_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.reform_batch_kernel_stats.restype = ctypes.c_int
     L.reform_batch_kernel_stats.argtypes = [ctypes.c_void_p, ctypes.c_int,
@@ -179,6 +181,11 @@ class Batch:
         a, b = ctypes.c_double(), ctypes.c_double()
         self._L.reform_batch_kernel_ms(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def set_compaction(self, on: bool = True) -> None:
+        """Rebuild the batched mode's tables over the running instances as
+        they finish (default on); results are the same either way."""
+        self._L.reform_batch_set_compaction(self._h, int(on))
 
     def set_timing(self, on: bool = True) -> None:
         """Bracket every step launch of the batched mode with HIP events."""

@@ -149,6 +149,7 @@ struct BatchImpl {
     std::vector<hipStream_t> group_streams;    // streams beyond s0 / s1
     std::vector<hipEvent_t> group_events;      // (start, end) per engine
     bool timing = false;
+    bool compaction = true;
 
     hipStream_t stream_for(int family, int g) {
         if (g == 0) return family == 0 ? s0 : s1;
@@ -486,7 +487,7 @@ int BatchImpl::solve_wide() {
                 auto e = std::make_unique<WidePraq>();
                 e->set_timing(timing);
                 std::vector<PraqJob<double>> sub(praq64.host.begin() + lo, praq64.host.begin() + hi);
-                if (e->setup(sub, f32 ? praq32.dev + lo : nullptr, praq64.dev + lo))
+                if (e->setup(std::move(sub), f32 ? praq32.dev + lo : nullptr, praq64.dev + lo))
                     return kErrorDevice;
                 e->set_deadlines(deadlines(0, lo, hi));
                 wide[0].push_back(std::move(e));
@@ -494,7 +495,7 @@ int BatchImpl::solve_wide() {
                 auto e = std::make_unique<WidePria>();
                 e->set_timing(timing);
                 std::vector<PriaJob<double>> sub(pria64.host.begin() + lo, pria64.host.begin() + hi);
-                if (e->setup(sub, f32 ? pria32.dev + lo : nullptr, pria64.dev + lo))
+                if (e->setup(std::move(sub), f32 ? pria32.dev + lo : nullptr, pria64.dev + lo))
                     return kErrorDevice;
                 e->set_deadlines(deadlines(1, lo, hi));
                 wide[1].push_back(std::move(e));
@@ -527,6 +528,7 @@ int BatchImpl::solve_wide() {
         l.end = group_events[2 * k + 1];
         l.engine->clear_timing();
         REFORM_HIP_CHECK(hipEventRecord(l.begin, l.stream));
+        l.run.shrink = compaction;
         if (l.run.begin(l.engine, l.stream, time_limit)) return kErrorDevice;
     }
     for (;;) {
@@ -888,6 +890,12 @@ extern "C" int reform_batch_kernel_ms(const reform_batch* batch, double* praq_ms
     if (!batch) return kErrorArgument;
     if (praq_ms) *praq_ms = batch->impl->praq_ms;
     if (pria_ms) *pria_ms = batch->impl->pria_ms;
+    return 0;
+}
+
+extern "C" int reform_batch_set_compaction(reform_batch* batch, int on) {
+    if (!batch) return kErrorArgument;
+    batch->impl->compaction = on != 0;
     return 0;
 }
 

@@ -391,7 +391,7 @@ int reform_server::rebuild(int family) {
                 REFORM_HIP_CHECK(
                     hipMemcpy(d32, h32.data(), sizeof(h32[0]) * count, hipMemcpyHostToDevice));
         }
-        if (praq_engine.setup(h64, f32 ? d32 : nullptr, d64)) return kErrorDevice;
+        if (praq_engine.setup(std::move(h64), f32 ? d32 : nullptr, d64)) return kErrorDevice;
     } else {
         if (grow(sizeof(PriaJob<float>), sizeof(PriaJob<double>))) return kErrorDevice;
         std::vector<PriaJob<double>> h64(count);
@@ -408,7 +408,7 @@ int reform_server::rebuild(int family) {
                 REFORM_HIP_CHECK(
                     hipMemcpy(d32, h32.data(), sizeof(h32[0]) * count, hipMemcpyHostToDevice));
         }
-        if (pria_engine.setup(h64, f32 ? d32 : nullptr, d64)) return kErrorDevice;
+        if (pria_engine.setup(std::move(h64), f32 ? d32 : nullptr, d64)) return kErrorDevice;
     }
     // the members' own time limits, on the server's clock
     std::vector<double> due(count, std::numeric_limits<double>::infinity());

@@ -16,6 +16,8 @@ namespace {
 
 constexpr int kWideBlock = 256;
 constexpr int kRoundsPerPoll = 2;
+// tables of fewer instances are not compacted further
+constexpr int kShrinkMin = 8;
 
 int grid_for(int threads) { return (threads + kWideBlock - 1) / kWideBlock; }
 
@@ -186,6 +188,20 @@ __global__ void wide_expire(const Job* jobs, int count, const int* mask) {
     }
     st->action = st->last_action == kActHalpernBlock ? kActDoneSX : kActDoneX;
     st->status = 2;
+}
+
+// One thread per table position: 1 while the instance has not finished.
+template <class Job>
+__global__ void wide_live(const Job* jobs, int count, int* live) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < count) live[q] = jobs[q].state->action != kActDoneX;
+}
+
+// One thread per subset position: the job record of full-table position keep[q].
+template <class Job>
+__global__ void wide_gather_jobs(const Job* src, const int* keep, int n, Job* dst) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = src[keep[q]];
 }
 
 // One block: finished step points have been copied (DONE_SX -> DONE_X);
@@ -534,7 +550,9 @@ int WideRun::begin(WideControl* engine, hipStream_t s, double limit) {
     stats = WideStats();
     t0 = std::chrono::steady_clock::now();
     finished = true;
-    if (!w || w->count == 0) return 0;
+    if (!w) return 0;
+    if (w->shrunk() && w->restore()) return kErrorDevice;
+    if (w->count == 0) return 0;
     w->expired.assign(w->deadlines.size(), 0);
     if (w->start(stream)) return kErrorDevice;
     // the round graph (timed rounds are launched directly instead: events
@@ -579,9 +597,22 @@ int WideRun::advance() {
             if (w->expire(stream)) return kErrorDevice;
             REFORM_HIP_CHECK(hipStreamSynchronize(stream));
         }
+        // the full tables again, for the results' accounting and the next solve
+        if (w->shrunk() && w->restore()) return kErrorDevice;
         finished = true;
         stats.seconds = elapsed;
         return 1;
+    }
+    // compaction: once at most half the table is active, the tables are
+    // rebuilt over the active instances, after the queued rounds have run
+    if (shrink && retired > 0 && w->count >= kShrinkMin) {
+        const int active_now = w->host_active[(retired - 1) & 1];
+        if (active_now > 0 && 2 * active_now <= w->count) {
+            if (queued > retired) return changed;
+            if (w->shrink_to_live(stream)) return kErrorDevice;
+            ++stats.shrinks;
+            changed = 1;
+        }
     }
     // instances past their own limits (again only once a round has run)
     if (!w->deadlines.empty() && retired > 0) {
@@ -594,7 +625,7 @@ int WideRun::advance() {
         const int rounds = graph ? kRoundsPerPoll : 1;
         for (int r = 0; r < rounds; ++r) {
             if (graph) {
-                REFORM_HIP_CHECK(hipGraphLaunch(w->exec, stream));
+                if (w->launch_round(stream)) return kErrorDevice;  // captured after a shrink
             } else {  // the round's marks (if timing) in the group's event set
                 w->mark_set = slot;
                 w->marks_used[slot] = 0;
@@ -649,22 +680,88 @@ void WideControl::set_deadlines(std::vector<double> d) {
 // go from 0 to 1, and only once their deadline has passed, so a copy that
 // sees a later write stops an instance that is already due, nothing else.
 int WideControl::expire_due(hipStream_t stream, double now) {
-    const int n = std::min(count, int(deadlines.size()));
+    if (deadlines.empty() || !count) return 0;
     expired.resize(deadlines.size(), 0);
+    const int nd = int(deadlines.size());
+    auto full = [&](int q) { return table_index.empty() ? q : table_index[q]; };
     int due = 0;
-    for (int q = 0; q < n; ++q)
-        if (!expired[q] && now > deadlines[q]) {
-            expired[q] = 1;
+    for (int q = 0; q < count; ++q) {
+        const int f = full(q);
+        if (f < nd && !expired[f] && now > deadlines[f]) {
+            expired[f] = 1;
             ++due;
         }
+    }
     if (!due) return 0;
-    if (n < count)
-        REFORM_HIP_CHECK(hipMemsetAsync(expire_mask + n, 0, sizeof(int) * size_t(count - n), stream));
-    REFORM_HIP_CHECK(hipMemcpyAsync(expire_mask, expired.data(), sizeof(int) * size_t(n),
+    mask_host.resize(count);
+    for (int q = 0; q < count; ++q) {
+        const int f = full(q);
+        mask_host[q] = f < nd ? expired[f] : 0;
+    }
+    REFORM_HIP_CHECK(hipMemcpyAsync(expire_mask, mask_host.data(), sizeof(int) * size_t(count),
                                     hipMemcpyHostToDevice, stream));
     if (expire(stream, expire_mask)) return kErrorDevice;
     return due;
 }
+
+int WideControl::shrink_to_live(hipStream_t stream) {
+    if (live_flags(stream)) return kErrorDevice;
+    std::vector<int> flags(count);
+    REFORM_HIP_CHECK(hipMemcpyAsync(flags.data(), live, sizeof(int) * size_t(count),
+                                    hipMemcpyDeviceToHost, stream));
+    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+    std::vector<int> keep;
+    for (int q = 0; q < count; ++q)
+        if (flags[q]) keep.push_back(table_index.empty() ? q : table_index[q]);
+    if (keep.empty() || int(keep.size()) == count) return 0;
+    return shrink(stream, keep);
+}
+
+template <class J32, class J64>
+int WideSubset<J32, J64>::reserve(int n) {
+    if (n <= cap) return 0;
+    if (mem) (void)hipFree(mem);
+    mem = nullptr;
+    cap = 0;
+    const size_t o64 = align256(sizeof(J32) * size_t(n));
+    const size_t ok = o64 + align256(sizeof(J64) * size_t(n));
+    REFORM_HIP_CHECK(hipMalloc(&mem, ok + sizeof(int) * size_t(n)));
+    char* b = static_cast<char*>(mem);
+    jobs32 = reinterpret_cast<J32*>(b);
+    jobs64 = reinterpret_cast<J64*>(b + o64);
+    keep = reinterpret_cast<int*>(b + ok);
+    cap = n;
+    return 0;
+}
+
+template struct WideSubset<PraqJob<float>, PraqJob<double>>;
+template struct WideSubset<PriaJob<float>, PriaJob<double>>;
+
+namespace {
+
+// The subset's records gathered on the device; the host copies for the
+// table build.
+template <class J32, class J64>
+int gather_subset(WideSubset<J32, J64>& sub, const std::vector<J64>& full64, const J32* dev32,
+                  const J64* dev64, const std::vector<int>& keep, hipStream_t stream,
+                  std::vector<J64>& host64) {
+    const int n = int(keep.size());
+    if (sub.reserve(n)) return kErrorDevice;
+    REFORM_HIP_CHECK(hipMemcpyAsync(sub.keep, keep.data(), sizeof(int) * size_t(n),
+                                    hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(wide_gather_jobs<J64>, dim3(grid_for(n)), dim3(kWideBlock), 0, stream,
+                       dev64, sub.keep, n, sub.jobs64);
+    if (dev32)
+        hipLaunchKernelGGL(wide_gather_jobs<J32>, dim3(grid_for(n)), dim3(kWideBlock), 0, stream,
+                           dev32, sub.keep, n, sub.jobs32);
+    REFORM_HIP_CHECK(hipGetLastError());
+    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+    host64.resize(n);
+    for (int q = 0; q < n; ++q) host64[q] = full64[keep[q]];
+    return 0;
+}
+
+}  // namespace
 
 WideControl::~WideControl() {
     release();
@@ -740,7 +837,7 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     const size_t o_partial = at;
     at = align256(at + sizeof(double) * partial_width * (partial_rows + 1));
     const size_t o_ctrl = at;
-    at = align256(at + sizeof(int) * (3 * size_t(count) + 1));
+    at = align256(at + sizeof(int) * (4 * size_t(count) + 1));
     // the previous arena serves while it is large enough (tables rebuilt
     // between rounds); a larger one grows by half again
     if (!arena || at > bytes) {
@@ -761,6 +858,7 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     pending = round_action + count;
     active = pending + count;
     expire_mask = active + 1;
+    live = expire_mask + count;
     if (!host_active) {
         hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host_active), 2 * sizeof(int),
                                      hipHostMallocDefault);
@@ -777,8 +875,38 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
 
 // ------------------------------------------------------------- PRAQ host --
 
-int WidePraq::setup(const std::vector<PraqJob<double>>& host64, const PraqJob<float>* dev32,
+int WidePraq::setup(std::vector<PraqJob<double>> host64, const PraqJob<float>* dev32,
                     const PraqJob<double>* dev64) {
+    full64 = std::move(host64);
+    full_dev32 = dev32;
+    full_dev64 = dev64;
+    full_count = int(full64.size());
+    table_index.clear();
+    return build_tables(full64, dev32, dev64);
+}
+
+int WidePraq::shrink(hipStream_t stream, const std::vector<int>& keep) {
+    std::vector<PraqJob<double>> host64;
+    if (gather_subset(subset, full64, full_dev32, full_dev64, keep, stream, host64))
+        return kErrorDevice;
+    table_index = keep;
+    return build_tables(host64, full_dev32 ? subset.jobs32 : nullptr, subset.jobs64);
+}
+
+int WidePraq::restore() {
+    table_index.clear();
+    return build_tables(full64, full_dev32, full_dev64);
+}
+
+int WidePraq::live_flags(hipStream_t stream) {
+    hipLaunchKernelGGL(wide_live<PraqJob<double>>, dim3(grid_for(count)), dim3(kWideBlock), 0,
+                       stream, jobs64, count, live);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int WidePraq::build_tables(const std::vector<PraqJob<double>>& host64,
+                           const PraqJob<float>* dev32, const PraqJob<double>* dev64) {
     count = int(host64.size());
     jobs32 = dev32;
     jobs64 = dev64;
@@ -906,8 +1034,38 @@ int WidePraq::expire(hipStream_t stream, const int* mask) {
 
 // ------------------------------------------------------------- PRIA host --
 
-int WidePria::setup(const std::vector<PriaJob<double>>& host64, const PriaJob<float>* dev32,
+int WidePria::setup(std::vector<PriaJob<double>> host64, const PriaJob<float>* dev32,
                     const PriaJob<double>* dev64) {
+    full64 = std::move(host64);
+    full_dev32 = dev32;
+    full_dev64 = dev64;
+    full_count = int(full64.size());
+    table_index.clear();
+    return build_tables(full64, dev32, dev64);
+}
+
+int WidePria::shrink(hipStream_t stream, const std::vector<int>& keep) {
+    std::vector<PriaJob<double>> host64;
+    if (gather_subset(subset, full64, full_dev32, full_dev64, keep, stream, host64))
+        return kErrorDevice;
+    table_index = keep;
+    return build_tables(host64, full_dev32 ? subset.jobs32 : nullptr, subset.jobs64);
+}
+
+int WidePria::restore() {
+    table_index.clear();
+    return build_tables(full64, full_dev32, full_dev64);
+}
+
+int WidePria::live_flags(hipStream_t stream) {
+    hipLaunchKernelGGL(wide_live<PriaJob<double>>, dim3(grid_for(count)), dim3(kWideBlock), 0,
+                       stream, jobs64, count, live);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
+                           const PriaJob<float>* dev32, const PriaJob<double>* dev64) {
     constexpr int TW = kWideTW;
     count = int(host64.size());
     jobs32 = dev32;

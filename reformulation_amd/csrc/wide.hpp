@@ -56,6 +56,7 @@ constexpr int kWideTW = 32;
 
 struct WideStats {
     int rounds = 0;
+    int shrinks = 0;
     double seconds = 0.0;
 };
 
@@ -72,6 +73,7 @@ struct WideControl {
     int* pending = nullptr;        // [count] anchor copy due this round
     int* active = nullptr;         // [1] instances not finished
     int* expire_mask = nullptr;    // [count] instances stopped by their own time limit
+    int* live = nullptr;           // [count] instances not finished (for shrink)
     int* host_active = nullptr;    // [2] pinned
     hipEvent_t polled[2] = {nullptr, nullptr};
     void* arena = nullptr;
@@ -96,12 +98,28 @@ struct WideControl {
     int64_t launches[kKinds] = {0, 0};
     // algorithmic bytes of one step per instance: [kind][precision][instance]
     std::vector<double> step_bytes[kKinds][2];
-    // Per-instance time limits, in table order: the clock reading (seconds,
-    // on the owner's clock) past which the instance is stopped at its
-    // current point; +inf for none, empty when no instance has a limit.
+    // Per-instance time limits, by full-table position: the clock reading
+    // (seconds, on the owner's clock) past which the instance is stopped at
+    // its current point; +inf for none, empty when no instance has a limit.
     // `expired` latches the instances already stopped.
     std::vector<double> deadlines;
     std::vector<int> expired;
+    std::vector<int> mask_host;    // the latches in table order, copied to expire_mask
+
+    // Compaction.  setup() builds the tables over a full set of instances;
+    // shrink() rebuilds them over a subset (full-table positions, ascending)
+    // between rounds, restore() over the full set again.  An instance's
+    // arithmetic depends only on its own blocks, so its results do not
+    // change.  table_index[q] is the full-table position of table position q.
+    int full_count = 0;
+    std::vector<int> table_index;
+    bool shrunk() const { return count != full_count; }
+    virtual int shrink(hipStream_t stream, const std::vector<int>& keep) = 0;
+    virtual int restore() = 0;
+    // live[q] = 1 for the table positions not finished (on the stream)
+    virtual int live_flags(hipStream_t stream) = 0;
+    // the table positions still active -> shrink (after the stream has drained)
+    int shrink_to_live(hipStream_t stream);
 
     virtual ~WideControl();
     // every instance of the tables cold-starts at its next round (states set
@@ -116,8 +134,8 @@ struct WideControl {
     // stopped yet: stopped (expire with the latched mask) on the stream;
     // returns < 0 on error, else how many
     int expire_due(hipStream_t stream, double now);
-    // set the deadlines (table order) and clear the latches; with no work of
-    // this engine's in flight
+    // set the deadlines (full-table order) and clear the latches; with no
+    // work of this engine's in flight
     void set_deadlines(std::vector<double> d);
     // one round: the captured graph replayed (captured at the first call
     // after a setup), or launched directly when use_graph is off
@@ -150,6 +168,7 @@ struct WideRun {
     hipStream_t stream = nullptr;
     double time_limit = 0.0;
     bool finished = true;
+    bool shrink = true;            // compact the tables as instances finish
     int queued = 0, retired = 0;   // groups of rounds
     std::chrono::steady_clock::time_point t0;
     WideStats stats;
@@ -157,6 +176,22 @@ struct WideRun {
     int begin(WideControl* engine, hipStream_t s, double limit);
     // returns < 0 on error, else whether anything changed
     int advance();
+};
+
+// Device arrays for a subset's job records (gathered from the full set's by
+// position) and the positions themselves; grown on demand, freed with the
+// engine.
+template <class J32, class J64>
+struct WideSubset {
+    void* mem = nullptr;
+    int cap = 0;
+    J32* jobs32 = nullptr;
+    J64* jobs64 = nullptr;
+    int* keep = nullptr;
+    ~WideSubset() {
+        if (mem) (void)hipFree(mem);
+    }
+    int reserve(int n);
 };
 
 // PRAQ: first = (node block range, edge block range) per instance; partial
@@ -168,14 +203,28 @@ struct WidePraq : WideControl {
     int2* edge_tab = nullptr;
     int node_blocks = 0, edge_blocks = 0;
 
+    // the full set's records: host copies, device arrays as setup() got them,
+    // and the subset arrays shrink() gathers into
+    std::vector<PraqJob<double>> full64;
+    const PraqJob<float>* full_dev32 = nullptr;
+    const PraqJob<double>* full_dev64 = nullptr;
+    WideSubset<PraqJob<float>, PraqJob<double>> subset;
+
     // tables for the instances of host64 (device records dev32 / dev64 in the
     // same order; dev32 null when FP32 images do not exist)
-    int setup(const std::vector<PraqJob<double>>& host64, const PraqJob<float>* dev32,
+    int setup(std::vector<PraqJob<double>> host64, const PraqJob<float>* dev32,
               const PraqJob<double>* dev64);
     int step(hipStream_t stream, int mode, int j);
     int start(hipStream_t stream) override;
     int enqueue_round(hipStream_t stream) override;
     int expire(hipStream_t stream, const int* mask = nullptr) override;
+    int shrink(hipStream_t stream, const std::vector<int>& keep) override;
+    int restore() override;
+    int live_flags(hipStream_t stream) override;
+
+  private:
+    int build_tables(const std::vector<PraqJob<double>>& host64, const PraqJob<float>* dev32,
+                     const PraqJob<double>* dev64);
 };
 
 // PRIA: first = (slice range, column-tile range) per instance; partial rows
@@ -189,12 +238,24 @@ struct WidePria : WideControl {
     int x_blocks = 0, s_blocks = 0;
     size_t lds = 0;                     // dynamic LDS of the x kernel
 
-    int setup(const std::vector<PriaJob<double>>& host64, const PriaJob<float>* dev32,
+    std::vector<PriaJob<double>> full64;
+    const PriaJob<float>* full_dev32 = nullptr;
+    const PriaJob<double>* full_dev64 = nullptr;
+    WideSubset<PriaJob<float>, PriaJob<double>> subset;
+
+    int setup(std::vector<PriaJob<double>> host64, const PriaJob<float>* dev32,
               const PriaJob<double>* dev64);
     int step(hipStream_t stream, int mode, int j);
     int start(hipStream_t stream) override;
     int enqueue_round(hipStream_t stream) override;
     int expire(hipStream_t stream, const int* mask = nullptr) override;
+    int shrink(hipStream_t stream, const std::vector<int>& keep) override;
+    int restore() override;
+    int live_flags(hipStream_t stream) override;
+
+  private:
+    int build_tables(const std::vector<PriaJob<double>>& host64, const PriaJob<float>* dev32,
+                     const PriaJob<double>* dev64);
 };
 
 }  // namespace reform

@@ -259,3 +259,28 @@ def test_instance_time_limit_stops_only_that_instance(f32, mode):
     assert np.isfinite(info.lower) and np.isfinite(info.upper)
     assert info.relative_gap > 1e-15
     b.close()
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_compaction_changes_no_result(f32):
+    """Tables rebuilt over the running instances as others finish give every
+    instance bitwise the answer, iterations and bounds of the uncompacted
+    solve, and the same algorithmic bytes per step kernel."""
+    members = _members() + _members()[:4]
+    b, index = _batch(members, f32)
+    b.set_timing(True)
+    b.set_compaction(False)
+    b.solve("batched")
+    plain = _results(b, index)
+    plain_stats = {s["name"]: s for s in b.kernel_stats()}
+    b.set_compaction(True)
+    b.solve("batched")
+    compact = _results(b, index)
+    stats = {s["name"]: s for s in b.kernel_stats()}
+    for q, (u, v) in enumerate(zip(compact, plain)):
+        assert _same_bits(u[0], v[0]) and _same_bits(u[1], v[1]), q
+        assert u[2].iterations == v[2].iterations, q
+        assert u[2].lower == v[2].lower and u[2].upper == v[2].upper, q
+    for name, s in stats.items():
+        assert s["bytes"] == plain_stats[name]["bytes"], name
+    b.close()
