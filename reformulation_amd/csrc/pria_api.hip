@@ -40,6 +40,25 @@ struct Buffer {
     }
 };
 
+// Device memory in a stream's order (hipMallocAsync / hipFreeAsync): a solve
+// neither synchronizes the device nor waits for other threads' solves when it
+// allocates or frees, so concurrent solves on their own streams compose.
+struct StreamBuffer {
+    void* ptr = nullptr;
+    hipStream_t stream = nullptr;
+    int alloc(size_t bytes, hipStream_t s) {
+        stream = s;
+        if (hipMallocAsync(&ptr, bytes ? bytes : 1, s) != hipSuccess) {
+            ptr = nullptr;
+            return kPriaErrorDevice;
+        }
+        return 0;
+    }
+    ~StreamBuffer() {
+        if (ptr) (void)hipFreeAsync(ptr, stream);
+    }
+};
+
 struct Stream {
     hipStream_t s = nullptr;
     ~Stream() {
@@ -179,7 +198,7 @@ int pick_tier(const PriaHostInstance& s, int tier) {
 // One instance's device image in one precision, and the tier that runs it.
 template <class Real>
 struct PriaImage {
-    Buffer buf;
+    StreamBuffer buf;
     PriaLayout L;
     PriaJob<Real> job;
     size_t job_off = 0, state_off = 0, work_off = 0;
@@ -192,10 +211,7 @@ struct PriaImage {
         job_off = pria_align256(L.total_bytes);
         state_off = pria_align256(job_off + sizeof(PriaJob<Real>));
         work_off = pria_align256(state_off + sizeof(LogicState));
-        if (hipMalloc(&buf.ptr, work_off + pria_device_work_bytes(s.N, s.T)) != hipSuccess) {
-            buf.ptr = nullptr;
-            return kPriaErrorDevice;
-        }
+        if (buf.alloc(work_off + pria_device_work_bytes(s.N, s.T), stream)) return kPriaErrorDevice;
         char* base = static_cast<char*>(buf.ptr);
         std::memset(&job, 0, sizeof(job));
         job.a = pria_bind<Real>(L, base, s);
@@ -507,17 +523,6 @@ extern "C" int pria_solve_ex(int64_t n, int64_t t, const double* order_costs, do
                        warm_x, warm_y, warm_z, x, y, z, info);
 }
 
-namespace {
-
-struct DeviceScratch {
-    void* ptr = nullptr;
-    ~DeviceScratch() {
-        if (ptr) (void)hipFree(ptr);
-    }
-};
-
-}  // namespace
-
 extern "C" int pria_solve_device(int64_t n, int64_t t, const double* order_costs,
                                  double joint_cost, const double* holding_costs,
                                  const pria_params* params, const pria_options* options,
@@ -541,11 +546,8 @@ extern "C" int pria_solve_device(int64_t n, int64_t t, const double* order_costs
     const size_t f_partial = 0, f_total = kDevScaleBlocks, f_max = f_total + 1,
                  f_dot = f_max + size_t(rows), f_x = f_dot + size_t(rows);
     const size_t doubles = f_x + (x ? 0 : size_t(rows) * t);
-    DeviceScratch scratch;
-    if (hipMalloc(&scratch.ptr, sizeof(double) * doubles) != hipSuccess) {
-        scratch.ptr = nullptr;
-        return kPriaErrorDevice;
-    }
+    StreamBuffer scratch;
+    if (scratch.alloc(sizeof(double) * doubles, stream)) return kPriaErrorDevice;
     double* w = static_cast<double*>(scratch.ptr);
     hipLaunchKernelGGL(pria_sumsq_rows, dim3(kDevScaleBlocks), dim3(kDevBlock), 0, stream,
                        holding_costs, rows, int(t), w + f_partial);

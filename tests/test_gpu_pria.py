@@ -104,3 +104,38 @@ def test_plain_config_fp64_against_oracle():
     assert info.iterations == gold["iterations"]
     assert abs(info.upper - gold["upper"]) <= 1e-9 * (1 + abs(gold["upper"]))
     assert abs(info.lower - gold["lower"]) <= 1e-9 * (1 + abs(gold["lower"]))
+
+
+def test_concurrent_solves_from_threads_equal_sequential_ones():
+    """pria_solve_ex on several host threads at once (each solve on its own
+    stream, stream-ordered allocations): every answer is bitwise the one the
+    same solve gives alone."""
+    import threading
+    import time
+    from reformulation_amd import pria, workloads
+    cases = [workloads.pria_c1(300 + q, n, t) for q, (n, t) in
+             enumerate([(20, 60), (30, 90), (8, 17), (40, 120), (12, 41), (25, 80)])]
+    params = pria.Parameters.with_auto_step(100_000, 1e-6)
+    alone = []
+    t0 = time.perf_counter()
+    for c in cases:
+        alone.append(pria.solve_with_info(c.problem(), params))
+    t_seq = time.perf_counter() - t0
+    together = [None] * len(cases)
+
+    def work(q):
+        together[q] = pria.solve_with_info(cases[q].problem(), params)
+
+    threads = [threading.Thread(target=work, args=(q,)) for q in range(len(cases))]
+    t0 = time.perf_counter()
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(120.0)
+    t_par = time.perf_counter() - t0
+    print(f"pria threads: sequential {t_seq:.3f} s, {len(cases)} threads {t_par:.3f} s", flush=True)
+    for q, (a, b) in enumerate(zip(alone, together)):
+        assert b is not None, q
+        for u, v in zip(a[:3], b[:3]):
+            assert np.array_equal(u, v), q
+        assert a[3].iterations == b[3].iterations and a[3].upper == b[3].upper, q
