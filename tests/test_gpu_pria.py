@@ -139,3 +139,45 @@ def test_concurrent_solves_from_threads_equal_sequential_ones():
         for u, v in zip(a[:3], b[:3]):
             assert np.array_equal(u, v), q
         assert a[3].iterations == b[3].iterations and a[3].upper == b[3].upper, q
+
+
+@pytest.mark.parametrize("tier", [1, 2])
+def test_designed_instances_follow_the_oracle_and_hit_their_values(tier):
+    """The reference's designed instances (fractional optima of denominators
+    3..34, the p = 8 tie and its hardest bump set; jrp/pria-jax/pria/
+    adversarial.py): FP64 iterations and restarts equal the oracle's, bounds
+    agree to 1e-9, and the bracket holds the designed LP value (item 1's
+    plus 2 T); FP32 iterates (FP64 finish) reach the same value to 1e-6."""
+    import adversarial as A
+    cases = [(f"fractional p={p}", A.fractional_optimum(p)) for p in (4, 6, 7, 8)]
+    cases += [("tie p=8", A.tie_split(())), ("tie p=8 hard bumps", A.tie_split())]
+    for name, (k, k0, h, info) in cases:
+        value = A.single_item_lp(info["item"], info["K"])[0] + 2 * info["T"]
+        print(f"designed {name} tier {tier}", flush=True)
+        o = oracle.pria_solve(k, k0, h, oracle.pria_params(tolerance=1e-9, max_iter=200_000))
+        x, y, z, g = _gpu(k, k0, h, 1e-9, tier=tier, max_iter=200_000)
+        assert g.status == 0 and g.tier == tier, name
+        assert g.iterations == o.iterations and g.restarts == o.restarts, name
+        sc = 1.0 + abs(o.upper)
+        assert abs(g.upper - o.upper) <= 1e-9 * sc and abs(g.lower - o.lower) <= 1e-9 * sc, name
+        tol = 1e-9 * (1 + abs(g.upper) + abs(g.lower))
+        assert g.lower <= value + tol and g.upper >= value - tol, name
+        _, _, _, g32 = _gpu(k, k0, h, 1e-6, tier=tier, f32=True, max_iter=200_000)
+        assert g32.status == 0 and abs(g32.upper - value) <= 2e-6 * (1 + abs(value)), name
+
+
+def test_designed_instance_with_the_reference_smoke_schedule():
+    """jrp/pria-jax/tests/test_adversarial.py:80-97 on the GPU: beta_necessary
+    0, beta_artificial 10, theta 0 at 1e-9; iterations as the oracle's."""
+    import adversarial as A
+    from reformulation_amd import pria
+    k, k0, h, _ = A.fractional_optimum(4)
+    o = oracle.pria_solve(k, k0, h, oracle.pria_params(
+        tolerance=1e-9, max_iter=50_000, beta_necessary=0.0, beta_artificial=10.0,
+        theta_initial=0.0, theta_target=0.0))
+    params = pria.Parameters.with_auto_step(50_000, 1e-9, beta_necessary=0.0,
+                                            beta_artificial=10.0, theta_initial=0.0,
+                                            theta_target=0.0)
+    x, y, z, info = pria.solve_with_info(pria.JrpProblem.new(k, k0, h), params)
+    assert 0 < info.iterations < 50_000 and info.iterations == o.iterations
+    assert all(np.isfinite(a).all() for a in (x, y, z))

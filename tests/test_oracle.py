@@ -259,3 +259,76 @@ def test_simplex_projection_with_entries_tied_at_the_threshold():
     _kkt_check(v, oracle.simplex_project(v, target), target)
     v = np.array([1.0, 1.0, 1.0] + [0.25] * 10 + [0.0] * 5)
     _kkt_check(v, oracle.simplex_project(v, 2.25), 2.25)
+
+
+# ------------------------------------------------- designed PRIA instances ----
+
+def test_designed_instances_reproduce_the_reference_known_answers():
+    """jrp/pria-jax/tests/test_adversarial.py:24-77: gadget determinants
+    {4: 3, 6: 9, 7: 15, 8: 34}, partners 25 and 2, interior vertices; the p = 4
+    fractional optimum's LP value is the designed one and one-third
+    fractional; the p = 4 tie pair's value is (K + 1) T; the hard bump set
+    raises the p = 8 tie by at most 1."""
+    import adversarial as A
+    for p, q in {4: 3, 6: 9, 7: 15, 8: 34}.items():
+        Z = A.gadget(p)
+        assert abs(round(np.linalg.det(Z))) == q, p
+        mu = np.linalg.solve(Z, np.ones(p))
+        assert (mu > 0).all() and (mu < 1).all(), p
+    for Z, q in ((A.PARTNER_8, 25), (A.PARTNER_4, 2)):
+        Z = np.array(Z)
+        assert abs(round(np.linalg.det(Z))) == q
+        mu = np.linalg.solve(Z, np.ones(len(Z)))
+        assert (mu > 0).all() and (mu < 1).all()
+    _, _, _, info = A.fractional_optimum(4)
+    value, x = A.single_item_lp(info["item"], info["K"])
+    assert abs(value - info["value"]) < 1e-7
+    fracs = [v for v in x if 1e-6 < v < 1 - 1e-6]
+    assert fracs and all(abs(3 * v - round(3 * v)) < 1e-6 for v in fracs)
+    T, K, H = A.tie_design(A.gadget(4), np.array(A.PARTNER_4))
+    assert abs(A.single_item_lp(H[1:, 1:], K)[0] - (K + 1) * T) < 1e-7
+    _, _, _, info = A.tie_split(())
+    assert abs(A.single_item_lp(info["item"], info["K"])[0] - info["tie"]) < 1e-7
+    _, _, _, info = A.tie_split()
+    v = A.single_item_lp(info["item"], info["K"])[0]
+    assert -1e-9 < v - info["tie"] <= 1.0 + 1e-9
+
+
+def _designed():
+    import adversarial as A
+    out = [(f"fractional p={p}", A.fractional_optimum(p)) for p in (4, 6, 7, 8)]
+    out += [("tie p=8", A.tie_split(())), ("tie p=8 hard bumps", A.tie_split())]
+    return out
+
+
+def test_oracle_brackets_the_designed_values():
+    """Each embedded instance's LP value is item 1's plus 2 T (HiGHS on the
+    full LP agrees); the oracle at 1e-9 ends with a bracket around it."""
+    from types import SimpleNamespace
+    import adversarial as A
+    from lp_models import pria_lp_value
+    for name, (k, k0, h, info) in _designed():
+        item = A.single_item_lp(info["item"], info["K"])[0]
+        value = item + 2 * info["T"]
+        full = pria_lp_value(SimpleNamespace(order_costs=k, joint_order_cost=k0, holding_costs=h))
+        assert abs(full - value) <= 1e-7 * (1 + abs(value)), name
+        s = oracle.pria_solve(k, k0, h, oracle.pria_params(tolerance=1e-9, max_iter=200_000))
+        assert s.status == 0, name
+        tol = 1e-9 * (1 + abs(s.upper) + abs(s.lower))
+        assert s.lower <= value + tol and s.upper >= value - tol, name
+        assert abs(s.upper - value) <= 2e-9 * (1 + abs(value)), name
+
+
+def test_oracle_with_the_reference_smoke_schedule():
+    """test_adversarial.py:80-97: the p = 4 fractional instance at 1e-9 with
+    beta_necessary 0, beta_artificial 10, theta 0 finishes in (0, 50000)
+    iterations with finite outputs."""
+    import adversarial as A
+    k, k0, h, _ = A.fractional_optimum(4)
+    s = oracle.pria_solve(k, k0, h, oracle.pria_params(
+        tolerance=1e-9, max_iter=50_000, beta_necessary=0.0, beta_artificial=10.0,
+        theta_initial=0.0, theta_target=0.0))
+    assert 0 < s.iterations < 50_000
+    for a in (s.x, s.y, s.z):
+        assert np.isfinite(a).all()
+    assert math.isfinite(s.lower) and math.isfinite(s.upper)
