@@ -50,3 +50,35 @@ def small_jrp(seed: int, n: int, t: int):
     k = np.maximum(1, np.round(rng.uniform(1, max(t, 1.01), n)))
     k0 = float(np.round(rng.uniform(1, max(t, 1.01)))) if n > 1 else 0.0
     return k.astype(float), k0, h
+
+
+def brute_force_optima(n, diag, rows, cols, vals):
+    """Every binary minimiser of sum d_i x_i + sum 2 W_ij x_i x_j."""
+    xs = ((np.arange(2 ** n)[:, None] >> np.arange(n)) & 1).astype(float)
+    f = xs @ np.asarray(diag) + (2.0 * np.asarray(vals) * xs[:, rows] * xs[:, cols]).sum(axis=1)
+    return xs[np.abs(f - f.min()) < 1e-9]
+
+
+def check_answer_properties(inst, x, y):
+    """qubo/praq-jax/tests/test_solver.py:71-134: the primal is optimal for the
+    relaxation (half-integral enumeration); the coordinates persistency fixes
+    (tolerance 1e-5) agree with some binary optimum (brute force); covering
+    duals have equal slots in [-w, 0], packing duals are nonnegative and sum
+    to w (instance units, w = 2 |W_ij|)."""
+    n, diag, rows, cols, vals = inst
+    opt = half_integral_optimum(*inst)
+    assert abs(relaxation_value(*inst, x) - opt) <= 1e-6 * (1 + abs(opt))
+    x = np.asarray(x, dtype=float)
+    rounded = np.round(x)
+    fixed = np.abs(x - rounded) <= 1e-5
+    optima = brute_force_optima(n, diag, rows, cols, vals)
+    assert (optima[:, fixed] == rounded[fixed]).all(axis=1).any()
+    y = np.asarray(y, dtype=float).reshape(-1, 2)
+    w = 2.0 * np.abs(np.asarray(vals))
+    cover = np.asarray(vals) > 0
+    yc, yp = y[cover], y[~cover]
+    big = 1.0 + w.max(initial=0.0)
+    assert np.abs(yc[:, 0] - yc[:, 1]).max(initial=0.0) <= 1e-9 * big
+    assert (yc[:, 0] <= 1e-9).all() and (yc[:, 0] >= -w[cover] - 1e-9).all()
+    assert np.abs(yp.sum(axis=1) - w[~cover]).max(initial=0.0) <= 1e-9 * big
+    assert (yp >= -1e-9).all()

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import half_integral_optimum, random_qubo, relaxation_value
+from helpers import check_answer_properties, half_integral_optimum, random_qubo, relaxation_value
 
 pytestmark = pytest.mark.gpu
 
@@ -171,3 +171,15 @@ def test_hub_nodes_by_chunks_match_the_oracle_and_the_workgroup_tier():
         assert np.max(np.abs(x - o.x)) <= 1e-7
     x, y, info = _gpu(inst, 1e-6, tier=2, f32=True, max_iter=200_000)
     assert info.status == 0 and abs(info.upper - o.upper) <= 2e-6 * sc
+
+
+
+@pytest.mark.parametrize("tier", [1, 2])
+def test_persistency_and_dual_feasibility_of_the_answer(tier):
+    """qubo/praq-jax/tests/test_solver.py:71-134 on the GPU's answers (the
+    checks in helpers.check_answer_properties)."""
+    for seed, n in [(11, 4), (12, 6), (13, 8), (14, 10), (7, 9)]:
+        inst = random_qubo(seed, n, 0.6)
+        x, y, info = _gpu(inst, 1e-9, tier=tier, max_iter=200_000)
+        assert info.status == 0, seed
+        check_answer_properties(inst, x, y)

@@ -11,7 +11,8 @@ import pytest
 from scipy.optimize import minimize_scalar
 
 import oracle
-from helpers import half_integral_optimum, random_qubo, relaxation_value, small_jrp
+from helpers import (check_answer_properties, half_integral_optimum, random_qubo,
+                     relaxation_value, small_jrp)
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
@@ -130,6 +131,14 @@ def test_praq_warm_start_from_the_solution():
     w = oracle.praq_solve(*inst, params=oracle.praq_params(tolerance=1e-9), warm=(s.x, s.y))
     assert w.iterations <= 16
     assert abs(w.upper - s.upper) <= 1e-7 * (1 + abs(s.upper))
+
+
+def test_praq_answer_is_optimal_persistent_and_dual_feasible():
+    for seed, n in [(11, 4), (12, 6), (13, 8), (14, 10), (7, 9)]:
+        inst = random_qubo(seed, n, 0.6)
+        s = oracle.praq_solve(*inst, params=oracle.praq_params(tolerance=1e-9, max_iter=200_000))
+        assert s.status == 0
+        check_answer_properties(inst, s.x, s.y)
 
 
 def test_praq_step_is_nonexpansive_in_the_canonical_metric():
