@@ -149,7 +149,9 @@ def serving(members, rate: float, seconds: float, service, use_f32: bool, thread
                speedup=(served["solves_per_s"] / plain["solves_per_s"])
                if plain["solves_per_s"] else None,
                rounds=stats["rounds"], bytes_peak=stats["bytes_peak"],
-               hbm_budget=stats["bytes_budget"])
+               hbm_budget=stats["bytes_budget"],
+               host_s=dict(retire=round(stats["retire_seconds"], 3),
+                           rebuild=round(stats["rebuild_seconds"], 3)))
     log(f"serving at {rate:g}/s for {seconds:g} s: {json.dumps(out)}")
     return out
 

@@ -62,6 +62,8 @@ typedef struct reform_server_counters {
     int64_t rounds, rebuilds;
     int64_t in_flight;
     int64_t bytes_in_use, bytes_peak, bytes_budget;
+    double retire_seconds;  /* retire threads (summed): copying answers out, freeing */
+    double rebuild_seconds; /* serving thread: rebuilding the block tables */
 } reform_server_counters;
 
 reform_server_config reform_server_default_config(void);

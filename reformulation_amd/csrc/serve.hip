@@ -41,6 +41,9 @@ namespace {
 constexpr int kErrorTicket = -4;
 constexpr int kErrorTimeout = -5;
 constexpr int kPending = -1;
+// host threads copying finished requests' answers out; at 1000 requests/s
+// three threads served no faster than one (the staged copies share the link)
+constexpr int kRetireThreads = 1;
 
 // flags[k] = member k of a family's tables has finished
 template <class Job>
@@ -156,8 +159,9 @@ struct reform_server {
     reform_server_config cfg{};
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     std::mutex mu;
-    std::condition_variable cv_build, cv_ready, cv_done;
+    std::condition_variable cv_build, cv_ready, cv_done, cv_retire;
     std::deque<Request*> to_build, ready;
+    std::deque<Request*> retiring;   // finished, left the tables, answers not yet copied
     std::deque<int64_t> finished;
     std::unordered_map<int64_t, std::unique_ptr<Request>> requests;
     std::atomic<bool> stop{false};
@@ -165,8 +169,10 @@ struct reform_server {
     int64_t n_submitted = 0, n_admitted = 0, n_completed = 0, n_rejected = 0;
     int n_building = 0;  // requests on the builders
     std::atomic<int64_t> n_rounds{0}, n_rebuilds{0}, n_in_flight{0};
+    std::atomic<double> retire_s{0.0}, rebuild_s{0.0};   // serving-thread host time
     std::vector<std::thread> builders;
     std::thread server;
+    std::vector<std::thread> retirers;              // kRetireThreads, a stream each
     int device = 0;
     int start_status = 0;
 
@@ -196,9 +202,12 @@ struct reform_server {
         stop = true;
         cv_build.notify_all();
         cv_ready.notify_all();
+        cv_retire.notify_all();
         for (auto& b : builders)
             if (b.joinable()) b.join();
         if (server.joinable()) server.join();
+        for (auto& r : retirers)
+            if (r.joinable()) r.join();
         for (auto& f : jobs_dev)
             for (void* p : f)
                 if (p) (void)hipFree(p);
@@ -215,6 +224,7 @@ struct reform_server {
 
     void build_loop();
     void serve_loop();
+    void retire_loop();
     int build(Request& r);
     int admit(Request& r);
     int rebuild(int family);
@@ -436,12 +446,14 @@ static int copy_out(std::vector<Real>& out, const Real* src, size_t count, hipSt
     return 0;
 }
 
+// With `mu` held by the caller's lock below: the pool and the slots are
+// shared with admission, which allocates under `mu`.
 void reform_server::retire(Request& r, int status) {
+    std::lock_guard<std::mutex> lock(mu);
     pool.release(r.off, r.bytes);
     bytes_in_use = pool.used();
     free_slots.push_back(r.slot);
     n_in_flight--;
-    std::lock_guard<std::mutex> lock(mu);
     r.status = status;
     r.t_done = seconds_since(t0);
     ++n_completed;
@@ -450,43 +462,80 @@ void reform_server::retire(Request& r, int status) {
 }
 
 // After the family's round: members whose state reads finished (the done
-// flags copied back with the round) leave the tables; their answers and
-// states are copied out and their memory and slots freed.
+// flags copied back with the round) leave the tables and go to the retire
+// thread, which copies their answers out and frees their memory and slots
+// while the family's next round (tables rebuilt without them) runs.
 int reform_server::retire_finished(int f) {
     auto& mem = members[f];
     const int* done = flags_host[f];
     std::vector<Request*> keep, gone;
     for (size_t q = 0; q < mem.size(); ++q) (done[q] ? gone : keep).push_back(mem[q]);
     if (gone.empty()) return 0;
-    hipStream_t s = streams[f];
-    for (Request* rp : gone) {
-        Request& r = *rp;
-        REFORM_HIP_CHECK(hipMemcpyAsync(&r.final, states + r.slot, sizeof(LogicState),
-                                        hipMemcpyDeviceToHost, s));
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        retiring.insert(retiring.end(), gone.begin(), gone.end());
     }
-    REFORM_HIP_CHECK(hipStreamSynchronize(s));
-    for (Request* rp : gone) {
-        Request& r = *rp;
-        r.in64 = r.final.in64 != 0;
-        int st = 0;
-        if (f == 0) {
-            const size_t n = r.ph.n, m2 = 2 * size_t(r.ph.m);
-            st = r.in64 ? (copy_out(r.d1, r.pj64.a.x, n, s) || copy_out(r.d2, r.pj64.a.y, m2, s))
-                        : (copy_out(r.f1, r.pj32.a.x, n, s) || copy_out(r.f2, r.pj32.a.y, m2, s));
-        } else {
-            const size_t nd = r.rh.ND;
-            st = r.in64 ? (copy_out(r.d1, r.rj64.a.x, nd, s) || copy_out(r.d2, r.rj64.a.y, nd, s) ||
-                           copy_out(r.d3, r.rj64.a.z, nd, s))
-                        : (copy_out(r.f1, r.rj32.a.x, nd, s) || copy_out(r.f2, r.rj32.a.y, nd, s) ||
-                           copy_out(r.f3, r.rj32.a.z, nd, s));
-        }
-        if (st) return kErrorDevice;
-    }
-    REFORM_HIP_CHECK(hipStreamSynchronize(s));
-    for (Request* rp : gone) retire(*rp, rp->final.status);
+    cv_retire.notify_all();
     mem.swap(keep);
     changed[f] = true;
     return 0;
+}
+
+// Each retire thread: final state and answer of a finished request, on a
+// stream of its own (nothing else touches a request once it has left the
+// tables), then its memory and slot back to admission.
+void reform_server::retire_loop() {
+    auto fail = [&](int code) {
+        std::lock_guard<std::mutex> lock(mu);
+        start_status = code;
+        stop = true;
+        cv_done.notify_all();
+        cv_ready.notify_all();
+    };
+    hipStream_t s = nullptr;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+        return fail(kErrorDevice);
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{s};
+    for (;;) {
+        std::vector<Request*> batch;
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            cv_retire.wait(lock, [&] { return stop || !retiring.empty(); });
+            if (stop) return;
+            batch.push_back(retiring.front());   // one at a time: the threads share the queue
+            retiring.pop_front();
+        }
+        const double t_start = seconds_since(t0);
+        for (Request* rp : batch) {
+            Request& r = *rp;
+            if (hipMemcpyAsync(&r.final, states + r.slot, sizeof(LogicState),
+                               hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                return fail(kErrorDevice);
+            r.in64 = r.final.in64 != 0;
+            int st = 0;
+            if (r.family == 0) {
+                const size_t n = r.ph.n, m2 = 2 * size_t(r.ph.m);
+                st = r.in64 ? (copy_out(r.d1, r.pj64.a.x, n, s) || copy_out(r.d2, r.pj64.a.y, m2, s))
+                            : (copy_out(r.f1, r.pj32.a.x, n, s) || copy_out(r.f2, r.pj32.a.y, m2, s));
+            } else {
+                const size_t nd = r.rh.ND;
+                st = r.in64 ? (copy_out(r.d1, r.rj64.a.x, nd, s) ||
+                               copy_out(r.d2, r.rj64.a.y, nd, s) ||
+                               copy_out(r.d3, r.rj64.a.z, nd, s))
+                            : (copy_out(r.f1, r.rj32.a.x, nd, s) ||
+                               copy_out(r.f2, r.rj32.a.y, nd, s) ||
+                               copy_out(r.f3, r.rj32.a.z, nd, s));
+            }
+            if (st || hipStreamSynchronize(s) != hipSuccess) return fail(kErrorDevice);
+            retire(r, r.final.status);
+        }
+        retire_s = retire_s + (seconds_since(t0) - t_start);
+    }
 }
 
 // One round of the family on its stream, then its done flags into pinned
@@ -568,9 +617,9 @@ void reform_server::serve_loop() {
                 ready.pop_front();
                 admitted.push_back(r);
             }
+            bytes_in_use = pool.used();
+            bytes_peak = pool.peak();
         }
-        bytes_in_use = pool.used();
-        bytes_peak = pool.peak();
         if (!admitted.empty()) {
             cv_build.notify_all();
             for (Request* r : admitted)
@@ -607,7 +656,9 @@ void reform_server::serve_loop() {
             }
             if (members[f].empty()) continue;
             if (changed[f]) {
+                const double t_rebuild = seconds_since(t0);
                 if (rebuild(f)) return fail(kErrorDevice);
+                rebuild_s = rebuild_s + (seconds_since(t0) - t_rebuild);
                 stable[f] = 0;
             }
             if (launch(f)) return fail(kErrorDevice);
@@ -651,6 +702,8 @@ extern "C" reform_server* reform_server_create(const reform_server_config* confi
     reform_server* raw = srv.get();
     for (int q = 0; q < c.host_threads; ++q) raw->builders.emplace_back([raw] { raw->build_loop(); });
     raw->server = std::thread([raw] { raw->serve_loop(); });
+    for (int q = 0; q < kRetireThreads; ++q)
+        raw->retirers.emplace_back([raw] { raw->retire_loop(); });
     return srv.release();
 }
 
@@ -844,5 +897,7 @@ extern "C" int reform_server_stats(reform_server* server, reform_server_counters
     stats->bytes_in_use = int64_t(server->bytes_in_use);
     stats->bytes_peak = int64_t(server->bytes_peak);
     stats->bytes_budget = int64_t(server->pool.capacity());
+    stats->retire_seconds = server->retire_s;
+    stats->rebuild_seconds = server->rebuild_s;
     return server->start_status;
 }

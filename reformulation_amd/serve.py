@@ -48,7 +48,8 @@ class RequestInfo(ctypes.Structure):
 class ServerCounters(ctypes.Structure):
     _fields_ = [("submitted", _i64), ("admitted", _i64), ("completed", _i64),
                 ("rejected", _i64), ("rounds", _i64), ("rebuilds", _i64), ("in_flight", _i64),
-                ("bytes_in_use", _i64), ("bytes_peak", _i64), ("bytes_budget", _i64)]
+                ("bytes_in_use", _i64), ("bytes_peak", _i64), ("bytes_budget", _i64),
+                ("retire_seconds", ctypes.c_double), ("rebuild_seconds", ctypes.c_double)]
 
 
 def _declare(L):
@@ -234,7 +235,8 @@ class Server:
         st = self._L.reform_server_stats(self._h, ctypes.byref(c))
         if st < 0:
             raise RuntimeError(f"serving failed ({st}): {_native.last_error()}")
-        return {name: int(getattr(c, name)) for name, _ in ServerCounters._fields_}
+        return {name: (float if kind is ctypes.c_double else int)(getattr(c, name))
+                for name, kind in ServerCounters._fields_}
 
 
 def poisson_arrivals(rate: float, seconds: float, seed: int) -> np.ndarray:
