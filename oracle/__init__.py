@@ -65,6 +65,10 @@ def lib():
         _lib.oracle_praq_check_interval.restype = _i64
         _lib.oracle_praq_check_interval.argtypes = [_i64, ctypes.c_double, ctypes.c_double,
                                                     ctypes.c_double]
+        _lib.oracle_praq_roof_dual.restype = ctypes.c_double
+        _lib.oracle_praq_roof_dual.argtypes = [_i64, _dp, _i64, ctypes.c_void_p,
+                                               ctypes.c_void_p, _dp, ctypes.c_double,
+                                               ctypes.POINTER(_i64)]
     return _lib
 
 
@@ -120,6 +124,23 @@ class Solved:
     status: int
     scale: float
     omega: float
+
+
+def praq_roof_dual(n, diag, rows, cols, vals, scale: float = 2.0 ** 30) -> float:
+    """The PRAQ relaxation's value by max-flow (maxflow.c): capacities on
+    the 1 / scale grid, exact for weights on it (integers with scale 1)."""
+    diag = np.ascontiguousarray(diag, dtype=np.float64)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    flow = _i64(0)
+    v = lib().oracle_praq_roof_dual(_i64(n), _d(diag), _i64(len(vals)),
+                                    rows.ctypes.data_as(ctypes.c_void_p),
+                                    cols.ctypes.data_as(ctypes.c_void_p), _d(vals),
+                                    float(scale), ctypes.byref(flow))
+    if v != v:
+        raise ValueError("oracle_praq_roof_dual failed")
+    return float(v)
 
 
 def praq_solve(n, diag, rows, cols, vals, params: OracleParams | None = None,

@@ -120,6 +120,13 @@ double oracle_pria_primal_objective(int64_t n, int64_t t, const double* k, doubl
 double oracle_pria_dual_objective(int64_t n, int64_t t, const double* k, double k0,
                                   const double* h, const double* y, const double* z);
 
+/* Roof dual (= the PRAQ relaxation's value) by max-flow with int64
+ * capacities round(a scale) for each posiform term a (maxflow.c); exact for
+ * weights on the 1 / scale grid.  NaN on bad arguments or memory. */
+double oracle_praq_roof_dual(int64_t n, const double* diag, int64_t m, const int64_t* rows,
+                             const int64_t* cols, const double* vals, double scale,
+                             int64_t* flow_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -183,3 +183,26 @@ def test_persistency_and_dual_feasibility_of_the_answer(tier):
         x, y, info = _gpu(inst, 1e-9, tier=tier, max_iter=200_000)
         assert info.status == 0, seed
         check_answer_properties(inst, x, y)
+
+
+def test_large_chung_lu_bracket_holds_the_max_flow_value():
+    """The large instance (n = 1e6, m = 1.6e7, Chung-Lu degrees, Gaussian
+    weights; device tier with hub chunks), FP64 and FP32 iterates at 1e-6:
+    the final bracket L <= OPT <= U holds around the relaxation value that
+    the oracle's int64 max-flow roof dual gives (tests/golden/
+    c2_roof_dual.json, to its capacity-rounding bound), so both ends are
+    within the requested gap of the optimum."""
+    from reformulation_amd import praq, workloads
+    ref = json.loads((GOLDEN / "c2_roof_dual.json").read_text())
+    opt, slack = ref["roof_dual"], ref["error_bound"]
+    w = workloads.praq_chung_lu(0)
+    assert (w.n, len(w.vals)) == (ref["n"], ref["m"])
+    problem = w.problem()
+    for f32 in (False, True):
+        print(f"chung-lu 1e6 f32={f32}", flush=True)
+        params = praq.Parameters.with_auto_step(100_000, 1e-6, use_float32=f32, time_limit=120.0)
+        _, _, info = praq.solve_with_info(problem, params)
+        assert info.status == 0, f32
+        assert info.lower <= opt + slack and info.upper >= opt - slack, (f32, info.lower, info.upper)
+        gap = 1e-6 * (1 + abs(info.upper) + abs(info.lower))
+        assert info.upper - opt <= gap + slack and opt - info.lower <= gap + slack, f32

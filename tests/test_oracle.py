@@ -341,3 +341,35 @@ def test_oracle_with_the_reference_smoke_schedule():
     for a in (s.x, s.y, s.z):
         assert np.isfinite(a).all()
     assert math.isfinite(s.lower) and math.isfinite(s.upper)
+
+
+def test_roof_dual_by_max_flow_equals_highs_and_enumeration():
+    """oracle_praq_roof_dual (int64 Dinic on the implication network): on
+    integer instances of the C0 kind it equals scipy's int32 max-flow and
+    HiGHS exactly; with Gaussian weights (capacities on the 2^-30 grid)
+    HiGHS to 1e-10; on tiny instances the half-integral enumeration."""
+    from types import SimpleNamespace
+    from lp_models import praq_lp_value, praq_maxflow_value
+    from reformulation_amd import workloads
+    for seed in range(2):
+        w = workloads.praq_er(seed, 2000, 8000)
+        inst = SimpleNamespace(n=w.n, diag=w.diag, rows=w.rows, cols=w.cols, vals=w.vals)
+        v = oracle.praq_roof_dual(w.n, w.diag, w.rows, w.cols, w.vals, scale=1.0)
+        assert v == praq_maxflow_value(inst)
+        assert abs(v - praq_lp_value(inst)) <= 1e-9 * (1 + abs(v))
+    rng = np.random.default_rng(5)
+    for _ in range(2):
+        n, m = 300, 1500
+        pairs = set()
+        while len(pairs) < m:
+            i, j = sorted(rng.integers(0, n, 2))
+            if i != j:
+                pairs.add((int(i), int(j)))
+        rows, cols = (np.array(c, dtype=np.int64) for c in zip(*sorted(pairs)))
+        vals, diag = rng.normal(size=m), rng.normal(size=n)
+        inst = SimpleNamespace(n=n, diag=diag, rows=rows, cols=cols, vals=vals)
+        v = oracle.praq_roof_dual(n, diag, rows, cols, vals)
+        assert abs(v - praq_lp_value(inst)) <= 1e-10 * (1 + abs(v))
+    for seed in range(4):
+        inst = random_qubo(40 + seed, 7, 0.6)
+        assert oracle.praq_roof_dual(*inst, scale=1.0) == half_integral_optimum(*inst)
