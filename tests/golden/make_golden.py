@@ -14,7 +14,9 @@
   pria_lp.json   HiGHS values of the explicit JRP LP on small instances.
   pria_c1.json   the oracle's FP64 solve (iterations, bounds) of the plain-PRIA
                  workload (N = 50, T = 200, seed 0) at 1e-6; the oracle itself
-                 is pinned to the reference CPU solver by pria_ref.npz.
+                 is pinned to the reference CPU solver by pria_ref.npz; and
+                 HiGHS's value of the instance's explicit LP (1.0M x columns,
+                 about a minute).
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -107,10 +109,14 @@ def make_pria_c1():
                           oracle.pria_params(tolerance=1e-6))
     (HERE / "pria_c1.json").write_text(json.dumps(dict(
         seed=0, items=50, periods=200, tolerance=1e-6, iterations=s.iterations,
-        restarts=s.restarts, upper=s.upper, lower=s.lower, gap=s.relative_gap), indent=1))
+        restarts=s.restarts, upper=s.upper, lower=s.lower, gap=s.relative_gap,
+        highs=pria_lp_value(inst)), indent=1))
 
 
 if __name__ == "__main__":
+    if sys.argv[1:] == ["pria_c1"]:
+        make_pria_c1()
+        sys.exit(0)
     make_pria_c1()
     make_pria_ref()
     make_praq_lp()

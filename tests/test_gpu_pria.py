@@ -104,6 +104,13 @@ def test_plain_config_fp64_against_oracle():
     assert info.iterations == gold["iterations"]
     assert abs(info.upper - gold["upper"]) <= 1e-9 * (1 + abs(gold["upper"]))
     assert abs(info.lower - gold["lower"]) <= 1e-9 * (1 + abs(gold["lower"]))
+    # the bracket holds HiGHS's value of the full explicit LP
+    v, slack = gold["highs"], 1e-8 * (1 + abs(gold["highs"]))
+    assert info.lower <= v + slack and info.upper >= v - slack
+    # FP32 iterates (FP64 finish) too
+    _, _, _, i32 = _gpu(inst.order_costs, inst.joint_order_cost, inst.holding_costs, 1e-6,
+                        tier=2, f32=True, max_iter=50_000)
+    assert i32.status == 0 and i32.lower <= v + slack and i32.upper >= v - slack
 
 
 def test_concurrent_solves_from_threads_equal_sequential_ones():

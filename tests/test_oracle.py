@@ -259,6 +259,15 @@ def test_pria_known_answers():
     assert lo <= up + 1e-9 and abs(up - s.upper) <= 1e-9 * (1 + abs(up))
 
 
+def test_pria_c1_fixture_bracket_holds_the_highs_value():
+    """The oracle's 1e-6 bracket on the plain-PRIA workload (N = 50, T =
+    200) holds HiGHS's value of its explicit LP (both in pria_c1.json)."""
+    g = json.loads((GOLDEN / "pria_c1.json").read_text())
+    slack = 1e-8 * (1 + abs(g["highs"]))
+    assert g["lower"] <= g["highs"] + slack and g["upper"] >= g["highs"] - slack
+    assert g["upper"] - g["lower"] <= 1e-6 * (1 + abs(g["upper"]) + abs(g["lower"]))
+
+
 def test_simplex_projection_with_entries_tied_at_the_threshold():
     """Entries equal to the threshold make the rho test hold off the prefix;
     the count-based rho (projection.py:113-128) still lands on the tie.
