@@ -305,7 +305,11 @@ __device__ __forceinline__ double group_michelot(const Real (&u)[VR], int len, i
 // Widths follow kPriaRow16 / kPriaRow32, not VR.
 // z slice: the whole workgroup, values in registers when the slice fits
 // (N (T' - s) <= blockDim * VZ), else re-read from memory each round; its
-// reductions take one barrier each.  `red`: 8 16 doubles of LDS.
+// reductions take one barrier each.  `red`: 8 16 doubles of LDS.  `zcache`
+// (LDS for `zcap` values, or null): on the memory path the slice's first
+// zcap sums z + r are kept there for the Michelot rounds, which then re-read
+// only the rest from memory (each thread reads back its own entries; the
+// values and their order are the same either way).
 // LDS doubles pria_yz_slice's `red` must hold.
 constexpr int kPriaYZRed = 8 * 16;
 
@@ -319,7 +323,8 @@ constexpr int kPriaRow16 = 80, kPriaRow32 = 160;
 
 template <class Real, int VR, int VZ>
 __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& st, int s,
-                              double* red, int part = 3, int i_lo = 0, int i_hi = 1 << 30) {
+                              double* red, int part = 3, int i_lo = 0, int i_hi = 1 << 30,
+                              Real* zcache = nullptr, int zcap = 0) {
     const int T = a.T, N = a.N;
     const int BS = blockDim.x, tid = threadIdx.x;
     const int lane = tid & (kWave - 1), wave = tid / kWave, waves = BS / kWave;
@@ -422,8 +427,11 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
             }
         } else {
             double v4[4] = {0.0, 0.0, 0.0, 0.0};
+            if (!zcache) zcap = 0;
             for (int e = tid; e < L; e += BS) {
-                const double v = double(Real(srcz[base + e] + a.r[base + e]));
+                const Real u = srcz[base + e] + a.r[base + e];
+                if (e < zcap) zcache[e] = u;
+                const double v = double(u);
                 v4[0] += v;
                 v4[1] += 1.0;
                 if (v > start) {
@@ -438,7 +446,8 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
             for (int round = 0; round <= L; ++round) {
                 double w2[2] = {0.0, 0.0};
                 for (int e = tid; e < L; e += BS) {
-                    const double v = double(Real(srcz[base + e] + a.r[base + e]));
+                    const double v = double(e < zcap ? zcache[e]
+                                                     : Real(srcz[base + e] + a.r[base + e]));
                     if (v > th) {
                         w2[0] += v;
                         w2[1] += 1.0;

@@ -16,6 +16,9 @@ namespace {
 
 constexpr int kWideBlock = 256;
 constexpr int kRoundsPerPoll = 2;
+// LDS of a y/z block caching a long z slice's sums for its Michelot rounds:
+// with 1 KB of reductions, three 512-thread blocks still fit a CU's 160 KB
+constexpr int kPriaZCacheBytes = 48 * 1024;
 // tables of fewer instances are not compacted further
 constexpr int kShrinkMin = 8;
 
@@ -402,17 +405,22 @@ __global__ void __launch_bounds__(512, compact ? 6 : 4)
 wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab, const int* round_action, int mode,
          int j) {
     __shared__ double red[kPriaYZRed];
+    extern __shared__ double zcache[];   // kPriaZCacheBytes: z + r of a long slice
     const int2 t = tab[blockIdx.x];
     if (t.x < 0) return;
     dispatch(j32, j64, t.x, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         PriaStep<Real> sc;
         if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+        Real* cache = reinterpret_cast<Real*>(zcache);
+        const int cap = kPriaZCacheBytes / int(sizeof(Real));
         if (compact)
             pria_yz_slice<Real, PriaCompactShape<Real>::VR, PriaCompactShape<Real>::VZ>(
-                job.a, sc, t.y, red);
+                job.a, sc, t.y, red, 3, 0, 1 << 30, cache, cap);
         else
-            pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(job.a, sc, t.y, red);
+            pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(job.a, sc, t.y, red,
+                                                                          3, 0, 1 << 30, cache,
+                                                                          cap);
     });
 }
 
@@ -1135,11 +1143,11 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
                        x_tab, round_action, mode, j);
     if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
     if (max_T <= kPriaCompactT)
-        hipLaunchKernelGGL(wpria_yz<true>, dim3(s_blocks), dim3(512), 0, stream, jobs32, jobs64,
-                           s_tab, round_action, mode, j);
+        hipLaunchKernelGGL(wpria_yz<true>, dim3(s_blocks), dim3(512), kPriaZCacheBytes, stream,
+                           jobs32, jobs64, s_tab, round_action, mode, j);
     else
-        hipLaunchKernelGGL(wpria_yz<false>, dim3(s_blocks), dim3(512), 0, stream, jobs32, jobs64,
-                           s_tab, round_action, mode, j);
+        hipLaunchKernelGGL(wpria_yz<false>, dim3(s_blocks), dim3(512), kPriaZCacheBytes, stream,
+                           jobs32, jobs64, s_tab, round_action, mode, j);
     if (mark(stream, 1, false)) return kErrorDevice;
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
