@@ -42,11 +42,13 @@ class OracleInfo(ctypes.Structure):
                 ("omega", ctypes.c_double), ("scale", ctypes.c_double), ("status", _i64)]
 
 
-def build() -> None:
-    """Compile oracle/*.c (and oracle/_ref where the reference tree exists)."""
-    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+def build(force: bool = False) -> None:
+    """Compile oracle/*.c (and oracle/_ref where the reference tree exists);
+    `force` recompiles what is already built."""
+    flags = ["-B"] if force else []
+    subprocess.run(["make", "-s", *flags, "-C", str(HERE), "all"], check=True)
     if Path("/root/reference/jrp/pria-cuda/tests/ref_impl.hpp").exists():
-        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+        subprocess.run(["make", "-s", *flags, "-C", str(HERE), "ref"], check=True)
 
 
 _lib = None
