@@ -95,6 +95,9 @@ typedef struct {
     int tier;               /* tier that ran                                 */
     int64_t switch_iteration; /* FP32 iterates handed over to FP64 at this
                                * iteration, having stalled; -1 if they did not */
+    int64_t device_bytes;   /* device memory the solve held at its peak: its
+                             * images (both precisions when FP32 handed over)
+                             * and work areas                               */
 } PraqInfo;
 
 PraqOptions praq_default_options(void);

@@ -100,6 +100,7 @@ typedef struct {
     int tier;
     int64_t switch_iteration; /* FP32 iterates handed over to FP64 at this
                                * iteration, having stalled; -1 if they did not */
+    int64_t device_bytes;     /* device memory the solve held at its peak     */
 } pria_info;
 
 pria_options pria_default_options(void);

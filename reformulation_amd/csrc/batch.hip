@@ -77,6 +77,7 @@ struct PraqEntry {
     PraqOptions o;
     PraqHostInstance host;
     size_t off32 = 0, off64 = 0;  // image offsets in the device allocation
+    size_t bytes = 0;             // the images' device bytes
     int tier = 0;
 };
 
@@ -88,6 +89,7 @@ struct PriaEntry {
     pria_options o;
     PriaHostInstance host;
     size_t off32 = 0, off64 = 0;
+    size_t bytes = 0;
     int tier = 0;
 };
 
@@ -287,6 +289,7 @@ int BatchImpl::upload() {
             praq[q].off32 = at;
             at += align256(pl32[q].total_bytes);
         }
+        praq[q].bytes = at - praq[q].off64;
     }
     for (size_t q = 0; q < pria.size(); ++q) {
         rl64[q] = pria_layout(pria[q].host.N, pria[q].host.T, pria[q].host.ND, sizeof(double));
@@ -297,6 +300,7 @@ int BatchImpl::upload() {
             pria[q].off32 = at;
             at += align256(rl32[q].total_bytes);
         }
+        pria[q].bytes = at - pria[q].off64;
         work_bytes = std::max(work_bytes, pria_device_work_bytes(pria[q].host.N, pria[q].host.T));
         pria_max_T = std::max(pria_max_T, pria[q].host.T);
     }
@@ -749,6 +753,7 @@ int BatchImpl::praq_result(int64_t q, double* x, double* y, PraqInfo* info) {
         info->status = st.status;
         info->tier = e.tier;
         info->switch_iteration = st.switch_iteration;
+        info->device_bytes = int64_t(e.bytes);
     }
     return 0;
 }
@@ -788,6 +793,7 @@ int BatchImpl::pria_result(int64_t q, double* x, double* y, double* z, pria_info
         info->status = st.status;
         info->tier = e.tier;
         info->switch_iteration = st.switch_iteration;
+        info->device_bytes = int64_t(e.bytes);
     }
     return 0;
 }

@@ -66,14 +66,15 @@ struct PraqImage {
     DeviceBuffer buf;
     PraqLayout L;
     PraqJob<Real> job;
-    size_t job_off = 0, state_off = 0, work_off = 0;
+    size_t job_off = 0, state_off = 0, work_off = 0, bytes = 0;
 
     int init(const PraqHostInstance& s, const LogicParams& lp, hipStream_t stream) {
         L = praq_layout(s, sizeof(Real));
         job_off = align256(L.total_bytes);
         state_off = align256(job_off + sizeof(PraqJob<Real>));
         work_off = align256(state_off + sizeof(LogicState));
-        if (buf.alloc(work_off + praq_device_work_bytes())) return kErrorDevice;
+        bytes = work_off + praq_device_work_bytes();
+        if (buf.alloc(bytes)) return kErrorDevice;
         char* base = static_cast<char*>(buf.ptr);
         std::vector<char> image(L.input_bytes, 0);
         praq_fill_image<Real>(s, L, image.data());
@@ -152,22 +153,26 @@ int solve_mixed(const PraqHostInstance& s, LogicParams lp, const PraqOptions& o,
     const int tier = pick_tier(s, o.tier);
     LogicState st;
     std::memset(&st, 0, sizeof(st));
+    size_t held = 0;
     if (!fp32) {
         PraqImage<double> d;
         if (d.init(s, lp, stream) || d.run(tier, st, o.time_limit, false, stream) || emit(d))
             return kErrorDevice;
+        held = d.bytes;
     } else {
         LogicParams lp32 = lp;
         apply_fp32_policy(lp32);
         PraqImage<float> f;
         if (f.init(s, lp32, stream) || f.run(tier, st, o.time_limit, false, stream))
             return kErrorDevice;
+        held = f.bytes;
         if (st.stalled) {
             PraqImage<double> d;
             st.switch_iteration = st.iteration;
             if (d.init(s, lp, stream) || praq_convert_state(f.job.a, d.job.a, stream) ||
                 d.run(tier, st, o.time_limit, true, stream) || emit(d))
                 return kErrorDevice;
+            held += d.bytes;
         } else if (emit(f)) {
             return kErrorDevice;
         }
@@ -184,6 +189,7 @@ int solve_mixed(const PraqHostInstance& s, LogicParams lp, const PraqOptions& o,
         info->status = st.status;
         info->tier = tier;
         info->switch_iteration = st.switch_iteration;
+        info->device_bytes = int64_t(held);
     }
     return 0;
 }

@@ -201,7 +201,7 @@ struct PriaImage {
     StreamBuffer buf;
     PriaLayout L;
     PriaJob<Real> job;
-    size_t job_off = 0, state_off = 0, work_off = 0;
+    size_t job_off = 0, state_off = 0, work_off = 0, bytes = 0;
 
     // `din` set: K and h gathered on the device from the caller's arrays
     // (cold start only); else the host image of `s` uploaded
@@ -211,7 +211,8 @@ struct PriaImage {
         job_off = pria_align256(L.total_bytes);
         state_off = pria_align256(job_off + sizeof(PriaJob<Real>));
         work_off = pria_align256(state_off + sizeof(LogicState));
-        if (buf.alloc(work_off + pria_device_work_bytes(s.N, s.T), stream)) return kPriaErrorDevice;
+        bytes = work_off + pria_device_work_bytes(s.N, s.T);
+        if (buf.alloc(bytes, stream)) return kPriaErrorDevice;
         char* base = static_cast<char*>(buf.ptr);
         std::memset(&job, 0, sizeof(job));
         job.a = pria_bind<Real>(L, base, s);
@@ -292,23 +293,27 @@ int solve_mixed(const PriaHostInstance& s, const LogicParams& lp, const pria_opt
     const int tier = pick_tier(s, o.tier);
     LogicState st;
     std::memset(&st, 0, sizeof(st));
+    size_t held = 0;
     if (!fp32) {
         PriaImage<double> d;
         if (d.init(s, lp, stream, din) || d.run(tier, st, o.time_limit, false, s.T, stream) ||
             emit(d))
             return kPriaErrorDevice;
+        held = d.bytes;
     } else {
         LogicParams lp32 = lp;
         apply_fp32_policy(lp32);
         PriaImage<float> f;
         if (f.init(s, lp32, stream, din) || f.run(tier, st, o.time_limit, false, s.T, stream))
             return kPriaErrorDevice;
+        held = f.bytes;
         if (st.stalled) {
             PriaImage<double> d;
             st.switch_iteration = st.iteration;
             if (d.init(s, lp, stream, din) || pria_convert_state(f.job.a, d.job.a, stream) ||
                 d.run(tier, st, o.time_limit, true, s.T, stream) || emit(d))
                 return kPriaErrorDevice;
+            held += d.bytes;
         } else if (emit(f)) {
             return kPriaErrorDevice;
         }
@@ -325,6 +330,7 @@ int solve_mixed(const PriaHostInstance& s, const LogicParams& lp, const pria_opt
         info->status = st.status;
         info->tier = tier;
         info->switch_iteration = st.switch_iteration;
+        info->device_bytes = int64_t(held);
     }
     return 0;
 }
@@ -590,6 +596,7 @@ extern "C" int pria_solve_device(int64_t n, int64_t t, const double* order_costs
         out->status = status;
         return status;
     }
+    out->device_bytes += int64_t(sizeof(double) * doubles);
     // the upper bound of the returned point on the caller's instance
     hipLaunchKernelGGL(pria_objective_rows, grid, blk, 0, stream, holding_costs, xd, int(t),
                        w + f_max, w + f_dot);

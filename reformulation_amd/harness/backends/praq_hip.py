@@ -13,7 +13,7 @@ import math
 import time
 
 from ... import praq
-from ..results import result_class
+from ..results import memory_fields, result_class
 
 #: statuses of the solver record -> harness statuses
 _STATUS = {0: "optimal", 1: "iteration_limit", 2: "time_limit"}
@@ -41,4 +41,4 @@ def solve(instance, *, tolerances, time_limit: float, use_float32: bool = False,
     return Result(status=status, objective=float(info.upper) if status == "optimal" else math.nan,
                   lower=float(info.lower), upper=float(info.upper),
                   iterations=int(info.iterations), setup_seconds=setup, solve_seconds=elapsed,
-                  **common)
+                  **memory_fields(info, sample_memory), **common)

@@ -9,7 +9,7 @@ import math
 import time
 
 from ... import pria
-from ..results import result_class
+from ..results import memory_fields, result_class
 
 _STATUS = {0: "optimal", 1: "iteration_limit", 2: "time_limit"}
 
@@ -38,4 +38,4 @@ def solve(instance, *, tolerances, time_limit: float, use_float32: bool = False,
     return Result(status=status, objective=float(info.upper) if status == "optimal" else math.nan,
                   lower=float(info.lower), upper=float(info.upper),
                   iterations=int(info.iterations), setup_seconds=setup, solve_seconds=elapsed,
-                  **common)
+                  **memory_fields(info, sample_memory), **common)

@@ -75,3 +75,16 @@ def result_class(kind: str):
     except Exception:
         pass
     return QuboResult if kind == "qubo" else JrpResult
+
+
+def memory_fields(info, sample_memory: bool) -> dict:
+    """peak_device_bytes: the device memory the solve held at its peak, as the
+    library counts it (its images and work areas); peak_host_bytes: this
+    process's resident-set high-water mark (the solve runs in-process, so it
+    includes everything the process did before).  Only with sample_memory,
+    as the reference's backends sample only then."""
+    if not sample_memory:
+        return {}
+    import resource
+    return dict(peak_device_bytes=float(info.device_bytes),
+                peak_host_bytes=float(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss) * 1024.0)

@@ -254,3 +254,39 @@ def test_tuning_table_round_trip_and_lane_rule_without_a_gpu(tmp_path):
     assert autotune.lanes_for(1000, 100) == 2 and autotune.lanes_for(10, 2000) == 64
     L.reform_tune_clear()
     autotune.load()                                  # the package's own table, if any
+
+
+def test_ctypes_records_match_the_c_layouts(tmp_path):
+    """Every record the Python side hands to the library has the size and
+    field offsets of its C struct (compiled from include/*.h with gcc): a
+    mismatch would let the library write past a ctypes buffer."""
+    import ctypes
+    import subprocess
+    from reformulation_amd import _native, batch, serve
+    pairs = [("PraqInstance", _native.PraqInstance), ("PraqParameters", _native.PraqParameters),
+             ("PraqResult", _native.PraqResult), ("PraqOptions", _native.PraqOptions),
+             ("PraqInfo", _native.PraqInfo), ("pria_params", _native.PriaParams),
+             ("pria_result_info", _native.PriaResultInfo), ("pria_options", _native.PriaOptions),
+             ("pria_info", _native.PriaInfo), ("reform_kernel_stats", batch.KernelStats),
+             ("reform_server_config", serve.ServerConfig),
+             ("reform_request_info", serve.RequestInfo),
+             ("reform_server_counters", serve.ServerCounters)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "praq_hip.h"',
+             '#include "pria_hip.h"', '#include "reform_batch.h"', '#include "reform_serve.h"',
+             'int main(void) {']
+    for cname, cls in pairs:
+        lines.append(f'    printf("{cname} %zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'    printf("{cname}.{field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ['    return 0;', '}']
+    src = tmp_path / "layouts.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layouts"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True,
+                                                       text=True, check=True).stdout.splitlines())
+    for cname, cls in pairs:
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert int(got[f"{cname}.{field}"]) == getattr(cls, field).offset, (cname, field)
