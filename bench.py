@@ -14,7 +14,10 @@ from the same mixture with its tolerance drawn from {1e-4, 1e-6}, through
 the serving loop (HBM-budgeted admission, continuous batching, backfill),
 p50/p99 latency (completion - arrival) and solves/s, beside the same
 arrivals through a one-at-a-time queue whose service times are the plain
-solver's measured per-instance times with inputs resident; the roofline of the dominant step kernel --
+solver's measured per-instance times with inputs resident (by default 200
+requests/s, below saturation, and 1000 requests/s, above it:
+`served_saturated` is the served rate there against the plain queue's); the
+roofline of the dominant step kernel --
 its launches' summed algorithmic bytes (PRAQ node n(4+7b) + 2m(4+b), edge
 m(8+7b); PRIA x 7b ND, y/z 8b ND per step, b = 4 for FP32) over their summed
 durations, each launch bracketed by HIP events on its stream in one more
@@ -167,7 +170,8 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--rates", default="200", help="serving arrival rates, comma-separated")
+    ap.add_argument("--rates", default="200,1000",
+                    help="serving arrival rates, comma-separated (the highest saturates)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-threads", type=int, default=12)
     ap.add_argument("--no-serve", action="store_true")
@@ -292,6 +296,11 @@ def main() -> int:
                          note=f"serving at {serve_runs[0]['rate']:g} requests/s")
                     if serve_runs else None),
         serving=serve_runs or None,
+        served_saturated=(dict(rate=serve_runs[-1]["rate"],
+                               solves_per_s=serve_runs[-1]["served"]["solves_per_s"],
+                               plain_solves_per_s=serve_runs[-1]["plain"]["solves_per_s"],
+                               ratio=serve_runs[-1]["speedup"])
+                          if len(serve_runs) > 1 else None),
         converged=status.count(0), max_gap=float(gaps.max()),
         roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=PEAK_HBM_GBS,
                       unit="GB/s", frac=achieved / PEAK_HBM_GBS,
