@@ -9,7 +9,6 @@ from __future__ import annotations
 
 import ctypes
 import math
-import os
 import subprocess
 from dataclasses import dataclass
 from pathlib import Path

@@ -7,7 +7,6 @@ the HIP kernels, and a missing library or a missing GPU raises.
 from __future__ import annotations
 
 import ctypes
-import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libreformulation_hip.so"
