@@ -150,6 +150,9 @@ def test_batch_validates_on_the_host_before_any_device_work():
     b.add_praq(loop, praq.Parameters.with_auto_step(100, 1e-6))
     with pytest.raises(RuntimeError, match=r"\(-2\)"):
         b.upload()
+    b.set_compaction(False)                    # host-side switches need no device
+    b.set_compaction(True)
+    assert b._L.reform_batch_set_compaction(None, 1) == -1
     b.close()
 
 
