@@ -190,8 +190,10 @@ __global__ void __launch_bounds__(256) pria_y_kernel(PriaArrays<Real> a, PriaSte
 template <class Real, bool compact>
 __global__ void __launch_bounds__(512) pria_yz_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
     __shared__ double red[kPriaYZRed];
+    extern __shared__ double zcache[];   // kPriaZCacheBytes
     pria_yz_slice<Real, PriaPick<Real, compact>::VR, PriaPick<Real, compact>::VZ>(
-        a, st, blockIdx.x, red, 2);
+        a, st, blockIdx.x, red, 2, 0, 1 << 30, reinterpret_cast<Real*>(zcache),
+        kPriaZCacheBytes / int(sizeof(Real)));
 }
 
 template <class Real>
@@ -304,13 +306,13 @@ struct PriaDevice {
         if (a.T <= kPriaCompactT) {
             hipLaunchKernelGGL((pria_y_kernel<Real, true>), dim3(a.T * chunks), dim3(256), 0,
                                stream, a, st, chunks);
-            hipLaunchKernelGGL((pria_yz_kernel<Real, true>), dim3(a.T), dim3(kPriaYZBlock), 0,
-                               stream, a, st);
+            hipLaunchKernelGGL((pria_yz_kernel<Real, true>), dim3(a.T), dim3(kPriaYZBlock),
+                               kPriaZCacheBytes, stream, a, st);
         } else {
             hipLaunchKernelGGL((pria_y_kernel<Real, false>), dim3(a.T * chunks), dim3(256), 0,
                                stream, a, st, chunks);
-            hipLaunchKernelGGL((pria_yz_kernel<Real, false>), dim3(a.T), dim3(kPriaYZBlock), 0,
-                               stream, a, st);
+            hipLaunchKernelGGL((pria_yz_kernel<Real, false>), dim3(a.T), dim3(kPriaYZBlock),
+                               kPriaZCacheBytes, stream, a, st);
         }
         return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
     }

@@ -312,6 +312,9 @@ __device__ __forceinline__ double group_michelot(const Real (&u)[VR], int len, i
 // values and their order are the same either way).
 // LDS doubles pria_yz_slice's `red` must hold.
 constexpr int kPriaYZRed = 8 * 16;
+// Dynamic LDS of a z-slice workgroup caching a long slice's z + r for its
+// Michelot rounds; with `red`, three 512-thread workgroups still fit a CU.
+constexpr int kPriaZCacheBytes = 48 * 1024;
 
 // Row group widths by row length, the same for every register shape: a row
 // of at most 80 entries takes 16 lanes, at most 160 takes 32, longer rows a

@@ -16,9 +16,6 @@ namespace {
 
 constexpr int kWideBlock = 256;
 constexpr int kRoundsPerPoll = 2;
-// LDS of a y/z block caching a long z slice's sums for its Michelot rounds:
-// with 1 KB of reductions, three 512-thread blocks still fit a CU's 160 KB
-constexpr int kPriaZCacheBytes = 48 * 1024;
 // tables of fewer instances are not compacted further
 constexpr int kShrinkMin = 8;
 
