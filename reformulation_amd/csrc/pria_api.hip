@@ -4,10 +4,13 @@
 #include "pria_hip.h"
 #include "pria_host.hpp"
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -413,6 +416,14 @@ int solve_reference_abi(int64_t n, int64_t t, const Real* k, Real k0, const Real
     return 0;
 }
 
+// pria_solve_batch_f64/f32: independent solo solves fanned out over at most
+// 16 host threads, each solve on a non-blocking stream of its own, so small
+// instances overlap on the device while each computes exactly what a solo
+// pria_solve_f64/f32 computes -- the same tier choice, arithmetic and
+// summation order, hence bitwise the solo answer (the promise of
+// jrp/pria-cuda/include/pria/pria.h:85-99; the same fan-out as
+// jrp/pria-cuda/src/pria.cu:15-44).  Returns 0 when every instance succeeded,
+// else the error code of a failing one.
 template <class Real>
 int solve_batch(int64_t count, int64_t n, int64_t t, const Real* k, const Real* k0,
                 const Real* h, const pria_params* params, Real* x, Real* y, Real* z,
@@ -420,76 +431,23 @@ int solve_batch(int64_t count, int64_t n, int64_t t, const Real* k, const Real* 
     if (count <= 0 || n <= 0 || t <= 0 || !k || !k0 || !h || !params_valid(params))
         return kPriaErrorArgument;
     const size_t sz = size_t(n) * t * t;
-    LogicParams lp = pria_logic_params(*params);
-    std::vector<PriaHostInstance> hosts(count);
-    for (int64_t b = 0; b < count; ++b) {
-        std::vector<double> kd = widen(k + b * n, n), hd = widen(h + b * sz, sz);
-        if (pria_build_host(n, t, kd.data(), double(k0[b]), hd.data(), nullptr, nullptr, nullptr,
-                            hosts[b]))
-            return kPriaErrorArgument;
-    }
-    // every instance as one workgroup of one launch (same code as the solo
-    // workgroup tier, so each result is what a solo workgroup solve gives)
-    const PriaHostInstance& s0 = hosts[0];
-    PriaLayout L = pria_layout(s0.N, s0.T, s0.ND, sizeof(Real));
-    const size_t per = pria_align256(L.total_bytes);
-    const size_t jobs_off = per * count;
-    const size_t states_off = pria_align256(jobs_off + sizeof(PriaJob<Real>) * count);
-    const size_t bytes = states_off + sizeof(LogicState) * count;
-    Buffer buf;
-    if (hipMalloc(&buf.ptr, bytes) != hipSuccess) return kPriaErrorDevice;
-    char* base = static_cast<char*>(buf.ptr);
-    Stream stream;
-    if (hipStreamCreateWithFlags(&stream.s, hipStreamNonBlocking) != hipSuccess)
-        return kPriaErrorDevice;
-    std::vector<char> image(L.input_bytes);
-    std::vector<PriaJob<Real>> jobs(count);
-    for (int64_t b = 0; b < count; ++b) {
-        std::fill(image.begin(), image.end(), 0);
-        pria_fill_image<Real>(hosts[b], L, image.data());
-        if (hipMemcpy(base + per * b, image.data(), pria_upload_bytes(hosts[b], L),
-                      hipMemcpyHostToDevice) != hipSuccess)
-            return kPriaErrorDevice;
-        std::memset(&jobs[b], 0, sizeof(PriaJob<Real>));
-        jobs[b].a = pria_bind<Real>(L, base + per * b, hosts[b]);
-        if (hosts[b].cold && pria_cold_start<Real>(jobs[b].a, stream.s)) return kPriaErrorDevice;
-        jobs[b].p = lp;
-        jobs[b].scale = hosts[b].scale;
-        jobs[b].time_limit_ticks = (long long)(kDefaultWgLimitSeconds * wall_clock_rate_hz());
-        jobs[b].state = reinterpret_cast<LogicState*>(base + states_off) + b;
-    }
-    if (hipMemcpyAsync(base + jobs_off, jobs.data(), sizeof(PriaJob<Real>) * count,
-                       hipMemcpyHostToDevice, stream.s) != hipSuccess)
-        return kPriaErrorDevice;
-    if (pria_launch_wg<Real>(reinterpret_cast<PriaJob<Real>*>(base + jobs_off), int(count), s0.T,
-                             stream.s))
-        return kPriaErrorDevice;
-    std::vector<LogicState> states(count);
-    if (hipMemcpyAsync(states.data(), base + states_off, sizeof(LogicState) * count,
-                       hipMemcpyDeviceToHost, stream.s) != hipSuccess)
-        return kPriaErrorDevice;
-    if (hipStreamSynchronize(stream.s) != hipSuccess) return kPriaErrorDevice;
-    std::vector<Real> v(s0.ND);
-    std::vector<double> dense(sz);
-    for (int64_t b = 0; b < count; ++b) {
-        Real* outs[3] = {x, y, z};
-        Real* srcs[3] = {jobs[b].a.x, jobs[b].a.y, jobs[b].a.z};
-        for (int q = 0; q < 3; ++q) {
-            if (!outs[q]) continue;
-            if (hipMemcpy(v.data(), srcs[q], sizeof(Real) * s0.ND, hipMemcpyDeviceToHost) !=
-                hipSuccess)
-                return kPriaErrorDevice;
-            pria_unpack(hosts[b], v, q, dense.data());
-            for (size_t e = 0; e < sz; ++e) outs[q][b * sz + e] = Real(dense[e]);
+    const int workers = int(std::min<int64_t>(count, 16));
+    std::atomic<int64_t> next(0);
+    std::atomic<int> err(0);
+    auto work = [&]() {
+        for (int64_t b; (b = next.fetch_add(1)) < count;) {
+            const int st = solve_reference_abi<Real>(
+                n, t, k + b * n, k0[b], h + b * sz, params, nullptr, nullptr, nullptr,
+                x ? x + b * sz : nullptr, y ? y + b * sz : nullptr, z ? z + b * sz : nullptr,
+                infos ? infos + b : nullptr);
+            if (st != 0) err.store(st);
         }
-        if (infos) {
-            infos[b].iterations = states[b].iteration;
-            infos[b].primal_objective = states[b].upper;
-            infos[b].dual_objective = states[b].lower;
-            infos[b].relative_gap = states[b].gap;
-        }
-    }
-    return 0;
+    };
+    std::vector<std::thread> threads;
+    for (int w = 1; w < workers; ++w) threads.emplace_back(work);
+    work();
+    for (auto& th : threads) th.join();
+    return err.load();
 }
 
 }  // namespace

@@ -94,19 +94,41 @@ int main(int argc, char** argv) {
         y.resize(size);
         z.resize(size);
     }
-    pria_options o = pria_default_options();
-    pria_info info;
+    // the reference's own entry points, as its CLI calls them: FP64 inputs
+    // and iterates, or the FP32 entry with inputs rounded to float
+    pria_result_info info;
     t0 = std::chrono::steady_clock::now();
-    const int status = pria_solve_ex(n, t, k.data(), k0, h.data(), &p, &o, f32 ? 1 : 0, nullptr,
-                                     nullptr, nullptr, solution ? x.data() : nullptr,
-                                     solution ? y.data() : nullptr, solution ? z.data() : nullptr,
-                                     &info);
+    int status;
+    if (f32) {
+        std::vector<float> kf(k.begin(), k.end()), hf(h.begin(), h.end());
+        std::vector<float> xf, yf, zf;
+        if (solution) {
+            xf.resize(size);
+            yf.resize(size);
+            zf.resize(size);
+        }
+        status = pria_solve_f32(n, t, kf.data(), float(k0), hf.data(), &p, nullptr, nullptr,
+                                nullptr, solution ? xf.data() : nullptr,
+                                solution ? yf.data() : nullptr, solution ? zf.data() : nullptr,
+                                &info);
+        for (size_t g = 0; solution && g < size; ++g) {
+            x[g] = xf[g];
+            y[g] = yf[g];
+            z[g] = zf[g];
+        }
+    } else {
+        status = pria_solve_f64(n, t, k.data(), k0, h.data(), &p, nullptr, nullptr, nullptr,
+                                solution ? x.data() : nullptr, solution ? y.data() : nullptr,
+                                solution ? z.data() : nullptr, &info);
+    }
     const double solve = since(t0);
-    if (status < 0) {
+    if (status != 0) {
         std::fprintf(stderr, "pria_solve_file: pria_solve failed with status %d\n", status);
         return 3;
     }
-    const double lower = info.lower, upper = info.upper;
+    // primal_objective is the objective at the returned point (the upper
+    // bound), dual_objective the lower bound
+    const double lower = info.dual_objective, upper = info.primal_objective;
     const int64_t converged = std::isfinite(info.relative_gap) && info.relative_gap <= tol ? 0 : 1;
 
     FILE* out = std::fopen(argv[2], "wb");

@@ -12,6 +12,20 @@
                  Hammer-Hansen-Simeone implication network, which equals the
                  roof dual.
   pria_lp.json   HiGHS values of the explicit JRP LP on small instances.
+  pria_ref_large.json
+                 the reference CPU solver (as pria_ref.npz) at the sizes the
+                 measured workloads run: the plain-PRIA instance (N = 50,
+                 T = 200) and two odd-T shapes of the batched mixture (37 x 151,
+                 100 x 299), all from the seeded workload generator and solved
+                 to 1e-6: iterations, status, padded-problem objectives, gap,
+                 and checksums of x, y, z (sum, sum of squares, and the dot
+                 product with the fixed weights w[i, s, t] = sin(0.7 k + 0.3),
+                 k the flat index) -- the arrays themselves are too large to
+                 keep.  Several minutes of CPU.
+  praq_c3.json   the oracle's FP64 solves of two PRAQ instances at the batched
+                 mixture's sizes (Erdos-Renyi, n = 2e4 and 1e5, integer
+                 weights) to 1e-6: iterations, restarts, bounds, checksums of
+                 x, and the max-flow roof dual, which the bracket must hold.
   pria_c1.json   the oracle's FP64 solve (iterations, bounds) of the plain-PRIA
                  workload (N = 50, T = 200, seed 0) at 1e-6; the oracle itself
                  is pinned to the reference CPU solver by pria_ref.npz; and
@@ -86,6 +100,65 @@ def make_pria_ref():
     np.savez_compressed(HERE / "pria_ref.npz", **out)
 
 
+PRIA_REF_LARGE = [(0, 50, 200), (301, 37, 151), (302, 100, 299)]
+
+
+def checksum_weights(shape) -> np.ndarray:
+    k = np.arange(int(np.prod(shape)), dtype=np.float64)
+    return np.sin(0.7 * k + 0.3).reshape(shape)
+
+
+def array_checksums(a: np.ndarray) -> list:
+    return [float(a.sum()), float((a * a).sum()), float((a * checksum_weights(a.shape)).sum())]
+
+
+def write_instance(path: Path, k, k0, h) -> None:
+    n, t = int(k.size), int(h.shape[-1])
+    with open(path, "wb") as f:
+        f.write(np.asarray([0x41495250, n, t], np.int64).tobytes())
+        f.write(np.asarray([k0], np.float64).tobytes())
+        f.write(np.ascontiguousarray(k, np.float64).tobytes())
+        f.write(np.ascontiguousarray(h, np.float64).tobytes())
+
+
+def make_pria_ref_large():
+    oracle.build()
+    binary = oracle.REF_BIN
+    if not binary.exists():
+        raise SystemExit("oracle/_ref/pria_ref is not built (needs the reference tree)")
+    rows = []
+    with tempfile.TemporaryDirectory() as tmp:
+        for seed, n, t in PRIA_REF_LARGE:
+            inst = workloads.pria_c1(seed, n, t)
+            src, res = Path(tmp) / "inst.bin", Path(tmp) / "res.bin"
+            write_instance(src, inst.order_costs, inst.joint_order_cost, inst.holding_costs)
+            subprocess.run([str(binary), "solve", str(src), "1e-6", "-1", str(res)], check=True)
+            it, primal, dual, gap, x, y, z = read_result(res, n, t)
+            status = int(np.frombuffer(res.read_bytes()[24:32], np.int64)[0])
+            rows.append(dict(seed=seed, items=n, periods=t, tolerance=1e-6, iterations=it,
+                             status=status, primal=primal, dual=dual, gap=gap,
+                             x=array_checksums(x), y=array_checksums(y), z=array_checksums(z)))
+            print(rows[-1], flush=True)
+    (HERE / "pria_ref_large.json").write_text(json.dumps(rows, indent=1))
+
+
+PRAQ_C3 = [(400, 20_000, 150_000), (401, 100_000, 900_000)]
+
+
+def make_praq_c3():
+    rows = []
+    for seed, n, m in PRAQ_C3:
+        w = workloads.praq_er(seed, n, m)
+        s = oracle.praq_solve(w.n, w.diag, w.rows, w.cols, w.vals,
+                              oracle.praq_params(tolerance=1e-6))
+        rows.append(dict(seed=seed, n=n, m=m, tolerance=1e-6, iterations=s.iterations,
+                         restarts=s.restarts, lower=s.lower, upper=s.upper,
+                         gap=s.relative_gap, x=array_checksums(s.x),
+                         roof_dual=praq_maxflow_value(w)))
+        print(rows[-1], flush=True)
+    (HERE / "praq_c3.json").write_text(json.dumps(rows, indent=1))
+
+
 def make_praq_lp():
     rows = []
     for seed in range(3):
@@ -117,8 +190,16 @@ if __name__ == "__main__":
     if sys.argv[1:] == ["pria_c1"]:
         make_pria_c1()
         sys.exit(0)
+    if sys.argv[1:] == ["praq_c3"]:
+        make_praq_c3()
+        sys.exit(0)
+    if sys.argv[1:] == ["pria_ref_large"]:
+        make_pria_ref_large()
+        sys.exit(0)
     make_pria_c1()
     make_pria_ref()
+    make_pria_ref_large()
+    make_praq_c3()
     make_praq_lp()
     make_pria_lp()
     print("written:", sorted(p.name for p in HERE.iterdir() if p.suffix in (".npz", ".json")))

@@ -65,12 +65,20 @@ def check_answer_properties(inst, x, y):
     (tolerance 1e-5) agree with some binary optimum (brute force); covering
     duals have equal slots in [-w, 0], packing duals are nonnegative and sum
     to w (instance units, w = 2 |W_ij|)."""
+    from reformulation_amd import praq
     n, diag, rows, cols, vals = inst
     opt = half_integral_optimum(*inst)
     assert abs(relaxation_value(*inst, x) - opt) <= 1e-6 * (1 + abs(opt))
+    # the package's post-solve functions (praq-jax relaxation_bounds and
+    # persistent_variables, solver.py:348-362, 592-611): the upper bound is
+    # F(x), the lower bound a weak-duality bound that the solve has closed
+    problem = praq.QuboProblem.from_sparse(n, diag, rows, cols, vals)
+    lower, upper = praq.relaxation_bounds(problem, x, y)
+    assert abs(upper - relaxation_value(*inst, x)) <= 1e-9 * (1 + abs(opt))
+    assert lower <= opt + 1e-9 * (1 + abs(opt)) and opt - lower <= 1e-6 * (1 + abs(opt))
+    fixed, rounded = praq.persistent_variables(x, tolerance=1e-5)
     x = np.asarray(x, dtype=float)
-    rounded = np.round(x)
-    fixed = np.abs(x - rounded) <= 1e-5
+    assert np.array_equal(fixed, np.abs(x - np.round(x)) <= 1e-5)
     optima = brute_force_optima(n, diag, rows, cols, vals)
     assert (optima[:, fixed] == rounded[fixed]).all(axis=1).any()
     y = np.asarray(y, dtype=float).reshape(-1, 2)
@@ -82,3 +90,4 @@ def check_answer_properties(inst, x, y):
     assert (yc[:, 0] <= 1e-9).all() and (yc[:, 0] >= -w[cover] - 1e-9).all()
     assert np.abs(yp.sum(axis=1) - w[~cover]).max(initial=0.0) <= 1e-9 * big
     assert (yp >= -1e-9).all()
+    return int(fixed.sum())
