@@ -1,8 +1,8 @@
 """Plain one-instance solves of the two single-instance configurations, GPU
 against the host oracle on the same instances.
 
-    python -m reformulation_amd.harness.plain [--seeds 10] [--cpu]
-    python -m reformulation_amd.harness.plain --family pria --items 350 --periods 450 \
+    python -m measure.plain [--seeds 10] [--cpu]
+    python -m measure.plain --family pria --items 350 --periods 450 \
         --tolerance 1e-4 --precisions fp32,fp64 --seeds 1
 
   praq   Erdos-Renyi n = 2000, m = 8000, integer weights (`workloads.praq_er`),

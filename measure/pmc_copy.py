@@ -1,6 +1,6 @@
 """Calibration for HBM byte counters: a plain device copy of known size.
 
-    rocprofv3 --pmc FETCH_SIZE -- python3 reformulation_amd/harness/pmc_copy.py
+    rocprofv3 --pmc FETCH_SIZE -- python3 measure/pmc_copy.py
 
 copies a 1 GiB buffer into another `repeats` times; each copy kernel reads
 and writes exactly 1 GiB, so FETCH_SIZE and WRITE_SIZE of those dispatches

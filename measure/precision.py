@@ -1,7 +1,7 @@
 """Precision study of one large PRAQ instance: FP32 iterates (FP64
 reductions, FP64 finish) against all-FP64 iterates, at each tolerance.
 
-    python -m reformulation_amd.harness.precision [--n 1000000] [--tolerances 1e-4,1e-6]
+    python -m measure.precision [--n 1000000] [--tolerances 1e-4,1e-6]
 
 The instance is the Chung-Lu power-law graph of `workloads.praq_chung_lu`
 (exponent 2.5, mean degree 32, q_ij and c_i ~ N(0, 1); n = 1e6 gives about

@@ -194,14 +194,6 @@ int solve_mixed(const PraqHostInstance& s, LogicParams lp, const PraqOptions& o,
     return 0;
 }
 
-bool params_valid(const PraqParameters& p) {
-    return p.tolerance >= 0.0 && p.omega > 0.0 && p.beta_sufficient > 0.0 &&
-           p.beta_necessary > 0.0 && p.beta_artificial >= 0.0 && std::isfinite(p.theta_initial) &&
-           std::isfinite(p.theta_target) && std::isfinite(p.theta_decay) &&
-           std::isfinite(p.interval_offset) && std::isfinite(p.interval_scale) &&
-           p.interval_power >= 0.0;
-}
-
 }  // namespace
 
 extern "C" PraqParameters praq_default_parameters(void) {
@@ -263,7 +255,7 @@ int solve_entry(const PraqInstance* instance, const PraqParameters* parameters,
                 const PraqOptions* options, const double* warm_x, const double* warm_y,
                 hipStream_t stream, bool own_stream, Emit&& emit, PraqInfo* info) {
     PraqParameters p = *parameters;
-    if (!params_valid(p)) return kErrorArgument;
+    if (!praq_params_valid(p)) return kErrorArgument;
     PraqOptions o = options ? *options : praq_default_options();
     if (o.tier < PRAQ_TIER_AUTO || o.tier > PRAQ_TIER_DEVICE) return kErrorArgument;
 
@@ -326,7 +318,7 @@ extern "C" int praq_solve_device(const PraqInstance* instance, const PraqParamet
                                  const PraqOptions* options, double* x, double* y,
                                  PraqInfo* info, void* stream_handle) {
     clear_info(info);
-    if (!instance || !parameters || !x || !params_valid(*parameters)) return kErrorArgument;
+    if (!instance || !parameters || !x || !praq_params_valid(*parameters)) return kErrorArgument;
     if (options && (options->tier < PRAQ_TIER_AUTO || options->tier > PRAQ_TIER_DEVICE))
         return kErrorArgument;
     const int64_t n = instance->n, m = instance->m;

@@ -21,6 +21,18 @@
 
 namespace reform {
 
+// Parameters every PRAQ entry rejects with an argument error (-1): a
+// non-positive omega or restart factor, a NaN or negative tolerance, a
+// non-finite schedule.
+inline bool praq_params_valid(const PraqParameters& p) {
+    return p.tolerance >= 0.0 && p.omega > 0.0 && p.beta_sufficient > 0.0 &&
+           p.beta_necessary > 0.0 && p.beta_artificial >= 0.0 && std::isfinite(p.theta_initial) &&
+           std::isfinite(p.theta_target) && std::isfinite(p.theta_decay) &&
+           std::isfinite(p.interval_offset) && std::isfinite(p.interval_scale) &&
+           p.interval_power >= 0.0;
+}
+
+
 constexpr int kErrorArgument = -1;
 constexpr int kErrorInstance = -2;
 constexpr int kErrorDevice = -3;

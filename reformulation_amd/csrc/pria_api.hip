@@ -69,9 +69,6 @@ struct Stream {
     }
 };
 
-bool params_valid(const pria_params* p) {
-    return p && p->omega > 0.0 && p->eta > 0.0 && p->tolerance >= 0.0;
-}
 
 
 // An instance whose arrays are in device memory: order_costs [n] and the
@@ -348,7 +345,7 @@ int solve_f64io(int64_t n, int64_t t, const double* k, double k0, const double* 
         info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
         info->status = kPriaErrorArgument;
     }
-    if (!params_valid(params)) return kPriaErrorArgument;
+    if (!pria_params_valid(params)) return kPriaErrorArgument;
     pria_options o = options ? *options : pria_default_options();
     PriaHostInstance s;
     if (pria_build_host(n, t, k, k0, h, wx, wy, wz, s)) return kPriaErrorArgument;
@@ -389,7 +386,7 @@ template <class Real>
 int solve_reference_abi(int64_t n, int64_t t, const Real* k, Real k0, const Real* h,
                         const pria_params* params, const Real* wx, const Real* wy,
                         const Real* wz, Real* x, Real* y, Real* z, pria_result_info* info) {
-    if (n <= 0 || t <= 0 || !k || !h || !params_valid(params)) return kPriaErrorArgument;
+    if (n <= 0 || t <= 0 || !k || !h || !pria_params_valid(params)) return kPriaErrorArgument;
     const size_t sz = size_t(n) * t * t;
     std::vector<double> kd = widen(k, n), hd = widen(h, sz), wxd, wyd, wzd;
     if (wx) wxd = widen(wx, sz);
@@ -428,7 +425,7 @@ template <class Real>
 int solve_batch(int64_t count, int64_t n, int64_t t, const Real* k, const Real* k0,
                 const Real* h, const pria_params* params, Real* x, Real* y, Real* z,
                 pria_result_info* infos) {
-    if (count <= 0 || n <= 0 || t <= 0 || !k || !k0 || !h || !params_valid(params))
+    if (count <= 0 || n <= 0 || t <= 0 || !k || !k0 || !h || !pria_params_valid(params))
         return kPriaErrorArgument;
     const size_t sz = size_t(n) * t * t;
     const int workers = int(std::min<int64_t>(count, 16));
@@ -497,7 +494,7 @@ extern "C" int pria_solve_device(int64_t n, int64_t t, const double* order_costs
         info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
         info->status = kPriaErrorArgument;
     }
-    if (!params_valid(params) || n <= 0 || t <= 0 || !order_costs || !holding_costs)
+    if (!pria_params_valid(params) || n <= 0 || t <= 0 || !order_costs || !holding_costs)
         return kPriaErrorArgument;
     const int64_t T = t + (t & 1);
     const int64_t nd = n * T * (T + 1) / 2;

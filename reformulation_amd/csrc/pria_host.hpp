@@ -22,6 +22,13 @@
 
 namespace reform {
 
+// Parameters every PRIA entry rejects with an argument error: a missing
+// record, a non-positive omega or eta, a NaN or negative tolerance.
+inline bool pria_params_valid(const pria_params* p) {
+    return p && p->omega > 0.0 && p->eta > 0.0 && p->tolerance >= 0.0;
+}
+
+
 constexpr int kPriaErrorArgument = -1;
 constexpr int kPriaErrorDevice = -3;
 

@@ -148,6 +148,10 @@ struct Request {
     bool in64 = true;
     std::vector<double> d1, d2, d3;
     std::vector<float> f1, f2, f3;
+    // PRIA: the upper bound of the unpadded original at the answer, computed
+    // in the retire thread while borrowed inputs are still the caller's to
+    // keep (until the request has finished)
+    double upper_original = 0.0;
     int status = kPending;
     double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
     double time_limit = 0.0;  // seconds from admission; <= 0: none
@@ -198,11 +202,25 @@ struct reform_server {
     size_t jobs_cap[2] = {0, 0};
     std::atomic<size_t> bytes_in_use{0}, bytes_peak{0};
 
-    ~reform_server() {
-        stop = true;
+    // Stop every thread: `stop` is set under the lock, so a thread that has
+    // checked its wait predicate under the lock either sees it or is already
+    // waiting when the notifications come (no lost wakeup).
+    void halt() {
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            stop = true;
+        }
+        notify_every_thread();
+    }
+    void notify_every_thread() {
         cv_build.notify_all();
         cv_ready.notify_all();
         cv_retire.notify_all();
+        cv_done.notify_all();
+    }
+
+    ~reform_server() {
+        halt();
         for (auto& b : builders)
             if (b.joinable()) b.join();
         if (server.joinable()) server.join();
@@ -486,11 +504,12 @@ int reform_server::retire_finished(int f) {
 // tables), then its memory and slot back to admission.
 void reform_server::retire_loop() {
     auto fail = [&](int code) {
-        std::lock_guard<std::mutex> lock(mu);
-        start_status = code;
-        stop = true;
-        cv_done.notify_all();
-        cv_ready.notify_all();
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            start_status = code;
+            stop = true;
+        }
+        notify_every_thread();
     };
     hipStream_t s = nullptr;
     if (hipSetDevice(device) != hipSuccess ||
@@ -532,6 +551,12 @@ void reform_server::retire_loop() {
                                copy_out(r.f3, r.rj32.a.z, nd, s));
             }
             if (st || hipStreamSynchronize(s) != hipSuccess) return fail(kErrorDevice);
+            if (r.family == 1) {
+                std::vector<double> x(size_t(r.n) * r.t * r.t);
+                if (r.in64) pria_unpack(r.rh, r.d1, 0, x.data());
+                else pria_unpack(r.rh, r.f1, 0, x.data());
+                r.upper_original = pria_objective_host(r.n, r.t, r.k, r.k0, r.h, x.data());
+            }
             retire(r, r.final.status);
         }
         retire_s = retire_s + (seconds_since(t0) - t_start);
@@ -581,10 +606,12 @@ int reform_server::launch(int f) {
 
 void reform_server::serve_loop() {
     auto fail = [&](int code) {
-        std::lock_guard<std::mutex> lock(mu);
-        start_status = code;
-        stop = true;
-        cv_done.notify_all();
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            start_status = code;
+            stop = true;
+        }
+        notify_every_thread();
     };
     if (hipSetDevice(device) != hipSuccess) return fail(kErrorDevice);
     for (int f = 0; f < 2; ++f) {
@@ -725,7 +752,9 @@ static int64_t enqueue(reform_server* srv, std::unique_ptr<Request> r) {
 extern "C" int64_t reform_server_submit_praq(reform_server* server, const PraqInstance* instance,
                                              const PraqParameters* parameters,
                                              const PraqOptions* options) {
-    if (!server || !instance || !parameters) return kErrorArgument;
+    if (!server || !instance || !parameters || !praq_params_valid(*parameters) ||
+        (options && !(options->eta > 0.0)))
+        return kErrorArgument;
     if (instance->n <= 0 || instance->m < 0 || !instance->diag ||
         (instance->m > 0 && (!instance->rows || !instance->cols || !instance->vals)))
         return kErrorInstance;
@@ -763,7 +792,8 @@ extern "C" int64_t reform_server_submit_pria(reform_server* server, int64_t n, i
                                              const double* order_costs, double joint_cost,
                                              const double* holding_costs, const pria_params* params,
                                              const pria_options* options) {
-    if (!server || !params || n <= 0 || t <= 0 || !order_costs || !holding_costs)
+    if (!server || !pria_params_valid(params) || n <= 0 || t <= 0 || !order_costs ||
+        !holding_costs)
         return kErrorArgument;
     auto r = std::make_unique<Request>();
     r->family = 1;
@@ -833,12 +863,7 @@ extern "C" int reform_server_info(reform_server* server, int64_t ticket, reform_
         info->lower = r->final.lower;
         info->upper = r->final.upper;
         info->relative_gap = r->final.gap;
-        if (r->family == 1) {  // the upper bound of the unpadded original at the answer
-            std::vector<double> x(size_t(r->n) * r->t * r->t);
-            if (r->in64) pria_unpack(r->rh, r->d1, 0, x.data());
-            else pria_unpack(r->rh, r->f1, 0, x.data());
-            info->upper = pria_objective_host(r->n, r->t, r->k, r->k0, r->h, x.data());
-        }
+        if (r->family == 1) info->upper = r->upper_original;
     }
     return 0;
 }

@@ -187,3 +187,47 @@ def test_request_time_limit_stops_only_that_request():
         x, y, z, rinfo = srv.pria_result(t_limited)
         assert np.all(np.isfinite(x)) and np.all(np.isfinite(z))
         assert np.isfinite(rinfo.lower) and np.isfinite(rinfo.upper)
+
+
+def test_submit_rejects_invalid_parameters():
+    """The parameters praq_solve_ex / pria_solve_ex reject are rejected at
+    submission with an argument error (-1), before any admission."""
+    from dataclasses import replace
+    from reformulation_amd import praq, pria
+    from reformulation_amd.serve import Server
+    members = _problems(_members()[:1] + _members()[4:5])
+    (_, qp, qparams), (_, rp, rparams) = members
+    with Server(use_float32=True, host_threads=1) as srv:
+        for bad in (replace(qparams, omega=-1.0), replace(qparams, tolerance=float("nan")),
+                    replace(qparams, theta=float("inf"))):
+            with pytest.raises(ValueError, match=r"\(-1\)"):
+                srv.submit_praq(qp, bad)
+        for bad in (replace(rparams, omega=0.0), replace(rparams, tolerance=float("nan"))):
+            with pytest.raises(ValueError, match=r"\(-1\)"):
+                srv.submit_pria(rp, bad)
+        t = srv.submit_pria(rp, rparams)
+        srv.wait_idle(60.0)
+        assert srv.info(t).status == 0
+
+
+def test_borrowed_inputs_may_go_once_the_request_has_finished():
+    """With borrowed inputs the caller's arrays need to live only until the
+    request has finished: overwritten afterwards (the observable stand-in for
+    freeing them), the request's info and result are unchanged."""
+    from reformulation_amd.serve import Server
+    items = _problems(_members()[4:6])
+    with Server(use_float32=True, host_threads=1) as srv:
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(60.0)
+        before = [srv.info(t) for t in tickets]
+    with Server(use_float32=True, host_threads=1, borrow_inputs=True) as srv:
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(60.0)
+        for t in tickets:
+            for a in srv._keep[t]:
+                a.fill(np.nan)
+        for t, b in zip(tickets, before):
+            info = srv.info(t)
+            assert info.status == 0
+            assert info.upper == b.upper and info.lower == b.lower
+            assert np.isfinite(srv.pria_result(t)[3].upper)
