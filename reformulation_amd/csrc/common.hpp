@@ -13,6 +13,21 @@ namespace reform {
 
 constexpr int kWave = 64;   // CDNA wavefront width
 
+// A pointer into global memory (address space 1).  Array pointers that a
+// kernel reads out of memory (a job record, a shared copy of one) are generic
+// to the compiler, which then emits FLAT accesses: they also count against
+// the LDS counter and, since they may alias the record and LDS, the compiler
+// reloads the record after every store and waits for each access before the
+// next.  Device code works on views whose pointers carry this type, so every
+// access is a global_load/global_store with loads kept in flight.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+    return (gptr<T>)p;
+}
+
 
 template <class T>
 __device__ __forceinline__ T wave_max(T v) {

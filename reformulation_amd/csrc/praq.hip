@@ -30,7 +30,7 @@ namespace reform {
 // ---------------------------------------------------------------------------
 
 template <class Real>
-__device__ void praq_wg_step(const PraqArrays<Real>& a, const PraqStep<Real>& st, int lanes) {
+__device__ void praq_wg_step(const PraqView<Real>& a, const PraqStep<Real>& st, int lanes) {
     const int tid = threadIdx.x, nthreads = blockDim.x;
     const int groups = nthreads / lanes, g = tid / lanes, lane = tid % lanes;
     for (int base = 0; base < a.n; base += groups) praq_node(a, st, base + g, lane, lanes);
@@ -40,8 +40,8 @@ __device__ void praq_wg_step(const PraqArrays<Real>& a, const PraqStep<Real>& st
 }
 
 template <class Real>
-__device__ void praq_wg_sums(const PraqArrays<Real>& a, int lanes, const Real* bx, const Real* by,
-                             double* red, double (&out)[kPraqSums]) {
+__device__ void praq_wg_sums(const PraqView<Real>& a, int lanes, gptr<const Real> bx,
+                             gptr<const Real> by, double* red, double (&out)[kPraqSums]) {
     const int tid = threadIdx.x, nthreads = blockDim.x;
     const int groups = nthreads / lanes, g = tid / lanes, lane = tid % lanes;
     double acc[kPraqSums];
@@ -55,7 +55,8 @@ __device__ void praq_wg_sums(const PraqArrays<Real>& a, int lanes, const Real* b
 }
 
 template <class Real>
-__device__ void praq_wg_copy(Real* dx, const Real* sx, int n, Real* dy, const Real* sy, int m2) {
+__device__ void praq_wg_copy(gptr<Real> dx, gptr<const Real> sx, int n, gptr<Real> dy,
+                             gptr<const Real> sy, int m2) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) dx[i] = sx[i];
     for (int e = threadIdx.x; e < m2; e += blockDim.x) dy[e] = sy[e];
     __syncthreads();
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(1024) praq_wg_solve(const PraqJob<Real>* jobs)
     __shared__ PraqJob<Real> job;
     if (threadIdx.x == 0) job = jobs[blockIdx.x];
     __syncthreads();
-    const PraqArrays<Real> a = job.a;
+    const PraqView<Real> a = praq_view(job.a);
     const LogicParams p = job.p;
     const int lanes = job.lanes;
     const long long t_start = wall_clock64();
@@ -159,21 +160,22 @@ __global__ void __launch_bounds__(1024) praq_wg_solve(const PraqJob<Real>* jobs)
 template <class Real>
 __global__ void praq_node_kernel(PraqArrays<Real> a, PraqStep<Real> st, int lanes) {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    praq_node(a, st, gid / lanes, gid % lanes, lanes);
+    praq_node(praq_view(a), st, gid / lanes, gid % lanes, lanes);
 }
 
 // hub chunks, one wavefront each: FP64 sums of src into chunk_sum
 template <class Real>
 __global__ void praq_chunk_kernel(PraqArrays<Real> a, const Real* src) {
     const int c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (c < a.nc) praq_chunk(a, src, c, threadIdx.x & (kWave - 1));
+    if (c < a.nc) praq_chunk(praq_view(a), to_global(src), c, threadIdx.x & (kWave - 1));
 }
 
 // hubs, one thread each: the chunk sums in order, then the node update
 template <class Real>
 __global__ void praq_hub_kernel(PraqArrays<Real> a, PraqStep<Real> st) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h < a.nh) praq_node_update(a, st, a.heavy[h], Real(praq_hub_total(a, h)));
+    const PraqView<Real> v = praq_view(a);
+    if (h < a.nh) praq_node_update(v, st, a.heavy[h], Real(praq_hub_total(v, h)));
 }
 
 // the hubs' terms of the check sums into one more partial row
@@ -183,8 +185,9 @@ __global__ void praq_hub_sums_kernel(PraqArrays<Real> a, const Real* bx, double*
     double acc[kPraqSums];
 #pragma unroll
     for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
+    const PraqView<Real> v = praq_view(a);
     for (int h = threadIdx.x; h < a.nh; h += blockDim.x)
-        praq_node_terms(a, a.heavy[h], praq_hub_total(a, h), bx, acc);
+        praq_node_terms(v, a.heavy[h], praq_hub_total(v, h), to_global(bx), acc);
     block_sums<kPraqSums>(acc, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < kPraqSums; ++k) row[k] = acc[k];
@@ -193,7 +196,7 @@ __global__ void praq_hub_sums_kernel(PraqArrays<Real> a, const Real* bx, double*
 template <class Real>
 __global__ void praq_edge_kernel(PraqArrays<Real> a, PraqStep<Real> st) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < a.m) praq_edge(a, st, e);
+    if (e < a.m) praq_edge(praq_view(a), st, e);
 }
 
 template <class Real>
@@ -206,9 +209,11 @@ __global__ void praq_sums_kernel(PraqArrays<Real> a, int lanes, const Real* bx, 
     const int nthreads = gridDim.x * blockDim.x;
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int groups = nthreads / lanes;
+    const PraqView<Real> v = praq_view(a);
+    const gptr<const Real> gx = to_global(bx), gy = to_global(by);
     for (int base = 0; base < a.n; base += groups)
-        praq_node_sums(a, base + gid / lanes, gid % lanes, lanes, bx, by, acc);
-    for (int e = gid; e < a.m; e += nthreads) praq_edge_sums(a, e, bx, by, true, acc);
+        praq_node_sums(v, base + gid / lanes, gid % lanes, lanes, gx, gy, acc);
+    for (int e = gid; e < a.m; e += nthreads) praq_edge_sums(v, e, gx, gy, true, acc);
     block_sums<kPraqSums>(acc, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < kPraqSums; ++k) partial[blockIdx.x * kPraqSums + k] = acc[k];
@@ -425,7 +430,7 @@ template int praq_device_solve<double>(const PraqJob<double>&, LogicState&, doub
 // precision switch: the whole Halpern state, FP32 -> FP64
 // ---------------------------------------------------------------------------
 
-__device__ inline void praq_convert_one(const PraqArrays<float>& f, const PraqArrays<double>& d,
+__device__ inline void praq_convert_one(const PraqView<float>& f, const PraqView<double>& d,
                                         int k) {
     if (k < f.n) {
         d.x[k] = f.x[k];
@@ -442,13 +447,13 @@ __device__ inline void praq_convert_one(const PraqArrays<float>& f, const PraqAr
 
 __global__ void praq_convert_kernel(PraqArrays<float> f, PraqArrays<double> d) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    praq_convert_one(f, d, k);
+    praq_convert_one(praq_view(f), praq_view(d), k);
 }
 
 // one workgroup per job, grid-stride inside the instance
 __global__ void praq_convert_jobs_kernel(const PraqJob<float>* from, const PraqJob<double>* to) {
-    const PraqArrays<float> f = from[blockIdx.x].a;
-    const PraqArrays<double> d = to[blockIdx.x].a;
+    const PraqView<float> f = praq_view(from[blockIdx.x].a);
+    const PraqView<double> d = praq_view(to[blockIdx.x].a);
     const int count = f.n > 2 * f.m ? f.n : 2 * f.m;
     for (int k = threadIdx.x; k < count; k += blockDim.x) praq_convert_one(f, d, k);
 }

@@ -59,6 +59,53 @@ struct PraqArrays {
     double* chunk_sum;
 };
 
+// The device view of PraqArrays: the same arrays through global-memory
+// pointers (see gptr); every device function works on a view.
+template <class Real>
+struct PraqView {
+    int n, m;
+    gptr<const int> e0, e1;
+    gptr<const Real> w;
+    gptr<const int> ptr, half;
+    gptr<const Real> diag, deg, xs, ys;
+    gptr<Real> x, sx, x0, refl, yc, y, sy, y0;
+    int nh, nc, heavy_degree;
+    gptr<const int> heavy, hchunk, chunk_node;
+    gptr<double> chunk_sum;
+};
+
+template <class Real>
+__device__ __forceinline__ PraqView<Real> praq_view(const PraqArrays<Real>& a) {
+    PraqView<Real> v;
+    v.n = a.n;
+    v.m = a.m;
+    v.e0 = to_global(a.e0);
+    v.e1 = to_global(a.e1);
+    v.w = to_global(a.w);
+    v.ptr = to_global(a.ptr);
+    v.half = to_global(a.half);
+    v.diag = to_global(a.diag);
+    v.deg = to_global(a.deg);
+    v.xs = to_global(a.xs);
+    v.ys = to_global(a.ys);
+    v.x = to_global(a.x);
+    v.sx = to_global(a.sx);
+    v.x0 = to_global(a.x0);
+    v.refl = to_global(a.refl);
+    v.yc = to_global(a.yc);
+    v.y = to_global(a.y);
+    v.sy = to_global(a.sy);
+    v.y0 = to_global(a.y0);
+    v.nh = a.nh;
+    v.nc = a.nc;
+    v.heavy_degree = a.heavy_degree;
+    v.heavy = to_global(a.heavy);
+    v.hchunk = to_global(a.hchunk);
+    v.chunk_node = to_global(a.chunk_node);
+    v.chunk_sum = to_global(a.chunk_sum);
+    return v;
+}
+
 // Half-edges per hub chunk: one wavefront, 16 per lane.
 constexpr int kPraqChunk = 1024;
 
@@ -98,7 +145,7 @@ __host__ __device__ inline PraqStep<Real> praq_step_scalars(int64_t restart_iter
 // group's lanes stride the CSR list and meet in an xor butterfly, so the sum
 // depends only on the instance and `lanes`.
 template <class Real>
-__device__ __forceinline__ void praq_node_update(const PraqArrays<Real>& a,
+__device__ __forceinline__ void praq_node_update(const PraqView<Real>& a,
                                                  const PraqStep<Real>& st, int i, Real acc) {
     Real xc = a.x[i], sxi = a.sx[i];
     Real x1 = st.reflect ? st.lw * (Real(2) * sxi - xc) + st.rw * a.x0[i] : sxi;
@@ -110,7 +157,7 @@ __device__ __forceinline__ void praq_node_update(const PraqArrays<Real>& a,
 }
 
 template <class Real>
-__device__ __forceinline__ void praq_node(const PraqArrays<Real>& a, const PraqStep<Real>& st,
+__device__ __forceinline__ void praq_node(const PraqView<Real>& a, const PraqStep<Real>& st,
                                           int i, int lane, int lanes) {
     Real acc = Real(0);
     int begin = 0, end = 0;
@@ -120,7 +167,7 @@ __device__ __forceinline__ void praq_node(const PraqArrays<Real>& a, const PraqS
     }
     const bool hub = end - begin > a.heavy_degree;  // done by its chunks instead
     if (hub) end = begin;
-    const Real* src = st.reflect ? a.yc : a.sy;
+    const gptr<const Real> src = st.reflect ? a.yc : a.sy;
     for (int k = begin + lane; k < end; k += lanes) acc += src[a.half[k]];
     acc = group_sum(acc, lanes);
     if (i >= a.n || lane != 0 || hub) return;
@@ -130,7 +177,7 @@ __device__ __forceinline__ void praq_node(const PraqArrays<Real>& a, const PraqS
 // One hub chunk by one wavefront (all 64 lanes call it): the FP64 sum of
 // src over the chunk's half-edges, lanes striding, DPP reduction.
 template <class Real>
-__device__ __forceinline__ void praq_chunk(const PraqArrays<Real>& a, const Real* src, int c,
+__device__ __forceinline__ void praq_chunk(const PraqView<Real>& a, gptr<const Real> src, int c,
                                            int lane) {
     const int h = a.chunk_node[c], i = a.heavy[h];
     const int begin = a.ptr[i] + (c - a.hchunk[h]) * kPraqChunk;
@@ -143,7 +190,7 @@ __device__ __forceinline__ void praq_chunk(const PraqArrays<Real>& a, const Real
 
 // Hub h's total, its chunks added in order.
 template <class Real>
-__device__ __forceinline__ double praq_hub_total(const PraqArrays<Real>& a, int h) {
+__device__ __forceinline__ double praq_hub_total(const PraqView<Real>& a, int h) {
     double acc = 0.0;
     for (int c = a.hchunk[h]; c < a.hchunk[h + 1]; ++c) acc += a.chunk_sum[c];
     return acc;
@@ -151,7 +198,7 @@ __device__ __forceinline__ double praq_hub_total(const PraqArrays<Real>& a, int 
 
 // Edge e's update: projection of y1 + sigma (q - S refl), and the next yc.
 template <class Real>
-__device__ __forceinline__ void praq_edge(const PraqArrays<Real>& a, const PraqStep<Real>& st,
+__device__ __forceinline__ void praq_edge(const PraqView<Real>& a, const PraqStep<Real>& st,
                                           int e) {
     const int m = a.m;
     Real y1a, y1b, ana, anb;
@@ -207,8 +254,8 @@ enum PraqSum {
 
 // Node i's terms of the check sums given (S^T by)_i.
 template <class Real>
-__device__ __forceinline__ void praq_node_terms(const PraqArrays<Real>& a, int i, double scat,
-                                                const Real* bx, double (&acc)[kPraqSums]) {
+__device__ __forceinline__ void praq_node_terms(const PraqView<Real>& a, int i, double scat,
+                                                gptr<const Real> bx, double (&acc)[kPraqSums]) {
     double xi = a.x[i], sxi = a.sx[i], deg = a.deg[i], di = a.diag[i];
     double dx = xi - sxi, dr = xi - double(a.x0[i]);
     acc[kPrimalNorm] += deg * dx * dx;
@@ -220,8 +267,8 @@ __device__ __forceinline__ void praq_node_terms(const PraqArrays<Real>& a, int i
 
 // Node part, by a lane group as praq_node; bounds at (bx, by).
 template <class Real>
-__device__ __forceinline__ void praq_node_sums(const PraqArrays<Real>& a, int i, int lane,
-                                               int lanes, const Real* bx, const Real* by,
+__device__ __forceinline__ void praq_node_sums(const PraqView<Real>& a, int i, int lane,
+                                               int lanes, gptr<const Real> bx, gptr<const Real> by,
                                                double (&acc)[kPraqSums]) {
     double scat = 0.0;
     int begin = 0, end = 0;
@@ -238,8 +285,8 @@ __device__ __forceinline__ void praq_node_sums(const PraqArrays<Real>& a, int i,
 }
 
 template <class Real>
-__device__ __forceinline__ void praq_edge_sums(const PraqArrays<Real>& a, int e, const Real* bx,
-                                               const Real* by, bool with_norms,
+__device__ __forceinline__ void praq_edge_sums(const PraqView<Real>& a, int e, gptr<const Real> bx,
+                                               gptr<const Real> by, bool with_norms,
                                                double (&acc)[kPraqSums]) {
     const int m = a.m;
     int i = a.e0[e], j = a.e1[e];

@@ -23,7 +23,7 @@ namespace reform {
 // ---------------------------------------------------------------------------
 
 template <class Real>
-__device__ void pria_wg_step(const PriaArrays<Real>& a, const PriaStep<Real>& st, Real* lds,
+__device__ void pria_wg_step(const PriaView<Real>& a, const PriaStep<Real>& st, Real* lds,
                              double* red) {
     constexpr int TW = PriaShape<Real>::TW;
     const int tiles = (a.T + TW - 1) / TW;
@@ -34,8 +34,8 @@ __device__ void pria_wg_step(const PriaArrays<Real>& a, const PriaStep<Real>& st
 }
 
 template <class Real>
-__device__ void pria_wg_sums(const PriaArrays<Real>& a, const Real* bx, const Real* by,
-                             const Real* bz, double* red, double (&out)[kPriaSums]) {
+__device__ void pria_wg_sums(const PriaView<Real>& a, gptr<const Real> bx, gptr<const Real> by,
+                             gptr<const Real> bz, double* red, double (&out)[kPriaSums]) {
     constexpr int TW = PriaShape<Real>::TW;
     double acc[kPriaSums];
     for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
@@ -47,8 +47,8 @@ __device__ void pria_wg_sums(const PriaArrays<Real>& a, const Real* bx, const Re
 }
 
 template <class Real>
-__device__ void pria_wg_copy3(Real* d1, const Real* s1, Real* d2, const Real* s2, Real* d3,
-                              const Real* s3, int nd) {
+__device__ void pria_wg_copy3(gptr<Real> d1, gptr<const Real> s1, gptr<Real> d2,
+                              gptr<const Real> s2, gptr<Real> d3, gptr<const Real> s3, int nd) {
     for (int e = threadIdx.x; e < nd; e += blockDim.x) {
         d1[e] = s1[e];
         d2[e] = s2[e];
@@ -66,14 +66,15 @@ __global__ void __launch_bounds__(1024) pria_wg_solve(const PriaJob<Real>* jobs)
     __shared__ PriaJob<Real> job;
     if (threadIdx.x == 0) job = jobs[blockIdx.x];
     __syncthreads();
-    const PriaArrays<Real> a = job.a;
+    const PriaView<Real> a = pria_view(job.a);
+    LogicState* const state = job.state;
     const LogicParams p = job.p;
     const long long t_start = wall_clock64();
 
     double sums[kPriaSums];
     if (job.resume) {
         if (threadIdx.x == 0) {
-            st = *job.state;
+            st = *state;
             st.stalled = 0;
         }
         __syncthreads();
@@ -143,7 +144,7 @@ __global__ void __launch_bounds__(1024) pria_wg_solve(const PriaJob<Real>* jobs)
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *job.state = st;
+    if (threadIdx.x == 0) *state = st;
 }
 
 // ---------------------------------------------------------------------------
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(512) pria_x_kernel(PriaArrays<Real> a, PriaSte
     // heaviest tiles (largest t) first
     const int b = blockIdx.x;
     const int q = tiles - 1 - (b % tiles), i = b / tiles;
-    pria_x_tile<Real, TW>(a, st, i, q * TW, reinterpret_cast<Real*>(dyn));
+    pria_x_tile<Real, TW>(pria_view(a), st, i, q * TW, reinterpret_cast<Real*>(dyn));
 }
 
 // The device tier's y rows: kPriaYRows rows of one slice per 256-thread
@@ -184,7 +185,7 @@ __global__ void __launch_bounds__(256) pria_y_kernel(PriaArrays<Real> a, PriaSte
                                                      int chunks) {
     const int s = blockIdx.x / chunks, i_lo = (blockIdx.x % chunks) * kPriaYRows;
     pria_yz_slice<Real, PriaPick<Real, compact>::VR, PriaPick<Real, compact>::VZ>(
-        a, st, s, nullptr, 1, i_lo, i_lo + kPriaYRows);
+        pria_view(a), st, s, nullptr, 1, i_lo, i_lo + kPriaYRows);
 }
 
 template <class Real, bool compact>
@@ -192,7 +193,7 @@ __global__ void __launch_bounds__(512) pria_yz_kernel(PriaArrays<Real> a, PriaSt
     __shared__ double red[kPriaYZRed];
     extern __shared__ double zcache[];   // kPriaZCacheBytes
     pria_yz_slice<Real, PriaPick<Real, compact>::VR, PriaPick<Real, compact>::VZ>(
-        a, st, blockIdx.x, red, 2, 0, 1 << 30, reinterpret_cast<Real*>(zcache),
+        pria_view(a), st, blockIdx.x, red, 2, 0, 1 << 30, reinterpret_cast<Real*>(zcache),
         kPriaZCacheBytes / int(sizeof(Real)));
 }
 
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(256) pria_check_slice_kernel(PriaArrays<Real> 
     __shared__ double red[kPriaSums * 16];
     double acc[kPriaSums];
     for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
-    pria_slice_sums(a, blockIdx.x, bx, red, acc);
+    pria_slice_sums(pria_view(a), blockIdx.x, to_global(bx), red, acc);
     if (threadIdx.x == 0)
         for (int q = 0; q < kPriaSums; ++q) partial[blockIdx.x * kPriaSums + q] = acc[q];
 }
@@ -215,7 +216,8 @@ __global__ void __launch_bounds__(256) pria_check_column_kernel(PriaArrays<Real>
     const int tiles = (a.T + TW - 1) / TW;
     double acc[kPriaSums];
     for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
-    pria_column_sums<Real, TW>(a, blockIdx.x / tiles, (blockIdx.x % tiles) * TW, by, bz, red, acc);
+    pria_column_sums<Real, TW>(pria_view(a), blockIdx.x / tiles, (blockIdx.x % tiles) * TW,
+                               to_global(by), to_global(bz), red, acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = acc[kPLower];
 }
 
@@ -235,7 +237,8 @@ __global__ void pria_sums_final(const double* slices, int nslices, const double*
 
 template <class Real>
 __global__ void pria_reset_thresholds_kernel(PriaArrays<Real> a) {
-    pria_reset_thresholds(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    pria_reset_thresholds(pria_view(a), blockIdx.x * blockDim.x + threadIdx.x,
+                          gridDim.x * blockDim.x);
 }
 
 template <class Real>
@@ -431,7 +434,7 @@ template <class Real>
 __global__ void __launch_bounds__(1024) pria_wg_step_kernel(PriaArrays<Real> a, PriaStep<Real> st) {
     extern __shared__ char dyn[];
     __shared__ double red[kPriaYZRed];
-    pria_wg_step(a, st, reinterpret_cast<Real*>(dyn), red);
+    pria_wg_step(pria_view(a), st, reinterpret_cast<Real*>(dyn), red);
 }
 
 template <class Real>
@@ -466,7 +469,7 @@ template int pria_device_solve<double>(const PriaJob<double>&, LogicState&, doub
                                        hipStream_t, bool);
 
 // precision switch: the whole Halpern state, FP32 -> FP64
-__device__ inline void pria_convert_one(const PriaArrays<float>& f, const PriaArrays<double>& d,
+__device__ inline void pria_convert_one(const PriaView<float>& f, const PriaView<double>& d,
                                         int e) {
     if (e < f.N * f.T) {
         d.thx[e] = f.thx[e];
@@ -489,12 +492,12 @@ __device__ inline void pria_convert_one(const PriaArrays<float>& f, const PriaAr
 
 __global__ void pria_convert_kernel(PriaArrays<float> f, PriaArrays<double> d) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < f.ND) pria_convert_one(f, d, e);
+    if (e < f.ND) pria_convert_one(pria_view(f), pria_view(d), e);
 }
 
 __global__ void pria_convert_jobs_kernel(const PriaJob<float>* from, const PriaJob<double>* to) {
-    const PriaArrays<float> f = from[blockIdx.x].a;
-    const PriaArrays<double> d = to[blockIdx.x].a;
+    const PriaView<float> f = pria_view(from[blockIdx.x].a);
+    const PriaView<double> d = pria_view(to[blockIdx.x].a);
     for (int e = threadIdx.x; e < f.ND; e += blockDim.x) pria_convert_one(f, d, e);
 }
 

@@ -57,6 +57,54 @@ struct PriaArrays {
     double *thx, *thy, *thz;    // last thresholds: [N T'] columns, [N T'] rows, [T'] slices
 };
 
+// The device view of PriaArrays: the same arrays through global-memory
+// pointers (see gptr); every device function works on a view.
+template <class Real>
+struct PriaView {
+    int N, T;
+    int ND;
+    gptr<const Real> h;
+    gptr<const Real> k;
+    Real k0;
+    gptr<const Real> xs, ys, zs;
+    gptr<Real> xc, x0, x, sx;
+    gptr<Real> yc, y0, y, sy;
+    gptr<Real> zc, z0, z, sz;
+    gptr<Real> r;
+    gptr<double> thx, thy, thz;
+};
+
+template <class Real>
+__device__ __forceinline__ PriaView<Real> pria_view(const PriaArrays<Real>& a) {
+    PriaView<Real> v;
+    v.N = a.N;
+    v.T = a.T;
+    v.ND = a.ND;
+    v.h = to_global(a.h);
+    v.k = to_global(a.k);
+    v.k0 = a.k0;
+    v.xs = to_global(a.xs);
+    v.ys = to_global(a.ys);
+    v.zs = to_global(a.zs);
+    v.xc = to_global(a.xc);
+    v.x0 = to_global(a.x0);
+    v.x = to_global(a.x);
+    v.sx = to_global(a.sx);
+    v.yc = to_global(a.yc);
+    v.y0 = to_global(a.y0);
+    v.y = to_global(a.y);
+    v.sy = to_global(a.sy);
+    v.zc = to_global(a.zc);
+    v.z0 = to_global(a.z0);
+    v.z = to_global(a.z);
+    v.sz = to_global(a.sz);
+    v.r = to_global(a.r);
+    v.thx = to_global(a.thx);
+    v.thy = to_global(a.thy);
+    v.thz = to_global(a.thz);
+    return v;
+}
+
 __host__ __device__ inline int pria_off(int N, int T, int s) {
     return N * (s * T - (s * (s - 1)) / 2);
 }
@@ -88,7 +136,7 @@ __host__ __device__ inline PriaStep<Real> pria_step_scalars(int64_t restart_iter
 // The thresholds of a fresh solve: none (NaN), so the first projection of
 // each segment starts from its full set; entries k, k + stride, ...
 template <class Real>
-__device__ inline void pria_reset_thresholds(const PriaArrays<Real>& a, int k, int stride) {
+__device__ inline void pria_reset_thresholds(const PriaView<Real>& a, int k, int stride) {
     const double none = __builtin_nan("");
     for (int e = k; e < a.N * a.T; e += stride) {
         a.thx[e] = none;
@@ -101,7 +149,7 @@ __device__ inline void pria_reset_thresholds(const PriaArrays<Real>& a, int k, i
 //
 // lds: (T' x (TW + 1)) pre-projection values + TW thresholds.
 template <class Real, int TW>
-__device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st, int i, int t0,
+__device__ void pria_x_tile(const PriaView<Real>& a, const PriaStep<Real>& st, int i, int t0,
                             Real* lds) {
     const int T = a.T, N = a.N;
     const int BS = blockDim.x, tid = threadIdx.x;
@@ -110,27 +158,31 @@ __device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st,
     Real* theta = lds + T * stride;
     const int tcount = min(TW, T - t0);
     const int smax = t0 + tcount - 1;
-    const Real* srcx = st.reflect ? a.xc : a.sx;
-    const Real* srcy = st.reflect ? a.yc : a.sy;
-    const Real* srcz = st.reflect ? a.zc : a.sz;
+    const gptr<const Real> srcx = st.reflect ? a.xc : a.sx;
+    const gptr<const Real> srcy = st.reflect ? a.yc : a.sy;
+    const gptr<const Real> srcz = st.reflect ? a.zc : a.sz;
 
     // load: lanes over columns, G rows of s at a time (coalesced per s); U
     // rows' loads are issued before any is used, so their latencies overlap
     constexpr int U = 8;
     const int c = tid % TW, row = tid / TW, t = t0 + c;
     const bool colok = c < tcount;
+    // (the loads are unconditional -- entries outside the tile read entry 0
+    // and are dropped -- so the compiler issues the batch without branches)
     for (int s0 = row; s0 <= smax; s0 += G * U) {
-        Real vx[U], vh[U], vy[U], vz[U];
+        int g[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int s = s0 + u * G;
-            if (colok && s <= t) {
-                const int g = pria_off(N, T, s) + i * (T - s) + (t - s);
-                vx[u] = srcx[g];
-                vh[u] = a.h[g];
-                vz[u] = srcz[g];
-                vy[u] = srcy[g];
-            }
+            g[u] = colok && s <= t ? pria_off(N, T, s) + i * (T - s) + (t - s) : 0;
+        }
+        Real vx[U], vh[U], vy[U], vz[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            vx[u] = srcx[g[u]];
+            vh[u] = a.h[g[u]];
+            vz[u] = srcz[g[u]];
+            vy[u] = srcy[g[u]];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -204,30 +256,36 @@ __device__ void pria_x_tile(const PriaArrays<Real>& a, const PriaStep<Real>& st,
     {
         const Real th = theta[c < TW ? c : 0];
         for (int s0 = row; s0 <= smax; s0 += G * U) {
-            Real vx[U], v0[U];
+            int g[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int s = s0 + u * G;
-                if (colok && s <= t) {
-                    const int g = pria_off(N, T, s) + i * (T - s) + (t - s);
-                    vx[u] = srcx[g];
-                    v0[u] = st.reflect ? a.x0[g] : vx[u];
-                }
+                g[u] = colok && s <= t ? pria_off(N, T, s) + i * (T - s) + (t - s) : 0;
+            }
+            Real vx[U], v0[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) vx[u] = srcx[g[u]];
+            if (st.reflect) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) v0[u] = a.x0[g[u]];
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) v0[u] = vx[u];
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int s = s0 + u * G;
                 if (colok && s <= t) {
-                    const int g = pria_off(N, T, s) + i * (T - s) + (t - s);
+                    const int gu = g[u];
                     const Real x1 = vx[u];
                     const Real d = lds[s * stride + c] - th;
                     const Real x2 = d > Real(0) ? d : Real(0);
                     const Real refl = Real(2) * x2 - x1;
-                    a.r[g] = st.sigma * refl;
-                    a.xc[g] = st.lwn * refl + st.rwn * v0[u];
+                    a.r[gu] = st.sigma * refl;
+                    a.xc[gu] = st.lwn * refl + st.rwn * v0[u];
                     if (st.store) {
-                        a.x[g] = x1;
-                        a.sx[g] = x2;
+                        a.x[gu] = x1;
+                        a.sx[gu] = x2;
                     }
                 }
             }
@@ -325,7 +383,7 @@ constexpr int kPriaZCacheBytes = 48 * 1024;
 constexpr int kPriaRow16 = 80, kPriaRow32 = 160;
 
 template <class Real, int VR, int VZ>
-__device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& st, int s,
+__device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st, int s,
                               double* red, int part = 3, int i_lo = 0, int i_hi = 1 << 30,
                               Real* zcache = nullptr, int zcap = 0) {
     const int T = a.T, N = a.N;
@@ -333,8 +391,8 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
     const int lane = tid & (kWave - 1), wave = tid / kWave, waves = BS / kWave;
     const int len = T - s;
     const int base = pria_off(N, T, s);
-    const Real* srcy = st.reflect ? a.yc : a.sy;
-    const Real* srcz = st.reflect ? a.zc : a.sz;
+    const gptr<const Real> srcy = st.reflect ? a.yc : a.sy;
+    const gptr<const Real> srcz = st.reflect ? a.zc : a.sz;
 
     // ---- y: rows [i_lo, i_hi) of the slice, 64 / width rows per wavefront
     // at a time ----
@@ -345,18 +403,33 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
         const int i = i0 + sub;
         const bool row = i < rows_end;
         const int rb = base + (row ? i : 0) * len;
-        Real y1[VR], u[VR], an[VR];
+        // unconditional loads (entries past the row read the row's first
+        // entry and are dropped), issued as one batch
+        int g[VR];
 #pragma unroll
         for (int k = 0; k < VR; ++k) {
             const int e = lr + width * k;
-            y1[k] = Real(0);
-            u[k] = Real(0);
-            an[k] = Real(0);
-            if (row && e < len) {
-                y1[k] = srcy[rb + e];
-                u[k] = y1[k] + a.r[rb + e];
-                an[k] = st.reflect ? a.y0[rb + e] : y1[k];
-            }
+            g[k] = rb + (row && e < len ? e : 0);
+        }
+        Real y1[VR], u[VR], an[VR];
+#pragma unroll
+        for (int k = 0; k < VR; ++k) {
+            y1[k] = srcy[g[k]];
+            u[k] = a.r[g[k]];
+        }
+        if (st.reflect) {
+#pragma unroll
+            for (int k = 0; k < VR; ++k) an[k] = a.y0[g[k]];
+        } else {
+#pragma unroll
+            for (int k = 0; k < VR; ++k) an[k] = y1[k];
+        }
+#pragma unroll
+        for (int k = 0; k < VR; ++k) {
+            const bool in = row && lr + width * k < len;
+            u[k] = in ? y1[k] + u[k] : Real(0);
+            y1[k] = in ? y1[k] : Real(0);
+            an[k] = in ? an[k] : Real(0);
         }
         const double target = row ? double(a.k[i]) : 0.0;
         const bool live = row && target > 0.0;
@@ -392,12 +465,20 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
         if (L <= BS * VZ) {
             Real u[VZ];
             double v4[4] = {0.0, 0.0, 0.0, 0.0};
+            // unconditional loads up to the last k any thread needs (a
+            // block-uniform bound); entries past the slice read its first
+            // entry and are dropped
+            const int kn = (L + BS - 1) / BS;
 #pragma unroll
             for (int k = 0; k < VZ; ++k) {
                 const int e = tid + BS * k;
-                u[k] = Real(0);
+                const int g = base + (e < L ? e : 0);
+                u[k] = k < kn ? srcz[g] + a.r[g] : Real(0);
+            }
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) {
+                const int e = tid + BS * k;
                 if (e < L) {
-                    u[k] = srcz[base + e] + a.r[base + e];
                     const double v = double(u[k]);
                     v4[0] += v;
                     v4[1] += 1.0;
@@ -469,15 +550,24 @@ __device__ void pria_yz_slice(const PriaArrays<Real>& a, const PriaStep<Real>& s
     constexpr int U = 4;
     const Real thr = Real(th);
     for (int e0 = tid; e0 < L; e0 += BS * U) {
-        Real v1[U], rr[U], an[U];
+        int g[U];
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             const int e = e0 + BS * q;
-            if (e < L) {
-                v1[q] = srcz[base + e];
-                rr[q] = a.r[base + e];
-                an[q] = st.reflect ? a.z0[base + e] : v1[q];
-            }
+            g[q] = base + (e < L ? e : 0);
+        }
+        Real v1[U], rr[U], an[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            v1[q] = srcz[g[q]];
+            rr[q] = a.r[g[q]];
+        }
+        if (st.reflect) {
+#pragma unroll
+            for (int q = 0; q < U; ++q) an[q] = a.z0[g[q]];
+        } else {
+#pragma unroll
+            for (int q = 0; q < U; ++q) an[q] = v1[q];
         }
 #pragma unroll
         for (int q = 0; q < U; ++q) {
@@ -532,7 +622,7 @@ enum PriaSum {
 // workgroup; returns the six sums in acc[0..5] of thread 0 (all threads hold
 // the totals after block_sums).  Bounds at (bx): upper bound terms.
 template <class Real>
-__device__ void pria_slice_sums(const PriaArrays<Real>& a, int s, const Real* bx, double* red,
+__device__ void pria_slice_sums(const PriaView<Real>& a, int s, gptr<const Real> bx, double* red,
                                 double (&acc)[kPriaSums]) {
     const int T = a.T, N = a.N;
     const int BS = blockDim.x, tid = threadIdx.x;
@@ -578,8 +668,8 @@ __device__ void pria_slice_sums(const PriaArrays<Real>& a, int s, const Real* bx
 // Column minima of (h + by + bz) over s <= t for the tile of item i starting
 // at t0 (TW columns); adds the tile's sum of minima to acc[kPLower].
 template <class Real, int TW>
-__device__ void pria_column_sums(const PriaArrays<Real>& a, int i, int t0, const Real* by,
-                                 const Real* bz, double* red, double (&acc)[kPriaSums]) {
+__device__ void pria_column_sums(const PriaView<Real>& a, int i, int t0, gptr<const Real> by,
+                                 gptr<const Real> bz, double* red, double (&acc)[kPriaSums]) {
     const int T = a.T, N = a.N;
     const int BS = blockDim.x, tid = threadIdx.x;
     const int G = BS / TW;

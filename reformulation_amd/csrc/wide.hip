@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(256) wpraq_node(const PJ32* j32, const PJ64* j
         if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
         const int lanes = job.lanes;
         const int gid = t.y * kWideBlock + threadIdx.x;
-        praq_node(job.a, sc, gid / lanes, gid % lanes, lanes);
+        praq_node(praq_view(job.a), sc, gid / lanes, gid % lanes, lanes);
     });
 }
 
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256) wpraq_edge(const PJ32* j32, const PJ64* j
         PraqStep<Real> sc;
         if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
         const int e = t.y * kWideBlock + threadIdx.x;
-        if (e < job.a.m) praq_edge(job.a, sc, e);
+        if (e < job.a.m) praq_edge(praq_view(job.a), sc, e);
     });
 }
 
@@ -263,9 +263,9 @@ __global__ void __launch_bounds__(256) wpraq_sums(const PJ32* j32, const PJ64* j
         using Real = RealOf<decltype(job)>;
         const int where = wide_sums_point(job, round_action, t.x, mode);
         if (!where) return;
-        const PraqArrays<Real>& a = job.a;
-        const Real* bx = where == 2 ? a.sx : a.x;
-        const Real* by = where == 2 ? a.sy : a.y;
+        const PraqView<Real> a = praq_view(job.a);
+        const gptr<const Real> bx = where == 2 ? a.sx : a.x;
+        const gptr<const Real> by = where == 2 ? a.sy : a.y;
         double acc[kPraqSums];
 #pragma unroll
         for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(256) wpraq_copy(const PJ32* j32, const PJ64* j
     dispatch(j32, j64, t.x, [&](const auto& job) {
         const int kind = wide_copy_kind(job.state, round_action, pending, t.x, mode);
         if (kind < 0) return;
-        const auto& a = job.a;
+        const auto a = praq_view(job.a);
         const int gid = t.y * kWideBlock + threadIdx.x;
         if (node) {
             if (gid % job.lanes) return;
@@ -346,8 +346,8 @@ __global__ void __launch_bounds__(256) wpraq_convert(const PJ32* j32, const PJ64
     const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
     const LogicState* st = j64[t.x].state;
     if (st->in64 || !st->stalled) return;
-    const PraqArrays<float>& f = j32[t.x].a;
-    const PraqArrays<double>& d = j64[t.x].a;
+    const PraqView<float> f = praq_view(j32[t.x].a);
+    const PraqView<double> d = praq_view(j64[t.x].a);
     const int gid = t.y * kWideBlock + threadIdx.x;
     if (node) {
         const int lanes = j32[t.x].lanes;
@@ -382,13 +382,14 @@ __global__ void __launch_bounds__(256) wpria_x(const RJ32* j32, const RJ64* j64,
         using Real = RealOf<decltype(job)>;
         PriaStep<Real> sc;
         if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+        const PriaView<Real> a = pria_view(job.a);
         if (sizeof(Real) == 4) {
-            pria_x_tile<Real, kWideTW>(job.a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
+            pria_x_tile<Real, kWideTW>(a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
         } else {  // FP64: the tile as two halves, so its LDS stays that of FP32
             constexpr int H = kWideTW / 2;
-            pria_x_tile<Real, H>(job.a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
-            if (t.z + H < job.a.T)
-                pria_x_tile<Real, H>(job.a, sc, t.y, t.z + H, reinterpret_cast<Real*>(dyn));
+            pria_x_tile<Real, H>(a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
+            if (t.z + H < a.T)
+                pria_x_tile<Real, H>(a, sc, t.y, t.z + H, reinterpret_cast<Real*>(dyn));
         }
     });
 }
@@ -411,13 +412,13 @@ wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab, const int* round_act
         if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
         Real* cache = reinterpret_cast<Real*>(zcache);
         const int cap = kPriaZCacheBytes / int(sizeof(Real));
+        const PriaView<Real> a = pria_view(job.a);
         if (compact)
             pria_yz_slice<Real, PriaCompactShape<Real>::VR, PriaCompactShape<Real>::VZ>(
-                job.a, sc, t.y, red, 3, 0, 1 << 30, cache, cap);
+                a, sc, t.y, red, 3, 0, 1 << 30, cache, cap);
         else
-            pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(job.a, sc, t.y, red,
-                                                                          3, 0, 1 << 30, cache,
-                                                                          cap);
+            pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, sc, t.y, red, 3, 0,
+                                                                          1 << 30, cache, cap);
     });
 }
 
@@ -434,7 +435,7 @@ __global__ void __launch_bounds__(256) wpria_sums(const RJ32* j32, const RJ64* j
         using Real = RealOf<decltype(job)>;
         const int where = wide_sums_point(job, round_action, inst, mode);
         if (!where) return;
-        const PriaArrays<Real>& a = job.a;
+        const PriaView<Real> a = pria_view(job.a);
         double acc[kPriaSums];
         for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
         if (slice) {
@@ -482,7 +483,7 @@ __global__ void __launch_bounds__(256) wpria_copy(const RJ32* j32, const RJ64* j
     dispatch(j32, j64, t.x, [&](const auto& job) {
         const int kind = wide_copy_kind(job.state, round_action, pending, t.x, mode);
         if (kind < 0) return;
-        const auto& a = job.a;
+        const auto a = pria_view(job.a);
         const int s = t.y, base = pria_off(a.N, a.T, s), L = a.N * (a.T - s);
         if (kind == 3) {  // no thresholds yet: column (i, s), row (i, s), slice s
             const double none = __builtin_nan("");
@@ -518,8 +519,8 @@ __global__ void __launch_bounds__(256) wpria_convert(const RJ32* j32, const RJ64
     const int2 t = s_tab[blockIdx.x];
     const LogicState* st = j64[t.x].state;
     if (st->in64 || !st->stalled) return;
-    const PriaArrays<float>& f = j32[t.x].a;
-    const PriaArrays<double>& d = j64[t.x].a;
+    const PriaView<float> f = pria_view(j32[t.x].a);
+    const PriaView<double> d = pria_view(j64[t.x].a);
     const int s = t.y, base = pria_off(f.N, f.T, s), L = f.N * (f.T - s);
     for (int i = threadIdx.x; i < f.N; i += blockDim.x) {
         d.thx[i * f.T + s] = f.thx[i * f.T + s];
