@@ -469,12 +469,20 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
             // block-uniform bound); entries past the slice read its first
             // entry and are dropped
             const int kn = (L + BS - 1) / BS;
+            Real rv[VZ];
 #pragma unroll
             for (int k = 0; k < VZ; ++k) {
                 const int e = tid + BS * k;
                 const int g = base + (e < L ? e : 0);
-                u[k] = k < kn ? srcz[g] + a.r[g] : Real(0);
+                u[k] = Real(0);
+                rv[k] = Real(0);
+                if (k < kn) {
+                    u[k] = srcz[g];
+                    rv[k] = a.r[g];
+                }
             }
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) u[k] += rv[k];
 #pragma unroll
             for (int k = 0; k < VZ; ++k) {
                 const int e = tid + BS * k;

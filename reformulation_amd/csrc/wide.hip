@@ -374,52 +374,66 @@ __global__ void __launch_bounds__(256) wpraq_convert(const PJ32* j32, const PJ64
 using RJ32 = PriaJob<float>;
 using RJ64 = PriaJob<double>;
 
-__global__ void __launch_bounds__(256) wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab,
-                                               const int* round_action, int mode, int j) {
-    extern __shared__ char dyn[];
-    const int4 t = tab[blockIdx.x];
-    dispatch(j32, j64, t.x, [&](const auto& job) {
-        using Real = RealOf<decltype(job)>;
-        PriaStep<Real> sc;
-        if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
-        const PriaView<Real> a = pria_view(job.a);
-        if (sizeof(Real) == 4) {
-            pria_x_tile<Real, kWideTW>(a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
-        } else {  // FP64: the tile as two halves, so its LDS stays that of FP32
-            constexpr int H = kWideTW / 2;
-            pria_x_tile<Real, H>(a, sc, t.y, t.z, reinterpret_cast<Real*>(dyn));
-            if (t.z + H < a.T)
-                pria_x_tile<Real, H>(a, sc, t.y, t.z + H, reinterpret_cast<Real*>(dyn));
-        }
-    });
+// The instance's job record in precision Real, or null when the instance
+// currently iterates in the other precision (each precision has its own
+// launch of the step kernels; a block of the other precision exits at once).
+template <class Real>
+__device__ __forceinline__ const PriaJob<Real>* pria_job_in(const RJ32* j32, const RJ64* j64,
+                                                            int inst) {
+    const bool in64 = j64[inst].state->in64 != 0;
+    if constexpr (sizeof(Real) == 8) return in64 ? &j64[inst] : nullptr;
+    else return in64 ? nullptr : &j32[inst];
 }
 
-// The y/z pass, one slice per 512-thread block.  With `compact` (every
-// instance's T' <= 320) the register shape fits 80 VGPRs and three blocks
-// share a CU; the shapes give the same sums (row widths follow the row
+// The x pass, one x tile (32 columns of one item) per block: FP32 in 256
+// threads (8 rows of the tile at a time), FP64 in 512 threads (16 rows at a
+// time; 256-byte tile rows, so LDS T' x 33 x 8 bytes: two blocks per CU, the
+// same 16 waves as four FP32 blocks).  TW = 16 is the FP64 fallback for T'
+// whose 32-column tile does not fit the LDS: the tile as two halves.
+template <class Real, int TW>
+__global__ void __launch_bounds__(sizeof(Real) == 4 ? 256 : 512)
+wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab, const int* round_action, int mode,
+        int j) {
+    extern __shared__ char dyn[];
+    const int4 t = tab[blockIdx.x];
+    const PriaJob<Real>* job = pria_job_in<Real>(j32, j64, t.x);
+    if (!job) return;
+    PriaStep<Real> sc;
+    if (!wide_step_for(*job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+    const PriaView<Real> a = pria_view(job->a);
+    Real* lds = reinterpret_cast<Real*>(dyn);
+    pria_x_tile<Real, TW>(a, sc, t.y, t.z, lds);
+    if (TW < kWideTW && t.z + TW < a.T) pria_x_tile<Real, TW>(a, sc, t.y, t.z + TW, lds);
+}
+
+// Register shape of the y/z pass by T' (the compact one up to T' = 320).
+template <class Real, bool compact>
+struct PriaWideShape {
+    static constexpr int VR = compact ? PriaCompactShape<Real>::VR : PriaShape<Real>::VR;
+    static constexpr int VZ = compact ? PriaCompactShape<Real>::VZ : PriaShape<Real>::VZ;
+};
+
+// The y/z pass, one slice per 512-thread block, in the instance's precision.
+// With `compact` (every instance's T' <= 320) the register shape is the
+// compact one; the shapes give the same sums (row widths follow the row
 // length only).
-template <bool compact>
-__global__ void __launch_bounds__(512, compact ? 6 : 4)
+template <class Real, bool compact>
+__global__ void __launch_bounds__(512)
 wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab, const int* round_action, int mode,
          int j) {
     __shared__ double red[kPriaYZRed];
     extern __shared__ double zcache[];   // kPriaZCacheBytes: z + r of a long slice
     const int2 t = tab[blockIdx.x];
     if (t.x < 0) return;
-    dispatch(j32, j64, t.x, [&](const auto& job) {
-        using Real = RealOf<decltype(job)>;
-        PriaStep<Real> sc;
-        if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
-        Real* cache = reinterpret_cast<Real*>(zcache);
-        const int cap = kPriaZCacheBytes / int(sizeof(Real));
-        const PriaView<Real> a = pria_view(job.a);
-        if (compact)
-            pria_yz_slice<Real, PriaCompactShape<Real>::VR, PriaCompactShape<Real>::VZ>(
-                a, sc, t.y, red, 3, 0, 1 << 30, cache, cap);
-        else
-            pria_yz_slice<Real, PriaShape<Real>::VR, PriaShape<Real>::VZ>(a, sc, t.y, red, 3, 0,
-                                                                          1 << 30, cache, cap);
-    });
+    const PriaJob<Real>* job = pria_job_in<Real>(j32, j64, t.x);
+    if (!job) return;
+    PriaStep<Real> sc;
+    if (!wide_step_for(*job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+    const PriaView<Real> a = pria_view(job->a);
+    Real* cache = reinterpret_cast<Real*>(zcache);
+    const int cap = kPriaZCacheBytes / int(sizeof(Real));
+    using Shape = PriaWideShape<Real, compact>;
+    pria_yz_slice<Real, Shape::VR, Shape::VZ>(a, sc, t.y, red, 3, 0, 1 << 30, cache, cap);
 }
 
 // slice blocks (six sums each), then tile blocks (a column-minimum sum each,
@@ -1123,29 +1137,57 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     REFORM_HIP_CHECK(hipMemcpy(x_tab, xt.data(), sizeof(int4) * xt.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(s_tab, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice));
-    // an FP32 tile, or one half of an FP64 tile
-    lds = std::max(sizeof(float) * (size_t(max_T) * (TW + 1) + TW),
-                   sizeof(double) * (size_t(max_T) * (TW / 2 + 1) + TW / 2));
-    if (lds > 65536)
-        REFORM_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(wpria_x),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             int(lds)));
+    // LDS of an x tile: FP32 32 columns; FP64 32 columns when T' x 33 x 8
+    // bytes fit two tiles to a CU, else two halves of 16
+    lds32 = sizeof(float) * (size_t(max_T) * (TW + 1) + TW);
+    lds64 = sizeof(double) * (size_t(max_T) * (TW + 1) + TW);
+    x64_full = lds64 <= kPriaX64MaxLds;
+    if (!x64_full) lds64 = sizeof(double) * (size_t(max_T) * (TW / 2 + 1) + TW / 2);
+    const void* x32 = reinterpret_cast<const void*>(wpria_x<float, kWideTW>);
+    const void* x64 = x64_full ? reinterpret_cast<const void*>(wpria_x<double, kWideTW>)
+                               : reinterpret_cast<const void*>(wpria_x<double, kWideTW / 2>);
+    if (lds32 > 65536)
+        REFORM_HIP_CHECK(hipFuncSetAttribute(x32, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             int(lds32)));
+    if (lds64 > 65536)
+        REFORM_HIP_CHECK(hipFuncSetAttribute(x64, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             int(lds64)));
     return 0;
 }
 
 // One step of every instance the mode selects: the x kernel, then the y/z
 // kernel, each bracketed by timing marks.
 int WidePria::step(hipStream_t stream, int mode, int j) {
+    // FP32 launches exist for batches with FP32 images; every batch can hold
+    // FP64 instances (FP64 batches, and FP32 ones after the handover)
     if (mark(stream, 0, true)) return kErrorDevice;
-    hipLaunchKernelGGL(wpria_x, dim3(x_blocks), dim3(kWideBlock), lds, stream, jobs32, jobs64,
-                       x_tab, round_action, mode, j);
-    if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
-    if (max_T <= kPriaCompactT)
-        hipLaunchKernelGGL(wpria_yz<true>, dim3(s_blocks), dim3(512), kPriaZCacheBytes, stream,
-                           jobs32, jobs64, s_tab, round_action, mode, j);
+    if (has32)
+        hipLaunchKernelGGL((wpria_x<float, kWideTW>), dim3(x_blocks), dim3(256), lds32, stream,
+                           jobs32, jobs64, x_tab, round_action, mode, j);
+    if (x64_full)
+        hipLaunchKernelGGL((wpria_x<double, kWideTW>), dim3(x_blocks), dim3(512), lds64, stream,
+                           jobs32, jobs64, x_tab, round_action, mode, j);
     else
-        hipLaunchKernelGGL(wpria_yz<false>, dim3(s_blocks), dim3(512), kPriaZCacheBytes, stream,
-                           jobs32, jobs64, s_tab, round_action, mode, j);
+        hipLaunchKernelGGL((wpria_x<double, kWideTW / 2>), dim3(x_blocks), dim3(512), lds64,
+                           stream, jobs32, jobs64, x_tab, round_action, mode, j);
+    if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
+    const bool compact = max_T <= kPriaCompactT;
+    if (has32) {
+        if (compact)
+            hipLaunchKernelGGL((wpria_yz<float, true>), dim3(s_blocks), dim3(512),
+                               kPriaZCacheBytes, stream, jobs32, jobs64, s_tab, round_action, mode,
+                               j);
+        else
+            hipLaunchKernelGGL((wpria_yz<float, false>), dim3(s_blocks), dim3(512),
+                               kPriaZCacheBytes, stream, jobs32, jobs64, s_tab, round_action, mode,
+                               j);
+    }
+    if (compact)
+        hipLaunchKernelGGL((wpria_yz<double, true>), dim3(s_blocks), dim3(512), kPriaZCacheBytes,
+                           stream, jobs32, jobs64, s_tab, round_action, mode, j);
+    else
+        hipLaunchKernelGGL((wpria_yz<double, false>), dim3(s_blocks), dim3(512), kPriaZCacheBytes,
+                           stream, jobs32, jobs64, s_tab, round_action, mode, j);
     if (mark(stream, 1, false)) return kErrorDevice;
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;

@@ -53,6 +53,8 @@ constexpr int kActSkip = -1;
 // x tile width of the batched PRIA kernels, both precisions (one tile table
 // serves an instance before and after its switch).
 constexpr int kWideTW = 32;
+// LDS budget of one FP64 x tile of 32 columns: two such tiles share a CU
+constexpr size_t kPriaX64MaxLds = 80 * 1024;
 
 struct WideStats {
     int rounds = 0;
@@ -236,7 +238,8 @@ struct WidePria : WideControl {
     int4* x_tab = nullptr;              // (instance, item, t0, column-tile row) per x tile
     int2* s_tab = nullptr;              // (instance, slice) per slice
     int x_blocks = 0, s_blocks = 0;
-    size_t lds = 0;                     // dynamic LDS of the x kernel
+    size_t lds32 = 0, lds64 = 0;        // dynamic LDS of the x kernels
+    bool x64_full = true;               // FP64 x tiles of 32 columns (else two halves)
 
     std::vector<PriaJob<double>> full64;
     const PriaJob<float>* full_dev32 = nullptr;
