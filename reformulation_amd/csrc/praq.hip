@@ -193,10 +193,15 @@ __global__ void praq_hub_sums_kernel(PraqArrays<Real> a, const Real* bx, double*
         for (int k = 0; k < kPraqSums; ++k) row[k] = acc[k];
 }
 
+// Edges per thread of the device tier's edge kernel (their loads in flight
+// together).
+constexpr int kEdgesPerThread = 4;
+
 template <class Real>
 __global__ void praq_edge_kernel(PraqArrays<Real> a, PraqStep<Real> st) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < a.m) praq_edge(praq_view(a), st, e);
+    praq_edges<Real, kEdgesPerThread>(praq_view(a), st,
+                                      blockIdx.x * blockDim.x * kEdgesPerThread + threadIdx.x,
+                                      blockDim.x);
 }
 
 template <class Real>
@@ -268,8 +273,9 @@ static int device_step(const PraqArrays<Real>& a, const PraqStep<Real>& st, int 
         hipLaunchKernelGGL(praq_hub_kernel<Real>, dim3(blocks_for(a.nh)), dim3(kBlock), 0, stream,
                            a, st);
     if (a.m > 0)
-        hipLaunchKernelGGL(praq_edge_kernel<Real>, dim3(blocks_for(a.m)), dim3(kBlock), 0,
-                           stream, a, st);
+        hipLaunchKernelGGL(praq_edge_kernel<Real>,
+                           dim3(blocks_for((int64_t(a.m) + kEdgesPerThread - 1) / kEdgesPerThread)),
+                           dim3(kBlock), 0, stream, a, st);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }

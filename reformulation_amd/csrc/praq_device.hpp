@@ -196,28 +196,17 @@ __device__ __forceinline__ double praq_hub_total(const PraqView<Real>& a, int h)
     return acc;
 }
 
-// Edge e's update: projection of y1 + sigma (q - S refl), and the next yc.
+// Edge e's update from its loaded values: projection of y1 + sigma (q -
+// S refl), the next yc, and (y, sy) when storing.
 template <class Real>
-__device__ __forceinline__ void praq_edge(const PraqView<Real>& a, const PraqStep<Real>& st,
-                                          int e) {
+__device__ __forceinline__ void praq_edge_update(const PraqView<Real>& a, const PraqStep<Real>& st,
+                                                 int e, Real sw, Real y1a, Real y1b, Real ana,
+                                                 Real anb, Real ra, Real rb) {
     const int m = a.m;
-    Real y1a, y1b, ana, anb;
-    if (st.reflect) {
-        y1a = a.yc[e];
-        y1b = a.yc[m + e];
-        ana = a.y0[e];
-        anb = a.y0[m + e];
-    } else {
-        y1a = a.sy[e];
-        y1b = a.sy[m + e];
-        ana = y1a;
-        anb = y1b;
-    }
-    Real sw = a.w[e];
     Real w = sw < Real(0) ? -sw : sw;
     bool covering = sw > Real(0);
-    Real va = y1a + st.sigma * ((covering ? Real(1) : Real(0)) - a.refl[a.e0[e]]);
-    Real vb = y1b + st.sigma * (Real(0) - a.refl[a.e1[e]]);
+    Real va = y1a + st.sigma * ((covering ? Real(1) : Real(0)) - ra);
+    Real vb = y1b + st.sigma * (Real(0) - rb);
     Real y2a, y2b;
     if (covering) {
         Real mu = clampr(Real(0.5) * (va + vb), -w, Real(0));
@@ -236,6 +225,59 @@ __device__ __forceinline__ void praq_edge(const PraqView<Real>& a, const PraqSte
     }
     a.yc[e] = st.lwn * (Real(2) * y2a - y1a) + st.rwn * ana;
     a.yc[m + e] = st.lwn * (Real(2) * y2b - y1b) + st.rwn * anb;
+}
+
+// Edge e's update.
+template <class Real>
+__device__ __forceinline__ void praq_edge(const PraqView<Real>& a, const PraqStep<Real>& st,
+                                          int e) {
+    const int m = a.m;
+    const gptr<const Real> y1 = st.reflect ? a.yc : a.sy;
+    const gptr<const Real> an = st.reflect ? a.y0 : a.sy;
+    praq_edge_update(a, st, e, a.w[e], y1[e], y1[m + e], an[e], an[m + e], a.refl[a.e0[e]],
+                     a.refl[a.e1[e]]);
+}
+
+// K edges per thread, first, first + stride, ...: every load of the K edges
+// (ends, then the reflected primal at both ends) is issued before any edge
+// is updated, so a thread keeps its edges' latency chains in flight
+// together; per edge the arithmetic of praq_edge.
+template <class Real, int K>
+__device__ __forceinline__ void praq_edges(const PraqView<Real>& a, const PraqStep<Real>& st,
+                                           int first, int stride) {
+    const int m = a.m;
+    int e[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = first + k * stride;
+        e[k] = q < m ? q : 0;   // past the end: edge 0, dropped
+    }
+    const gptr<const Real> y1 = st.reflect ? a.yc : a.sy;
+    const gptr<const Real> an = st.reflect ? a.y0 : a.sy;
+    int i0[K], i1[K];
+    Real sw[K], ya[K], yb[K], aa[K], ab[K], ra[K], rb[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        i0[k] = a.e0[e[k]];
+        i1[k] = a.e1[e[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        sw[k] = a.w[e[k]];
+        ya[k] = y1[e[k]];
+        yb[k] = y1[m + e[k]];
+        aa[k] = an[e[k]];
+        ab[k] = an[m + e[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        ra[k] = a.refl[i0[k]];
+        rb[k] = a.refl[i1[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (first + k * stride < m)
+            praq_edge_update(a, st, e[k], sw[k], ya[k], yb[k], aa[k], ab[k], ra[k], rb[k]);
 }
 
 // The nine check sums, accumulated per thread.
