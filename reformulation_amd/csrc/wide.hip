@@ -374,33 +374,92 @@ __global__ void __launch_bounds__(256) wpraq_convert(const PJ32* j32, const PJ64
 using RJ32 = PriaJob<float>;
 using RJ64 = PriaJob<double>;
 
-// The instance's job record in precision Real, or null when the instance
-// currently iterates in the other precision (each precision has its own
-// launch of the step kernels; a block of the other precision exits at once).
-template <class Real>
-__device__ __forceinline__ const PriaJob<Real>* pria_job_in(const RJ32* j32, const RJ64* j64,
-                                                            int inst) {
-    const bool in64 = j64[inst].state->in64 != 0;
-    if constexpr (sizeof(Real) == 8) return in64 ? &j64[inst] : nullptr;
-    else return in64 ? nullptr : &j32[inst];
+// ---- work lists ----
+//
+// Each round the step kernels run over lists of the table entries whose
+// instances take steps, one list per (table, precision): x tiles FP32, x
+// tiles FP64, slices FP32, slices FP64 (kListX32 ...), in table order.  A
+// step launch has a block per table entry; block b takes item b of its list,
+// so the blocks with work are dispatched first and the rest exit after one
+// load: no block reads the job record of an instance of the other precision
+// or of one that takes no step.  Which block takes an item does not change
+// the item's arithmetic.
+enum { kListX32 = 0, kListX64 = 1, kListS32 = 2, kListS64 = 3, kLists = 4 };
+
+constexpr int kListBlock = 1024;
+
+// One block: the instances taking steps in this round (mode 0) or taking a
+// restart step (mode 1), in table order; per instance its x tiles
+// [first[4q+2], +first[4q+3]) and slices [first[4q], +first[4q+1]), in the
+// list of its precision.  The lengths go to lcount[0..3].
+__global__ void __launch_bounds__(kListBlock)
+wpria_lists(const RJ64* j64, int count, const int* first, const int* round_action, int mode,
+            int* lists, int x_cap, int s_cap, int* lcount) {
+    __shared__ int scan[kLists][kListBlock];
+    const int tid = threadIdx.x;
+    // a contiguous chunk of instances per thread keeps the lists in table order
+    const int chunk = (count + kListBlock - 1) / kListBlock;
+    const int q0 = min(count, tid * chunk), q1 = min(count, q0 + chunk);
+    int mine[kLists] = {0, 0, 0, 0};
+    for (int q = q0; q < q1; ++q) {
+        const LogicState* st = j64[q].state;
+        const bool runs = mode == 0 ? wide_runs_steps(round_action[q]) : wide_restarting(st);
+        if (!runs) continue;
+        const int p = st->in64 ? 1 : 0;
+        mine[kListX32 + p] += first[4 * q + 3];
+        mine[kListS32 + p] += first[4 * q + 1];
+    }
+    for (int l = 0; l < kLists; ++l) scan[l][tid] = mine[l];
+    __syncthreads();
+    // inclusive scan (Hillis-Steele) of each list's counts over the threads
+    for (int off = 1; off < kListBlock; off <<= 1) {
+        int v[kLists];
+        for (int l = 0; l < kLists; ++l) v[l] = tid >= off ? scan[l][tid - off] : 0;
+        __syncthreads();
+        for (int l = 0; l < kLists; ++l) scan[l][tid] += v[l];
+        __syncthreads();
+    }
+    int at[kLists];
+    for (int l = 0; l < kLists; ++l) at[l] = scan[l][tid] - mine[l];
+    int* out[kLists] = {lists, lists + x_cap, lists + 2 * x_cap, lists + 2 * x_cap + s_cap};
+    for (int q = q0; q < q1; ++q) {
+        const LogicState* st = j64[q].state;
+        const bool runs = mode == 0 ? wide_runs_steps(round_action[q]) : wide_restarting(st);
+        if (!runs) continue;
+        const int p = st->in64 ? 1 : 0;
+        int* lx = out[kListX32 + p];
+        int* ls = out[kListS32 + p];
+        for (int k = 0; k < first[4 * q + 3]; ++k) lx[at[kListX32 + p]++] = first[4 * q + 2] + k;
+        for (int k = 0; k < first[4 * q + 1]; ++k) ls[at[kListS32 + p]++] = first[4 * q] + k;
+    }
+    if (tid == kListBlock - 1)
+        for (int l = 0; l < kLists; ++l) lcount[l] = scan[l][tid];
 }
 
-// The x pass, one x tile (32 columns of one item) per block: FP32 in 256
-// threads (8 rows of the tile at a time), FP64 in 512 threads (16 rows at a
-// time; 256-byte tile rows, so LDS T' x 33 x 8 bytes: two blocks per CU, the
-// same 16 waves as four FP32 blocks).  TW = 16 is the FP64 fallback for T'
-// whose 32-column tile does not fit the LDS: the tile as two halves.
+template <class Real>
+__device__ __forceinline__ const PriaJob<Real>& pria_job(const RJ32* j32, const RJ64* j64,
+                                                         int inst) {
+    if constexpr (sizeof(Real) == 8) return j64[inst];
+    else return j32[inst];
+}
+
+// The x pass over the list of one precision, one x tile (32 columns of one
+// item) per item: FP32 in 256-thread blocks (8 rows of the tile at a time),
+// FP64 in 512 threads (16 rows at a time; 256-byte tile rows, so LDS
+// T' x 33 x 8 bytes: two blocks per CU).  TW = 16 is the FP64 fallback for
+// T' whose 32-column tile does not fit the LDS: the tile as two halves (the
+// same 16 lanes per column, so the same sums).
 template <class Real, int TW>
 __global__ void __launch_bounds__(sizeof(Real) == 4 ? 256 : 512)
-wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab, const int* round_action, int mode,
-        int j) {
+wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab, const int* list, const int* lcount,
+        const int* round_action, int mode, int j) {
     extern __shared__ char dyn[];
-    const int4 t = tab[blockIdx.x];
-    const PriaJob<Real>* job = pria_job_in<Real>(j32, j64, t.x);
-    if (!job) return;
+    if (int(blockIdx.x) >= *lcount) return;
+    const int4 t = tab[list[blockIdx.x]];
+    const PriaJob<Real>& job = pria_job<Real>(j32, j64, t.x);
     PriaStep<Real> sc;
-    if (!wide_step_for(*job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
-    const PriaView<Real> a = pria_view(job->a);
+    if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+    const PriaView<Real> a = pria_view(job.a);
     Real* lds = reinterpret_cast<Real*>(dyn);
     pria_x_tile<Real, TW>(a, sc, t.y, t.z, lds);
     if (TW < kWideTW && t.z + TW < a.T) pria_x_tile<Real, TW>(a, sc, t.y, t.z + TW, lds);
@@ -413,27 +472,26 @@ struct PriaWideShape {
     static constexpr int VZ = compact ? PriaCompactShape<Real>::VZ : PriaShape<Real>::VZ;
 };
 
-// The y/z pass, one slice per 512-thread block, in the instance's precision.
-// With `compact` (every instance's T' <= 320) the register shape is the
-// compact one; the shapes give the same sums (row widths follow the row
+// The y/z pass over the list of one precision, one slice per item, 512
+// threads.  With `compact` (every instance's T' <= 320) the register shape is
+// the compact one, held to 80 VGPRs so that three blocks share a CU (six
+// waves per SIMD); the shapes give the same sums (row widths follow the row
 // length only).
 template <class Real, bool compact>
-__global__ void __launch_bounds__(512)
-wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab, const int* round_action, int mode,
-         int j) {
+__global__ void __launch_bounds__(512, compact ? 6 : 3)
+wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab, const int* list, const int* lcount,
+         const int* round_action, int mode, int j) {
     __shared__ double red[kPriaYZRed];
     extern __shared__ double zcache[];   // kPriaZCacheBytes: z + r of a long slice
-    const int2 t = tab[blockIdx.x];
-    if (t.x < 0) return;
-    const PriaJob<Real>* job = pria_job_in<Real>(j32, j64, t.x);
-    if (!job) return;
+    if (int(blockIdx.x) >= *lcount) return;
+    const int2 t = tab[list[blockIdx.x]];
+    const PriaJob<Real>& job = pria_job<Real>(j32, j64, t.x);
     PriaStep<Real> sc;
-    if (!wide_step_for(*job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
-    const PriaView<Real> a = pria_view(job->a);
-    Real* cache = reinterpret_cast<Real*>(zcache);
-    const int cap = kPriaZCacheBytes / int(sizeof(Real));
+    if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
     using Shape = PriaWideShape<Real, compact>;
-    pria_yz_slice<Real, Shape::VR, Shape::VZ>(a, sc, t.y, red, 3, 0, 1 << 30, cache, cap);
+    pria_yz_slice<Real, Shape::VR, Shape::VZ>(pria_view(job.a), sc, t.y, red, 3, 0, 1 << 30,
+                                             reinterpret_cast<Real*>(zcache),
+                                             kPriaZCacheBytes / int(sizeof(Real)));
 }
 
 // slice blocks (six sums each), then tile blocks (a column-minimum sum each,
@@ -1129,11 +1187,18 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     }
     x_blocks = int(xt.size());
     s_blocks = int(st.size());
+    // arena: x table, slice table, the four work lists and their lengths
     const size_t x_bytes = align256(sizeof(int4) * xt.size());
-    const size_t tab_bytes = x_bytes + sizeof(int2) * st.size();
+    const size_t s_bytes = align256(sizeof(int2) * st.size());
+    const size_t l_bytes = align256(sizeof(int) * (2 * size_t(x_blocks) + 2 * size_t(s_blocks)));
+    const size_t c_bytes = sizeof(int) * kLists;
+    const size_t tab_bytes = x_bytes + s_bytes + l_bytes + c_bytes;
     if (allocate(tab_bytes, xt.size() + st.size(), kPriaSums)) return kErrorDevice;
-    x_tab = static_cast<int4*>(arena);
-    s_tab = reinterpret_cast<int2*>(static_cast<char*>(arena) + x_bytes);
+    char* ab = static_cast<char*>(arena);
+    x_tab = reinterpret_cast<int4*>(ab);
+    s_tab = reinterpret_cast<int2*>(ab + x_bytes);
+    lists = reinterpret_cast<int*>(ab + x_bytes + s_bytes);
+    lcount = reinterpret_cast<int*>(ab + x_bytes + s_bytes + l_bytes);
     REFORM_HIP_CHECK(hipMemcpy(x_tab, xt.data(), sizeof(int4) * xt.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(s_tab, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice));
@@ -1155,39 +1220,49 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     return 0;
 }
 
-// One step of every instance the mode selects: the x kernel, then the y/z
-// kernel, each bracketed by timing marks.
+int WidePria::build_lists(hipStream_t stream, int mode) {
+    hipLaunchKernelGGL(wpria_lists, dim3(1), dim3(kListBlock), 0, stream, jobs64, count, first,
+                       round_action, mode, lists, x_blocks, s_blocks, lcount);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// One step of every instance the mode selects: the x kernels, then the y/z
+// kernels, over the round's work lists (built by build_lists for the mode),
+// each pair bracketed by timing marks.
 int WidePria::step(hipStream_t stream, int mode, int j) {
+    const int* lx32 = lists;
+    const int* lx64 = lists + x_blocks;
+    const int* ls32 = lists + 2 * x_blocks;
+    const int* ls64 = lists + 2 * x_blocks + s_blocks;
     // FP32 launches exist for batches with FP32 images; every batch can hold
     // FP64 instances (FP64 batches, and FP32 ones after the handover)
     if (mark(stream, 0, true)) return kErrorDevice;
     if (has32)
         hipLaunchKernelGGL((wpria_x<float, kWideTW>), dim3(x_blocks), dim3(256), lds32, stream,
-                           jobs32, jobs64, x_tab, round_action, mode, j);
+                           jobs32, jobs64, x_tab, lx32, lcount + kListX32, round_action, mode, j);
     if (x64_full)
         hipLaunchKernelGGL((wpria_x<double, kWideTW>), dim3(x_blocks), dim3(512), lds64, stream,
-                           jobs32, jobs64, x_tab, round_action, mode, j);
+                           jobs32, jobs64, x_tab, lx64, lcount + kListX64, round_action, mode, j);
     else
         hipLaunchKernelGGL((wpria_x<double, kWideTW / 2>), dim3(x_blocks), dim3(512), lds64,
-                           stream, jobs32, jobs64, x_tab, round_action, mode, j);
+                           stream, jobs32, jobs64, x_tab, lx64, lcount + kListX64, round_action,
+                           mode, j);
     if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
     const bool compact = max_T <= kPriaCompactT;
-    if (has32) {
+    auto yz = [&](auto real, const int* list, int l) {
+        using Real = decltype(real);
         if (compact)
-            hipLaunchKernelGGL((wpria_yz<float, true>), dim3(s_blocks), dim3(512),
-                               kPriaZCacheBytes, stream, jobs32, jobs64, s_tab, round_action, mode,
-                               j);
+            hipLaunchKernelGGL((wpria_yz<Real, true>), dim3(s_blocks), dim3(512),
+                               kPriaZCacheBytes, stream, jobs32, jobs64, s_tab, list, lcount + l,
+                               round_action, mode, j);
         else
-            hipLaunchKernelGGL((wpria_yz<float, false>), dim3(s_blocks), dim3(512),
-                               kPriaZCacheBytes, stream, jobs32, jobs64, s_tab, round_action, mode,
-                               j);
-    }
-    if (compact)
-        hipLaunchKernelGGL((wpria_yz<double, true>), dim3(s_blocks), dim3(512), kPriaZCacheBytes,
-                           stream, jobs32, jobs64, s_tab, round_action, mode, j);
-    else
-        hipLaunchKernelGGL((wpria_yz<double, false>), dim3(s_blocks), dim3(512), kPriaZCacheBytes,
-                           stream, jobs32, jobs64, s_tab, round_action, mode, j);
+            hipLaunchKernelGGL((wpria_yz<Real, false>), dim3(s_blocks), dim3(512),
+                               kPriaZCacheBytes, stream, jobs32, jobs64, s_tab, list, lcount + l,
+                               round_action, mode, j);
+    };
+    if (has32) yz(float(0), ls32, kListS32);
+    yz(double(0), ls64, kListS64);
     if (mark(stream, 1, false)) return kErrorDevice;
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
@@ -1207,6 +1282,7 @@ int WidePria::enqueue_round(hipStream_t stream) {
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_pre<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
                        count, round_action, pending);
+    if (build_lists(stream, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
                        round_action, pending, 0);
     for (int j = 0; j < op_steps; ++j)
@@ -1220,7 +1296,7 @@ int WidePria::enqueue_round(hipStream_t stream) {
         hipLaunchKernelGGL(wide_promote<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream,
                            jobs64, count);
     }
-    if (step(stream, 1, 0)) return kErrorDevice;
+    if (build_lists(stream, 1) || step(stream, 1, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
                        round_action, pending, 1);
     hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,

@@ -240,6 +240,10 @@ struct WidePria : WideControl {
     int x_blocks = 0, s_blocks = 0;
     size_t lds32 = 0, lds64 = 0;        // dynamic LDS of the x kernels
     bool x64_full = true;               // FP64 x tiles of 32 columns (else two halves)
+    // work lists of the round (x tiles FP32, FP64, slices FP32, FP64) and
+    // their lengths
+    int* lists = nullptr;
+    int* lcount = nullptr;
 
     std::vector<PriaJob<double>> full64;
     const PriaJob<float>* full_dev32 = nullptr;
@@ -249,6 +253,8 @@ struct WidePria : WideControl {
     int setup(std::vector<PriaJob<double>> host64, const PriaJob<float>* dev32,
               const PriaJob<double>* dev64);
     int step(hipStream_t stream, int mode, int j);
+    // the work lists of mode 0 (steps of the round) or 1 (restart steps)
+    int build_lists(hipStream_t stream, int mode);
     int start(hipStream_t stream) override;
     int enqueue_round(hipStream_t stream) override;
     int expire(hipStream_t stream, const int* mask = nullptr) override;
