@@ -128,6 +128,29 @@ int pria_solve_device(int64_t n, int64_t t, const double* order_costs, double jo
                       const pria_options* options, int use_float32, double* x, double* y,
                       double* z, pria_info* info, void* stream);
 
+/* Bytes of the device array pria_solve_device_noalloc works in, per GPU. */
+#define PRIA_NOALLOC_BYTES (64ll << 20)
+
+/* pria_solve_device that allocates nothing (no counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments, results and codes as pria_solve_device, and the same kernels
+ * give the same bits.  Every scratch byte of the solve (scale partials,
+ * objective rows, x when the caller takes none, the iterate images of
+ * both precisions) lives in one static device array of PRIA_NOALLOC_BYTES
+ * per GPU, held from the library's load until the process ends.  An
+ * instance that needs more (about 140 bytes per entry N T'(T'+1)/2 with
+ * FP64 iterates, 210 with FP32 ones and their FP64 handover) is rejected
+ * with -1 and an explanation in reform_last_error().  Calls on one GPU
+ * share the array: within a process they take turns (a call waits for the
+ * one before it to return); the array keeps the last call's contents, which
+ * no call reads. */
+int pria_solve_device_noalloc(int64_t n, int64_t t, const double* order_costs,
+                              double joint_cost, const double* holding_costs,
+                              const pria_params* params, const pria_options* options,
+                              int use_float32, double* x, double* y, double* z,
+                              pria_info* info, void* stream);
+
 /* Test hook: one PDHG step x2 = P_X(x1 - tau (h + y1 + z1)), r = sigma
  * (2 x2 - x1), y2 = P_Y(y1 + r), z2 = P_Z(z1 + r) on an even-T instance whose
  * h, K, K0 are already scaled; dense [n][t][t] in and out.  tier 1: one

@@ -78,7 +78,7 @@ EXPORTS = (
     "praq_solve_ex", "praq_solve_device",
     "pria_params_auto", "pria_solve_f64", "pria_solve_f32", "pria_solve_batch_f64",
     "pria_solve_batch_f32", "pria_objective_f64", "pria_default_options", "pria_solve_ex",
-    "pria_solve_device",
+    "pria_solve_device", "pria_solve_device_noalloc",
     "reform_batch_create", "reform_batch_destroy", "reform_batch_add_praq",
     "reform_batch_add_pria", "reform_batch_upload", "reform_batch_solve",
     "reform_batch_set_shape", "reform_batch_count", "reform_batch_kernel_ms",
@@ -169,11 +169,13 @@ def lib() -> ctypes.CDLL:
         L.pria_solve_ex.argtypes = [_i64, _i64, _dp, _dbl, _dp, ctypes.POINTER(PriaParams),
                                     ctypes.POINTER(PriaOptions), _int, _dp, _dp, _dp, _dp, _dp,
                                     _dp, ctypes.POINTER(PriaInfo)]
-        L.pria_solve_device.restype = _int
-        L.pria_solve_device.argtypes = [_i64, _i64, ctypes.c_void_p, _dbl, ctypes.c_void_p,
-                                        ctypes.POINTER(PriaParams), ctypes.POINTER(PriaOptions),
-                                        _int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                        ctypes.POINTER(PriaInfo), ctypes.c_void_p]
+        for name in ("pria_solve_device", "pria_solve_device_noalloc"):
+            fn = getattr(L, name)
+            fn.restype = _int
+            fn.argtypes = [_i64, _i64, ctypes.c_void_p, _dbl, ctypes.c_void_p,
+                           ctypes.POINTER(PriaParams), ctypes.POINTER(PriaOptions), _int,
+                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.POINTER(PriaInfo), ctypes.c_void_p]
     _lib = L
     from . import autotune        # the measured shape table, when one is present
     autotune.load(L=L)

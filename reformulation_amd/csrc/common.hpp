@@ -192,6 +192,12 @@ inline void record_error(const char* call, hipError_t err, const char* file, int
     last_error_text() = std::string(call) + " at " + file + ":" + std::to_string(line) + ": " +
                         hipGetErrorString(err);
 }
+// A failure that is no HIP call's (an argument an entry point rejects with
+// an explanation), readable through reform_last_error().
+inline void record_message(const std::string& text) {
+    std::lock_guard<std::mutex> lock(last_error_mutex());
+    last_error_text() = text;
+}
 
 }  // namespace reform
 

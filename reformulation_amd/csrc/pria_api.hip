@@ -10,6 +10,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
+#include <string>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -45,20 +47,27 @@ struct Buffer {
 
 // Device memory in a stream's order (hipMallocAsync / hipFreeAsync): a solve
 // neither synchronizes the device nor waits for other threads' solves when it
-// allocates or frees, so concurrent solves on their own streams compose.
+// allocates or frees, so concurrent solves on their own streams compose.  A
+// borrowed buffer is memory the caller provides; it is not freed.
 struct StreamBuffer {
     void* ptr = nullptr;
     hipStream_t stream = nullptr;
+    bool owned = false;
     int alloc(size_t bytes, hipStream_t s) {
         stream = s;
         if (hipMallocAsync(&ptr, bytes ? bytes : 1, s) != hipSuccess) {
             ptr = nullptr;
             return kPriaErrorDevice;
         }
+        owned = true;
         return 0;
     }
+    void borrow(void* p) {
+        ptr = p;
+        owned = false;
+    }
     ~StreamBuffer() {
-        if (ptr) (void)hipFreeAsync(ptr, stream);
+        if (ptr && owned) (void)hipFreeAsync(ptr, stream);
     }
 };
 
@@ -203,16 +212,30 @@ struct PriaImage {
     PriaJob<Real> job;
     size_t job_off = 0, state_off = 0, work_off = 0, bytes = 0;
 
-    // `din` set: K and h gathered on the device from the caller's arrays
-    // (cold start only); else the host image of `s` uploaded
-    int init(const PriaHostInstance& s, const LogicParams& lp, hipStream_t stream,
-             const PriaDeviceInput* din = nullptr) {
+    void place(const PriaHostInstance& s) {
         L = pria_layout(s.N, s.T, s.ND, sizeof(Real));
         job_off = pria_align256(L.total_bytes);
         state_off = pria_align256(job_off + sizeof(PriaJob<Real>));
         work_off = pria_align256(state_off + sizeof(LogicState));
         bytes = work_off + pria_device_work_bytes(s.N, s.T);
-        if (buf.alloc(bytes, stream)) return kPriaErrorDevice;
+    }
+
+    // device bytes of the image of an instance of s's shape
+    static size_t size_for(const PriaHostInstance& s) {
+        PriaImage probe;
+        probe.place(s);
+        return probe.bytes;
+    }
+
+    // `din` set: K and h gathered on the device from the caller's arrays
+    // (cold start only); else the host image of `s` uploaded.  `at` set: the
+    // image lives there (size_for(s) bytes, 256-byte aligned), else in a
+    // stream-ordered allocation.
+    int init(const PriaHostInstance& s, const LogicParams& lp, hipStream_t stream,
+             const PriaDeviceInput* din = nullptr, char* at = nullptr) {
+        place(s);
+        if (at) buf.borrow(at);
+        else if (buf.alloc(bytes, stream)) return kPriaErrorDevice;
         char* base = static_cast<char*>(buf.ptr);
         std::memset(&job, 0, sizeof(job));
         job.a = pria_bind<Real>(L, base, s);
@@ -283,34 +306,44 @@ struct PriaImage {
     }
 };
 
+// Device bytes of solve_mixed's images when they are carved from an arena:
+// the FP64 image, or the FP32 image and the FP64 one of a handover.
+size_t mixed_image_bytes(const PriaHostInstance& s, bool fp32) {
+    const size_t d = PriaImage<double>::size_for(s);
+    return fp32 ? pria_align256(PriaImage<float>::size_for(s)) + d : d;
+}
+
 // FP64 iterates, or FP32 iterates handing over to FP64 when they stall; all
 // work on `stream`.  `emit(image)` takes the answer of the image the solve
-// finished in.
+// finished in.  `arena` set: the images are carved from it (at least
+// mixed_image_bytes(s, fp32) bytes, 256-byte aligned) instead of allocated.
 template <class Emit>
 int solve_mixed(const PriaHostInstance& s, const LogicParams& lp, const pria_options& o,
                 bool fp32, const PriaDeviceInput* din, hipStream_t stream, Emit&& emit,
-                pria_info* info) {
+                pria_info* info, char* arena = nullptr) {
     const int tier = pick_tier(s, o.tier);
     LogicState st;
     std::memset(&st, 0, sizeof(st));
     size_t held = 0;
     if (!fp32) {
         PriaImage<double> d;
-        if (d.init(s, lp, stream, din) || d.run(tier, st, o.time_limit, false, s.T, stream) ||
-            emit(d))
+        if (d.init(s, lp, stream, din, arena) ||
+            d.run(tier, st, o.time_limit, false, s.T, stream) || emit(d))
             return kPriaErrorDevice;
         held = d.bytes;
     } else {
         LogicParams lp32 = lp;
         apply_fp32_policy(lp32);
         PriaImage<float> f;
-        if (f.init(s, lp32, stream, din) || f.run(tier, st, o.time_limit, false, s.T, stream))
+        if (f.init(s, lp32, stream, din, arena) ||
+            f.run(tier, st, o.time_limit, false, s.T, stream))
             return kPriaErrorDevice;
         held = f.bytes;
         if (st.stalled) {
             PriaImage<double> d;
             st.switch_iteration = st.iteration;
-            if (d.init(s, lp, stream, din) || pria_convert_state(f.job.a, d.job.a, stream) ||
+            char* at = arena ? arena + pria_align256(f.bytes) : nullptr;
+            if (d.init(s, lp, stream, din, at) || pria_convert_state(f.job.a, d.job.a, stream) ||
                 d.run(tier, st, o.time_limit, true, s.T, stream) || emit(d))
                 return kPriaErrorDevice;
             held += d.bytes;
@@ -484,55 +517,73 @@ extern "C" int pria_solve_ex(int64_t n, int64_t t, const double* order_costs, do
                        warm_x, warm_y, warm_z, x, y, z, info);
 }
 
-extern "C" int pria_solve_device(int64_t n, int64_t t, const double* order_costs,
-                                 double joint_cost, const double* holding_costs,
-                                 const pria_params* params, const pria_options* options,
-                                 int use_float32, double* x, double* y, double* z,
-                                 pria_info* info, void* stream_handle) {
-    if (info) {
-        std::memset(info, 0, sizeof(*info));
-        info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
-        info->status = kPriaErrorArgument;
-    }
+namespace {
+
+// The shape of a device-IO solve (scale unknown until the device has summed
+// the squares); kPriaErrorArgument for inputs the entry points reject.
+int device_io_shape(int64_t n, int64_t t, const double* order_costs, const double* holding_costs,
+                    const pria_params* params, PriaHostInstance& s) {
     if (!pria_params_valid(params) || n <= 0 || t <= 0 || !order_costs || !holding_costs)
         return kPriaErrorArgument;
     const int64_t T = t + (t & 1);
     const int64_t nd = n * T * (T + 1) / 2;
     if (nd >= (int64_t(1) << 31) || T > 640) return kPriaErrorArgument;
-    hipStream_t stream = static_cast<hipStream_t>(stream_handle);
-    const pria_options o = options ? *options : pria_default_options();
-    const int64_t rows = n * t;
-    // scratch: scale partials, the objective's row maxima and dots, and x
-    // when the caller takes none
-    const size_t f_partial = 0, f_total = kDevScaleBlocks, f_max = f_total + 1,
-                 f_dot = f_max + size_t(rows), f_x = f_dot + size_t(rows);
-    const size_t doubles = f_x + (x ? 0 : size_t(rows) * t);
-    StreamBuffer scratch;
-    if (scratch.alloc(sizeof(double) * doubles, stream)) return kPriaErrorDevice;
-    double* w = static_cast<double*>(scratch.ptr);
-    hipLaunchKernelGGL(pria_sumsq_rows, dim3(kDevScaleBlocks), dim3(kDevBlock), 0, stream,
-                       holding_costs, rows, int(t), w + f_partial);
-    hipLaunchKernelGGL(pria_sumsq_total, dim3(1), dim3(kDevBlock), 0, stream, order_costs,
-                       int(n), joint_cost, w + f_partial, kDevScaleBlocks, w + f_total);
-    double sq = 0.0;
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(&sq, w + f_total, sizeof(double), hipMemcpyDeviceToHost, stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess)
-        return kPriaErrorDevice;
-    PriaHostInstance s;  // shape and scale only: the arrays stay on the device
     s.N = int(n);
     s.T = int(T);
     s.Torig = int(t);
     s.ND = int(nd);
+    s.cold = true;
+    return 0;
+}
+
+// Scratch of a device-IO solve besides the images, in doubles: the scale
+// partials and total, the objective's row maxima and dots, and x when the
+// caller takes none.
+struct DeviceIoScratch {
+    size_t f_partial = 0, f_total = 0, f_max = 0, f_dot = 0, f_x = 0, doubles = 0;
+    DeviceIoScratch(int64_t rows, int64_t t, bool x_given) {
+        f_total = kDevScaleBlocks;
+        f_max = f_total + 1;
+        f_dot = f_max + size_t(rows);
+        f_x = f_dot + size_t(rows);
+        doubles = f_x + (x_given ? 0 : size_t(rows) * size_t(t));
+    }
+    size_t bytes() const { return pria_align256(sizeof(double) * doubles); }
+};
+
+// Shared body of pria_solve_device and pria_solve_device_noalloc.  `arena`
+// null: scratch and images are stream-ordered allocations; else everything
+// is carved from `arena` (device memory of at least DeviceIoScratch::bytes()
+// + mixed_image_bytes() bytes, 256-byte aligned), and nothing is allocated.
+int solve_device_io(PriaHostInstance& s, const double* order_costs, double joint_cost,
+                    const double* holding_costs, const pria_params* params,
+                    const pria_options* options, int use_float32, double* x, double* y,
+                    double* z, pria_info* info, hipStream_t stream, char* arena) {
+    const pria_options o = options ? *options : pria_default_options();
+    const int64_t n = s.N, t = s.Torig, rows = n * t;
+    const DeviceIoScratch sc(rows, t, x != nullptr);
+    StreamBuffer scratch_buf;
+    if (arena) scratch_buf.borrow(arena);
+    else if (scratch_buf.alloc(sizeof(double) * sc.doubles, stream)) return kPriaErrorDevice;
+    double* w = static_cast<double*>(scratch_buf.ptr);
+    hipLaunchKernelGGL(pria_sumsq_rows, dim3(kDevScaleBlocks), dim3(kDevBlock), 0, stream,
+                       holding_costs, rows, int(t), w + sc.f_partial);
+    hipLaunchKernelGGL(pria_sumsq_total, dim3(1), dim3(kDevBlock), 0, stream, order_costs,
+                       int(n), joint_cost, w + sc.f_partial, kDevScaleBlocks, w + sc.f_total);
+    double sq = 0.0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&sq, w + sc.f_total, sizeof(double), hipMemcpyDeviceToHost, stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return kPriaErrorDevice;
+    // shape and scale only: the arrays stay on the device
     s.scale = sq > 0.0 ? std::sqrt(sq) : 1.0;
     s.k0 = joint_cost / s.scale;
-    s.cold = true;
     const PriaDeviceInput din{order_costs, holding_costs, int(t)};
     const LogicParams lp = pria_logic_params(*params);
     pria_info local;
     pria_info* out = info ? info : &local;
-    double* xd = x ? x : w + f_x;
+    double* xd = x ? x : w + sc.f_x;
     const dim3 grid{static_cast<unsigned>(rows)}, blk{unsigned(kDevBlock)};
     auto emit = [&](auto& image) -> int {
         using Real = std::remove_reference_t<decltype(*image.job.a.x)>;
@@ -546,23 +597,122 @@ extern "C" int pria_solve_device(int64_t n, int64_t t, const double* order_costs
                                order_costs, s.k0, 2, s.N, int(t), s.T, s.scale, z);
         return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
     };
-    int status = solve_mixed(s, lp, o, use_float32 != 0, &din, stream, emit, out);
+    int status = solve_mixed(s, lp, o, use_float32 != 0, &din, stream, emit, out,
+                             arena ? arena + sc.bytes() : nullptr);
     if (status) {
         out->status = status;
         return status;
     }
-    out->device_bytes += int64_t(sizeof(double) * doubles);
+    out->device_bytes += int64_t(sizeof(double) * sc.doubles);
     // the upper bound of the returned point on the caller's instance
     hipLaunchKernelGGL(pria_objective_rows, grid, blk, 0, stream, holding_costs, xd, int(t),
-                       w + f_max, w + f_dot);
+                       w + sc.f_max, w + sc.f_dot);
     hipLaunchKernelGGL(pria_objective_total, dim3(1), blk, 0, stream, order_costs, joint_cost,
-                       int(n), int(t), w + f_max, w + f_dot, w + f_total);
+                       int(n), int(t), w + sc.f_max, w + sc.f_dot, w + sc.f_total);
     if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(&out->upper, w + f_total, sizeof(double), hipMemcpyDeviceToHost,
+        hipMemcpyAsync(&out->upper, w + sc.f_total, sizeof(double), hipMemcpyDeviceToHost,
                        stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
         return kPriaErrorDevice;
     return out->status == 0 ? 0 : 1;
+}
+
+void clear_info(pria_info* info) {
+    if (!info) return;
+    std::memset(info, 0, sizeof(*info));
+    info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
+    info->status = kPriaErrorArgument;
+}
+
+// ---- pria_solve_device_noalloc's memory ----
+//
+// One static array per GPU serves every call on that GPU: scratch, the
+// FP32 image and the FP64 image of a handover.  Host code reaches the array
+// only through the address hipGetSymbolAddress gives for the current device
+// (the array's name in host code is a stand-in, not a device address).
+constexpr size_t kPriaNoAllocBytes = size_t(PRIA_NOALLOC_BYTES);
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) char scratch[kPriaNoAllocBytes];
+
+struct NoAllocSlot {
+    std::mutex mu;          // held for the whole of a call: calls on one GPU take turns
+    char* ptr = nullptr;    // the array's device address on this GPU
+};
+
+NoAllocSlot no_alloc_slots[kNoAllocDevices];
+
+// Bytes of the array a pria_solve_device_noalloc call uses (-1: shape rejected).
+int64_t noalloc_bytes(int64_t n, int64_t t, int use_float32, int with_x) {
+    if (n <= 0 || t <= 0) return -1;
+    const int64_t T = t + (t & 1);
+    const int64_t nd = n * T * (T + 1) / 2;
+    if (nd >= (int64_t(1) << 31) || T > 640) return -1;
+    PriaHostInstance s;
+    s.N = int(n);
+    s.T = int(T);
+    s.Torig = int(t);
+    s.ND = int(nd);
+    const DeviceIoScratch sc(n * t, t, with_x != 0);
+    return int64_t(sc.bytes() + mixed_image_bytes(s, use_float32 != 0));
+}
+
+}  // namespace
+
+extern "C" int pria_solve_device(int64_t n, int64_t t, const double* order_costs,
+                                 double joint_cost, const double* holding_costs,
+                                 const pria_params* params, const pria_options* options,
+                                 int use_float32, double* x, double* y, double* z,
+                                 pria_info* info, void* stream_handle) {
+    clear_info(info);
+    PriaHostInstance s;
+    if (device_io_shape(n, t, order_costs, holding_costs, params, s)) return kPriaErrorArgument;
+    return solve_device_io(s, order_costs, joint_cost, holding_costs, params, options,
+                           use_float32, x, y, z, info, static_cast<hipStream_t>(stream_handle),
+                           nullptr);
+}
+
+extern "C" int pria_solve_device_noalloc(int64_t n, int64_t t, const double* order_costs,
+                                         double joint_cost, const double* holding_costs,
+                                         const pria_params* params, const pria_options* options,
+                                         int use_float32, double* x, double* y, double* z,
+                                         pria_info* info, void* stream_handle) {
+    clear_info(info);
+    PriaHostInstance s;
+    if (device_io_shape(n, t, order_costs, holding_costs, params, s)) return kPriaErrorArgument;
+    const int64_t need = noalloc_bytes(n, t, use_float32, x != nullptr);
+    if (need > int64_t(kPriaNoAllocBytes)) {
+        record_message("pria_solve_device_noalloc: an instance of n = " + std::to_string(n) +
+                       ", t = " + std::to_string(t) + " needs " + std::to_string(need) +
+                       " bytes of scratch, more than the " + std::to_string(kPriaNoAllocBytes) +
+                       " the entry point holds per GPU");
+        return kPriaErrorArgument;
+    }
+    int dev = 0;
+    const hipError_t de = hipGetDevice(&dev);
+    if (de != hipSuccess) {
+        record_error("hipGetDevice", de, __FILE__, __LINE__);
+        return kPriaErrorDevice;
+    }
+    if (dev < 0 || dev >= kNoAllocDevices) {
+        record_message("pria_solve_device_noalloc: device ordinal " + std::to_string(dev) +
+                       " beyond the " + std::to_string(kNoAllocDevices) + " it serves");
+        return kPriaErrorArgument;
+    }
+    NoAllocSlot& slot = no_alloc_slots[dev];
+    std::lock_guard<std::mutex> lock(slot.mu);
+    if (!slot.ptr) {  // first call on this GPU, with it current
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            record_error("hipGetSymbolAddress(scratch)", e, __FILE__, __LINE__);
+            return kPriaErrorDevice;
+        }
+        slot.ptr = static_cast<char*>(p);
+    }
+    return solve_device_io(s, order_costs, joint_cost, holding_costs, params, options,
+                           use_float32, x, y, z, info, static_cast<hipStream_t>(stream_handle),
+                           slot.ptr);
 }
 
 extern "C" int pria_solve_f64(int64_t n, int64_t t, const double* order_costs, double joint_cost,

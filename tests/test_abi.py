@@ -127,6 +127,34 @@ def test_device_pointer_entries_validate_before_any_device_work():
                                  None, None, None) == -1
 
 
+def test_no_alloc_entry_validates_and_sizes_before_any_device_work():
+    """pria_solve_device_noalloc rejects what pria_solve_device rejects, and an
+    instance its static array cannot hold, with an explanation, before any
+    HIP call (the pointers here are not device memory)."""
+    import ctypes
+    from reformulation_amd import _native
+    lib = _native.lib()
+    q = lib.pria_params_auto(100, 1e-6)
+    info = _native.PriaInfo()
+    call = lambda n, t, q, f32=0: lib.pria_solve_device_noalloc(
+        n, t, 4096, 1.0, 4096, ctypes.byref(q), None, f32, 4096, None, None,
+        ctypes.byref(info), None)
+    assert call(0, 10, q) == -1
+    assert call(2, 641, q) == -1                           # T' above 640
+    # 100 x 300: 4.5e6 entries, about 140 bytes each with FP64 iterates
+    assert call(100, 300, q) == -1 and info.status == -1
+    err = _native.last_error()
+    assert "needs" in err and str(64 << 20) in err
+    # 30 x 200 (603000 entries) with FP32 iterates and room for their FP64
+    # handover image (about 210 bytes an entry): some 120 MiB
+    assert call(30, 200, q, 1) == -1 and "needs" in _native.last_error()
+    bad = lib.pria_params_auto(100, 1e-6)
+    bad.omega = 0.0
+    assert call(2, 10, bad) == -1
+    assert lib.pria_solve_device_noalloc(2, 10, None, 1.0, 4096, ctypes.byref(q), None, 0, None,
+                                         None, None, None, None) == -1
+
+
 def test_batch_validates_on_the_host_before_any_device_work():
     import numpy as np
     from reformulation_amd import praq, pria
