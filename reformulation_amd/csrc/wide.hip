@@ -389,12 +389,14 @@ enum { kListX32 = 0, kListX64 = 1, kListS32 = 2, kListS64 = 3, kLists = 4 };
 constexpr int kListBlock = 1024;
 
 // One block: the instances taking steps in this round (mode 0) or taking a
-// restart step (mode 1), in table order; per instance its x tiles
-// [first[4q+2], +first[4q+3]) and slices [first[4q], +first[4q+1]), in the
-// list of its precision.  The lengths go to lcount[0..3].
+// restart step (mode 1), in table order; per instance its x-table entries
+// [xr[2q], +xr[2q+1]) and slices [first[4q], +first[4q+1]), in the list of
+// its precision.  The lengths go to lcount[0..3].  Every instance's x range
+// is a whole number of groups of kXcds entries, so each range starts in
+// its list at a multiple of kXcds.
 __global__ void __launch_bounds__(kListBlock)
-wpria_lists(const RJ64* j64, int count, const int* first, const int* round_action, int mode,
-            int* lists, int x_cap, int s_cap, int* lcount) {
+wpria_lists(const RJ64* j64, int count, const int* first, const int* xr,
+            const int* round_action, int mode, int* lists, int x_cap, int s_cap, int* lcount) {
     __shared__ int scan[kLists][kListBlock];
     const int tid = threadIdx.x;
     // a contiguous chunk of instances per thread keeps the lists in table order
@@ -406,7 +408,7 @@ wpria_lists(const RJ64* j64, int count, const int* first, const int* round_actio
         const bool runs = mode == 0 ? wide_runs_steps(round_action[q]) : wide_restarting(st);
         if (!runs) continue;
         const int p = st->in64 ? 1 : 0;
-        mine[kListX32 + p] += first[4 * q + 3];
+        mine[kListX32 + p] += xr[2 * q + 1];
         mine[kListS32 + p] += first[4 * q + 1];
     }
     for (int l = 0; l < kLists; ++l) scan[l][tid] = mine[l];
@@ -429,7 +431,7 @@ wpria_lists(const RJ64* j64, int count, const int* first, const int* round_actio
         const int p = st->in64 ? 1 : 0;
         int* lx = out[kListX32 + p];
         int* ls = out[kListS32 + p];
-        for (int k = 0; k < first[4 * q + 3]; ++k) lx[at[kListX32 + p]++] = first[4 * q + 2] + k;
+        for (int k = 0; k < xr[2 * q + 1]; ++k) lx[at[kListX32 + p]++] = xr[2 * q] + k;
         for (int k = 0; k < first[4 * q + 1]; ++k) ls[at[kListS32 + p]++] = first[4 * q] + k;
     }
     if (tid == kListBlock - 1)
@@ -456,6 +458,7 @@ wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab, const int* list, cons
     extern __shared__ char dyn[];
     if (int(blockIdx.x) >= *lcount) return;
     const int4 t = tab[list[blockIdx.x]];
+    if (t.y < 0) return;   // the padding of an item group
     const PriaJob<Real>& job = pria_job<Real>(j32, j64, t.x);
     PriaStep<Real> sc;
     if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
@@ -502,6 +505,7 @@ __global__ void __launch_bounds__(256) wpria_sums(const RJ32* j32, const RJ64* j
                                                   int mode, double* partial) {
     __shared__ double red[kPriaSums * 16];
     const bool slice = int(blockIdx.x) < s_blocks;
+    if (!slice && x_tab[blockIdx.x - s_blocks].y < 0) return;   // item-group padding
     const int inst = slice ? s_tab[blockIdx.x].x : x_tab[blockIdx.x - s_blocks].x;
     dispatch(j32, j64, inst, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
@@ -1155,7 +1159,7 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     }
     std::vector<int4> xt;
     std::vector<int2> st;
-    std::vector<int> fi(4 * size_t(count));
+    std::vector<int> fi(4 * size_t(count)), xr(2 * size_t(count));
     op_steps = 2;
     max_T = 2;
     int tile_rows = 0;
@@ -1174,10 +1178,22 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
         fi[4 * q + 2] = tile_rows;
         fi[4 * q + 3] = a.N * tiles;
         for (int s = 0; s < a.T; ++s) st.push_back(make_int2(q, s));
-        // heaviest tiles (largest t) first; the check row is (item, tile)
-        for (int k = tiles - 1; k >= 0; --k)
-            for (int i = 0; i < a.N; ++i)
-                xt.push_back(make_int4(q, i, k * TW, tile_rows + i * tiles + k));
+        // Items in groups of kXcds; a group's entries by tile, heaviest
+        // (largest t) first, then by item, so an item's tiles are kXcds
+        // entries apart.  Launches deal workgroups to the XCDs round robin
+        // and every range starts at a multiple of kXcds in its list, so all
+        // tiles of an item run on one XCD at about the same time, and the
+        // 128-byte lines that neighbouring tiles' rows share are fetched
+        // into that XCD's L2 once.  A short last group is padded with
+        // entries of item -1, which the kernels skip.  The check row is
+        // (item, tile).
+        xr[2 * q] = int(xt.size());
+        for (int g = 0; g < a.N; g += kXcds)
+            for (int k = tiles - 1; k >= 0; --k)
+                for (int i = g; i < g + kXcds; ++i)
+                    xt.push_back(i < a.N ? make_int4(q, i, k * TW, tile_rows + i * tiles + k)
+                                         : make_int4(q, -1, 0, -1));
+        xr[2 * q + 1] = int(xt.size()) - xr[2 * q];
         tile_rows += a.N * tiles;
         for (int p = 0; p < 2; ++p) {
             const double b = p ? 8.0 : 4.0, nd = a.ND;
@@ -1187,18 +1203,21 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     }
     x_blocks = int(xt.size());
     s_blocks = int(st.size());
-    // arena: x table, slice table, the four work lists and their lengths
+    // arena: x table, slice table, the four work lists, their lengths and
+    // the instances' x ranges
     const size_t x_bytes = align256(sizeof(int4) * xt.size());
     const size_t s_bytes = align256(sizeof(int2) * st.size());
     const size_t l_bytes = align256(sizeof(int) * (2 * size_t(x_blocks) + 2 * size_t(s_blocks)));
-    const size_t c_bytes = sizeof(int) * kLists;
-    const size_t tab_bytes = x_bytes + s_bytes + l_bytes + c_bytes;
+    const size_t c_bytes = align256(sizeof(int) * kLists);
+    const size_t tab_bytes = x_bytes + s_bytes + l_bytes + c_bytes + sizeof(int) * xr.size();
     if (allocate(tab_bytes, xt.size() + st.size(), kPriaSums)) return kErrorDevice;
     char* ab = static_cast<char*>(arena);
     x_tab = reinterpret_cast<int4*>(ab);
     s_tab = reinterpret_cast<int2*>(ab + x_bytes);
     lists = reinterpret_cast<int*>(ab + x_bytes + s_bytes);
     lcount = reinterpret_cast<int*>(ab + x_bytes + s_bytes + l_bytes);
+    x_range = reinterpret_cast<int*>(ab + x_bytes + s_bytes + l_bytes + c_bytes);
+    REFORM_HIP_CHECK(hipMemcpy(x_range, xr.data(), sizeof(int) * xr.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(x_tab, xt.data(), sizeof(int4) * xt.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(s_tab, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice));
@@ -1222,7 +1241,7 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
 
 int WidePria::build_lists(hipStream_t stream, int mode) {
     hipLaunchKernelGGL(wpria_lists, dim3(1), dim3(kListBlock), 0, stream, jobs64, count, first,
-                       round_action, mode, lists, x_blocks, s_blocks, lcount);
+                       x_range, round_action, mode, lists, x_blocks, s_blocks, lcount);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -50,6 +50,9 @@ constexpr int kActInit = 5;
 constexpr int kActNew = 6;
 constexpr int kActSkip = -1;
 
+// XCDs of the device: a launch's workgroups go to them round robin.
+constexpr int kXcds = 8;
+
 // x tile width of the batched PRIA kernels, both precisions (one tile table
 // serves an instance before and after its switch).
 constexpr int kWideTW = 32;
@@ -244,6 +247,7 @@ struct WidePria : WideControl {
     // their lengths
     int* lists = nullptr;
     int* lcount = nullptr;
+    int* x_range = nullptr;             // [2 count]: (first x-table entry, entries)
 
     std::vector<PriaJob<double>> full64;
     const PriaJob<float>* full_dev32 = nullptr;
