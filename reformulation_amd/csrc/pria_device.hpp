@@ -459,10 +459,15 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
     const double target = double(a.k0);
     double th = 0.0;
     int parity = 0;   // block_sums_alt's LDS half
+    // register path: u = z1 + r of entry tid + BS k in u[k]; with `kept`
+    // (the slice fits the LDS cache) z1 of entry e also in zcache[e], and the
+    // register path writes the slice itself, reading neither z1 nor r again
+    bool kept = false;
     if (target > 0.0) {
         const double start = a.thz[s];
         bool first;
         if (L <= BS * VZ) {
+            kept = zcache != nullptr && L <= zcap;
             Real u[VZ];
             double v4[4] = {0.0, 0.0, 0.0, 0.0};
             // unconditional loads up to the last k any thread needs (a
@@ -479,6 +484,13 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
                 if (k < kn) {
                     u[k] = srcz[g];
                     rv[k] = a.r[g];
+                }
+            }
+            if (kept) {
+#pragma unroll
+                for (int k = 0; k < VZ; ++k) {
+                    const int e = tid + BS * k;
+                    if (e < L) zcache[e] = u[k];
                 }
             }
 #pragma unroll
@@ -502,6 +514,12 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
             th = ((first ? v4[2] : v4[0]) - target) / C;
             for (int round = 0; round <= L; ++round) {
                 double w2[2] = {0.0, 0.0};
+                // u as the rounds' input, so its FP64 widening is redone each
+                // round rather than held (2 VZ registers) beside u for the write
+                if (sizeof(Real) == 4) {
+#pragma unroll
+                    for (int k = 0; k < VZ; ++k) asm volatile("" : "+v"(u[k]));
+                }
 #pragma unroll
                 for (int k = 0; k < VZ; ++k) {
                     const int e = tid + BS * k;
@@ -516,6 +534,40 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
                 C = w2[1];
                 th = (w2[0] - target) / w2[1];
                 first = false;
+            }
+            if (kept) {
+                if (tid == 0) a.thz[s] = th;
+                // the write from the registers and the cache; only z0 is read
+                // (when reflecting).  d is u - thr with u = z1 + r rounded as
+                // the reloading write below rounds it: the same bits.
+                const Real thr = Real(th);
+                constexpr int U = 4;
+#pragma unroll
+                for (int k0 = 0; k0 < VZ; k0 += U) {
+                    if (tid + BS * k0 >= L) break;
+                    Real an[U];
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const int e = tid + BS * (k0 + q);
+                        an[q] = st.reflect ? a.z0[base + (e < L ? e : 0)] : Real(0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const int e = tid + BS * (k0 + q);
+                        if (e < L) {
+                            const int g = base + e;
+                            const Real v1 = zcache[e];
+                            const Real d = u[k0 + q] - thr;
+                            const Real z2 = d > Real(0) ? d : Real(0);
+                            a.zc[g] = st.lwn * (Real(2) * z2 - v1) +
+                                      st.rwn * (st.reflect ? an[q] : v1);
+                            if (st.store) {
+                                a.z[g] = v1;
+                                a.sz[g] = z2;
+                            }
+                        }
+                    }
+                }
             }
         } else {
             double v4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -554,9 +606,13 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
         }
         if (tid == 0) a.thz[s] = th;
     }
+    if (kept) {   // written by the register path
+        __syncthreads();
+        return;
+    }
+    const Real thr = Real(th);
     // write: loads of U entries issued before any is used
     constexpr int U = 4;
-    const Real thr = Real(th);
     for (int e0 = tid; e0 < L; e0 += BS * U) {
         int g[U];
 #pragma unroll
