@@ -14,9 +14,10 @@ from the same mixture with its tolerance drawn from {1e-4, 1e-6}, through
 the serving loop (HBM-budgeted admission, continuous batching, backfill),
 p50/p99 latency (completion - arrival) and solves/s, beside the same
 arrivals through a one-at-a-time queue whose service times are the plain
-solver's measured per-instance times with inputs resident (by default 200
-requests/s, below saturation, and 1000 requests/s, above it:
-`served_saturated` is the served rate there against the plain queue's); the
+solver's measured per-instance times with inputs resident (by default 50
+and 200 requests/s, below saturation -- `latency_ms` quotes the first --
+and 1000 requests/s, above it: `served_saturated` is the served rate there
+against the plain queue's); the
 roofline of the dominant step kernel --
 its launches' summed algorithmic bytes (PRAQ node n(4+7b) + 2m(4+b), edge
 m(8+7b); PRIA x 7b ND, y/z 8b ND per step, b = 4 for FP32) over their summed
@@ -152,7 +153,8 @@ def serving(members, rate: float, seconds: float, service, use_f32: bool, thread
                speedup=(served["solves_per_s"] / plain["solves_per_s"])
                if plain["solves_per_s"] else None,
                rounds=stats["rounds"], bytes_peak=stats["bytes_peak"],
-               hbm_budget=stats["bytes_budget"],
+               hbm_budget=stats["bytes_budget"], in_flight_peak=stats["in_flight_peak"],
+               handovers=stats["handovers"], handover_waits=stats["handover_waits"],
                host_s=dict(retire=round(stats["retire_seconds"], 3),
                            rebuild=round(stats["rebuild_seconds"], 3)))
     log(f"serving at {rate:g}/s for {seconds:g} s: {json.dumps(out)}")
@@ -170,7 +172,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--rates", default="200,1000",
+    ap.add_argument("--rates", default="50,200,1000",
                     help="serving arrival rates, comma-separated (the highest saturates)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-threads", type=int, default=12)

@@ -11,7 +11,13 @@
  * batch, so its result does not depend on what else was in flight.
  *
  * Requests are admitted in submission order; one whose images exceed the
- * whole budget is rejected (status -2).  A request's options.time_limit
+ * whole budget is rejected (status -2).  On an FP32 server a request takes
+ * pool memory for its FP32 image at admission and for its FP64 image only
+ * at its handover (one that converges in FP32 never does); a request at
+ * its handover waits between rounds, its state intact, while the pool has
+ * no room.  Admission keeps the pool's largest free block at least the
+ * largest FP64 image a request in flight may still ask for, so one waiting
+ * handover can always go ahead and every request finishes.  A request's options.time_limit
  * (seconds, > 0) counts from its admission: past it the request stops at its
  * current point between two rounds and finishes with status 2, as the plain
  * solvers do at their limits.
@@ -54,7 +60,7 @@ typedef struct reform_request_info {
     int64_t switch_iteration;   /* FP32 -> FP64 handover, -1 if none */
     double lower, upper, relative_gap;
     double submitted, admitted, completed;  /* seconds since the server started */
-    int64_t device_bytes;   /* pool bytes the request's images take */
+    int64_t device_bytes;   /* pool bytes the request's images took at the end */
 } reform_request_info;
 
 typedef struct reform_server_counters {
@@ -64,6 +70,9 @@ typedef struct reform_server_counters {
     int64_t bytes_in_use, bytes_peak, bytes_budget;
     double retire_seconds;  /* retire threads (summed): copying answers out, freeing */
     double rebuild_seconds; /* serving thread: rebuilding the block tables */
+    int64_t in_flight_peak; /* most requests on the device at once */
+    int64_t handovers;      /* FP64 images placed at FP32 requests' handovers */
+    int64_t handover_waits; /* rounds a request at its handover waited for room */
 } reform_server_counters;
 
 reform_server_config reform_server_default_config(void);

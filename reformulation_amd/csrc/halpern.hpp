@@ -99,6 +99,11 @@ struct LogicState {
     int32_t in64;                // iterates currently in the FP64 image
     int32_t last_action;         // the last action executed
     int64_t steps32, steps64;    // steps taken in each precision
+    // the FP64 image is not there yet (a serving request admitted with its
+    // FP32 image only): a handover waits, stalled, until the host has
+    // allocated the image and cleared this
+    int32_t hold64;
+    int32_t reserved;
 };
 
 // One line per check of the device tier's host loop, when verbose.
@@ -161,6 +166,8 @@ __host__ __device__ inline void logic_init(LogicState& st, const LogicParams& p)
     st.best_gap = INFINITY;
     st.best_check = 0;
     st.switch_iteration = -1;
+    st.hold64 = 0;
+    st.reserved = 0;
 }
 
 __host__ __device__ inline void finish(LogicState& st, const LogicParams& p, int32_t action) {

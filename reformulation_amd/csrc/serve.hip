@@ -29,6 +29,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <set>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -45,11 +46,14 @@ constexpr int kPending = -1;
 // three threads served no faster than one (the staged copies share the link)
 constexpr int kRetireThreads = 1;
 
-// flags[k] = member k of a family's tables has finished
+// flags[k] for member k of a family's tables: 1 finished, 2 waiting for its
+// FP64 image (an FP32 request at its handover), else 0
 template <class Job>
 __global__ void serve_done_flags(const Job* jobs, int count, int* flags) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) flags[k] = jobs[k].state->action == kActDoneX;
+    if (k >= count) return;
+    const LogicState* st = jobs[k].state;
+    flags[k] = st->action == kActDoneX ? 1 : (!st->in64 && st->stalled && st->hold64 ? 2 : 0);
 }
 
 // First-fit allocator over one device allocation.
@@ -69,12 +73,23 @@ public:
     ~Pool() {
         if (base_) (void)hipFree(base_);
     }
-    // offset of a block of `bytes`, or SIZE_MAX
-    size_t alloc(size_t bytes) {
+    // offset of a block of `bytes`, or SIZE_MAX: the first free block that
+    // fits and leaves a free block of at least `headroom` bytes
+    size_t alloc(size_t bytes, size_t headroom = 0) {
         bytes = align256(bytes ? bytes : 1);
+        size_t big1 = 0, big2 = 0;   // the two largest free blocks
+        for (const auto& f : free_) {
+            if (f.second > big1) {
+                big2 = big1;
+                big1 = f.second;
+            } else if (f.second > big2) {
+                big2 = f.second;
+            }
+        }
         for (auto it = free_.begin(); it != free_.end(); ++it) {
             if (it->second < bytes) continue;
             const size_t off = it->first, left = it->second - bytes;
+            if (headroom && std::max(left, it->second == big1 ? big2 : big1) < headroom) continue;
             free_.erase(it);
             if (left) free_[off + bytes] = left;
             used_ += bytes;
@@ -135,8 +150,11 @@ struct Request {
     std::vector<char> img64, img32;
     size_t up64 = 0, up32 = 0;
     int build_status = 0;
-    // device
+    // device: the first image (FP32 on a server with FP32 iterates, else
+    // FP64) from admission; on an FP32 server the FP64 image from the
+    // request's handover (bytes64 its size, off64 SIZE_MAX until then)
     size_t off = SIZE_MAX, bytes = 0;
+    size_t off64 = SIZE_MAX, bytes64 = 0;
     int slot = -1;
     PraqJob<double> pj64{};
     PraqJob<float> pj32{};
@@ -172,7 +190,11 @@ struct reform_server {
     int64_t next_ticket = 0;
     int64_t n_submitted = 0, n_admitted = 0, n_completed = 0, n_rejected = 0;
     int n_building = 0;  // requests on the builders
-    std::atomic<int64_t> n_rounds{0}, n_rebuilds{0}, n_in_flight{0};
+    std::atomic<int64_t> n_rounds{0}, n_rebuilds{0}, n_in_flight{0}, n_in_flight_peak{0};
+    std::atomic<int64_t> n_handovers{0}, n_handover_waits{0};
+    // FP64 image sizes of the FP32 requests in flight that have not handed
+    // over (guarded by mu): admission keeps room for the largest
+    std::multiset<size_t> pending64;
     std::atomic<double> retire_s{0.0}, rebuild_s{0.0};   // serving-thread host time
     std::vector<std::thread> builders;
     std::thread server;
@@ -245,6 +267,7 @@ struct reform_server {
     void retire_loop();
     int build(Request& r);
     int admit(Request& r);
+    int grant64(Request& r, hipStream_t s);
     int rebuild(int family);
     int launch(int family);
     int retire_finished(int family);
@@ -269,7 +292,8 @@ int reform_server::build(Request& r) {
             r.img32.assign(r.pl32.input_bytes, 0);
             praq_fill_image<float>(r.ph, r.pl32, r.img32.data());
             r.up32 = r.pl32.input_bytes;
-            r.bytes += align256(r.pl32.total_bytes);
+            r.bytes64 = r.bytes;
+            r.bytes = align256(r.pl32.total_bytes);
         }
     } else {
         int st = pria_build_host(r.n, r.t, r.k, r.k0, r.h, nullptr, nullptr, nullptr, r.rh);
@@ -284,7 +308,8 @@ int reform_server::build(Request& r) {
             r.up32 = pria_upload_bytes(r.rh, r.rl32);
             r.img32.assign(r.up32, 0);
             pria_fill_image<float>(r.rh, r.rl32, r.img32.data());
-            r.bytes += align256(r.rl32.total_bytes);
+            r.bytes64 = r.bytes;
+            r.bytes = align256(r.rl32.total_bytes);
         }
     }
     return 0;
@@ -306,7 +331,8 @@ void reform_server::build_loop() {
             ++n_building;
         }
         int st = build(*r);
-        if (!st && r->bytes > pool.capacity()) st = kErrorInstance;  // never fits
+        // never fits: the first image and, at a handover, the FP64 one
+        if (!st && r->bytes + r->bytes64 > pool.capacity()) st = kErrorInstance;
         std::lock_guard<std::mutex> lock(mu);
         --n_building;
         if (st) {
@@ -328,25 +354,31 @@ void reform_server::build_loop() {
 
 int reform_server::admit(Request& r) {
     const bool f32 = cfg.use_float32 != 0;
-    char* base = pool.base() + r.off;
+    char* base = pool.base() + r.off;   // the first image
     hipStream_t s = upload;
     LogicState* state = states + r.slot;
-    size_t off64 = 0, off32 = 0;
+    // On an FP32 server the FP64 record holds the shapes only (its arrays
+    // are bound by grant64 at the handover); no kernel reads an instance's
+    // FP64 arrays before its state says in64 or the conversion, which waits
+    // for hold64 to clear.
+    char* base64 = f32 ? nullptr : base;
     if (r.family == 0) {
-        off32 = align256(r.pl64.total_bytes);
-        REFORM_HIP_CHECK(hipMemcpyAsync(base + off64, r.img64.data(), r.up64, hipMemcpyHostToDevice, s));
-        if (f32)
+        if (f32) {
             REFORM_HIP_CHECK(
-                hipMemcpyAsync(base + off32, r.img32.data(), r.up32, hipMemcpyHostToDevice, s));
+                hipMemcpyAsync(base, r.img32.data(), r.up32, hipMemcpyHostToDevice, s));
+        } else {
+            REFORM_HIP_CHECK(
+                hipMemcpyAsync(base, r.img64.data(), r.up64, hipMemcpyHostToDevice, s));
+        }
         r.pj64 = PraqJob<double>{};
-        r.pj64.a = praq_bind<double>(r.pl64, base + off64, r.ph);
+        r.pj64.a = praq_bind<double>(r.pl64, base64, r.ph);
         r.pj64.p = r.lp;
         r.pj64.scale = r.ph.scale;
         r.pj64.lanes = r.ph.lanes;
         r.pj64.state = state;
         if (f32) {
             r.pj32 = PraqJob<float>{};
-            r.pj32.a = praq_bind<float>(r.pl32, base + off32, r.ph);
+            r.pj32.a = praq_bind<float>(r.pl32, base, r.ph);
             r.pj32.p = r.lp;
             apply_fp32_policy(r.pj32.p);
             r.pj32.scale = r.ph.scale;
@@ -354,24 +386,25 @@ int reform_server::admit(Request& r) {
             r.pj32.state = state;
         }
     } else {
-        off32 = align256(r.rl64.total_bytes);
-        REFORM_HIP_CHECK(hipMemcpyAsync(base + off64, r.img64.data(), r.up64, hipMemcpyHostToDevice, s));
         r.rj64 = PriaJob<double>{};
-        r.rj64.a = pria_bind<double>(r.rl64, base + off64, r.rh);
-        if (r.rh.cold && pria_cold_start<double>(r.rj64.a, s)) return kErrorDevice;
+        r.rj64.a = pria_bind<double>(r.rl64, base64, r.rh);
         r.rj64.p = r.lp;
         r.rj64.scale = r.rh.scale;
         r.rj64.state = state;
         if (f32) {
             REFORM_HIP_CHECK(
-                hipMemcpyAsync(base + off32, r.img32.data(), r.up32, hipMemcpyHostToDevice, s));
+                hipMemcpyAsync(base, r.img32.data(), r.up32, hipMemcpyHostToDevice, s));
             r.rj32 = PriaJob<float>{};
-            r.rj32.a = pria_bind<float>(r.rl32, base + off32, r.rh);
+            r.rj32.a = pria_bind<float>(r.rl32, base, r.rh);
             if (r.rh.cold && pria_cold_start<float>(r.rj32.a, s)) return kErrorDevice;
             r.rj32.p = r.lp;
             apply_fp32_policy(r.rj32.p);
             r.rj32.scale = r.rh.scale;
             r.rj32.state = state;
+        } else {
+            REFORM_HIP_CHECK(
+                hipMemcpyAsync(base, r.img64.data(), r.up64, hipMemcpyHostToDevice, s));
+            if (r.rh.cold && pria_cold_start<double>(r.rj64.a, s)) return kErrorDevice;
         }
     }
     LogicState init{};
@@ -381,8 +414,29 @@ int reform_server::admit(Request& r) {
     init.last_action = kActInit;
     init.steps32 = init.steps64 = 0;
     init.switch_iteration = -1;
+    init.hold64 = f32 ? 1 : 0;
     r.init = init;
     REFORM_HIP_CHECK(hipMemcpyAsync(state, &r.init, sizeof(init), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+// An FP32 request at its handover, its FP64 image just placed at r.off64:
+// the image uploaded (and cold-started, as admission does on an FP64 server,
+// so the conversion meets the same memory), the FP64 record bound, and the
+// state's hold64 cleared -- all on the family's stream before its next
+// round, whose conversion then takes the request to FP64.
+int reform_server::grant64(Request& r, hipStream_t s) {
+    static const int32_t kZero = 0;
+    char* base = pool.base() + r.off64;
+    REFORM_HIP_CHECK(hipMemcpyAsync(base, r.img64.data(), r.up64, hipMemcpyHostToDevice, s));
+    if (r.family == 0) {
+        r.pj64.a = praq_bind<double>(r.pl64, base, r.ph);
+    } else {
+        r.rj64.a = pria_bind<double>(r.rl64, base, r.rh);
+        if (r.rh.cold && pria_cold_start<double>(r.rj64.a, s)) return kErrorDevice;
+    }
+    REFORM_HIP_CHECK(hipMemcpyAsync(&states[r.slot].hold64, &kZero, sizeof(kZero),
+                                    hipMemcpyHostToDevice, s));
     return 0;
 }
 
@@ -469,6 +523,8 @@ static int copy_out(std::vector<Real>& out, const Real* src, size_t count, hipSt
 void reform_server::retire(Request& r, int status) {
     std::lock_guard<std::mutex> lock(mu);
     pool.release(r.off, r.bytes);
+    if (r.off64 != SIZE_MAX) pool.release(r.off64, r.bytes64);
+    else if (r.bytes64) pending64.erase(pending64.find(r.bytes64));
     bytes_in_use = pool.used();
     free_slots.push_back(r.slot);
     n_in_flight--;
@@ -482,20 +538,50 @@ void reform_server::retire(Request& r, int status) {
 // After the family's round: members whose state reads finished (the done
 // flags copied back with the round) leave the tables and go to the retire
 // thread, which copies their answers out and frees their memory and slots
-// while the family's next round (tables rebuilt without them) runs.
+// while the family's next round (tables rebuilt without them) runs.  FP32
+// members at their handover get their FP64 image when the pool has room
+// (else they wait for a later round); the tables are then rebuilt with their
+// FP64 records.
 int reform_server::retire_finished(int f) {
     auto& mem = members[f];
-    const int* done = flags_host[f];
+    const int* flag = flags_host[f];
     std::vector<Request*> keep, gone;
-    for (size_t q = 0; q < mem.size(); ++q) (done[q] ? gone : keep).push_back(mem[q]);
-    if (gone.empty()) return 0;
-    {
-        std::lock_guard<std::mutex> lock(mu);
-        retiring.insert(retiring.end(), gone.begin(), gone.end());
+    for (size_t q = 0; q < mem.size(); ++q) {
+        Request* r = mem[q];
+        if (flag[q] == 1) {
+            gone.push_back(r);
+            continue;
+        }
+        keep.push_back(r);
+        if (flag[q] != 2 || r->off64 != SIZE_MAX) continue;
+        size_t off;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            off = pool.alloc(r->bytes64);
+            if (off != SIZE_MAX) {
+                pending64.erase(pending64.find(r->bytes64));
+                bytes_in_use = pool.used();
+                bytes_peak = pool.peak();
+            }
+        }
+        if (off == SIZE_MAX) {
+            n_handover_waits++;
+            continue;
+        }
+        r->off64 = off;
+        if (grant64(*r, streams[f])) return kErrorDevice;
+        n_handovers++;
+        changed[f] = true;
     }
-    cv_retire.notify_all();
+    if (!gone.empty()) {
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            retiring.insert(retiring.end(), gone.begin(), gone.end());
+        }
+        cv_retire.notify_all();
+        changed[f] = true;
+    }
     mem.swap(keep);
-    changed[f] = true;
     return 0;
 }
 
@@ -551,6 +637,7 @@ void reform_server::retire_loop() {
                                copy_out(r.f3, r.rj32.a.z, nd, s));
             }
             if (st || hipStreamSynchronize(s) != hipSuccess) return fail(kErrorDevice);
+            r.img64 = std::vector<char>();
             if (r.family == 1) {
                 std::vector<double> x(size_t(r.n) * r.t * r.t);
                 if (r.in64) pria_unpack(r.rh, r.d1, 0, x.data());
@@ -636,8 +723,15 @@ void reform_server::serve_loop() {
             if (stop) break;
             while (!ready.empty() && !free_slots.empty()) {
                 Request* r = ready.front();
-                const size_t off = pool.alloc(r->bytes);
+                // room kept for the largest FP64 image a request in flight
+                // (this one included) may still ask for at its handover
+                const size_t headroom =
+                    r->bytes64 ? std::max(r->bytes64, pending64.empty() ? size_t(0)
+                                                                        : *pending64.rbegin())
+                               : size_t(0);
+                const size_t off = pool.alloc(r->bytes, headroom);
                 if (off == SIZE_MAX) break;  // wait for memory: the queue keeps its order
+                if (r->bytes64) pending64.insert(r->bytes64);
                 r->off = off;
                 r->slot = free_slots.back();
                 free_slots.pop_back();
@@ -659,9 +753,14 @@ void reform_server::serve_loop() {
                 for (Request* r : admitted) r->t_admit = now;
             }
             for (Request* r : admitted) {
-                r->img64 = std::vector<char>();  // the device holds the images now
+                // the device holds the first image now (an FP32 request's
+                // FP64 image waits in host memory for its handover)
+                if (!r->bytes64) r->img64 = std::vector<char>();
                 r->img32 = std::vector<char>();
-                n_in_flight++;
+                const int64_t now_in = ++n_in_flight;
+                int64_t peak = n_in_flight_peak.load();
+                while (now_in > peak && !n_in_flight_peak.compare_exchange_weak(peak, now_in)) {
+                }
                 joining[r->family].push_back(r);
             }
             progress = true;
@@ -854,7 +953,7 @@ extern "C" int reform_server_info(reform_server* server, int64_t ticket, reform_
     info->submitted = r->t_submit;
     info->admitted = r->t_admit;
     info->completed = r->t_done;
-    info->device_bytes = int64_t(r->bytes);
+    info->device_bytes = int64_t(r->bytes + (r->off64 != SIZE_MAX ? r->bytes64 : 0));
     info->switch_iteration = -1;
     if (r->status >= 0) {
         info->iterations = r->final.iteration;
@@ -922,6 +1021,9 @@ extern "C" int reform_server_stats(reform_server* server, reform_server_counters
     stats->bytes_in_use = int64_t(server->bytes_in_use);
     stats->bytes_peak = int64_t(server->bytes_peak);
     stats->bytes_budget = int64_t(server->pool.capacity());
+    stats->in_flight_peak = server->n_in_flight_peak;
+    stats->handovers = server->n_handovers;
+    stats->handover_waits = server->n_handover_waits;
     stats->retire_seconds = server->retire_s;
     stats->rebuild_seconds = server->rebuild_s;
     return server->start_status;

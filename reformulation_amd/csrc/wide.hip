@@ -154,7 +154,14 @@ __global__ void wide_init(const Job* jobs, int count, int first_in64) {
     s.in64 = first_in64;
     s.last_action = kActInit;
     s.steps32 = s.steps64 = 0;
+    s.hold64 = 0;
     *jobs[inst].state = s;
+}
+
+// An FP32 instance whose logic asked for FP64 and whose FP64 image is there:
+// its state is converted, then it continues in FP64.
+__device__ __forceinline__ bool wide_hands_over(const LogicState* st) {
+    return !st->in64 && st->stalled && !st->hold64;
 }
 
 // One thread per instance: FP32 instances whose logic asked for FP64 (their
@@ -164,7 +171,7 @@ __global__ void wide_promote(const Job* jobs, int count) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= count) return;
     LogicState* st = jobs[inst].state;
-    if (!st->in64 && st->stalled) {
+    if (wide_hands_over(st)) {
         st->in64 = 1;
         st->stalled = 0;
         st->switch_iteration = st->iteration;
@@ -344,8 +351,7 @@ __global__ void __launch_bounds__(256) wpraq_convert(const PJ32* j32, const PJ64
                                                      const int2* edge_tab) {
     const bool node = int(blockIdx.x) < node_blocks;
     const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
-    const LogicState* st = j64[t.x].state;
-    if (st->in64 || !st->stalled) return;
+    if (!wide_hands_over(j64[t.x].state)) return;
     const PraqView<float> f = praq_view(j32[t.x].a);
     const PraqView<double> d = praq_view(j64[t.x].a);
     const int gid = t.y * kWideBlock + threadIdx.x;
@@ -593,8 +599,7 @@ __global__ void __launch_bounds__(256) wpria_copy(const RJ32* j32, const RJ64* j
 __global__ void __launch_bounds__(256) wpria_convert(const RJ32* j32, const RJ64* j64,
                                                      const int2* s_tab) {
     const int2 t = s_tab[blockIdx.x];
-    const LogicState* st = j64[t.x].state;
-    if (st->in64 || !st->stalled) return;
+    if (!wide_hands_over(j64[t.x].state)) return;
     const PriaView<float> f = pria_view(j32[t.x].a);
     const PriaView<double> d = pria_view(j64[t.x].a);
     const int s = t.y, base = pria_off(f.N, f.T, s), L = f.N * (f.T - s);

@@ -49,7 +49,8 @@ class ServerCounters(ctypes.Structure):
     _fields_ = [("submitted", _i64), ("admitted", _i64), ("completed", _i64),
                 ("rejected", _i64), ("rounds", _i64), ("rebuilds", _i64), ("in_flight", _i64),
                 ("bytes_in_use", _i64), ("bytes_peak", _i64), ("bytes_budget", _i64),
-                ("retire_seconds", ctypes.c_double), ("rebuild_seconds", ctypes.c_double)]
+                ("retire_seconds", ctypes.c_double), ("rebuild_seconds", ctypes.c_double),
+                ("in_flight_peak", _i64), ("handovers", _i64), ("handover_waits", _i64)]
 
 
 def _declare(L):

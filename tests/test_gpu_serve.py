@@ -231,3 +231,79 @@ def test_borrowed_inputs_may_go_once_the_request_has_finished():
             assert info.status == 0
             assert info.upper == b.upper and info.lower == b.lower
             assert np.isfinite(srv.pria_result(t)[3].upper)
+
+
+def test_fp64_images_are_placed_at_handovers_only():
+    """On an FP32 server a request takes its FP64 image at its handover: one
+    that converges in FP32 (tolerance 1e-3) ends holding less than the FP64
+    image alone (as much, for the tiniest), one that hands over holds both
+    images (more than the FP64 image alone), the handover count equals the requests that switched, and
+    the answers are still the batched ones."""
+    from reformulation_amd.serve import Server
+    members = _members()
+    b, index = _batch(members, True)
+    b.solve("batched")
+    reference = _results(b, index)
+    b.close()
+    items = _problems(members) + _problems(members[:3] + members[4:6], tol=1e-3)
+    with Server(use_float32=False, host_threads=2) as srv:
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(120.0)
+        fp64_bytes = [srv.info(t).device_bytes for t in tickets]
+    print(f"lazy FP64 images: {len(items)} requests", flush=True)
+    with Server(use_float32=True, host_threads=2) as srv:
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(120.0)
+        stats = srv.stats()
+        infos = [srv.info(t) for t in tickets]
+        for q in range(len(reference)):
+            _check_same(_served(srv, items[q][0], tickets[q]), reference[q], q)
+    switched = [i.switch_iteration >= 0 for i in infos]
+    assert stats["handovers"] == sum(switched) > 0
+    assert not all(switched)
+    for q, info in enumerate(infos):
+        assert info.status == 0, q
+        if switched[q]:
+            assert info.device_bytes > fp64_bytes[q], q
+        else:   # equal for images so small that both round to the same blocks
+            assert info.device_bytes <= fp64_bytes[q], q
+    assert any(i.device_bytes < b for i, b, s in zip(infos, fp64_bytes, switched) if not s)
+    assert stats["bytes_in_use"] == 0 and stats["in_flight"] == 0
+
+
+def test_a_pool_with_room_for_one_fp64_image_at_a_time_finishes_every_request():
+    """A budget that holds every request's FP32 image but only one FP64 image
+    beside them: handovers wait for room between rounds, admission keeps room
+    for the largest FP64 image still to come, every request finishes with the
+    batched answer, and more requests are in flight at once than eager FP32 +
+    FP64 images would allow."""
+    from reformulation_amd.serve import Server
+    members = _members()
+    b, index = _batch(members, True)
+    b.solve("batched")
+    reference = _results(b, index)
+    b.close()
+    items = _problems(members)
+    with Server(use_float32=False, host_threads=2) as srv:
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(120.0)
+        b64 = [srv.info(t).device_bytes for t in tickets]
+    with Server(use_float32=True, host_threads=2) as srv:
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(120.0)
+        both = [srv.info(t).device_bytes for t in tickets]
+        switched = [srv.info(t).switch_iteration >= 0 for t in tickets]
+    b32 = [u - v if s else u for u, v, s in zip(both, b64, switched)]
+    budget = sum(b32) + max(b64) + (1 << 16)
+    eager = budget // max(u + v for u, v in zip(b32, b64))
+    print(f"one FP64 image at a time: budget {budget}, eager fit {eager}", flush=True)
+    with Server(use_float32=True, host_threads=2, hbm_budget=budget) as srv:
+        tickets = [_submit(srv, item) for item in items]
+        srv.wait_idle(120.0)
+        stats = srv.stats()
+        for q, ((kind, _, _), t) in enumerate(zip(items, tickets)):
+            _check_same(_served(srv, kind, t), reference[q], q)
+        assert stats["completed"] == len(items) and stats["rejected"] == 0
+        assert stats["bytes_peak"] <= stats["bytes_budget"] <= budget
+        assert stats["handovers"] == sum(switched)
+        assert stats["in_flight_peak"] > eager
