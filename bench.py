@@ -161,6 +161,27 @@ def serving(members, rate: float, seconds: float, service, use_f32: bool, thread
     return out
 
 
+def precision_line() -> int:
+    """The precision study (measure/precision.py) as one JSON line: `value`
+    is the FP32-iterate run's objective error against the FP64-iterate run
+    at tolerance 1e-6, |U32 - U64| / (1 + |U64|); each record carries both
+    runs' iterations, handover iteration, bounds and certified brackets."""
+    from measure.precision import study
+    recs = study(log=log)
+    at = next(r for r in recs if r["tolerance"] == 1e-6)
+    line = dict(
+        metric="FP32-iterate objective error vs FP64 iterates at 1e-6 (one large PRAQ)",
+        value=at["upper_error"], unit="relative", n_gpus=1, higher_is_better=False,
+        vs_baseline=None, dtype="f32", data="synthetic",
+        config=dict(workload=f"PRAQ roof dual of {at['instance']} (Chung-Lu, exponent 2.5, "
+                             f"mean degree 32, n {at['n']}, m {at['m']}), plain device tier, "
+                             "FP32 iterates (FP64 reductions, FP64 finish) against FP64 "
+                             "iterates at tolerances 1e-4 and 1e-6", parallelism="none"),
+        records=recs)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +200,9 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the serving runs (one JSON line per rate)")
+    ap.add_argument("--precision", action="store_true",
+                    help="only the precision study: FP32 against FP64 iterates on the "
+                         "Chung-Lu graph of 1e6 nodes (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps(dict(error="this build measures one GPU; --gpus must be 1 "
@@ -188,6 +212,8 @@ def main() -> int:
 
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.precision:
+        return precision_line()
     from reformulation_amd import workloads
 
     t0 = time.perf_counter()

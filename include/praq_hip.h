@@ -98,6 +98,16 @@ typedef struct {
     int64_t device_bytes;   /* device memory the solve held at its peak: its
                              * images (both precisions when FP32 handed over)
                              * and work areas                               */
+    double bracket;         /* upper - lower, instance units (NaN on error).
+                             * A solve that finished in FP64 (FP64 iterates,
+                             * or FP32 ones after switch_iteration) brackets
+                             * the LP optimum: lower <= OPT <= upper, upper
+                             * the objective at x, so x's objective error is
+                             * at most this.  One that finished in FP32
+                             * (switch_iteration -1) brackets the optimum of
+                             * the instance as its FP32 image holds it (each
+                             * cost rounded to FP32), within 2^-24 (sum |c_i|
+                             * + sum |q_ij|) of OPT.                        */
 } PraqInfo;
 
 PraqOptions praq_default_options(void);

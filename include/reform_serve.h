@@ -61,6 +61,8 @@ typedef struct reform_request_info {
     double lower, upper, relative_gap;
     double submitted, admitted, completed;  /* seconds since the server started */
     int64_t device_bytes;   /* pool bytes the request's images took at the end */
+    double bracket;         /* upper - lower: bounds the answer's objective error
+                               (NaN until finished) */
 } reform_request_info;
 
 typedef struct reform_server_counters {

@@ -22,6 +22,47 @@ import time
 import numpy as np
 
 
+def study(n: int = 1_000_000, seed: int = 0, degree: float = 32.0,
+          tolerances=(1e-4, 1e-6), max_iter: int = 200_000, time_limit: float = 600.0,
+          log=lambda text: print(text, file=sys.stderr, flush=True)) -> list[dict]:
+    """One record per tolerance (see the module text)."""
+    from reformulation_amd import praq, workloads
+    t0 = time.perf_counter()
+    inst = workloads.praq_chung_lu(seed, n, mean_degree=degree)
+    problem = inst.problem()
+    log(f"instance {inst.name}: n={inst.n} m={inst.edges} built in "
+        f"{time.perf_counter() - t0:.1f} s")
+    out = []
+    for tol in tolerances:
+        runs = {}
+        for f32 in (False, True):
+            params = praq.Parameters.with_auto_step(max_iter, tol)
+            params = dataclasses.replace(params, use_float32=f32, time_limit=time_limit)
+            t1 = time.perf_counter()
+            x, _, info = praq.solve_with_info(problem, params)
+            dt = time.perf_counter() - t1
+            runs[f32] = (x, info, dt)
+            log(f"tol {tol:g} {'fp32' if f32 else 'fp64'}: {info.iterations} iterations, "
+                f"gap {info.relative_gap:.2e}, {dt:.1f} s")
+        x64, i64, t64 = runs[False]
+        x32, i32, t32 = runs[True]
+
+        def rec(info, dt):
+            return dict(iterations=int(info.iterations), restarts=int(info.restarts),
+                        status=int(info.status), lower=info.lower, upper=info.upper,
+                        gap=info.relative_gap, bracket=info.bracket, seconds=dt,
+                        switch_iteration=int(info.switch_iteration))
+
+        scale = 1.0 + abs(i64.upper)
+        out.append(dict(
+            instance=inst.name, n=inst.n, m=inst.edges, tolerance=tol,
+            fp64=rec(i64, t64), fp32=rec(i32, t32),
+            upper_error=abs(i32.upper - i64.upper) / scale,
+            lower_error=abs(i32.lower - i64.lower) / (1.0 + abs(i64.lower)),
+            primal_max_diff=float(np.max(np.abs(x32 - x64)))))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -31,40 +72,9 @@ def main() -> int:
     ap.add_argument("--max-iter", type=int, default=200_000)
     ap.add_argument("--time-limit", type=float, default=600.0)
     args = ap.parse_args()
-
-    from reformulation_amd import praq, workloads
-    t0 = time.perf_counter()
-    inst = workloads.praq_chung_lu(args.seed, args.n, mean_degree=args.degree)
-    problem = inst.problem()
-    print(f"instance {inst.name}: n={inst.n} m={inst.edges} built in "
-          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    for tol in (float(t) for t in args.tolerances.split(",")):
-        runs = {}
-        for f32 in (False, True):
-            params = praq.Parameters.with_auto_step(args.max_iter, tol)
-            params = dataclasses.replace(params, use_float32=f32, time_limit=args.time_limit)
-            t1 = time.perf_counter()
-            x, _, info = praq.solve_with_info(problem, params)
-            dt = time.perf_counter() - t1
-            runs[f32] = (x, info, dt)
-            print(f"tol {tol:g} {'fp32' if f32 else 'fp64'}: {info.iterations} iterations, "
-                  f"gap {info.relative_gap:.2e}, {dt:.1f} s", file=sys.stderr, flush=True)
-        x64, i64, t64 = runs[False]
-        x32, i32, t32 = runs[True]
-
-        def rec(info, dt):
-            return dict(iterations=int(info.iterations), restarts=int(info.restarts),
-                        status=int(info.status), lower=info.lower, upper=info.upper,
-                        gap=info.relative_gap, seconds=dt,
-                        switch_iteration=int(info.switch_iteration))
-
-        scale = 1.0 + abs(i64.upper)
-        print(json.dumps(dict(
-            instance=inst.name, n=inst.n, m=inst.edges, tolerance=tol,
-            fp64=rec(i64, t64), fp32=rec(i32, t32),
-            upper_error=abs(i32.upper - i64.upper) / scale,
-            lower_error=abs(i32.lower - i64.lower) / (1.0 + abs(i64.lower)),
-            primal_max_diff=float(np.max(np.abs(x32 - x64))))), flush=True)
+    tols = [float(t) for t in args.tolerances.split(",")]
+    for rec in study(args.n, args.seed, args.degree, tols, args.max_iter, args.time_limit):
+        print(json.dumps(rec), flush=True)
     return 0
 
 

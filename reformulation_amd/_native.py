@@ -45,7 +45,7 @@ class PraqInfo(ctypes.Structure):
     _fields_ = [("lower", _dbl), ("upper", _dbl), ("relative_gap", _dbl),
                 ("iterations", _i64), ("restarts", _i64), ("checks", _i64),
                 ("omega", _dbl), ("scale", _dbl), ("status", _int), ("tier", _int),
-                ("switch_iteration", _i64), ("device_bytes", _i64)]
+                ("switch_iteration", _i64), ("device_bytes", _i64), ("bracket", _dbl)]
 
 
 class PriaParams(ctypes.Structure):
@@ -69,7 +69,8 @@ class PriaInfo(ctypes.Structure):
     _fields_ = [("iterations", _i64), ("restarts", _i64), ("checks", _i64),
                 ("lower", _dbl), ("upper", _dbl), ("upper_iterate", _dbl),
                 ("relative_gap", _dbl), ("omega", _dbl), ("scale", _dbl), ("status", _int),
-                ("tier", _int), ("switch_iteration", _i64), ("device_bytes", _i64)]
+                ("tier", _int), ("switch_iteration", _i64), ("device_bytes", _i64),
+                ("bracket", _dbl)]
 
 
 #: Every symbol include/*.h declares; a test checks the library exports them.

@@ -754,6 +754,7 @@ int BatchImpl::praq_result(int64_t q, double* x, double* y, PraqInfo* info) {
         info->tier = e.tier;
         info->switch_iteration = st.switch_iteration;
         info->device_bytes = int64_t(e.bytes);
+        info->bracket = info->upper - info->lower;
     }
     return 0;
 }
@@ -794,6 +795,7 @@ int BatchImpl::pria_result(int64_t q, double* x, double* y, double* z, pria_info
         info->tier = e.tier;
         info->switch_iteration = st.switch_iteration;
         info->device_bytes = int64_t(e.bytes);
+        info->bracket = info->upper - info->lower;
     }
     return 0;
 }

@@ -190,6 +190,7 @@ int solve_mixed(const PraqHostInstance& s, LogicParams lp, const PraqOptions& o,
         info->tier = tier;
         info->switch_iteration = st.switch_iteration;
         info->device_bytes = int64_t(held);
+        info->bracket = st.upper - st.lower;
     }
     return 0;
 }
@@ -296,7 +297,8 @@ int solve_entry(const PraqInstance* instance, const PraqParameters* parameters,
 void clear_info(PraqInfo* info) {
     if (!info) return;
     std::memset(info, 0, sizeof(*info));
-    info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
+    info->lower = info->upper = info->relative_gap = info->bracket =
+        std::numeric_limits<double>::quiet_NaN();
     info->status = kErrorArgument;
 }
 

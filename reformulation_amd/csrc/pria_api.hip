@@ -375,7 +375,8 @@ int solve_f64io(int64_t n, int64_t t, const double* k, double k0, const double* 
                 double* z, pria_info* info) {
     if (info) {
         std::memset(info, 0, sizeof(*info));
-        info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
+        info->lower = info->upper = info->relative_gap = info->bracket =
+            std::numeric_limits<double>::quiet_NaN();
         info->status = kPriaErrorArgument;
     }
     if (!pria_params_valid(params)) return kPriaErrorArgument;
@@ -403,6 +404,7 @@ int solve_f64io(int64_t n, int64_t t, const double* k, double k0, const double* 
         return status;
     }
     out->upper = pria_objective_host(n, t, k, k0, h, xd.data());
+    out->bracket = out->upper - out->lower;
     if (x) std::memcpy(x, xd.data(), sizeof(double) * xd.size());
     return out->status == 0 ? 0 : 1;
 }
@@ -614,13 +616,15 @@ int solve_device_io(PriaHostInstance& s, const double* order_costs, double joint
                        stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
         return kPriaErrorDevice;
+    out->bracket = out->upper - out->lower;
     return out->status == 0 ? 0 : 1;
 }
 
 void clear_info(pria_info* info) {
     if (!info) return;
     std::memset(info, 0, sizeof(*info));
-    info->lower = info->upper = info->relative_gap = std::numeric_limits<double>::quiet_NaN();
+    info->lower = info->upper = info->relative_gap = info->bracket =
+        std::numeric_limits<double>::quiet_NaN();
     info->status = kPriaErrorArgument;
 }
 

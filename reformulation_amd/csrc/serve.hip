@@ -955,6 +955,7 @@ extern "C" int reform_server_info(reform_server* server, int64_t ticket, reform_
     info->completed = r->t_done;
     info->device_bytes = int64_t(r->bytes + (r->off64 != SIZE_MAX ? r->bytes64 : 0));
     info->switch_iteration = -1;
+    info->bracket = std::numeric_limits<double>::quiet_NaN();
     if (r->status >= 0) {
         info->iterations = r->final.iteration;
         info->restarts = r->final.restarts;
@@ -963,6 +964,7 @@ extern "C" int reform_server_info(reform_server* server, int64_t ticket, reform_
         info->upper = r->final.upper;
         info->relative_gap = r->final.gap;
         if (r->family == 1) info->upper = r->upper_original;
+        info->bracket = info->upper - info->lower;
     }
     return 0;
 }

@@ -42,7 +42,8 @@ class RequestInfo(ctypes.Structure):
     _fields_ = [("ticket", _i64), ("family", _int), ("status", _int), ("iterations", _i64),
                 ("restarts", _i64), ("switch_iteration", _i64), ("lower", _dbl),
                 ("upper", _dbl), ("relative_gap", _dbl), ("submitted", _dbl),
-                ("admitted", _dbl), ("completed", _dbl), ("device_bytes", _i64)]
+                ("admitted", _dbl), ("completed", _dbl), ("device_bytes", _i64),
+                ("bracket", _dbl)]
 
 
 class ServerCounters(ctypes.Structure):
@@ -103,6 +104,7 @@ class Info:
     admitted: float
     completed: float
     device_bytes: int
+    bracket: float
 
     @property
     def latency(self) -> float:
@@ -204,7 +206,8 @@ class Server:
             raise KeyError(ticket)
         return Info(int(r.ticket), "praq" if r.family == 0 else "pria", int(r.status),
                     int(r.iterations), int(r.restarts), int(r.switch_iteration), r.lower, r.upper,
-                    r.relative_gap, r.submitted, r.admitted, r.completed, int(r.device_bytes))
+                    r.relative_gap, r.submitted, r.admitted, r.completed, int(r.device_bytes),
+                    r.bracket)
 
     def praq_result(self, ticket: int):
         _, n, m = self._shapes[ticket]
