@@ -3,6 +3,8 @@
 // hipGraph.
 #include "wide.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -869,17 +871,23 @@ void WideControl::release() {
     bytes = 0;
 }
 
-int WideControl::mark(hipStream_t stream, int kind, bool begin) {
+int WideControl::mark_events(int kind, hipEvent_t* begin, hipEvent_t* end) {
+    *begin = *end = nullptr;
     if (!timing) return 0;
     std::vector<hipEvent_t>& set = marks[mark_set];
-    const int k = marks_used[mark_set]++;
-    if (k == int(set.size())) {
-        hipEvent_t e;
-        REFORM_HIP_CHECK(hipEventCreate(&e));
-        set.push_back(e);
+    for (hipEvent_t* out : {begin, end}) {
+        const int k = marks_used[mark_set]++;
+        if (k == int(set.size())) {
+            // timing only: no system-scope fence (its cache writeback would
+            // be timed with the kernels); the poll event that follows them
+            // on the stream keeps its fence
+            hipEvent_t e;
+            REFORM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            set.push_back(e);
+        }
+        *out = set[k];
     }
-    if (begin) mark_kinds[mark_set].push_back(kind);
-    REFORM_HIP_CHECK(hipEventRecord(set[k], stream));
+    mark_kinds[mark_set].push_back(kind);
     return 0;
 }
 
@@ -1046,17 +1054,16 @@ int WidePraq::build_tables(const std::vector<PraqJob<double>>& host64,
 // edge kernel, each bracketed by timing marks.
 int WidePraq::step(hipStream_t stream, int mode, int j) {
     const dim3 blk(kWideBlock);
+    hipEvent_t b, e;
     if (node_blocks) {
-        if (mark(stream, 0, true)) return kErrorDevice;
-        hipLaunchKernelGGL(wpraq_node, dim3(node_blocks), blk, 0, stream, jobs32, jobs64, node_tab,
-                           round_action, mode, j);
-        if (mark(stream, 0, false)) return kErrorDevice;
+        if (mark_events(0, &b, &e)) return kErrorDevice;
+        hipExtLaunchKernelGGL(wpraq_node, dim3(node_blocks), blk, 0, stream, b, e, 0, jobs32,
+                              jobs64, node_tab, round_action, mode, j);
     }
     if (edge_blocks) {
-        if (mark(stream, 1, true)) return kErrorDevice;
-        hipLaunchKernelGGL(wpraq_edge, dim3(edge_blocks), blk, 0, stream, jobs32, jobs64, edge_tab,
-                           round_action, mode, j);
-        if (mark(stream, 1, false)) return kErrorDevice;
+        if (mark_events(1, &b, &e)) return kErrorDevice;
+        hipExtLaunchKernelGGL(wpraq_edge, dim3(edge_blocks), blk, 0, stream, b, e, 0, jobs32,
+                              jobs64, edge_tab, round_action, mode, j);
     }
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
@@ -1253,7 +1260,8 @@ int WidePria::build_lists(hipStream_t stream, int mode) {
 
 // One step of every instance the mode selects: the x kernels, then the y/z
 // kernels, over the round's work lists (built by build_lists for the mode),
-// each pair bracketed by timing marks.
+// each pair bracketed by timing events (the first kernel's start, the last
+// kernel's stop).
 int WidePria::step(hipStream_t stream, int mode, int j) {
     const int* lx32 = lists;
     const int* lx64 = lists + x_blocks;
@@ -1261,33 +1269,36 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
     const int* ls64 = lists + 2 * x_blocks + s_blocks;
     // FP32 launches exist for batches with FP32 images; every batch can hold
     // FP64 instances (FP64 batches, and FP32 ones after the handover)
-    if (mark(stream, 0, true)) return kErrorDevice;
+    hipEvent_t b, e;
+    if (mark_events(0, &b, &e)) return kErrorDevice;
     if (has32)
-        hipLaunchKernelGGL((wpria_x<float, kWideTW>), dim3(x_blocks), dim3(256), lds32, stream,
-                           jobs32, jobs64, x_tab, lx32, lcount + kListX32, round_action, mode, j);
+        hipExtLaunchKernelGGL((wpria_x<float, kWideTW>), dim3(x_blocks), dim3(256), lds32, stream,
+                              b, nullptr, 0, jobs32, jobs64, x_tab, lx32, lcount + kListX32,
+                              round_action, mode, j);
+    hipEvent_t b64 = has32 ? nullptr : b;
     if (x64_full)
-        hipLaunchKernelGGL((wpria_x<double, kWideTW>), dim3(x_blocks), dim3(512), lds64, stream,
-                           jobs32, jobs64, x_tab, lx64, lcount + kListX64, round_action, mode, j);
+        hipExtLaunchKernelGGL((wpria_x<double, kWideTW>), dim3(x_blocks), dim3(512), lds64,
+                              stream, b64, e, 0, jobs32, jobs64, x_tab, lx64, lcount + kListX64,
+                              round_action, mode, j);
     else
-        hipLaunchKernelGGL((wpria_x<double, kWideTW / 2>), dim3(x_blocks), dim3(512), lds64,
-                           stream, jobs32, jobs64, x_tab, lx64, lcount + kListX64, round_action,
-                           mode, j);
-    if (mark(stream, 0, false) || mark(stream, 1, true)) return kErrorDevice;
+        hipExtLaunchKernelGGL((wpria_x<double, kWideTW / 2>), dim3(x_blocks), dim3(512), lds64,
+                              stream, b64, e, 0, jobs32, jobs64, x_tab, lx64, lcount + kListX64,
+                              round_action, mode, j);
+    if (mark_events(1, &b, &e)) return kErrorDevice;
     const bool compact = max_T <= kPriaCompactT;
-    auto yz = [&](auto real, const int* list, int l) {
+    auto yz = [&](auto real, const int* list, int l, hipEvent_t start, hipEvent_t stop) {
         using Real = decltype(real);
         if (compact)
-            hipLaunchKernelGGL((wpria_yz<Real, true>), dim3(s_blocks), dim3(512),
-                               kPriaZCacheBytes, stream, jobs32, jobs64, s_tab, list, lcount + l,
-                               round_action, mode, j);
+            hipExtLaunchKernelGGL((wpria_yz<Real, true>), dim3(s_blocks), dim3(512),
+                                  kPriaZCacheBytes, stream, start, stop, 0, jobs32, jobs64, s_tab,
+                                  list, lcount + l, round_action, mode, j);
         else
-            hipLaunchKernelGGL((wpria_yz<Real, false>), dim3(s_blocks), dim3(512),
-                               kPriaZCacheBytes, stream, jobs32, jobs64, s_tab, list, lcount + l,
-                               round_action, mode, j);
+            hipExtLaunchKernelGGL((wpria_yz<Real, false>), dim3(s_blocks), dim3(512),
+                                  kPriaZCacheBytes, stream, start, stop, 0, jobs32, jobs64, s_tab,
+                                  list, lcount + l, round_action, mode, j);
     };
-    if (has32) yz(float(0), ls32, kListS32);
-    yz(double(0), ls64, kListS64);
-    if (mark(stream, 1, false)) return kErrorDevice;
+    if (has32) yz(float(0), ls32, kListS32, b, nullptr);
+    yz(double(0), ls64, kListS64, has32 ? nullptr : b, e);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }

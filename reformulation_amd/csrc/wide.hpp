@@ -151,8 +151,10 @@ struct WideControl {
     // drops the previous tables and graph
     int allocate(size_t table_bytes, size_t partial_rows, int partial_width);
     void release();
-    // bracket one step launch of `kind` (no-op unless timing)
-    int mark(hipStream_t stream, int kind, bool begin);
+    // the events that bracket one step launch of `kind`, null unless timing:
+    // *begin goes to its first kernel's start, *end to its last kernel's
+    // stop (hipExtLaunchKernelGGL), so the interval is the kernels' own
+    int mark_events(int kind, hipEvent_t* begin, hipEvent_t* end);
     // add the event pairs of a completed set to kernel_ms / launches
     int collect(int set);
     // turn timing on or off (the totals of the last solve stay readable)
