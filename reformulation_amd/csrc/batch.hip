@@ -396,7 +396,8 @@ int BatchImpl::upload() {
         PriaEntry& e = pria[q];
         e.tier = e.o.tier != PRIA_TIER_AUTO
                      ? e.o.tier
-                     : (e.host.ND <= pria_wg ? PRIA_TIER_WORKGROUP : PRIA_TIER_DEVICE);
+                     : (e.host.ND <= pria_wg && e.host.T <= kPriaWgMaxT ? PRIA_TIER_WORKGROUP
+                                                                         : PRIA_TIER_DEVICE);
         auto fill = [&](auto& job, const PriaLayout& L, size_t off, auto tag) -> int {
             using Real = decltype(tag);
             size_t up = pria_upload_bytes(e.host, L);

@@ -518,6 +518,11 @@ int pria_convert_jobs(const PriaJob<float>* from, const PriaJob<double>* to, int
 template <class Real>
 int pria_launch_wg(const PriaJob<Real>* jobs, int count, int max_T, hipStream_t stream) {
     if (count <= 0) return 0;
+    if (max_T > kPriaWgMaxT) {
+        record_message("the one-workgroup PRIA tier takes T' <= " + std::to_string(kPriaWgMaxT) +
+                       ", not " + std::to_string(max_T) + "; use the device tier");
+        return kPriaErrorArgument;
+    }
     size_t lds = pria_x_lds_bytes<Real>(max_T);
     if (lds > 65536) {
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(pria_wg_solve<Real>),

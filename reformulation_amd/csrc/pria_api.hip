@@ -201,7 +201,8 @@ __global__ void __launch_bounds__(kDevBlock) pria_objective_total(const double* 
 
 int pick_tier(const PriaHostInstance& s, int tier) {
     if (tier != PRIA_TIER_AUTO) return tier;
-    return s.ND <= pria_wg_max() ? PRIA_TIER_WORKGROUP : PRIA_TIER_DEVICE;
+    return s.ND <= pria_wg_max() && s.T <= kPriaWgMaxT ? PRIA_TIER_WORKGROUP
+                                                        : PRIA_TIER_DEVICE;
 }
 
 // One instance's device image in one precision, and the tier that runs it.
@@ -322,6 +323,7 @@ int solve_mixed(const PriaHostInstance& s, const LogicParams& lp, const pria_opt
                 bool fp32, const PriaDeviceInput* din, hipStream_t stream, Emit&& emit,
                 pria_info* info, char* arena = nullptr) {
     const int tier = pick_tier(s, o.tier);
+    if (tier == PRIA_TIER_WORKGROUP && s.T > kPriaWgMaxT) return kPriaErrorArgument;
     LogicState st;
     std::memset(&st, 0, sizeof(st));
     size_t held = 0;
@@ -529,7 +531,7 @@ int device_io_shape(int64_t n, int64_t t, const double* order_costs, const doubl
         return kPriaErrorArgument;
     const int64_t T = t + (t & 1);
     const int64_t nd = n * T * (T + 1) / 2;
-    if (nd >= (int64_t(1) << 31) || T > 640) return kPriaErrorArgument;
+    if (nd >= (int64_t(1) << 31) || T > kPriaMaxT) return kPriaErrorArgument;
     s.N = int(n);
     s.T = int(T);
     s.Torig = int(t);
@@ -651,7 +653,7 @@ int64_t noalloc_bytes(int64_t n, int64_t t, int use_float32, int with_x) {
     if (n <= 0 || t <= 0) return -1;
     const int64_t T = t + (t & 1);
     const int64_t nd = n * T * (T + 1) / 2;
-    if (nd >= (int64_t(1) << 31) || T > 640) return -1;
+    if (nd >= (int64_t(1) << 31) || T > kPriaMaxT) return -1;
     PriaHostInstance s;
     s.N = int(n);
     s.T = int(T);

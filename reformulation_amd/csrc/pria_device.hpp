@@ -353,13 +353,87 @@ __device__ __forceinline__ double group_michelot(const Real (&u)[VR], int len, i
     return th;
 }
 
+// A y row longer than a lane group's registers hold (len > 64 VR, only with
+// T' > 64 VR): one wavefront, values u = y1 + r read from memory on every
+// Michelot pass (a row is at most T' values, resident in L2).  Lane lr sums
+// its entries lr + 64 k in order of k and the wavefront's DPP sum follows,
+// as in group_michelot; every tier runs rows of this length here, so they
+// give the same bits.
+template <class Real>
+__device__ __forceinline__ void pria_y_row_memory(const PriaView<Real>& a,
+                                                  const PriaStep<Real>& st,
+                                                  gptr<const Real> srcy, int rb, int len,
+                                                  int lane, bool row, int i, int s) {
+    const double target = row ? double(a.k[i]) : 0.0;
+    const bool live = row && target > 0.0;
+    const double start = live ? a.thy[i * a.T + s] : 0.0;
+    auto u_at = [&](int e) { return double(Real(srcy[rb + e] + a.r[rb + e])); };
+    double S = 0.0, C = 0.0, Sw = 0.0, Cw = 0.0;
+    if (live)
+        for (int e = lane; e < len; e += kWave) {
+            const double v = u_at(e);
+            S += v;
+            C += 1.0;
+            if (v > start) {
+                Sw += v;
+                Cw += 1.0;
+            }
+        }
+    S = wave_sum(S);
+    C = wave_sum(C);
+    Sw = wave_sum(Sw);
+    Cw = wave_sum(Cw);
+    bool first = Cw > 0.0;
+    if (first) {
+        S = Sw;
+        C = Cw;
+    }
+    double th = live ? (S - target) / C : 0.0;
+    bool done = !live;
+    for (int round = 0; round <= len && !done; ++round) {   // wave-uniform: one row
+        double S2 = 0.0, C2 = 0.0;
+        for (int e = lane; e < len; e += kWave) {
+            const double v = u_at(e);
+            if (v > th) {
+                S2 += v;
+                C2 += 1.0;
+            }
+        }
+        S2 = wave_sum(S2);
+        C2 = wave_sum(C2);
+        if (C2 == C || C2 == 0.0 || (C2 > C && !first)) {
+            done = true;
+        } else {
+            C = C2;
+            th = (S2 - target) / C2;
+        }
+        first = false;
+    }
+    if (live && lane == 0) a.thy[i * a.T + s] = th;
+    if (!row) return;
+    for (int e = lane; e < len; e += kWave) {
+        const int g = rb + e;
+        const Real v1 = srcy[g];
+        const Real u = v1 + a.r[g];
+        const Real an = st.reflect ? a.y0[g] : v1;
+        const Real d = u - Real(th);
+        const Real y2 = live ? (d > Real(0) ? d : Real(0)) : Real(0);
+        a.yc[g] = st.lwn * (Real(2) * y2 - v1) + st.rwn * an;
+        if (st.store) {
+            a.y[g] = v1;
+            a.sy[g] = y2;
+        }
+    }
+}
+
 // --- y/z pass: one slice s -------------------------------------------------
 //
 // part: 1 the y rows [i_lo, i_hi), 2 the z slice, 3 both; a block running
 // only rows needs no LDS and no barrier.
 // y rows: all N rows of the slice have len = T' - s entries; each row goes
 // to a group of 16, 32 or 64 lanes (the smallest holding len in VR values
-// per lane), so a wavefront projects 4, 2 or 1 rows at once (T' <= 64 VR).
+// per lane), so a wavefront projects 4, 2 or 1 rows at once; rows longer
+// than 64 VR take pria_y_row_memory, one per wavefront.
 // Widths follow kPriaRow16 / kPriaRow32, not VR.
 // z slice: the whole workgroup, values in registers when the slice fits
 // (N (T' - s) <= blockDim * VZ), else re-read from memory each round; its
@@ -399,10 +473,15 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
     const int width = len <= kPriaRow16 ? 16 : (len <= kPriaRow32 ? 32 : 64);
     const int rpw = kWave / width, lr = lane % width, sub = lane / width;
     const int rows_end = (part & 1) ? min(N, i_hi) : 0;
+    const bool long_rows = len > kWave * VR;   // block-uniform: every row of the slice
     for (int i0 = i_lo + wave * rpw; i0 < rows_end; i0 += waves * rpw) {
         const int i = i0 + sub;
         const bool row = i < rows_end;
         const int rb = base + (row ? i : 0) * len;
+        if (long_rows) {   // width 64: one row per wavefront
+            pria_y_row_memory<Real>(a, st, srcy, rb, len, lane, row, i, s);
+            continue;
+        }
         // unconditional loads (entries past the row read the row's first
         // entry and are dropped), issued as one batch
         int g[VR];
@@ -653,7 +732,7 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
 
 // Tile and register shapes.  x tiles of 128-byte rows: 32 FP32 or 16 FP64
 // columns, so a 512-thread tile of T' = 450 rows takes 60 KB of LDS and two
-// fit a CU.  VR: row values per lane (T' <= 64 VR); VZ: slice values per
+// fit a CU.  VR: row values per lane (longer rows from memory); VZ: slice values per
 // thread held in registers.  The compact shape (T' <= 320) keeps the
 // batched y/z kernel within 80 VGPRs, three 512-thread blocks per CU.
 template <class Real> struct PriaShape;

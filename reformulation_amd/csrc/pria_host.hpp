@@ -32,6 +32,12 @@ inline bool pria_params_valid(const pria_params* p) {
 constexpr int kPriaErrorArgument = -1;
 constexpr int kPriaErrorDevice = -3;
 
+// Largest padded period count T': the x tile of 16 FP64 columns (T' x 17
+// doubles) fits the 160 KB LDS.  The one-workgroup tier, whose x tile shares
+// the LDS with the workgroup's own arrays, takes T' up to 640.
+constexpr int kPriaMaxT = 1200;
+constexpr int kPriaWgMaxT = 640;
+
 template <class Real>
 struct PriaJob {
     using real_type = Real;
@@ -87,7 +93,7 @@ inline int pria_build_host(int64_t n, int64_t t, const double* k, double k0, con
     if (n <= 0 || t <= 0 || !k || !h) return kPriaErrorArgument;
     const int64_t T = t + (t & 1);
     const int64_t nd = n * T * (T + 1) / 2;
-    if (nd >= (int64_t(1) << 31) || T > 640) return kPriaErrorArgument;
+    if (nd >= (int64_t(1) << 31) || T > kPriaMaxT) return kPriaErrorArgument;
     s.N = int(n);
     s.T = int(T);
     s.Torig = int(t);

@@ -120,7 +120,7 @@ def test_device_pointer_entries_validate_before_any_device_work():
                                                   ctypes.byref(po), 0, 4096, None, None,
                                                   ctypes.byref(pinfo), None)
     assert pcall(0, 10, q) == -1
-    assert pcall(2, 641, q) == -1                          # T' above 640
+    assert pcall(2, 1201, q) == -1                         # T' above 1200
     q.omega = 0.0
     assert pcall(2, 10, q) == -1
     assert lib.pria_solve_device(2, 10, None, 1.0, 4096, ctypes.byref(q), None, 0, None, None,
@@ -140,7 +140,7 @@ def test_no_alloc_entry_validates_and_sizes_before_any_device_work():
         n, t, 4096, 1.0, 4096, ctypes.byref(q), None, f32, 4096, None, None,
         ctypes.byref(info), None)
     assert call(0, 10, q) == -1
-    assert call(2, 641, q) == -1                           # T' above 640
+    assert call(2, 1201, q) == -1                          # T' above 1200
     # 100 x 300: 4.5e6 entries, about 140 bytes each with FP64 iterates
     assert call(100, 300, q) == -1 and info.status == -1
     err = _native.last_error()

@@ -294,3 +294,44 @@ def test_op_steps_zero_takes_the_size_rule_in_real_solves():
         assert info.iterations == o.iterations and info.restarts == o.restarts
         assert abs(info.upper - o.upper) <= 1e-9 * (1 + abs(o.upper))
     b.close()
+
+
+@pytest.mark.parametrize("shape", [(2, 700), (3, 701)])
+def test_periods_beyond_640_match_the_oracle(shape):
+    """T' above 64 VR = 640: y rows longer than a lane group's registers go
+    to the wavefront-per-row memory path.  FP64 on the device tier and in the
+    batched mode: iterations equal the oracle's, bounds to 1e-9, x to 1e-8;
+    FP32 iterates in the batched mode close the bracket on the oracle's.  The
+    one-workgroup tier, limited to T' <= 640, refuses the instance."""
+    import dataclasses
+    from reformulation_amd import pria, workloads
+    from reformulation_amd.batch import Batch
+    n, t = shape
+    j = workloads.pria_c1(500, n, t)
+    problem = j.problem()
+    print(f"long periods {n}x{t}: oracle", flush=True)
+    o = oracle.pria_solve(j.order_costs, j.joint_order_cost, j.holding_costs,
+                          oracle.pria_params(tolerance=1e-6))
+    sc = 1.0 + abs(o.upper)
+    params = pria.Parameters.with_auto_step(50_000, 1e-6)
+    x, _, _, info = pria.solve_with_info(problem, dataclasses.replace(params, tier=2))
+    assert info.status == 0 and info.tier == 2
+    assert info.iterations == o.iterations
+    assert abs(info.upper - o.upper) <= 1e-9 * sc and abs(info.lower - o.lower) <= 1e-9 * sc
+    assert np.max(np.abs(x - o.x)) <= 1e-8
+    with pytest.raises(ValueError):
+        pria.solve_with_info(problem, dataclasses.replace(params, tier=1))
+    for f32 in (False, True):
+        print(f"long periods {n}x{t}: batched f32={f32}", flush=True)
+        b = Batch(use_float32=f32)
+        q = b.add_pria(problem, params)
+        b.upload()
+        b.solve("batched")
+        _, _, _, bi = b.pria_result(q, with_solution=False)
+        b.close()
+        assert bi.status == 0
+        if f32:
+            assert bi.lower <= o.upper + 2e-6 * sc and bi.upper >= o.lower - 2e-6 * sc
+        else:
+            assert bi.iterations == o.iterations
+            assert abs(bi.upper - o.upper) <= 1e-9 * sc
