@@ -27,6 +27,19 @@
 
 static int64_t chunk_lo(int64_t len, int c) { return (len * c) / NCHUNK; }
 
+#define SCALE_CHUNK 4096
+
+static double sumsq_blocked(const double *v, int64_t len, int weights) {
+    double total = 0.0;
+    for (int64_t c0 = 0; c0 < len; c0 += SCALE_CHUNK) {
+        int64_t c1 = c0 + SCALE_CHUNK < len ? c0 + SCALE_CHUNK : len;
+        double part = 0.0;
+        for (int64_t k = c0; k < c1; ++k) part += weights ? 4.0 * v[k] * v[k] : v[k] * v[k];
+        total += part;
+    }
+    return total;
+}
+
 /* ---- structure ---------------------------------------------------------- */
 
 typedef struct {
@@ -63,10 +76,12 @@ static int praq_build(Praq* s, int64_t n, int64_t m, const double* diag, const i
     s->ptr = calloc(n + 1, sizeof(int64_t));
     s->half = malloc(sizeof(int64_t) * (2 * m + 1));
 
-    /* objective_scale: ||(diag; 2|W_ij|)||_2, 1 if it is 0 */
-    double sq = 0.0;
-    for (int64_t i = 0; i < n; ++i) sq += diag[i] * diag[i];
-    for (int64_t e = 0; e < m; ++e) sq += 4.0 * vals[e] * vals[e];
+    /* objective_scale: ||(diag; 2|W_ij|)||_2, 1 if it is 0.  The squares in
+     * a fixed blocked order (chunks of 4096 terms in order, the chunk sums in
+     * order, diagonal and weights as two such sums), the order the solver
+     * builds in (the reference's jnp.linalg.norm reduces in an order of its
+     * own), so the oracle's scale has the solver's bits. */
+    double sq = sumsq_blocked(diag, n, 0) + sumsq_blocked(vals, m, 1);
     s->scale = sq > 0.0 ? sqrt(sq) : 1.0;
 
     for (int64_t e = 0; e < m; ++e) {

@@ -68,7 +68,10 @@ struct PraqImage {
     PraqJob<Real> job;
     size_t job_off = 0, state_off = 0, work_off = 0, bytes = 0;
 
-    int init(const PraqHostInstance& s, const LogicParams& lp, hipStream_t stream) {
+    // `dev` set: the input part written on the device from its build
+    // (praq_solve_device); else the host image of `s` uploaded
+    int init(const PraqHostInstance& s, const LogicParams& lp, hipStream_t stream,
+             const PraqDeviceBuild* dev = nullptr) {
         L = praq_layout(s, sizeof(Real));
         job_off = align256(L.total_bytes);
         state_off = align256(job_off + sizeof(PraqJob<Real>));
@@ -76,10 +79,14 @@ struct PraqImage {
         bytes = work_off + praq_device_work_bytes();
         if (buf.alloc(bytes)) return kErrorDevice;
         char* base = static_cast<char*>(buf.ptr);
-        std::vector<char> image(L.input_bytes, 0);
-        praq_fill_image<Real>(s, L, image.data());
-        REFORM_HIP_CHECK(hipMemcpyAsync(base, image.data(), L.input_bytes, hipMemcpyHostToDevice,
-                                        stream));
+        if (dev) {
+            if (praq_fill_image_device<Real>(*dev, L, base, stream)) return kErrorDevice;
+        } else {
+            std::vector<char> image(L.input_bytes, 0);
+            praq_fill_image<Real>(s, L, image.data());
+            REFORM_HIP_CHECK(hipMemcpyAsync(base, image.data(), L.input_bytes,
+                                            hipMemcpyHostToDevice, stream));
+        }
         REFORM_HIP_CHECK(hipStreamSynchronize(stream));
         std::memset(&job, 0, sizeof(job));
         job.a = praq_bind<Real>(L, base, s);
@@ -149,27 +156,29 @@ struct PraqImage {
 // finished in.
 template <class Emit>
 int solve_mixed(const PraqHostInstance& s, LogicParams lp, const PraqOptions& o, bool fp32,
-                hipStream_t stream, Emit&& emit, PraqInfo* info) {
+                hipStream_t stream, Emit&& emit, PraqInfo* info,
+                const PraqDeviceBuild* dev = nullptr) {
     const int tier = pick_tier(s, o.tier);
     LogicState st;
     std::memset(&st, 0, sizeof(st));
     size_t held = 0;
     if (!fp32) {
         PraqImage<double> d;
-        if (d.init(s, lp, stream) || d.run(tier, st, o.time_limit, false, stream) || emit(d))
+        if (d.init(s, lp, stream, dev) || d.run(tier, st, o.time_limit, false, stream) ||
+            emit(d))
             return kErrorDevice;
         held = d.bytes;
     } else {
         LogicParams lp32 = lp;
         apply_fp32_policy(lp32);
         PraqImage<float> f;
-        if (f.init(s, lp32, stream) || f.run(tier, st, o.time_limit, false, stream))
+        if (f.init(s, lp32, stream, dev) || f.run(tier, st, o.time_limit, false, stream))
             return kErrorDevice;
         held = f.bytes;
         if (st.stalled) {
             PraqImage<double> d;
             st.switch_iteration = st.iteration;
-            if (d.init(s, lp, stream) || praq_convert_state(f.job.a, d.job.a, stream) ||
+            if (d.init(s, lp, stream, dev) || praq_convert_state(f.job.a, d.job.a, stream) ||
                 d.run(tier, st, o.time_limit, true, stream) || emit(d))
                 return kErrorDevice;
             held += d.bytes;
@@ -330,26 +339,36 @@ extern "C" int praq_solve_device(const PraqInstance* instance, const PraqParamet
     if (n > std::numeric_limits<int>::max() || m > std::numeric_limits<int>::max() / 2)
         return kErrorInstance;
     hipStream_t stream = static_cast<hipStream_t>(stream_handle);
-    // the COO instance to the host for the O(n + m) CSR build
-    std::vector<double> diag(n), vals(m);
-    std::vector<int64_t> rows(m), cols(m);
-    REFORM_HIP_CHECK(hipMemcpyAsync(diag.data(), instance->diag, sizeof(double) * n,
-                                    hipMemcpyDeviceToHost, stream));
-    if (m > 0) {
-        REFORM_HIP_CHECK(hipMemcpyAsync(rows.data(), instance->rows, sizeof(int64_t) * m,
-                                        hipMemcpyDeviceToHost, stream));
-        REFORM_HIP_CHECK(hipMemcpyAsync(cols.data(), instance->cols, sizeof(int64_t) * m,
-                                        hipMemcpyDeviceToHost, stream));
-        REFORM_HIP_CHECK(hipMemcpyAsync(vals.data(), instance->vals, sizeof(double) * m,
-                                        hipMemcpyDeviceToHost, stream));
+    PraqParameters p = *parameters;
+    const PraqOptions o = options ? *options : praq_default_options();
+    // lanes per node for the tier the size picks, as praq_solve_ex's build
+    PraqHostInstance shape;
+    shape.n = int(n);
+    shape.m = int(m);
+    const int lanes = pick_tier(shape, o.tier) == PRAQ_TIER_DEVICE
+                          ? praq_device_lanes(int(n), int(m))
+                          : praq_lanes_for(int(n), int(m));
+    // the instance built on the device: no copy of the COO arrays
+    PraqDeviceBuild build;
+    int status = praq_build_device(*instance, o.dense_degree, lanes, stream, build, shape);
+    if (status) {
+        if (info) info->status = status;
+        return status;
     }
-    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
-    const PraqInstance host{n, m, diag.data(), rows.data(), cols.data(), vals.data()};
-    return solve_entry(&host, parameters, options, nullptr, nullptr, stream, false,
-                       [&](auto& image, const PraqHostInstance& s, hipStream_t on) {
-                           return image.outputs_device(s, x, y, on);
-                       },
-                       info);
+    if (p.max_iter <= 0) p.max_iter = -1;
+    if (p.op_steps <= 0) p.op_steps = praq_check_interval(n, &p);
+    if (p.op_steps < 2) p.op_steps = 2;
+    const LogicParams lp = praq_logic_params(p, o);
+    PraqInfo local;
+    PraqInfo* out = info ? info : &local;
+    status = solve_mixed(shape, lp, o, p.use_float32 != 0, stream,
+                         [&](auto& image) { return image.outputs_device(shape, x, y, stream); },
+                         out, &build);
+    if (status < 0) {
+        out->status = status;
+        return status;
+    }
+    return out->status == 0 ? 0 : 1;
 }
 
 extern "C" int praq_solve(const PraqInstance* instance, const PraqParameters* parameters,

@@ -113,6 +113,24 @@ inline void project_edge_host(double* v, double w, bool covering) {
     }
 }
 
+// The objective scale's squares, ||(diag; 2 W)||^2, in a fixed blocked
+// order the host build, the device build (praq_build.hip) and the oracle all
+// follow, so they give the same bits: chunks of kPraqScaleChunk terms summed
+// in order, the chunk sums in order, the diagonal's d d and the weights'
+// (4 v) v as two such sums, added.
+constexpr int kPraqScaleChunk = 4096;
+
+inline double praq_sumsq_blocked(const double* v, int64_t len, bool weights) {
+    double total = 0.0;
+    for (int64_t c0 = 0; c0 < len; c0 += kPraqScaleChunk) {
+        const int64_t c1 = c0 + kPraqScaleChunk < len ? c0 + kPraqScaleChunk : len;
+        double part = 0.0;
+        for (int64_t k = c0; k < c1; ++k) part += weights ? 4.0 * v[k] * v[k] : v[k] * v[k];
+        total += part;
+    }
+    return total;
+}
+
 // Validates and builds; returns 0 or kErrorInstance.
 inline int praq_build_host(const PraqInstance& in, int dense_degree, const double* warm_x,
                            const double* warm_y, PraqHostInstance& s) {
@@ -131,9 +149,8 @@ inline int praq_build_host(const PraqInstance& in, int dense_degree, const doubl
         s.e0[e] = int(i);
         s.e1[e] = int(j);
     }
-    double squares = 0.0;
-    for (int i = 0; i < n; ++i) squares += in.diag[i] * in.diag[i];
-    for (int e = 0; e < m; ++e) squares += 4.0 * in.vals[e] * in.vals[e];
+    const double squares =
+        praq_sumsq_blocked(in.diag, n, false) + praq_sumsq_blocked(in.vals, m, true);
     s.scale = squares > 0.0 ? std::sqrt(squares) : 1.0;
     s.diag.resize(n);
     for (int i = 0; i < n; ++i) s.diag[i] = in.diag[i] / s.scale;
@@ -247,6 +264,35 @@ inline void praq_fill_image(const PraqHostInstance& s, const PraqLayout& L, char
     put_real(PraqLayout::kXS, s.x_start);
     put_real(PraqLayout::kYS, s.y_start);
 }
+
+// An instance built on the device from a COO instance in device memory
+// (praq_build.hip): the arrays of praq_fill_image's image, in FP64 where
+// the image holds reals, and their counts; freed in stream order.
+struct PraqDeviceBuild {
+    hipStream_t stream = nullptr;
+    void* arena = nullptr;
+    void* hub_arena = nullptr;
+    int n = 0, m = 0, nh = 0, nc = 0;
+    int *e0 = nullptr, *e1 = nullptr, *ptr = nullptr, *half = nullptr;
+    int *heavy = nullptr, *hchunk = nullptr, *chunk_node = nullptr;
+    double *w = nullptr, *diag = nullptr, *deg = nullptr, *xs = nullptr, *ys = nullptr;
+    PraqDeviceBuild() = default;
+    PraqDeviceBuild(const PraqDeviceBuild&) = delete;
+    PraqDeviceBuild& operator=(const PraqDeviceBuild&) = delete;
+    ~PraqDeviceBuild();
+};
+
+// The cold-start build of `in` (device pointers) with `lanes` per node:
+// validation (kErrorInstance as praq_build_host), then the arrays on the
+// device; `shape` receives the counts, scale and lanes the layout and the
+// binding read (its arrays stay empty).  Work on stream s.
+int praq_build_device(const PraqInstance& in, int dense_degree, int lanes, hipStream_t s,
+                      PraqDeviceBuild& b, PraqHostInstance& shape);
+
+// praq_fill_image from a device build, on the device (stream s).
+template <class Real>
+int praq_fill_image_device(const PraqDeviceBuild& b, const PraqLayout& L, char* image,
+                           hipStream_t s);
 
 // Binds with the hub path off (heavy_degree = INT_MAX): every tier but the
 // device tier walks every node with its lane group.

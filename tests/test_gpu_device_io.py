@@ -46,6 +46,65 @@ def test_praq_device_entry_equals_the_host_entry(f32, size):
     assert np.array_equal(y.cpu().numpy(), hy)
 
 
+def _praq_device_vs_host(problem, params):
+    import torch
+    from reformulation_amd import praq
+    n, m = problem.n(), problem.m()
+    hx, hy, hinfo = praq.solve_with_info(problem, params)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        y = torch.zeros((m, 2), dtype=torch.float64, device="cuda") if m else None
+        info = praq.solve_device(_dev(problem.diag), _dev(problem.rows, torch.int64),
+                                 _dev(problem.cols, torch.int64), _dev(problem.vals), params, x,
+                                 y, stream=stream)
+    stream.synchronize()
+    assert info.status == hinfo.status and info.tier == hinfo.tier
+    assert info.iterations == hinfo.iterations and info.scale == hinfo.scale
+    assert info.lower == hinfo.lower and info.upper == hinfo.upper
+    assert np.array_equal(x.cpu().numpy(), hx)
+    if m:
+        assert np.array_equal(y.cpu().numpy(), hy)
+    return info
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_praq_device_build_equals_the_host_build(f32):
+    """praq_solve_device builds the half-edge CSR on the device (a stable
+    radix sort of the half-edges by node, hub lists, scale, start): a
+    power-law graph whose hubs take the chunked gather, the dense-degree mode
+    and an edgeless instance solve bit for bit as the host build's."""
+    import dataclasses
+    from reformulation_amd import praq, workloads
+    params = praq.Parameters.with_auto_step(200_000, 1e-6, use_float32=f32)
+    cl = workloads.praq_chung_lu(3, 100_000, mean_degree=16.0)
+    print(f"device build: chung-lu n={cl.n} m={cl.edges} f32={f32}", flush=True)
+    info = _praq_device_vs_host(cl.problem(), params)
+    assert info.tier == 2
+    er = workloads.praq_er(8, 3000, 12000)
+    print("device build: dense degree", flush=True)
+    _praq_device_vs_host(er.problem(), dataclasses.replace(params, dense_degree=1))
+    edgeless = praq.QuboProblem.from_sparse(5, np.array([1.0, -2.0, 0.5, 0.0, -1.0]),
+                                            np.zeros(0, np.int64), np.zeros(0, np.int64),
+                                            np.zeros(0))
+    print("device build: edgeless", flush=True)
+    _praq_device_vs_host(edgeless, params)
+
+
+def test_praq_device_build_rejects_invalid_edges():
+    import torch
+    from reformulation_amd import praq
+    params = praq.Parameters.with_auto_step(1000, 1e-6)
+    diag = _dev(np.zeros(4))
+    vals = _dev(np.ones(2))
+    x = torch.zeros(4, dtype=torch.float64, device="cuda")
+    for rows, cols in (([0, 2], [1, 2]), ([0, 1], [1, 4]), ([-1, 0], [2, 3])):
+        print(f"device build: invalid edges {rows} {cols}", flush=True)
+        with pytest.raises(ValueError, match="status -2"):
+            praq.solve_device(diag, _dev(np.array(rows), torch.int64),
+                              _dev(np.array(cols), torch.int64), vals, params, x)
+
+
 def _pria_case(n, t, seed):
     from reformulation_amd import workloads
     j = workloads.pria_c1(seed, n, t)
