@@ -94,13 +94,15 @@ typedef struct {
     int status;             /* 0 converged, 1 iteration limit, 2 time limit  */
     int tier;               /* tier that ran                                 */
     int64_t switch_iteration; /* FP32 iterates handed over to FP64 at this
-                               * iteration, having stalled; -1 if they did not */
+                               * iteration (gap below 3e-5, stall, or a
+                               * tolerance below 3e-5 met); -1 if they did not */
     int64_t device_bytes;   /* device memory the solve held at its peak: its
                              * images (both precisions when FP32 handed over)
                              * and work areas                               */
     double bracket;         /* upper - lower, instance units (NaN on error).
                              * A solve that finished in FP64 (FP64 iterates,
-                             * or FP32 ones after switch_iteration) brackets
+                             * or FP32 ones after switch_iteration; every
+                             * FP32 solve with a tolerance below 3e-5) brackets
                              * the LP optimum: lower <= OPT <= upper, upper
                              * the objective at x, so x's objective error is
                              * at most this.  One that finished in FP32

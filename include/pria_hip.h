@@ -99,21 +99,24 @@ typedef struct {
     int status;               /* 0 converged, 1 iteration limit, 2 time limit */
     int tier;
     int64_t switch_iteration; /* FP32 iterates handed over to FP64 at this
-                               * iteration, having stalled; -1 if they did not */
+                               * iteration (gap below 3e-5, stall, or a
+                               * tolerance below 3e-5 met); -1 if they did not */
     int64_t device_bytes;     /* device memory the solve held at its peak     */
     double bracket;           /* upper - lower, instance units (NaN on error).
                                * upper is the objective of x on the original
-                               * instance, so upper >= OPT.  A solve that
-                               * finished in FP64 (FP64 iterates, or FP32
-                               * ones after switch_iteration) has lower <= OPT
-                               * too (the padded period of an odd t costs
-                               * nothing, so padding leaves OPT alone): x's
-                               * objective error is at most this.  One that
-                               * finished in FP32 (switch_iteration -1) has
-                               * lower <= the optimum of the instance as its
-                               * FP32 image holds it (costs rounded to FP32),
-                               * within 2^-24 (sum |h| + t sum K_i + t K0) of
-                               * OPT.                                         */
+                               * instance.  A solve that finished in FP64
+                               * (FP64 iterates, or FP32 ones after
+                               * switch_iteration; every FP32 solve with a
+                               * tolerance below 3e-5) has lower <= OPT <=
+                               * upper to FP64 rounding (the padded period of
+                               * an odd t costs nothing, so padding leaves
+                               * OPT alone): x's objective error is at most
+                               * this.  One that finished in FP32
+                               * (switch_iteration -1) brackets the optimum
+                               * of the instance as its FP32 image holds it
+                               * (costs rounded to FP32), and x's columns sum
+                               * to 1 to FP32 rounding: both bounds within
+                               * 2^-24 (sum |h| + t sum K_i + t K0) of OPT.   */
 } pria_info;
 
 pria_options pria_default_options(void);

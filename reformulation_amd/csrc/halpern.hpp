@@ -175,6 +175,15 @@ __host__ __device__ inline void finish(LogicState& st, const LogicParams& p, int
     st.status = st.gap <= p.tolerance ? 0 : 1;
 }
 
+// FP32 iterates meeting a tolerance below the switch gap do not finish: they
+// hand over, so the check that ends the solve is taken on FP64 iterates (whose
+// primal is feasible to FP64 rounding, which makes upper a bound of the LP).
+// Looser tolerances finish in FP32.
+__host__ __device__ inline bool certify_in_fp64(const LogicState& st, const LogicParams& p) {
+    return p.switch_gap > 0.0 && p.tolerance < p.switch_gap && st.gap <= p.tolerance &&
+           within_max_iter(st.iteration, p.max_iter);
+}
+
 // After the start step or a WARM block: bounds were taken at (x, y), drift
 // is from the projected start (the anchor arrays still hold it).
 __host__ __device__ inline void logic_after_warm_check(LogicState& st, const LogicParams& p,
@@ -201,7 +210,10 @@ __host__ __device__ inline void logic_after_warm_check(LogicState& st, const Log
     st.restart_iter = 0;
     st.anchor_pending = 1;
     bool terminated = !(within_max_iter(st.iteration, p.max_iter) && !(st.gap <= p.tolerance));
-    if (terminated) {
+    if (terminated && certify_in_fp64(st, p)) {
+        st.stalled = 1;
+        st.status = 3;
+    } else if (terminated) {
         finish(st, p, kActDoneX);
         return;
     }
@@ -224,10 +236,11 @@ __host__ __device__ inline void logic_after_halpern_check(LogicState& st, const 
     bool artificial = double(st.restart_iter) >= p.beta_artificial * double(st.iteration);
     bool restart = sufficient || necessary || artificial;
 
-    if (terminated) {
+    if (terminated && !certify_in_fp64(st, p)) {
         finish(st, p, kActDoneSX);
         return;
     }
+    // (a certify_in_fp64 case has gap <= tolerance < switch_gap: it stalls here)
     if (p.switch_gap > 0.0 && st.gap <= p.switch_gap) {
         st.stalled = 1;
         st.status = 3;

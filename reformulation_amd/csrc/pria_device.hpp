@@ -37,6 +37,8 @@
 // usually suffice (against about seven from the full set on x columns).
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "halpern.hpp"
 
@@ -147,8 +149,10 @@ __device__ inline void pria_reset_thresholds(const PriaView<Real>& a, int k, int
 
 // --- x pass: one tile of TW consecutive columns t0.. of item i ------------
 //
-// lds: (T' x (TW + 1)) pre-projection values + TW thresholds.
-template <class Real, int TW>
+// lds: (T' x (TW + 1)) pre-projection values + TW thresholds.  With HOLD > 0
+// a thread keeps x1 of its first HOLD batches of U rows in registers for the
+// write phase instead of reading it again (the same values, so the same bits).
+template <class Real, int TW, int HOLD = 0>
 __device__ void pria_x_tile(const PriaView<Real>& a, const PriaStep<Real>& st, int i, int t0,
                             Real* lds) {
     const int T = a.T, N = a.N;
@@ -167,9 +171,10 @@ __device__ void pria_x_tile(const PriaView<Real>& a, const PriaStep<Real>& st, i
     constexpr int U = 8;
     const int c = tid % TW, row = tid / TW, t = t0 + c;
     const bool colok = c < tcount;
+    Real held[HOLD > 0 ? HOLD * U : 1];
     // (the loads are unconditional -- entries outside the tile read entry 0
     // and are dropped -- so the compiler issues the batch without branches)
-    for (int s0 = row; s0 <= smax; s0 += G * U) {
+    auto load_batch = [&](int s0, auto keep, Real* kept) {
         int g[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -187,9 +192,15 @@ __device__ void pria_x_tile(const PriaView<Real>& a, const PriaStep<Real>& st, i
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int s = s0 + u * G;
+            if constexpr (decltype(keep)::value) kept[u] = vx[u];
             if (colok && s <= t) lds[s * stride + c] = vx[u] - st.tau * ((vh[u] + vz[u]) + vy[u]);
         }
-    }
+    };
+#pragma unroll
+    for (int j = 0; j < HOLD; ++j)
+        if (row + j * G * U <= smax) load_batch(row + j * G * U, std::true_type(), held + j * U);
+    for (int s0 = row + HOLD * G * U; s0 <= smax; s0 += G * U)
+        load_batch(s0, std::false_type(), nullptr);
     __syncthreads();
     // project: G lanes per column, spread across the wave (column = wave-local
     // index, q = the lane's share), reduced by xor over the q bits
@@ -255,7 +266,7 @@ __device__ void pria_x_tile(const PriaView<Real>& a, const PriaStep<Real>& st, i
     // write: x2, r, the next xc (and x, sx when storing); loads batched as above
     {
         const Real th = theta[c < TW ? c : 0];
-        for (int s0 = row; s0 <= smax; s0 += G * U) {
+        auto write_batch = [&](int s0, auto keep, const Real* kept) {
             int g[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -264,7 +275,10 @@ __device__ void pria_x_tile(const PriaView<Real>& a, const PriaStep<Real>& st, i
             }
             Real vx[U], v0[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) vx[u] = srcx[g[u]];
+            for (int u = 0; u < U; ++u) {
+                if constexpr (decltype(keep)::value) vx[u] = kept[u];
+                else vx[u] = srcx[g[u]];
+            }
             if (st.reflect) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) v0[u] = a.x0[g[u]];
@@ -289,7 +303,13 @@ __device__ void pria_x_tile(const PriaView<Real>& a, const PriaStep<Real>& st, i
                     }
                 }
             }
-        }
+        };
+#pragma unroll
+        for (int j = 0; j < HOLD; ++j)
+            if (row + j * G * U <= smax)
+                write_batch(row + j * G * U, std::true_type(), held + j * U);
+        for (int s0 = row + HOLD * G * U; s0 <= smax; s0 += G * U)
+            write_batch(s0, std::false_type(), nullptr);
     }
     __syncthreads();
 }
@@ -474,6 +494,13 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
     const int rpw = kWave / width, lr = lane % width, sub = lane / width;
     const int rows_end = (part & 1) ? min(N, i_hi) : 0;
     const bool long_rows = len > kWave * VR;   // block-uniform: every row of the slice
+    // a block running the whole slice parks the r values the rows read (the
+    // slice's entries e < rcap) in zcache for the z part, which then reads
+    // them from LDS instead of memory (with the compact register shapes, 64
+    // bytes of slice values per thread: the wider ones would spill)
+    constexpr bool park = VZ * int(sizeof(Real)) <= 64;
+    const int rcap =
+        (park && part == 3 && zcache != nullptr && !long_rows) ? min(zcap, N * len) : 0;
     for (int i0 = i_lo + wave * rpw; i0 < rows_end; i0 += waves * rpw) {
         const int i = i0 + sub;
         const bool row = i < rows_end;
@@ -502,6 +529,13 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
         } else {
 #pragma unroll
             for (int k = 0; k < VR; ++k) an[k] = y1[k];
+        }
+        if (park && rcap > 0) {
+#pragma unroll
+            for (int k = 0; k < VR; ++k) {
+                const int e = lr + width * k, es = (rb - base) + e;
+                if (row && e < len && es < rcap) zcache[es] = u[k];
+            }
         }
 #pragma unroll
         for (int k = 0; k < VR; ++k) {
@@ -534,10 +568,12 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
 
     // ---- z: the whole slice ----
     if (!(part & 2)) return;
+    if (park && rcap > 0) __syncthreads();   // the rows' r values are in zcache
     const int L = N * len;
     const double target = double(a.k0);
     double th = 0.0;
     int parity = 0;   // block_sums_alt's LDS half
+    int ucap = 0;     // the memory path kept u = z1 + r of entries e < ucap in zcache
     // register path: u = z1 + r of entry tid + BS k in u[k]; with `kept`
     // (the slice fits the LDS cache) z1 of entry e also in zcache[e], and the
     // register path writes the slice itself, reading neither z1 nor r again
@@ -551,18 +587,34 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
             double v4[4] = {0.0, 0.0, 0.0, 0.0};
             // unconditional loads up to the last k any thread needs (a
             // block-uniform bound); entries past the slice read its first
-            // entry and are dropped
+            // entry and are dropped.  r from zcache when the rows parked the
+            // whole slice's (each thread then overwrites its own entries with
+            // z1 below)
             const int kn = (L + BS - 1) / BS;
             Real rv[VZ];
+            if (park && L <= rcap) {
 #pragma unroll
-            for (int k = 0; k < VZ; ++k) {
-                const int e = tid + BS * k;
-                const int g = base + (e < L ? e : 0);
-                u[k] = Real(0);
-                rv[k] = Real(0);
-                if (k < kn) {
-                    u[k] = srcz[g];
-                    rv[k] = a.r[g];
+                for (int k = 0; k < VZ; ++k) {
+                    const int e = tid + BS * k;
+                    const int el = e < L ? e : 0;
+                    u[k] = Real(0);
+                    rv[k] = Real(0);
+                    if (k < kn) {
+                        u[k] = srcz[base + el];
+                        rv[k] = zcache[el];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < VZ; ++k) {
+                    const int e = tid + BS * k;
+                    const int g = base + (e < L ? e : 0);
+                    u[k] = Real(0);
+                    rv[k] = Real(0);
+                    if (k < kn) {
+                        u[k] = srcz[g];
+                        rv[k] = a.r[g];
+                    }
                 }
             }
             if (kept) {
@@ -651,8 +703,10 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
         } else {
             double v4[4] = {0.0, 0.0, 0.0, 0.0};
             if (!zcache) zcap = 0;
+            if (park) ucap = zcap;
             for (int e = tid; e < L; e += BS) {
-                const Real u = srcz[base + e] + a.r[base + e];
+                const Real u =
+                    srcz[base + e] + (park && e < rcap ? zcache[e] : Real(a.r[base + e]));
                 if (e < zcap) zcache[e] = u;
                 const double v = double(u);
                 v4[0] += v;
@@ -690,7 +744,9 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
         return;
     }
     const Real thr = Real(th);
-    // write: loads of U entries issued before any is used
+    // write: loads of U entries issued before any is used; with the compact
+    // shapes u = z1 + r from the cache where the memory path kept it (the
+    // same bits), and r not read at all without a projection (z2 = 0)
     constexpr int U = 4;
     for (int e0 = tid; e0 < L; e0 += BS * U) {
         int g[U];
@@ -703,7 +759,11 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             v1[q] = srcz[g[q]];
-            rr[q] = a.r[g[q]];
+            rr[q] = Real(0);
+        }
+        if (!park || (target > 0.0 && e0 + BS * (U - 1) >= ucap)) {
+#pragma unroll
+            for (int q = 0; q < U; ++q) rr[q] = a.r[g[q]];
         }
         if (st.reflect) {
 #pragma unroll
@@ -717,7 +777,7 @@ __device__ void pria_yz_slice(const PriaView<Real>& a, const PriaStep<Real>& st,
             const int e = e0 + BS * q;
             if (e < L) {
                 const int g = base + e;
-                const Real d = (v1[q] + rr[q]) - thr;
+                const Real d = (park && e < ucap ? zcache[e] : Real(v1[q] + rr[q])) - thr;
                 const Real z2 = target > 0.0 ? (d > Real(0) ? d : Real(0)) : Real(0);
                 a.zc[g] = st.lwn * (Real(2) * z2 - v1[q]) + st.rwn * an[q];
                 if (st.store) {

@@ -446,6 +446,10 @@ wpria_lists(const RJ64* j64, int count, const int* first, const int* xr,
         for (int l = 0; l < kLists; ++l) lcount[l] = scan[l][tid];
 }
 
+// Batches of x1 an FP32 x-tile thread holds for its write phase when every
+// T' <= 320 (256 threads, 8 rows per pass, 8 rows per batch: 320 rows)
+constexpr int kXHold32 = 5;
+
 template <class Real>
 __device__ __forceinline__ const PriaJob<Real>& pria_job(const RJ32* j32, const RJ64* j64,
                                                          int inst) {
@@ -459,7 +463,7 @@ __device__ __forceinline__ const PriaJob<Real>& pria_job(const RJ32* j32, const 
 // T' x 33 x 8 bytes: two blocks per CU).  TW = 16 is the FP64 fallback for
 // T' whose 32-column tile does not fit the LDS: the tile as two halves (the
 // same 16 lanes per column, so the same sums).
-template <class Real, int TW>
+template <class Real, int TW, int HOLD = 0>
 __global__ void __launch_bounds__(sizeof(Real) == 4 ? 256 : 512)
 wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab, const int* list, const int* lcount,
         const int* round_action, int mode, int j) {
@@ -472,8 +476,8 @@ wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab, const int* list, cons
     if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
     const PriaView<Real> a = pria_view(job.a);
     Real* lds = reinterpret_cast<Real*>(dyn);
-    pria_x_tile<Real, TW>(a, sc, t.y, t.z, lds);
-    if (TW < kWideTW && t.z + TW < a.T) pria_x_tile<Real, TW>(a, sc, t.y, t.z + TW, lds);
+    pria_x_tile<Real, TW, HOLD>(a, sc, t.y, t.z, lds);
+    if (TW < kWideTW && t.z + TW < a.T) pria_x_tile<Real, TW, HOLD>(a, sc, t.y, t.z + TW, lds);
 }
 
 // Register shape of the y/z pass by T' (the compact one up to T' = 320).
@@ -1239,7 +1243,9 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     lds64 = sizeof(double) * (size_t(max_T) * (TW + 1) + TW);
     x64_full = lds64 <= kPriaX64MaxLds;
     if (!x64_full) lds64 = sizeof(double) * (size_t(max_T) * (TW / 2 + 1) + TW / 2);
-    const void* x32 = reinterpret_cast<const void*>(wpria_x<float, kWideTW>);
+    const void* x32 = max_T <= kPriaCompactT
+                          ? reinterpret_cast<const void*>(wpria_x<float, kWideTW, kXHold32>)
+                          : reinterpret_cast<const void*>(wpria_x<float, kWideTW>);
     const void* x64 = x64_full ? reinterpret_cast<const void*>(wpria_x<double, kWideTW>)
                                : reinterpret_cast<const void*>(wpria_x<double, kWideTW / 2>);
     if (lds32 > 65536)
@@ -1271,7 +1277,11 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
     // FP64 instances (FP64 batches, and FP32 ones after the handover)
     hipEvent_t b, e;
     if (mark_events(0, &b, &e)) return kErrorDevice;
-    if (has32)
+    if (has32 && max_T <= kPriaCompactT)
+        hipExtLaunchKernelGGL((wpria_x<float, kWideTW, kXHold32>), dim3(x_blocks), dim3(256),
+                              lds32, stream, b, nullptr, 0, jobs32, jobs64, x_tab, lx32,
+                              lcount + kListX32, round_action, mode, j);
+    else if (has32)
         hipExtLaunchKernelGGL((wpria_x<float, kWideTW>), dim3(x_blocks), dim3(256), lds32, stream,
                               b, nullptr, 0, jobs32, jobs64, x_tab, lx32, lcount + kListX32,
                               round_action, mode, j);
