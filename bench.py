@@ -338,9 +338,21 @@ def main() -> int:
                       launches=K["launches"], avg_launch_us=K["ms"] / K["launches"] * 1e3,
                       bytes_per_launch=K["bytes"] / K["launches"]),
         cpu_baseline=cpu,
+        tuning=tuning_in_use(),
     )
     print(json.dumps(line), flush=True)
     return 0
+
+
+def tuning_in_use() -> dict:
+    """Which shape table the batched tiers ran with: its file and the
+    batched entries the library holds (unset entries keep the built-in
+    choices)."""
+    from reformulation_amd import autotune
+    env = os.environ.get("REFORM_TUNING")
+    source = "off" if env == "off" else (env or str(autotune.DEFAULT_TABLE.relative_to(ROOT)))
+    entries = {k: v for k, v in autotune.table_in_use().items() if k.startswith("batch.") and v}
+    return dict(table=source, batched_entries=entries)
 
 
 if __name__ == "__main__":

@@ -6,14 +6,26 @@
  * reform_time_praq_steps on the GPU and writes a table, which the Python
  * package loads at import.  Unset keys keep the built-in rules.
  *
- * Keys:
+ * Keys (a value of 0 leaves the built-in choice):
  *   "praq.lanes.<b>", b = 0..7: lanes per node (1, 2, ..., 64) of
  *   single-instance device-tier PRAQ solves whose mean degree 2m/n lies in
  *   [2^b, 2^(b+1)) (b = 0 also below 1, b = 7 also above 256).  Lanes set
  *   the CSR gather's lane groups, and hubs (degree above 64 lanes) take the
- *   chunked gather.  The batched and one-workgroup tiers keep the built-in
- *   rule (at most 8 half-edges per lane at the mean degree), which measured
- *   faster there.
+ *   chunked gather.
+ *   "batch.praq.lanes.<b>": the same for the instances of a batch
+ *   (reform_batch.h) and of a server (reform_serve.h); built-in: at most 8
+ *   half-edges per lane at the mean degree.
+ *   "batch.pria.x_threads": 256 or 512 threads per FP32 x-tile workgroup of
+ *   the batched and served PRIA step (8 or 16 lanes per column); built-in
+ *   256.
+ *   "batch.pria.x_hold": 1 FP32 x-tile threads keep x1 in registers for the
+ *   write phase (when every T' <= 320), 2 they read it again; built-in 1.
+ *   "batch.pria.yz_shape": 1 the compact y/z register shape when every T'
+ *   <= 320, 2 always the full one; built-in 1.
+ * A batch reads the table when its first batched solve starts, a server
+ * when it is created.  x_hold and yz_shape give the same results bit for
+ * bit; lanes and x_threads change the order of FP32/FP64 sums (x_threads
+ * only FP32's), not the algorithm.
  */
 #ifndef REFORMULATION_REFORM_TUNE_H
 #define REFORMULATION_REFORM_TUNE_H
@@ -34,6 +46,8 @@ void reform_tune_clear(void);
 
 /* Lanes per node of a device-tier PRAQ solve of an instance of this shape. */
 int reform_praq_lanes_for(int64_t n, int64_t m);
+/* Lanes per node of the same instance in a batch or a server. */
+int reform_batch_praq_lanes_for(int64_t n, int64_t m);
 
 /* Mean milliseconds of one device-tier PDHG step of `instance` (hub chunks,
  * node, hub and edge kernels; a reflected Halpern step) with `lanes` lanes

@@ -88,6 +88,7 @@ EXPORTS = (
     "reform_batch_set_compaction",
     "reform_batch_instance_seconds",
     "reform_tune_set", "reform_tune_get", "reform_tune_clear", "reform_praq_lanes_for",
+    "reform_batch_praq_lanes_for",
     "reform_time_praq_steps",
     "reform_server_default_config", "reform_server_create", "reform_server_destroy",
     "reform_server_submit_praq", "reform_server_submit_pria", "reform_server_poll",

@@ -20,6 +20,7 @@
 #include "reform_batch.h"
 #include "praq_host.hpp"
 #include "pria_host.hpp"
+#include "tune.hpp"
 #include "wide.hpp"
 
 #include <algorithm>
@@ -190,12 +191,17 @@ struct BatchImpl {
     int praq_result(int64_t q, double* x, double* y, PraqInfo* info);
     int pria_result(int64_t q, double* x, double* y, double* z, pria_info* info);
 
-    // one plain PRAQ solve of launch slot s from job `job` (its precision)
+    // one plain PRAQ solve of launch slot s from job `job` (its precision),
+    // with the lanes of the instance's host build (`lanes`): the plain
+    // solver's lane rule, and the lanes its hub lists were built for, which
+    // the device tier's hub threshold must match (the job records carry the
+    // batched tiers' lanes)
     template <class Real>
     int praq_plain(const PraqJob<Real>& job, PraqJob<Real>* devslot, int tier, bool resume,
-                   double time_limit) {
+                   double time_limit, int lanes) {
+        PraqJob<Real> j = job;
+        j.lanes = lanes;
         if (tier == PRAQ_TIER_WORKGROUP) {
-            PraqJob<Real> j = job;
             j.resume = resume ? 1 : 0;
             REFORM_HIP_CHECK(hipMemcpyAsync(devslot, &j, sizeof(j), hipMemcpyHostToDevice, s0));
             if (praq_launch_wg<Real>(devslot, 1, 1024, s0)) return kErrorDevice;
@@ -204,7 +210,7 @@ struct BatchImpl {
         LogicState st;
         if (resume && hipMemcpy(&st, job.state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
             return kErrorDevice;
-        if (praq_device_solve<Real>(job, st, work, time_limit, s0, resume)) return kErrorDevice;
+        if (praq_device_solve<Real>(j, st, work, time_limit, s0, resume)) return kErrorDevice;
         REFORM_HIP_CHECK(hipMemcpyAsync(job.state, &st, sizeof(st), hipMemcpyHostToDevice, s0));
         return hipStreamSynchronize(s0) == hipSuccess ? 0 : kErrorDevice;
     }
@@ -378,7 +384,7 @@ int BatchImpl::upload() {
             job.a = praq_bind<Real>(L, base + off, e.host);
             job.p = e.lp;
             job.scale = e.host.scale;
-            job.lanes = e.host.lanes;
+            job.lanes = batch_praq_lanes(e.host.n, e.host.m);
             job.time_limit_ticks = ticks_for(e.o.time_limit);
             job.state = praq_states_dev + s;
             return 0;
@@ -671,16 +677,18 @@ int BatchImpl::solve_sequential() {
     };
     for (size_t q = 0; q < praq.size(); ++q) {
         const int64_t s = praq_slot[q];
-        const int tier = praq[q].tier;
+        const int tier = praq[q].tier, lanes = praq[q].host.lanes;
         const double limit = praq[q].o.time_limit;
         const clock::time_point t0 = clock::now();
         if (!f32) {
-            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, false, limit)) return kErrorDevice;
+            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, false, limit, lanes))
+                return kErrorDevice;
             praq_in64[q] = 1;
             praq_seconds[q] = since(t0);
             continue;
         }
-        if (praq_plain<float>(praq32.host[s], praq32.sub, tier, false, limit)) return kErrorDevice;
+        if (praq_plain<float>(praq32.host[s], praq32.sub, tier, false, limit, lanes))
+            return kErrorDevice;
         LogicState st;
         REFORM_HIP_CHECK(hipMemcpy(&st, praq_states_dev + s, sizeof(st), hipMemcpyDeviceToHost));
         praq_in64[q] = 0;
@@ -688,7 +696,8 @@ int BatchImpl::solve_sequential() {
             st.switch_iteration = st.iteration;
             REFORM_HIP_CHECK(hipMemcpy(praq_states_dev + s, &st, sizeof(st), hipMemcpyHostToDevice));
             if (praq_convert_state(praq32.host[s].a, praq64.host[s].a, s0)) return kErrorDevice;
-            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, true, limit)) return kErrorDevice;
+            if (praq_plain<double>(praq64.host[s], praq64.sub, tier, true, limit, lanes))
+                return kErrorDevice;
             praq_in64[q] = 1;
         }
         praq_seconds[q] = since(t0);

@@ -147,6 +147,9 @@ __device__ inline void pria_reset_thresholds(const PriaView<Real>& a, int k, int
     for (int e = k; e < a.T; e += stride) a.thz[e] = none;
 }
 
+// Rows of an x tile a thread loads as one batch (per G rows of the tile).
+constexpr int kPriaXBatch = 8;
+
 // --- x pass: one tile of TW consecutive columns t0.. of item i ------------
 //
 // lds: (T' x (TW + 1)) pre-projection values + TW thresholds.  With HOLD > 0
@@ -168,7 +171,7 @@ __device__ void pria_x_tile(const PriaView<Real>& a, const PriaStep<Real>& st, i
 
     // load: lanes over columns, G rows of s at a time (coalesced per s); U
     // rows' loads are issued before any is used, so their latencies overlap
-    constexpr int U = 8;
+    constexpr int U = kPriaXBatch;
     const int c = tid % TW, row = tid / TW, t = t0 + c;
     const bool colok = c < tcount;
     Real held[HOLD > 0 ? HOLD * U : 1];

@@ -18,6 +18,7 @@
 // so requests join and leave at round boundaries while the others keep
 // their Halpern state on the device.
 #include "reform_serve.h"
+#include "tune.hpp"
 #include "wide.hpp"
 
 #include <algorithm>
@@ -374,7 +375,7 @@ int reform_server::admit(Request& r) {
         r.pj64.a = praq_bind<double>(r.pl64, base64, r.ph);
         r.pj64.p = r.lp;
         r.pj64.scale = r.ph.scale;
-        r.pj64.lanes = r.ph.lanes;
+        r.pj64.lanes = batch_praq_lanes(r.ph.n, r.ph.m);
         r.pj64.state = state;
         if (f32) {
             r.pj32 = PraqJob<float>{};
@@ -382,7 +383,7 @@ int reform_server::admit(Request& r) {
             r.pj32.p = r.lp;
             apply_fp32_policy(r.pj32.p);
             r.pj32.scale = r.ph.scale;
-            r.pj32.lanes = r.ph.lanes;
+            r.pj32.lanes = r.pj64.lanes;
             r.pj32.state = state;
         }
     } else {

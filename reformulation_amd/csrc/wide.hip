@@ -2,6 +2,7 @@
 // batch, per-instance round control and precision, one round captured in a
 // hipGraph.
 #include "wide.hpp"
+#include "tune.hpp"
 
 #include <hip/hip_ext.h>
 
@@ -446,9 +447,12 @@ wpria_lists(const RJ64* j64, int count, const int* first, const int* xr,
         for (int l = 0; l < kLists; ++l) lcount[l] = scan[l][tid];
 }
 
-// Batches of x1 an FP32 x-tile thread holds for its write phase when every
-// T' <= 320 (256 threads, 8 rows per pass, 8 rows per batch: 320 rows)
-constexpr int kXHold32 = 5;
+// Batches of x1 an FP32 x-tile thread of a `bs`-thread workgroup holds for
+// its write phase when every T' <= 320 (bs / 32 rows per pass, kPriaXBatch
+// passes per batch): 5 with 256 threads, 3 with 512
+constexpr int x_hold_batches(int bs) {
+    return (kPriaCompactT + (bs / kWideTW) * kPriaXBatch - 1) / ((bs / kWideTW) * kPriaXBatch);
+}
 
 template <class Real>
 __device__ __forceinline__ const PriaJob<Real>& pria_job(const RJ32* j32, const RJ64* j64,
@@ -458,13 +462,14 @@ __device__ __forceinline__ const PriaJob<Real>& pria_job(const RJ32* j32, const 
 }
 
 // The x pass over the list of one precision, one x tile (32 columns of one
-// item) per item: FP32 in 256-thread blocks (8 rows of the tile at a time),
+// item) per item: FP32 in 256-thread blocks (8 rows of the tile at a time;
+// 512 threads and 16 rows with the tuning table's batch.pria.x_threads),
 // FP64 in 512 threads (16 rows at a time; 256-byte tile rows, so LDS
 // T' x 33 x 8 bytes: two blocks per CU).  TW = 16 is the FP64 fallback for
 // T' whose 32-column tile does not fit the LDS: the tile as two halves (the
 // same 16 lanes per column, so the same sums).
-template <class Real, int TW, int HOLD = 0>
-__global__ void __launch_bounds__(sizeof(Real) == 4 ? 256 : 512)
+template <class Real, int TW, int HOLD = 0, int BS = (sizeof(Real) == 4 ? 256 : 512)>
+__global__ void __launch_bounds__(BS)
 wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab, const int* list, const int* lcount,
         const int* round_action, int mode, int j) {
     extern __shared__ char dyn[];
@@ -1132,8 +1137,23 @@ int WidePraq::expire(hipStream_t stream, const int* mask) {
 
 // ------------------------------------------------------------- PRIA host --
 
+// The FP32 x kernel of the table's shape.
+const void* WidePria::x32_kernel() const {
+    if (x32_threads == 512)
+        return x32_hold ? reinterpret_cast<const void*>(
+                              wpria_x<float, kWideTW, x_hold_batches(512), 512>)
+                        : reinterpret_cast<const void*>(wpria_x<float, kWideTW, 0, 512>);
+    return x32_hold
+               ? reinterpret_cast<const void*>(wpria_x<float, kWideTW, x_hold_batches(256), 256>)
+               : reinterpret_cast<const void*>(wpria_x<float, kWideTW, 0, 256>);
+}
+
 int WidePria::setup(std::vector<PriaJob<double>> host64, const PriaJob<float>* dev32,
                     const PriaJob<double>* dev64) {
+    // the tuning table's shapes, fixed for the engine's life
+    x32_threads = pria_knob(kPriaXThreads) == 512 ? 512 : 256;
+    x32_hold_knob = pria_knob(kPriaXHold) != 2;
+    yz_compact_knob = pria_knob(kPriaYZShape) != 2;
     full64 = std::move(host64);
     full_dev32 = dev32;
     full_dev64 = dev64;
@@ -1243,9 +1263,9 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     lds64 = sizeof(double) * (size_t(max_T) * (TW + 1) + TW);
     x64_full = lds64 <= kPriaX64MaxLds;
     if (!x64_full) lds64 = sizeof(double) * (size_t(max_T) * (TW / 2 + 1) + TW / 2);
-    const void* x32 = max_T <= kPriaCompactT
-                          ? reinterpret_cast<const void*>(wpria_x<float, kWideTW, kXHold32>)
-                          : reinterpret_cast<const void*>(wpria_x<float, kWideTW>);
+    x32_hold = x32_hold_knob && max_T <= kPriaCompactT;
+    compact = yz_compact_knob && max_T <= kPriaCompactT;
+    const void* x32 = x32_kernel();
     const void* x64 = x64_full ? reinterpret_cast<const void*>(wpria_x<double, kWideTW>)
                                : reinterpret_cast<const void*>(wpria_x<double, kWideTW / 2>);
     if (lds32 > 65536)
@@ -1277,14 +1297,12 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
     // FP64 instances (FP64 batches, and FP32 ones after the handover)
     hipEvent_t b, e;
     if (mark_events(0, &b, &e)) return kErrorDevice;
-    if (has32 && max_T <= kPriaCompactT)
-        hipExtLaunchKernelGGL((wpria_x<float, kWideTW, kXHold32>), dim3(x_blocks), dim3(256),
-                              lds32, stream, b, nullptr, 0, jobs32, jobs64, x_tab, lx32,
-                              lcount + kListX32, round_action, mode, j);
-    else if (has32)
-        hipExtLaunchKernelGGL((wpria_x<float, kWideTW>), dim3(x_blocks), dim3(256), lds32, stream,
-                              b, nullptr, 0, jobs32, jobs64, x_tab, lx32, lcount + kListX32,
-                              round_action, mode, j);
+    if (has32) {
+        const int* lcount_x32 = lcount + kListX32;
+        void* args[] = {&jobs32, &jobs64, &x_tab, &lx32, &lcount_x32, &round_action, &mode, &j};
+        REFORM_HIP_CHECK(hipExtLaunchKernel(x32_kernel(), dim3(x_blocks), dim3(x32_threads), args,
+                                            lds32, stream, b, nullptr, 0));
+    }
     hipEvent_t b64 = has32 ? nullptr : b;
     if (x64_full)
         hipExtLaunchKernelGGL((wpria_x<double, kWideTW>), dim3(x_blocks), dim3(512), lds64,
@@ -1295,7 +1313,6 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
                               stream, b64, e, 0, jobs32, jobs64, x_tab, lx64, lcount + kListX64,
                               round_action, mode, j);
     if (mark_events(1, &b, &e)) return kErrorDevice;
-    const bool compact = max_T <= kPriaCompactT;
     auto yz = [&](auto real, const int* list, int l, hipEvent_t start, hipEvent_t stop) {
         using Real = decltype(real);
         if (compact)

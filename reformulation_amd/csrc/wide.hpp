@@ -245,6 +245,12 @@ struct WidePria : WideControl {
     int x_blocks = 0, s_blocks = 0;
     size_t lds32 = 0, lds64 = 0;        // dynamic LDS of the x kernels
     bool x64_full = true;               // FP64 x tiles of 32 columns (else two halves)
+    // shapes from the tuning table (read by setup): FP32 x-tile threads,
+    // whether FP32 x tiles may hold x1 in registers, whether the y/z pass may
+    // take the compact register shape; and what the tables allow of them
+    int x32_threads = 256;
+    bool x32_hold_knob = true, yz_compact_knob = true;
+    bool x32_hold = false, compact = false;
     // work lists of the round (x tiles FP32, FP64, slices FP32, FP64) and
     // their lengths
     int* lists = nullptr;
@@ -259,6 +265,7 @@ struct WidePria : WideControl {
     int setup(std::vector<PriaJob<double>> host64, const PriaJob<float>* dev32,
               const PriaJob<double>* dev64);
     int step(hipStream_t stream, int mode, int j);
+    const void* x32_kernel() const;
     // the work lists of mode 0 (steps of the round) or 1 (restart steps)
     int build_lists(hipStream_t stream, int mode);
     int start(hipStream_t stream) override;

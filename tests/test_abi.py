@@ -287,6 +287,38 @@ def test_tuning_table_round_trip_and_lane_rule_without_a_gpu(tmp_path):
     autotune.load()                                  # the package's own table, if any
 
 
+def test_batched_tuning_keys_without_a_gpu():
+    """The batched tiers' keys: lanes per bucket read by the batch rule
+    (the built-in one while unset, the device-tier entries not consulted),
+    the PRIA knobs accept exactly their documented values."""
+    from reformulation_amd import _native, autotune
+    L = _native.lib()
+    autotune._declare(L)
+    saved = autotune.table_in_use(L)
+    try:
+        L.reform_tune_clear()
+        assert autotune.batch_lanes_for(1000, 16000) == 4          # rule, mean degree 32
+        assert L.reform_tune_set(b"praq.lanes.5", 64) == 0         # device tier only
+        assert autotune.batch_lanes_for(1000, 16000) == 4
+        assert L.reform_tune_set(b"batch.praq.lanes.5", 8) == 0
+        assert autotune.batch_lanes_for(1000, 16000) == 8
+        assert autotune.batch_lanes_for(1000, 15000) == 4          # bucket 4, unset
+        for key, values in autotune.PRIA_KNOBS.items():
+            for v in values + (0,):
+                assert L.reform_tune_set(key.encode(), v) == 0
+                assert L.reform_tune_get(key.encode()) == v
+        for bad in ((b"batch.pria.x_threads", 128), (b"batch.pria.x_hold", 3),
+                    (b"batch.pria.yz_shape", -1), (b"batch.praq.lanes.9", 4),
+                    (b"batch.praq.lanes.3", 6), (b"batch.pria", 1)):
+            assert L.reform_tune_set(*bad) == -1
+        used = autotune.table_in_use(L)
+        assert used["batch.praq.lanes.5"] == 8 and used["praq.lanes.5"] == 64
+        assert used["batch.pria.x_threads"] == 0
+    finally:
+        L.reform_tune_clear()
+        autotune.apply({k: v for k, v in saved.items() if v}, L)
+
+
 def test_ctypes_records_match_the_c_layouts(tmp_path):
     """Every record the Python side hands to the library has the size and
     field offsets of its C struct (compiled from include/*.h with gcc): a
