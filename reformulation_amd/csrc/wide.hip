@@ -392,20 +392,39 @@ using RJ64 = PriaJob<double>;
 // so the blocks with work are dispatched first and the rest exit after one
 // load: no block reads the job record of an instance of the other precision
 // or of one that takes no step.  Which block takes an item does not change
-// the item's arithmetic.
+// the item's arithmetic.  The lists hold the table entries themselves, and
+// the step scalars of each listed instance sit in a RoundStep record, so a
+// step block's first loads are its entry, then its job record and
+// RoundStep side by side, then its data.
 enum { kListX32 = 0, kListX64 = 1, kListS32 = 2, kListS64 = 3, kLists = 4 };
 
 constexpr int kListBlock = 1024;
 
+// Step j's scalars (wide_step_for's, from the RoundStep), or false when the
+// instance takes no step j.
+template <class Real>
+__device__ __forceinline__ bool round_step(const RoundStep& rs, int j, PriaStep<Real>* out) {
+    if (j >= rs.steps) return false;
+    if (rs.act == kActRestart) {
+        *out = pria_step_scalars<Real>(0, rs.omega, rs.eta, false, true);
+        return true;
+    }
+    const bool reflect = rs.act == kActHalpernBlock;
+    *out = pria_step_scalars<Real>(rs.r0 + j, rs.omega, rs.eta, reflect,
+                                   !reflect || j + 1 == rs.steps);
+    return true;
+}
+
 // One block: the instances taking steps in this round (mode 0) or taking a
-// restart step (mode 1), in table order; per instance its x-table entries
-// [xr[2q], +xr[2q+1]) and slices [first[4q], +first[4q+1]), in the list of
-// its precision.  The lengths go to lcount[0..3].  Every instance's x range
-// is a whole number of groups of kXcds entries, so each range starts in
-// its list at a multiple of kXcds.
+// restart step (mode 1), in table order: per instance its RoundStep and the
+// offsets at[3q..3q+2] = (x offset, slice offset, precision) of its ranges in
+// the lists of its precision (-1: no steps); the lengths go to
+// lcount[0..3].  wpria_fill_lists then copies the entries.  Every instance's
+// x range is a whole number of groups of kXcds entries, so each range starts
+// in its list at a multiple of kXcds.
 __global__ void __launch_bounds__(kListBlock)
 wpria_lists(const RJ64* j64, int count, const int* first, const int* xr,
-            const int* round_action, int mode, int* lists, int x_cap, int s_cap, int* lcount) {
+            const int* round_action, int mode, RoundStep* rstep, int* at_out, int* lcount) {
     __shared__ int scan[kLists][kListBlock];
     const int tid = threadIdx.x;
     // a contiguous chunk of instances per thread keeps the lists in table order
@@ -414,11 +433,19 @@ wpria_lists(const RJ64* j64, int count, const int* first, const int* xr,
     int mine[kLists] = {0, 0, 0, 0};
     for (int q = q0; q < q1; ++q) {
         const LogicState* st = j64[q].state;
-        const bool runs = mode == 0 ? wide_runs_steps(round_action[q]) : wide_restarting(st);
+        const int act = mode == 0 ? round_action[q] : int(kActRestart);
+        const bool runs = mode == 0 ? wide_runs_steps(act) : wide_restarting(st);
         if (!runs) continue;
         const int p = st->in64 ? 1 : 0;
         mine[kListX32 + p] += xr[2 * q + 1];
         mine[kListS32 + p] += first[4 * q + 1];
+        RoundStep rs;
+        rs.r0 = mode == 0 ? st->restart_iter : 0;
+        rs.omega = st->omega;
+        rs.eta = j64[q].p.eta;
+        rs.act = act;
+        rs.steps = mode == 0 && act != kActInit ? int(j64[q].p.op_steps) : 1;
+        rstep[q] = rs;
     }
     for (int l = 0; l < kLists; ++l) scan[l][tid] = mine[l];
     __syncthreads();
@@ -432,19 +459,39 @@ wpria_lists(const RJ64* j64, int count, const int* first, const int* xr,
     }
     int at[kLists];
     for (int l = 0; l < kLists; ++l) at[l] = scan[l][tid] - mine[l];
-    int* out[kLists] = {lists, lists + x_cap, lists + 2 * x_cap, lists + 2 * x_cap + s_cap};
     for (int q = q0; q < q1; ++q) {
         const LogicState* st = j64[q].state;
         const bool runs = mode == 0 ? wide_runs_steps(round_action[q]) : wide_restarting(st);
-        if (!runs) continue;
+        if (!runs) {
+            at_out[3 * q] = -1;
+            continue;
+        }
         const int p = st->in64 ? 1 : 0;
-        int* lx = out[kListX32 + p];
-        int* ls = out[kListS32 + p];
-        for (int k = 0; k < xr[2 * q + 1]; ++k) lx[at[kListX32 + p]++] = xr[2 * q] + k;
-        for (int k = 0; k < first[4 * q + 1]; ++k) ls[at[kListS32 + p]++] = first[4 * q] + k;
+        at_out[3 * q] = at[kListX32 + p];
+        at_out[3 * q + 1] = at[kListS32 + p];
+        at_out[3 * q + 2] = p;
+        at[kListX32 + p] += xr[2 * q + 1];
+        at[kListS32 + p] += first[4 * q + 1];
     }
     if (tid == kListBlock - 1)
         for (int l = 0; l < kLists; ++l) lcount[l] = scan[l][tid];
+}
+
+// One block per instance: its table entries into the lists at the offsets
+// wpria_lists left.
+__global__ void __launch_bounds__(256)
+wpria_fill_lists(const int* first, const int* xr, const int* at, const int4* x_tab,
+                 const int2* s_tab, int4* xlists, int2* slists, int x_cap, int s_cap) {
+    const int q = blockIdx.x;
+    const int ax = at[3 * q];
+    if (ax < 0) return;
+    const int as = at[3 * q + 1], p = at[3 * q + 2];
+    int4* lx = xlists + size_t(p) * x_cap + ax;
+    int2* ls = slists + size_t(p) * s_cap + as;
+    const int x0 = xr[2 * q], nx = xr[2 * q + 1];
+    const int s0 = first[4 * q], ns = first[4 * q + 1];
+    for (int k = threadIdx.x; k < nx; k += blockDim.x) lx[k] = x_tab[x0 + k];
+    for (int k = threadIdx.x; k < ns; k += blockDim.x) ls[k] = s_tab[s0 + k];
 }
 
 // Batches of x1 an FP32 x-tile thread of a `bs`-thread workgroup holds for
@@ -470,15 +517,15 @@ __device__ __forceinline__ const PriaJob<Real>& pria_job(const RJ32* j32, const 
 // same 16 lanes per column, so the same sums).
 template <class Real, int TW, int HOLD = 0, int BS = (sizeof(Real) == 4 ? 256 : 512)>
 __global__ void __launch_bounds__(BS)
-wpria_x(const RJ32* j32, const RJ64* j64, const int4* tab, const int* list, const int* lcount,
-        const int* round_action, int mode, int j) {
+wpria_x(const RJ32* j32, const RJ64* j64, const int4* list, const int* lcount,
+        const RoundStep* rstep, int j) {
     extern __shared__ char dyn[];
-    if (int(blockIdx.x) >= *lcount) return;
-    const int4 t = tab[list[blockIdx.x]];
-    if (t.y < 0) return;   // the padding of an item group
+    const int n = *lcount;
+    const int4 t = list[blockIdx.x];   // the grid is the list's capacity
+    if (int(blockIdx.x) >= n || t.y < 0) return;   // past the list; item-group padding
     const PriaJob<Real>& job = pria_job<Real>(j32, j64, t.x);
     PriaStep<Real> sc;
-    if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+    if (!round_step<Real>(rstep[t.x], j, &sc)) return;
     const PriaView<Real> a = pria_view(job.a);
     Real* lds = reinterpret_cast<Real*>(dyn);
     pria_x_tile<Real, TW, HOLD>(a, sc, t.y, t.z, lds);
@@ -499,15 +546,16 @@ struct PriaWideShape {
 // length only).
 template <class Real, bool compact>
 __global__ void __launch_bounds__(512, compact ? 6 : 3)
-wpria_yz(const RJ32* j32, const RJ64* j64, const int2* tab, const int* list, const int* lcount,
-         const int* round_action, int mode, int j) {
+wpria_yz(const RJ32* j32, const RJ64* j64, const int2* list, const int* lcount,
+         const RoundStep* rstep, int j) {
     __shared__ double red[kPriaYZRed];
     extern __shared__ double zcache[];   // kPriaZCacheBytes: z + r of a long slice
-    if (int(blockIdx.x) >= *lcount) return;
-    const int2 t = tab[list[blockIdx.x]];
+    const int n = *lcount;
+    const int2 t = list[blockIdx.x];   // the grid is the list's capacity
+    if (int(blockIdx.x) >= n) return;
     const PriaJob<Real>& job = pria_job<Real>(j32, j64, t.x);
     PriaStep<Real> sc;
-    if (!wide_step_for(job, round_action, t.x, mode, j, pria_step_scalars<Real>, &sc)) return;
+    if (!round_step<Real>(rstep[t.x], j, &sc)) return;
     using Shape = PriaWideShape<Real, compact>;
     pria_yz_slice<Real, Shape::VR, Shape::VZ>(pria_view(job.a), sc, t.y, red, 3, 0, 1 << 30,
                                              reinterpret_cast<Real*>(zcache),
@@ -1239,20 +1287,31 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     }
     x_blocks = int(xt.size());
     s_blocks = int(st.size());
-    // arena: x table, slice table, the four work lists, their lengths and
-    // the instances' x ranges
+    // arena: x table, slice table, the x lists (two of x_blocks entries),
+    // the slice lists (two of s_blocks), their lengths, the instances'
+    // RoundStep records, list offsets and x ranges
     const size_t x_bytes = align256(sizeof(int4) * xt.size());
     const size_t s_bytes = align256(sizeof(int2) * st.size());
-    const size_t l_bytes = align256(sizeof(int) * (2 * size_t(x_blocks) + 2 * size_t(s_blocks)));
+    const size_t lx_bytes = align256(sizeof(int4) * 2 * size_t(x_blocks));
+    const size_t ls_bytes = align256(sizeof(int2) * 2 * size_t(s_blocks));
     const size_t c_bytes = align256(sizeof(int) * kLists);
-    const size_t tab_bytes = x_bytes + s_bytes + l_bytes + c_bytes + sizeof(int) * xr.size();
+    const size_t r_bytes = align256(sizeof(RoundStep) * size_t(count));
+    const size_t a_bytes = align256(sizeof(int) * 3 * size_t(count));
+    size_t o = 0;
+    const size_t o_x = o, o_s = (o += x_bytes), o_lx = (o += s_bytes), o_ls = (o += lx_bytes),
+                 o_c = (o += ls_bytes), o_r = (o += c_bytes), o_a = (o += r_bytes),
+                 o_xr = (o += a_bytes);
+    const size_t tab_bytes = o_xr + sizeof(int) * xr.size();
     if (allocate(tab_bytes, xt.size() + st.size(), kPriaSums)) return kErrorDevice;
     char* ab = static_cast<char*>(arena);
-    x_tab = reinterpret_cast<int4*>(ab);
-    s_tab = reinterpret_cast<int2*>(ab + x_bytes);
-    lists = reinterpret_cast<int*>(ab + x_bytes + s_bytes);
-    lcount = reinterpret_cast<int*>(ab + x_bytes + s_bytes + l_bytes);
-    x_range = reinterpret_cast<int*>(ab + x_bytes + s_bytes + l_bytes + c_bytes);
+    x_tab = reinterpret_cast<int4*>(ab + o_x);
+    s_tab = reinterpret_cast<int2*>(ab + o_s);
+    xlists = reinterpret_cast<int4*>(ab + o_lx);
+    slists = reinterpret_cast<int2*>(ab + o_ls);
+    lcount = reinterpret_cast<int*>(ab + o_c);
+    rstep = reinterpret_cast<RoundStep*>(ab + o_r);
+    list_at = reinterpret_cast<int*>(ab + o_a);
+    x_range = reinterpret_cast<int*>(ab + o_xr);
     REFORM_HIP_CHECK(hipMemcpy(x_range, xr.data(), sizeof(int) * xr.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(x_tab, xt.data(), sizeof(int4) * xt.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(s_tab, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice));
@@ -1279,7 +1338,9 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
 
 int WidePria::build_lists(hipStream_t stream, int mode) {
     hipLaunchKernelGGL(wpria_lists, dim3(1), dim3(kListBlock), 0, stream, jobs64, count, first,
-                       x_range, round_action, mode, lists, x_blocks, s_blocks, lcount);
+                       x_range, round_action, mode, rstep, list_at, lcount);
+    hipLaunchKernelGGL(wpria_fill_lists, dim3(count), dim3(256), 0, stream, first, x_range,
+                       list_at, x_tab, s_tab, xlists, slists, x_blocks, s_blocks);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -1289,40 +1350,41 @@ int WidePria::build_lists(hipStream_t stream, int mode) {
 // each pair bracketed by timing events (the first kernel's start, the last
 // kernel's stop).
 int WidePria::step(hipStream_t stream, int mode, int j) {
-    const int* lx32 = lists;
-    const int* lx64 = lists + x_blocks;
-    const int* ls32 = lists + 2 * x_blocks;
-    const int* ls64 = lists + 2 * x_blocks + s_blocks;
+    const int4* lx32 = xlists;
+    const int4* lx64 = xlists + x_blocks;
+    const int2* ls32 = slists;
+    const int2* ls64 = slists + s_blocks;
     // FP32 launches exist for batches with FP32 images; every batch can hold
     // FP64 instances (FP64 batches, and FP32 ones after the handover)
     hipEvent_t b, e;
     if (mark_events(0, &b, &e)) return kErrorDevice;
     if (has32) {
         const int* lcount_x32 = lcount + kListX32;
-        void* args[] = {&jobs32, &jobs64, &x_tab, &lx32, &lcount_x32, &round_action, &mode, &j};
+        const RoundStep* rs = rstep;
+        void* args[] = {&jobs32, &jobs64, &lx32, &lcount_x32, &rs, &j};
         REFORM_HIP_CHECK(hipExtLaunchKernel(x32_kernel(), dim3(x_blocks), dim3(x32_threads), args,
                                             lds32, stream, b, nullptr, 0));
     }
     hipEvent_t b64 = has32 ? nullptr : b;
     if (x64_full)
         hipExtLaunchKernelGGL((wpria_x<double, kWideTW>), dim3(x_blocks), dim3(512), lds64,
-                              stream, b64, e, 0, jobs32, jobs64, x_tab, lx64, lcount + kListX64,
-                              round_action, mode, j);
+                              stream, b64, e, 0, jobs32, jobs64, lx64, lcount + kListX64,
+                              rstep, j);
     else
         hipExtLaunchKernelGGL((wpria_x<double, kWideTW / 2>), dim3(x_blocks), dim3(512), lds64,
-                              stream, b64, e, 0, jobs32, jobs64, x_tab, lx64, lcount + kListX64,
-                              round_action, mode, j);
+                              stream, b64, e, 0, jobs32, jobs64, lx64, lcount + kListX64,
+                              rstep, j);
     if (mark_events(1, &b, &e)) return kErrorDevice;
-    auto yz = [&](auto real, const int* list, int l, hipEvent_t start, hipEvent_t stop) {
+    auto yz = [&](auto real, const int2* list, int l, hipEvent_t start, hipEvent_t stop) {
         using Real = decltype(real);
         if (compact)
             hipExtLaunchKernelGGL((wpria_yz<Real, true>), dim3(s_blocks), dim3(512),
-                                  kPriaZCacheBytes, stream, start, stop, 0, jobs32, jobs64, s_tab,
-                                  list, lcount + l, round_action, mode, j);
+                                  kPriaZCacheBytes, stream, start, stop, 0, jobs32, jobs64, list,
+                                  lcount + l, rstep, j);
         else
             hipExtLaunchKernelGGL((wpria_yz<Real, false>), dim3(s_blocks), dim3(512),
-                                  kPriaZCacheBytes, stream, start, stop, 0, jobs32, jobs64, s_tab,
-                                  list, lcount + l, round_action, mode, j);
+                                  kPriaZCacheBytes, stream, start, stop, 0, jobs32, jobs64, list,
+                                  lcount + l, rstep, j);
     };
     if (has32) yz(float(0), ls32, kListS32, b, nullptr);
     yz(double(0), ls64, kListS64, has32 ? nullptr : b, e);

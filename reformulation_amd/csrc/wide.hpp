@@ -236,6 +236,16 @@ struct WidePraq : WideControl {
 
 // PRIA: first = (slice range, column-tile range) per instance; partial rows
 // (kPriaSums wide) per slice, then one per column tile in (item, tile) order.
+// What a PRIA step block needs of its instance besides the job record's
+// arrays, for the steps a work-list build enqueues: mode 0 the round's
+// steps, mode 1 the restart step.
+struct RoundStep {
+    int64_t r0;       // restart counter at the round's first step
+    double omega, eta;
+    int act;          // the round's action (mode 0), kActRestart (mode 1)
+    int steps;        // steps taken: op_steps, 1 for kActInit and a restart
+};
+
 struct WidePria : WideControl {
     int max_T = 2;
     const PriaJob<float>* jobs32 = nullptr;
@@ -251,10 +261,14 @@ struct WidePria : WideControl {
     int x32_threads = 256;
     bool x32_hold_knob = true, yz_compact_knob = true;
     bool x32_hold = false, compact = false;
-    // work lists of the round (x tiles FP32, FP64, slices FP32, FP64) and
-    // their lengths
-    int* lists = nullptr;
+    // work lists of the round: table entries of x tiles (FP32, then FP64)
+    // and of slices (FP32, then FP64), their lengths, the listed instances'
+    // step scalars and their ranges' list offsets
+    int4* xlists = nullptr;
+    int2* slists = nullptr;
     int* lcount = nullptr;
+    RoundStep* rstep = nullptr;
+    int* list_at = nullptr;
     int* x_range = nullptr;             // [2 count]: (first x-table entry, entries)
 
     std::vector<PriaJob<double>> full64;
