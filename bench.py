@@ -24,8 +24,12 @@ m(8+7b); PRIA x 7b ND, y/z 8b ND per step, b = 4 for FP32) over their summed
 durations, each launch bracketed by HIP events on its stream in one more
 solve of the same batch after the timed steps, with `traffic` the measured
 HBM bytes per launch of that kernel (two rocprofv3 counter passes over this
-workload, calibrated on a device copy, kept in profiles/); and the oracle (FP64 C restatement, OpenMP) timed on a bounded sample
-of the same mixture on the host's cores.
+workload, calibrated on a device copy, kept in profiles/); the oracle (FP64 C restatement, OpenMP) timed on a bounded sample
+of the same mixture on the host's cores; and the two single-instance
+configurations through the plain solver (PRAQ Erdos-Renyi n = 2000,
+m = 8000; PRIA N = 50, T = 200; FP64 to 1e-6), GPU and oracle medians
+(`single_instance`).  `--precision` prints the FP32-against-FP64 study on
+the large Chung-Lu PRAQ instance instead.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--count 1024]
     python bench.py --serve-only --rates 50,200,1000 --serve-seconds 120
@@ -200,6 +204,8 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the serving runs (one JSON line per rate)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the two single-instance plain configurations")
     ap.add_argument("--precision", action="store_true",
                     help="only the precision study: FP32 against FP64 iterates on the "
                          "Chung-Lu graph of 1e6 nodes (one JSON line)")
@@ -303,6 +309,7 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(members, args.tolerance, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
+    single = None if args.no_single else single_instance(not args.no_cpu)
 
     ms = 1000.0 * total / args.steps
     line = dict(
@@ -338,10 +345,36 @@ def main() -> int:
                       launches=K["launches"], avg_launch_us=K["ms"] / K["launches"] * 1e3,
                       bytes_per_launch=K["bytes"] / K["launches"]),
         cpu_baseline=cpu,
+        single_instance=single,
         tuning=tuning_in_use(),
     )
     print(json.dumps(line), flush=True)
     return 0
+
+
+def single_instance(with_cpu: bool) -> dict:
+    """The two single-instance configurations through the plain solver's C
+    ABI (host build, upload, solve, download per call), FP64 iterates to
+    1e-6: PRAQ Erdos-Renyi n = 2000, m = 8000 (seeds 0..4) and PRIA
+    N = 50, T = 200 (seeds 0, 1); medians, with the oracle on the host's
+    cores beside them (its iteration counts must agree)."""
+    from measure import plain
+    out = {}
+    for key, workload, rows in (
+            ("plain_praq_er2000", "PRAQ Erdos-Renyi n=2000 m=8000, FP64, 1e-6",
+             plain._praq(5, with_cpu)),
+            ("plain_pria_n50t200", "PRIA N=50 T=200, FP64, 1e-6", plain._pria(2, with_cpu))):
+        rec = dict(workload=workload, instances=len(rows),
+                   gpu_ms=1e3 * float(np.median([r["seconds"] for r in rows])),
+                   iterations=[r["iterations"] for r in rows])
+        if with_cpu:
+            rec.update(cpu_ms=1e3 * float(np.median([r["cpu_seconds"] for r in rows])),
+                       cpu_cores=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                       iterations_equal=all(r["iterations"] == r["cpu_iterations"] for r in rows),
+                       max_upper_diff=max(r["upper_diff"] for r in rows))
+        out[key] = rec
+        log(f"{key}: {rec}")
+    return out
 
 
 def tuning_in_use() -> dict:
