@@ -204,7 +204,9 @@ __global__ void wide_expire(const Job* jobs, int count, const int* mask) {
 template <class Job>
 __global__ void wide_live(const Job* jobs, int count, int* live) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < count) live[q] = jobs[q].state->action != kActDoneX;
+    if (q >= count) return;
+    const LogicState* st = jobs[q].state;
+    live[q] = st->action == kActDoneX ? 0 : (st->in64 ? 2 : 1);
 }
 
 // One thread per subset position: the job record of full-table position keep[q].
@@ -857,8 +859,11 @@ int WideControl::shrink_to_live(hipStream_t stream) {
                                     hipMemcpyDeviceToHost, stream));
     REFORM_HIP_CHECK(hipStreamSynchronize(stream));
     std::vector<int> keep;
-    for (int q = 0; q < count; ++q)
+    keep_has32 = false;
+    for (int q = 0; q < count; ++q) {
         if (flags[q]) keep.push_back(table_index.empty() ? q : table_index[q]);
+        keep_has32 = keep_has32 || flags[q] == 1;
+    }
     if (keep.empty() || int(keep.size()) == count) return 0;
     return shrink(stream, keep);
 }
@@ -1215,7 +1220,11 @@ int WidePria::shrink(hipStream_t stream, const std::vector<int>& keep) {
     if (gather_subset(subset, full64, full_dev32, full_dev64, keep, stream, host64))
         return kErrorDevice;
     table_index = keep;
-    return build_tables(host64, full_dev32 ? subset.jobs32 : nullptr, subset.jobs64);
+    if (build_tables(host64, full_dev32 ? subset.jobs32 : nullptr, subset.jobs64)) return kErrorDevice;
+    // instances only ever go from FP32 to FP64: a subset with none left in
+    // FP32 needs no FP32 step launches
+    step32 = has32 && keep_has32;
+    return 0;
 }
 
 int WidePria::restore() {
@@ -1237,6 +1246,7 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     jobs32 = dev32;
     jobs64 = dev64;
     has32 = dev32 != nullptr;
+    step32 = has32;
     if (!count) {
         release();
         return 0;
@@ -1354,18 +1364,20 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
     const int4* lx64 = xlists + x_blocks;
     const int2* ls32 = slists;
     const int2* ls64 = slists + s_blocks;
-    // FP32 launches exist for batches with FP32 images; every batch can hold
-    // FP64 instances (FP64 batches, and FP32 ones after the handover)
+    // FP32 launches while the tables may hold an instance in FP32 (batches
+    // with FP32 images, until a compaction finds every instance in FP64);
+    // every batch can hold FP64 instances (FP64 batches, and FP32 ones
+    // after the handover)
     hipEvent_t b, e;
     if (mark_events(0, &b, &e)) return kErrorDevice;
-    if (has32) {
+    if (step32) {
         const int* lcount_x32 = lcount + kListX32;
         const RoundStep* rs = rstep;
         void* args[] = {&jobs32, &jobs64, &lx32, &lcount_x32, &rs, &j};
         REFORM_HIP_CHECK(hipExtLaunchKernel(x32_kernel(), dim3(x_blocks), dim3(x32_threads), args,
                                             lds32, stream, b, nullptr, 0));
     }
-    hipEvent_t b64 = has32 ? nullptr : b;
+    hipEvent_t b64 = step32 ? nullptr : b;
     if (x64_full)
         hipExtLaunchKernelGGL((wpria_x<double, kWideTW>), dim3(x_blocks), dim3(512), lds64,
                               stream, b64, e, 0, jobs32, jobs64, lx64, lcount + kListX64,
@@ -1386,8 +1398,8 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
                                   kPriaZCacheBytes, stream, start, stop, 0, jobs32, jobs64, list,
                                   lcount + l, rstep, j);
     };
-    if (has32) yz(float(0), ls32, kListS32, b, nullptr);
-    yz(double(0), ls64, kListS64, has32 ? nullptr : b, e);
+    if (step32) yz(float(0), ls32, kListS32, b, nullptr);
+    yz(double(0), ls64, kListS64, step32 ? nullptr : b, e);
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -84,6 +84,7 @@ struct WideControl {
     void* arena = nullptr;
     size_t bytes = 0;
     bool has32 = false;            // FP32 images exist (FP32 iterates first)
+    bool keep_has32 = true;        // the last shrink kept an instance still in FP32
     bool use_graph = true;         // replay one captured round (else launch directly)
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -261,6 +262,7 @@ struct WidePria : WideControl {
     int x32_threads = 256;
     bool x32_hold_knob = true, yz_compact_knob = true;
     bool x32_hold = false, compact = false;
+    bool step32 = false;   // FP32 step launches: FP32 images, some instance still in FP32
     // work lists of the round: table entries of x tiles (FP32, then FP64)
     // and of slices (FP32, then FP64), their lengths, the listed instances'
     // step scalars and their ranges' list offsets
