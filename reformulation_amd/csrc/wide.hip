@@ -231,19 +231,59 @@ __global__ void wide_count(const Job* jobs, int count, int* active) {
     if (threadIdx.x == 0) *active = int(n);
 }
 
+// Step j's scalars (wide_step_for's, from a RoundStep the list build wrote),
+// or false when the instance takes no step j.
+template <class Step, class MakeStep>
+__device__ __forceinline__ bool round_step(const RoundStep& rs, int j, MakeStep make, Step* out) {
+    if (j >= rs.steps) return false;
+    if (rs.act == kActRestart) {
+        *out = make(0, rs.omega, rs.eta, false, true);
+        return true;
+    }
+    const bool reflect = rs.act == kActHalpernBlock;
+    *out = make(rs.r0 + j, rs.omega, rs.eta, reflect, !reflect || j + 1 == rs.steps);
+    return true;
+}
+
+// The RoundStep of an instance that takes steps in a round (mode 0: action
+// `act`) or a restart step (mode 1).
+template <class Job>
+__device__ __forceinline__ RoundStep round_step_of(const Job& job, const LogicState* st, int act,
+                                                  int mode) {
+    RoundStep rs;
+    rs.r0 = mode == 0 ? st->restart_iter : 0;
+    rs.omega = st->omega;
+    rs.eta = job.p.eta;
+    rs.act = mode == 0 ? act : int(kActRestart);
+    rs.steps = mode == 0 && act != kActInit ? int(job.p.op_steps) : 1;
+    return rs;
+}
+
 // ---------------------------------------------------------------- PRAQ ----
 
 using PJ32 = PraqJob<float>;
 using PJ64 = PraqJob<double>;
 
+// The step kernels run over the round's work lists (wpraq_lists): entries
+// (instance, local block, precision) of the instances taking steps; block b
+// takes entry b, blocks past the list's length exit.
+template <class F>
+__device__ __forceinline__ void praq_listed(const PJ32* j32, const PJ64* j64, const int4& t,
+                                            F&& f) {
+    if (t.z) f(j64[t.x]);
+    else f(j32[t.x]);
+}
+
 __global__ void __launch_bounds__(256) wpraq_node(const PJ32* j32, const PJ64* j64,
-                                                  const int2* tab, const int* round_action,
-                                                  int mode, int j) {
-    const int2 t = tab[blockIdx.x];
-    dispatch(j32, j64, t.x, [&](const auto& job) {
+                                                  const int4* list, const int* lcount,
+                                                  const RoundStep* rstep, int j) {
+    const int n = *lcount;
+    const int4 t = list[blockIdx.x];   // the grid is the list's capacity
+    if (int(blockIdx.x) >= n) return;
+    praq_listed(j32, j64, t, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         PraqStep<Real> sc;
-        if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
+        if (!round_step(rstep[t.x], j, praq_step_scalars<Real>, &sc)) return;
         const int lanes = job.lanes;
         const int gid = t.y * kWideBlock + threadIdx.x;
         praq_node(praq_view(job.a), sc, gid / lanes, gid % lanes, lanes);
@@ -251,16 +291,84 @@ __global__ void __launch_bounds__(256) wpraq_node(const PJ32* j32, const PJ64* j
 }
 
 __global__ void __launch_bounds__(256) wpraq_edge(const PJ32* j32, const PJ64* j64,
-                                                  const int2* tab, const int* round_action,
-                                                  int mode, int j) {
-    const int2 t = tab[blockIdx.x];
-    dispatch(j32, j64, t.x, [&](const auto& job) {
+                                                  const int4* list, const int* lcount,
+                                                  const RoundStep* rstep, int j) {
+    const int n = *lcount;
+    const int4 t = list[blockIdx.x];
+    if (int(blockIdx.x) >= n) return;
+    praq_listed(j32, j64, t, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
         PraqStep<Real> sc;
-        if (!wide_step_for(job, round_action, t.x, mode, j, praq_step_scalars<Real>, &sc)) return;
+        if (!round_step(rstep[t.x], j, praq_step_scalars<Real>, &sc)) return;
         const int e = t.y * kWideBlock + threadIdx.x;
         if (e < job.a.m) praq_edge(praq_view(job.a), sc, e);
     });
+}
+
+constexpr int kPraqListBlock = 1024;
+
+// One block: the instances taking steps (mode 0) or a restart step (mode
+// 1), in table order: per instance its RoundStep and its node and edge
+// blocks' offsets in the two lists (at[3q] = -1: no steps), with its
+// precision at[3q + 2]; the lengths go to lcount[0..1] (node, edge).
+__global__ void __launch_bounds__(kPraqListBlock)
+wpraq_lists(const PJ64* j64, int count, const int* first, const int* round_action, int mode,
+            RoundStep* rstep, int* at_out, int* lcount) {
+    __shared__ int scan[2][kPraqListBlock];
+    const int tid = threadIdx.x;
+    const int chunk = (count + kPraqListBlock - 1) / kPraqListBlock;
+    const int q0 = min(count, tid * chunk), q1 = min(count, q0 + chunk);
+    int mine[2] = {0, 0};
+    for (int q = q0; q < q1; ++q) {
+        const LogicState* st = j64[q].state;
+        const int act = mode == 0 ? round_action[q] : int(kActRestart);
+        const bool runs = mode == 0 ? wide_runs_steps(act) : wide_restarting(st);
+        if (!runs) continue;
+        mine[0] += first[4 * q + 1];
+        mine[1] += first[4 * q + 3];
+        rstep[q] = round_step_of(j64[q], st, act, mode);
+    }
+    scan[0][tid] = mine[0];
+    scan[1][tid] = mine[1];
+    __syncthreads();
+    for (int off = 1; off < kPraqListBlock; off <<= 1) {
+        const int v0 = tid >= off ? scan[0][tid - off] : 0;
+        const int v1 = tid >= off ? scan[1][tid - off] : 0;
+        __syncthreads();
+        scan[0][tid] += v0;
+        scan[1][tid] += v1;
+        __syncthreads();
+    }
+    int at0 = scan[0][tid] - mine[0], at1 = scan[1][tid] - mine[1];
+    for (int q = q0; q < q1; ++q) {
+        const LogicState* st = j64[q].state;
+        const bool runs = mode == 0 ? wide_runs_steps(round_action[q]) : wide_restarting(st);
+        if (!runs) {
+            at_out[3 * q] = -1;
+            continue;
+        }
+        at_out[3 * q] = at0;
+        at_out[3 * q + 1] = at1;
+        at_out[3 * q + 2] = st->in64 ? 1 : 0;
+        at0 += first[4 * q + 1];
+        at1 += first[4 * q + 3];
+    }
+    if (tid == kPraqListBlock - 1) {
+        lcount[0] = scan[0][tid];
+        lcount[1] = scan[1][tid];
+    }
+}
+
+// One block per instance: its node and edge blocks into the lists.
+__global__ void __launch_bounds__(256)
+wpraq_fill_lists(const int* first, const int* at, int4* node_list, int4* edge_list) {
+    const int q = blockIdx.x;
+    const int an = at[3 * q];
+    if (an < 0) return;
+    const int ae = at[3 * q + 1], p = at[3 * q + 2];
+    const int nb = first[4 * q + 1], eb = first[4 * q + 3];
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) node_list[an + k] = make_int4(q, k, p, 0);
+    for (int k = threadIdx.x; k < eb; k += blockDim.x) edge_list[ae + k] = make_int4(q, k, p, 0);
 }
 
 // node blocks, then edge blocks; one partial row per block
@@ -402,21 +510,6 @@ enum { kListX32 = 0, kListX64 = 1, kListS32 = 2, kListS64 = 3, kLists = 4 };
 
 constexpr int kListBlock = 1024;
 
-// Step j's scalars (wide_step_for's, from the RoundStep), or false when the
-// instance takes no step j.
-template <class Real>
-__device__ __forceinline__ bool round_step(const RoundStep& rs, int j, PriaStep<Real>* out) {
-    if (j >= rs.steps) return false;
-    if (rs.act == kActRestart) {
-        *out = pria_step_scalars<Real>(0, rs.omega, rs.eta, false, true);
-        return true;
-    }
-    const bool reflect = rs.act == kActHalpernBlock;
-    *out = pria_step_scalars<Real>(rs.r0 + j, rs.omega, rs.eta, reflect,
-                                   !reflect || j + 1 == rs.steps);
-    return true;
-}
-
 // One block: the instances taking steps in this round (mode 0) or taking a
 // restart step (mode 1), in table order: per instance its RoundStep and the
 // offsets at[3q..3q+2] = (x offset, slice offset, precision) of its ranges in
@@ -441,13 +534,7 @@ wpria_lists(const RJ64* j64, int count, const int* first, const int* xr,
         const int p = st->in64 ? 1 : 0;
         mine[kListX32 + p] += xr[2 * q + 1];
         mine[kListS32 + p] += first[4 * q + 1];
-        RoundStep rs;
-        rs.r0 = mode == 0 ? st->restart_iter : 0;
-        rs.omega = st->omega;
-        rs.eta = j64[q].p.eta;
-        rs.act = act;
-        rs.steps = mode == 0 && act != kActInit ? int(j64[q].p.op_steps) : 1;
-        rstep[q] = rs;
+        rstep[q] = round_step_of(j64[q], st, act, mode);
     }
     for (int l = 0; l < kLists; ++l) scan[l][tid] = mine[l];
     __syncthreads();
@@ -527,7 +614,7 @@ wpria_x(const RJ32* j32, const RJ64* j64, const int4* list, const int* lcount,
     if (int(blockIdx.x) >= n || t.y < 0) return;   // past the list; item-group padding
     const PriaJob<Real>& job = pria_job<Real>(j32, j64, t.x);
     PriaStep<Real> sc;
-    if (!round_step<Real>(rstep[t.x], j, &sc)) return;
+    if (!round_step(rstep[t.x], j, pria_step_scalars<Real>, &sc)) return;
     const PriaView<Real> a = pria_view(job.a);
     Real* lds = reinterpret_cast<Real*>(dyn);
     pria_x_tile<Real, TW, HOLD>(a, sc, t.y, t.z, lds);
@@ -557,7 +644,7 @@ wpria_yz(const RJ32* j32, const RJ64* j64, const int2* list, const int* lcount,
     if (int(blockIdx.x) >= n) return;
     const PriaJob<Real>& job = pria_job<Real>(j32, j64, t.x);
     PriaStep<Real> sc;
-    if (!round_step<Real>(rstep[t.x], j, &sc)) return;
+    if (!round_step(rstep[t.x], j, pria_step_scalars<Real>, &sc)) return;
     using Shape = PriaWideShape<Real, compact>;
     pria_yz_slice<Real, Shape::VR, Shape::VZ>(pria_view(job.a), sc, t.y, red, 3, 0, 1 << 30,
                                              reinterpret_cast<Real*>(zcache),
@@ -1101,31 +1188,54 @@ int WidePraq::build_tables(const std::vector<PraqJob<double>>& host64,
     }
     node_blocks = int(nt.size());
     edge_blocks = int(et.size());
-    const size_t tab_bytes = sizeof(int2) * (nt.size() + et.size());
+    // arena: node and edge tables, the step lists (node, edge), their
+    // lengths, the RoundStep records and list offsets
+    const size_t t_bytes = align256(sizeof(int2) * (nt.size() + et.size()));
+    const size_t l_bytes = align256(sizeof(int4) * (nt.size() + et.size()));
+    const size_t c_bytes = align256(sizeof(int) * 2);
+    const size_t r_bytes = align256(sizeof(RoundStep) * size_t(count));
+    const size_t tab_bytes = t_bytes + l_bytes + c_bytes + r_bytes + sizeof(int) * 3 * size_t(count);
     if (allocate(tab_bytes, nt.size() + et.size(), kPraqSums)) return kErrorDevice;
-    node_tab = static_cast<int2*>(arena);
+    char* ab = static_cast<char*>(arena);
+    node_tab = reinterpret_cast<int2*>(ab);
     edge_tab = node_tab + nt.size();
+    node_list = reinterpret_cast<int4*>(ab + t_bytes);
+    edge_list = node_list + nt.size();
+    lcount = reinterpret_cast<int*>(ab + t_bytes + l_bytes);
+    rstep = reinterpret_cast<RoundStep*>(ab + t_bytes + l_bytes + c_bytes);
+    list_at = reinterpret_cast<int*>(ab + t_bytes + l_bytes + c_bytes + r_bytes);
     std::vector<int2> tabs(nt);
     tabs.insert(tabs.end(), et.begin(), et.end());
-    REFORM_HIP_CHECK(hipMemcpy(node_tab, tabs.data(), tab_bytes, hipMemcpyHostToDevice));
+    REFORM_HIP_CHECK(
+        hipMemcpy(node_tab, tabs.data(), sizeof(int2) * tabs.size(), hipMemcpyHostToDevice));
     REFORM_HIP_CHECK(hipMemcpy(first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice));
     return 0;
 }
 
-// One step of every instance the mode selects: the node kernel, then the
-// edge kernel, each bracketed by timing marks.
+int WidePraq::build_lists(hipStream_t stream, int mode) {
+    hipLaunchKernelGGL(wpraq_lists, dim3(1), dim3(kPraqListBlock), 0, stream, jobs64, count,
+                       first, round_action, mode, rstep, list_at, lcount);
+    hipLaunchKernelGGL(wpraq_fill_lists, dim3(count), dim3(256), 0, stream, first, list_at,
+                       node_list, edge_list);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// One step of every instance the mode selects (over the lists build_lists
+// made for the mode): the node kernel, then the edge kernel, each bracketed
+// by timing marks.
 int WidePraq::step(hipStream_t stream, int mode, int j) {
     const dim3 blk(kWideBlock);
     hipEvent_t b, e;
     if (node_blocks) {
         if (mark_events(0, &b, &e)) return kErrorDevice;
         hipExtLaunchKernelGGL(wpraq_node, dim3(node_blocks), blk, 0, stream, b, e, 0, jobs32,
-                              jobs64, node_tab, round_action, mode, j);
+                              jobs64, node_list, lcount, rstep, j);
     }
     if (edge_blocks) {
         if (mark_events(1, &b, &e)) return kErrorDevice;
         hipExtLaunchKernelGGL(wpraq_edge, dim3(edge_blocks), blk, 0, stream, b, e, 0, jobs32,
-                              jobs64, edge_tab, round_action, mode, j);
+                              jobs64, edge_list, lcount + 1, rstep, j);
     }
     REFORM_HIP_CHECK(hipGetLastError());
     return 0;
@@ -1146,6 +1256,7 @@ int WidePraq::enqueue_round(hipStream_t stream) {
     const dim3 blk(kWideBlock);
     hipLaunchKernelGGL(wide_pre<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream, jobs64,
                        count, round_action, pending);
+    if (build_lists(stream, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
                        node_blocks, edge_tab, round_action, pending, 0);
     for (int j = 0; j < op_steps; ++j)
@@ -1161,7 +1272,7 @@ int WidePraq::enqueue_round(hipStream_t stream) {
                            jobs64, count);
     }
     // restarts: one uncounted vanilla step, the new anchor, the norms
-    if (step(stream, 1, 0)) return kErrorDevice;
+    if (build_lists(stream, 1) || step(stream, 1, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
                        node_blocks, edge_tab, round_action, pending, 1);
     hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,

@@ -202,6 +202,16 @@ struct WideSubset {
     int reserve(int n);
 };
 
+// What a step block needs of its instance besides the job record's arrays,
+// for the steps a work-list build enqueues: mode 0 the round's steps, mode
+// 1 the restart step.
+struct RoundStep {
+    int64_t r0;       // restart counter at the round's first step
+    double omega, eta;
+    int act;          // the round's action (mode 0), kActRestart (mode 1)
+    int steps;        // steps taken: op_steps, 1 for kActInit and a restart
+};
+
 // PRAQ: first = (node block range, edge block range) per instance; partial
 // rows (kPraqSums each) per node block, then per edge block.
 struct WidePraq : WideControl {
@@ -210,6 +220,14 @@ struct WidePraq : WideControl {
     int2* node_tab = nullptr;                 // (instance, local block) per node block
     int2* edge_tab = nullptr;
     int node_blocks = 0, edge_blocks = 0;
+    // the step kernels' work lists of the round: (instance, local block,
+    // precision) of the node and edge blocks of instances taking steps,
+    // their lengths, the instances' step scalars and list offsets
+    int4* node_list = nullptr;
+    int4* edge_list = nullptr;
+    int* lcount = nullptr;
+    RoundStep* rstep = nullptr;
+    int* list_at = nullptr;
 
     // the full set's records: host copies, device arrays as setup() got them,
     // and the subset arrays shrink() gathers into
@@ -223,6 +241,8 @@ struct WidePraq : WideControl {
     int setup(std::vector<PraqJob<double>> host64, const PraqJob<float>* dev32,
               const PraqJob<double>* dev64);
     int step(hipStream_t stream, int mode, int j);
+    // the work lists of mode 0 (steps of the round) or 1 (restart steps)
+    int build_lists(hipStream_t stream, int mode);
     int start(hipStream_t stream) override;
     int enqueue_round(hipStream_t stream) override;
     int expire(hipStream_t stream, const int* mask = nullptr) override;
@@ -237,16 +257,6 @@ struct WidePraq : WideControl {
 
 // PRIA: first = (slice range, column-tile range) per instance; partial rows
 // (kPriaSums wide) per slice, then one per column tile in (item, tile) order.
-// What a PRIA step block needs of its instance besides the job record's
-// arrays, for the steps a work-list build enqueues: mode 0 the round's
-// steps, mode 1 the restart step.
-struct RoundStep {
-    int64_t r0;       // restart counter at the round's first step
-    double omega, eta;
-    int act;          // the round's action (mode 0), kActRestart (mode 1)
-    int steps;        // steps taken: op_steps, 1 for kActInit and a restart
-};
-
 struct WidePria : WideControl {
     int max_T = 2;
     const PriaJob<float>* jobs32 = nullptr;
