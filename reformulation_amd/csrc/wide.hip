@@ -63,19 +63,6 @@ __device__ __forceinline__ bool wide_step_for(const Job& job, const int* round_a
     return true;
 }
 
-// The point a sums launch bounds: 1 = (x, y), 2 = the step point (sx, sy),
-// 0 = the instance takes no check in this launch.
-template <class Job>
-__device__ __forceinline__ int wide_sums_point(const Job& job, const int* round_action, int inst,
-                                               int mode) {
-    if (mode == 0) {
-        const int act = round_action[inst];
-        if (act == kActHalpernBlock) return 2;
-        return wide_runs_steps(act) ? 1 : 0;
-    }
-    return wide_restarting(job.state) ? 2 : 0;
-}
-
 // The copy a copy launch makes for this instance: mode 0 (a round's start)
 // 3 = the start copies of an instance starting, 0 = anchors where pending;
 // mode 1: 1 = restart anchors; mode 2: 2 = finished step points; -1: none.
@@ -371,18 +358,31 @@ wpraq_fill_lists(const int* first, const int* at, int4* node_list, int4* edge_li
     for (int k = threadIdx.x; k < eb; k += blockDim.x) edge_list[ae + k] = make_int4(q, k, p, 0);
 }
 
-// node blocks, then edge blocks; one partial row per block
+// The point a sums block of a listed instance bounds: the step point (sx,
+// sy) after Halpern steps and a restart step, else (x, y).
+__device__ __forceinline__ int listed_sums_point(const RoundStep& rs) {
+    return rs.act == kActHalpernBlock || rs.act == kActRestart ? 2 : 1;
+}
+
+// Over the work lists of the mode (the instances that took its steps):
+// blocks [0, node_blocks) the node list, the rest the edge list; one
+// partial row per table block, at its table position.
 __global__ void __launch_bounds__(256) wpraq_sums(const PJ32* j32, const PJ64* j64,
-                                                  const int2* node_tab, int node_blocks,
-                                                  const int2* edge_tab, const int* round_action,
-                                                  int mode, double* partial) {
+                                                  const int4* node_list, int node_blocks,
+                                                  const int4* edge_list, const int* lcount,
+                                                  const RoundStep* rstep, const int* first,
+                                                  double* partial) {
     __shared__ double red[kPraqSums * 16];
     const bool node = int(blockIdx.x) < node_blocks;
-    const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
-    dispatch(j32, j64, t.x, [&](const auto& job) {
+    const int k = node ? int(blockIdx.x) : int(blockIdx.x) - node_blocks;
+    const int n = lcount[node ? 0 : 1];
+    const int4 t = node ? node_list[k] : edge_list[k];
+    if (k >= n) return;
+    const size_t row = node ? size_t(first[4 * t.x] + t.y)
+                            : size_t(node_blocks + first[4 * t.x + 2] + t.y);
+    praq_listed(j32, j64, t, [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
-        const int where = wide_sums_point(job, round_action, t.x, mode);
-        if (!where) return;
+        const int where = listed_sums_point(rstep[t.x]);
         const PraqView<Real> a = praq_view(job.a);
         const gptr<const Real> bx = where == 2 ? a.sx : a.x;
         const gptr<const Real> by = where == 2 ? a.sy : a.y;
@@ -396,8 +396,7 @@ __global__ void __launch_bounds__(256) wpraq_sums(const PJ32* j32, const PJ64* j
             praq_edge_sums(a, gid, bx, by, true, acc);
         block_sums<kPraqSums>(acc, red);
         if (threadIdx.x == 0)
-            for (int k = 0; k < kPraqSums; ++k)
-                partial[size_t(blockIdx.x) * kPraqSums + k] = acc[k];
+            for (int q = 0; q < kPraqSums; ++q) partial[row * kPraqSums + q] = acc[q];
     });
 }
 
@@ -651,35 +650,49 @@ wpria_yz(const RJ32* j32, const RJ64* j64, const int2* list, const int* lcount,
                                              kPriaZCacheBytes / int(sizeof(Real)));
 }
 
-// slice blocks (six sums each), then tile blocks (a column-minimum sum each,
-// stored at the tile's (item, tile) row)
+// Over the work lists of the mode (the instances that took its steps):
+// blocks [0, s_blocks) the slice lists (FP32, then FP64), six sums each at
+// the slice's table row; the rest the x-tile lists, a column-minimum sum
+// each at the tile's (item, tile) row.
 __global__ void __launch_bounds__(256) wpria_sums(const RJ32* j32, const RJ64* j64,
-                                                  const int2* s_tab, int s_blocks,
-                                                  const int4* x_tab, const int* round_action,
-                                                  int mode, double* partial) {
+                                                  const int2* slists, int s_blocks,
+                                                  const int4* xlists, int x_blocks,
+                                                  const int* lcount, const RoundStep* rstep,
+                                                  const int* first, double* partial) {
     __shared__ double red[kPriaSums * 16];
     const bool slice = int(blockIdx.x) < s_blocks;
-    if (!slice && x_tab[blockIdx.x - s_blocks].y < 0) return;   // item-group padding
-    const int inst = slice ? s_tab[blockIdx.x].x : x_tab[blockIdx.x - s_blocks].x;
-    dispatch(j32, j64, inst, [&](const auto& job) {
+    int k = slice ? int(blockIdx.x) : int(blockIdx.x) - s_blocks;
+    const int n32 = lcount[slice ? kListS32 : kListX32], n64 = lcount[slice ? kListS64 : kListX64];
+    const bool in64 = k >= n32;
+    if (in64) k -= n32;
+    if (k >= (in64 ? n64 : n32)) return;
+    int4 t;
+    if (slice) {
+        const int2 e = slists[(in64 ? s_blocks : 0) + k];
+        t = make_int4(e.x, e.y, 0, 0);
+    } else {
+        t = xlists[(in64 ? x_blocks : 0) + k];
+    }
+    if (!slice && t.y < 0) return;   // item-group padding
+    auto body = [&](const auto& job) {
         using Real = RealOf<decltype(job)>;
-        const int where = wide_sums_point(job, round_action, inst, mode);
-        if (!where) return;
+        const int where = listed_sums_point(rstep[t.x]);
         const PriaView<Real> a = pria_view(job.a);
         double acc[kPriaSums];
         for (int q = 0; q < kPriaSums; ++q) acc[q] = 0.0;
         if (slice) {
-            pria_slice_sums(a, s_tab[blockIdx.x].y, where == 2 ? a.sx : a.x, red, acc);
+            pria_slice_sums(a, t.y, where == 2 ? a.sx : a.x, red, acc);
+            const size_t row = size_t(first[4 * t.x] + t.y);
             if (threadIdx.x == 0)
-                for (int q = 0; q < kPLower; ++q)
-                    partial[size_t(blockIdx.x) * kPriaSums + q] = acc[q];
+                for (int q = 0; q < kPLower; ++q) partial[row * kPriaSums + q] = acc[q];
         } else {
-            const int4 t = x_tab[blockIdx.x - s_blocks];
             pria_column_sums<Real, kWideTW>(a, t.y, t.z, where == 2 ? a.sy : a.y,
                                             where == 2 ? a.sz : a.z, red, acc);
             if (threadIdx.x == 0) partial[size_t(s_blocks + t.w) * kPriaSums] = acc[kPLower];
         }
-    });
+    };
+    if (in64) body(j64[t.x]);
+    else body(j32[t.x]);
 }
 
 __global__ void __launch_bounds__(256) wpria_logic(const RJ32* j32, const RJ64* j64,
@@ -1261,8 +1274,8 @@ int WidePraq::enqueue_round(hipStream_t stream) {
                        node_blocks, edge_tab, round_action, pending, 0);
     for (int j = 0; j < op_steps; ++j)
         if (step(stream, 0, j)) return kErrorDevice;
-    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
-                       node_blocks, edge_tab, round_action, 0, partial);
+    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, node_list,
+                       node_blocks, edge_list, lcount, rstep, first, partial);
     hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first,
                        node_blocks, partial, round_action, 0);
     if (has32) {
@@ -1275,8 +1288,8 @@ int WidePraq::enqueue_round(hipStream_t stream) {
     if (build_lists(stream, 1) || step(stream, 1, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
                        node_blocks, edge_tab, round_action, pending, 1);
-    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
-                       node_blocks, edge_tab, round_action, 1, partial);
+    hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, node_list,
+                       node_blocks, edge_list, lcount, rstep, first, partial);
     hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first,
                        node_blocks, partial, round_action, 1);
     hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
@@ -1535,7 +1548,7 @@ int WidePria::enqueue_round(hipStream_t stream) {
     for (int j = 0; j < op_steps; ++j)
         if (step(stream, 0, j)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
-                       s_tab, s_blocks, x_tab, round_action, 0, partial);
+                       slists, s_blocks, xlists, x_blocks, lcount, rstep, first, partial);
     hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first, s_blocks,
                        partial, round_action, 0);
     if (has32) {
@@ -1547,7 +1560,7 @@ int WidePria::enqueue_round(hipStream_t stream) {
     hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
                        round_action, pending, 1);
     hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
-                       s_tab, s_blocks, x_tab, round_action, 1, partial);
+                       slists, s_blocks, xlists, x_blocks, lcount, rstep, first, partial);
     hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first, s_blocks,
                        partial, round_action, 1);
     hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
