@@ -278,12 +278,252 @@ int pria_cold_start(const PriaArrays<Real>& a, hipStream_t stream) {
 template int pria_cold_start<float>(const PriaArrays<float>&, hipStream_t);
 template int pria_cold_start<double>(const PriaArrays<double>&, hipStream_t);
 
+// ---- the device tier's z pass over chunks -----------------------------------
+//
+// For a large instance (ND >= 2^22 entries with slices longer than the z
+// cache of one workgroup) one workgroup per slice leaves most of the chip
+// idle and re-reads each long slice on every Michelot round.  There the z
+// projection runs over chunks of kPriaZChunk entries of a slice instead,
+// one 256-thread workgroup each: a pass sums its chunk's values (the first
+// pass also those above the slice's last threshold), a combine (one thread
+// per slice) adds the slice's chunk sums in chunk order and takes the
+// Michelot step -- the same test and update as pria_yz_slice's rounds -- and
+// kPriaZRounds such rounds run over all chunks; a slice whose active set
+// has not settled by then finishes its rounds in one workgroup, and a write
+// pass over the chunks applies each slice's threshold.  Every sum is in a
+// fixed order, so the result does not depend on scheduling; it is theta of
+// the final active set, as every tier's.
+constexpr int kPriaZChunk = 4096;
+constexpr int kPriaZBlock = 256;
+constexpr int kPriaZRounds = 4;
+enum { kZTh = 0, kZC = 1, kZFirst = 2, kZDone = 3, kZState = 4 };
+
+// Whether the device tier runs the z pass over chunks for an instance.
+inline bool pria_zsplit(int N, int T, size_t real_bytes) {
+    const int64_t nd = int64_t(N) * T * (T + 1) / 2;
+    return nd >= (int64_t(1) << 22) &&
+           int64_t(N) * T > int64_t(kPriaZCacheBytes) / int64_t(real_bytes);
+}
+
+__host__ __device__ inline int pria_zchunks_of(int N, int T, int s) {
+    const int L = N * (T - s);
+    return (L + kPriaZChunk - 1) / kPriaZChunk;
+}
+
+inline size_t pria_zchunks(int N, int T) {
+    size_t n = 0;
+    for (int s = 0; s < T; ++s) n += size_t(pria_zchunks_of(N, T, s));
+    return n;
+}
+
+// zfirst[s] = the first chunk of slice s (zfirst[T] = all chunks); zchunk[k]
+// = the slice of chunk k.  One thread: once per solve.
+__global__ void pria_zchunk_table_kernel(int N, int T, int* zfirst, int* zchunk) {
+    int k = 0;
+    for (int s = 0; s < T; ++s) {
+        zfirst[s] = k;
+        const int n = pria_zchunks_of(N, T, s);
+        for (int c = 0; c < n; ++c) zchunk[k++] = s;
+    }
+    zfirst[T] = k;
+}
+
+// Round 0: per chunk (sum, count) of all u = z1 + r and of those above the
+// slice's last threshold; round > 0: (sum, count) of those above the
+// slice's current threshold, for slices not settled.
+template <class Real>
+__global__ void __launch_bounds__(kPriaZBlock)
+pria_zpass_kernel(PriaArrays<Real> a, PriaStep<Real> st, const int* zchunk, const int* zfirst,
+                  const double* zst, double* zp, int round) {
+    __shared__ double red[4 * 16];
+    const PriaView<Real> v = pria_view(a);
+    if (double(v.k0) <= 0.0) return;   // z = 0: no sums (the combine settles it)
+    const int k = blockIdx.x, s = zchunk[k], c = k - zfirst[s];
+    if (round > 0 && zst[s * kZState + kZDone] != 0.0) return;
+    const int L = a.N * (a.T - s), base = pria_off(a.N, a.T, s);
+    const int e0 = c * kPriaZChunk, e1 = min(L, e0 + kPriaZChunk);
+    const gptr<const Real> srcz = st.reflect ? v.zc : v.sz;
+    const double th = round == 0 ? v.thz[s] : zst[s * kZState + kZTh];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    constexpr int P = kPriaZChunk / kPriaZBlock;   // entries per thread
+    Real u[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int e = e0 + int(threadIdx.x) + kPriaZBlock * q;
+        const int g = base + (e < e1 ? e : e0);   // past the chunk: its first entry, dropped
+        u[q] = srcz[g] + v.r[g];
+    }
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int e = e0 + int(threadIdx.x) + kPriaZBlock * q;
+        if (e >= e1) continue;
+        const double x = double(u[q]);
+        if (round == 0) {
+            acc[0] += x;
+            acc[1] += 1.0;
+            if (x > th) {
+                acc[2] += x;
+                acc[3] += 1.0;
+            }
+        } else if (x > th) {
+            acc[0] += x;
+            acc[1] += 1.0;
+        }
+    }
+    block_sums<4>(acc, red);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) zp[size_t(k) * 4 + q] = acc[q];
+}
+
+// One wavefront per slice: its chunk sums (lane l adds chunks l, l + 64,
+// ... in order, then a DPP sum over the lanes), then the first threshold
+// (round 0) or one Michelot step (round > 0); a settled slice keeps its
+// threshold for the next step's start.
+__global__ void __launch_bounds__(256)
+pria_zcombine_kernel(const double* zp, const int* zfirst, double* zst, double* thz, double target,
+                     int T, int round) {
+    const int s = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, lane = threadIdx.x % kWave;
+    if (s >= T) return;   // wave-uniform
+    double* z = zst + size_t(s) * kZState;
+    if (target <= 0.0) {
+        if (round == 0 && lane == 0) {
+            z[kZTh] = 0.0;
+            z[kZDone] = 1.0;
+        }
+        return;
+    }
+    if (round > 0 && z[kZDone] != 0.0) return;
+    double S = 0.0, C = 0.0, Sw = 0.0, Cw = 0.0;
+    for (int k = zfirst[s] + lane; k < zfirst[s + 1]; k += kWave) {
+        S += zp[size_t(k) * 4];
+        C += zp[size_t(k) * 4 + 1];
+        Sw += zp[size_t(k) * 4 + 2];
+        Cw += zp[size_t(k) * 4 + 3];
+    }
+    S = wave_sum(S);
+    C = wave_sum(C);
+    Sw = wave_sum(Sw);
+    Cw = wave_sum(Cw);
+    if (lane != 0) return;
+    if (round == 0) {
+        const bool first = Cw > 0.0;
+        const double Cn = first ? Cw : C;
+        z[kZTh] = ((first ? Sw : S) - target) / Cn;
+        z[kZC] = Cn;
+        z[kZFirst] = first ? 1.0 : 0.0;
+        z[kZDone] = 0.0;
+        return;
+    }
+    const double C0 = z[kZC];
+    const bool first = z[kZFirst] != 0.0;
+    if (C == C0 || C == 0.0 || (C > C0 && !first)) {
+        z[kZDone] = 1.0;
+        thz[s] = z[kZTh];
+    } else {
+        z[kZC] = C;
+        z[kZTh] = (S - target) / C;
+        z[kZFirst] = 0.0;
+    }
+}
+
+// Slices not settled after the chunk rounds: the rest of their rounds in
+// one workgroup (values re-read each round), as pria_yz_slice's memory path.
+template <class Real>
+__global__ void __launch_bounds__(512)
+pria_zfinish_kernel(PriaArrays<Real> a, PriaStep<Real> st, double* zst) {
+    __shared__ double red[kPriaYZRed];
+    const PriaView<Real> v = pria_view(a);
+    const int s = blockIdx.x;
+    double* z = zst + size_t(s) * kZState;
+    const double target = double(v.k0);
+    if (target <= 0.0 || z[kZDone] != 0.0) return;
+    const int L = a.N * (a.T - s), base = pria_off(a.N, a.T, s);
+    const gptr<const Real> srcz = st.reflect ? v.zc : v.sz;
+    double th = z[kZTh], C = z[kZC];
+    bool first = z[kZFirst] != 0.0;
+    int parity = 0;
+    for (int round = 0; round <= L; ++round) {
+        double w2[2] = {0.0, 0.0};
+        for (int e = threadIdx.x; e < L; e += blockDim.x) {
+            const double x = double(Real(srcz[base + e] + v.r[base + e]));
+            if (x > th) {
+                w2[0] += x;
+                w2[1] += 1.0;
+            }
+        }
+        block_sums_alt<2>(w2, red, parity);
+        if (w2[1] == C || w2[1] == 0.0 || (w2[1] > C && !first)) break;
+        C = w2[1];
+        th = (w2[0] - target) / w2[1];
+        first = false;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        z[kZTh] = th;
+        z[kZDone] = 1.0;
+        v.thz[s] = th;
+    }
+}
+
+// Per chunk: z2 from its slice's threshold, the next zc (and z, sz when
+// storing), with pria_yz_slice's arithmetic.
+template <class Real>
+__global__ void __launch_bounds__(kPriaZBlock)
+pria_zwrite_kernel(PriaArrays<Real> a, PriaStep<Real> st, const int* zchunk, const int* zfirst,
+                   const double* zst) {
+    const PriaView<Real> v = pria_view(a);
+    const int k = blockIdx.x, s = zchunk[k], c = k - zfirst[s];
+    const int L = a.N * (a.T - s), base = pria_off(a.N, a.T, s);
+    const int e0 = c * kPriaZChunk, e1 = min(L, e0 + kPriaZChunk);
+    const double target = double(v.k0);
+    const Real thr = Real(zst[s * kZState + kZTh]);
+    const gptr<const Real> srcz = st.reflect ? v.zc : v.sz;
+    constexpr int U = 4;
+    for (int b0 = e0 + int(threadIdx.x); b0 < e1; b0 += kPriaZBlock * U) {
+        int g[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int e = b0 + kPriaZBlock * q;
+            g[q] = base + (e < e1 ? e : e0);
+        }
+        Real v1[U], rr[U], an[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            v1[q] = srcz[g[q]];
+            rr[q] = v.r[g[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) an[q] = st.reflect ? Real(v.z0[g[q]]) : v1[q];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int e = b0 + kPriaZBlock * q;
+            if (e < e1) {
+                const Real d = (v1[q] + rr[q]) - thr;
+                const Real z2 = target > 0.0 ? (d > Real(0) ? d : Real(0)) : Real(0);
+                v.zc[g[q]] = st.lwn * (Real(2) * z2 - v1[q]) + st.rwn * an[q];
+                if (st.store) {
+                    v.z[g[q]] = v1[q];
+                    v.sz[g[q]] = z2;
+                }
+            }
+        }
+    }
+}
+
 size_t pria_device_work_bytes(int N, int T) {
     // the narrower precision's tiles bound the column rows
     constexpr int TW = PriaShape<float>::TW < PriaShape<double>::TW ? PriaShape<float>::TW
                                                                     : PriaShape<double>::TW;
     size_t tiles = size_t(N) * ((T + TW - 1) / TW);
-    return sizeof(double) * (size_t(T) * kPriaSums + tiles + kPriaSums + 8);
+    size_t bytes = sizeof(double) * (size_t(T) * kPriaSums + tiles + kPriaSums + 8);
+    // the chunked z pass: slice states, chunk sums, chunk table (FP64's
+    // test is the looser one: its z cache holds fewer values)
+    if (pria_zsplit(N, T, sizeof(double))) {
+        const size_t chunks = pria_zchunks(N, T);
+        bytes += sizeof(double) * (size_t(T) * kZState + chunks * 4) +
+                 sizeof(int) * (size_t(T) + 1 + chunks);
+    }
+    return bytes;
 }
 
 template <class Real>
@@ -293,6 +533,42 @@ struct PriaDevice {
     double* slices;
     double* cols;
     double* dsums;
+    // the chunked z pass (zst null: one workgroup per slice)
+    double* zst = nullptr;
+    double* zp = nullptr;
+    int* zfirst = nullptr;
+    int* zchunk = nullptr;
+    int nchunks = 0;
+
+    // Places the chunked z pass's arrays after the check rows (`end` = the
+    // work area's first free double) when the instance takes it, and builds
+    // the chunk table.
+    int place_zsplit(double* end) {
+        if (!pria_zsplit(a.N, a.T, sizeof(Real))) return 0;
+        nchunks = int(pria_zchunks(a.N, a.T));
+        zst = end;
+        zp = zst + size_t(a.T) * kZState;
+        zfirst = reinterpret_cast<int*>(zp + size_t(nchunks) * 4);
+        zchunk = zfirst + a.T + 1;
+        hipLaunchKernelGGL(pria_zchunk_table_kernel, dim3(1), dim3(1), 0, stream, a.N, a.T, zfirst,
+                           zchunk);
+        return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
+    }
+
+    // The z part of a step over chunks (place_zsplit done).
+    void z_chunked(const PriaStep<Real>& st) {
+        const double target = double(a.k0);
+        const int cb = (a.T * kWave + 255) / 256;
+        for (int round = 0; round <= kPriaZRounds; ++round) {
+            hipLaunchKernelGGL(pria_zpass_kernel<Real>, dim3(nchunks), dim3(kPriaZBlock), 0,
+                               stream, a, st, zchunk, zfirst, zst, zp, round);
+            hipLaunchKernelGGL(pria_zcombine_kernel, dim3(cb), dim3(256), 0, stream, zp, zfirst,
+                               zst, a.thz, target, a.T, round);
+        }
+        hipLaunchKernelGGL(pria_zfinish_kernel<Real>, dim3(a.T), dim3(512), 0, stream, a, st, zst);
+        hipLaunchKernelGGL(pria_zwrite_kernel<Real>, dim3(nchunks), dim3(kPriaZBlock), 0, stream,
+                           a, st, zchunk, zfirst, zst);
+    }
 
     int reset_thresholds() {
         const int n = a.N * a.T;
@@ -309,14 +585,17 @@ struct PriaDevice {
         if (a.T <= kPriaCompactT) {
             hipLaunchKernelGGL((pria_y_kernel<Real, true>), dim3(a.T * chunks), dim3(256), 0,
                                stream, a, st, chunks);
-            hipLaunchKernelGGL((pria_yz_kernel<Real, true>), dim3(a.T), dim3(kPriaYZBlock),
-                               kPriaZCacheBytes, stream, a, st);
+            if (!zst)
+                hipLaunchKernelGGL((pria_yz_kernel<Real, true>), dim3(a.T), dim3(kPriaYZBlock),
+                                   kPriaZCacheBytes, stream, a, st);
         } else {
             hipLaunchKernelGGL((pria_y_kernel<Real, false>), dim3(a.T * chunks), dim3(256), 0,
                                stream, a, st, chunks);
-            hipLaunchKernelGGL((pria_yz_kernel<Real, false>), dim3(a.T), dim3(kPriaYZBlock),
-                               kPriaZCacheBytes, stream, a, st);
+            if (!zst)
+                hipLaunchKernelGGL((pria_yz_kernel<Real, false>), dim3(a.T), dim3(kPriaYZBlock),
+                                   kPriaZCacheBytes, stream, a, st);
         }
+        if (zst) z_chunked(st);
         return hipGetLastError() == hipSuccess ? 0 : kPriaErrorDevice;
     }
     int copy3(Real* d1, const Real* s1, Real* d2, const Real* s2, Real* d3, const Real* s3) {
@@ -348,8 +627,9 @@ int pria_device_solve(const PriaJob<Real>& job, LogicState& st, double* work,
     const LogicParams& p = job.p;
     constexpr int TW = PriaShape<Real>::TW;
     const int tiles = (a.T + TW - 1) / TW;
-    PriaDevice<Real> d{a, stream, work, work + size_t(a.T) * kPriaSums,
-                       work + size_t(a.T) * kPriaSums + size_t(a.N) * tiles};
+    double* const dsums = work + size_t(a.T) * kPriaSums + size_t(a.N) * tiles;
+    PriaDevice<Real> d{a, stream, work, work + size_t(a.T) * kPriaSums, dsums};
+    if (d.place_zsplit(dsums + kPriaSums + 8)) return kPriaErrorDevice;
     if (pria_x_lds_bytes<Real>(a.T) > 65536) {
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(pria_x_kernel<Real>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -76,6 +76,18 @@ __device__ __forceinline__ int wide_copy_kind(const LogicState* st, const int* r
     return st->action == kActDoneSX ? 2 : -1;
 }
 
+// Whether a copy launch of this mode can have work for the instance, from
+// per-instance arrays alone, before its job record and state are read:
+// mode 0 a start or a pending anchor copy; mode 1 the instance is in the
+// restart step's work lists (list_at of the latest list build); mode 2
+// (finished step points) is decided by the state.
+__device__ __forceinline__ bool copy_may_apply(int mode, const int* round_action,
+                                               const int* pending, const int* list_at, int inst) {
+    if (mode == 0) return round_action[inst] == kActInit || pending[inst];
+    if (mode == 1) return list_at[3 * inst] >= 0;
+    return true;
+}
+
 // The action a logic launch applies to this instance, or -1.
 template <class Job>
 __device__ __forceinline__ int wide_logic_action(const Job& job, const int* round_action, int inst,
@@ -429,9 +441,11 @@ __global__ void __launch_bounds__(256) wpraq_logic(const PJ32* j32, const PJ64* 
 __global__ void __launch_bounds__(256) wpraq_copy(const PJ32* j32, const PJ64* j64,
                                                   const int2* node_tab, int node_blocks,
                                                   const int2* edge_tab, const int* round_action,
-                                                  const int* pending, int mode) {
+                                                  const int* pending, const int* list_at,
+                                                  int mode) {
     const bool node = int(blockIdx.x) < node_blocks;
     const int2 t = node ? node_tab[blockIdx.x] : edge_tab[blockIdx.x - node_blocks];
+    if (!copy_may_apply(mode, round_action, pending, list_at, t.x)) return;
     dispatch(j32, j64, t.x, [&](const auto& job) {
         const int kind = wide_copy_kind(job.state, round_action, pending, t.x, mode);
         if (kind < 0) return;
@@ -721,8 +735,10 @@ __global__ void __launch_bounds__(256) wpria_logic(const RJ32* j32, const RJ64* 
 // clear the slice's thresholds
 __global__ void __launch_bounds__(256) wpria_copy(const RJ32* j32, const RJ64* j64,
                                                   const int2* s_tab, const int* round_action,
-                                                  const int* pending, int mode) {
+                                                  const int* pending, const int* list_at,
+                                                  int mode) {
     const int2 t = s_tab[blockIdx.x];
+    if (!copy_may_apply(mode, round_action, pending, list_at, t.x)) return;
     dispatch(j32, j64, t.x, [&](const auto& job) {
         const int kind = wide_copy_kind(job.state, round_action, pending, t.x, mode);
         if (kind < 0) return;
@@ -1271,7 +1287,7 @@ int WidePraq::enqueue_round(hipStream_t stream) {
                        count, round_action, pending);
     if (build_lists(stream, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
-                       node_blocks, edge_tab, round_action, pending, 0);
+                       node_blocks, edge_tab, round_action, pending, list_at, 0);
     for (int j = 0; j < op_steps; ++j)
         if (step(stream, 0, j)) return kErrorDevice;
     hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, node_list,
@@ -1287,13 +1303,13 @@ int WidePraq::enqueue_round(hipStream_t stream) {
     // restarts: one uncounted vanilla step, the new anchor, the norms
     if (build_lists(stream, 1) || step(stream, 1, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
-                       node_blocks, edge_tab, round_action, pending, 1);
+                       node_blocks, edge_tab, round_action, pending, list_at, 1);
     hipLaunchKernelGGL(wpraq_sums, dim3(all), blk, 0, stream, jobs32, jobs64, node_list,
                        node_blocks, edge_list, lcount, rstep, first, partial);
     hipLaunchKernelGGL(wpraq_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first,
                        node_blocks, partial, round_action, 1);
     hipLaunchKernelGGL(wpraq_copy, dim3(all), blk, 0, stream, jobs32, jobs64, node_tab,
-                       node_blocks, edge_tab, round_action, pending, 2);
+                       node_blocks, edge_tab, round_action, pending, list_at, 2);
     hipLaunchKernelGGL(wide_count<PraqJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());
@@ -1305,7 +1321,7 @@ int WidePraq::expire(hipStream_t stream, const int* mask) {
     hipLaunchKernelGGL(wide_expire<PraqJob<double>>, dim3(grid_for(count)), blk, 0, stream,
                        jobs64, count, mask);
     hipLaunchKernelGGL(wpraq_copy, dim3(node_blocks + edge_blocks), blk, 0, stream, jobs32,
-                       jobs64, node_tab, node_blocks, edge_tab, round_action, pending, 2);
+                       jobs64, node_tab, node_blocks, edge_tab, round_action, pending, list_at, 2);
     hipLaunchKernelGGL(wide_count<PraqJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());
@@ -1544,7 +1560,7 @@ int WidePria::enqueue_round(hipStream_t stream) {
                        count, round_action, pending);
     if (build_lists(stream, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
-                       round_action, pending, 0);
+                       round_action, pending, list_at, 0);
     for (int j = 0; j < op_steps; ++j)
         if (step(stream, 0, j)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
@@ -1558,13 +1574,13 @@ int WidePria::enqueue_round(hipStream_t stream) {
     }
     if (build_lists(stream, 1) || step(stream, 1, 0)) return kErrorDevice;
     hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
-                       round_action, pending, 1);
+                       round_action, pending, list_at, 1);
     hipLaunchKernelGGL(wpria_sums, dim3(s_blocks + x_blocks), blk, 0, stream, jobs32, jobs64,
                        slists, s_blocks, xlists, x_blocks, lcount, rstep, first, partial);
     hipLaunchKernelGGL(wpria_logic, dim3(count), blk, 0, stream, jobs32, jobs64, first, s_blocks,
                        partial, round_action, 1);
     hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
-                       round_action, pending, 2);
+                       round_action, pending, list_at, 2);
     hipLaunchKernelGGL(wide_count<PriaJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());
@@ -1576,7 +1592,7 @@ int WidePria::expire(hipStream_t stream, const int* mask) {
     hipLaunchKernelGGL(wide_expire<PriaJob<double>>, dim3(grid_for(count)), blk, 0, stream,
                        jobs64, count, mask);
     hipLaunchKernelGGL(wpria_copy, dim3(s_blocks), blk, 0, stream, jobs32, jobs64, s_tab,
-                       round_action, pending, 2);
+                       round_action, pending, list_at, 2);
     hipLaunchKernelGGL(wide_count<PriaJob<double>>, dim3(1), blk, 0, stream, jobs64, count,
                        active);
     REFORM_HIP_CHECK(hipGetLastError());

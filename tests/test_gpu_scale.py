@@ -335,3 +335,36 @@ def test_periods_beyond_640_match_the_oracle(shape):
         else:
             assert bi.iterations == o.iterations
             assert abs(bi.upper - o.upper) <= 1e-9 * sc
+
+
+@pytest.mark.parametrize("joint", ["k0", "zero"])
+def test_device_tier_chunked_z_pass_agrees_with_the_batched_tier(joint):
+    """N x T = 60 x 380 (ND = 4.3M, slices of up to 22,800 entries): the
+    device tier runs its z projection over chunks of 4096 entries with
+    multi-block Michelot rounds (one workgroup finishing any slice not
+    settled after four), the batched tier one workgroup per slice.  FP64
+    iterations, restarts and bounds agree; with K0 = 0 the z part is 0."""
+    from reformulation_amd import pria, workloads
+    from reformulation_amd.batch import Batch
+    inst = workloads.pria_c1(7, 60, 380)
+    k0 = inst.joint_order_cost if joint == "k0" else 0.0
+    problem = pria.JrpProblem.new(inst.order_costs, k0, inst.holding_costs)
+    print(f"device tier 60x380 K0={k0}", flush=True)
+    params = pria.Parameters.with_auto_step(None, 1e-6, tier=pria.TIER_DEVICE, time_limit=120.0)
+    x, y, z, info = pria.solve_with_info(problem, params)
+    assert info.tier == pria.TIER_DEVICE and info.status == 0
+    print("batched 60x380", flush=True)
+    b = Batch(use_float32=False)
+    q = b.add_pria(problem, pria.Parameters.with_auto_step(None, 1e-6))
+    b.upload()
+    b.solve("batched")
+    bx, by, bz, binfo = b.pria_result(q)
+    b.close()
+    assert binfo.status == 0
+    assert info.iterations == binfo.iterations and info.restarts == binfo.restarts
+    sc = 1.0 + abs(binfo.upper)
+    assert abs(info.upper - binfo.upper) <= 1e-9 * sc
+    assert abs(info.lower - binfo.lower) <= 1e-9 * sc
+    assert np.max(np.abs(x - bx)) <= 1e-7
+    if joint == "zero":
+        assert not np.any(z) and not np.any(bz)
