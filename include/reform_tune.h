@@ -22,10 +22,13 @@
  *   write phase (when every T' <= 320), 2 they read it again; built-in 1.
  *   "batch.pria.yz_shape": 1 the compact y/z register shape when every T'
  *   <= 320, 2 always the full one; built-in 1.
- * A batch reads the table when its first batched solve starts, a server
- * when it is created.  x_hold and yz_shape give the same results bit for
- * bit; lanes and x_threads change the order of FP32/FP64 sums (x_threads
- * only FP32's), not the algorithm.
+ * A batch takes its instances' lanes when it uploads them and the PRIA
+ * shapes when its first batched solve starts; a server takes a request's
+ * lanes when it builds the request and the PRIA shapes whenever it
+ * rebuilds its tables, so change the table only while no server runs.
+ * x_hold and yz_shape give the same results bit for bit; lanes and
+ * x_threads change the order of FP32/FP64 sums (x_threads only FP32's),
+ * not the algorithm.
  */
 #ifndef REFORMULATION_REFORM_TUNE_H
 #define REFORMULATION_REFORM_TUNE_H
