@@ -615,23 +615,26 @@ __device__ __forceinline__ const PriaJob<Real>& pria_job(const RJ32* j32, const 
 // 512 threads and 16 rows with the tuning table's batch.pria.x_threads),
 // FP64 in 512 threads (16 rows at a time; 256-byte tile rows, so LDS
 // T' x 33 x 8 bytes: two blocks per CU).  TW = 16 is the FP64 fallback for
-// T' whose 32-column tile does not fit the LDS: the tile as two halves (the
-// same 16 lanes per column, so the same sums).
+// T' whose 32-column tile does not fit the LDS: the tile's two halves in
+// two 256-thread workgroups, still 16 lanes per column, so an instance's
+// sums do not depend on which form its batch takes.
 template <class Real, int TW, int HOLD = 0, int BS = (sizeof(Real) == 4 ? 256 : 512)>
 __global__ void __launch_bounds__(BS)
 wpria_x(const RJ32* j32, const RJ64* j64, const int4* list, const int* lcount,
         const RoundStep* rstep, int j) {
     extern __shared__ char dyn[];
-    const int n = *lcount;
-    const int4 t = list[blockIdx.x];   // the grid is the list's capacity
-    if (int(blockIdx.x) >= n || t.y < 0) return;   // past the list; item-group padding
+    // TW < kWideTW: two workgroups per table entry, one per half tile
+    constexpr int per = kWideTW / TW;
+    const int n = *lcount, b = int(blockIdx.x) / per;
+    const int4 t = list[b];   // the grid is per times the list's capacity
+    if (b >= n || t.y < 0) return;   // past the list; item-group padding
+    const int t0 = t.z + TW * (int(blockIdx.x) % per);
     const PriaJob<Real>& job = pria_job<Real>(j32, j64, t.x);
+    if (t0 >= job.a.T) return;
     PriaStep<Real> sc;
     if (!round_step(rstep[t.x], j, pria_step_scalars<Real>, &sc)) return;
     const PriaView<Real> a = pria_view(job.a);
-    Real* lds = reinterpret_cast<Real*>(dyn);
-    pria_x_tile<Real, TW, HOLD>(a, sc, t.y, t.z, lds);
-    if (TW < kWideTW && t.z + TW < a.T) pria_x_tile<Real, TW, HOLD>(a, sc, t.y, t.z + TW, lds);
+    pria_x_tile<Real, TW, HOLD>(a, sc, t.y, t0, reinterpret_cast<Real*>(dyn));
 }
 
 // Register shape of the y/z pass by T' (the compact one up to T' = 320).
@@ -1475,8 +1478,9 @@ int WidePria::build_tables(const std::vector<PriaJob<double>>& host64,
     x32_hold = x32_hold_knob && max_T <= kPriaCompactT;
     compact = yz_compact_knob && max_T <= kPriaCompactT;
     const void* x32 = x32_kernel();
-    const void* x64 = x64_full ? reinterpret_cast<const void*>(wpria_x<double, kWideTW>)
-                               : reinterpret_cast<const void*>(wpria_x<double, kWideTW / 2>);
+    const void* x64 = x64_full
+                          ? reinterpret_cast<const void*>(wpria_x<double, kWideTW>)
+                          : reinterpret_cast<const void*>(wpria_x<double, kWideTW / 2, 0, 256>);
     if (lds32 > 65536)
         REFORM_HIP_CHECK(hipFuncSetAttribute(x32, hipFuncAttributeMaxDynamicSharedMemorySize,
                                              int(lds32)));
@@ -1522,10 +1526,10 @@ int WidePria::step(hipStream_t stream, int mode, int j) {
         hipExtLaunchKernelGGL((wpria_x<double, kWideTW>), dim3(x_blocks), dim3(512), lds64,
                               stream, b64, e, 0, jobs32, jobs64, lx64, lcount + kListX64,
                               rstep, j);
-    else
-        hipExtLaunchKernelGGL((wpria_x<double, kWideTW / 2>), dim3(x_blocks), dim3(512), lds64,
-                              stream, b64, e, 0, jobs32, jobs64, lx64, lcount + kListX64,
-                              rstep, j);
+    else   // half tiles in 256 threads: 16 lanes per column, as the full tile's
+        hipExtLaunchKernelGGL((wpria_x<double, kWideTW / 2, 0, 256>), dim3(2 * x_blocks),
+                              dim3(256), lds64, stream, b64, e, 0, jobs32, jobs64, lx64,
+                              lcount + kListX64, rstep, j);
     if (mark_events(1, &b, &e)) return kErrorDevice;
     auto yz = [&](auto real, const int2* list, int l, hipEvent_t start, hipEvent_t stop) {
         using Real = decltype(real);

@@ -141,6 +141,32 @@ def test_result_is_the_same_alone_in_the_mix_and_on_a_second_solve(f32):
     b.close()
 
 
+@pytest.mark.parametrize("f32", [False, True])
+def test_a_long_horizon_member_changes_no_other_result(f32):
+    """A member with T = 331 makes the batch take the wide y/z register
+    shape, FP32 x tiles without registers held and FP64 x tiles as two
+    256-thread half-tile workgroups (a 32-column tile no longer fits two to
+    a CU's LDS), all with the same sums: the other members' results are
+    bitwise those they have without it."""
+    from reformulation_amd import workloads
+    small = [m for m in _members() if m[0] == "pria"][:3]
+    j = workloads.pria_c1(9, 6, 331)
+    big = ("pria", (j.order_costs, j.joint_order_cost, j.holding_costs))
+    b1, i1 = _batch(small, f32)
+    b1.solve("batched")
+    alone = _results(b1, i1)
+    b1.close()
+    b2, i2 = _batch(small + [big], f32)
+    b2.solve("batched")
+    mixed = _results(b2, i2)
+    b2.close()
+    assert mixed[-1][2].status == 0
+    for u, v in zip(alone, mixed):
+        assert _same_bits(u[0], v[0]) and _same_bits(u[1], v[1])
+        assert u[2].iterations == v[2].iterations
+        assert u[2].upper == v[2].upper and u[2].lower == v[2].lower
+
+
 def test_timed_launches_give_the_same_results_and_count_every_step():
     members = _members()
     b, index = _batch(members, True)
