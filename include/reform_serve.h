@@ -22,6 +22,15 @@
  * current point between two rounds and finishes with status 2, as the plain
  * solvers do at their limits.
  *
+ * A request whose inputs hold a NaN or an infinity is rejected (status -2)
+ * by its builder: the plain solvers iterate on such an instance to their
+ * iteration cap (qubo/praq-jax/praq/solver.py:375-381), which on a server
+ * would hold pool memory for nothing.  reform_server_cancel stops a request:
+ * one not yet admitted finishes at once with status -6 and no answer; one in
+ * flight stops at its current point at its family's next round, as at a time
+ * limit, and finishes with status 3 and that point as its answer; one that
+ * converges or reaches a limit first keeps that status.
+ *
  * Return codes: >= 0 success (submit returns the ticket), -1 argument,
  * -2 instance, -3 device, -4 unknown or unfinished ticket, -5 timeout.
  */
@@ -54,7 +63,9 @@ typedef struct reform_request_info {
     int64_t ticket;
     int family;             /* 0 PRAQ, 1 PRIA */
     int status;             /* 0 converged, 1 iteration limit, 2 time limit,
-                               -1 not finished, -2 instance rejected, -3 device */
+                               3 cancelled in flight, -1 not finished,
+                               -2 instance rejected, -3 device,
+                               -6 cancelled before admission (no answer) */
     int64_t iterations;
     int64_t restarts;
     int64_t switch_iteration;   /* FP32 -> FP64 handover, -1 if none */
@@ -75,6 +86,7 @@ typedef struct reform_server_counters {
     int64_t in_flight_peak; /* most requests on the device at once */
     int64_t handovers;      /* FP64 images placed at FP32 requests' handovers */
     int64_t handover_waits; /* rounds a request at its handover waited for room */
+    int64_t cancelled;      /* requests finished by a cancellation (status 3 or -6) */
 } reform_server_counters;
 
 reform_server_config reform_server_default_config(void);
@@ -102,6 +114,9 @@ int reform_server_info(reform_server* server, int64_t ticket, reform_request_inf
 int reform_server_praq_result(reform_server* server, int64_t ticket, double* x, double* y);
 int reform_server_pria_result(reform_server* server, int64_t ticket, double* x, double* y,
                               double* z);
+/* Cancels an unfinished request (see above); -4 if the ticket is unknown,
+ * finished or already cancelled.  Its record stays until released. */
+int reform_server_cancel(reform_server* server, int64_t ticket);
 /* Drops a finished request's record and results. */
 int reform_server_release(reform_server* server, int64_t ticket);
 int reform_server_stats(reform_server* server, reform_server_counters* stats);

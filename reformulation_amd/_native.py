@@ -94,6 +94,7 @@ EXPORTS = (
     "reform_server_submit_praq", "reform_server_submit_pria", "reform_server_poll",
     "reform_server_wait_idle", "reform_server_info", "reform_server_praq_result",
     "reform_server_pria_result", "reform_server_release", "reform_server_stats",
+    "reform_server_cancel", "reform_debug_fail_device_alloc", "reform_debug_device_allocs",
 )
 
 _lib = None

@@ -169,7 +169,17 @@ struct BatchImpl {
             for (auto& e : family) e->set_timing(on);
     }
 
+    // every stream of the batch idle, so that nothing of a solve still runs
+    // when its memory goes
+    void drain() {
+        for (hipStream_t gs : group_streams)
+            if (gs) (void)hipStreamSynchronize(gs);
+        if (s0) (void)hipStreamSynchronize(s0);
+        if (s1) (void)hipStreamSynchronize(s1);
+    }
+
     ~BatchImpl() {
+        drain();
         for (auto& family : wide) family.clear();
         for (hipStream_t gs : group_streams)
             if (gs) (void)hipStreamDestroy(gs);
@@ -326,7 +336,7 @@ int BatchImpl::upload() {
     const size_t off_work = at;
     at = align256(at + work_bytes);
     dev_bytes = at;
-    if (hipMalloc(&dev, dev_bytes) != hipSuccess) {
+    if (device_malloc(&dev, dev_bytes) != hipSuccess) {
         dev = nullptr;
         return kErrorDevice;
     }
@@ -450,7 +460,13 @@ int BatchImpl::solve(int mode) {
     if (mode == REFORM_BATCHED) st = solve_wide();
     else if (mode == REFORM_SEQUENTIAL) st = solve_sequential();
     else if (mode == REFORM_WORKGROUPS) st = solve_workgroups();
-    if (st) return st;
+    if (st) {
+        // a failed solve (a device call or allocation refused part way)
+        // leaves no work running, and the next solve sets its engines up anew
+        drain();
+        for (auto& family : wide) family.clear();
+        return st;
+    }
     if (!praq.empty() &&
         hipMemcpyAsync(praq_states.data(), praq_states_dev, sizeof(LogicState) * praq.size(),
                        hipMemcpyDeviceToHost, s0) != hipSuccess)

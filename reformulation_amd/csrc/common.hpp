@@ -177,6 +177,30 @@ __host__ __device__ inline void halpern_weights(int64_t r, double* lw, double* r
     *rw = 1.0 / (k + 1.0);
 }
 
+// Every device allocation of the library goes through device_malloc or
+// device_malloc_async, so that a test can make one of them fail as out of
+// memory (include/reform_debug.h); device_alloc_fault() counts the
+// allocation and says whether it is the one to fail.
+bool device_alloc_fault();
+
+template <class T>
+inline hipError_t device_malloc(T** p, size_t bytes) {
+    if (device_alloc_fault()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(reinterpret_cast<void**>(p), bytes);
+}
+
+template <class T>
+inline hipError_t device_malloc_async(T** p, size_t bytes, hipStream_t stream) {
+    if (device_alloc_fault()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMallocAsync(reinterpret_cast<void**>(p), bytes, stream);
+}
+
 // The last failed HIP call of the library (call text, place, HIP error),
 // readable through reform_last_error().
 inline std::mutex& last_error_mutex() {

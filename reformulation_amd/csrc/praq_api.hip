@@ -38,7 +38,7 @@ struct DeviceBuffer {
     ~DeviceBuffer() {
         if (ptr) (void)hipFree(ptr);
     }
-    int alloc(size_t bytes) { return hipMalloc(&ptr, bytes ? bytes : 1) == hipSuccess ? 0 : -1; }
+    int alloc(size_t bytes) { return device_malloc(&ptr, bytes ? bytes : 1) == hipSuccess ? 0 : -1; }
 };
 
 

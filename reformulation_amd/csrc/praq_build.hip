@@ -319,7 +319,7 @@ int praq_build_device(const PraqInstance& in, int dense_degree, int lanes, hipSt
                                               scan_scratch(int64_t(kRadixBins) * tiles)})),
                  o_part = carve(8 * size_t(chunks_d + chunks_w)), o_scalars = carve(64);
     b.stream = s;
-    REFORM_HIP_CHECK(hipMallocAsync(&b.arena, at, s));
+    REFORM_HIP_CHECK(device_malloc_async(&b.arena, at, s));
     char* base = static_cast<char*>(b.arena);
     auto I = [&](size_t off) { return reinterpret_cast<int*>(base + off); };
     auto D = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
@@ -408,7 +408,7 @@ int praq_build_device(const PraqInstance& in, int dense_degree, int lanes, hipSt
     // the hub lists, in a second arena now that their sizes are known
     const size_t hub_bytes = align256(4 * size_t(b.nh)) + align256(4 * (size_t(b.nh) + 1)) +
                              align256(4 * size_t(b.nc) + 4);
-    REFORM_HIP_CHECK(hipMallocAsync(&b.hub_arena, hub_bytes, s));
+    REFORM_HIP_CHECK(device_malloc_async(&b.hub_arena, hub_bytes, s));
     char* hb = static_cast<char*>(b.hub_arena);
     b.heavy = reinterpret_cast<int*>(hb);
     b.hchunk = reinterpret_cast<int*>(hb + align256(4 * size_t(b.nh)));

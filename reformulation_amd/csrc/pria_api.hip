@@ -55,7 +55,7 @@ struct StreamBuffer {
     bool owned = false;
     int alloc(size_t bytes, hipStream_t s) {
         stream = s;
-        if (hipMallocAsync(&ptr, bytes ? bytes : 1, s) != hipSuccess) {
+        if (device_malloc_async(&ptr, bytes ? bytes : 1, s) != hipSuccess) {
             ptr = nullptr;
             return kPriaErrorDevice;
         }
@@ -775,7 +775,7 @@ int debug_step(int64_t n, int64_t t, const double* k, double k0, const double* h
     s.scale = 1.0;
     PriaLayout L = pria_layout(s.N, s.T, s.ND, sizeof(Real));
     Buffer buf;
-    if (hipMalloc(&buf.ptr, L.total_bytes) != hipSuccess) return kPriaErrorDevice;
+    if (device_malloc(&buf.ptr, L.total_bytes) != hipSuccess) return kPriaErrorDevice;
     char* base = static_cast<char*>(buf.ptr);
     std::vector<char> image(L.input_bytes, 0);
     pria_fill_image<Real>(s, L, image.data());

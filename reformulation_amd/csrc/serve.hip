@@ -43,6 +43,10 @@ namespace {
 constexpr int kErrorTicket = -4;
 constexpr int kErrorTimeout = -5;
 constexpr int kPending = -1;
+// cancelled after admission (the answer is the point it stopped at) and
+// before it (no answer)
+constexpr int kCancelled = 3;
+constexpr int kCancelledQueued = -6;
 // host threads copying finished requests' answers out; at 1000 requests/s
 // three threads served no faster than one (the staged copies share the link)
 constexpr int kRetireThreads = 1;
@@ -62,7 +66,7 @@ class Pool {
 public:
     int init(size_t bytes) {
         capacity_ = bytes & ~size_t(255);
-        hipError_t e = hipMalloc(&base_, capacity_);
+        hipError_t e = device_malloc(&base_, capacity_);
         if (e != hipSuccess) {
             base_ = nullptr;
             record_error("hipMalloc(serving pool)", e, __FILE__, __LINE__);
@@ -174,7 +178,26 @@ struct Request {
     int status = kPending;
     double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
     double time_limit = 0.0;  // seconds from admission; <= 0: none
+    // set (under the server's lock) by reform_server_cancel; read by the
+    // serving thread when it rebuilds the tables
+    std::atomic<bool> cancel{false};
 };
+
+// Every input value finite (the server turns away what the plain solvers
+// would iterate on to their caps).
+bool all_finite(const double* v, size_t count) {
+    for (size_t i = 0; i < count; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+// The holding costs a PRIA instance reads, H[i][s][t] for s <= t, finite.
+bool holding_finite(const double* h, int64_t n, int64_t t) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t s = 0; s < t; ++s)
+            if (!all_finite(h + (size_t(i) * t + s) * t + s, size_t(t - s))) return false;
+    return true;
+}
 
 }  // namespace
 
@@ -190,6 +213,10 @@ struct reform_server {
     std::atomic<bool> stop{false};
     int64_t next_ticket = 0;
     int64_t n_submitted = 0, n_admitted = 0, n_completed = 0, n_rejected = 0;
+    int64_t n_cancelled = 0;
+    // tickets of cancelled requests already admitted, for the serving thread
+    // (under mu)
+    std::vector<int64_t> cancels;
     int n_building = 0;  // requests on the builders
     std::atomic<int64_t> n_rounds{0}, n_rebuilds{0}, n_in_flight{0}, n_in_flight_peak{0};
     std::atomic<int64_t> n_handovers{0}, n_handover_waits{0};
@@ -273,12 +300,20 @@ struct reform_server {
     int launch(int family);
     int retire_finished(int family);
     void retire(Request& r, int status);
+    void finish_unadmitted(Request& r, int status);
+    void stop_cancelled();
 };
 
 // ------------------------------------------------------------ host builds --
 
 int reform_server::build(Request& r) {
     const bool f32 = cfg.use_float32 != 0;
+    const bool finite =
+        r.family == 0
+            ? all_finite(r.diag, size_t(r.n)) && all_finite(r.vals, size_t(r.m))
+            : all_finite(r.k, size_t(r.n)) && std::isfinite(r.k0) &&
+                  holding_finite(r.h, r.n, r.t);
+    if (!finite) return kErrorInstance;
     if (r.family == 0) {
         PraqInstance in{r.n, r.m, r.diag, r.rows, r.cols, r.vals};
         int st = praq_build_host(in, r.po.dense_degree, nullptr, nullptr, r.ph);
@@ -336,19 +371,30 @@ void reform_server::build_loop() {
         if (!st && r->bytes + r->bytes64 > pool.capacity()) st = kErrorInstance;
         std::lock_guard<std::mutex> lock(mu);
         --n_building;
-        if (st) {
-            r->status = st < 0 ? st : kErrorInstance;
-            r->t_done = seconds_since(t0);
+        if (r->cancel) {
+            finish_unadmitted(*r, kCancelledQueued);
+        } else if (st) {
             ++n_rejected;
-            ++n_completed;
-            finished.push_back(r->ticket);
-            cv_done.notify_all();
-            cv_build.notify_one();
+            finish_unadmitted(*r, st < 0 ? st : kErrorInstance);
         } else {
             ready.push_back(r);
             cv_ready.notify_all();
         }
     }
+}
+
+// A request that never reached the device finishes (under mu): rejected,
+// or cancelled while it waited.
+void reform_server::finish_unadmitted(Request& r, int status) {
+    r.status = status;
+    r.t_done = seconds_since(t0);
+    r.img64 = std::vector<char>();
+    r.img32 = std::vector<char>();
+    if (status == kCancelledQueued) ++n_cancelled;
+    ++n_completed;
+    finished.push_back(r.ticket);
+    cv_done.notify_all();
+    cv_build.notify_one();
 }
 
 // -------------------------------------------------------------- admission --
@@ -451,10 +497,13 @@ int reform_server::rebuild(int family) {
     auto grow = [&](size_t rec32, size_t rec64) -> int {
         if (count <= jobs_cap[family]) return 0;
         size_t cap = std::max<size_t>(count, 2 * jobs_cap[family]);
-        for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < 2; ++p) {
             if (jobs_dev[family][p]) (void)hipFree(jobs_dev[family][p]);
-        REFORM_HIP_CHECK(hipMalloc(&jobs_dev[family][0], rec32 * cap));
-        REFORM_HIP_CHECK(hipMalloc(&jobs_dev[family][1], rec64 * cap));
+            jobs_dev[family][p] = nullptr;
+        }
+        jobs_cap[family] = 0;
+        REFORM_HIP_CHECK(device_malloc(&jobs_dev[family][0], rec32 * cap));
+        REFORM_HIP_CHECK(device_malloc(&jobs_dev[family][1], rec64 * cap));
         jobs_cap[family] = cap;
         return 0;
     };
@@ -496,11 +545,16 @@ int reform_server::rebuild(int family) {
     // the members' own time limits, on the server's clock
     std::vector<double> due(count, std::numeric_limits<double>::infinity());
     bool any = false;
-    for (size_t q = 0; q < count; ++q)
+    for (size_t q = 0; q < count; ++q) {
         if (mem[q]->time_limit > 0.0) {
             due[q] = mem[q]->t_admit + mem[q]->time_limit;
             any = true;
         }
+        if (mem[q]->cancel) {   // stops at its first round in these tables
+            due[q] = -std::numeric_limits<double>::infinity();
+            any = true;
+        }
+    }
     if (!any) due.clear();
     WideControl& w = family == 0 ? static_cast<WideControl&>(praq_engine) : pria_engine;
     w.set_deadlines(std::move(due));
@@ -529,11 +583,38 @@ void reform_server::retire(Request& r, int status) {
     bytes_in_use = pool.used();
     free_slots.push_back(r.slot);
     n_in_flight--;
+    // stopped by its cancellation (a request that converged or ran out of
+    // iterations first keeps that status)
+    if (r.cancel && status == 2) status = kCancelled;
+    if (status == kCancelled) ++n_cancelled;
     r.status = status;
     r.t_done = seconds_since(t0);
     ++n_completed;
     finished.push_back(r.ticket);
     cv_done.notify_all();
+}
+
+// Cancelled requests in the tables stop at the family's next round: their
+// deadlines moved to the past (a cancelled request not in the tables yet is
+// stopped by the rebuild that brings it in, and one already finished keeps
+// its outcome).
+void reform_server::stop_cancelled() {
+    // under the lock throughout: a request found here cannot be released
+    std::lock_guard<std::mutex> lock(mu);
+    std::vector<int64_t> todo;
+    todo.swap(cancels);
+    for (int64_t ticket : todo) {
+        auto found = requests.find(ticket);
+        if (found == requests.end()) continue;
+        Request* r = found->second.get();
+        const int f = r->family;
+        if (changed[f]) continue;   // the coming rebuild reads the flag
+        const auto& mem = members[f];
+        auto it = std::find(mem.begin(), mem.end(), r);
+        if (it == mem.end()) continue;
+        WideControl& w = f == 0 ? static_cast<WideControl&>(praq_engine) : pria_engine;
+        w.stop_position(int(it - mem.begin()));
+    }
 }
 
 // After the family's round: members whose state reads finished (the done
@@ -664,7 +745,7 @@ int reform_server::launch(int f) {
         if (flags_host[f]) REFORM_HIP_CHECK(hipHostFree(flags_host[f]));
         flags_dev[f] = flags_host[f] = nullptr;
         flags_cap[f] = std::max(count, 2 * flags_cap[f]);
-        REFORM_HIP_CHECK(hipMalloc(&flags_dev[f], sizeof(int) * flags_cap[f]));
+        REFORM_HIP_CHECK(device_malloc(&flags_dev[f], sizeof(int) * flags_cap[f]));
         REFORM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&flags_host[f]),
                                        sizeof(int) * flags_cap[f], hipHostMallocDefault));
     }
@@ -694,6 +775,10 @@ int reform_server::launch(int f) {
 
 void reform_server::serve_loop() {
     auto fail = [&](int code) {
+        // no round or copy of this thread's left running on the pool
+        for (hipStream_t st : streams)
+            if (st) (void)hipStreamSynchronize(st);
+        if (upload) (void)hipStreamSynchronize(upload);
         {
             std::lock_guard<std::mutex> lock(mu);
             start_status = code;
@@ -766,6 +851,7 @@ void reform_server::serve_loop() {
             }
             progress = true;
         }
+        stop_cancelled();
         // each family on its own: retire after its round, then the next round
         for (int f = 0; f < 2; ++f) {
             if (in_flight[f]) {
@@ -821,7 +907,7 @@ extern "C" reform_server* reform_server_create(const reform_server_config* confi
     srv->cfg = c;
     if (hipGetDevice(&srv->device) != hipSuccess) return nullptr;
     if (srv->pool.init(size_t(c.hbm_budget))) return nullptr;
-    if (hipMalloc(&srv->states, sizeof(LogicState) * size_t(c.max_in_flight)) != hipSuccess) {
+    if (device_malloc(&srv->states, sizeof(LogicState) * size_t(c.max_in_flight)) != hipSuccess) {
         srv->states = nullptr;
         return nullptr;
     }
@@ -1002,6 +1088,26 @@ extern "C" int reform_server_pria_result(reform_server* server, int64_t ticket, 
     return 0;
 }
 
+extern "C" int reform_server_cancel(reform_server* server, int64_t ticket) {
+    if (!server) return kErrorArgument;
+    std::lock_guard<std::mutex> lock(server->mu);
+    Request* r = find(server, ticket);
+    if (!r || r->status != kPending || r->cancel) return kErrorTicket;
+    r->cancel = true;
+    auto drop = [&](std::deque<Request*>& q) {
+        auto it = std::find(q.begin(), q.end(), r);
+        if (it == q.end()) return false;
+        q.erase(it);
+        return true;
+    };
+    if (drop(server->to_build) || drop(server->ready))
+        server->finish_unadmitted(*r, kCancelledQueued);
+    else if (r->slot >= 0)
+        server->cancels.push_back(ticket);
+    // else on a builder, which finishes it when the build ends
+    return 0;
+}
+
 extern "C" int reform_server_release(reform_server* server, int64_t ticket) {
     if (!server) return kErrorArgument;
     std::lock_guard<std::mutex> lock(server->mu);
@@ -1027,6 +1133,7 @@ extern "C" int reform_server_stats(reform_server* server, reform_server_counters
     stats->in_flight_peak = server->n_in_flight_peak;
     stats->handovers = server->n_handovers;
     stats->handover_waits = server->n_handover_waits;
+    stats->cancelled = server->n_cancelled;
     stats->retire_seconds = server->retire_s;
     stats->rebuild_seconds = server->rebuild_s;
     return server->start_status;

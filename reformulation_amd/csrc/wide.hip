@@ -942,6 +942,14 @@ void WideControl::set_deadlines(std::vector<double> d) {
     expired.assign(deadlines.size(), 0);
 }
 
+void WideControl::stop_position(int f) {
+    if (f < 0 || f >= full_count) return;
+    if (deadlines.empty())
+        deadlines.assign(size_t(full_count), std::numeric_limits<double>::infinity());
+    expired.resize(deadlines.size(), 0);
+    deadlines[size_t(f)] = -std::numeric_limits<double>::infinity();
+}
+
 // The mask is copied from the host latches on the stream (pageable memory:
 // the copy may read them after this call returns).  The latches only ever
 // go from 0 to 1, and only once their deadline has passed, so a copy that
@@ -995,7 +1003,7 @@ int WideSubset<J32, J64>::reserve(int n) {
     cap = 0;
     const size_t o64 = align256(sizeof(J32) * size_t(n));
     const size_t ok = o64 + align256(sizeof(J64) * size_t(n));
-    REFORM_HIP_CHECK(hipMalloc(&mem, ok + sizeof(int) * size_t(n)));
+    REFORM_HIP_CHECK(device_malloc(&mem, ok + sizeof(int) * size_t(n)));
     char* b = static_cast<char*>(mem);
     jobs32 = reinterpret_cast<J32*>(b);
     jobs64 = reinterpret_cast<J64*>(b + o64);
@@ -1119,7 +1127,7 @@ int WideControl::allocate(size_t table_bytes, size_t partial_rows, int partial_w
     if (!arena || at > bytes) {
         const size_t grow = arena ? std::max(at, bytes + bytes / 2) : at;
         release();
-        hipError_t e = hipMalloc(&arena, grow);
+        hipError_t e = device_malloc(&arena, grow);
         if (e != hipSuccess) {
             arena = nullptr;
             record_error("hipMalloc(wide arena)", e, __FILE__, __LINE__);

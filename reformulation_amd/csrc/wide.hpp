@@ -140,6 +140,10 @@ struct WideControl {
     // stopped yet: stopped (expire with the latched mask) on the stream;
     // returns < 0 on error, else how many
     int expire_due(hipStream_t stream, double now);
+    // the instance at full-table position f stops at the next expire_due, as
+    // if its deadline had passed (a cancelled request); with no work of this
+    // engine's in flight
+    void stop_position(int f);
     // set the deadlines (full-table order) and clear the latches; with no
     // work of this engine's in flight
     void set_deadlines(std::vector<double> d);

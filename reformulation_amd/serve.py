@@ -51,7 +51,8 @@ class ServerCounters(ctypes.Structure):
                 ("rejected", _i64), ("rounds", _i64), ("rebuilds", _i64), ("in_flight", _i64),
                 ("bytes_in_use", _i64), ("bytes_peak", _i64), ("bytes_budget", _i64),
                 ("retire_seconds", ctypes.c_double), ("rebuild_seconds", ctypes.c_double),
-                ("in_flight_peak", _i64), ("handovers", _i64), ("handover_waits", _i64)]
+                ("in_flight_peak", _i64), ("handovers", _i64), ("handover_waits", _i64),
+                ("cancelled", _i64)]
 
 
 def _declare(L):
@@ -82,6 +83,8 @@ def _declare(L):
     L.reform_server_praq_result.argtypes = [vp, _i64, _native._dp, _native._dp]
     L.reform_server_pria_result.restype = _int
     L.reform_server_pria_result.argtypes = [vp, _i64, _native._dp, _native._dp, _native._dp]
+    L.reform_server_cancel.restype = _int
+    L.reform_server_cancel.argtypes = [vp, _i64]
     L.reform_server_release.restype = _int
     L.reform_server_release.argtypes = [vp, _i64]
     L.reform_server_stats.restype = _int
@@ -226,6 +229,12 @@ class Server:
         if st < 0:
             raise RuntimeError(f"reform_server_pria_result failed ({st})")
         return x, y, z, self.info(ticket)
+
+    def cancel(self, ticket: int) -> bool:
+        """Stop an unfinished request: status -6 (no answer) if it had not
+        been admitted, else 3 at its point at the next round.  False if the
+        ticket had already finished (or was cancelled before)."""
+        return self._L.reform_server_cancel(self._h, ticket) == 0
 
     def release(self, ticket: int) -> None:
         st = self._L.reform_server_release(self._h, ticket)

@@ -353,3 +353,18 @@ def test_ctypes_records_match_the_c_layouts(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(cls), cname
         for field, _ in cls._fields_:
             assert int(got[f"{cname}.{field}"]) == getattr(cls, field).offset, (cname, field)
+
+
+def test_fault_injector_arms_and_disarms_without_a_gpu():
+    """include/reform_debug.h: the injector is process state only; arming
+    and disarming it touch no device, and the allocation count only grows."""
+    import ctypes
+    from reformulation_amd import _native
+    L = _native.lib()
+    L.reform_debug_fail_device_alloc.argtypes = [ctypes.c_int64]
+    L.reform_debug_fail_device_alloc.restype = ctypes.c_int
+    L.reform_debug_device_allocs.restype = ctypes.c_int64
+    before = L.reform_debug_device_allocs()
+    assert L.reform_debug_fail_device_alloc(3) == 0
+    assert L.reform_debug_fail_device_alloc(-7) == 0
+    assert L.reform_debug_device_allocs() == before >= 0

@@ -307,3 +307,95 @@ def test_a_pool_with_room_for_one_fp64_image_at_a_time_finishes_every_request():
         assert stats["bytes_peak"] <= stats["bytes_budget"] <= budget
         assert stats["handovers"] == sum(switched)
         assert stats["in_flight_peak"] > eager
+
+
+def _finished(srv, t, timeout=60.0):
+    end = time.time() + timeout
+    while srv.info(t).status == -1:
+        assert time.time() < end, t
+        time.sleep(0.005)
+    return srv.info(t)
+
+
+def test_cancel_before_admission_and_in_flight():
+    """One state slot: a request cancelled while it waits for the slot
+    finishes at once with status -6 and no answer; the one holding the slot,
+    cancelled in flight, stops at its next round with status 3, its point
+    as the answer and its bounds still a bracket, as at a time limit; the
+    server goes on serving."""
+    from reformulation_amd import pria, workloads
+    from reformulation_amd.serve import Server
+    j = workloads.pria_c1(5, 60, 200)
+    prob = pria.JrpProblem.new(j.order_costs, j.joint_order_cost, j.holding_costs)
+    # tolerance 0: runs until its bounds meet exactly, far past this test
+    # (the time limit is a backstop only)
+    endless = pria.Parameters.with_auto_step(10**8, 0.0, time_limit=120.0)
+    quick = pria.Parameters.with_auto_step(50_000, 1e-6)
+    with Server(use_float32=True, host_threads=2, max_in_flight=1) as srv:
+        a = srv.submit_pria(prob, endless)
+        b = srv.submit_pria(prob, quick)
+        end = time.time() + 60
+        while srv.info(a).admitted < 0:
+            assert time.time() < end
+            time.sleep(0.005)
+        assert srv.cancel(b)
+        ib = _finished(srv, b)
+        assert ib.status == -6 and ib.admitted < 0
+        with pytest.raises(RuntimeError):
+            srv.pria_result(b)
+        time.sleep(0.1)
+        assert srv.cancel(a), srv.info(a)
+        assert not srv.cancel(a)           # already cancelled
+        srv.wait_idle(60)
+        x, y, z, ia = srv.pria_result(a)
+        assert ia.status == 3 and ia.iterations > 0
+        assert all(np.isfinite(v).all() for v in (x, y, z))
+        assert np.isfinite(ia.lower) and np.isfinite(ia.upper)
+        assert ia.lower <= ia.upper + 1e-9 * abs(ia.upper)
+        assert not srv.cancel(a) and not srv.cancel(b)   # finished
+        assert not srv.cancel(10**9)                      # unknown
+        c = srv.submit_pria(prob, quick)
+        srv.wait_idle(60)
+        assert srv.info(c).status == 0
+        stats = srv.stats()
+        assert stats["cancelled"] == 2 and stats["completed"] == 3
+
+
+def test_cancelled_request_that_converged_first_keeps_its_answer():
+    """A cancel that comes after the request has converged on the device but
+    before its answer is out changes nothing: status 0 and the batched
+    answer."""
+    from reformulation_amd.serve import Server
+    members = [m for m in _members() if m[0] == "pria"][:1]
+    b, index = _batch(members, True)
+    b.solve("batched")
+    want = _results(b, index)[0]
+    b.close()
+    items = _problems(members)
+    with Server(use_float32=True, host_threads=1) as srv:
+        t = _submit(srv, items[0])
+        srv.wait_idle(60)
+        assert not srv.cancel(t)
+        _check_same(_served(srv, "pria", t), want, 0)
+
+
+def test_non_finite_inputs_are_rejected():
+    from reformulation_amd import praq, pria
+    from reformulation_amd.serve import Server
+    items = _problems(_members())
+    kind, p, params = items[0]
+    diag = p.diag.copy()
+    diag[3] = np.nan
+    bad_praq = praq.QuboProblem.from_sparse(p.n(), diag, p.rows, p.cols, p.vals)
+    pk = next(it for it in items if it[0] == "pria")
+    h = pk[1].holding_costs.copy()
+    h[0, 0, -1] = np.inf
+    bad_pria = pria.JrpProblem.new(pk[1].order_costs, pk[1].joint_order_cost, h)
+    with Server(use_float32=True, host_threads=2) as srv:
+        t1 = srv.submit_praq(bad_praq, params)
+        t2 = srv.submit_pria(bad_pria, pk[2])
+        t3 = _submit(srv, pk)
+        srv.wait_idle(60)
+        assert srv.info(t1).status == -2 and srv.info(t2).status == -2
+        assert srv.info(t3).status == 0
+        assert srv.stats()["rejected"] == 2
