@@ -28,7 +28,9 @@ workload, calibrated on a device copy, kept in profiles/); the oracle (FP64 C re
 of the same mixture on the host's cores; and the two single-instance
 configurations through the plain solver (PRAQ Erdos-Renyi n = 2000,
 m = 8000; PRIA N = 50, T = 200; FP64 to 1e-6), GPU and oracle medians
-(`single_instance`).  `--precision` prints the FP32-against-FP64 study on
+(`single_instance`); and each family of the mixture batched on its own
+(`families_alone`: its solve seconds and its step kernels' live rates with
+the chip to themselves, beside the side-by-side step time).  `--precision` prints the FP32-against-FP64 study on
 the large Chung-Lu PRAQ instance instead.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--count 1024]
@@ -79,6 +81,33 @@ def build_batch(members, tolerance: float, use_float32: bool, max_iter: int):
             params = pria.Parameters.with_auto_step(max_iter, tolerance)
             index.append(("pria", b.add_pria(inst.problem(), params), inst))
     return b, index
+
+
+def families_alone(members, tolerance: float, use_float32: bool, side_by_side_s: float) -> dict:
+    """Each family of the mixture batched on its own (the other family's
+    instances left out): solve seconds, and per step kernel of the family
+    its live average and algorithmic rate -- the kernels with the chip to
+    themselves, against the mixture's run where the two families' rounds
+    share it."""
+    out = dict(side_by_side_s=side_by_side_s)
+    for kind in ("pria", "praq"):
+        b, _ = build_batch([m for m in members if m[0] == kind], tolerance, use_float32, 200_000)
+        b.upload()
+        b.solve("batched")
+        dt = min(b.solve("batched") for _ in range(2))
+        b.set_timing(True)
+        b.solve("batched")
+        b.set_timing(False)
+        ks = {k["name"]: dict(avg_launch_us=k["ms"] / k["launches"] * 1e3,
+                              gbs=k["bytes"] / (k["ms"] * 1e-3) / 1e9,
+                              frac=k["bytes"] / (k["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS)
+              for k in b.kernel_stats() if k["launches"]}
+        b.close()
+        out[kind] = dict(solve_s=dt, kernels=ks)
+        log(f"{kind} family alone: {dt:.3f} s per batched solve; "
+            + ", ".join(f"{n} {v['avg_launch_us']:.0f} us {v['frac']:.1%}" for n, v in ks.items()))
+    out["one_after_the_other_s"] = out["pria"]["solve_s"] + out["praq"]["solve_s"]
+    return out
 
 
 def cpu_baseline(members, tolerance: float, budget_s: float):
@@ -291,6 +320,8 @@ def main() -> int:
         log(f"kernel {name}: {k['launches']} launches, {k['ms'] / k['launches'] * 1e3:.1f} us avg, "
             f"{k['bytes'] / (k['ms'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
 
+    alone = families_alone(members, args.tolerance, use_f32, min(times))
+
     plain = None
     if not args.no_plain:
         dt = batch.solve("sequential")
@@ -345,6 +376,7 @@ def main() -> int:
                       launches=K["launches"], avg_launch_us=K["ms"] / K["launches"] * 1e3,
                       bytes_per_launch=K["bytes"] / K["launches"]),
         cpu_baseline=cpu,
+        families_alone=alone,
         single_instance=single,
         tuning=tuning_in_use(),
     )
