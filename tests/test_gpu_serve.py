@@ -399,3 +399,34 @@ def test_non_finite_inputs_are_rejected():
         assert srv.info(t1).status == -2 and srv.info(t2).status == -2
         assert srv.info(t3).status == 0
         assert srv.stats()["rejected"] == 2
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_cancel_in_flight_praq(f32):
+    """A PRAQ request cancelled in flight stops at its next round with
+    status 3; its answer is a point of the box with a bracket, and a request
+    of the other family beside it converges to the batched answer."""
+    from reformulation_amd import praq, workloads
+    from reformulation_amd.serve import Server
+    w = workloads.praq_er(77, 60000, 400000)
+    endless = praq.Parameters.with_auto_step(10**8, 0.0, time_limit=120.0)
+    other = [m for m in _members() if m[0] == "pria"][:1]
+    b, index = _batch(other, f32)
+    b.solve("batched")
+    want = _results(b, index)[0]
+    b.close()
+    with Server(use_float32=f32, host_threads=2) as srv:
+        a = srv.submit_praq(w.problem(), endless)
+        t = _submit(srv, _problems(other)[0])
+        end = time.time() + 60
+        while srv.info(a).admitted < 0:
+            assert time.time() < end
+            time.sleep(0.005)
+        time.sleep(0.1)
+        assert srv.cancel(a), srv.info(a)
+        srv.wait_idle(60)
+        x, y, ia = srv.praq_result(a)
+        assert ia.status == 3 and ia.iterations > 0
+        assert np.all((x >= 0.0) & (x <= 1.0))
+        assert ia.lower <= ia.upper + 1e-9 * abs(ia.upper)
+        _check_same(_served(srv, "pria", t), want, 0)
