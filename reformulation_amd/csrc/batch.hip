@@ -293,7 +293,12 @@ int BatchImpl::upload() {
     // one allocation: FP64 images, FP32 images (FP32 batch), job records,
     // states, one device-tier work area
     const bool f32 = use_float32 != 0;
-    size_t at = 0, work_bytes = praq_device_work_bytes();
+    int64_t max_n = 0, max_m = 0;   // the work area serves every instance in turn
+    for (const auto& e : praq) {
+        max_n = std::max<int64_t>(max_n, e.host.n);
+        max_m = std::max<int64_t>(max_m, e.host.m);
+    }
+    size_t at = 0, work_bytes = praq_device_work_bytes(max_n, max_m);
     std::vector<PraqLayout> pl64(praq.size()), pl32(praq.size());
     std::vector<PriaLayout> rl64(pria.size()), rl32(pria.size());
     for (size_t q = 0; q < praq.size(); ++q) {

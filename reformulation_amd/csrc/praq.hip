@@ -17,6 +17,7 @@
 #include "praq_host.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -204,9 +205,101 @@ __global__ void praq_edge_kernel(PraqArrays<Real> a, PraqStep<Real> st) {
                                       blockDim.x);
 }
 
+// (refl, x, sx) of every node in one record of four values aligned to
+// its size, for the check step's edge kernel: one gather per edge end
+// brings all three.
+template <class Real>
+__global__ void praq_quad_kernel(PraqArrays<Real> a, Real* quad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const PraqView<Real> v = praq_view(a);
+    const gptr<Real> q = to_global(quad) + 4 * size_t(i);
+    q[0] = v.refl[i];
+    q[1] = v.x[i];
+    q[2] = v.sx[i];
+    q[3] = Real(0);
+}
+
+// The last step of a Halpern block with the edge terms of its check folded
+// in: praq_edge_kernel's update of each edge (the reflected primal at its
+// ends read from the node records), then, from the values the update holds
+// and (x, sx) at both ends, the terms praq_edge_sums takes at the step point
+// (sx, sy) -- one partial row per block.
+template <class Real>
+__global__ void __launch_bounds__(256)
+praq_edge_check_kernel(PraqArrays<Real> a, PraqStep<Real> st, const Real* quad,
+                       double* rows) {
+    __shared__ double red[kPraqSums * 16];
+    constexpr int K = kEdgesPerThread;
+    const PraqView<Real> v = praq_view(a);
+    const int m = a.m;
+    const int first = blockIdx.x * blockDim.x * K + threadIdx.x, stride = blockDim.x;
+    const gptr<const Real> qr = to_global(
+        static_cast<const Real*>(__builtin_assume_aligned(quad, 4 * sizeof(Real))));
+    int e[K], i0[K], i1[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = first + k * stride;
+        e[k] = q < m ? q : 0;   // past the end: edge 0, dropped
+    }
+    Real sw[K], ya[K], yb[K], aa[K], ab[K], ra[K], rb[K], xa[K], sa[K], xb[K], sb[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        i0[k] = v.e0[e[k]];
+        i1[k] = v.e1[e[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        sw[k] = v.w[e[k]];
+        ya[k] = v.yc[e[k]];
+        yb[k] = v.yc[m + e[k]];
+        aa[k] = v.y0[e[k]];
+        ab[k] = v.y0[m + e[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const gptr<const Real> qa = qr + 4 * i0[k], qb = qr + 4 * i1[k];
+        ra[k] = qa[0];
+        xa[k] = qa[1];
+        sa[k] = qa[2];
+        rb[k] = qb[0];
+        xb[k] = qb[1];
+        sb[k] = qb[2];
+    }
+    double acc[kPraqSums];
+#pragma unroll
+    for (int q = 0; q < kPraqSums; ++q) acc[q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (first + k * stride >= m) continue;
+        Real y2a, y2b;
+        praq_edge_update(v, st, e[k], sw[k], ya[k], yb[k], aa[k], ab[k], ra[k], rb[k], y2a, y2b);
+        const double swd = sw[k];
+        const double w = fabs(swd);
+        const double da = double(ya[k]) - double(y2a), db = double(yb[k]) - double(y2b);
+        acc[kDualNorm] += da * da + db * db;
+        acc[kCross] += (double(xa[k]) - double(sa[k])) * da + (double(xb[k]) - double(sb[k])) * db;
+        const double qa = double(ya[k]) - double(aa[k]), qb = double(yb[k]) - double(ab[k]);
+        acc[kDualDrift] += qa * qa + qb * qb;
+        const double bi = sa[k], bj = sb[k];
+        if (swd > 0.0) {
+            const double t = bi + bj - 1.0;
+            acc[kUpperEdge] += w * (t > 0.0 ? t : 0.0);
+            acc[kLowerEdge] += double(y2a);
+        } else {
+            acc[kUpperEdge] += -w * (bi < bj ? bi : bj);
+        }
+    }
+    block_sums<kPraqSums>(acc, red);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kPraqSums; ++q) rows[size_t(blockIdx.x) * kPraqSums + q] = acc[q];
+}
+
+// Check sums over grid-stride node groups and (with_edges) edges, one
+// partial row per block.
 template <class Real>
 __global__ void praq_sums_kernel(PraqArrays<Real> a, int lanes, const Real* bx, const Real* by,
-                                 double* partial) {
+                                 double* partial, bool with_edges) {
     __shared__ double red[kPraqSums * 16];
     double acc[kPraqSums];
 #pragma unroll
@@ -218,7 +311,8 @@ __global__ void praq_sums_kernel(PraqArrays<Real> a, int lanes, const Real* bx, 
     const gptr<const Real> gx = to_global(bx), gy = to_global(by);
     for (int base = 0; base < a.n; base += groups)
         praq_node_sums(v, base + gid / lanes, gid % lanes, lanes, gx, gy, acc);
-    for (int e = gid; e < a.m; e += nthreads) praq_edge_sums(v, e, gx, gy, true, acc);
+    if (with_edges)
+        for (int e = gid; e < a.m; e += nthreads) praq_edge_sums(v, e, gx, gy, true, acc);
     block_sums<kPraqSums>(acc, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < kPraqSums; ++k) partial[blockIdx.x * kPraqSums + k] = acc[k];
@@ -231,7 +325,7 @@ __global__ void praq_sums_final(const double* partial, int rows, double* out) {
 #pragma unroll
     for (int k = 0; k < kPraqSums; ++k) acc[k] = 0.0;
     for (int r = threadIdx.x; r < rows; r += blockDim.x)
-        for (int k = 0; k < kPraqSums; ++k) acc[k] += partial[r * kPraqSums + k];
+        for (int k = 0; k < kPraqSums; ++k) acc[k] += partial[size_t(r) * kPraqSums + k];
     block_sums<kPraqSums>(acc, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < kPraqSums; ++k) out[k] = acc[k];
@@ -295,13 +389,64 @@ static int device_sums(const PraqArrays<Real>& a, int lanes, const Real* bx, con
                        double* partial, double* dsums, double* host, hipStream_t stream) {
     device_chunks(a, by, stream);
     hipLaunchKernelGGL(praq_sums_kernel<Real>, dim3(kSumBlocks), dim3(kBlock), 0, stream, a,
-                       lanes, bx, by, partial);
+                       lanes, bx, by, partial, true);
     int rows = kSumBlocks;
     if (has_hubs(a)) {
         hipLaunchKernelGGL(praq_hub_sums_kernel<Real>, dim3(1), dim3(kBlock), 0, stream, a, bx,
                            partial + size_t(kSumBlocks) * kPraqSums);
         ++rows;
     }
+    hipLaunchKernelGGL(praq_sums_final, dim3(1), dim3(1024), 0, stream, partial, rows, dsums);
+    REFORM_HIP_CHECK(hipGetLastError());
+    REFORM_HIP_CHECK(hipMemcpyAsync(host, dsums, sizeof(double) * kPraqSums,
+                                    hipMemcpyDeviceToHost, stream));
+    REFORM_HIP_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// Blocks (and partial rows) of the edge kernels.
+static int edge_blocks(int m) {
+    return blocks_for((int64_t(m) + kEdgesPerThread - 1) / kEdgesPerThread);
+}
+
+// The last step of a Halpern block: as device_step, with the node records
+// (refl, x, sx) written after the node pass and the edge pass taking its
+// check terms into `rows`.
+template <class Real>
+static int device_check_step(const PraqArrays<Real>& a, const PraqStep<Real>& st, int lanes,
+                             Real* quad, double* rows, hipStream_t stream) {
+    device_chunks(a, static_cast<const Real*>(a.yc), stream);
+    hipLaunchKernelGGL(praq_node_kernel<Real>, dim3(blocks_for(int64_t(a.n) * lanes)),
+                       dim3(kBlock), 0, stream, a, st, lanes);
+    if (has_hubs(a))
+        hipLaunchKernelGGL(praq_hub_kernel<Real>, dim3(blocks_for(a.nh)), dim3(kBlock), 0, stream,
+                           a, st);
+    hipLaunchKernelGGL(praq_quad_kernel<Real>, dim3(blocks_for(a.n)), dim3(kBlock), 0, stream, a,
+                       quad);
+    if (a.m > 0)
+        hipLaunchKernelGGL(praq_edge_check_kernel<Real>, dim3(edge_blocks(a.m)), dim3(kBlock), 0,
+                           stream, a, st, static_cast<const Real*>(quad), rows);
+    REFORM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// The check after device_check_step, at (sx, sy): the node terms (CSR
+// gather of sy) into rows [0, kSumBlocks), the hubs' into row kSumBlocks,
+// the edge rows after them as the step left them; all rows added in order.
+template <class Real>
+static int device_check_sums(const PraqArrays<Real>& a, int lanes, double* partial, int eb,
+                             double* dsums, double* host, hipStream_t stream) {
+    device_chunks(a, static_cast<const Real*>(a.sy), stream);
+    hipLaunchKernelGGL(praq_sums_kernel<Real>, dim3(kSumBlocks), dim3(kBlock), 0, stream, a,
+                       lanes, static_cast<const Real*>(a.sx), static_cast<const Real*>(a.sy),
+                       partial, false);
+    double* hub_row = partial + size_t(kSumBlocks) * kPraqSums;
+    if (has_hubs(a))
+        hipLaunchKernelGGL(praq_hub_sums_kernel<Real>, dim3(1), dim3(kBlock), 0, stream, a,
+                           static_cast<const Real*>(a.sx), hub_row);
+    else
+        REFORM_HIP_CHECK(hipMemsetAsync(hub_row, 0, sizeof(double) * kPraqSums, stream));
+    const int rows = kSumBlocks + 1 + (a.m > 0 ? eb : 0);
     hipLaunchKernelGGL(praq_sums_final, dim3(1), dim3(1024), 0, stream, partial, rows, dsums);
     REFORM_HIP_CHECK(hipGetLastError());
     REFORM_HIP_CHECK(hipMemcpyAsync(host, dsums, sizeof(double) * kPraqSums,
@@ -319,8 +464,14 @@ int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
     // hubs by chunks in this tier
     PraqArrays<Real> a = job.a;
     a.heavy_degree = praq_heavy_degree(lanes);
+    // work: partial rows (check sums, hubs, the check step's edge blocks),
+    // the summed row, the node records (32-byte aligned)
+    const int eb = edge_blocks(a.m);
     double* partial = work;
-    double* dsums = work + (kSumBlocks + 1) * kPraqSums;
+    double* edge_rows = partial + size_t(kSumBlocks + 1) * kPraqSums;
+    double* dsums = partial + size_t(kSumBlocks + 1 + eb) * kPraqSums;
+    auto* quad = reinterpret_cast<Real*>(
+        (reinterpret_cast<uintptr_t>(dsums + kPraqSums) + 31) & ~uintptr_t(31));
     double sums[kPraqSums];
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto t0 = now();
@@ -363,12 +514,16 @@ int praq_device_solve(const PraqJob<Real>& job, LogicState& st, double* work,
                 if (device_copy(a.x0, a.x, a.n, a.y0, a.y, 2 * a.m, stream)) return kErrorDevice;
                 st.anchor_pending = 0;
             }
-            for (int64_t j = 0; j < p.op_steps; ++j)
-                if (device_step(a, praq_step_scalars<Real>(r0 + j, omega, p.eta, true,
-                                                           j + 1 == p.op_steps),
-                                lanes, stream))
+            // the last step takes the edge terms of the check with it
+            for (int64_t j = 0; j < p.op_steps; ++j) {
+                const bool last = j + 1 == p.op_steps;
+                const PraqStep<Real> sc =
+                    praq_step_scalars<Real>(r0 + j, omega, p.eta, true, last);
+                if (last ? device_check_step(a, sc, lanes, quad, edge_rows, stream)
+                         : device_step(a, sc, lanes, stream))
                     return kErrorDevice;
-            if (device_sums(a, lanes, a.sx, a.sy, partial, dsums, sums, stream))
+            }
+            if (device_check_sums(a, lanes, partial, eb, dsums, sums, stream))
                 return kErrorDevice;
             st.restart_iter += p.op_steps;
             st.iteration += p.op_steps;
@@ -479,7 +634,10 @@ int praq_convert_jobs(const PraqJob<float>* from, const PraqJob<double>* to, int
     return hipGetLastError() == hipSuccess ? 0 : kErrorDevice;
 }
 
-size_t praq_device_work_bytes() { return sizeof(double) * (kSumBlocks + 2) * kPraqSums; }
+size_t praq_device_work_bytes(int64_t n, int64_t m) {
+    const int64_t eb = (m / kEdgesPerThread + 1) / kBlock + 1;   // >= edge_blocks(m)
+    return sizeof(double) * (size_t(kSumBlocks + 2 + eb) * kPraqSums + 4 * size_t(n) + 4);
+}
 
 template <class Real>
 int praq_launch_wg(const PraqJob<Real>* jobs, int count, int block, hipStream_t stream) {

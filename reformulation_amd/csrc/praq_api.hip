@@ -76,7 +76,7 @@ struct PraqImage {
         job_off = align256(L.total_bytes);
         state_off = align256(job_off + sizeof(PraqJob<Real>));
         work_off = align256(state_off + sizeof(LogicState));
-        bytes = work_off + praq_device_work_bytes();
+        bytes = work_off + praq_device_work_bytes(s.n, s.m);
         if (buf.alloc(bytes)) return kErrorDevice;
         char* base = static_cast<char*>(buf.ptr);
         if (dev) {

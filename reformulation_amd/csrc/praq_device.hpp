@@ -197,11 +197,13 @@ __device__ __forceinline__ double praq_hub_total(const PraqView<Real>& a, int h)
 }
 
 // Edge e's update from its loaded values: projection of y1 + sigma (q -
-// S refl), the next yc, and (y, sy) when storing.
+// S refl), the next yc, and (y, sy) when storing; y2 (the projection) also
+// into (o2a, o2b).
 template <class Real>
 __device__ __forceinline__ void praq_edge_update(const PraqView<Real>& a, const PraqStep<Real>& st,
                                                  int e, Real sw, Real y1a, Real y1b, Real ana,
-                                                 Real anb, Real ra, Real rb) {
+                                                 Real anb, Real ra, Real rb, Real& o2a,
+                                                 Real& o2b) {
     const int m = a.m;
     Real w = sw < Real(0) ? -sw : sw;
     bool covering = sw > Real(0);
@@ -225,6 +227,16 @@ __device__ __forceinline__ void praq_edge_update(const PraqView<Real>& a, const 
     }
     a.yc[e] = st.lwn * (Real(2) * y2a - y1a) + st.rwn * ana;
     a.yc[m + e] = st.lwn * (Real(2) * y2b - y1b) + st.rwn * anb;
+    o2a = y2a;
+    o2b = y2b;
+}
+
+template <class Real>
+__device__ __forceinline__ void praq_edge_update(const PraqView<Real>& a, const PraqStep<Real>& st,
+                                                 int e, Real sw, Real y1a, Real y1b, Real ana,
+                                                 Real anb, Real ra, Real rb) {
+    Real o2a, o2b;
+    praq_edge_update(a, st, e, sw, y1a, y1b, ana, anb, ra, rb, o2a, o2b);
 }
 
 // Edge e's update.

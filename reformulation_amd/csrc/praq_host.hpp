@@ -362,7 +362,8 @@ int praq_convert_jobs(const PraqJob<float>* from, const PraqJob<double>* to, int
                       hipStream_t stream);
 template <class Real>
 int praq_launch_wg(const PraqJob<Real>* jobs, int count, int block, hipStream_t stream);
-size_t praq_device_work_bytes();
+// Device-tier work area of an instance with n nodes and m edges.
+size_t praq_device_work_bytes(int64_t n, int64_t m);
 // Mean milliseconds of one reflected device-tier step after `warmup` steps
 // (HIP events over `steps` steps), from the job's start point.
 template <class Real>
