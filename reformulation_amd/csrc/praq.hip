@@ -258,7 +258,7 @@ praq_edge_check_kernel(PraqArrays<Real> a, PraqStep<Real> st, const Real* quad,
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const gptr<const Real> qa = qr + 4 * i0[k], qb = qr + 4 * i1[k];
+        const gptr<const Real> qa = qr + 4 * size_t(i0[k]), qb = qr + 4 * size_t(i1[k]);
         ra[k] = qa[0];
         xa[k] = qa[1];
         sa[k] = qa[2];
